@@ -1,0 +1,887 @@
+// C-ABI layer of libfxcorr.so: validation, plans, device tables, launches.
+// Mirrors the per-sector seam of M-AKIMOTOO/fx-corr (src/main.rs:2173-2591)
+// and the helpers it calls; every entry point cites its reference function.
+#include "../../include/fxcorr.h"
+#include "fxc_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+using namespace fxc;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+fxc_status fail(fxc_status code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                    \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return fail(FXC_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));     \
+    } while (0)
+
+bool is_pow2(int64_t n) { return n > 0 && (n & (n - 1)) == 0; }
+
+// Device buffer that grows on demand; freed with the plan.
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes) {
+        if (bytes <= cap) return hipSuccess;
+        if (p) {
+            hipError_t e = hipFree(p);
+            if (e != hipSuccess) return e;
+            p = nullptr;
+            cap = 0;
+        }
+        size_t want = std::max<size_t>(bytes, 256);
+        hipError_t e = hipMalloc(&p, want);
+        if (e == hipSuccess) cap = want;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+// Rust usize::from_str / f64::from_str for the list parsers (args.rs).
+bool parse_usize_tok(const std::string &t, unsigned long long &out) {
+    size_t i = 0;
+    if (i < t.size() && t[i] == '+') i++;
+    if (i >= t.size()) return false;
+    unsigned long long v = 0;
+    for (; i < t.size(); ++i) {
+        if (t[i] < '0' || t[i] > '9') return false;
+        unsigned long long nv = v * 10ull + (unsigned long long)(t[i] - '0');
+        if (nv / 10ull != v) return false;
+        v = nv;
+    }
+    out = v;
+    return true;
+}
+
+bool parse_f64_tok(const std::string &t, double &out) {
+    if (t.empty()) return false;
+    for (char c : t)
+        if (c == 'x' || c == 'X' || isspace((unsigned char)c)) return false;
+    char *end = nullptr;
+    double v = strtod(t.c_str(), &end);
+    if (end == t.c_str() || *end != 0) return false;
+    out = v;
+    return true;
+}
+
+std::string trim(const std::string &s) {
+    size_t b = 0, e = s.size();
+    while (b < e && isspace((unsigned char)s[b])) b++;
+    while (e > b && isspace((unsigned char)s[e - 1])) e--;
+    return s.substr(b, e - b);
+}
+
+std::vector<std::string> split_commas(const char *list) {
+    std::vector<std::string> out;
+    std::string cur;
+    for (const char *p = list; *p; ++p) {
+        if (*p == ',') {
+            out.push_back(cur);
+            cur.clear();
+        } else {
+            cur.push_back(*p);
+        }
+    }
+    out.push_back(cur);
+    return out;
+}
+
+int g_num_cus = -1;
+std::mutex g_dev_mu;
+
+int num_cus() {
+    std::lock_guard<std::mutex> lk(g_dev_mu);
+    if (g_num_cus < 0) {
+        int dev = 0;
+        hipDeviceProp_t prop;
+        if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+            g_num_cus = prop.multiProcessorCount;
+        else
+            g_num_cus = 256;
+    }
+    return g_num_cus;
+}
+
+}  // namespace
+
+struct fxc_plan {
+    fxc_plan_desc desc;
+    std::vector<double> levels[2];
+    int N = 0, M = 0, half = 0;
+    int T = 0;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    StationDev st[2];
+    int32_t decode_ok[2] = {1, 1};
+    // device tables
+    DevBuf tw_fft, tw_post, lev_dev[2], out_lev;
+    // scratch
+    DevBuf part, red_scratch, table_dev, d_raw[2], d_acc, d_raw_out, d_tmp_in, d_tmp_out,
+        d_shifts;
+    int max_wg = 0;
+    std::vector<double> table_host;  // last uploaded geometric table
+    // timing
+    bool timing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending, pool;
+    double timed_ms = 0.0;
+    int64_t timed_launches = 0;
+};
+
+// ====================================================================== misc
+extern "C" const char *fxc_version(void) { return "fxcorr-mi355x 0.1"; }
+
+extern "C" const char *fxc_strerror(fxc_status s) {
+    switch (s) {
+    case FXC_OK: return "ok";
+    case FXC_ERR_INVALID_ARG: return "invalid argument";
+    case FXC_ERR_UNSUPPORTED: return "unsupported configuration";
+    case FXC_ERR_HIP: return "HIP runtime error";
+    case FXC_ERR_NO_OVERLAP: return "no band overlap";
+    case FXC_ERR_PARSE: return "parse error";
+    case FXC_ERR_IO: return "I/O error";
+    case FXC_ERR_NOMEM: return "out of memory";
+    default: return "unknown status";
+    }
+}
+
+extern "C" const char *fxc_last_error(void) { return g_last_error.c_str(); }
+
+extern "C" fxc_status fxc_device_count(int32_t *n) {
+    if (!n) return fail(FXC_ERR_INVALID_ARG, "n is null");
+    int c = 0;
+    hipError_t e = hipGetDeviceCount(&c);
+    if (e != hipSuccess) {
+        *n = 0;
+        return fail(FXC_ERR_HIP, "hipGetDeviceCount: %s", hipGetErrorString(e));
+    }
+    *n = c;
+    return FXC_OK;
+}
+
+extern "C" fxc_status fxc_dev_alloc(size_t bytes, void **dptr) {
+    if (!dptr) return fail(FXC_ERR_INVALID_ARG, "dptr is null");
+    HIP_TRY(hipMalloc(dptr, bytes ? bytes : 1));
+    return FXC_OK;
+}
+extern "C" fxc_status fxc_dev_free(void *dptr) {
+    if (dptr) HIP_TRY(hipFree(dptr));
+    return FXC_OK;
+}
+extern "C" fxc_status fxc_memcpy_h2d(void *dst, const void *src, size_t bytes, void *stream) {
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, (hipStream_t)stream));
+    HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    return FXC_OK;
+}
+extern "C" fxc_status fxc_memcpy_d2h(void *dst, const void *src, size_t bytes, void *stream) {
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, (hipStream_t)stream));
+    HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    return FXC_OK;
+}
+extern "C" fxc_status fxc_memset_d(void *dst, int value, size_t bytes, void *stream) {
+    HIP_TRY(hipMemsetAsync(dst, value, bytes, (hipStream_t)stream));
+    return FXC_OK;
+}
+extern "C" fxc_status fxc_stream_create(void **stream) {
+    if (!stream) return fail(FXC_ERR_INVALID_ARG, "stream is null");
+    hipStream_t s;
+    HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *stream = s;
+    return FXC_OK;
+}
+extern "C" fxc_status fxc_stream_destroy(void *stream) {
+    if (stream) HIP_TRY(hipStreamDestroy((hipStream_t)stream));
+    return FXC_OK;
+}
+extern "C" fxc_status fxc_stream_sync(void *stream) {
+    HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    return FXC_OK;
+}
+extern "C" fxc_status fxc_event_create(void **ev) {
+    if (!ev) return fail(FXC_ERR_INVALID_ARG, "ev is null");
+    hipEvent_t e;
+    HIP_TRY(hipEventCreate(&e));
+    *ev = e;
+    return FXC_OK;
+}
+extern "C" fxc_status fxc_event_destroy(void *ev) {
+    if (ev) HIP_TRY(hipEventDestroy((hipEvent_t)ev));
+    return FXC_OK;
+}
+extern "C" fxc_status fxc_event_record(void *ev, void *stream) {
+    HIP_TRY(hipEventRecord((hipEvent_t)ev, (hipStream_t)stream));
+    return FXC_OK;
+}
+extern "C" fxc_status fxc_event_elapsed_ms(void *a, void *b, float *ms) {
+    if (!ms) return fail(FXC_ERR_INVALID_ARG, "ms is null");
+    HIP_TRY(hipEventSynchronize((hipEvent_t)b));
+    HIP_TRY(hipEventElapsedTime(ms, (hipEvent_t)a, (hipEvent_t)b));
+    return FXC_OK;
+}
+
+// ============================================================ host helpers
+// args.rs:202-210
+extern "C" fxc_status fxc_parse_shuffle(const char *list, int32_t internal_out[32]) {
+    if (!list || !internal_out) return fail(FXC_ERR_INVALID_ARG, "null argument");
+    std::vector<std::string> toks = split_commas(list);
+    std::vector<unsigned long long> ext;
+    for (auto &tk : toks) {
+        unsigned long long v;
+        if (!parse_usize_tok(trim(tk), v))
+            return fail(FXC_ERR_PARSE, "invalid shuffle entry '%s'", trim(tk).c_str());
+        ext.push_back(v);
+    }
+    if (ext.size() != 32) return fail(FXC_ERR_PARSE, "Shuffle map must contain exactly 32 entries");
+    std::vector<unsigned long long> sorted = ext;
+    std::sort(sorted.begin(), sorted.end());
+    for (unsigned long long i = 0; i < 32; ++i)
+        if (sorted[i] != i) return fail(FXC_ERR_PARSE, "Shuffle map must be a permutation of 0..31");
+    for (int i = 0; i < 32; ++i) internal_out[31 - i] = (int32_t)ext[i];
+    return FXC_OK;
+}
+
+// args.rs:194-200
+extern "C" fxc_status fxc_parse_levels(int32_t bits, const char *list, double *out, int32_t cap,
+                                       int32_t *n_out) {
+    if (!list || !out || !n_out) return fail(FXC_ERR_INVALID_ARG, "null argument");
+    if (bits == 0) return fail(FXC_ERR_PARSE, "Bit depth must be at least 1");
+    if (bits < 0 || bits > 24) return fail(FXC_ERR_PARSE, "bit depth %d out of range", bits);
+    const int64_t expected = int64_t(1) << bits;
+    std::vector<std::string> toks = split_commas(list);
+    std::vector<double> vals;
+    for (auto &tk : toks) {
+        double v;
+        if (!parse_f64_tok(trim(tk), v))
+            return fail(FXC_ERR_PARSE, "invalid level '%s'", trim(tk).c_str());
+        vals.push_back(v);
+    }
+    if ((int64_t)vals.size() != expected)
+        return fail(FXC_ERR_PARSE, "Expected %lld levels, received %zu", (long long)expected, vals.size());
+    if ((int64_t)cap < expected) return fail(FXC_ERR_INVALID_ARG, "output capacity too small");
+    std::copy(vals.begin(), vals.end(), out);
+    *n_out = (int32_t)expected;
+    return FXC_OK;
+}
+
+// main.rs:378-387
+extern "C" fxc_status fxc_quantization_mean_power(const double *levels, int32_t n, double *p_out) {
+    if (!levels || !p_out) return fail(FXC_ERR_INVALID_ARG, "null argument");
+    if (n <= 0) return fail(FXC_ERR_INVALID_ARG, "levels are empty");
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s += levels[i] * levels[i];
+    double p = s / (double)n;
+    if (!std::isfinite(p) || p <= 0.0) return fail(FXC_ERR_INVALID_ARG, "mean-square level is invalid: %g", p);
+    *p_out = p;
+    return FXC_OK;
+}
+
+static size_t f64_to_usize_sat(double v) {
+    if (v != v || v <= 0.0) return 0;
+    if (v >= 18446744073709551615.0) return SIZE_MAX;
+    return (size_t)v;
+}
+static int64_t f64_to_i64_sat_host(double v) {
+    if (v != v) return 0;
+    if (v >= 9223372036854775807.0) return INT64_MAX;
+    if (v <= -9223372036854775808.0) return INT64_MIN;
+    return (int64_t)v;
+}
+
+// main.rs:360-372.  The reference's usize cast of a negative right edge
+// would wrap; here a non-positive overlap is reported as no overlap.
+extern "C" fxc_status fxc_compute_band_alignment(int32_t fft, double fs, double c1, double c2,
+                                                 double bw1, double bw2, fxc_band_alignment *out) {
+    if (!out || fft <= 0 || !(fs > 0.0)) return fail(FXC_ERR_INVALID_ARG, "bad band alignment args");
+    double df = fs / (double)fft;
+    size_t h = (size_t)fft / 2 + 1;
+    size_t lim = (fft % 2 == 0) ? h - 1 : h;
+    size_t v1 = f64_to_usize_sat(std::floor((bw1 * 1e6) / df));
+    v1 = std::min(v1 == SIZE_MAX ? v1 : v1 + 1, lim);
+    size_t v2 = f64_to_usize_sat(std::floor((bw2 * 1e6) / df));
+    v2 = std::min(v2 == SIZE_MAX ? v2 : v2 + 1, lim);
+    double sr = (c1 * 1e6 - c2 * 1e6) / df;
+    int64_t sb = f64_to_i64_sat_host(std::round(sr));
+    int64_t a1s = std::max<int64_t>(0, -sb);
+    int64_t a1e = std::min<int64_t>((int64_t)v1, (int64_t)v2 - sb);
+    if (a1e <= a1s) return fail(FXC_ERR_NO_OVERLAP, "No band overlap");
+    out->shift_bins = sb;
+    out->a1s = a1s;
+    out->a1e = a1e;
+    out->a2s = a1s + sb;
+    return FXC_OK;
+}
+
+// main.rs:2058-2072
+extern "C" fxc_status fxc_sector_partition(int64_t total_f, double frame_sec, int64_t *counts,
+                                           int64_t cap, int64_t *n_out) {
+    if (!n_out || total_f < 0) return fail(FXC_ERR_INVALID_ARG, "bad partition args");
+    double total_duration = (double)total_f * frame_sec;
+    double r = std::round(total_duration);
+    size_t sector_count = f64_to_usize_sat(r > 1.0 ? r : 1.0);
+    size_t tf = (size_t)(total_f > 1 ? total_f : 1);
+    if (sector_count > tf) sector_count = tf;
+    size_t base = (size_t)total_f / sector_count, extra = (size_t)total_f % sector_count;
+    int64_t n = 0;
+    for (size_t si = 0; si < sector_count; ++si) {
+        int64_t nf = (int64_t)(base + (si < extra ? 1 : 0));
+        if (nf > 0) {
+            if (counts) {
+                if (n >= cap) return fail(FXC_ERR_INVALID_ARG, "counts capacity too small");
+                counts[n] = nf;
+            }
+            n++;
+        }
+    }
+    *n_out = n;
+    return FXC_OK;
+}
+
+// main.rs:126-130
+extern "C" fxc_status fxc_split_delay(double d, double fs, int64_t *n_out, double *frac_out) {
+    if (!n_out || !frac_out) return fail(FXC_ERR_INVALID_ARG, "null argument");
+    int64_t n = f64_to_i64_sat_host(std::round(d * fs));
+    *n_out = n;
+    *frac_out = d - ((double)n / fs);
+    return FXC_OK;
+}
+
+// =================================================================== plans
+static fxc_status validate_station(const fxc_station_desc &s, int idx) {
+    if (s.bits < 1 || s.bits > kMaxBits)
+        return fail(FXC_ERR_UNSUPPORTED, "station %d: bits %d outside 1..%d", idx + 1, s.bits, kMaxBits);
+    if (!s.levels || s.nlevels != (1 << s.bits))
+        return fail(FXC_ERR_INVALID_ARG, "station %d: expected %d levels, got %d", idx + 1,
+                    1 << s.bits, s.nlevels);
+    int seen[32] = {0};
+    for (int i = 0; i < 32; ++i) {
+        if (s.shuffle_internal[i] < 0 || s.shuffle_internal[i] > 31 || seen[s.shuffle_internal[i]]++)
+            return fail(FXC_ERR_INVALID_ARG, "station %d: shuffle is not a permutation of 0..31", idx + 1);
+    }
+    return FXC_OK;
+}
+
+// utils.rs:173-213 build_decode_plan (+ the decoded-length rule of
+// decode_block_into_with_plan, utils.rs:372-374, evaluated once per plan).
+static void build_station(fxc_plan *p, int i) {
+    const fxc_station_desc &s = p->desc.st[i];
+    StationDev &d = p->st[i];
+    memset(&d, 0, sizeof d);
+    d.bits = s.bits;
+    d.lsb = s.lsb ? 1 : 0;
+    bool ident = true, pair = true;
+    for (int k = 0; k < 32; ++k) {
+        d.shifts[k] = s.shuffle_internal[k];
+        if (s.shuffle_internal[k] != k) ident = false;
+        if (s.shuffle_internal[k] != (k ^ 1)) pair = false;
+    }
+    d.shuffle_kind = ident ? SHUF_IDENTITY : (pair ? SHUF_PAIRSWAP : SHUF_GENERIC);
+    d.bpf = (int32_t)(((int64_t)p->N * s.bits + 7) / 8);
+    d.nwords = (d.bpf + 3) / 4;
+    d.aligned = (d.bpf % 4 == 0) ? 1 : 0;
+    d.code_mask = (s.bits >= 32) ? 0xFFFFFFFFu : ((1u << s.bits) - 1u);
+    for (int k = 0; k < 16 && k < s.nlevels; ++k) d.lev4[k] = (float)s.levels[k];
+    int64_t samples;
+    if (s.bits == 2 && ident) samples = (int64_t)d.bpf * 8 / 2;       // 16-bit LUT path + tail bytes
+    else samples = ((int64_t)(d.bpf / 4) * 32) / s.bits;             // whole 32-bit words only
+    d.decode_ok = samples >= p->N ? 1 : 0;
+    p->decode_ok[i] = d.decode_ok;
+}
+
+static void fill_args(fxc_plan *p, SectorArgs &a) {
+    memset(&a, 0, sizeof a);
+    a.st[0] = p->st[0];
+    a.st[1] = p->st[1];
+    a.st[0].levels = (const float *)p->lev_dev[0].p;
+    a.st[1].levels = (const float *)p->lev_dev[1].p;
+    a.tw_fft = (const float2 *)p->tw_fft.p;
+    a.tw_post = (const float2 *)p->tw_post.p;
+    a.N = p->N;
+    a.M = p->M;
+    a.a1s = (int32_t)p->desc.a1s;
+    a.a1e = (int32_t)p->desc.a1e;
+    a.a2s = (int32_t)p->desc.a2s;
+    a.ov = (int32_t)(p->desc.a1e - p->desc.a1s);
+    a.grid = p->desc.out_grid;
+    a.df = p->desc.df_hz;
+    a.w1 = p->desc.w1;
+    a.w2 = p->desc.w2;
+    a.bit_out = p->desc.bit_out;
+    a.output_lsb = p->desc.output_lsb;
+    for (int k = 0; k < 32; ++k) a.out_shifts[k] = p->desc.st[0].shuffle_internal[k];
+    a.out_levels = (const float *)p->out_lev.p;
+    a.out_nlevels = p->desc.st[0].nlevels;
+    a.bpf_out = (int32_t)(((int64_t)p->N * p->desc.bit_out + 7) / 8);
+    uint32_t fl = 0;
+    if (p->desc.products & FXC_PRODUCT_ACF) fl |= FLAG_ACF;
+    if (p->desc.products & FXC_PRODUCT_XCF) fl |= FLAG_XCF;
+    if (p->desc.products & FXC_PRODUCT_PHASED) fl |= FLAG_PHASED;
+    if (p->desc.products & FXC_PRODUCT_RAW_OUT) fl |= FLAG_RAW_OUT;
+    if (p->desc.products & FXC_PRODUCT_ACF_OVERLAP_ONLY) fl |= FLAG_ACF_OVERLAP;
+    a.flags = (int32_t)fl;
+}
+
+extern "C" fxc_status fxc_plan_create(const fxc_plan_desc *desc, fxc_plan **out) {
+    if (!desc || !out) return fail(FXC_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    const int N = desc->fft_len;
+    if (!is_pow2(N) || !sector_supported(N))
+        return fail(FXC_ERR_UNSUPPORTED, "fft_len %d: the GPU path supports powers of two 64..16384", N);
+    for (int i = 0; i < 2; ++i) {
+        fxc_status s = validate_station(desc->st[i], i);
+        if (s != FXC_OK) return s;
+    }
+    const int64_t M = N / 2, half = M + 1;
+    if (desc->a1s < 0 || desc->a2s < 0 || desc->a1e <= desc->a1s)
+        return fail(FXC_ERR_NO_OVERLAP, "empty band overlap [%lld, %lld)", (long long)desc->a1s,
+                    (long long)desc->a1e);
+    const int64_t ov = desc->a1e - desc->a1s;
+    if (desc->a1e > half || desc->a2s + ov > half)
+        return fail(FXC_ERR_INVALID_ARG, "band alignment exceeds the %lld-bin half spectrum", (long long)half);
+    if (desc->out_grid != FXC_GRID_ANT1 && desc->out_grid != FXC_GRID_ANT2)
+        return fail(FXC_ERR_INVALID_ARG, "out_grid must be 0 or 1");
+    if (desc->precision != FXC_PRECISION_PLAIN_FP32 && desc->precision != FXC_PRECISION_CHOSEN)
+        return fail(FXC_ERR_INVALID_ARG, "unknown precision %d", desc->precision);
+    if (!(desc->df_hz > 0.0)) return fail(FXC_ERR_INVALID_ARG, "df_hz must be > 0");
+    const uint32_t known = FXC_PRODUCT_ACF | FXC_PRODUCT_XCF | FXC_PRODUCT_PHASED |
+                           FXC_PRODUCT_RAW_OUT | FXC_PRODUCT_ACF_OVERLAP_ONLY;
+    if (desc->products & ~known) return fail(FXC_ERR_INVALID_ARG, "unknown product bits");
+    if (desc->products & (FXC_PRODUCT_PHASED | FXC_PRODUCT_RAW_OUT)) {
+        if (desc->products & (FXC_PRODUCT_ACF | FXC_PRODUCT_XCF))
+            return fail(FXC_ERR_INVALID_ARG, "phased products are computed by their own plan");
+        if (desc->bit_out < 1 || desc->bit_out > 16)
+            return fail(FXC_ERR_UNSUPPORTED, "bit_out %d outside 1..16", desc->bit_out);
+        if (((int64_t)N * desc->bit_out) % 32 != 0)
+            return fail(FXC_ERR_UNSUPPORTED, "fft_len*bit_out must fill whole 32-bit words");
+        if (desc->st[0].nlevels > 256)
+            return fail(FXC_ERR_UNSUPPORTED, "phased output supports at most 256 station-1 levels");
+    }
+
+    int dev_count = 0;
+    hipError_t he = hipGetDeviceCount(&dev_count);
+    if (he != hipSuccess || dev_count < 1)
+        return fail(FXC_ERR_HIP, "no HIP device available (%s)", hipGetErrorString(he));
+
+    fxc_plan *p = new fxc_plan();
+    p->desc = *desc;
+    p->N = N;
+    p->M = (int)M;
+    p->half = (int)half;
+    p->T = sector_threads_per_frame(N);
+    (void)hipGetDevice(&p->device);
+    for (int i = 0; i < 2; ++i) {
+        p->levels[i].assign(desc->st[i].levels, desc->st[i].levels + desc->st[i].nlevels);
+        p->desc.st[i].levels = p->levels[i].data();
+        build_station(p, i);
+    }
+    auto cleanup = [&](fxc_status s) {
+        fxc_plan_destroy(p);
+        return s;
+    };
+    if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess)
+        return cleanup(fail(FXC_ERR_HIP, "hipStreamCreate failed"));
+
+    // twiddles: W_M^k = exp(-2*pi*i*k/M); post: (cos, sin)(2*pi*k/N)
+    std::vector<float> twf(2 * M), twp(2 * half);
+    for (int64_t k = 0; k < M; ++k) {
+        double ang = -2.0 * M_PI * (double)k / (double)M;
+        twf[2 * k] = (float)std::cos(ang);
+        twf[2 * k + 1] = (float)std::sin(ang);
+    }
+    for (int64_t k = 0; k < half; ++k) {
+        double ang = 2.0 * M_PI * (double)k / (double)N;
+        twp[2 * k] = (float)std::cos(ang);
+        twp[2 * k + 1] = (float)std::sin(ang);
+    }
+    if (p->tw_fft.ensure(twf.size() * 4) != hipSuccess || p->tw_post.ensure(twp.size() * 4) != hipSuccess)
+        return cleanup(fail(FXC_ERR_NOMEM, "device allocation failed"));
+    if (hipMemcpy(p->tw_fft.p, twf.data(), twf.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(p->tw_post.p, twp.data(), twp.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
+        return cleanup(fail(FXC_ERR_HIP, "twiddle upload failed"));
+    for (int i = 0; i < 2; ++i) {
+        std::vector<float> lv(p->levels[i].begin(), p->levels[i].end());
+        if (p->lev_dev[i].ensure(lv.size() * 4) != hipSuccess ||
+            hipMemcpy(p->lev_dev[i].p, lv.data(), lv.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
+            return cleanup(fail(FXC_ERR_HIP, "level upload failed"));
+    }
+    {
+        std::vector<float> lv(p->levels[0].begin(), p->levels[0].end());
+        if (p->out_lev.ensure(lv.size() * 4) != hipSuccess ||
+            hipMemcpy(p->out_lev.p, lv.data(), lv.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
+            return cleanup(fail(FXC_ERR_HIP, "level upload failed"));
+    }
+    // persistent-style grid: ~2048 one-wave workgroups' worth of threads
+    const int cus = num_cus();
+    p->max_wg = std::max(64, (cus * 8 * 64) / p->T);
+    if (p->part.ensure((size_t)p->max_wg * kNumProducts * half * sizeof(double)) != hipSuccess ||
+        p->red_scratch.ensure((size_t)64 * kNumProducts * half * sizeof(double)) != hipSuccess)
+        return cleanup(fail(FXC_ERR_NOMEM, "scratch allocation failed"));
+    *out = p;
+    return FXC_OK;
+}
+
+extern "C" fxc_status fxc_plan_destroy(fxc_plan *p) {
+    if (!p) return FXC_OK;
+    if (p->stream) (void)hipStreamSynchronize(p->stream);
+    for (auto &e : p->pending) {
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
+    }
+    for (auto &e : p->pool) {
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
+    }
+    DevBuf *bufs[] = {&p->tw_fft, &p->tw_post, &p->lev_dev[0], &p->lev_dev[1], &p->out_lev,
+                      &p->part, &p->red_scratch, &p->table_dev, &p->d_raw[0], &p->d_raw[1],
+                      &p->d_acc, &p->d_raw_out, &p->d_tmp_in, &p->d_tmp_out, &p->d_shifts};
+    for (DevBuf *b : bufs) b->release();
+    if (p->stream) (void)hipStreamDestroy(p->stream);
+    delete p;
+    return FXC_OK;
+}
+
+extern "C" fxc_status fxc_plan_get_info(const fxc_plan *p, fxc_plan_info *info) {
+    if (!p || !info) return fail(FXC_ERR_INVALID_ARG, "null argument");
+    info->fft_len = p->N;
+    info->half = p->half;
+    info->bpf1 = p->st[0].bpf;
+    info->bpf2 = p->st[1].bpf;
+    info->bpf_out = (int32_t)(((int64_t)p->N * p->desc.bit_out + 7) / 8);
+    info->decode_ok1 = p->decode_ok[0];
+    info->decode_ok2 = p->decode_ok[1];
+    info->fft_form = 0;
+    info->threads_per_frame = p->T;
+    info->device_id = p->device;
+    return FXC_OK;
+}
+
+static void fill_delay(const fxc_delay_cfg &c, const double *dev_table, DelayDev &d) {
+    d.frame_dt = c.frame_dt;
+    d.fs = c.fs;
+    d.time_offset_s = c.time_offset_s;
+    d.table = dev_table;
+    d.table_len = dev_table ? c.geom_table_len : 0;
+    d.coarse = c.coarse_delay_s;
+    d.user_samples = c.delay_user_samples;
+    d.extra_rate = c.extra_delay_rate_sps;
+    d.c1d = c.clock1_delay_s;
+    d.c1r = c.clock1_rate_sps;
+    d.c2d = c.clock2_delay_s;
+    d.c2r = c.clock2_rate_sps;
+    d.d_seek = c.d_seek;
+    d.nd1 = c.net_d1_base;
+    d.tr1 = c.total_rate1_base;
+    d.ta1 = c.total_accel1_base;
+    d.nd2 = c.net_d2_base;
+    d.tr2 = c.total_rate2_base;
+    d.ta2 = c.total_accel2_base;
+    d.lo1 = c.lo1_hz;
+    d.lo2 = c.lo2_hz;
+}
+
+// Upload the geometric table when it differs from the last one.
+static fxc_status upload_table(fxc_plan *p, const fxc_delay_cfg &c, hipStream_t s,
+                               const double **dev_table) {
+    *dev_table = nullptr;
+    if (!c.geom_table || c.geom_table_len <= 0) return FXC_OK;
+    const size_t n = (size_t)c.geom_table_len;
+    bool same = p->table_host.size() == n &&
+                memcmp(p->table_host.data(), c.geom_table, n * sizeof(double)) == 0;
+    if (!same) {
+        if (p->table_dev.ensure(n * sizeof(double)) != hipSuccess)
+            return fail(FXC_ERR_NOMEM, "table allocation failed");
+        p->table_host.assign(c.geom_table, c.geom_table + n);
+        HIP_TRY(hipMemcpyAsync(p->table_dev.p, p->table_host.data(), n * sizeof(double),
+                               hipMemcpyHostToDevice, s));
+        HIP_TRY(hipStreamSynchronize(s));
+    }
+    *dev_table = (const double *)p->table_dev.p;
+    return FXC_OK;
+}
+
+static fxc_status collect_timing(fxc_plan *p) {
+    for (auto &e : p->pending) {
+        HIP_TRY(hipEventSynchronize(e.second));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, e.first, e.second));
+        p->timed_ms += ms;
+        p->timed_launches++;
+        p->pool.push_back(e);
+    }
+    p->pending.clear();
+    return FXC_OK;
+}
+
+// Enqueue the fused kernel + deterministic reduce for one sector.
+static fxc_status enqueue_sector(fxc_plan *p, const uint8_t *d_raw1, const uint8_t *d_raw2,
+                                 int64_t first_frame, int64_t nf, const fxc_delay_cfg &dc,
+                                 double *d_acc, hipStream_t s) {
+    const double *dtab = nullptr;
+    fxc_status st = upload_table(p, dc, s, &dtab);
+    if (st != FXC_OK) return st;
+    const size_t acc_bytes = (size_t)kNumProducts * p->half * sizeof(double);
+    if (nf <= 0 || !p->decode_ok[0] || !p->decode_ok[1]) {
+        HIP_TRY(hipMemsetAsync(d_acc, 0, acc_bytes, s));
+        return FXC_OK;
+    }
+    if (p->desc.products & (FXC_PRODUCT_PHASED | FXC_PRODUCT_RAW_OUT))
+        return fail(FXC_ERR_UNSUPPORTED, "phased products are not available in this build");
+    SectorArgs a;
+    fill_args(p, a);
+    fill_delay(dc, dtab, a.dly);
+    a.raw1 = d_raw1;
+    a.raw2 = d_raw2;
+    a.first_frame = first_frame;
+    a.nf = nf;
+    int64_t fpw = (nf + p->max_wg - 1) / p->max_wg;
+    fpw = ((fpw + kFrameBatch - 1) / kFrameBatch) * kFrameBatch;
+    if (fpw < kFrameBatch) fpw = kFrameBatch;
+    const int n_wg = (int)((nf + fpw - 1) / fpw);
+    a.frames_per_wg = fpw;
+    a.part = (double *)p->part.p;
+    std::pair<hipEvent_t, hipEvent_t> ev{nullptr, nullptr};
+    if (p->timing) {
+        if (!p->pool.empty()) {
+            ev = p->pool.back();
+            p->pool.pop_back();
+        } else {
+            HIP_TRY(hipEventCreate(&ev.first));
+            HIP_TRY(hipEventCreate(&ev.second));
+        }
+        HIP_TRY(hipEventRecord(ev.first, s));
+    }
+    HIP_TRY(launch_sector(a, p->desc.precision, n_wg, s));
+    if (p->timing) {
+        HIP_TRY(hipEventRecord(ev.second, s));
+        p->pending.push_back(ev);
+    }
+    HIP_TRY(launch_reduce((const double *)p->part.p, n_wg, p->half, d_acc,
+                          (double *)p->red_scratch.p, 64, s));
+    return FXC_OK;
+}
+
+extern "C" fxc_status fxc_sector_run_device(fxc_plan *p, const uint8_t *d_raw1,
+                                            const uint8_t *d_raw2, int64_t first_frame, int64_t nf,
+                                            const fxc_delay_cfg *delay, double *d_acc,
+                                            uint8_t *d_raw_out, void *stream) {
+    if (!p || !delay || !d_acc) return fail(FXC_ERR_INVALID_ARG, "null argument");
+    if (nf < 0) return fail(FXC_ERR_INVALID_ARG, "nf < 0");
+    if (nf > 0 && (!d_raw1 || !d_raw2)) return fail(FXC_ERR_INVALID_ARG, "null raw buffer");
+    (void)d_raw_out;
+    hipStream_t s = stream ? (hipStream_t)stream : p->stream;
+    return enqueue_sector(p, d_raw1, d_raw2, first_frame, nf, *delay, d_acc, s);
+}
+
+extern "C" fxc_status fxc_sector_run(fxc_plan *p, const uint8_t *raw1, size_t len1,
+                                     const uint8_t *raw2, size_t len2, int64_t first_frame,
+                                     int64_t nf, const fxc_delay_cfg *delay, double *acc_ph,
+                                     double *acc11, double *acc12, double *acc22,
+                                     uint8_t *raw_out, int64_t *n_failed) {
+    if (!p || !delay) return fail(FXC_ERR_INVALID_ARG, "null argument");
+    if (nf < 0) return fail(FXC_ERR_INVALID_ARG, "nf < 0");
+    const size_t need1 = (size_t)nf * p->st[0].bpf, need2 = (size_t)nf * p->st[1].bpf;
+    if (nf > 0 && (!raw1 || !raw2 || len1 < need1 || len2 < need2))
+        return fail(FXC_ERR_INVALID_ARG, "raw buffers shorter than nf*bpf (%zu/%zu < %zu/%zu)", len1,
+                    len2, need1, need2);
+    (void)raw_out;
+    hipStream_t s = p->stream;
+    const size_t acc_bytes = (size_t)kNumProducts * p->half * sizeof(double);
+    if (p->d_acc.ensure(acc_bytes) != hipSuccess) return fail(FXC_ERR_NOMEM, "device allocation failed");
+    if (nf > 0) {
+        if (p->d_raw[0].ensure(need1) != hipSuccess || p->d_raw[1].ensure(need2) != hipSuccess)
+            return fail(FXC_ERR_NOMEM, "device allocation of %zu bytes failed", need1 + need2);
+        HIP_TRY(hipMemcpyAsync(p->d_raw[0].p, raw1, need1, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(p->d_raw[1].p, raw2, need2, hipMemcpyHostToDevice, s));
+    }
+    fxc_status st = enqueue_sector(p, (const uint8_t *)p->d_raw[0].p, (const uint8_t *)p->d_raw[1].p,
+                                   first_frame, nf, *delay, (double *)p->d_acc.p, s);
+    if (st != FXC_OK) return st;
+    std::vector<double> host((size_t)kNumProducts * p->half);
+    HIP_TRY(hipMemcpyAsync(host.data(), p->d_acc.p, acc_bytes, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (p->timing) {
+        fxc_status ts = collect_timing(p);
+        if (ts != FXC_OK) return ts;
+    }
+    const int h = p->half;
+    if (acc_ph) memcpy(acc_ph, host.data() + P_PH * h, h * sizeof(double));
+    if (acc11) memcpy(acc11, host.data() + P_11 * h, h * sizeof(double));
+    if (acc22) memcpy(acc22, host.data() + P_22 * h, h * sizeof(double));
+    if (acc12)
+        for (int k = 0; k < h; ++k) {
+            acc12[2 * k] = host[P_12R * h + k];
+            acc12[2 * k + 1] = host[P_12I * h + k];
+        }
+    if (n_failed) *n_failed = (p->decode_ok[0] && p->decode_ok[1]) ? 0 : nf;
+    return FXC_OK;
+}
+
+extern "C" fxc_status fxc_plan_set_timing(fxc_plan *p, int32_t enable) {
+    if (!p) return fail(FXC_ERR_INVALID_ARG, "null plan");
+    p->timing = enable != 0;
+    p->timed_ms = 0.0;
+    p->timed_launches = 0;
+    return FXC_OK;
+}
+
+extern "C" fxc_status fxc_plan_get_timing(fxc_plan *p, double *total_ms, int64_t *launches) {
+    if (!p || !total_ms || !launches) return fail(FXC_ERR_INVALID_ARG, "null argument");
+    fxc_status st = collect_timing(p);
+    if (st != FXC_OK) return st;
+    *total_ms = p->timed_ms;
+    *launches = p->timed_launches;
+    return FXC_OK;
+}
+
+// ================================================== per-function entries
+extern "C" fxc_status fxc_decode_frames(fxc_plan *p, int32_t station, const uint8_t *raw,
+                                        size_t len, int64_t nframes, const int64_t *shifts,
+                                        float *out, int64_t *n_failed) {
+    if (!p || !out || (station != 0 && station != 1) || nframes < 0)
+        return fail(FXC_ERR_INVALID_ARG, "bad decode arguments");
+    const StationDev &sd = p->st[station];
+    const size_t need = (size_t)nframes * sd.bpf;
+    if (nframes > 0 && (!raw || len < need)) return fail(FXC_ERR_INVALID_ARG, "raw shorter than nframes*bpf");
+    if (n_failed) *n_failed = sd.decode_ok ? 0 : nframes;
+    const size_t out_bytes = (size_t)nframes * p->N * sizeof(float);
+    if (nframes == 0) return FXC_OK;
+    if (!sd.decode_ok) {
+        memset(out, 0, out_bytes);
+        return FXC_OK;
+    }
+    hipStream_t s = p->stream;
+    if (p->d_tmp_in.ensure(need) != hipSuccess || p->d_tmp_out.ensure(out_bytes) != hipSuccess)
+        return fail(FXC_ERR_NOMEM, "device allocation failed");
+    HIP_TRY(hipMemcpyAsync(p->d_tmp_in.p, raw, need, hipMemcpyHostToDevice, s));
+    const int64_t *dsh = nullptr;
+    if (shifts) {
+        if (p->d_shifts.ensure((size_t)nframes * 8) != hipSuccess)
+            return fail(FXC_ERR_NOMEM, "device allocation failed");
+        HIP_TRY(hipMemcpyAsync(p->d_shifts.p, shifts, (size_t)nframes * 8, hipMemcpyHostToDevice, s));
+        dsh = (const int64_t *)p->d_shifts.p;
+    }
+    SectorArgs a;
+    fill_args(p, a);
+    a.nf = nframes;
+    if (station == 0) a.raw1 = (const uint8_t *)p->d_tmp_in.p;
+    else a.raw2 = (const uint8_t *)p->d_tmp_in.p;
+    HIP_TRY(launch_decode_frames(a, station, dsh, (float *)p->d_tmp_out.p, s));
+    HIP_TRY(hipMemcpyAsync(out, p->d_tmp_out.p, out_bytes, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return FXC_OK;
+}
+
+static fxc_status run_fft(fxc_plan *p, const float *in, int64_t nframes, float *out, bool inverse) {
+    if (!p || !in || !out || nframes < 0) return fail(FXC_ERR_INVALID_ARG, "bad fft arguments");
+    if (nframes == 0) return FXC_OK;
+    const size_t real_bytes = (size_t)nframes * p->N * sizeof(float);
+    const size_t cplx_bytes = (size_t)nframes * p->half * 2 * sizeof(float);
+    const size_t in_bytes = inverse ? cplx_bytes : real_bytes;
+    const size_t out_bytes = inverse ? real_bytes : cplx_bytes;
+    hipStream_t s = p->stream;
+    if (p->d_tmp_in.ensure(in_bytes) != hipSuccess || p->d_tmp_out.ensure(out_bytes) != hipSuccess)
+        return fail(FXC_ERR_NOMEM, "device allocation failed");
+    HIP_TRY(hipMemcpyAsync(p->d_tmp_in.p, in, in_bytes, hipMemcpyHostToDevice, s));
+    SectorArgs a;
+    fill_args(p, a);
+    if (inverse)
+        HIP_TRY(launch_irfft_frames(a, (const float2 *)p->d_tmp_in.p, nframes, (float *)p->d_tmp_out.p, s));
+    else
+        HIP_TRY(launch_rfft_frames(a, (const float *)p->d_tmp_in.p, nframes, (float2 *)p->d_tmp_out.p, s));
+    HIP_TRY(hipMemcpyAsync(out, p->d_tmp_out.p, out_bytes, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return FXC_OK;
+}
+
+extern "C" fxc_status fxc_rfft_frames(fxc_plan *p, const float *in, int64_t nframes, float *out) {
+    return run_fft(p, in, nframes, out, false);
+}
+extern "C" fxc_status fxc_irfft_frames(fxc_plan *p, const float *in, int64_t nframes, float *out) {
+    return run_fft(p, in, nframes, out, true);
+}
+
+extern "C" fxc_status fxc_delay_eval(const fxc_delay_cfg *cfg, int64_t first, int64_t n,
+                                     fxc_frame_delay *out) {
+    static_assert(sizeof(fxc_frame_delay) == sizeof(FrameDelayDev), "layout mismatch");
+    if (!cfg || !out || n < 0) return fail(FXC_ERR_INVALID_ARG, "bad delay_eval arguments");
+    if (n == 0) return FXC_OK;
+    int dc = 0;
+    if (hipGetDeviceCount(&dc) != hipSuccess || dc < 1) return fail(FXC_ERR_HIP, "no HIP device");
+    void *dtab = nullptr, *dout = nullptr;
+    hipStream_t s = nullptr;
+    HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    fxc_status rc = FXC_OK;
+    do {
+        if (cfg->geom_table && cfg->geom_table_len > 0) {
+            if (hipMalloc(&dtab, (size_t)cfg->geom_table_len * 8) != hipSuccess) {
+                rc = fail(FXC_ERR_NOMEM, "table allocation failed");
+                break;
+            }
+            if (hipMemcpy(dtab, cfg->geom_table, (size_t)cfg->geom_table_len * 8,
+                          hipMemcpyHostToDevice) != hipSuccess) {
+                rc = fail(FXC_ERR_HIP, "table upload failed");
+                break;
+            }
+        }
+        if (hipMalloc(&dout, (size_t)n * sizeof(FrameDelayDev)) != hipSuccess) {
+            rc = fail(FXC_ERR_NOMEM, "output allocation failed");
+            break;
+        }
+        DelayDev d;
+        fill_delay(*cfg, (const double *)dtab, d);
+        hipError_t e = launch_delay_eval(d, first, n, (FrameDelayDev *)dout, s);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(out, dout, (size_t)n * sizeof(FrameDelayDev), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) rc = fail(FXC_ERR_HIP, "delay_eval: %s", hipGetErrorString(e));
+    } while (0);
+    if (dtab) (void)hipFree(dtab);
+    if (dout) (void)hipFree(dout);
+    (void)hipStreamDestroy(s);
+    return rc;
+}
+
+extern "C" fxc_status fxc_quantise_frames(fxc_plan *p, const float *in, int64_t nframes,
+                                          uint8_t *out) {
+    if (!p || !in || !out || nframes < 0) return fail(FXC_ERR_INVALID_ARG, "bad quantise arguments");
+    if (p->desc.bit_out < 1 || p->desc.bit_out > 16 || ((int64_t)p->N * p->desc.bit_out) % 32 != 0)
+        return fail(FXC_ERR_UNSUPPORTED, "fft_len*bit_out must fill whole 32-bit words (bit_out 1..16)");
+    if (p->desc.st[0].nlevels > 256) return fail(FXC_ERR_UNSUPPORTED, "at most 256 output levels");
+    if (nframes == 0) return FXC_OK;
+    const size_t bpo = (size_t)p->N * p->desc.bit_out / 8;
+    const size_t in_bytes = (size_t)nframes * p->N * sizeof(float), out_bytes = (size_t)nframes * bpo;
+    hipStream_t s = p->stream;
+    if (p->d_tmp_in.ensure(in_bytes) != hipSuccess || p->d_tmp_out.ensure(out_bytes) != hipSuccess)
+        return fail(FXC_ERR_NOMEM, "device allocation failed");
+    HIP_TRY(hipMemcpyAsync(p->d_tmp_in.p, in, in_bytes, hipMemcpyHostToDevice, s));
+    SectorArgs a;
+    fill_args(p, a);
+    HIP_TRY(launch_quantise_frames(a, (const float *)p->d_tmp_in.p, nframes, (uint8_t *)p->d_tmp_out.p, s));
+    HIP_TRY(hipMemcpyAsync(out, p->d_tmp_out.p, out_bytes, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return FXC_OK;
+}

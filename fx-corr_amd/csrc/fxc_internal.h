@@ -1,0 +1,101 @@
+// Internal declarations shared by the C-ABI layer (fxc_api.cpp) and the
+// device code (fxc_device.hip).  Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace fxc {
+
+constexpr int kMaxBits = 16;       // widest sample the GPU decoder accepts
+constexpr int kFrameBatch = 8;     // frames whose delay entries one batch computes
+constexpr int kMaxPow = 14;        // step^(2^b), b < kMaxPow (covers M <= 8192)
+constexpr int kNumProducts = 5;    // [ph | 11 | 22 | 12re | 12im]
+enum { P_PH = 0, P_11 = 1, P_22 = 2, P_12R = 3, P_12I = 4 };
+
+enum { SHUF_IDENTITY = 0, SHUF_PAIRSWAP = 1, SHUF_GENERIC = 2 };
+enum { FLAG_ACF = 1, FLAG_XCF = 2, FLAG_PHASED = 4, FLAG_RAW_OUT = 8, FLAG_ACF_OVERLAP = 16 };
+
+// One station's decode description, device side (utils.rs:147-155).
+struct StationDev {
+    int32_t bits;
+    int32_t lsb;
+    int32_t shuffle_kind;
+    int32_t bpf;          // bytes per frame
+    int32_t nwords;       // 32-bit words staged per frame (ceil(bpf/4))
+    int32_t aligned;      // bpf % 4 == 0: frames start on a word boundary
+    int32_t decode_ok;    // plan-level: every frame decodes to N samples
+    uint32_t code_mask;
+    int32_t shifts[32];   // out bit i <- in bit shifts[i]
+    float lev4[16];       // levels for bits <= 4 (kernel-argument resident)
+    const float *levels;  // device table, 1<<bits entries
+};
+
+// DelayEvalConfig (main.rs:161-182), device side.
+struct DelayDev {
+    double frame_dt, fs, time_offset_s;
+    const double *table;
+    int64_t table_len;
+    double coarse, user_samples, extra_rate;
+    double c1d, c1r, c2d, c2r;
+    double d_seek;
+    double nd1, tr1, ta1, nd2, tr2, ta2;
+    double lo1, lo2;
+};
+
+// FrameDelayEntry (main.rs:151-159), device side.
+struct FrameDelayDev {
+    int64_t int1, int2;
+    double frac1, frac2;
+    float fr1[2], fr2[2];
+};
+
+struct SectorArgs {
+    const uint8_t *raw1;
+    const uint8_t *raw2;
+    int64_t first_frame;     // global index of frame 0 of this launch
+    int64_t nf;              // frames in this launch
+    int64_t frames_per_wg;
+    StationDev st[2];
+    DelayDev dly;
+    const float2 *tw_fft;    // W_M^k, k < M
+    const float2 *tw_post;   // (cos, sin)(2*pi*k/N), k <= M
+    int32_t N, M;
+    int32_t a1s, a1e, a2s, ov;
+    int32_t grid;            // 0 = Ant1, 1 = Ant2
+    int32_t flags;
+    double df;
+    double w1, w2;
+    int32_t bit_out;
+    int32_t output_lsb;
+    int32_t out_shifts[32];  // station-1 shuffle used on output words
+    const float *out_levels; // station-1 levels (f32 as the reference compares)
+    int32_t out_nlevels;
+    int32_t bpf_out;
+    uint8_t *raw_out;        // phased .raw frames (nf * bpf_out) or null
+    double *part;            // [n_wg][kNumProducts][half]
+};
+
+// Launchers implemented in fxc_device.hip.  All enqueue on `stream`.
+hipError_t launch_sector(const SectorArgs &a, int precision, int n_wg,
+                         hipStream_t stream);
+hipError_t launch_reduce(const double *part, int n_wg, int half, double *out,
+                         double *scratch, int n_chunks, hipStream_t stream);
+int sector_threads_per_frame(int N);
+bool sector_supported(int N);
+
+hipError_t launch_decode_frames(const SectorArgs &a, int station,
+                                const int64_t *shifts, float *out,
+                                hipStream_t stream);
+hipError_t launch_rfft_frames(const SectorArgs &a, const float *in, int64_t nframes,
+                              float2 *out, hipStream_t stream);
+hipError_t launch_irfft_frames(const SectorArgs &a, const float2 *in,
+                               int64_t nframes, float *out, hipStream_t stream);
+hipError_t launch_delay_eval(const DelayDev &d, int64_t first, int64_t n,
+                             FrameDelayDev *out, hipStream_t stream);
+hipError_t launch_quantise_frames(const SectorArgs &a, const float *in,
+                                  int64_t nframes, uint8_t *out,
+                                  hipStream_t stream);
+
+}  // namespace fxc

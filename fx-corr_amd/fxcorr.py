@@ -1,0 +1,395 @@
+"""ctypes binding of libfxcorr.so (the C ABI declared in include/fxcorr.h).
+
+The Python side only marshals arguments; every computation runs in the HIP
+library.  There is no CPU fallback: if the library or a GPU is missing the
+calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfxcorr.so")
+_lib = None
+
+OK = 0
+GRID_ANT1, GRID_ANT2 = 0, 1
+PRODUCT_ACF, PRODUCT_XCF, PRODUCT_PHASED, PRODUCT_RAW_OUT, PRODUCT_ACF_OVERLAP_ONLY = 1, 2, 4, 8, 16
+PRECISION_PLAIN_FP32, PRECISION_CHOSEN = 0, 1
+N_PRODUCTS = 5  # [ph | 11 | 22 | 12re | 12im]
+
+EXPORTED = [
+    "fxc_version", "fxc_strerror", "fxc_last_error", "fxc_device_count", "fxc_dev_alloc",
+    "fxc_dev_free", "fxc_memcpy_h2d", "fxc_memcpy_d2h", "fxc_memset_d", "fxc_stream_create",
+    "fxc_stream_destroy", "fxc_stream_sync", "fxc_event_create", "fxc_event_destroy",
+    "fxc_event_record", "fxc_event_elapsed_ms", "fxc_parse_shuffle", "fxc_parse_levels",
+    "fxc_quantization_mean_power", "fxc_compute_band_alignment", "fxc_sector_partition",
+    "fxc_split_delay", "fxc_plan_create", "fxc_plan_destroy", "fxc_plan_get_info",
+    "fxc_sector_run", "fxc_sector_run_device", "fxc_plan_set_timing", "fxc_plan_get_timing",
+    "fxc_decode_frames", "fxc_rfft_frames", "fxc_irfft_frames", "fxc_delay_eval",
+    "fxc_quantise_frames",
+]
+
+
+class FxcError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"fxcorr status {status}: {msg}")
+        self.status = status
+
+
+class StationDesc(C.Structure):
+    _fields_ = [("bits", C.c_int32), ("lsb", C.c_int32), ("shuffle_internal", C.c_int32 * 32),
+                ("levels", C.POINTER(C.c_double)), ("nlevels", C.c_int32)]
+
+
+class PlanDesc(C.Structure):
+    _fields_ = [("fft_len", C.c_int32), ("st", StationDesc * 2), ("a1s", C.c_int64),
+                ("a1e", C.c_int64), ("a2s", C.c_int64), ("out_grid", C.c_int32),
+                ("df_hz", C.c_double), ("products", C.c_uint32), ("w1", C.c_double),
+                ("w2", C.c_double), ("bit_out", C.c_int32), ("output_lsb", C.c_int32),
+                ("precision", C.c_int32)]
+
+
+class DelayCfg(C.Structure):
+    _fields_ = [
+        ("frame_dt", C.c_double), ("fs", C.c_double), ("time_offset_s", C.c_double),
+        ("geom_table", C.POINTER(C.c_double)), ("geom_table_len", C.c_int64),
+        ("coarse_delay_s", C.c_double), ("delay_user_samples", C.c_double),
+        ("extra_delay_rate_sps", C.c_double),
+        ("clock1_delay_s", C.c_double), ("clock1_rate_sps", C.c_double),
+        ("clock2_delay_s", C.c_double), ("clock2_rate_sps", C.c_double),
+        ("d_seek", C.c_double),
+        ("net_d1_base", C.c_double), ("total_rate1_base", C.c_double),
+        ("total_accel1_base", C.c_double),
+        ("net_d2_base", C.c_double), ("total_rate2_base", C.c_double),
+        ("total_accel2_base", C.c_double),
+        ("lo1_hz", C.c_double), ("lo2_hz", C.c_double),
+    ]
+
+
+class FrameDelay(C.Structure):
+    _fields_ = [("int1", C.c_int64), ("int2", C.c_int64), ("frac1", C.c_double),
+                ("frac2", C.c_double), ("fr_lo1", C.c_float * 2), ("fr_lo2", C.c_float * 2)]
+
+
+class BandAlignment(C.Structure):
+    _fields_ = [("shift_bins", C.c_int64), ("a1s", C.c_int64), ("a1e", C.c_int64),
+                ("a2s", C.c_int64)]
+
+
+class PlanInfo(C.Structure):
+    _fields_ = [("fft_len", C.c_int32), ("half", C.c_int32), ("bpf1", C.c_int32),
+                ("bpf2", C.c_int32), ("bpf_out", C.c_int32), ("decode_ok1", C.c_int32),
+                ("decode_ok2", C.c_int32), ("fft_form", C.c_int32),
+                ("threads_per_frame", C.c_int32), ("device_id", C.c_int32)]
+
+
+def _P(t):
+    return C.POINTER(t)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FileNotFoundError(
+                f"{LIB_PATH} is missing: build it with `python fx-corr_amd/build.py`")
+        L = C.CDLL(LIB_PATH)
+        vp, i32, i64, sz, d = C.c_void_p, C.c_int32, C.c_int64, C.c_size_t, C.c_double
+        sig = {
+            "fxc_version": ([], C.c_char_p), "fxc_strerror": ([i32], C.c_char_p),
+            "fxc_last_error": ([], C.c_char_p), "fxc_device_count": ([_P(i32)], i32),
+            "fxc_dev_alloc": ([sz, _P(vp)], i32), "fxc_dev_free": ([vp], i32),
+            "fxc_memcpy_h2d": ([vp, vp, sz, vp], i32), "fxc_memcpy_d2h": ([vp, vp, sz, vp], i32),
+            "fxc_memset_d": ([vp, C.c_int, sz, vp], i32),
+            "fxc_stream_create": ([_P(vp)], i32), "fxc_stream_destroy": ([vp], i32),
+            "fxc_stream_sync": ([vp], i32), "fxc_event_create": ([_P(vp)], i32),
+            "fxc_event_destroy": ([vp], i32), "fxc_event_record": ([vp, vp], i32),
+            "fxc_event_elapsed_ms": ([vp, vp, _P(C.c_float)], i32),
+            "fxc_parse_shuffle": ([C.c_char_p, _P(i32)], i32),
+            "fxc_parse_levels": ([i32, C.c_char_p, _P(d), i32, _P(i32)], i32),
+            "fxc_quantization_mean_power": ([_P(d), i32, _P(d)], i32),
+            "fxc_compute_band_alignment": ([i32, d, d, d, d, d, _P(BandAlignment)], i32),
+            "fxc_sector_partition": ([i64, d, _P(i64), i64, _P(i64)], i32),
+            "fxc_split_delay": ([d, d, _P(i64), _P(d)], i32),
+            "fxc_plan_create": ([_P(PlanDesc), _P(vp)], i32),
+            "fxc_plan_destroy": ([vp], i32),
+            "fxc_plan_get_info": ([vp, _P(PlanInfo)], i32),
+            "fxc_sector_run": ([vp, vp, sz, vp, sz, i64, i64, _P(DelayCfg), vp, vp, vp, vp, vp,
+                                _P(i64)], i32),
+            "fxc_sector_run_device": ([vp, vp, vp, i64, i64, _P(DelayCfg), vp, vp, vp], i32),
+            "fxc_plan_set_timing": ([vp, i32], i32),
+            "fxc_plan_get_timing": ([vp, _P(d), _P(i64)], i32),
+            "fxc_decode_frames": ([vp, i32, vp, sz, i64, vp, vp, _P(i64)], i32),
+            "fxc_rfft_frames": ([vp, vp, i64, vp], i32),
+            "fxc_irfft_frames": ([vp, vp, i64, vp], i32),
+            "fxc_delay_eval": ([_P(DelayCfg), i64, i64, _P(FrameDelay)], i32),
+            "fxc_quantise_frames": ([vp, vp, i64, vp], i32),
+        }
+        for name, (args, res) in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+def check(status: int):
+    if status != OK:
+        raise FxcError(status, lib().fxc_last_error().decode(errors="replace"))
+
+
+def _vp(a: np.ndarray):
+    return C.c_void_p(a.ctypes.data)
+
+
+# ------------------------------------------------------------ host helpers
+def parse_shuffle(text: str) -> np.ndarray:
+    out = (C.c_int32 * 32)()
+    check(lib().fxc_parse_shuffle(text.encode(), out))
+    return np.array(out[:], dtype=np.int32)
+
+
+def parse_levels(bits: int, text: str) -> np.ndarray:
+    cap = 1 << min(max(bits, 0), 20)
+    buf = (C.c_double * cap)()
+    n = C.c_int32()
+    check(lib().fxc_parse_levels(bits, text.encode(), buf, cap, C.byref(n)))
+    return np.array(buf[:n.value])
+
+
+def mean_power(levels) -> float:
+    lv = np.ascontiguousarray(levels, dtype=np.float64)
+    p = C.c_double()
+    check(lib().fxc_quantization_mean_power(lv.ctypes.data_as(_P(C.c_double)), len(lv),
+                                            C.byref(p)))
+    return p.value
+
+
+def band_alignment(fft, fs, c1_mhz, c2_mhz, bw1_mhz, bw2_mhz):
+    o = BandAlignment()
+    check(lib().fxc_compute_band_alignment(fft, fs, c1_mhz, c2_mhz, bw1_mhz, bw2_mhz,
+                                           C.byref(o)))
+    return (o.shift_bins, o.a1s, o.a1e, o.a2s)
+
+
+def sector_partition(total_frames: int, frame_sec: float):
+    n = C.c_int64()
+    check(lib().fxc_sector_partition(total_frames, frame_sec, None, 0, C.byref(n)))
+    buf = (C.c_int64 * max(1, n.value))()
+    check(lib().fxc_sector_partition(total_frames, frame_sec, buf, n.value, C.byref(n)))
+    return list(buf[:n.value])
+
+
+def split_delay(delay_s: float, fs: float):
+    n = C.c_int64()
+    f = C.c_double()
+    check(lib().fxc_split_delay(delay_s, fs, C.byref(n), C.byref(f)))
+    return n.value, f.value
+
+
+def device_count() -> int:
+    n = C.c_int32()
+    st = lib().fxc_device_count(C.byref(n))
+    return n.value if st == OK else 0
+
+
+def make_delay_cfg(table=None, **kw) -> DelayCfg:
+    cfg = DelayCfg()
+    for k, v in kw.items():
+        setattr(cfg, k, v)
+    if table is not None:
+        t = np.ascontiguousarray(table, dtype=np.float64)
+        cfg._table = t
+        cfg.geom_table = t.ctypes.data_as(_P(C.c_double))
+        cfg.geom_table_len = len(t)
+    else:
+        cfg.geom_table = None
+        cfg.geom_table_len = 0
+    return cfg
+
+
+def delay_eval(cfg: DelayCfg, first: int, n: int):
+    out = (FrameDelay * max(1, n))()
+    check(lib().fxc_delay_eval(C.byref(cfg), first, n, out))
+    return out[:n]
+
+
+IDENTITY_SHUFFLE = np.arange(32, dtype=np.int32)
+
+
+# --------------------------------------------------------------------- plan
+class Plan:
+    """One fxc_plan (per-sector constants + device tables)."""
+
+    def __init__(self, fft_len, bits=(2, 2), levels=None, shuffles=None, lsb=(False, False),
+                 band=None, df_hz=1.0, products=PRODUCT_ACF | PRODUCT_XCF, out_grid=GRID_ANT1,
+                 w=(1.0, 1.0), bit_out=None, output_lsb=False, precision=PRECISION_PLAIN_FP32):
+        d = PlanDesc()
+        d.fft_len = fft_len
+        self._keep = []
+        for i in range(2):
+            lv = np.ascontiguousarray(levels[i], dtype=np.float64)
+            self._keep.append(lv)
+            sh = IDENTITY_SHUFFLE if shuffles is None or shuffles[i] is None else shuffles[i]
+            d.st[i].bits = bits[i]
+            d.st[i].lsb = int(bool(lsb[i]))
+            d.st[i].shuffle_internal[:] = [int(x) for x in sh]
+            d.st[i].levels = lv.ctypes.data_as(_P(C.c_double))
+            d.st[i].nlevels = len(lv)
+        half = fft_len // 2 + 1
+        if band is None:
+            band = (0, 0, fft_len // 2, 0)
+        d.a1s, d.a1e, d.a2s = band[1], band[2], band[3]
+        d.out_grid = out_grid
+        d.df_hz = df_hz
+        d.products = products
+        d.w1, d.w2 = w
+        d.bit_out = bit_out if bit_out is not None else max(bits)
+        d.output_lsb = int(bool(output_lsb))
+        d.precision = precision
+        self.desc = d
+        self.fft_len = fft_len
+        self.half = half
+        h = C.c_void_p()
+        check(lib().fxc_plan_create(C.byref(d), C.byref(h)))
+        self.h = h
+        info = PlanInfo()
+        check(lib().fxc_plan_get_info(h, C.byref(info)))
+        self.info = info
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().fxc_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- per-sector hot path on host buffers
+    def sector_run(self, raw1, raw2, first_frame, nf, cfg: DelayCfg):
+        raw1 = np.ascontiguousarray(raw1, dtype=np.uint8)
+        raw2 = np.ascontiguousarray(raw2, dtype=np.uint8)
+        h = self.half
+        acc_ph, acc11, acc22 = np.zeros(h), np.zeros(h), np.zeros(h)
+        acc12 = np.zeros(2 * h)
+        raw_out = np.zeros(max(1, nf * self.info.bpf_out), dtype=np.uint8)
+        nfail = C.c_int64()
+        check(lib().fxc_sector_run(self.h, _vp(raw1), raw1.nbytes, _vp(raw2), raw2.nbytes,
+                                   first_frame, nf, C.byref(cfg), _vp(acc_ph), _vp(acc11),
+                                   _vp(acc12), _vp(acc22), _vp(raw_out), C.byref(nfail)))
+        return dict(acc_ph=acc_ph, acc11=acc11, acc22=acc22, acc12=acc12.view(np.complex128),
+                    raw_out=raw_out[:nf * self.info.bpf_out], n_failed=nfail.value)
+
+    def sector_run_device(self, d_raw1, d_raw2, first_frame, nf, cfg, d_acc, d_raw_out=None,
+                          stream=None):
+        check(lib().fxc_sector_run_device(self.h, C.c_void_p(d_raw1), C.c_void_p(d_raw2),
+                                          first_frame, nf, C.byref(cfg), C.c_void_p(d_acc),
+                                          C.c_void_p(d_raw_out or 0), C.c_void_p(stream or 0)))
+
+    def set_timing(self, on: bool):
+        check(lib().fxc_plan_set_timing(self.h, int(on)))
+
+    def get_timing(self):
+        ms = C.c_double()
+        n = C.c_int64()
+        check(lib().fxc_plan_get_timing(self.h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    # ---- per-function entries
+    def decode_frames(self, station, raw, nframes, shifts=None):
+        raw = np.ascontiguousarray(raw, dtype=np.uint8)
+        out = np.zeros(max(1, nframes * self.fft_len), dtype=np.float32)
+        nfail = C.c_int64()
+        sh = None
+        if shifts is not None:
+            sh = np.ascontiguousarray(shifts, dtype=np.int64)
+        check(lib().fxc_decode_frames(self.h, station, _vp(raw), raw.nbytes, nframes,
+                                      _vp(sh) if sh is not None else None, _vp(out),
+                                      C.byref(nfail)))
+        return out[:nframes * self.fft_len].reshape(nframes, self.fft_len), nfail.value
+
+    def rfft_frames(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, self.fft_len)
+        out = np.zeros((x.shape[0], self.half), dtype=np.complex64)
+        check(lib().fxc_rfft_frames(self.h, _vp(x), x.shape[0], _vp(out)))
+        return out
+
+    def irfft_frames(self, spec):
+        s = np.ascontiguousarray(spec, dtype=np.complex64).reshape(-1, self.half)
+        out = np.zeros((s.shape[0], self.fft_len), dtype=np.float32)
+        check(lib().fxc_irfft_frames(self.h, _vp(s), s.shape[0], _vp(out)))
+        return out
+
+    def quantise_frames(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, self.fft_len)
+        out = np.zeros((x.shape[0], self.info.bpf_out), dtype=np.uint8)
+        check(lib().fxc_quantise_frames(self.h, _vp(x), x.shape[0], _vp(out)))
+        return out
+
+
+# ------------------------------------------------------- device memory helpers
+class DeviceBuffer:
+    def __init__(self, nbytes: int):
+        p = C.c_void_p()
+        check(lib().fxc_dev_alloc(nbytes, C.byref(p)))
+        self.ptr = p.value
+        self.nbytes = nbytes
+
+    def upload(self, arr: np.ndarray, stream=None):
+        a = np.ascontiguousarray(arr)
+        check(lib().fxc_memcpy_h2d(C.c_void_p(self.ptr), _vp(a), a.nbytes,
+                                   C.c_void_p(stream or 0)))
+
+    def download(self, arr: np.ndarray, stream=None):
+        check(lib().fxc_memcpy_d2h(_vp(arr), C.c_void_p(self.ptr), arr.nbytes,
+                                   C.c_void_p(stream or 0)))
+        return arr
+
+    def free(self):
+        if self.ptr:
+            lib().fxc_dev_free(C.c_void_p(self.ptr))
+            self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Stream:
+    def __init__(self):
+        p = C.c_void_p()
+        check(lib().fxc_stream_create(C.byref(p)))
+        self.ptr = p.value
+
+    def sync(self):
+        check(lib().fxc_stream_sync(C.c_void_p(self.ptr)))
+
+    def destroy(self):
+        if self.ptr:
+            lib().fxc_stream_destroy(C.c_void_p(self.ptr))
+            self.ptr = 0
+
+
+class Event:
+    def __init__(self):
+        p = C.c_void_p()
+        check(lib().fxc_event_create(C.byref(p)))
+        self.ptr = p.value
+
+    def record(self, stream: Stream | None):
+        check(lib().fxc_event_record(C.c_void_p(self.ptr),
+                                     C.c_void_p(stream.ptr if stream else 0)))
+
+    def elapsed_ms(self, end: "Event") -> float:
+        ms = C.c_float()
+        check(lib().fxc_event_elapsed_ms(C.c_void_p(self.ptr), C.c_void_p(end.ptr), C.byref(ms)))
+        return ms.value

@@ -1,0 +1,245 @@
+/*
+ * fxcorr.h — C ABI of the MI355X FX hot path (libfxcorr.so).
+ *
+ * Replaces the per-sector compute of M-AKIMOTOO/fx-corr (src/main.rs:2173-2591)
+ * and the per-frame helpers it calls (src/utils.rs, src/main.rs:110-241,
+ * src/args.rs:194-210).  The reference is Rust with no FFI; the entry points
+ * below are what an FFI crate for that seam would bind (see INTEGRATION.md for
+ * the Rust `extern "C"` block).  Plain pointers and sizes only.
+ *
+ * Conventions
+ *   - Every function returns an fxc_status (0 = FXC_OK).  No exceptions cross
+ *     the ABI.  fxc_last_error() returns a thread-local message for the last
+ *     failure on the calling thread.
+ *   - Host buffers are caller-owned and borrowed for the duration of a call.
+ *   - Device state (tables, scratch) is owned by the plan.  One plan per host
+ *     thread at a time; calls on one plan are serialised by the caller.
+ *   - The product path has no CPU fallback: without a usable HIP device,
+ *     fxc_plan_create fails with FXC_ERR_HIP.
+ */
+#ifndef FXCORR_H
+#define FXCORR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t fxc_status;
+enum {
+    FXC_OK = 0,
+    FXC_ERR_INVALID_ARG = 1,
+    FXC_ERR_UNSUPPORTED = 2,
+    FXC_ERR_HIP = 3,
+    FXC_ERR_NO_OVERLAP = 4,
+    FXC_ERR_PARSE = 5,
+    FXC_ERR_IO = 6,
+    FXC_ERR_NOMEM = 7
+};
+
+/* Output grid: which station's bins index the products (main.rs:243-244, 1212). */
+enum { FXC_GRID_ANT1 = 0, FXC_GRID_ANT2 = 1 };
+
+/* Product mask (main.rs:2154-2160). */
+enum {
+    FXC_PRODUCT_ACF = 1u,              /* need_acf_products                     */
+    FXC_PRODUCT_XCF = 2u,              /* need_xcf_products                     */
+    FXC_PRODUCT_PHASED = 4u,           /* need_phased_products (write_raw path) */
+    FXC_PRODUCT_RAW_OUT = 8u,          /* also emit the quantised .raw frames   */
+    FXC_PRODUCT_ACF_OVERLAP_ONLY = 16u /* acf_overlap_only (yi-acf)             */
+};
+
+/* Arithmetic form of the hot path. */
+enum {
+    /* FP32 FFT; XCF phase reproduces the reference's f32 start/step
+     * (main.rs:133-149, 2513-2527) and f32 fringe phasors; FP64 integration. */
+    FXC_PRECISION_PLAIN_FP32 = 0,
+    /* FP64 fringe and per-bin fractional-delay phase (reduced mod 2*pi before
+     * the cast), FP64 integration; FFT as the plain form unless the plan is
+     * eligible for the FP16-MFMA four-step first stage (see fxc_plan_info). */
+    FXC_PRECISION_CHOSEN = 1
+};
+
+/* Per-station decode description.  Replaces build_decode_plan's inputs
+ * (utils.rs:173) plus the sideband flag (main.rs:1092). */
+typedef struct {
+    int32_t bits;                 /* 1..16 bits per sample                          */
+    int32_t lsb;                  /* 1 = LSB input: negate odd samples (to USB)     */
+    int32_t shuffle_internal[32]; /* out bit i <- in bit shuffle_internal[i]        */
+    const double *levels;         /* 2^bits levels (args.rs:194-200)                */
+    int32_t nlevels;
+} fxc_station_desc;
+
+/* Everything run_once captures for the per-sector closure (main.rs:2073-2160). */
+typedef struct {
+    int32_t fft_len;              /* power of two, 64..16384 (GPU path)             */
+    fxc_station_desc st[2];
+    int64_t a1s, a1e, a2s;        /* BandAlignment (main.rs:361)                    */
+    int32_t out_grid;             /* FXC_GRID_*                                     */
+    double df_hz;                 /* fs / fft_len (main.rs:1320)                    */
+    uint32_t products;            /* FXC_PRODUCT_* mask                             */
+    double w1, w2;                /* phased weights (main.rs:1306-1307)             */
+    int32_t bit_out;              /* phased .raw bits = max(bit1, bit2)             */
+    int32_t output_lsb;           /* lsb1 && lsb2 (main.rs:1093)                    */
+    int32_t precision;            /* FXC_PRECISION_*                                */
+} fxc_plan_desc;
+
+/* DelayEvalConfig (main.rs:161-182) as a POD.  geom_table is the 1-s
+ * geometric-delay table (main.rs:1629-1644); NULL selects the analytic branch
+ * (main.rs:209-230).  The table is copied to the device on each call. */
+typedef struct {
+    double frame_dt, fs, time_offset_s;
+    const double *geom_table;
+    int64_t geom_table_len;
+    double coarse_delay_s, delay_user_samples, extra_delay_rate_sps;
+    double clock1_delay_s, clock1_rate_sps, clock2_delay_s, clock2_rate_sps;
+    double d_seek;
+    double net_d1_base, total_rate1_base, total_accel1_base;
+    double net_d2_base, total_rate2_base, total_accel2_base;
+    double lo1_hz, lo2_hz;
+} fxc_delay_cfg;
+
+/* FrameDelayEntry (main.rs:151-159). */
+typedef struct {
+    int64_t int1, int2;
+    double frac1, frac2;
+    float fr_lo1[2], fr_lo2[2];
+} fxc_frame_delay;
+
+/* BandAlignment (main.rs:361). */
+typedef struct {
+    int64_t shift_bins, a1s, a1e, a2s;
+} fxc_band_alignment;
+
+typedef struct fxc_plan fxc_plan;
+
+/* Plan info for reporting. */
+typedef struct {
+    int32_t fft_len;
+    int32_t half;                 /* fft_len/2 + 1                                  */
+    int32_t bpf1, bpf2, bpf_out;  /* bytes per station-frame (main.rs:1214-1215)    */
+    int32_t decode_ok1, decode_ok2; /* 0: every frame fails decode (utils.rs:372)   */
+    int32_t fft_form;             /* 0 = FP32 Stockham, 1 = FP16-MFMA four-step     */
+    int32_t threads_per_frame;
+    int32_t device_id;
+} fxc_plan_info;
+
+/* ----------------------------------------------------------------------- */
+/* Library / device helpers (no reference counterpart: plumbing for hosts   */
+/* that hand over device-resident data).                                    */
+/* ----------------------------------------------------------------------- */
+const char *fxc_version(void);
+const char *fxc_strerror(fxc_status s);
+const char *fxc_last_error(void);
+fxc_status fxc_device_count(int32_t *n);
+fxc_status fxc_dev_alloc(size_t bytes, void **dptr);
+fxc_status fxc_dev_free(void *dptr);
+fxc_status fxc_memcpy_h2d(void *dst, const void *src, size_t bytes, void *stream);
+fxc_status fxc_memcpy_d2h(void *dst, const void *src, size_t bytes, void *stream);
+fxc_status fxc_memset_d(void *dst, int value, size_t bytes, void *stream);
+fxc_status fxc_stream_create(void **stream);
+fxc_status fxc_stream_destroy(void *stream);
+fxc_status fxc_stream_sync(void *stream);
+fxc_status fxc_event_create(void **ev);
+fxc_status fxc_event_destroy(void *ev);
+fxc_status fxc_event_record(void *ev, void *stream);
+fxc_status fxc_event_elapsed_ms(void *ev_start, void *ev_end, float *ms);
+
+/* ----------------------------------------------------------------------- */
+/* Host-side helpers of the hot-path boundary                               */
+/* ----------------------------------------------------------------------- */
+/* args.rs:202-210 parse_shuffle: external list -> internal map. */
+fxc_status fxc_parse_shuffle(const char *list, int32_t internal_out[32]);
+/* args.rs:194-200 parse_levels.  *n_out = 1<<bits on success. */
+fxc_status fxc_parse_levels(int32_t bits, const char *list, double *out,
+                            int32_t cap, int32_t *n_out);
+/* main.rs:378-387 quantization_mean_power. */
+fxc_status fxc_quantization_mean_power(const double *levels, int32_t n,
+                                       double *p_out);
+/* main.rs:360-372 compute_band_alignment (centres / bandwidths in MHz). */
+fxc_status fxc_compute_band_alignment(int32_t fft_len, double fs_hz,
+                                      double c1_mhz, double c2_mhz,
+                                      double bw1_mhz, double bw2_mhz,
+                                      fxc_band_alignment *out);
+/* main.rs:2058-2072 sector partition: counts[] (cap entries) of frames per
+ * sector; *n_sectors_out. */
+fxc_status fxc_sector_partition(int64_t total_frames, double frame_sec,
+                                int64_t *counts, int64_t cap,
+                                int64_t *n_sectors_out);
+/* main.rs:126-130 split_delay_to_integer_and_fractional (host). */
+fxc_status fxc_split_delay(double delay_s, double fs_hz, int64_t *n_out,
+                           double *frac_out);
+
+/* ----------------------------------------------------------------------- */
+/* Plans and the per-sector hot path                                        */
+/* ----------------------------------------------------------------------- */
+/* build_decode_plan x2 + FftHelper::new + captured constants
+ * (utils.rs:173-213, utils.rs:22-29, main.rs:2073-2074). */
+fxc_status fxc_plan_create(const fxc_plan_desc *desc, fxc_plan **out);
+fxc_status fxc_plan_destroy(fxc_plan *plan);
+fxc_status fxc_plan_get_info(const fxc_plan *plan, fxc_plan_info *info);
+
+/* The per-sector closure of run_once (main.rs:2173-2591) on host buffers.
+ * raw1/raw2: nf frames of bpf1/bpf2 bytes (len1/len2 >= nf*bpf).
+ * first_frame: global frame index of the sector's first frame (emitted).
+ * Outputs (length half, acc12 = 2*half interleaved re,im), OVERWRITTEN:
+ *   acc11/acc22/acc12 for ACF/XCF, acc_ph for PHASED; any may be NULL when
+ *   not requested.  raw_out: nf*bpf_out bytes when FXC_PRODUCT_RAW_OUT.
+ * n_failed: frames skipped (sector_failures, main.rs:2221, 2592). */
+fxc_status fxc_sector_run(fxc_plan *plan, const uint8_t *raw1, size_t len1,
+                          const uint8_t *raw2, size_t len2,
+                          int64_t first_frame, int64_t nf,
+                          const fxc_delay_cfg *delay, double *acc_ph,
+                          double *acc11, double *acc12, double *acc22,
+                          uint8_t *raw_out, int64_t *n_failed);
+
+/* Device-resident variant: d_raw1/d_raw2/d_acc/d_raw_out are device pointers.
+ * d_acc receives 5*half doubles laid out [ph | 11 | 22 | 12re | 12im].
+ * Enqueued on `stream` (NULL = the plan's stream); returns without waiting. */
+fxc_status fxc_sector_run_device(fxc_plan *plan, const uint8_t *d_raw1,
+                                 const uint8_t *d_raw2, int64_t first_frame,
+                                 int64_t nf, const fxc_delay_cfg *delay,
+                                 double *d_acc, uint8_t *d_raw_out,
+                                 void *stream);
+
+/* Kernel timing of the dominant (fused per-frame) kernel measured with HIP
+ * events on the launch stream; enabled per plan. */
+fxc_status fxc_plan_set_timing(fxc_plan *plan, int32_t enable);
+fxc_status fxc_plan_get_timing(fxc_plan *plan, double *total_ms,
+                               int64_t *launches);
+
+/* ----------------------------------------------------------------------- */
+/* Per-function entry points (each runs the device code of the fused path  */
+/* on its own; host buffers).                                               */
+/* ----------------------------------------------------------------------- */
+/* decode_block_into_with_plan (utils.rs:215-376) + optional
+ * apply_integer_sample_shift_zerofill (utils.rs:126-145) for nframes frames
+ * of station `station` (0/1).  shifts may be NULL (no shift).
+ * out: nframes*fft_len floats.  *n_failed: frames that fail decode. */
+fxc_status fxc_decode_frames(fxc_plan *plan, int32_t station,
+                             const uint8_t *raw, size_t len, int64_t nframes,
+                             const int64_t *shifts, float *out,
+                             int64_t *n_failed);
+/* FftHelper::forward_r2c_process (utils.rs:37-59) on nframes real frames:
+ * in nframes*fft_len floats, out nframes*(half) interleaved complex. */
+fxc_status fxc_rfft_frames(fxc_plan *plan, const float *in, int64_t nframes,
+                           float *out);
+/* FftHelper::inverse_c2r_process (utils.rs:60-66): in nframes*half complex,
+ * out nframes*fft_len floats (scaled 1/fft_len). */
+fxc_status fxc_irfft_frames(fxc_plan *plan, const float *in, int64_t nframes,
+                            float *out);
+/* compute_frame_delay_entry (main.rs:184-241) evaluated on the device in
+ * FP64 for frames first..first+n-1. */
+fxc_status fxc_delay_eval(const fxc_delay_cfg *cfg, int64_t first, int64_t n,
+                          fxc_frame_delay *out);
+/* quantise_frame (utils.rs:378-410) with the plan's bit_out / station-1
+ * levels / station-1 shuffle: in nframes*fft_len floats -> nframes*bpf_out. */
+fxc_status fxc_quantise_frames(fxc_plan *plan, const float *in, int64_t nframes,
+                               uint8_t *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FXCORR_H */

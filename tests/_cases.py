@@ -1,0 +1,75 @@
+"""Workload descriptions shared by the parity tests: one dict drives both
+the oracle (fxref) and the HIP library (fxcorr) on the same seeded bytes."""
+from __future__ import annotations
+
+import numpy as np
+
+import fxcorr as F
+import fxref as R
+from _synth import IDENT, LEVELS_2BIT, make_pair_raw, uniform_levels
+
+ZERO_DELAY = dict(time_offset_s=0.0, coarse_delay_s=0.0, delay_user_samples=0.0,
+                  extra_delay_rate_sps=0.0, clock1_delay_s=0.0, clock1_rate_sps=0.0,
+                  clock2_delay_s=0.0, clock2_rate_sps=0.0, d_seek=0.0, net_d1_base=0.0,
+                  total_rate1_base=0.0, total_accel1_base=0.0, net_d2_base=0.0,
+                  total_rate2_base=0.0, total_accel2_base=0.0, lo1_hz=0.0, lo2_hz=0.0)
+
+
+def make_case(n=1024, fs=64e6, nf=256, bits=(2, 2), shuffles=(None, None), lsb=(False, False),
+              band=None, grid=0, products=F.PRODUCT_ACF | F.PRODUCT_XCF, seed=0, rho=0.05,
+              delay_samples=5.3, table=None, first_frame=0, **delay_kw):
+    sh = tuple(IDENT if s is None else np.asarray(s, dtype=np.int32) for s in shuffles)
+    raw1, raw2, levels = make_pair_raw(nf, n, bits=bits, sh=sh, lsb=lsb, rho=rho,
+                                       delay=delay_samples, seed=seed)
+    if band is None:
+        band = (0, 0, n // 2, 0)
+    dk = dict(ZERO_DELAY)
+    dk.update(frame_dt=n / fs, fs=fs)
+    dk.update(delay_kw)
+    if table is None and "no_table" not in delay_kw:
+        table = np.zeros(max(2, int(np.ceil(nf * n / fs)) + 1))
+    dk.pop("no_table", None)
+    return dict(n=n, fs=fs, nf=nf, bits=bits, levels=levels, shuffles=sh, lsb=lsb, band=band,
+                grid=grid, products=products, raw1=raw1, raw2=raw2, delay=dk, table=table,
+                first_frame=first_frame)
+
+
+def oracle_run(c, mode=R.MODE_F32, threads=8):
+    n = c["n"]
+    d = dict(fft_len=n, bits1=c["bits"][0], bits2=c["bits"][1], levels1=c["levels"][0],
+             levels2=c["levels"][1], shuffle1=c["shuffles"][0], shuffle2=c["shuffles"][1],
+             lsb1=int(c["lsb"][0]), lsb2=int(c["lsb"][1]), a1s=c["band"][1], a1e=c["band"][2],
+             a2s=c["band"][3], out_grid=c["grid"], df_hz=c["fs"] / n,
+             need_acf=int(bool(c["products"] & F.PRODUCT_ACF)),
+             need_xcf=int(bool(c["products"] & F.PRODUCT_XCF)),
+             acf_overlap_only=int(bool(c["products"] & F.PRODUCT_ACF_OVERLAP_ONLY)),
+             cpu_threads=threads)
+    cfg = R.make_delay_cfg(c["table"], **c["delay"])
+    return R.sector_run(d, c["raw1"], c["raw2"], c["first_frame"], c["nf"], cfg, mode=mode,
+                        threads=threads)
+
+
+def gpu_plan(c, precision=F.PRECISION_PLAIN_FP32):
+    return F.Plan(c["n"], bits=c["bits"], levels=c["levels"], shuffles=c["shuffles"],
+                  lsb=c["lsb"], band=c["band"], df_hz=c["fs"] / c["n"], products=c["products"],
+                  out_grid=c["grid"], precision=precision)
+
+
+def gpu_run(c, precision=F.PRECISION_PLAIN_FP32, plan=None):
+    p = plan or gpu_plan(c, precision)
+    cfg = F.make_delay_cfg(c["table"], **c["delay"])
+    return p.sector_run(c["raw1"], c["raw2"], c["first_frame"], c["nf"], cfg)
+
+
+def rel_err(got, ref):
+    """Per-bin relative error |got-ref|/|ref| over bins where ref != 0."""
+    got = np.asarray(got)
+    ref = np.asarray(ref)
+    m = np.abs(ref) > 0
+    if not np.any(m):
+        return 0.0
+    return float(np.max(np.abs(got[m] - ref[m]) / np.abs(ref[m])))
+
+
+def zero_pattern_equal(got, ref):
+    return np.array_equal(np.asarray(got) == 0, np.asarray(ref) == 0)

@@ -1,0 +1,254 @@
+"""GPU parity of the HIP hot path against the oracle, called through the C ABI.
+
+Bars (north star): decode, integer shifts and delay integers bit-exact;
+plain FP32 spectra within 1e-5 relative of the reference-faithful oracle;
+chosen form within 1e-4 relative of the FP64 oracle."""
+import numpy as np
+import pytest
+
+import fxcorr as F
+import fxref as R
+from _cases import gpu_plan, gpu_run, make_case, oracle_run, rel_err, zero_pattern_equal
+from _synth import IDENT, LEVELS_2BIT, uniform_levels
+
+pytestmark = pytest.mark.gpu
+
+PLAIN_TOL = 1e-5
+CHOSEN_TOL = 1e-4
+
+
+def _plan(n, bits=(2, 2), levels=None, shuffles=(None, None), lsb=(False, False), **kw):
+    if levels is None:
+        levels = [LEVELS_2BIT if b == 2 else uniform_levels(b) for b in bits]
+    return F.Plan(n, bits=bits, levels=levels, shuffles=shuffles, lsb=lsb, df_hz=1.0, **kw)
+
+
+# ----------------------------------------------------------------- decode
+@pytest.mark.parametrize("bits", [1, 2, 3, 4, 5, 8, 12, 16])
+@pytest.mark.parametrize("kind", ["ident", "pair", "generic"])
+def test_decode_bit_exact(bits, kind):
+    rng = np.random.default_rng(100 * bits + len(kind))
+    n, nfr = 1024, 24
+    sh = {"ident": IDENT, "pair": IDENT ^ 1, "generic": rng.permutation(32).astype(np.int32)}[kind]
+    lv = rng.standard_normal(1 << bits)
+    bpf = n * bits // 8
+    raw = rng.integers(0, 256, size=nfr * bpf, dtype=np.uint8)
+    shifts = rng.integers(-40, 40, size=nfr)
+    shifts[:4] = [0, n, -n, 5000]
+    for lsb in (False, True):
+        p = _plan(n, bits=(bits, 2), levels=[lv, LEVELS_2BIT], shuffles=(sh, None),
+                  lsb=(lsb, False))
+        got, nfail = p.decode_frames(0, raw, nfr, shifts)
+        assert nfail == 0
+        op = R.DecodePlan(bits, sh, lv)
+        for f in range(nfr):
+            ok, v = op.decode(raw[f * bpf:(f + 1) * bpf], n, lsb)
+            assert ok
+            exp = R.shift_zerofill(v, int(shifts[f]))
+            assert np.array_equal(got[f].view(np.uint32), exp.view(np.uint32)), (f, lsb)
+        p.close()
+
+
+def test_decode_station2_and_no_shift():
+    rng = np.random.default_rng(9)
+    n, nfr = 256, 10
+    sh = rng.permutation(32).astype(np.int32)
+    raw = rng.integers(0, 256, size=nfr * n // 4, dtype=np.uint8)
+    p = _plan(n, shuffles=(None, sh), lsb=(False, True))
+    got, _ = p.decode_frames(1, raw, nfr)
+    op = R.DecodePlan(2, sh, LEVELS_2BIT)
+    for f in range(nfr):
+        ok, v = op.decode(raw[f * 64:(f + 1) * 64], n, True)
+        assert ok and np.array_equal(got[f], v)
+
+
+# ------------------------------------------------------------------ delay
+def _delay_cfg_pair(table, **kw):
+    base = dict(frame_dt=1024 / 1024e6, fs=1024e6, time_offset_s=3.0, coarse_delay_s=1.1e-9,
+                delay_user_samples=0.2, extra_delay_rate_sps=7.46e-12, clock1_delay_s=0.0,
+                clock1_rate_sps=0.0, clock2_delay_s=1.2e-6, clock2_rate_sps=1e-12,
+                d_seek=1536 / 1024e6, net_d1_base=3.1e-7, total_rate1_base=2.2e-11,
+                total_accel1_base=1e-15, net_d2_base=0.0, total_rate2_base=0.0,
+                total_accel2_base=0.0, lo1_hz=6.7e9, lo2_hz=6.7e9)
+    base.update(kw)
+    return R.make_delay_cfg(table, **base), F.make_delay_cfg(table, **base)
+
+
+@pytest.mark.parametrize("with_table", [True, False])
+def test_delay_eval_bit_exact(with_table):
+    table = (1.5e-7 + 3e-9 * np.sin(np.arange(10) / 3.0)) if with_table else None
+    rc, fc = _delay_cfg_pair(table)
+    idx = np.concatenate([np.arange(0, 2000), np.arange(999_000, 1_001_000),
+                          np.arange(7_999_000, 8_000_000)])
+    got = F.delay_eval(fc, 0, int(idx.max()) + 1)
+    ulp_bad = 0
+    for i in idx[::7]:
+        g, o = got[int(i)], R.frame_delay(int(i), rc)
+        assert g.int1 == o.int1 and g.int2 == o.int2
+        assert g.frac1 == o.frac1 and g.frac2 == o.frac2
+        for a, b in ((g.fr_lo1, o.fr_lo1), (g.fr_lo2, o.fr_lo2)):
+            ga = np.array(a[:], dtype=np.float32).view(np.int32)
+            ob = np.array(b[:], dtype=np.float32).view(np.int32)
+            d = np.abs(ga.astype(np.int64) - ob.astype(np.int64))
+            assert np.all(d <= 1)       # f32 cos/sin: libm vs device, at most 1 ulp
+            ulp_bad += int(np.sum(d))
+    assert ulp_bad <= len(idx[::7]) // 50
+
+
+# -------------------------------------------------------------------- FFT
+@pytest.mark.parametrize("n", [64, 128, 256, 512, 1024, 2048, 4096, 8192, 16384])
+def test_rfft_vs_numpy_and_oracle(n):
+    p = _plan(n)
+    x = np.random.default_rng(n).standard_normal((5, n)).astype(np.float32)
+    got = p.rfft_frames(x)
+    ref = np.fft.rfft(x.astype(np.float64), axis=1)
+    scale = np.sqrt(n)
+    err = np.sqrt(np.mean(np.abs(got - ref) ** 2, axis=1)) / scale
+    assert np.all(err < 3e-7 * np.sqrt(np.log2(n)))
+    assert np.all(got[:, 0].imag == 0) and np.all(got[:, -1].imag == 0)
+    orc = np.stack([R.rfft_f32(r) for r in x])
+    assert np.max(np.abs(got - orc)) / scale < 2e-6
+    y = p.irfft_frames(got)
+    assert np.max(np.abs(y - x)) < 1e-5 * np.sqrt(np.log2(n))
+    p.close()
+
+
+def test_irfft_matches_oracle():
+    n = 1024
+    p = _plan(n)
+    rng = np.random.default_rng(4)
+    spec = (rng.standard_normal((3, n // 2 + 1)) + 1j * rng.standard_normal((3, n // 2 + 1)))
+    spec = spec.astype(np.complex64)
+    got = p.irfft_frames(spec)
+    for i in range(3):
+        assert np.allclose(got[i], R.irfft_f32(spec[i], n), atol=2e-6)
+
+
+# --------------------------------------------------------------- quantise
+@pytest.mark.parametrize("bits", [1, 2, 4, 8])
+def test_quantise_bit_exact(bits):
+    rng = np.random.default_rng(bits + 40)
+    n = 1024
+    lv = uniform_levels(bits) * 0.9
+    sh = rng.permutation(32).astype(np.int32)
+    p = F.Plan(n, bits=(bits, bits), levels=[lv, lv], shuffles=(sh, None), df_hz=1.0,
+               products=F.PRODUCT_ACF, bit_out=bits)
+    x = (rng.standard_normal((6, n)) * lv.max()).astype(np.float32)
+    x[0, :5] = [np.nan, np.inf, -np.inf, 0.0, lv[0]]
+    got = p.quantise_frames(x)
+    for f in range(6):
+        assert np.array_equal(got[f], R.quantise_frame(x[f], bits, lv, sh))
+    p.close()
+
+
+# ------------------------------------------------------------- sector MAC
+def _compare(got, ref, tol, keys=("acc11", "acc22", "acc12")):
+    for k in keys:
+        assert zero_pattern_equal(got[k], ref[k]), k
+        e = rel_err(got[k], ref[k])
+        assert e < tol, (k, e)
+
+
+def test_c1_full_plain_and_chosen():
+    """SURVEY C1 shape in full: 64 Msps 2-bit, N=1024, 62,499 frames, USB,
+    37.3-sample delay (seek 32 -> int1 = 5, frac1 = 0.3), fringe at 6.7 GHz."""
+    fs, n = 64e6, 1024
+    c = make_case(n=n, fs=fs, nf=62499, rho=0.02, delay_samples=37.3 - 32, seed=0,
+                  delay_user_samples=37.3, d_seek=32 / fs, extra_delay_rate_sps=0.5 / 6.7e9,
+                  lo1_hz=6.7e9, lo2_hz=6.7e9)
+    g = gpu_run(c, F.PRECISION_PLAIN_FP32)
+    o32 = oracle_run(c, R.MODE_F32)
+    assert g["n_failed"] == 0 and o32["n_failed"] == 0
+    _compare(g, o32, PLAIN_TOL)
+    gc = gpu_run(c, F.PRECISION_CHOSEN)
+    o64 = oracle_run(c, R.MODE_F64)
+    _compare(gc, o64, CHOSEN_TOL)
+
+
+@pytest.mark.parametrize("n", [64, 256, 512, 1024, 2048, 4096, 8192, 16384])
+def test_sector_sizes(n):
+    nf = max(16, 2 ** 18 // n)
+    fs = 256e6
+    c = make_case(n=n, fs=fs, nf=nf, rho=0.1, delay_samples=2.6, seed=n,
+                  delay_user_samples=2.6, extra_delay_rate_sps=3e-9, lo1_hz=1.0e9, lo2_hz=1.0e9)
+    _compare(gpu_run(c), oracle_run(c), PLAIN_TOL)
+    _compare(gpu_run(c, F.PRECISION_CHOSEN), oracle_run(c, R.MODE_F64), CHOSEN_TOL)
+
+
+def test_c2_shape_lsb_shuffle_geometry():
+    """C2-like: 1024 Msps 2-bit N=1024, B LSB with a seeded shuffle, geometric
+    table + clock 1.2 us + rate 1e-12 + residuals; a 20000-frame slice of
+    sector 3 (first_frame != 0)."""
+    fs, n = 1024e6, 1024
+    rng = np.random.default_rng(2)
+    sh = rng.permutation(32).astype(np.int32)
+    table = 2.9e-7 + 1.5e-9 * np.arange(10)
+    c = make_case(n=n, fs=fs, nf=20000, shuffles=(None, sh), lsb=(False, True), rho=0.005,
+                  delay_samples=3.0, seed=3, table=table, first_frame=3_000_000,
+                  time_offset_s=0.0, clock2_delay_s=1.2e-6, clock2_rate_sps=1e-12,
+                  delay_user_samples=0.2, extra_delay_rate_sps=0.05 / 6.7e9,
+                  d_seek=(1526 // 16 * 16) / fs, lo1_hz=6.7e9, lo2_hz=6.7e9)
+    _compare(gpu_run(c), oracle_run(c), PLAIN_TOL)
+    _compare(gpu_run(c, F.PRECISION_CHOSEN), oracle_run(c, R.MODE_F64), CHOSEN_TOL)
+
+
+@pytest.mark.parametrize("grid", [0, 1])
+@pytest.mark.parametrize("overlap_only", [False, True])
+def test_band_shift_grids(grid, overlap_only):
+    """Mixed bits (4/1) with a band shift as in C3, scaled to N=4096."""
+    n, fs = 4096, 512e6
+    sb = -128
+    band = (sb, 128, 2048, 0) if grid == 0 else (-sb, 0, 2048 - 128, 128)
+    prods = F.PRODUCT_ACF | F.PRODUCT_XCF
+    if overlap_only:
+        prods = F.PRODUCT_ACF | F.PRODUCT_ACF_OVERLAP_ONLY
+    c = make_case(n=n, fs=fs, nf=400, bits=(4, 1), band=band, grid=grid, products=prods,
+                  rho=0.2, delay_samples=1.4, seed=11, delay_user_samples=1.4,
+                  lo1_hz=8.4e9, lo2_hz=8.4e9)
+    _compare(gpu_run(c), oracle_run(c), PLAIN_TOL)
+    _compare(gpu_run(c, F.PRECISION_CHOSEN), oracle_run(c, R.MODE_F64), CHOSEN_TOL)
+
+
+def test_analytic_delay_branch_and_large_shifts():
+    # no geometric table: quadratic delay model; shifts up to beyond +-N
+    n, fs = 512, 128e6
+    c = make_case(n=n, fs=fs, nf=600, no_table=True, net_d1_base=-6.0e-6,
+                  total_rate1_base=1.2e-2, total_accel1_base=1e-3, lo1_hz=2e9, seed=5)
+    d = R.make_delay_cfg(None, **c["delay"])
+    ints = [R.frame_delay(i, d).int1 for i in range(600)]
+    assert min(ints) < -n and max(ints) > 0   # frames fully zeroed and shifted both ways
+    _compare(gpu_run(c), oracle_run(c), PLAIN_TOL)
+
+
+def test_device_entry_matches_host_entry_and_is_deterministic():
+    c = make_case(n=1024, fs=64e6, nf=3000, seed=8, delay_user_samples=4.2, lo1_hz=1e9)
+    p = gpu_plan(c)
+    cfg = F.make_delay_cfg(c["table"], **c["delay"])
+    a = p.sector_run(c["raw1"], c["raw2"], 0, c["nf"], cfg)
+    b = p.sector_run(c["raw1"], c["raw2"], 0, c["nf"], cfg)
+    for k in ("acc11", "acc22", "acc12"):
+        assert np.array_equal(a[k], b[k])
+    d1 = F.DeviceBuffer(c["raw1"].nbytes)
+    d2 = F.DeviceBuffer(c["raw2"].nbytes)
+    dacc = F.DeviceBuffer(F.N_PRODUCTS * p.half * 8)
+    d1.upload(c["raw1"])
+    d2.upload(c["raw2"])
+    p.sector_run_device(d1.ptr, d2.ptr, 0, c["nf"], cfg, dacc.ptr)
+    host = dacc.download(np.zeros(F.N_PRODUCTS * p.half))
+    h = p.half
+    assert np.array_equal(host[h:2 * h], a["acc11"])
+    assert np.array_equal(host[2 * h:3 * h], a["acc22"])
+    assert np.array_equal(host[3 * h:4 * h] + 1j * host[4 * h:], a["acc12"])
+    p.close()
+
+
+def test_timing_counts_launches():
+    c = make_case(n=1024, fs=64e6, nf=500, seed=1)
+    p = gpu_plan(c)
+    p.set_timing(True)
+    cfg = F.make_delay_cfg(c["table"], **c["delay"])
+    for _ in range(3):
+        p.sector_run(c["raw1"], c["raw2"], 0, c["nf"], cfg)
+    ms, n = p.get_timing()
+    assert n == 3 and ms > 0
+    p.close()
