@@ -1,0 +1,222 @@
+#!/usr/bin/env python3
+"""Throughput of the FX hot path on one MI355X (Msamples/s/station).
+
+Workload: two stations, 1024 Msps, 2-bit, FFT 1024, 8 s each (2.048 GB packed
+per station, 8,000,000 frames, 8 one-second sectors), station B LSB with a
+seeded 32-entry bit shuffle, geometric delay from two ECEF positions ~108 m
+apart toward a seeded RA/Dec + clock 1.2 us / 1e-12 s/s + residual 0.2
+samples / 0.05 Hz, correlated Gaussian signal (rho = 0.005) generated on the
+device.  One step = the whole 8-sector job (ACF + XCF) with the input already
+resident in HBM.  Both arithmetic forms are timed; `value` is the chosen form.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (os.path.join(ROOT, "fx-corr_amd"), os.path.join(ROOT, "oracle")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+import numpy as np  # noqa: E402
+
+import fxcorr as F  # noqa: E402
+
+FS = 1024e6
+N = 1024
+BITS = 2
+SECONDS = 8.0
+OBS_HZ = 6.7e9
+LEVELS = np.array([-3.0, -1.0, 1.0, 3.0])
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+C_LIGHT = 299792458.0
+ECEF1 = np.array([-3.50e6, 3.66e6, 3.50e6])
+ECEF2 = ECEF1 + np.array([-22.989, -80.501, 67.923])
+
+
+def geometric_table(ra, dec, mjd0, nsec):
+    """1-s baseline delay table -(B . s)/c, s from Earth rotation angle (input only)."""
+    b = ECEF2 - ECEF1
+    out = []
+    for sec in range(nsec + 1):
+        jd = mjd0 + sec / 86400.0 + 2400000.5
+        era = 2 * math.pi * ((0.7790572732640 + 1.00273781191135448 * (jd - 2451545.0)) % 1.0)
+        ha = era - ra
+        s = np.array([math.cos(dec) * math.cos(ha), -math.cos(dec) * math.sin(ha), math.sin(dec)])
+        out.append(-(b @ s) / C_LIGHT)
+    return np.array(out)
+
+
+def c2_setup(seed=2025):
+    rng = np.random.default_rng(seed)
+    ra, dec = rng.uniform(0, 2 * math.pi), math.radians(rng.uniform(-30, 80))
+    table = geometric_table(ra, dec, 60000.25, int(SECONDS))
+    clock2, clock2_rate, user, rate_user = 1.2e-6, 1e-12, 0.2, 0.05
+    net_d0 = table[0] + user / FS + clock2                       # main.rs:1183-1184
+    init = int(round(net_d0 * FS))                               # main.rs:1279
+    start2 = (max(init, 0) // 16) * 16                           # word-aligned seek, 2-bit
+    start1 = (max(-init, 0) // 16) * 16
+    d_seek = (start2 - start1) / FS                              # main.rs:1615-1616
+    cfg = dict(frame_dt=N / FS, fs=FS, time_offset_s=0.0, coarse_delay_s=0.0,
+               delay_user_samples=user, extra_delay_rate_sps=rate_user / OBS_HZ,
+               clock1_delay_s=0.0, clock1_rate_sps=0.0, clock2_delay_s=clock2,
+               clock2_rate_sps=clock2_rate, d_seek=d_seek, net_d1_base=0.0,
+               total_rate1_base=0.0, total_accel1_base=0.0, net_d2_base=0.0,
+               total_rate2_base=0.0, total_accel2_base=0.0, lo1_hz=OBS_HZ, lo2_hz=OBS_HZ)
+    residual = int(round((net_d0 - d_seek) * FS))
+    shuffle_b = rng.permutation(32).astype(np.int32)
+    return table, cfg, shuffle_b, residual
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-frames", type=int, default=1_000_000)
+    ap.add_argument("--forms", default="plain,chosen")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (this build has no multi-GPU path); "
+              f"--gpus {args.gpus} is not supported", file=sys.stderr)
+        sys.exit(2)
+
+    total_frames = int(math.floor(SECONDS * FS / N))             # 8,000,000
+    sectors = F.sector_partition(total_frames, N / FS)           # 8 x 1,000,000
+    bpf = N * BITS // 8
+    samples = total_frames * N
+    table, dcfg, shuffle_b, residual = c2_setup()
+
+    stream = F.Stream()
+    raw1 = F.DeviceBuffer(total_frames * bpf)
+    raw2 = F.DeviceBuffer(total_frames * bpf)
+    F.synth_station_device(raw1.ptr, samples, BITS, None, False, 7, 0, 0.005, residual,
+                           stream.ptr)
+    F.synth_station_device(raw2.ptr, samples, BITS, shuffle_b, True, 7, 1, 0.005, residual,
+                           stream.ptr)
+    stream.sync()
+    half = N // 2 + 1
+    acc = F.DeviceBuffer(len(sectors) * F.N_PRODUCTS * half * 8)
+    cfg = F.make_delay_cfg(table, **dcfg)
+    print(f"[bench] workload: {len(sectors)} sectors, {total_frames} frames, "
+          f"{samples / 1e9:.3f} Gsamples/station, residual delay {residual} samples",
+          file=sys.stderr, flush=True)
+
+    results = {}
+    for form in [f.strip() for f in args.forms.split(",") if f.strip()]:
+        prec = F.PRECISION_CHOSEN if form == "chosen" else F.PRECISION_PLAIN_FP32
+        plan = F.Plan(N, bits=(BITS, BITS), levels=[LEVELS, LEVELS],
+                      shuffles=(None, shuffle_b), lsb=(False, True), band=(0, 0, N // 2, 0),
+                      df_hz=FS / N, products=F.PRODUCT_ACF | F.PRODUCT_XCF, precision=prec)
+
+        def step():
+            first = 0
+            for si, nf in enumerate(sectors):
+                plan.sector_run_device(raw1.ptr + first * bpf, raw2.ptr + first * bpf, first,
+                                       nf, cfg, acc.ptr + si * F.N_PRODUCTS * half * 8,
+                                       stream=stream.ptr)
+                first += nf
+
+        for _ in range(args.warmup):
+            step()
+        stream.sync()
+        plan.set_timing(True)
+        e0, e1 = F.Event(), F.Event()
+        e0.record(stream)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        e1.record(stream)
+        stream.sync()
+        wall = time.perf_counter() - t0
+        ms_total = e0.elapsed_ms(e1)
+        kern_ms, launches = plan.get_timing()
+        plan.set_timing(False)
+        out = acc.download(np.zeros(len(sectors) * F.N_PRODUCTS * half))
+        assert np.all(np.isfinite(out)), "non-finite accumulators"
+        ms_step = ms_total / args.steps
+        value = samples / (ms_step * 1e-3) / 1e6
+        avg_launch_ms = kern_ms / max(launches, 1)
+        nf_avg = total_frames / len(sectors)
+        flops = nf_avg * 2 * 2.5 * N * math.log2(N)
+        bytes_in = nf_avg * 2 * bpf
+        results[form] = dict(value=value, ms_per_step=ms_step, wall_ms_per_step=wall * 1e3 / args.steps,
+                             avg_launch_ms=avg_launch_ms, launches=launches,
+                             tflops=flops / (avg_launch_ms * 1e-3) / 1e12,
+                             gbs=bytes_in / (avg_launch_ms * 1e-3) / 1e9,
+                             acc11_sum=float(out[half:2 * half].sum()))
+        print(f"[bench] {form}: {value:,.0f} Msamples/s/station, {ms_step:.3f} ms/step, "
+              f"fused kernel {avg_launch_ms:.3f} ms/launch", file=sys.stderr, flush=True)
+        plan.close()
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        import fxref as R
+
+        nfc = min(args.cpu_frames, sectors[0])
+        h1 = raw1.download(np.zeros(nfc * bpf, dtype=np.uint8))
+        h2 = raw2.download(np.zeros(nfc * bpf, dtype=np.uint8))
+        threads = max(1, min(os.cpu_count() or 1, 16))
+        desc = dict(fft_len=N, bits1=BITS, bits2=BITS, levels1=LEVELS, levels2=LEVELS,
+                    shuffle1=np.arange(32, dtype=np.int32), shuffle2=shuffle_b, lsb1=0, lsb2=1,
+                    a1s=0, a1e=N // 2, a2s=0, df_hz=FS / N, need_acf=1, need_xcf=1,
+                    cpu_threads=max(1, threads - 2))
+        rcfg = R.make_delay_cfg(table, **dcfg)
+        t0 = time.perf_counter()
+        R.sector_run(desc, h1, h2, 0, nfc, rcfg, mode=R.MODE_F32, threads=threads)
+        el = time.perf_counter() - t0
+        cpu = {"value": nfc * N / el / 1e6, "unit": "Msamples/s/station", "cores": threads,
+               "kind": "port",
+               "sample": f"first {nfc} frames (sector 1, {nfc * N / FS:.3f} s) of the same "
+                         f"workload, oracle FP32 sector runner, {threads} threads, "
+                         f"{el:.1f} s wall"}
+        print(f"[bench] cpu baseline: {cpu['value']:.1f} Msamples/s/station "
+              f"({threads} threads, {el:.1f} s)", file=sys.stderr, flush=True)
+
+    main_form = "chosen" if "chosen" in results else next(iter(results))
+    r = results[main_form]
+    line = {
+        "metric": "Msamples/s/station @2-bit FFT1024",
+        "value": round(r["value"], 1),
+        "unit": "Msamples/s/station",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(r["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (device-generated correlated 2-bit Gaussian pair, seeded)",
+        "config": {"workload": "two stations, 1024 Msps 2-bit, FFT 1024, 8 s per station "
+                               "(8 sectors x 1,000,000 frames), B LSB + seeded shuffle, "
+                               "ACF+XCF, input resident in HBM",
+                   "form": main_form, "fft_len": N, "frames": total_frames,
+                   "sectors": len(sectors)},
+        "forms": {k: {"value": round(v["value"], 1), "ms_per_step": round(v["ms_per_step"], 4),
+                      "fused_kernel_ms": round(v["avg_launch_ms"], 4),
+                      "fft_tflops": round(v["tflops"], 2), "input_gbs": round(v["gbs"], 1)}
+                  for k, v in results.items()},
+        "roofline": {"bound": "mfma", "achieved": round(r["tflops"], 3),
+                     "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(r["tflops"] / FP32_PEAK_TFLOPS, 4), "traffic": None,
+                     "kernel": "fused decode+shift+FFT+MAC (k_sector<8,64>)",
+                     "hbm_frac": round(r["gbs"] / HBM_PEAK_GBS, 4)},
+        "cpu_baseline": cpu,
+    }
+    if "plain" in results and "chosen" in results:
+        line["chosen_over_plain"] = round(results["chosen"]["value"] / results["plain"]["value"], 3)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -885,3 +885,23 @@ extern "C" fxc_status fxc_quantise_frames(fxc_plan *p, const float *in, int64_t 
     HIP_TRY(hipStreamSynchronize(s));
     return FXC_OK;
 }
+
+// ============================================================ synthetic input
+extern "C" fxc_status fxc_synth_station_device(uint8_t *d_out, int64_t nsamples, int32_t bits,
+                                               const int32_t shuffle_internal[32], int32_t lsb,
+                                               uint64_t seed, int32_t station, double rho,
+                                               int64_t delay_samples, void *stream) {
+    if (!d_out || !shuffle_internal || nsamples < 0 || (station != 0 && station != 1))
+        return fail(FXC_ERR_INVALID_ARG, "bad synth arguments");
+    if (bits < 1 || bits > 16 || 32 % bits != 0 || (nsamples * bits) % 32 != 0)
+        return fail(FXC_ERR_UNSUPPORTED, "synth needs bits dividing 32 and whole words");
+    if (!(rho >= 0.0 && rho <= 1.0)) return fail(FXC_ERR_INVALID_ARG, "rho outside [0, 1]");
+    int seen[32] = {0};
+    for (int i = 0; i < 32; ++i)
+        if (shuffle_internal[i] < 0 || shuffle_internal[i] > 31 || seen[shuffle_internal[i]]++)
+            return fail(FXC_ERR_INVALID_ARG, "shuffle is not a permutation of 0..31");
+    if (nsamples == 0) return FXC_OK;
+    HIP_TRY(launch_synth(d_out, nsamples, bits, shuffle_internal, lsb, seed, station, rho,
+                         delay_samples, (hipStream_t)stream));
+    return FXC_OK;
+}

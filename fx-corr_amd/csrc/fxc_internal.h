@@ -98,4 +98,8 @@ hipError_t launch_quantise_frames(const SectorArgs &a, const float *in,
                                   int64_t nframes, uint8_t *out,
                                   hipStream_t stream);
 
+hipError_t launch_synth(uint8_t *d_out, int64_t nsamples, int bits, const int32_t *shuffle,
+                        int lsb, uint64_t seed, int station, double rho, int64_t delay,
+                        hipStream_t stream);
+
 }  // namespace fxc

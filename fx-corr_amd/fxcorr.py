@@ -30,7 +30,7 @@ EXPORTED = [
     "fxc_split_delay", "fxc_plan_create", "fxc_plan_destroy", "fxc_plan_get_info",
     "fxc_sector_run", "fxc_sector_run_device", "fxc_plan_set_timing", "fxc_plan_get_timing",
     "fxc_decode_frames", "fxc_rfft_frames", "fxc_irfft_frames", "fxc_delay_eval",
-    "fxc_quantise_frames",
+    "fxc_quantise_frames", "fxc_synth_station_device",
 ]
 
 
@@ -128,6 +128,8 @@ def lib():
             "fxc_irfft_frames": ([vp, vp, i64, vp], i32),
             "fxc_delay_eval": ([_P(DelayCfg), i64, i64, _P(FrameDelay)], i32),
             "fxc_quantise_frames": ([vp, vp, i64, vp], i32),
+            "fxc_synth_station_device": ([vp, i64, i32, _P(i32), i32, C.c_uint64, i32, d, i64,
+                                          vp], i32),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -393,3 +395,10 @@ class Event:
         ms = C.c_float()
         check(lib().fxc_event_elapsed_ms(C.c_void_p(self.ptr), C.c_void_p(end.ptr), C.byref(ms)))
         return ms.value
+
+
+def synth_station_device(d_out: int, nsamples: int, bits: int, shuffle=None, lsb=False, seed=0,
+                         station=0, rho=0.005, delay_samples=0, stream=None):
+    sh = (C.c_int32 * 32)(*[int(x) for x in (IDENTITY_SHUFFLE if shuffle is None else shuffle)])
+    check(lib().fxc_synth_station_device(C.c_void_p(d_out), nsamples, bits, sh, int(lsb), seed,
+                                         station, rho, delay_samples, C.c_void_p(stream or 0)))

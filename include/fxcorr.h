@@ -239,6 +239,21 @@ fxc_status fxc_delay_eval(const fxc_delay_cfg *cfg, int64_t first, int64_t n,
 fxc_status fxc_quantise_frames(fxc_plan *plan, const float *in, int64_t nframes,
                                uint8_t *out);
 
+/* ----------------------------------------------------------------------- */
+/* Synthetic input (no reference counterpart: seeded station data generated */
+/* on the device for benchmarks and large tests).                           */
+/* ----------------------------------------------------------------------- */
+/* Station `station` (0/1) of a correlated pair: unit Gaussians mixed with
+ * coefficient rho, station 1 delayed by delay_samples, odd samples negated
+ * when lsb, quantised (2-bit at +-0.98 sigma), packed LSB-first into
+ * little-endian words and stored through the inverse of shuffle_internal so
+ * that decoding with that map recovers the codes.  bits must divide 32 and
+ * nsamples*bits must be a multiple of 32.  Enqueued on `stream`. */
+fxc_status fxc_synth_station_device(uint8_t *d_out, int64_t nsamples, int32_t bits,
+                                    const int32_t shuffle_internal[32], int32_t lsb,
+                                    uint64_t seed, int32_t station, double rho,
+                                    int64_t delay_samples, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -27,6 +27,21 @@ def shuffle_out_words(words: np.ndarray, sh: np.ndarray) -> np.ndarray:
 
 def pack_codes(codes: np.ndarray, bits: int, n_per_frame: int, sh=IDENT) -> np.ndarray:
     """Pack codes (nframes*n) frame by frame into nframes*bpf bytes."""
+    if bits in (1, 2, 4, 8, 16) and (n_per_frame * bits) % 32 == 0:
+        c = np.asarray(codes).reshape(-1, n_per_frame)
+        nfr = c.shape[0]
+        if bits == 16:
+            by = c.astype("<u2").view(np.uint8).reshape(nfr, -1)
+        else:
+            per = 8 // bits
+            cc = c.astype(np.uint8).reshape(nfr, -1, per)
+            by = np.zeros(cc.shape[:2], dtype=np.uint8)
+            for j in range(per):
+                by |= (cc[:, :, j] << np.uint8(bits * j)).astype(np.uint8)
+        if not np.array_equal(np.asarray(sh), IDENT):
+            words = np.ascontiguousarray(by).view("<u4").reshape(-1)
+            by = shuffle_out_words(words, np.asarray(sh)).astype("<u4").view(np.uint8)
+        return np.ascontiguousarray(by).reshape(-1)
     codes = np.asarray(codes, dtype=np.uint64).reshape(-1, n_per_frame)
     nfr = codes.shape[0]
     bpf = (n_per_frame * bits + 7) // 8
