@@ -71,5 +71,30 @@ def rel_err(got, ref):
     return float(np.max(np.abs(got[m] - ref[m]) / np.abs(ref[m])))
 
 
+def xcf_err(got, ref):
+    """Relative error of a cross spectrum.  Per-bin |got-ref|/|ref| on bins at
+    least 0.1x the spectrum's RMS magnitude, and |got-ref|/RMS on every bin:
+    in a bin where the frame sum nearly cancels, per-frame FP32 rounding noise
+    (uncorrelated across frames) divided by the near-zero sum says nothing
+    about the arithmetic, so those bins are held to the RMS scale instead."""
+    got = np.asarray(got)
+    ref = np.asarray(ref)
+    m = np.abs(ref) > 0
+    if not np.any(m):
+        return 0.0
+    rms = float(np.sqrt(np.mean(np.abs(ref[m]) ** 2)))
+    strong = np.abs(ref) >= 0.1 * rms
+    e_strong = float(np.max(np.abs(got[strong] - ref[strong]) / np.abs(ref[strong])))
+    e_all = float(np.max(np.abs(got - ref)) / rms)
+    return max(e_strong, e_all)
+
+
+def spectra_err(got, ref):
+    """{acc11, acc22: per-bin relative; acc12: xcf_err} for result dicts."""
+    return {"acc11": rel_err(got["acc11"], ref["acc11"]),
+            "acc22": rel_err(got["acc22"], ref["acc22"]),
+            "acc12": xcf_err(got["acc12"], ref["acc12"])}
+
+
 def zero_pattern_equal(got, ref):
     return np.array_equal(np.asarray(got) == 0, np.asarray(ref) == 0)
