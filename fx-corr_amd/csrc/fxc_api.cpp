@@ -534,9 +534,10 @@ extern "C" fxc_status fxc_plan_create(const fxc_plan_desc *desc, fxc_plan **out)
             hipMemcpy(p->out_lev.p, lv.data(), lv.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
             return cleanup(fail(FXC_ERR_HIP, "level upload failed"));
     }
-    // persistent-style grid: ~2048 one-wave workgroups' worth of threads
+    // grid: about 16 waves per CU worth of workgroups (p->T threads each); each
+    // workgroup then runs a contiguous range of frames
     const int cus = num_cus();
-    p->max_wg = std::max(64, (cus * 8 * 64) / p->T);
+    p->max_wg = std::max(64, (cus * 1024) / p->T);
     if (p->part.ensure((size_t)p->max_wg * kNumProducts * half * sizeof(double)) != hipSuccess ||
         p->red_scratch.ensure((size_t)64 * kNumProducts * half * sizeof(double)) != hipSuccess)
         return cleanup(fail(FXC_ERR_NOMEM, "scratch allocation failed"));
@@ -692,7 +693,7 @@ extern "C" fxc_status fxc_sector_run_device(fxc_plan *p, const uint8_t *d_raw1,
     if (nf < 0) return fail(FXC_ERR_INVALID_ARG, "nf < 0");
     if (nf > 0 && (!d_raw1 || !d_raw2)) return fail(FXC_ERR_INVALID_ARG, "null raw buffer");
     (void)d_raw_out;
-    hipStream_t s = stream ? (hipStream_t)stream : p->stream;
+    hipStream_t s = (hipStream_t)stream;  // NULL: the HIP null stream
     return enqueue_sector(p, d_raw1, d_raw2, first_frame, nf, *delay, d_acc, s);
 }
 

@@ -195,11 +195,11 @@ __device__ __forceinline__ uint32_t shuffle_word(uint32_t w, const StationDev &s
     return w;
 }
 
-// Stage the frame's packed words (little-endian, shuffle applied) in LDS.
+// Stage the frame's packed words (little-endian, shuffle applied) in LDS;
+// T threads, t = this thread's index among them.
 template <int T>
 __device__ __forceinline__ void stage_words(const uint8_t *frame, const StationDev &st,
-                                            uint32_t *words) {
-    const int t = threadIdx.x;
+                                            uint32_t *words, int t) {
     if (st.aligned) {
         const uint32_t *src = reinterpret_cast<const uint32_t *>(frame);
         for (int w = t; w < st.nwords; w += T) words[w] = shuffle_word(__builtin_nontemporal_load(src + w), st);
@@ -227,21 +227,6 @@ __device__ __forceinline__ float level_of(uint32_t code, const StationDev &st,
     if (st.bits == 1) return code ? st.lev4[1] : st.lev4[0];
     if (st.bits <= 8) return lev_lds[code];
     return st.levels[code];
-}
-
-// Decoded, integer-shifted sample n of the frame (utils.rs:215-376 then
-// utils.rs:126-145): sample n holds decoded sample n - shift, zero outside.
-// The LSB->USB flip negates odd decode-order samples.
-__device__ __forceinline__ float shifted_sample(int n, int shift, int N, const uint32_t *words,
-                                                const StationDev &st, const float *lev_lds) {
-    int src = n - shift;
-    if ((unsigned)src >= (unsigned)N) return 0.0f;
-    uint32_t bitpos = (uint32_t)src * (uint32_t)st.bits;
-    uint32_t w0 = bitpos >> 5, off = bitpos & 31u;
-    uint32_t v = words[w0] >> off;
-    if (off + (uint32_t)st.bits > 32u) v |= words[w0 + 1] << (32u - off);
-    float val = level_of(v & st.code_mask, st, lev_lds);
-    return (st.lsb && (src & 1)) ? -val : val;
 }
 
 // ------------------------------------------------------------------ FFT
@@ -276,43 +261,74 @@ __device__ __forceinline__ void dft8(float2 *a) {
     a[3] = c_add(e3, t3);
     a[7] = c_sub(e3, t3);
 }
+// Radix-16 as 4 x 4: y[n2][k1] = DFT4_{n1}(a[4 n1 + n2]) * W16^{n2 k1},
+// X[k1 + 4 k2] = DFT4_{n2}(y[n2][k1]).
+__device__ __forceinline__ void dft16(float2 *a) {
+    float2 y[4][4];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; ++n2) {
+        y[n2][0] = a[n2];
+        y[n2][1] = a[4 + n2];
+        y[n2][2] = a[8 + n2];
+        y[n2][3] = a[12 + n2];
+        dft4(y[n2][0], y[n2][1], y[n2][2], y[n2][3]);
+    }
+    const float c1 = 0.92387953251128675613f, s1 = 0.38268343236508977173f;
+    const float r2 = 0.70710678118654752440f;
+    // W16^1 = (c1, -s1), W16^2 = (r2, -r2), W16^3 = (s1, -c1), W16^4 = -i,
+    // W16^6 = (-r2, -r2), W16^9 = (-c1, s1)
+    y[1][1] = c_mul(y[1][1], make_float2(c1, -s1));
+    y[1][2] = c_mul(y[1][2], make_float2(r2, -r2));
+    y[1][3] = c_mul(y[1][3], make_float2(s1, -c1));
+    y[2][1] = c_mul(y[2][1], make_float2(r2, -r2));
+    y[2][2] = make_float2(y[2][2].y, -y[2][2].x);
+    y[2][3] = c_mul(y[2][3], make_float2(-r2, -r2));
+    y[3][1] = c_mul(y[3][1], make_float2(s1, -c1));
+    y[3][2] = c_mul(y[3][2], make_float2(-r2, -r2));
+    y[3][3] = c_mul(y[3][3], make_float2(-c1, s1));
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+        float2 b0 = y[0][k1], b1 = y[1][k1], b2 = y[2][k1], b3 = y[3][k1];
+        dft4(b0, b1, b2, b3);
+        a[k1] = b0;
+        a[k1 + 4] = b1;
+        a[k1 + 8] = b2;
+        a[k1 + 12] = b3;
+    }
+}
+
 template <int R>
 __device__ __forceinline__ void dftR(float2 *a) {
     if constexpr (R == 2) dft2(a[0], a[1]);
     else if constexpr (R == 4) dft4(a[0], a[1], a[2], a[3]);
-    else dft8(a);
+    else if constexpr (R == 8) dft8(a);
+    else dft16(a);
 }
 
-// One Stockham radix-R pass (Govindaraju et al. 2008 formulation) over two
-// in-place LDS buffers: butterfly j reads z[j + r*M/R], twiddles by
-// W_M^{r*(j%NS)*M/(NS*R)}, writes z[(j/NS)*NS*R + j%NS + r*NS].
+// One Stockham radix-R pass (Govindaraju et al. 2008 formulation) over an
+// in-place LDS buffer of M = E*T points, T threads per buffer (t = thread
+// index within its buffer): butterfly j reads z[j + r*M/R], twiddles by
+// W_M^{r*(j%NS)*M/(NS*R)}, writes z[(j/NS)*NS*R + j%NS + r*NS].  Every
+// thread of the workgroup runs the same pass sequence (the barriers are
+// workgroup-wide; each half of the workgroup owns one buffer).
 template <int E, int T, int NS>
-__device__ __forceinline__ void stockham_rest(float2 *za, float2 *zb, const float2 *tw) {
+__device__ __forceinline__ void stockham_rest(float2 *z, const float2 *tw, int t) {
     constexpr int M = E * T;
     if constexpr (NS < M) {
         constexpr int R = (M / NS < E) ? (M / NS) : E;
         constexpr int BPT = E / R;
-        const int t = threadIdx.x;
-        float2 a[BPT][R], b[BPT][R];
+        float2 a[BPT][R];
 #pragma unroll
         for (int u = 0; u < BPT; ++u) {
             const int j = t + T * u;
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
-                a[u][r] = za[pad(j + r * (M / R))];
-                b[u][r] = zb[pad(j + r * (M / R))];
-            }
+            for (int r = 0; r < R; ++r) a[u][r] = z[pad(j + r * (M / R))];
             if constexpr (NS > 1) {
                 const int jm = j % NS;
 #pragma unroll
-                for (int r = 1; r < R; ++r) {
-                    float2 w = tw[r * jm * (M / (NS * R))];
-                    a[u][r] = c_mul(a[u][r], w);
-                    b[u][r] = c_mul(b[u][r], w);
-                }
+                for (int r = 1; r < R; ++r) a[u][r] = c_mul(a[u][r], tw[r * jm * (M / (NS * R))]);
             }
             dftR<R>(a[u]);
-            dftR<R>(b[u]);
         }
         __syncthreads();
 #pragma unroll
@@ -320,28 +336,28 @@ __device__ __forceinline__ void stockham_rest(float2 *za, float2 *zb, const floa
             const int j = t + T * u;
             const int base = (j / NS) * NS * R + (j % NS);
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
-                za[pad(base + r * NS)] = a[u][r];
-                zb[pad(base + r * NS)] = b[u][r];
-            }
+            for (int r = 0; r < R; ++r) z[pad(base + r * NS)] = a[u][r];
         }
         __syncthreads();
-        stockham_rest<E, T, NS * R>(za, zb, tw);
+        stockham_rest<E, T, NS * R>(z, tw, t);
     }
 }
 
 // First pass from registers: v[r] = z[t + T*r] (NS = 1, no twiddles).
 template <int E, int T>
-__device__ __forceinline__ void stockham_first(float2 *za, float2 *zb, float2 *va, float2 *vb) {
-    const int t = threadIdx.x;
-    dftR<E>(va);
-    dftR<E>(vb);
+__device__ __forceinline__ void stockham_first(float2 *z, float2 *v, int t) {
+    dftR<E>(v);
 #pragma unroll
-    for (int r = 0; r < E; ++r) {
-        za[pad(t * E + r)] = va[r];
-        zb[pad(t * E + r)] = vb[r];
-    }
+    for (int r = 0; r < E; ++r) z[pad(t * E + r)] = v[r];
     __syncthreads();
+}
+
+// Forward complex FFT of M = E*T points held as v[r] = z[t + T*r]; result
+// in natural order in z.
+template <int E, int T>
+__device__ __forceinline__ void fft_forward(float2 *z, float2 *v, const float2 *tw, int t) {
+    stockham_first<E, T>(z, v, t);
+    stockham_rest<E, T, E>(z, tw, t);
 }
 
 // X[k] of the real N-point input from Z = FFT_{N/2}(x[2m] + i x[2m+1])
@@ -363,28 +379,57 @@ __device__ __forceinline__ float2 untangle(const float2 *zs, int k, int M, const
 __device__ __forceinline__ float norm_sqr(float2 a) { return a.x * a.x + a.y * a.y; }
 
 // ------------------------------------------------------- fused kernel
+// Decoded, integer-shifted samples 2m and 2m+1 (one complex FFT input
+// element) from the staged words: the two codes are adjacent in the bit
+// stream, so one funnel shift of two words yields both.
+__device__ __forceinline__ float2 decode_elem(const uint32_t *words, const StationDev &st,
+                                              const float *lev_lds, int m, int shift, int N) {
+    const int src0 = 2 * m - shift;
+    const bool in0 = (unsigned)src0 < (unsigned)N;
+    const bool in1 = (unsigned)(src0 + 1) < (unsigned)N;
+    const int bp = src0 * st.bits;              // may be negative (arithmetic shift floors)
+    const int w0 = bp >> 5;
+    const uint32_t off = (uint32_t)bp & 31u;
+    const uint32_t lo = (w0 >= 0 && w0 < st.nwords) ? words[w0] : 0u;
+    const uint32_t hi = (w0 + 1 >= 0 && w0 + 1 < st.nwords) ? words[w0 + 1] : 0u;
+    const uint32_t v = __builtin_amdgcn_alignbit(hi, lo, off);
+    float x0 = level_of(v & st.code_mask, st, lev_lds);
+    float x1 = level_of((v >> st.bits) & st.code_mask, st, lev_lds);
+    if (st.lsb) {                               // LSB->USB flip of odd decode-order samples
+        if (src0 & 1) x0 = -x0;
+        else x1 = -x1;
+    }
+    return make_float2(in0 ? x0 : 0.0f, in1 ? x1 : 0.0f);
+}
+
+// The workgroup holds 2T threads: threads [0,T) decode and transform
+// station 1, [T,2T) station 2 (M = E*T complex points each); then every
+// thread owns output-grid bins g = tid + 2T*i for the products.
 template <int E, int T, int PREC>
-__global__ __launch_bounds__(T) void k_sector(SectorArgs a) {
+__global__ __launch_bounds__(2 * T) void k_sector(SectorArgs a) {
     constexpr int M = E * T;
     constexpr int MP = M + M / 16 + 2;
-    constexpr int NB = E + 1;
-    constexpr int LOGT = __builtin_ctz(T);
+    constexpr int NT = 2 * T;
+    constexpr int NB = (M + NT) / NT;            // ceil((M+1)/NT)
+    constexpr int LOGNT = __builtin_ctz(NT);
     constexpr int LOGM = __builtin_ctz(M);
-    static_assert(LOGM <= kMaxPow, "phase power table too small");
+    static_assert(LOGM <= kMaxPow && LOGNT < kMaxPow, "phase power table too small");
     __shared__ float2 zbuf[2][MP];
     __shared__ FrameParam fp[kFrameBatch];
     __shared__ float lev[2][256];
 
-    const int t = threadIdx.x;
+    const int tid = threadIdx.x;
+    const int s = __builtin_amdgcn_readfirstlane(tid / T);   // station of this half (wave-uniform)
+    const int t = tid - s * T;
     const int N = 2 * M;
     const int64_t f_begin = (int64_t)blockIdx.x * a.frames_per_wg;
     int64_t f_end = f_begin + a.frames_per_wg;
     if (f_end > a.nf) f_end = a.nf;
+    const StationDev &st = a.st[s];
+    const uint8_t *raw = s ? a.raw2 : a.raw1;
 
-#pragma unroll
-    for (int s = 0; s < 2; ++s)
-        if (a.st[s].bits > 2 && a.st[s].bits <= 8)
-            for (int i = t; i < 256; i += T) lev[s][i] = (i <= (int)a.st[s].code_mask) ? a.st[s].levels[i] : 0.0f;
+    if (st.bits > 2 && st.bits <= 8)
+        for (int i = t; i < 256; i += T) lev[s][i] = (i <= (int)st.code_mask) ? st.levels[i] : 0.0f;
 
     double acc[4][NB];
 #pragma unroll
@@ -398,34 +443,26 @@ __global__ __launch_bounds__(T) void k_sector(SectorArgs a) {
     const int gs = (a.grid == 0) ? a.a1s : a.a2s;   // grid index of overlap bin 0
     const int ge = gs + a.ov;
     const int sb_other = (a.grid == 0) ? (a.a2s - a.a1s) : (a.a1s - a.a2s);
-    float2 *z0 = zbuf[0];
-    float2 *z1 = zbuf[1];
-    uint32_t *w0 = reinterpret_cast<uint32_t *>(zbuf[0]);
-    uint32_t *w1 = reinterpret_cast<uint32_t *>(zbuf[1]);
+    float2 *zs = zbuf[s];
+    uint32_t *ws = reinterpret_cast<uint32_t *>(zbuf[s]);
+    const float2 *zg = zbuf[a.grid == 0 ? 0 : 1];
+    const float2 *zo = zbuf[a.grid == 0 ? 1 : 0];
     __syncthreads();
 
     for (int64_t fb = f_begin; fb < f_end; fb += kFrameBatch) {
         const int nb = (int)((f_end - fb) < kFrameBatch ? (f_end - fb) : kFrameBatch);
-        if (t < nb) make_frame_param<PREC>(a.first_frame + fb + t, a, LOGM, fp[t]);
+        if (tid < nb) make_frame_param<PREC>(a.first_frame + fb + tid, a, LOGM, fp[tid]);
         __syncthreads();
         for (int jf = 0; jf < nb; ++jf) {
             const int64_t f = fb + jf;
-            stage_words<T>(a.raw1 + f * a.st[0].bpf, a.st[0], w0);
-            stage_words<T>(a.raw2 + f * a.st[1].bpf, a.st[1], w1);
+            stage_words<T>(raw + f * st.bpf, st, ws, t);
             __syncthreads();
-            const int s1 = fp[jf].s1, s2 = fp[jf].s2;
-            float2 va[E], vb[E];
+            const int sh = s ? fp[jf].s2 : fp[jf].s1;
+            float2 v[E];
 #pragma unroll
-            for (int r = 0; r < E; ++r) {
-                const int m = t + T * r;
-                va[r] = make_float2(shifted_sample(2 * m, s1, N, w0, a.st[0], lev[0]),
-                                    shifted_sample(2 * m + 1, s1, N, w0, a.st[0], lev[0]));
-                vb[r] = make_float2(shifted_sample(2 * m, s2, N, w1, a.st[1], lev[1]),
-                                    shifted_sample(2 * m + 1, s2, N, w1, a.st[1], lev[1]));
-            }
+            for (int r = 0; r < E; ++r) v[r] = decode_elem(ws, st, lev[s], t + T * r, sh, N);
             __syncthreads();
-            stockham_first<E, T>(z0, z1, va, vb);
-            stockham_rest<E, T, E>(z0, z1, a.tw_fft);
+            fft_forward<E, T>(zs, v, a.tw_fft, t);
 
             // ---- products on the output grid (main.rs:2491-2568)
             const float2 frmix = fp[jf].frmix;
@@ -433,22 +470,21 @@ __global__ __launch_bounds__(T) void k_sector(SectorArgs a) {
             cd P = cd{0.0, 0.0};
 #pragma unroll
             for (int i = 0; i < NB; ++i) {
-                const int g = t + T * i;
-                if (i < E || g <= M) {
-                const bool in_ov = (g >= gs) && (g < ge);
-                float2 Xg = untangle(a.grid == 0 ? z0 : z1, g, M, a.tw_post);
-                float pg = norm_sqr(Xg);
-                float2 Xo = make_float2(0.0f, 0.0f);
-                if (in_ov) Xo = untangle(a.grid == 0 ? z1 : z0, g + sb_other, M, a.tw_post);
-                if (do_acf) {
-                    // grid station: every bin (or overlap only); other station: overlap
-                    const float pgrid = (!ov_only || in_ov) ? pg : 0.0f;
-                    const float poth = in_ov ? norm_sqr(Xo) : 0.0f;
-                    acc[0][i] += (double)(a.grid == 0 ? pgrid : poth);
-                    acc[1][i] += (double)(a.grid == 0 ? poth : pgrid);
-                }
-                if (in_ov) {
-                    if (do_xcf) {
+                const int g = tid + NT * i;
+                if (g <= M) {
+                    const bool in_ov = (g >= gs) && (g < ge);
+                    const float2 Xg = untangle(zg, g, M, a.tw_post);
+                    const float pg = norm_sqr(Xg);
+                    float2 Xo = make_float2(0.0f, 0.0f);
+                    if (in_ov) Xo = untangle(zo, g + sb_other, M, a.tw_post);
+                    if (do_acf) {
+                        // grid station: every bin (or overlap only); other station: overlap
+                        const float pgrid = (!ov_only || in_ov) ? pg : 0.0f;
+                        const float poth = in_ov ? norm_sqr(Xo) : 0.0f;
+                        acc[0][i] += (double)(a.grid == 0 ? pgrid : poth);
+                        acc[1][i] += (double)(a.grid == 0 ? poth : pgrid);
+                    }
+                    if (in_ov && do_xcf) {
                         const int k = g - gs;
                         if (!have) {
                             P = fp[jf].start;
@@ -457,18 +493,17 @@ __global__ __launch_bounds__(T) void k_sector(SectorArgs a) {
                                 if ((k >> b) & 1) P = d_mul(P, fp[jf].pw[b]);
                             have = true;
                         } else {
-                            P = d_mul(P, fp[jf].pw[LOGT]);
+                            P = d_mul(P, fp[jf].pw[LOGNT]);
                         }
                         const float2 ph = make_float2((float)P.x, (float)P.y);
                         const float2 X1 = (a.grid == 0) ? Xg : Xo;
                         const float2 X2 = (a.grid == 0) ? Xo : Xg;
-                        float2 v = c_mul(X1, c_conj(X2));
-                        if (PREC == 0) v = c_mul(v, frmix);
-                        v = c_mul(v, ph);
-                        acc[2][i] += (double)v.x;
-                        acc[3][i] += (double)v.y;
+                        float2 v2 = c_mul(X1, c_conj(X2));
+                        if (PREC == 0) v2 = c_mul(v2, frmix);
+                        v2 = c_mul(v2, ph);
+                        acc[2][i] += (double)v2.x;
+                        acc[3][i] += (double)v2.y;
                     }
-                }
                 }
             }
             __syncthreads();
@@ -480,7 +515,7 @@ __global__ __launch_bounds__(T) void k_sector(SectorArgs a) {
     double *out = a.part + (size_t)blockIdx.x * kNumProducts * half;
 #pragma unroll
     for (int i = 0; i < NB; ++i) {
-        const int g = t + T * i;
+        const int g = tid + NT * i;
         if (g <= M) {
             out[P_PH * half + g] = 0.0;
             out[P_11 * half + g] = acc[0][i];
@@ -528,10 +563,14 @@ hipError_t launch_reduce(const double *part, int n_wg, int half, double *out, do
 }
 
 // ---------------------------------------------------------- dispatch
-// Elements per thread E and threads per frame T for a given N (M = N/2 = E*T).
+// Elements per thread E and threads per station T for a given N
+// (M = N/2 = E*T; the workgroup holds 2T threads).
 static void shape_for(int N, int &E, int &T) {
     const int M = N / 2;
-    if (M >= 512) {
+    if (M >= 8192) {
+        E = 16;
+        T = M / 16;
+    } else if (M >= 512) {
         E = 8;
         T = M / 8;
     } else if (M == 256) {
@@ -547,15 +586,15 @@ bool sector_supported(int N) { return N >= 64 && N <= 16384 && (N & (N - 1)) == 
 int sector_threads_per_frame(int N) {
     int E, T;
     shape_for(N, E, T);
-    return T;
+    return 2 * T;
 }
 
 template <int E, int T>
 static hipError_t launch_sector_t(const SectorArgs &a, int precision, int n_wg, hipStream_t s) {
     if (precision == 0)
-        hipLaunchKernelGGL((k_sector<E, T, 0>), dim3(n_wg), dim3(T), 0, s, a);
+        hipLaunchKernelGGL((k_sector<E, T, 0>), dim3(n_wg), dim3(2 * T), 0, s, a);
     else
-        hipLaunchKernelGGL((k_sector<E, T, 1>), dim3(n_wg), dim3(T), 0, s, a);
+        hipLaunchKernelGGL((k_sector<E, T, 1>), dim3(n_wg), dim3(2 * T), 0, s, a);
     return hipGetLastError();
 }
 
@@ -569,13 +608,14 @@ hipError_t launch_sector(const SectorArgs &a, int precision, int n_wg, hipStream
     case 2048: return launch_sector_t<8, 128>(a, precision, n_wg, s);
     case 4096: return launch_sector_t<8, 256>(a, precision, n_wg, s);
     case 8192: return launch_sector_t<8, 512>(a, precision, n_wg, s);
-    case 16384: return launch_sector_t<8, 1024>(a, precision, n_wg, s);
+    case 16384: return launch_sector_t<16, 512>(a, precision, n_wg, s);
     default: return hipErrorInvalidValue;
     }
 }
 
 // ------------------------------------------ per-function entry kernels
-// Decode (+ shift) of one frame per workgroup into out[f*N + n].
+// Decode (+ shift) of one frame per workgroup into out[f*N + n], through the
+// fused kernel's staging and element decode.
 template <int T>
 __global__ __launch_bounds__(T) void k_decode_frames(SectorArgs a, int station,
                                                      const int64_t *shifts, float *out) {
@@ -587,10 +627,14 @@ __global__ __launch_bounds__(T) void k_decode_frames(SectorArgs a, int station,
     if (st.bits > 2 && st.bits <= 8)
         for (int i = t; i < 256; i += T) lev[i] = (i <= (int)st.code_mask) ? st.levels[i] : 0.0f;
     const uint8_t *raw = (station == 0 ? a.raw1 : a.raw2) + f * st.bpf;
-    stage_words<T>(raw, st, words);
+    stage_words<T>(raw, st, words, t);
     __syncthreads();
     const int s = shifts ? clamp_shift(shifts[f], a.N) : 0;
-    for (int n = t; n < a.N; n += T) out[f * a.N + n] = shifted_sample(n, s, a.N, words, st, lev);
+    for (int m = t; m < a.M; m += T) {
+        float2 v = decode_elem(words, st, lev, m, s, a.N);
+        out[f * a.N + 2 * m] = v.x;
+        out[f * a.N + 2 * m + 1] = v.y;
+    }
 }
 
 hipError_t launch_decode_frames(const SectorArgs &a, int station, const int64_t *shifts,
@@ -602,29 +646,26 @@ hipError_t launch_decode_frames(const SectorArgs &a, int station, const int64_t 
     return hipGetLastError();
 }
 
-// Forward R2C of two frames per workgroup with the fused kernel's FFT code.
+// Forward R2C with the fused kernel's FFT code: each half of the workgroup
+// transforms one frame.
 template <int E, int T>
-__global__ __launch_bounds__(T) void k_rfft_frames(SectorArgs a, const float *in, int64_t nframes,
-                                                   float2 *out) {
+__global__ __launch_bounds__(2 * T) void k_rfft_frames(SectorArgs a, const float *in,
+                                                       int64_t nframes, float2 *out) {
     constexpr int M = E * T;
     constexpr int MP = M + M / 16 + 2;
     __shared__ float2 zbuf[2][MP];
-    const int t = threadIdx.x;
-    const int64_t fa = 2 * (int64_t)blockIdx.x, fbk = fa + 1;
-    const bool hb = fbk < nframes;
-    float2 va[E], vb[E];
+    const int s = __builtin_amdgcn_readfirstlane(threadIdx.x / T);
+    const int t = threadIdx.x - s * T;
+    const int64_t f = 2 * (int64_t)blockIdx.x + s;
+    const bool live = f < nframes;
+    float2 v[E];
 #pragma unroll
-    for (int r = 0; r < E; ++r) {
-        const int m = t + T * r;
-        va[r] = reinterpret_cast<const float2 *>(in + fa * 2 * M)[m];
-        vb[r] = hb ? reinterpret_cast<const float2 *>(in + fbk * 2 * M)[m] : make_float2(0.f, 0.f);
-    }
-    stockham_first<E, T>(zbuf[0], zbuf[1], va, vb);
-    stockham_rest<E, T, E>(zbuf[0], zbuf[1], a.tw_fft);
-    for (int k = t; k <= M; k += T) {
-        out[fa * (M + 1) + k] = untangle(zbuf[0], k, M, a.tw_post);
-        if (hb) out[fbk * (M + 1) + k] = untangle(zbuf[1], k, M, a.tw_post);
-    }
+    for (int r = 0; r < E; ++r)
+        v[r] = live ? reinterpret_cast<const float2 *>(in + f * 2 * M)[t + T * r]
+                    : make_float2(0.f, 0.f);
+    fft_forward<E, T>(zbuf[s], v, a.tw_fft, t);
+    if (live)
+        for (int k = t; k <= M; k += T) out[f * (M + 1) + k] = untangle(zbuf[s], k, M, a.tw_post);
 }
 
 // Inverse C2R (scaled 1/N): build Z' = E' + i O' from the half spectrum,
@@ -632,8 +673,7 @@ __global__ __launch_bounds__(T) void k_rfft_frames(SectorArgs a, const float *in
 // parts are ignored (the reference zeroes them first).
 template <int E, int T>
 __device__ __forceinline__ void irfft_prepare(const float2 *X, int M, const float2 *tw_post,
-                                              float2 *v) {
-    const int t = threadIdx.x;
+                                              float2 *v, int t) {
 #pragma unroll
     for (int r = 0; r < E; ++r) {
         const int k = t + T * r;
@@ -654,32 +694,27 @@ __device__ __forceinline__ void irfft_prepare(const float2 *X, int M, const floa
 }
 
 template <int E, int T>
-__global__ __launch_bounds__(T) void k_irfft_frames(SectorArgs a, const float2 *in,
-                                                    int64_t nframes, float *out) {
+__global__ __launch_bounds__(2 * T) void k_irfft_frames(SectorArgs a, const float2 *in,
+                                                        int64_t nframes, float *out) {
     constexpr int M = E * T;
     constexpr int MP = M + M / 16 + 2;
     __shared__ float2 zbuf[2][MP];
-    const int t = threadIdx.x;
-    const int64_t fa = 2 * (int64_t)blockIdx.x, fbk = fa + 1;
-    const bool hb = fbk < nframes;
-    float2 va[E], vb[E];
-    irfft_prepare<E, T>(in + fa * (M + 1), M, a.tw_post, va);
-    if (hb) irfft_prepare<E, T>(in + fbk * (M + 1), M, a.tw_post, vb);
+    const int s = __builtin_amdgcn_readfirstlane(threadIdx.x / T);
+    const int t = threadIdx.x - s * T;
+    const int64_t f = 2 * (int64_t)blockIdx.x + s;
+    const bool live = f < nframes;
+    float2 v[E];
+    if (live) irfft_prepare<E, T>(in + f * (M + 1), M, a.tw_post, v, t);
     else
-        for (int r = 0; r < E; ++r) vb[r] = make_float2(0.f, 0.f);
-    stockham_first<E, T>(zbuf[0], zbuf[1], va, vb);
-    stockham_rest<E, T, E>(zbuf[0], zbuf[1], a.tw_fft);
+        for (int r = 0; r < E; ++r) v[r] = make_float2(0.f, 0.f);
+    fft_forward<E, T>(zbuf[s], v, a.tw_fft, t);
     const float scale = 1.0f / (float)(2 * M);
-    for (int m = t; m < M; m += T) {
-        float2 za = zbuf[0][pad(m)];
-        out[fa * 2 * M + 2 * m] = za.x * scale;
-        out[fa * 2 * M + 2 * m + 1] = -za.y * scale;
-        if (hb) {
-            float2 zb = zbuf[1][pad(m)];
-            out[fbk * 2 * M + 2 * m] = zb.x * scale;
-            out[fbk * 2 * M + 2 * m + 1] = -zb.y * scale;
+    if (live)
+        for (int m = t; m < M; m += T) {
+            float2 z = zbuf[s][pad(m)];
+            out[f * 2 * M + 2 * m] = z.x * scale;
+            out[f * 2 * M + 2 * m + 1] = -z.y * scale;
         }
-    }
 }
 
 template <int E, int T>
@@ -687,10 +722,10 @@ static hipError_t launch_fft_t(const SectorArgs &a, const void *in, int64_t nfra
                                bool inverse, hipStream_t s) {
     dim3 grid((unsigned)((nframes + 1) / 2));
     if (inverse)
-        hipLaunchKernelGGL((k_irfft_frames<E, T>), grid, dim3(T), 0, s, a, (const float2 *)in,
+        hipLaunchKernelGGL((k_irfft_frames<E, T>), grid, dim3(2 * T), 0, s, a, (const float2 *)in,
                            nframes, (float *)out);
     else
-        hipLaunchKernelGGL((k_rfft_frames<E, T>), grid, dim3(T), 0, s, a, (const float *)in,
+        hipLaunchKernelGGL((k_rfft_frames<E, T>), grid, dim3(2 * T), 0, s, a, (const float *)in,
                            nframes, (float2 *)out);
     return hipGetLastError();
 }
@@ -707,7 +742,7 @@ static hipError_t launch_fft(const SectorArgs &a, const void *in, int64_t nframe
     case 2048: return launch_fft_t<8, 128>(a, in, nframes, out, inverse, s);
     case 4096: return launch_fft_t<8, 256>(a, in, nframes, out, inverse, s);
     case 8192: return launch_fft_t<8, 512>(a, in, nframes, out, inverse, s);
-    case 16384: return launch_fft_t<8, 1024>(a, in, nframes, out, inverse, s);
+    case 16384: return launch_fft_t<16, 512>(a, in, nframes, out, inverse, s);
     default: return hipErrorInvalidValue;
     }
 }

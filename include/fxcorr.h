@@ -197,7 +197,9 @@ fxc_status fxc_sector_run(fxc_plan *plan, const uint8_t *raw1, size_t len1,
 
 /* Device-resident variant: d_raw1/d_raw2/d_acc/d_raw_out are device pointers.
  * d_acc receives 5*half doubles laid out [ph | 11 | 22 | 12re | 12im].
- * Enqueued on `stream` (NULL = the plan's stream); returns without waiting. */
+ * Enqueued on `stream` (NULL = the HIP null stream); returns without waiting.
+ * The plan's partial-sum scratch is reused by every call: calls on one plan
+ * must be ordered (same stream, or synchronised by the caller). */
 fxc_status fxc_sector_run_device(fxc_plan *plan, const uint8_t *d_raw1,
                                  const uint8_t *d_raw2, int64_t first_frame,
                                  int64_t nf, const fxc_delay_cfg *delay,
