@@ -14,7 +14,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfxcorr.so")
 
-SOURCES = ["fxc_device.hip", "fxc_synth.hip", "fxc_api.cpp"]
+SOURCES = ["fxc_device.hip", "fxc_mfma.hip", "fxc_synth.hip", "fxc_api.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 

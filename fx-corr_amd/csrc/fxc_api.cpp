@@ -140,7 +140,8 @@ struct fxc_plan {
     StationDev st[2];
     int32_t decode_ok[2] = {1, 1};
     // device tables
-    DevBuf tw_fft, tw_post, lev_dev[2], out_lev;
+    DevBuf tw_fft, tw_post, lev_dev[2], out_lev, mfma_b, lev16_dev[2];
+    int fft_form = 0;  // 0: FP32 Stockham, 1: FP16-MFMA four-step (N = 1024)
     // scratch
     DevBuf part, red_scratch, table_dev, d_raw[2], d_acc, d_raw_out, d_tmp_in, d_tmp_out,
         d_shifts;
@@ -443,6 +444,92 @@ static void fill_args(fxc_plan *p, SectorArgs &a) {
     if (p->desc.products & FXC_PRODUCT_RAW_OUT) fl |= FLAG_RAW_OUT;
     if (p->desc.products & FXC_PRODUCT_ACF_OVERLAP_ONLY) fl |= FLAG_ACF_OVERLAP;
     a.flags = (int32_t)fl;
+    a.mfma_b = (const uint4 *)p->mfma_b.p;
+    a.lev16[0] = (const uint16_t *)p->lev16_dev[0].p;
+    a.lev16[1] = (const uint16_t *)p->lev16_dev[1].p;
+}
+
+// IEEE binary16 bits of a float, round to nearest even (normals, subnormals,
+// overflow to infinity).
+static uint16_t f32_to_f16_bits(float f) {
+    uint32_t x;
+    memcpy(&x, &f, 4);
+    const uint32_t sign = (x >> 16) & 0x8000u;
+    const int32_t exp = (int32_t)((x >> 23) & 0xFF) - 127 + 15;
+    uint32_t mant = x & 0x7FFFFFu;
+    if (((x >> 23) & 0xFF) == 0xFF) return (uint16_t)(sign | 0x7C00u | (mant ? 0x200u : 0u));
+    if (exp >= 31) return (uint16_t)(sign | 0x7C00u);
+    if (exp <= 0) {
+        if (exp < -10) return (uint16_t)sign;
+        mant |= 0x800000u;
+        const uint32_t shift = (uint32_t)(14 - exp);
+        uint32_t h = mant >> shift;
+        const uint32_t rem = mant & ((1u << shift) - 1u), halfway = 1u << (shift - 1);
+        if (rem > halfway || (rem == halfway && (h & 1u))) h++;
+        return (uint16_t)(sign | h);
+    }
+    uint32_t h = ((uint32_t)exp << 10) | (mant >> 13);
+    const uint32_t rem = mant & 0x1FFFu;
+    if (rem > 0x1000u || (rem == 0x1000u && (h & 1u))) h++;
+    return (uint16_t)(sign | h);
+}
+
+static float f16_bits_to_f32(uint16_t h) {
+    const uint32_t sign = (uint32_t)(h & 0x8000u) << 16;
+    const uint32_t exp = (h >> 10) & 0x1Fu, mant = h & 0x3FFu;
+    float v;
+    if (exp == 0) v = std::ldexp((float)mant, -24);
+    else if (exp == 31) v = mant ? NAN : INFINITY;
+    else v = std::ldexp((float)(mant | 0x400u), (int)exp - 25);
+    uint32_t b;
+    memcpy(&b, &v, 4);
+    b |= sign;
+    memcpy(&v, &b, 4);
+    return v;
+}
+
+// The FP16-MFMA first stage applies when N = 1024, the chosen precision is
+// requested, ACF/XCF products are computed, and every level (as the f32 the
+// decode produces, utils.rs:186) is exact in FP16.
+static bool mfma_eligible(const fxc_plan *p) {
+    if (p->N != 1024 || p->desc.precision != FXC_PRECISION_CHOSEN) return false;
+    if (p->desc.products & (FXC_PRODUCT_PHASED | FXC_PRODUCT_RAW_OUT)) return false;
+    for (int i = 0; i < 2; ++i) {
+        const int b = p->desc.st[i].bits;
+        if (b != 1 && b != 2 && b != 4 && b != 8) return false;
+        for (double lv : p->levels[i]) {
+            const float f = (float)lv;
+            if (!std::isfinite(f) || f16_bits_to_f32(f32_to_f16_bits(f)) != f) return false;
+        }
+    }
+    return true;
+}
+
+// DFT-32 operands of the complex first stage (fxc_mfma.hip): K rows
+// kk = 32e + m1 (e = 0: Re z, 1: Im z), columns c = k1 (Re) / 32 + k1 (Im).
+// Per lane l: half8 of B[32e + 8(l/16) + j][16t + l%16], j = 0..7, stored as
+// [lane][hi: e*4+t | lo: 8 + e*4+t] with B = hi + 2^-11 lo.
+static std::vector<uint16_t> mfma_b_table() {
+    std::vector<uint16_t> tab(64 * 16 * 8);
+    for (int l = 0; l < 64; ++l)
+        for (int e = 0; e < 2; ++e)
+            for (int t = 0; t < 4; ++t)
+                for (int j = 0; j < 8; ++j) {
+                    const int m1 = 8 * (l / 16) + j;
+                    const int c = 16 * t + (l % 16);
+                    const bool im = c >= 32;
+                    const int k1 = im ? c - 32 : c;
+                    const double th = 2.0 * M_PI * (double)((m1 * k1) % 32) / 32.0;
+                    double v;
+                    if (e == 0) v = im ? -std::sin(th) : std::cos(th);
+                    else v = im ? std::cos(th) : std::sin(th);
+                    const uint16_t hi = f32_to_f16_bits((float)v);
+                    const double rest = (v - (double)f16_bits_to_f32(hi)) * 2048.0;
+                    const uint16_t lo = f32_to_f16_bits((float)rest);
+                    tab[((size_t)l * 16 + (size_t)(e * 4 + t)) * 8 + j] = hi;
+                    tab[((size_t)l * 16 + (size_t)(8 + e * 4 + t)) * 8 + j] = lo;
+                }
+    return tab;
 }
 
 extern "C" fxc_status fxc_plan_create(const fxc_plan_desc *desc, fxc_plan **out) {
@@ -534,6 +621,22 @@ extern "C" fxc_status fxc_plan_create(const fxc_plan_desc *desc, fxc_plan **out)
             hipMemcpy(p->out_lev.p, lv.data(), lv.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
             return cleanup(fail(FXC_ERR_HIP, "level upload failed"));
     }
+    if (mfma_eligible(p)) {
+        std::vector<uint16_t> tab = mfma_b_table();
+        if (p->mfma_b.ensure(tab.size() * 2) != hipSuccess ||
+            hipMemcpy(p->mfma_b.p, tab.data(), tab.size() * 2, hipMemcpyHostToDevice) != hipSuccess)
+            return cleanup(fail(FXC_ERR_HIP, "MFMA table upload failed"));
+        for (int i = 0; i < 2; ++i) {
+            std::vector<uint16_t> h(256, 0);
+            for (size_t k = 0; k < p->levels[i].size() && k < 256; ++k)
+                h[k] = f32_to_f16_bits((float)p->levels[i][k]);
+            if (p->lev16_dev[i].ensure(h.size() * 2) != hipSuccess ||
+                hipMemcpy(p->lev16_dev[i].p, h.data(), h.size() * 2, hipMemcpyHostToDevice) != hipSuccess)
+                return cleanup(fail(FXC_ERR_HIP, "level upload failed"));
+        }
+        p->fft_form = 1;
+        p->T = 128;
+    }
     // grid: about 16 waves per CU worth of workgroups (p->T threads each); each
     // workgroup then runs a contiguous range of frames
     const int cus = num_cus();
@@ -558,7 +661,8 @@ extern "C" fxc_status fxc_plan_destroy(fxc_plan *p) {
     }
     DevBuf *bufs[] = {&p->tw_fft, &p->tw_post, &p->lev_dev[0], &p->lev_dev[1], &p->out_lev,
                       &p->part, &p->red_scratch, &p->table_dev, &p->d_raw[0], &p->d_raw[1],
-                      &p->d_acc, &p->d_raw_out, &p->d_tmp_in, &p->d_tmp_out, &p->d_shifts};
+                      &p->d_acc, &p->d_raw_out, &p->d_tmp_in, &p->d_tmp_out, &p->d_shifts,
+                      &p->mfma_b, &p->lev16_dev[0], &p->lev16_dev[1]};
     for (DevBuf *b : bufs) b->release();
     if (p->stream) (void)hipStreamDestroy(p->stream);
     delete p;
@@ -574,7 +678,7 @@ extern "C" fxc_status fxc_plan_get_info(const fxc_plan *p, fxc_plan_info *info) 
     info->bpf_out = (int32_t)(((int64_t)p->N * p->desc.bit_out + 7) / 8);
     info->decode_ok1 = p->decode_ok[0];
     info->decode_ok2 = p->decode_ok[1];
-    info->fft_form = 0;
+    info->fft_form = p->fft_form;
     info->threads_per_frame = p->T;
     info->device_id = p->device;
     return FXC_OK;
@@ -675,7 +779,8 @@ static fxc_status enqueue_sector(fxc_plan *p, const uint8_t *d_raw1, const uint8
         }
         HIP_TRY(hipEventRecord(ev.first, s));
     }
-    HIP_TRY(launch_sector(a, p->desc.precision, n_wg, s));
+    if (p->fft_form == 1) HIP_TRY(launch_sector_mfma(a, p->desc.precision, n_wg, s));
+    else HIP_TRY(launch_sector(a, p->desc.precision, n_wg, s));
     if (p->timing) {
         HIP_TRY(hipEventRecord(ev.second, s));
         p->pending.push_back(ev);
@@ -811,6 +916,8 @@ static fxc_status run_fft(fxc_plan *p, const float *in, int64_t nframes, float *
     fill_args(p, a);
     if (inverse)
         HIP_TRY(launch_irfft_frames(a, (const float2 *)p->d_tmp_in.p, nframes, (float *)p->d_tmp_out.p, s));
+    else if (p->fft_form == 1)
+        HIP_TRY(launch_rfft_mfma(a, (const float *)p->d_tmp_in.p, nframes, (float2 *)p->d_tmp_out.p, s));
     else
         HIP_TRY(launch_rfft_frames(a, (const float *)p->d_tmp_in.p, nframes, (float2 *)p->d_tmp_out.p, s));
     HIP_TRY(hipMemcpyAsync(out, p->d_tmp_out.p, out_bytes, hipMemcpyDeviceToHost, s));

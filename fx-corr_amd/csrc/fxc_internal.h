@@ -75,6 +75,8 @@ struct SectorArgs {
     int32_t bpf_out;
     uint8_t *raw_out;        // phased .raw frames (nf * bpf_out) or null
     double *part;            // [n_wg][kNumProducts][half]
+    const uint4 *mfma_b;     // FP16 DFT-32 operands, [lane][hi e0t0..e1t3 | lo ...] (N = 1024)
+    const uint16_t *lev16[2];  // FP16 bit patterns of the levels (bits <= 8)
 };
 
 // Launchers implemented in fxc_device.hip.  All enqueue on `stream`.
@@ -97,6 +99,10 @@ hipError_t launch_delay_eval(const DelayDev &d, int64_t first, int64_t n,
 hipError_t launch_quantise_frames(const SectorArgs &a, const float *in,
                                   int64_t nframes, uint8_t *out,
                                   hipStream_t stream);
+
+hipError_t launch_sector_mfma(const SectorArgs &a, int precision, int n_wg, hipStream_t stream);
+hipError_t launch_rfft_mfma(const SectorArgs &a, const float *in, int64_t nframes, float2 *out,
+                            hipStream_t stream);
 
 hipError_t launch_synth(uint8_t *d_out, int64_t nsamples, int bits, const int32_t *shuffle,
                         int lsb, uint64_t seed, int station, double rho, int64_t delay,
