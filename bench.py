@@ -209,7 +209,9 @@ def main():
         "roofline": {"bound": "mfma", "achieved": round(r["tflops"], 3),
                      "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(r["tflops"] / FP32_PEAK_TFLOPS, 4), "traffic": None,
-                     "kernel": "fused decode+shift+FFT+MAC (k_sector<8,64>)",
+                     "kernel": ("fused decode+shift+MFMA four-step FFT+MAC (k_sector_mf2)"
+                                if main_form == "chosen" else
+                                "fused decode+shift+FFT+MAC (k_sector<8,64,0>)"),
                      "hbm_frac": round(r["gbs"] / HBM_PEAK_GBS, 4)},
         "cpu_baseline": cpu,
     }

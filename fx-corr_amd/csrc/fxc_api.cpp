@@ -140,8 +140,8 @@ struct fxc_plan {
     StationDev st[2];
     int32_t decode_ok[2] = {1, 1};
     // device tables
-    DevBuf tw_fft, tw_post, lev_dev[2], out_lev, mfma_b, lev16_dev[2];
-    int fft_form = 0;  // 0: FP32 Stockham, 1: FP16-MFMA four-step (N = 1024)
+    DevBuf tw_fft, tw_post, lev_dev[2], out_lev, mf_tab;
+    int fft_form = 0;  // 0: FP32 Stockham, 1: FP16/FP32-MFMA four-step (N = 1024)
     // scratch
     DevBuf part, red_scratch, table_dev, d_raw[2], d_acc, d_raw_out, d_tmp_in, d_tmp_out,
         d_shifts;
@@ -444,9 +444,7 @@ static void fill_args(fxc_plan *p, SectorArgs &a) {
     if (p->desc.products & FXC_PRODUCT_RAW_OUT) fl |= FLAG_RAW_OUT;
     if (p->desc.products & FXC_PRODUCT_ACF_OVERLAP_ONLY) fl |= FLAG_ACF_OVERLAP;
     a.flags = (int32_t)fl;
-    a.mfma_b = (const uint4 *)p->mfma_b.p;
-    a.lev16[0] = (const uint16_t *)p->lev16_dev[0].p;
-    a.lev16[1] = (const uint16_t *)p->lev16_dev[1].p;
+    a.mf_tab = p->mf_tab.p;
 }
 
 // IEEE binary16 bits of a float, round to nearest even (normals, subnormals,
@@ -488,12 +486,14 @@ static float f16_bits_to_f32(uint16_t h) {
     return v;
 }
 
-// The FP16-MFMA first stage applies when N = 1024, the chosen precision is
-// requested, ACF/XCF products are computed, and every level (as the f32 the
-// decode produces, utils.rs:186) is exact in FP16.
+// The MFMA four-step form applies when N = 1024, the chosen precision is
+// requested, ACF/XCF products are computed on a common band start
+// (a1s == a2s, so a bin and its mirror M-k share one lane), and every level
+// (as the f32 the decode produces, utils.rs:186) is exact in FP16.
 static bool mfma_eligible(const fxc_plan *p) {
     if (p->N != 1024 || p->desc.precision != FXC_PRECISION_CHOSEN) return false;
     if (p->desc.products & (FXC_PRODUCT_PHASED | FXC_PRODUCT_RAW_OUT)) return false;
+    if (p->desc.a1s != p->desc.a2s) return false;
     for (int i = 0; i < 2; ++i) {
         const int b = p->desc.st[i].bits;
         if (b != 1 && b != 2 && b != 4 && b != 8) return false;
@@ -505,30 +505,104 @@ static bool mfma_eligible(const fxc_plan *p) {
     return true;
 }
 
-// DFT-32 operands of the complex first stage (fxc_mfma.hip): K rows
-// kk = 32e + m1 (e = 0: Re z, 1: Im z), columns c = k1 (Re) / 32 + k1 (Im).
-// Per lane l: half8 of B[32e + 8(l/16) + j][16t + l%16], j = 0..7, stored as
-// [lane][hi: e*4+t | lo: 8 + e*4+t] with B = hi + 2^-11 lo.
-static std::vector<uint16_t> mfma_b_table() {
-    std::vector<uint16_t> tab(64 * 16 * 8);
-    for (int l = 0; l < 64; ++l)
-        for (int e = 0; e < 2; ++e)
-            for (int t = 0; t < 4; ++t)
-                for (int j = 0; j < 8; ++j) {
-                    const int m1 = 8 * (l / 16) + j;
-                    const int c = 16 * t + (l % 16);
-                    const bool im = c >= 32;
-                    const int k1 = im ? c - 32 : c;
-                    const double th = 2.0 * M_PI * (double)((m1 * k1) % 32) / 32.0;
-                    double v;
-                    if (e == 0) v = im ? -std::sin(th) : std::cos(th);
-                    else v = im ? std::cos(th) : std::sin(th);
+// Tables of the MFMA four-step kernel (fxc_mfma.hip), MFT_* layout, all in
+// [idx][lane] order (lane l: g = l/16, c = l%16).
+//  B1 idx (kap*4 + t)*2 + lohi: half8 slot s of B[K][col] with K = (m1 =
+//     16 kap + 4 g + s/2, e = s%2: re|im of z), col = (k1 = c + 16(t&1),
+//     part = t/2: re|im of Y); B = 1/2 * DFT-32 (the realfft split's 1/2),
+//     stored as hi + 2^-11 lo.
+//  A2 idx rt*8 + b: DFT-16 coefficient of output (k2 = c, rt: re|im) and
+//     input (m2 = 4 g + b/2, b%2: re|im).
+//  TW2 idx (ct*4 + i)*2 + part: W512^{m2 k1}, m2 = 4 g + i, k1 = c + 16 ct.
+//  TWU idx 2j + part: W1024^p, p = l + 64 j.
+//  DLUT per station: FP16 samples of each byte value (code = low bits
+//     first, odd samples negated for LSB stations); 8-bit keeps an odd-byte
+//     copy at +256.  SLUT per station: generic shuffle as 4 byte tables.
+static std::vector<uint8_t> mfma_tables(const fxc_plan *p) {
+    std::vector<uint8_t> tab(MFT_BYTES, 0);
+    auto put16 = [&](size_t off, uint16_t v) { memcpy(&tab[off], &v, 2); };
+    auto putf = [&](size_t off, float v) { memcpy(&tab[off], &v, 4); };
+    auto put32 = [&](size_t off, uint32_t v) { memcpy(&tab[off], &v, 4); };
+    for (int kap = 0; kap < 2; ++kap)
+        for (int t = 0; t < 4; ++t)
+            for (int l = 0; l < 64; ++l)
+                for (int s = 0; s < 8; ++s) {
+                    const int g = l >> 4, c = l & 15;
+                    const int k1 = c + 16 * (t & 1), part = t >> 1;
+                    const int m1 = 16 * kap + 4 * g + (s >> 1), e = s & 1;
+                    const int r = (m1 * k1) % 32;
+                    double cs, sn;
+                    if (r % 8 == 0) {   // exact quadrant values
+                        const int q = r / 8;
+                        cs = (q == 0) ? 1.0 : (q == 2 ? -1.0 : 0.0);
+                        sn = (q == 1) ? 1.0 : (q == 3 ? -1.0 : 0.0);
+                    } else {
+                        cs = std::cos(2.0 * M_PI * r / 32.0);
+                        sn = std::sin(2.0 * M_PI * r / 32.0);
+                    }
+                    const double v = 0.5 * (part == 0 ? (e == 0 ? cs : sn) : (e == 0 ? -sn : cs));
                     const uint16_t hi = f32_to_f16_bits((float)v);
-                    const double rest = (v - (double)f16_bits_to_f32(hi)) * 2048.0;
-                    const uint16_t lo = f32_to_f16_bits((float)rest);
-                    tab[((size_t)l * 16 + (size_t)(e * 4 + t)) * 8 + j] = hi;
-                    tab[((size_t)l * 16 + (size_t)(8 + e * 4 + t)) * 8 + j] = lo;
+                    const uint16_t lo = f32_to_f16_bits((float)((v - (double)f16_bits_to_f32(hi)) * 2048.0));
+                    const size_t idx_hi = (size_t)((kap * 4 + t) * 2 + 0), idx_lo = idx_hi + 1;
+                    put16(MFT_B1 + ((idx_hi * 64 + l) * 8 + s) * 2, hi);
+                    put16(MFT_B1 + ((idx_lo * 64 + l) * 8 + s) * 2, lo);
                 }
+    for (int rt = 0; rt < 2; ++rt)
+        for (int b = 0; b < 8; ++b)
+            for (int l = 0; l < 64; ++l) {
+                const int k2 = l & 15, m2 = 4 * (l >> 4) + (b >> 1), pin = b & 1;
+                const double th = 2.0 * M_PI * (double)((m2 * k2) % 16) / 16.0;
+                const double cs = std::cos(th), sn = std::sin(th);
+                const double v = rt == 0 ? (pin == 0 ? cs : sn) : (pin == 0 ? -sn : cs);
+                putf(MFT_A2 + ((size_t)(rt * 8 + b) * 64 + l) * 4, (float)v);
+            }
+    for (int ct = 0; ct < 2; ++ct)
+        for (int i = 0; i < 4; ++i)
+            for (int l = 0; l < 64; ++l) {
+                const int m2 = 4 * (l >> 4) + i, k1 = (l & 15) + 16 * ct;
+                const double th = -2.0 * M_PI * (double)((m2 * k1) % 512) / 512.0;
+                const size_t idx = (size_t)(ct * 4 + i) * 2;
+                putf(MFT_TW2 + (idx * 64 + l) * 4, (float)std::cos(th));
+                putf(MFT_TW2 + ((idx + 1) * 64 + l) * 4, (float)std::sin(th));
+            }
+    for (int j = 0; j < 4; ++j)
+        for (int l = 0; l < 64; ++l) {
+            const double th = -2.0 * M_PI * (double)(l + 64 * j) / 1024.0;
+            putf(MFT_TWU + ((size_t)(2 * j) * 64 + l) * 4, (float)std::cos(th));
+            putf(MFT_TWU + ((size_t)(2 * j + 1) * 64 + l) * 4, (float)std::sin(th));
+        }
+    for (int st = 0; st < 2; ++st) {
+        const int bits = p->desc.st[st].bits;
+        const bool lsb = p->desc.st[st].lsb != 0;
+        const uint32_t mask = (1u << bits) - 1u;
+        const size_t base = MFT_DLUT + (size_t)st * MFT_DLUT_HALVES * 2;
+        auto lev = [&](uint32_t code, bool neg) {
+            uint16_t h = f32_to_f16_bits((float)p->levels[st][code]);
+            return (uint16_t)(neg ? (h ^ 0x8000u) : h);
+        };
+        if (bits == 8) {
+            for (int v = 0; v < 256; ++v) {
+                put16(base + (size_t)v * 2, lev((uint32_t)v, false));
+                put16(base + (size_t)(256 + v) * 2, lev((uint32_t)v, lsb));
+            }
+        } else {
+            const int spb = 8 / bits;   // samples per byte (even)
+            for (int v = 0; v < 256; ++v)
+                for (int i = 0; i < spb; ++i)
+                    put16(base + (size_t)(v * spb + i) * 2,
+                          lev(((uint32_t)v >> (bits * i)) & mask, lsb && (i & 1)));
+        }
+        const size_t sbase = MFT_SLUT + (size_t)st * 4 * 256 * 4;
+        for (int bp = 0; bp < 4; ++bp)
+            for (int v = 0; v < 256; ++v) {
+                uint32_t o = 0;
+                for (int i = 0; i < 32; ++i) {
+                    const int src = p->desc.st[st].shuffle_internal[i];
+                    if (src >= 8 * bp && src < 8 * bp + 8 && ((v >> (src - 8 * bp)) & 1)) o |= 1u << i;
+                }
+                put32(sbase + ((size_t)bp * 256 + v) * 4, o);
+            }
+    }
     return tab;
 }
 
@@ -621,26 +695,19 @@ extern "C" fxc_status fxc_plan_create(const fxc_plan_desc *desc, fxc_plan **out)
             hipMemcpy(p->out_lev.p, lv.data(), lv.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
             return cleanup(fail(FXC_ERR_HIP, "level upload failed"));
     }
-    if (mfma_eligible(p)) {
-        std::vector<uint16_t> tab = mfma_b_table();
-        if (p->mfma_b.ensure(tab.size() * 2) != hipSuccess ||
-            hipMemcpy(p->mfma_b.p, tab.data(), tab.size() * 2, hipMemcpyHostToDevice) != hipSuccess)
-            return cleanup(fail(FXC_ERR_HIP, "MFMA table upload failed"));
-        for (int i = 0; i < 2; ++i) {
-            std::vector<uint16_t> h(256, 0);
-            for (size_t k = 0; k < p->levels[i].size() && k < 256; ++k)
-                h[k] = f32_to_f16_bits((float)p->levels[i][k]);
-            if (p->lev16_dev[i].ensure(h.size() * 2) != hipSuccess ||
-                hipMemcpy(p->lev16_dev[i].p, h.data(), h.size() * 2, hipMemcpyHostToDevice) != hipSuccess)
-                return cleanup(fail(FXC_ERR_HIP, "level upload failed"));
-        }
-        p->fft_form = 1;
-        p->T = 128;
-    }
-    // grid: about 16 waves per CU worth of workgroups (p->T threads each); each
-    // workgroup then runs a contiguous range of frames
     const int cus = num_cus();
-    p->max_wg = std::max(64, (cus * 1024) / p->T);
+    if (mfma_eligible(p)) {
+        std::vector<uint8_t> tab = mfma_tables(p);
+        if (p->mf_tab.ensure(tab.size()) != hipSuccess ||
+            hipMemcpy(p->mf_tab.p, tab.data(), tab.size(), hipMemcpyHostToDevice) != hipSuccess)
+            return cleanup(fail(FXC_ERR_HIP, "MFMA table upload failed"));
+        p->fft_form = 1;
+        p->T = 64;   // one wave per frame pair
+    }
+    // grid: plain form about 16 waves per CU worth of workgroups (p->T
+    // threads each); MFMA form one 8-wave workgroup per CU.  Each workgroup
+    // runs a contiguous range of frames.
+    p->max_wg = p->fft_form == 1 ? cus : std::max(64, (cus * 1024) / p->T);
     if (p->part.ensure((size_t)p->max_wg * kNumProducts * half * sizeof(double)) != hipSuccess ||
         p->red_scratch.ensure((size_t)64 * kNumProducts * half * sizeof(double)) != hipSuccess)
         return cleanup(fail(FXC_ERR_NOMEM, "scratch allocation failed"));
@@ -662,7 +729,7 @@ extern "C" fxc_status fxc_plan_destroy(fxc_plan *p) {
     DevBuf *bufs[] = {&p->tw_fft, &p->tw_post, &p->lev_dev[0], &p->lev_dev[1], &p->out_lev,
                       &p->part, &p->red_scratch, &p->table_dev, &p->d_raw[0], &p->d_raw[1],
                       &p->d_acc, &p->d_raw_out, &p->d_tmp_in, &p->d_tmp_out, &p->d_shifts,
-                      &p->mfma_b, &p->lev16_dev[0], &p->lev16_dev[1]};
+                      &p->mf_tab};
     for (DevBuf *b : bufs) b->release();
     if (p->stream) (void)hipStreamDestroy(p->stream);
     delete p;
@@ -763,8 +830,9 @@ static fxc_status enqueue_sector(fxc_plan *p, const uint8_t *d_raw1, const uint8
     a.first_frame = first_frame;
     a.nf = nf;
     int64_t fpw = (nf + p->max_wg - 1) / p->max_wg;
-    fpw = ((fpw + kFrameBatch - 1) / kFrameBatch) * kFrameBatch;
-    if (fpw < kFrameBatch) fpw = kFrameBatch;
+    const int batch = p->fft_form == 1 ? 8 : kFrameBatch;   // MFMA form: whole frames per wave
+    fpw = ((fpw + batch - 1) / batch) * batch;
+    if (fpw < batch) fpw = batch;
     const int n_wg = (int)((nf + fpw - 1) / fpw);
     a.frames_per_wg = fpw;
     a.part = (double *)p->part.p;

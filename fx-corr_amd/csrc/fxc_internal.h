@@ -75,9 +75,19 @@ struct SectorArgs {
     int32_t bpf_out;
     uint8_t *raw_out;        // phased .raw frames (nf * bpf_out) or null
     double *part;            // [n_wg][kNumProducts][half]
-    const uint4 *mfma_b;     // FP16 DFT-32 operands, [lane][hi e0t0..e1t3 | lo ...] (N = 1024)
-    const uint16_t *lev16[2];  // FP16 bit patterns of the levels (bits <= 8)
+    const void *mf_tab;      // chosen-form MFMA tables (N = 1024), layout MFT_* below
 };
+
+// Chosen-form MFMA tables (fxc_mfma.hip), byte offsets into one device
+// buffer.  [idx][lane] order so a wave's loads are contiguous.
+constexpr int MFT_B1 = 0;                      // 16 x 64 uint4: DFT-32 FP16 hi/lo operands
+constexpr int MFT_A2 = MFT_B1 + 16 * 64 * 16;  // 16 x 64 f32: DFT-16 A operands
+constexpr int MFT_TW2 = MFT_A2 + 16 * 64 * 4;  // 16 x 64 f32: W512^{m2 k1}
+constexpr int MFT_TWU = MFT_TW2 + 16 * 64 * 4; // 8 x 64 f32: W1024^p of the untangle pairs
+constexpr int MFT_DLUT_HALVES = 2048;          // per station: byte -> FP16 samples
+constexpr int MFT_DLUT = MFT_TWU + 8 * 64 * 4;
+constexpr int MFT_SLUT = MFT_DLUT + 2 * MFT_DLUT_HALVES * 2;  // 2 x 4 x 256 u32
+constexpr int MFT_BYTES = MFT_SLUT + 2 * 4 * 256 * 4;
 
 // Launchers implemented in fxc_device.hip.  All enqueue on `stream`.
 hipError_t launch_sector(const SectorArgs &a, int precision, int n_wg,
@@ -101,6 +111,7 @@ hipError_t launch_quantise_frames(const SectorArgs &a, const float *in,
                                   hipStream_t stream);
 
 hipError_t launch_sector_mfma(const SectorArgs &a, int precision, int n_wg, hipStream_t stream);
+int mfma_wg_threads();
 hipError_t launch_rfft_mfma(const SectorArgs &a, const float *in, int64_t nframes, float2 *out,
                             hipStream_t stream);
 

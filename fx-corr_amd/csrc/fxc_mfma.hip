@@ -1,21 +1,25 @@
 // Chosen-form fused kernel for N = 1024 (M = 512 complex points per
-// station-frame): four-step FFT whose first stage runs on the matrix cores.
+// station-frame, z[m] = x[2m] + i x[2m+1]): a four-step FFT whose two
+// stages both run on the matrix cores.
 //
-//   z[m] = x[2m] + i x[2m+1],  m = 16*m1 + m2,  k = k1 + 32*k2
-//   step 1 (MFMA, FP16 in / FP32 acc):  Y[k1][m2] = sum_m1 z[16 m1 + m2] W32^{m1 k1}
-//   step 2 (FP32):                      Y'[k1][m2] = Y[k1][m2] W512^{m2 k1}
-//   step 3 (FP32, radix-16):            Z[k1 + 32 k2] = sum_m2 Y'[k1][m2] W16^{m2 k2}
+//   m = 16 m1 + m2,  k = k1 + 32 k2
+//   stage 1 (FP16 MFMA, FP32 acc): Y[m2][k1]  = sum_m1 z[16 m1 + m2] W32^{m1 k1}
+//   twiddle (FP32):                Y'[m2][k1] = Y[m2][k1] W512^{m2 k1}
+//   stage 2 (FP32 MFMA):           Z[k1 + 32 k2] = sum_m2 Y'[m2][k1] W16^{m2 k2}
 //
-// Step 1 is a real GEMM D[m2][col] = sum_kk A[m2][kk] B[kk][col] with
-// A = [Re z | Im z] (K = 64) and B = [[cos, -sin], [sin, cos]] (64 x 64
-// outputs: Re / Im for k1 = 0..31).  The sample levels are exact in FP16;
-// the DFT coefficients are split B = B_hi + 2^-11 B_lo (both FP16, B_lo
-// scaled so it stays a normal number) with separate FP32 accumulators.
+// Stage 1 is the real GEMM D[m2][(k1,re|im)] = sum_(m1,re|im) A B with
+// A = the samples (exact in FP16 for 1/2/4/8-bit levels) and B the DFT-32
+// coefficients split as B = B_hi + 2^-11 B_lo (both FP16) with separate
+// FP32 accumulators.  Its accumulator layout (lane l: rows m2 = 4(l/16)+r,
+// column l%16) is exactly the B-operand layout stage 2 needs (K = (m2,
+// re|im), column k1), so the data never leaves registers between stages.
 //
-// Workgroup = 2 waves; wave s transforms station s for two frames per
-// iteration (each MFMA's 16 rows are one frame's m2), so step 3 gives every
-// lane one row.  The untangle to the real spectrum and the ACF/XCF products
-// are the plain kernel's.
+// One wave carries a whole frame pair (both stations): decode into a
+// zero-padded FP16 staging row (the integer shift and its zero fill become
+// an offset into that row), both stages for both stations, the untangle to
+// the real spectrum in bin pairs (k, M-k), and the ACF/XCF products into
+// FP64 registers.  Eight waves per workgroup share the read-only tables and
+// never wait for each other inside the frame loop.
 #include "fxc_dev.h"
 
 namespace fxc {
@@ -23,295 +27,520 @@ namespace fxc {
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
-constexpr int MF_M = 512;
-constexpr int MF_YS = 36;                      // floats per transposed row (16 complex + pad)
-constexpr int MF_BUF = 32 * MF_YS;             // Y rows; the padded Z spectrum aliases it
-static_assert(MF_BUF >= 2 * (MF_M + MF_M / 16 + 2), "Z must fit in the Y buffer");
-constexpr int MF_WORDS = 256 + 4;              // staged words of one station-frame (bits <= 8)
+namespace mf {
+constexpr int N = 1024, M = 512;
+constexpr int WAVES = 8;                 // waves per workgroup
+constexpr int PAD = 64;                  // zero FP16 samples each side of a staged frame
+constexpr int SROW = N + 2 * PAD;        // halves per staged station-frame
+constexpr int FB = 64;                   // frames whose parameters one batch computes
+constexpr float LO_SCALE = 1.0f / 2048.0f;
+}  // namespace mf
 
-__device__ __forceinline__ half8 load_half8(const uint4 *p) {
-    uint4 v = *p;
-    half8 h;
-    __builtin_memcpy(&h, &v, sizeof h);
-    return h;
+struct MfShared {
+    uint4 b1[16 * 64];                           // stage-1 B operands [idx][lane]
+    float a2[16 * 64];                           // stage-2 A operands [rt*8 + b][lane]
+    float tw2[16 * 64];                          // W512^{m2 k1} [(ct*4 + i)*2 + part][lane]
+    float twu[8 * 64];                           // W1024^p [2j + part][lane]
+    double mid[mf::WAVES][4];                    // per-wave accumulators of bin M/2
+    uint32_t slut[2][4 * 256];                   // generic shuffle: byte -> output bits
+    uint16_t dlut[2][MFT_DLUT_HALVES];           // byte -> FP16 samples
+    uint16_t stage[mf::WAVES][2][mf::SROW];      // [wave][station] decoded frame + zero pad
+    float2 z[mf::WAVES][2][mf::M];               // [wave][station] Z/2 in natural order
+};
+
+// Orders this wave's LDS writes before its later LDS reads (a wave's LDS
+// operations complete in issue order; this keeps the compiler from moving
+// them across each other).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// One frame's A operand for K-step e (0: Re z, 1: Im z) of lane l: row
-// m2 = l%16, K slots m1 = 8*(l/16) + j.  Element = sample 32*m1 + 2*m2 + e of
-// the integer-shifted frame, as its FP16 level bits.
-__device__ __forceinline__ half8 build_a(const uint32_t *words, const StationDev &st,
-                                         const uint16_t *lev16, int l, int e, int sh, int N) {
-    const int m2 = l & 15, g = l >> 4;
-    const int base = 2 * m2 + e - sh;            // sample index minus 32*m1
-    const int bb = base * st.bits;
-    const int wq = bb >> 5;                      // floor(base*bits/32)
-    const uint32_t off = (uint32_t)bb & 31u;
-    const uint16_t sgn = (st.lsb && (base & 1)) ? 0x8000u : 0u;
-    uint16_t v[8];
+__device__ __forceinline__ double readlane_d(double v, int lane) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)u, lane);
+    const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(u >> 32), lane);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+__device__ __forceinline__ float readlane_f(float v, int lane) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(uint32_t, v), lane));
+}
+
+// (cos, sin)(2 pi r) for |r| <= 0.5 turns: quadrant reduction to
+// |t| <= 1/8 turn and Taylor polynomials through x^9 / x^8 (truncation
+// error below 3e-8 on |x| <= pi/4).
+__device__ __forceinline__ float2 sincos_turns(float r) {
+    const float q = rintf(4.0f * r);
+    const float t = r - 0.25f * q;
+    const float x = t * 6.28318530717958647692f;
+    const float x2 = x * x;
+    float s = fmaf(x2, 2.75573192e-6f, -1.98412698e-4f);
+    s = fmaf(x2, s, 8.33333333e-3f);
+    s = fmaf(x2, s, -1.66666667e-1f);
+    s = fmaf(x2 * x, s, x);
+    float c = fmaf(x2, 2.48015873e-5f, -1.38888889e-3f);
+    c = fmaf(x2, c, 4.16666667e-2f);
+    c = fmaf(x2, c, -0.5f);
+    c = fmaf(x2, c, 1.0f);
+    const int qi = ((int)q) & 3;
+    float co = c, si = s;
+    if (qi == 1) { co = -s; si = c; }
+    else if (qi == 2) { co = -c; si = -s; }
+    else if (qi == 3) { co = s; si = -c; }
+    return make_float2(co, si);
+}
+
+__device__ __forceinline__ double wrap_turns(double x) { return x - rint(x); }
+
+// Per-frame values of the chosen form (main.rs:184-241, 133-149 restated in
+// turns): integer shifts, the fringe difference phw = w1 - w2 and the
+// per-bin XCF phase step dth = -df (frac1 - frac2), so that the XCF phasor
+// of grid bin k is exp(2 pi i (phw + k dth)) when both stations share the
+// band start (a1s == a2s).  F1 = exp(2 pi i 64 dth), C = exp(2 pi i (2 phw
+// + 512 dth)) turn the per-lane phasor into the phasors of its other bins.
+struct MfFrame {
+    int32_t s1, s2;
+    double phw, dth;
+    float2 F1, C;
+};
+
+__device__ MfFrame mf_frame(int64_t idx, const SectorArgs &a) {
+    MfFrame o;
+    double tau1, tau2, tf1, tf2;
+    frame_taus(idx, a.dly, tau1, tau2, tf1, tf2);
+    int64_t n1, n2;
+    double fr1, fr2;
+    split_delay(tau1, a.dly.fs, n1, fr1);
+    split_delay(tau2, a.dly.fs, n2, fr2);
+    o.s1 = clamp_shift(n1, mf::N);
+    o.s2 = clamp_shift(n2, mf::N);
+    const double w1 = a.dly.lo1 == 0.0 ? 0.0 : wrap_turns(-a.dly.lo1 * tf1);
+    const double w2 = a.dly.lo2 == 0.0 ? 0.0 : wrap_turns(-a.dly.lo2 * tf2);
+    o.phw = w1 - w2;
+    o.dth = -a.df * (fr1 - fr2);
+    o.F1 = sincos_turns((float)wrap_turns(64.0 * o.dth));
+    o.C = sincos_turns((float)wrap_turns(2.0 * o.phw + 512.0 * o.dth));
+    return o;
+}
+
+// ------------------------------------------------------------ decode
+__device__ __forceinline__ uint32_t mf_shuffle(uint32_t v, int kind, const uint32_t *slut) {
+    if (kind == SHUF_PAIRSWAP) return ((v & 0xAAAAAAAAu) >> 1) | ((v & 0x55555555u) << 1);
+    if (kind == SHUF_GENERIC)
+        return slut[v & 255u] | slut[256 + ((v >> 8) & 255u)] | slut[512 + ((v >> 16) & 255u)] |
+               slut[768 + (v >> 24)];
+    return v;
+}
+
+// The packed words of one station-frame this lane stages (word l + 64 u).
+__device__ __forceinline__ void mf_load(const uint8_t *frame, int nwords, int l, uint32_t w[4]) {
+    const uint32_t *src = reinterpret_cast<const uint32_t *>(frame);
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-        const int m1 = 8 * g + j;
-        const int src = 32 * m1 + base;
-        uint16_t h = 0;
-        if ((unsigned)src < (unsigned)N) {
-            const uint32_t code = (words[m1 * st.bits + wq] >> off) & st.code_mask;
-            h = (uint16_t)(lev16[code] ^ sgn);
+    for (int u = 0; u < 4; ++u)
+        if (l + 64 * u < nwords) w[u] = __builtin_nontemporal_load(src + l + 64 * u);
+}
+
+// Decode this lane's words into FP16 samples at their frame positions
+// (utils.rs:215-376: LSB-first codes, levels, odd samples negated for LSB
+// stations -- the sign is folded into the per-station byte table).
+__device__ __forceinline__ void mf_stage(const uint32_t w[4], int bits, int kind, int nwords,
+                                         const uint32_t *slut, const uint16_t *dlut,
+                                         uint16_t *frame, int l) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int wi = l + 64 * u;
+        if (wi >= nwords) continue;
+        const uint32_t v = mf_shuffle(w[u], kind, slut);
+        const uint32_t b0 = v & 255u, b1 = (v >> 8) & 255u, b2 = (v >> 16) & 255u, b3 = v >> 24;
+        if (bits == 2) {
+            const uint2 e0 = *reinterpret_cast<const uint2 *>(dlut + 4 * b0);
+            const uint2 e1 = *reinterpret_cast<const uint2 *>(dlut + 4 * b1);
+            const uint2 e2 = *reinterpret_cast<const uint2 *>(dlut + 4 * b2);
+            const uint2 e3 = *reinterpret_cast<const uint2 *>(dlut + 4 * b3);
+            uint4 *dst = reinterpret_cast<uint4 *>(frame + 16 * wi);
+            dst[0] = make_uint4(e0.x, e0.y, e1.x, e1.y);
+            dst[1] = make_uint4(e2.x, e2.y, e3.x, e3.y);
+        } else if (bits == 1) {
+            uint4 *dst = reinterpret_cast<uint4 *>(frame + 32 * wi);
+            dst[0] = *reinterpret_cast<const uint4 *>(dlut + 8 * b0);
+            dst[1] = *reinterpret_cast<const uint4 *>(dlut + 8 * b1);
+            dst[2] = *reinterpret_cast<const uint4 *>(dlut + 8 * b2);
+            dst[3] = *reinterpret_cast<const uint4 *>(dlut + 8 * b3);
+        } else if (bits == 4) {
+            const uint32_t *d32 = reinterpret_cast<const uint32_t *>(dlut);
+            *reinterpret_cast<uint4 *>(frame + 8 * wi) = make_uint4(d32[b0], d32[b1], d32[b2], d32[b3]);
+        } else {  // 8 bits: one sample per byte; odd bytes are odd samples
+            const uint32_t h0 = dlut[b0], h1 = dlut[256 + b1], h2 = dlut[b2], h3 = dlut[256 + b3];
+            *reinterpret_cast<uint2 *>(frame + 4 * wi) = make_uint2(h0 | (h1 << 16), h2 | (h3 << 16));
         }
-        v[j] = h;
     }
-    half8 r;
-    __builtin_memcpy(&r, v, sizeof r);
-    return r;
 }
 
-template <int PREC>
-__global__ __launch_bounds__(128, 2) void k_sector_mfma(SectorArgs a) {
-    __shared__ float buf[2][2][MF_BUF];          // [station][frame]: Y rows, then Z
-    __shared__ uint32_t words[2][2][MF_WORDS];   // [station][frame]
-    __shared__ FrameParam fp[kFrameBatch];
-    __shared__ uint16_t lev16[2][256];
-    __shared__ uint4 bsh[64 * 16];               // B operands [lane][hi 0..7 | lo 8..15]
+// Stage-1 A operands of one station-frame.  Lane l (m2 = l%16, g = l/16)
+// holds, in K-step kap, slots s = 2j + e: sample 2(16 m1 + m2) + e of the
+// shifted frame with m1 = 16 kap + 4 g + j, i.e. the FP16 pair at staged
+// position PAD + 32 m1 + 2 m2 - sh.  Positions outside the staged row read
+// the zero padding.
+__device__ __forceinline__ void mf_build_a(const uint16_t *row, int sh, int l, half8 &a0, half8 &a1) {
+    const uint32_t *r32 = reinterpret_cast<const uint32_t *>(row);
+    const int base = mf::PAD + 2 * (l & 15) - sh + 128 * (l >> 4);
+    uint32_t d[8];
+    const bool near = (sh >= -mf::PAD) && (sh <= mf::PAD);
+    if ((sh & 1) == 0) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            int p = base + 512 * (k >> 2) + 32 * (k & 3);
+            if (!near) p = (p >= 0 && p < mf::SROW - 1) ? p : 0;
+            d[k] = r32[p >> 1];
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            int p = base + 512 * (k >> 2) + 32 * (k & 3);
+            if (!near) p = (p >= 0 && p < mf::SROW - 1) ? p : 1;
+            d[k] = __builtin_amdgcn_alignbyte(r32[(p + 1) >> 1], r32[(p - 1) >> 1], 2);
+        }
+    }
+    a0 = __builtin_bit_cast(half8, make_uint4(d[0], d[1], d[2], d[3]));
+    a1 = __builtin_bit_cast(half8, make_uint4(d[4], d[5], d[6], d[7]));
+}
 
-    constexpr int M = MF_M, NT = 128, NB = (M + NT) / NT;
-    constexpr int LOGNT = 7, LOGM = 9;
+// ------------------------------------------------------------- FFT
+// Stage 1 for one station-frame, column tile t (t&1: k1 half, t>>1: re|im):
+// y[r] = Y[m2 = 4(l/16) + r][k1 = l%16 + 16(t&1)] (re or im).
+__device__ __forceinline__ f4 mf_stage1(half8 a0, half8 a1, half8 bh0, half8 bl0, half8 bh1,
+                                        half8 bl1) {
+    f4 dh = {0.f, 0.f, 0.f, 0.f}, dl = {0.f, 0.f, 0.f, 0.f};
+    dh = __builtin_amdgcn_mfma_f32_16x16x32_f16(a0, bh0, dh, 0, 0, 0);
+    dl = __builtin_amdgcn_mfma_f32_16x16x32_f16(a0, bl0, dl, 0, 0, 0);
+    dh = __builtin_amdgcn_mfma_f32_16x16x32_f16(a1, bh1, dh, 0, 0, 0);
+    dl = __builtin_amdgcn_mfma_f32_16x16x32_f16(a1, bl1, dl, 0, 0, 0);
+    f4 y;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) y[r] = fmaf(dl[r], mf::LO_SCALE, dh[r]);
+    return y;
+}
+
+__device__ __forceinline__ half8 ld_b1(const uint4 *b1, int idx, int l) {
+    return __builtin_bit_cast(half8, b1[idx * 64 + l]);
+}
+
+// Twiddle and stage 2 of one station-frame; writes Z/2 in natural order.
+// y[t][r] from stage 1; tw2 / a2 are the [idx][lane] tables.
+__device__ __forceinline__ void mf_stage2(const f4 y[4], const float *tw2, const float *a2,
+                                          float2 *z, int l) {
+    float2 yc[2][4];
+#pragma unroll
+    for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            yc[ct][i] = c_mul(make_float2(y[ct][i], y[2 + ct][i]),
+                              make_float2(tw2[((ct * 4 + i) * 2) * 64 + l],
+                                          tw2[((ct * 4 + i) * 2 + 1) * 64 + l]));
+    f4 d[2][2];
+#pragma unroll
+    for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct) d[rt][ct] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct) {
+            const float bv = (b & 1) ? yc[ct][b >> 1].y : yc[ct][b >> 1].x;
+#pragma unroll
+            for (int rt = 0; rt < 2; ++rt)
+                d[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a2[(rt * 8 + b) * 64 + l], bv,
+                                                                 d[rt][ct], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            z[(l & 15) + 16 * ct + 32 * (4 * (l >> 4) + r)] = make_float2(d[0][ct][r], d[1][ct][r]);
+}
+
+// Real-spectrum bins p and M-p from Z/2 (realfft's split method, paired):
+// E = Z'[p] + conj Z'[M-p], O = Z'[p] - conj Z'[M-p], T = W_N^p O,
+// X[p] = E - iT, X[M-p] = conj(E + iT).  p = 0 gives DC and Nyquist.
+__device__ __forceinline__ void untangle_pair(float2 a, float2 b, float2 w, float2 &xp, float2 &xm) {
+    const float ex = a.x + b.x, ey = a.y - b.y;
+    const float ox = a.x - b.x, oy = a.y + b.y;
+    const float tx = w.x * ox - w.y * oy, ty = w.x * oy + w.y * ox;
+    xp = make_float2(ex + ty, ey - tx);
+    xm = make_float2(ex - ty, -(ey + tx));
+}
+
+// ------------------------------------------------------- fused kernel
+// Bins of lane l: slot 2j -> p = l + 64 j, slot 2j+1 -> M - p (j < 4);
+// slot 8 -> M/2 (lane 0 only).
+__device__ __forceinline__ int mf_bin(int slot, int l) {
+    if (slot == 8) return mf::M / 2;
+    const int p = l + 64 * (slot >> 1);
+    return (slot & 1) ? mf::M - p : p;
+}
+
+__global__ __launch_bounds__(512) void k_sector_mf2(SectorArgs a) {
+    __shared__ MfShared sh;
+    constexpr int M = mf::M;
     const int tid = threadIdx.x;
-    const int s = __builtin_amdgcn_readfirstlane(tid >> 6);   // station of this wave
     const int l = tid & 63;
-    const int N = 2 * M;
-    const StationDev &st = a.st[s];
-    const uint8_t *raw = s ? a.raw2 : a.raw1;
-    const int64_t f_begin = (int64_t)blockIdx.x * a.frames_per_wg;
-    int64_t f_end = f_begin + a.frames_per_wg;
-    if (f_end > a.nf) f_end = a.nf;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const uint8_t *tab = reinterpret_cast<const uint8_t *>(a.mf_tab);
 
-    for (int i = l; i < 256; i += 64) lev16[s][i] = (i <= (int)st.code_mask) ? a.lev16[s][i] : 0;
-
-    for (int i = tid; i < 64 * 16; i += 128) bsh[i] = a.mfma_b[i];
-    const uint4 *bl_ = bsh + l * 16;           // this lane's B operands
-    const int k1 = l & 31, qrow = l >> 5;   // this lane's step-3 row
-
-    double acc[4][NB];
-#pragma unroll
-    for (int p = 0; p < 4; ++p)
-#pragma unroll
-        for (int i = 0; i < NB; ++i) acc[p][i] = 0.0;
+    // ---- read-only tables and the zero padding
+    {
+        const uint4 *g = reinterpret_cast<const uint4 *>(tab + MFT_B1);
+        for (int i = tid; i < 16 * 64; i += 512) sh.b1[i] = g[i];
+        const uint32_t *gs = reinterpret_cast<const uint32_t *>(tab + MFT_SLUT);
+        for (int i = tid; i < 2 * 4 * 256; i += 512) (&sh.slut[0][0])[i] = gs[i];
+        const uint32_t *gd = reinterpret_cast<const uint32_t *>(tab + MFT_DLUT);
+        for (int i = tid; i < 2 * MFT_DLUT_HALVES / 2; i += 512)
+            reinterpret_cast<uint32_t *>(&sh.dlut[0][0])[i] = gd[i];
+        uint32_t *st32 = reinterpret_cast<uint32_t *>(&sh.stage[0][0][0]);
+        for (int i = tid; i < mf::WAVES * 2 * mf::SROW / 2; i += 512) st32[i] = 0u;
+        const float *ga = reinterpret_cast<const float *>(tab + MFT_A2);
+        const float *gt = reinterpret_cast<const float *>(tab + MFT_TW2);
+        const float *gu = reinterpret_cast<const float *>(tab + MFT_TWU);
+        for (int i = tid; i < 16 * 64; i += 512) {
+            sh.a2[i] = ga[i];
+            sh.tw2[i] = gt[i];
+        }
+        for (int i = tid; i < 8 * 64; i += 512) sh.twu[i] = gu[i];
+        if (tid < mf::WAVES * 4) (&sh.mid[0][0])[tid] = 0.0;
+    }
+    __syncthreads();
 
     const bool do_acf = (a.flags & FLAG_ACF) != 0;
     const bool do_xcf = (a.flags & FLAG_XCF) != 0;
     const bool ov_only = (a.flags & FLAG_ACF_OVERLAP) != 0;
-    const int gs = (a.grid == 0) ? a.a1s : a.a2s;
-    const int ge = gs + a.ov;
-    const int sb_other = (a.grid == 0) ? (a.a2s - a.a1s) : (a.a1s - a.a2s);
-    const float lo_scale = 1.0f / 2048.0f;
+    const int gs = a.a1s, ge = a.a1s + a.ov;   // a1s == a2s on this path
+    // per-slot masks: grid-station ACF, other-station ACF / XCF (overlap)
+    uint32_t m_grid = 0, m_ov = 0;
+#pragma unroll
+    for (int s = 0; s < 9; ++s) {
+        const int k = mf_bin(s, l);
+        const bool in_ov = k >= gs && k < ge;
+        if (!ov_only || in_ov) m_grid |= 1u << s;
+        if (in_ov) m_ov |= 1u << s;
+    }
+    const uint32_t m11 = a.grid == 0 ? m_grid : m_ov;
+    const uint32_t m22 = a.grid == 0 ? m_ov : m_grid;
 
-    for (int64_t fb = f_begin; fb < f_end; fb += kFrameBatch) {
-        const int nb = (int)((f_end - fb) < kFrameBatch ? (f_end - fb) : kFrameBatch);
-        __syncthreads();
-        if (tid < nb) make_frame_param<PREC>(a.first_frame + fb + tid, a, LOGM, fp[tid]);
-        __syncthreads();
-        for (int jf = 0; jf < nb; jf += 2) {
-            const int nq = (nb - jf) >= 2 ? 2 : 1;
-            // ---- stage this station's words for the (up to) two frames
+    double acc[8][4];
 #pragma unroll
-            for (int q = 0; q < 2; ++q)
-                if (q < nq) stage_words<64>(raw + (fb + jf + q) * st.bpf, st, words[s][q], l);
-            __syncthreads();
-            // ---- step 1 on the matrix cores, transposed into rows [k1][m2]
+    for (int s = 0; s < 8; ++s)
 #pragma unroll
-            for (int q = 0; q < 2; ++q) {
-                const int sh = (q < nq) ? (s ? fp[jf + q].s2 : fp[jf + q].s1) : N;
-                const half8 a0 = build_a(words[s][q], st, lev16[s], l, 0, sh, N);
-                const half8 a1 = build_a(words[s][q], st, lev16[s], l, 1, sh, N);
-                float *yq = buf[s][q];
+        for (int c = 0; c < 4; ++c) acc[s][c] = 0.0;
+
+    const int bits1 = a.st[0].bits, bits2 = a.st[1].bits;
+    const int kind1 = a.st[0].shuffle_kind, kind2 = a.st[1].shuffle_kind;
+    const int nw1 = a.st[0].nwords, nw2 = a.st[1].nwords;
+    const int64_t bpf1 = a.st[0].bpf, bpf2 = a.st[1].bpf;
+    uint16_t *row1 = sh.stage[wv][0];
+    uint16_t *row2 = sh.stage[wv][1];
+    float2 *z1 = sh.z[wv][0];
+    float2 *z2 = sh.z[wv][1];
+
+    // frames of this wave: a contiguous slice of the workgroup's range
+    const int64_t wg0 = (int64_t)blockIdx.x * a.frames_per_wg;
+    int64_t wg1 = wg0 + a.frames_per_wg;
+    if (wg1 > a.nf) wg1 = a.nf;
+    const int64_t per_wave = (a.frames_per_wg + mf::WAVES - 1) / mf::WAVES;
+    const int64_t f0 = wg0 + per_wave * wv;
+    const int64_t f1 = (f0 + per_wave < wg1) ? f0 + per_wave : wg1;
+
+    uint32_t cur1[4] = {0, 0, 0, 0}, cur2[4] = {0, 0, 0, 0};
+    if (f0 < f1) {
+        mf_load(a.raw1 + f0 * bpf1, nw1, l, cur1);
+        mf_load(a.raw2 + f0 * bpf2, nw2, l, cur2);
+    }
+    for (int64_t fb = f0; fb < f1; fb += mf::FB) {
+        const int nb = (int)((f1 - fb) < mf::FB ? (f1 - fb) : mf::FB);
+        MfFrame pf = {};
+        if (l < nb) pf = mf_frame(a.first_frame + fb + l, a);
+        for (int jf = 0; jf < nb; ++jf) {
+            const int64_t f = fb + jf;
+            uint32_t nxt1[4] = {0, 0, 0, 0}, nxt2[4] = {0, 0, 0, 0};
+            if (f + 1 < f1) {
+                mf_load(a.raw1 + (f + 1) * bpf1, nw1, l, nxt1);
+                mf_load(a.raw2 + (f + 1) * bpf2, nw2, l, nxt2);
+            }
+            const int s1 = __builtin_amdgcn_readlane(pf.s1, jf);
+            const int s2 = __builtin_amdgcn_readlane(pf.s2, jf);
+
+            // ---- decode both stations into their staging rows
+            mf_stage(cur1, bits1, kind1, nw1, sh.slut[0], sh.dlut[0], row1 + mf::PAD, l);
+            mf_stage(cur2, bits2, kind2, nw2, sh.slut[1], sh.dlut[1], row2 + mf::PAD, l);
+            wave_lds_sync();
+            half8 a10, a11, a20, a21;
+            mf_build_a(row1, s1, l, a10, a11);
+            mf_build_a(row2, s2, l, a20, a21);
+
+            // ---- stage 1 (B operands shared by both stations)
+            f4 y1[4], y2[4];
 #pragma unroll
-                for (int t = 0; t < 4; ++t) {
-                    f4 dh = {0.f, 0.f, 0.f, 0.f}, dl = {0.f, 0.f, 0.f, 0.f};
-                    dh = __builtin_amdgcn_mfma_f32_16x16x32_f16(a0, load_half8(bl_ + t), dh, 0, 0, 0);
-                    dh = __builtin_amdgcn_mfma_f32_16x16x32_f16(a1, load_half8(bl_ + 4 + t), dh, 0, 0, 0);
-                    dl = __builtin_amdgcn_mfma_f32_16x16x32_f16(a0, load_half8(bl_ + 8 + t), dl, 0, 0, 0);
-                    dl = __builtin_amdgcn_mfma_f32_16x16x32_f16(a1, load_half8(bl_ + 12 + t), dl, 0, 0, 0);
-                    const int col = (l & 15) + 16 * (t & 1);   // k1
-                    const int part = t >> 1;                   // 0: Re, 1: Im
+            for (int t = 0; t < 4; ++t) {
+                const half8 bh0 = ld_b1(sh.b1, (0 * 4 + t) * 2 + 0, l);
+                const half8 bl0 = ld_b1(sh.b1, (0 * 4 + t) * 2 + 1, l);
+                const half8 bh1 = ld_b1(sh.b1, (1 * 4 + t) * 2 + 0, l);
+                const half8 bl1 = ld_b1(sh.b1, (1 * 4 + t) * 2 + 1, l);
+                y1[t] = mf_stage1(a10, a11, bh0, bl0, bh1, bl1);
+                y2[t] = mf_stage1(a20, a21, bh0, bl0, bh1, bl1);
+            }
+            // ---- twiddle + stage 2, Z/2 to LDS
+            mf_stage2(y1, sh.tw2, sh.a2, z1, l);
+            mf_stage2(y2, sh.tw2, sh.a2, z2, l);
+            wave_lds_sync();
+
+            // ---- XCF phasor of this lane's first bin (FP64 angle reduced
+            // mod 1 turn before the f32 sincos)
+            const double phw = readlane_d(pf.phw, jf);
+            const double dth = readlane_d(pf.dth, jf);
+            const float2 F1 = make_float2(readlane_f(pf.F1.x, jf), readlane_f(pf.F1.y, jf));
+            const float2 Cm = make_float2(readlane_f(pf.C.x, jf), readlane_f(pf.C.y, jf));
+            float2 P = sincos_turns((float)wrap_turns(fma((double)l, dth, phw)));
+
+            // ---- products on the output grid (main.rs:2491-2568)
 #pragma unroll
-                    for (int i = 0; i < 4; ++i) {
-                        const int m2 = 4 * (l >> 4) + i;
-                        yq[col * MF_YS + 2 * m2 + part] = dh[i] + dl[i] * lo_scale;
-                    }
+            for (int j = 0; j < 4; ++j) {
+                const int p = l + 64 * j;
+                const int pm = (M - p) & (M - 1);
+                const float2 w = make_float2(sh.twu[(2 * j) * 64 + l], sh.twu[(2 * j + 1) * 64 + l]);
+                float2 x1p, x1m, x2p, x2m;
+                untangle_pair(z1[p], z1[pm], w, x1p, x1m);
+                untangle_pair(z2[p], z2[pm], w, x2p, x2m);
+                const int sp = 2 * j, sm = 2 * j + 1;
+                if (do_acf) {
+                    acc[sp][0] += (double)(((m11 >> sp) & 1u) ? norm_sqr(x1p) : 0.0f);
+                    acc[sp][1] += (double)(((m22 >> sp) & 1u) ? norm_sqr(x2p) : 0.0f);
+                    acc[sm][0] += (double)(((m11 >> sm) & 1u) ? norm_sqr(x1m) : 0.0f);
+                    acc[sm][1] += (double)(((m22 >> sm) & 1u) ? norm_sqr(x2m) : 0.0f);
+                }
+                if (do_xcf) {
+                    const float2 Pp = ((m_ov >> sp) & 1u) ? P : make_float2(0.f, 0.f);
+                    const float2 Pm = ((m_ov >> sm) & 1u) ? c_mul(Cm, c_conj(P)) : make_float2(0.f, 0.f);
+                    const float2 vp = c_mul(c_mul(x1p, c_conj(x2p)), Pp);
+                    const float2 vm = c_mul(c_mul(x1m, c_conj(x2m)), Pm);
+                    acc[sp][2] += (double)vp.x;
+                    acc[sp][3] += (double)vp.y;
+                    acc[sm][2] += (double)vm.x;
+                    acc[sm][3] += (double)vm.y;
+                }
+                P = c_mul(P, F1);
+            }
+            if (l == 0) {   // k = M/2 pairs with itself: X = 2 conj(Z'[M/2]); P is P(M/2)
+                const float2 u1 = z1[M / 2], u2 = z2[M / 2];
+                const float2 x1 = make_float2(2.0f * u1.x, -2.0f * u1.y);
+                const float2 x2 = make_float2(2.0f * u2.x, -2.0f * u2.y);
+                double *mid = sh.mid[wv];
+                if (do_acf) {
+                    mid[0] += (double)(((m11 >> 8) & 1u) ? norm_sqr(x1) : 0.0f);
+                    mid[1] += (double)(((m22 >> 8) & 1u) ? norm_sqr(x2) : 0.0f);
+                }
+                if (do_xcf) {
+                    const float2 Pp = ((m_ov >> 8) & 1u) ? P : make_float2(0.f, 0.f);
+                    const float2 v = c_mul(c_mul(x1, c_conj(x2)), Pp);
+                    mid[2] += (double)v.x;
+                    mid[3] += (double)v.y;
                 }
             }
-            __syncthreads();
-            // ---- steps 2 and 3: one row (k1, frame qrow) per lane
-            {
-                const float4 *row = reinterpret_cast<const float4 *>(buf[s][qrow] + k1 * MF_YS);
-                float2 y[16];
+            wave_lds_sync();
 #pragma unroll
-                for (int j = 0; j < 8; ++j) {
-                    const float4 v = row[j];
-                    y[2 * j] = make_float2(v.x, v.y);
-                    y[2 * j + 1] = make_float2(v.z, v.w);
-                }
-#pragma unroll
-                for (int m2 = 1; m2 < 16; ++m2) y[m2] = c_mul(y[m2], a.tw_fft[(m2 * k1) & (M - 1)]);
-                dft16(y);
-                __syncthreads();                       // every row read before Z overwrites it
-                float2 *zq = reinterpret_cast<float2 *>(buf[s][qrow]);
-#pragma unroll
-                for (int k2 = 0; k2 < 16; ++k2) zq[pad(k1 + 32 * k2)] = y[k2];
+            for (int u = 0; u < 4; ++u) {
+                cur1[u] = nxt1[u];
+                cur2[u] = nxt2[u];
             }
-            __syncthreads();
-            // ---- products on the output grid (main.rs:2491-2568), per frame
-            for (int q = 0; q < nq; ++q) {
-                const FrameParam &P0 = fp[jf + q];
-                const float2 *zg = reinterpret_cast<const float2 *>(buf[a.grid == 0 ? 0 : 1][q]);
-                const float2 *zo = reinterpret_cast<const float2 *>(buf[a.grid == 0 ? 1 : 0][q]);
-                const float2 frmix = P0.frmix;
-                bool have = false;
-                cd P = cd{0.0, 0.0};
-#pragma unroll
-                for (int i = 0; i < NB; ++i) {
-                    const int g = tid + NT * i;
-                    if (g <= M) {
-                        const bool in_ov = (g >= gs) && (g < ge);
-                        const float2 Xg = untangle(zg, g, M, a.tw_post);
-                        const float pg = norm_sqr(Xg);
-                        float2 Xo = make_float2(0.0f, 0.0f);
-                        if (in_ov) Xo = untangle(zo, g + sb_other, M, a.tw_post);
-                        if (do_acf) {
-                            const float pgrid = (!ov_only || in_ov) ? pg : 0.0f;
-                            const float poth = in_ov ? norm_sqr(Xo) : 0.0f;
-                            acc[0][i] += (double)(a.grid == 0 ? pgrid : poth);
-                            acc[1][i] += (double)(a.grid == 0 ? poth : pgrid);
-                        }
-                        if (in_ov && do_xcf) {
-                            const int k = g - gs;
-                            if (!have) {
-                                P = P0.start;
-                                // first overlap bin of this thread: k < NT
-#pragma unroll 1
-                                for (int b = 0; b < LOGNT; ++b)
-                                    if ((k >> b) & 1) P = d_mul(P, P0.pw[b]);
-                                have = true;
-                            } else {
-                                P = d_mul(P, P0.pw[LOGNT]);
-                            }
-                            const float2 ph = make_float2((float)P.x, (float)P.y);
-                            const float2 X1 = (a.grid == 0) ? Xg : Xo;
-                            const float2 X2 = (a.grid == 0) ? Xo : Xg;
-                            float2 v2 = c_mul(X1, c_conj(X2));
-                            if (PREC == 0) v2 = c_mul(v2, frmix);
-                            v2 = c_mul(v2, ph);
-                            acc[2][i] += (double)v2.x;
-                            acc[3][i] += (double)v2.y;
-                        }
-                    }
-                }
-            }
-            __syncthreads();
         }
     }
 
-    const int half = M + 1;
-    double *out = a.part + (size_t)blockIdx.x * kNumProducts * half;
+    // ---- workgroup partial: waves add their bins in a fixed order
+    __syncthreads();
+    double *red = reinterpret_cast<double *>(&sh.z[0][0][0]);   // [4][M + 1]
+    constexpr int H = M + 1;
+    for (int w = 0; w < mf::WAVES; ++w) {
+        if (wv == w) {
 #pragma unroll
-    for (int i = 0; i < NB; ++i) {
-        const int g = tid + NT * i;
-        if (g <= M) {
-            out[P_PH * half + g] = 0.0;
-            out[P_11 * half + g] = acc[0][i];
-            out[P_22 * half + g] = acc[1][i];
-            out[P_12R * half + g] = acc[2][i];
-            out[P_12I * half + g] = acc[3][i];
+            for (int s = 0; s < 8; ++s) {
+                const int k = mf_bin(s, l);
+#pragma unroll
+                for (int c = 0; c < 4; ++c) red[c * H + k] = (w == 0 ? 0.0 : red[c * H + k]) + acc[s][c];
+            }
+            if (l == 0)
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                    red[c * H + M / 2] = (w == 0 ? 0.0 : red[c * H + M / 2]) + sh.mid[w][c];
         }
+        __syncthreads();
+    }
+    double *out = a.part + (size_t)blockIdx.x * kNumProducts * H;
+    for (int k = tid; k < H; k += 512) {
+        out[P_PH * H + k] = 0.0;
+        out[P_11 * H + k] = red[0 * H + k];
+        out[P_22 * H + k] = red[1 * H + k];
+        out[P_12R * H + k] = red[2 * H + k];
+        out[P_12I * H + k] = red[3 * H + k];
     }
 }
 
 hipError_t launch_sector_mfma(const SectorArgs &a, int precision, int n_wg, hipStream_t s) {
-    if (a.N != 2 * MF_M || a.mfma_b == nullptr) return hipErrorInvalidValue;
-    if (precision == 0)
-        hipLaunchKernelGGL(k_sector_mfma<0>, dim3(n_wg), dim3(128), 0, s, a);
-    else
-        hipLaunchKernelGGL(k_sector_mfma<1>, dim3(n_wg), dim3(128), 0, s, a);
+    if (a.N != mf::N || a.mf_tab == nullptr || precision != 1 || a.a1s != a.a2s)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_sector_mf2, dim3(n_wg), dim3(512), 0, s, a);
     return hipGetLastError();
 }
 
-// Forward R2C of frames through the MFMA four-step FFT (test entry): two
-// frames per wave, one wave per workgroup.
+int mfma_wg_threads() { return 512; }
+
+// Forward R2C of frames through the same two-stage MFMA FFT (test entry):
+// one wave per frame.  Inputs must be exact in FP16.
 __global__ __launch_bounds__(64) void k_rfft_mfma(SectorArgs a, const float *in, int64_t nframes,
                                                   float2 *out) {
-    __shared__ float buf[2][MF_BUF];
-    constexpr int M = MF_M;
+    __shared__ uint4 b1[16 * 64];
+    __shared__ uint16_t row[mf::SROW];
+    __shared__ float2 z[mf::M];
     const int l = threadIdx.x;
-    half8 bh[2][4], bl[2][4];
-#pragma unroll
-    for (int e = 0; e < 2; ++e)
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            bh[e][t] = load_half8(a.mfma_b + l * 16 + e * 4 + t);
-            bl[e][t] = load_half8(a.mfma_b + l * 16 + 8 + e * 4 + t);
-        }
-    const int k1 = l & 31, qrow = l >> 5;
-#pragma unroll
-    for (int q = 0; q < 2; ++q) {
-        const int64_t f = 2 * (int64_t)blockIdx.x + q;
-        const int m2 = l & 15, g = l >> 4;
-        _Float16 va[8], vb[8];
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const int m1 = 8 * g + j;
-            const float x0 = (f < nframes) ? in[f * 2 * M + 32 * m1 + 2 * m2] : 0.f;
-            const float x1 = (f < nframes) ? in[f * 2 * M + 32 * m1 + 2 * m2 + 1] : 0.f;
-            va[j] = (_Float16)x0;
-            vb[j] = (_Float16)x1;
-        }
-        half8 a0, a1;
-        __builtin_memcpy(&a0, va, sizeof a0);
-        __builtin_memcpy(&a1, vb, sizeof a1);
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            f4 dh = {0.f, 0.f, 0.f, 0.f}, dl = {0.f, 0.f, 0.f, 0.f};
-            dh = __builtin_amdgcn_mfma_f32_16x16x32_f16(a0, bh[0][t], dh, 0, 0, 0);
-            dh = __builtin_amdgcn_mfma_f32_16x16x32_f16(a1, bh[1][t], dh, 0, 0, 0);
-            dl = __builtin_amdgcn_mfma_f32_16x16x32_f16(a0, bl[0][t], dl, 0, 0, 0);
-            dl = __builtin_amdgcn_mfma_f32_16x16x32_f16(a1, bl[1][t], dl, 0, 0, 0);
-            const int col = (l & 15) + 16 * (t & 1);
-            const int part = t >> 1;
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-                buf[q][col * MF_YS + 2 * (4 * (l >> 4) + i) + part] = dh[i] + dl[i] * (1.0f / 2048.0f);
-        }
+    const int64_t f = blockIdx.x;
+    const uint8_t *tab = reinterpret_cast<const uint8_t *>(a.mf_tab);
+    const uint4 *g = reinterpret_cast<const uint4 *>(tab + MFT_B1);
+    for (int i = l; i < 16 * 64; i += 64) b1[i] = g[i];
+    for (int i = l; i < mf::SROW; i += 64) {
+        const int n = i - mf::PAD;
+        const float v = (n >= 0 && n < mf::N) ? in[f * mf::N + n] : 0.0f;
+        row[i] = __builtin_bit_cast(uint16_t, (_Float16)v);
     }
+    const float *ga = reinterpret_cast<const float *>(tab + MFT_A2);
+    const float *gt = reinterpret_cast<const float *>(tab + MFT_TW2);
+    const float *gu = reinterpret_cast<const float *>(tab + MFT_TWU);
     __syncthreads();
-    float2 y[16];
-    const float4 *row = reinterpret_cast<const float4 *>(buf[qrow] + k1 * MF_YS);
+    half8 a0, a1;
+    mf_build_a(row, 0, l, a0, a1);
+    f4 y[4];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-        const float4 v = row[j];
-        y[2 * j] = make_float2(v.x, v.y);
-        y[2 * j + 1] = make_float2(v.z, v.w);
+    for (int t = 0; t < 4; ++t)
+        y[t] = mf_stage1(a0, a1, ld_b1(b1, t * 2, l), ld_b1(b1, t * 2 + 1, l),
+                         ld_b1(b1, (4 + t) * 2, l), ld_b1(b1, (4 + t) * 2 + 1, l));
+    mf_stage2(y, gt, ga, z, l);
+    __syncthreads();
+    float2 *o = out + f * (mf::M + 1);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int p = l + 64 * j;
+        float2 xp, xm;
+        untangle_pair(z[p], z[(mf::M - p) & (mf::M - 1)],
+                      make_float2(gu[(2 * j) * 64 + l], gu[(2 * j + 1) * 64 + l]), xp, xm);
+        o[p] = xp;
+        o[mf::M - p] = xm;
     }
-#pragma unroll
-    for (int m2 = 1; m2 < 16; ++m2) y[m2] = c_mul(y[m2], a.tw_fft[(m2 * k1) & (M - 1)]);
-    dft16(y);
-    __syncthreads();
-    float2 *zq = reinterpret_cast<float2 *>(buf[qrow]);
-#pragma unroll
-    for (int k2 = 0; k2 < 16; ++k2) zq[pad(k1 + 32 * k2)] = y[k2];
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < 2; ++q) {
-        const int64_t f = 2 * (int64_t)blockIdx.x + q;
-        if (f < nframes)
-            for (int k = l; k <= M; k += 64)
-                out[f * (M + 1) + k] = untangle(reinterpret_cast<const float2 *>(buf[q]), k, M, a.tw_post);
-    }
+    if (l == 0) o[mf::M / 2] = make_float2(2.0f * z[mf::M / 2].x, -2.0f * z[mf::M / 2].y);
 }
 
 hipError_t launch_rfft_mfma(const SectorArgs &a, const float *in, int64_t nframes, float2 *out,
                             hipStream_t s) {
-    if (a.N != 2 * MF_M || a.mfma_b == nullptr) return hipErrorInvalidValue;
+    if (a.N != mf::N || a.mf_tab == nullptr) return hipErrorInvalidValue;
     if (nframes <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_rfft_mfma, dim3((unsigned)((nframes + 1) / 2)), dim3(64), 0, s, a, in,
-                       nframes, out);
+    hipLaunchKernelGGL(k_rfft_mfma, dim3((unsigned)nframes), dim3(64), 0, s, a, in, nframes, out);
     return hipGetLastError();
 }
 

@@ -322,3 +322,47 @@ def test_mfma_sector_mixed_bits(bits):
     p = gpu_plan(c, F.PRECISION_CHOSEN)
     assert p.info.fft_form == 1
     _compare(gpu_run(c, plan=p), oracle_run(c, R.MODE_F64), CHOSEN_TOL)
+
+
+@pytest.mark.parametrize("grid", [0, 1])
+@pytest.mark.parametrize("overlap_only", [False, True])
+def test_mfma_partial_band_grids(grid, overlap_only):
+    """Common band start with a partial overlap (bins 100..399): the grid
+    station's ACF covers every bin, the other station's and the XCF only
+    the overlap (or both ACFs only the overlap)."""
+    n, fs = 1024, 256e6
+    prods = F.PRODUCT_ACF | F.PRODUCT_XCF
+    if overlap_only:
+        prods |= F.PRODUCT_ACF_OVERLAP_ONLY
+    c = make_case(n=n, fs=fs, nf=2000, band=(0, 100, 400, 100), grid=grid, products=prods,
+                  rho=0.5, delay_samples=2.6, seed=31, delay_user_samples=2.6, lo1_hz=8e9,
+                  lo2_hz=8e9)
+    p = gpu_plan(c, F.PRECISION_CHOSEN)
+    assert p.info.fft_form == 1
+    got = gpu_run(c, plan=p)
+    ref = oracle_run(c, R.MODE_F64)
+    _compare(got, ref, CHOSEN_TOL)
+    for k in ("acc11", "acc22", "acc12"):
+        assert zero_pattern_equal(got[k], ref[k]), k
+
+
+def test_mfma_large_and_odd_shifts():
+    """Analytic delay model sweeping the integer shift from about -1100 to
+    +200 samples: odd shifts, shifts past the staging pad and whole frames
+    shifted out."""
+    n, fs = 1024, 128e6
+    c = make_case(n=n, fs=fs, nf=1500, no_table=True, net_d1_base=-8.6e-6,
+                  total_rate1_base=8.5e-4, lo1_hz=2e9, seed=6, rho=0.3)
+    d = R.make_delay_cfg(None, **c["delay"])
+    ints = [R.frame_delay(i, d).int1 for i in range(c["nf"])]
+    assert min(ints) < -n and max(ints) > 64 and any(i % 2 for i in ints)
+    p = gpu_plan(c, F.PRECISION_CHOSEN)
+    assert p.info.fft_form == 1
+    _compare(gpu_run(c, plan=p), oracle_run(c, R.MODE_F64), CHOSEN_TOL)
+
+
+def test_mfma_needs_common_band_start():
+    lv2 = [LEVELS_2BIT, LEVELS_2BIT]
+    p = F.Plan(1024, levels=lv2, df_hz=1.0, band=(-16, 16, 512, 0),
+               precision=F.PRECISION_CHOSEN)
+    assert p.info.fft_form == 0
