@@ -76,6 +76,70 @@ def c2_setup(seed=2025):
     return table, cfg, shuffle_b, residual
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_c2.json")
+KERNEL_KEYS = {"chosen": "k_sector_mf2", "plain": "k_sector<8, 64, 0>"}
+
+
+def _pmc_rows(d):
+    """(kernel name, counter value) per dispatch from a rocprofv3 --pmc CSV run."""
+    import csv
+    import glob
+
+    files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+    if not files:
+        raise SystemExit(f"no *counter_collection.csv under {d}")
+    rows = []
+    with open(files[0], newline="") as fh:
+        for r in csv.DictReader(fh):
+            rows.append((r["Kernel_Name"], float(r["Counter_Value"])))
+    return rows
+
+
+def pmc_summary(fetch_dir, write_dir, out_json, copy_mib):
+    """HBM bytes per launch of each kernel: FETCH_SIZE and WRITE_SIZE (KiB per
+    dispatch) scaled by what they read for the device-to-device copy of known
+    size that 'bench.py --copy-mib M' issues last."""
+    out = {"kernels": {}}
+    for counter, d in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
+        rows = _pmc_rows(d)
+        copies = [v for k, v in rows if "copyBuffer" in k]
+        if not copies:
+            raise SystemExit(f"{counter}: no copy kernel dispatch found")
+        copy_kib = max(copies)   # the calibration copy is the largest one
+        out.setdefault("copy", {})[counter] = copy_kib
+        per = {}
+        for k, v in rows:
+            if "copyBuffer" in k:
+                continue
+            per.setdefault(k, []).append(v)
+        for k, vals in per.items():
+            e = out["kernels"].setdefault(k, {"dispatches": len(vals)})
+            e[counter + "_kib_avg"] = sum(vals) / len(vals)
+    copy_bytes = float(copy_mib) * 2 ** 20
+    fs = copy_bytes / 1024.0 / out["copy"]["FETCH_SIZE"]
+    ws = copy_bytes / 1024.0 / out["copy"]["WRITE_SIZE"]
+    out["copy"].update(bytes=copy_bytes, fetch_scale=fs, write_scale=ws)
+    for k, e in out["kernels"].items():
+        e["traffic_bytes"] = 1024.0 * (e.get("FETCH_SIZE_kib_avg", 0.0) * fs +
+                                       e.get("WRITE_SIZE_kib_avg", 0.0) * ws)
+    with open(out_json, "w") as fh:
+        json.dump(out, fh, indent=1, sort_keys=True)
+    print(json.dumps(out, indent=1, sort_keys=True))
+
+
+def load_traffic(form):
+    """Bytes per launch of the form's fused kernel from the committed summary."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            kern = json.load(fh)["kernels"]
+    except (OSError, ValueError, KeyError):
+        return None
+    for name, e in kern.items():
+        if KERNEL_KEYS[form] in name:
+            return e.get("traffic_bytes")
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -84,7 +148,20 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-frames", type=int, default=1_000_000)
     ap.add_argument("--forms", default="plain,chosen")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="after timing, one device-to-device copy of this many MiB "
+                         "(reference kernel for rocprofv3 FETCH_SIZE/WRITE_SIZE scaling)")
+    ap.add_argument("--no-pcie", action="store_true",
+                    help="skip the PCIe-inclusive (host-buffer) sector timing")
+    ap.add_argument("--pmc-summary", nargs=3, metavar=("FETCH_DIR", "WRITE_DIR", "OUT_JSON"),
+                    help="summarise two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of "
+                         "'bench.py --copy-mib M' into OUT_JSON and exit")
     args = ap.parse_args()
+    if args.pmc_summary:
+        if args.copy_mib <= 0:
+            raise SystemExit("--pmc-summary needs --copy-mib M (the calibration copy size)")
+        pmc_summary(*args.pmc_summary, args.copy_mib)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (this build has no multi-GPU path); "
               f"--gpus {args.gpus} is not supported", file=sys.stderr)
@@ -158,13 +235,49 @@ def main():
               f"fused kernel {avg_launch_ms:.3f} ms/launch", file=sys.stderr, flush=True)
         plan.close()
 
+    if args.copy_mib > 0:
+        nb = args.copy_mib * 2 ** 20
+        src, dst = F.DeviceBuffer(nb), F.DeviceBuffer(nb)
+        dst.copy_from(src, nb, stream=stream.ptr)
+        stream.sync()
+        print(f"[bench] calibration copy: {args.copy_mib} MiB device to device", file=sys.stderr,
+              flush=True)
+        src.free()
+        dst.free()
+
+    main_form = "chosen" if "chosen" in results else next(iter(results))
+    h1 = h2 = None
+    if not (args.no_cpu_baseline and args.no_pcie):
+        nfc = min(args.cpu_frames, sectors[0])
+        h1 = raw1.download(np.zeros(nfc * bpf, dtype=np.uint8))
+        h2 = raw2.download(np.zeros(nfc * bpf, dtype=np.uint8))
+
+    pcie = None
+    if not args.no_pcie:
+        # host buffers in, spectra out: H2D of both stations' bytes + kernels + D2H
+        prec = F.PRECISION_CHOSEN if main_form == "chosen" else F.PRECISION_PLAIN_FP32
+        plan = F.Plan(N, bits=(BITS, BITS), levels=[LEVELS, LEVELS],
+                      shuffles=(None, shuffle_b), lsb=(False, True), band=(0, 0, N // 2, 0),
+                      df_hz=FS / N, products=F.PRODUCT_ACF | F.PRODUCT_XCF, precision=prec)
+        nfp = len(h1) // bpf
+        plan.sector_run(h1, h2, 0, nfp, cfg)
+        reps = 3
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            plan.sector_run(h1, h2, 0, nfp, cfg)
+        el = (time.perf_counter() - t0) / reps
+        pcie = {"value": round(nfp * N / el / 1e6, 1), "unit": "Msamples/s/station",
+                "sample": f"one {nfp}-frame sector from pageable host memory "
+                          f"({2 * nfp * bpf / 2 ** 20:.0f} MiB H2D), mean of {reps}"}
+        print(f"[bench] PCIe-inclusive ({main_form}): {pcie['value']:,.0f} Msamples/s/station",
+              file=sys.stderr, flush=True)
+        plan.close()
+
     cpu = None
     if not args.no_cpu_baseline:
         import fxref as R
 
-        nfc = min(args.cpu_frames, sectors[0])
-        h1 = raw1.download(np.zeros(nfc * bpf, dtype=np.uint8))
-        h2 = raw2.download(np.zeros(nfc * bpf, dtype=np.uint8))
+        nfc = len(h1) // bpf
         threads = max(1, min(os.cpu_count() or 1, 16))
         desc = dict(fft_len=N, bits1=BITS, bits2=BITS, levels1=LEVELS, levels2=LEVELS,
                     shuffle1=np.arange(32, dtype=np.int32), shuffle2=shuffle_b, lsb1=0, lsb2=1,
@@ -182,7 +295,6 @@ def main():
         print(f"[bench] cpu baseline: {cpu['value']:.1f} Msamples/s/station "
               f"({threads} threads, {el:.1f} s)", file=sys.stderr, flush=True)
 
-    main_form = "chosen" if "chosen" in results else next(iter(results))
     r = results[main_form]
     line = {
         "metric": "Msamples/s/station @2-bit FFT1024",
@@ -208,12 +320,14 @@ def main():
                   for k, v in results.items()},
         "roofline": {"bound": "mfma", "achieved": round(r["tflops"], 3),
                      "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(r["tflops"] / FP32_PEAK_TFLOPS, 4), "traffic": None,
+                     "frac": round(r["tflops"] / FP32_PEAK_TFLOPS, 4),
+                     "traffic": load_traffic(main_form),
                      "kernel": ("fused decode+shift+MFMA four-step FFT+MAC (k_sector_mf2)"
                                 if main_form == "chosen" else
                                 "fused decode+shift+FFT+MAC (k_sector<8,64,0>)"),
                      "hbm_frac": round(r["gbs"] / HBM_PEAK_GBS, 4)},
         "cpu_baseline": cpu,
+        "pcie_inclusive": pcie,
     }
     if "plain" in results and "chosen" in results:
         line["chosen_over_plain"] = round(results["chosen"]["value"] / results["plain"]["value"], 3)

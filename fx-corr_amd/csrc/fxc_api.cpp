@@ -208,6 +208,10 @@ extern "C" fxc_status fxc_memset_d(void *dst, int value, size_t bytes, void *str
     HIP_TRY(hipMemsetAsync(dst, value, bytes, (hipStream_t)stream));
     return FXC_OK;
 }
+extern "C" fxc_status fxc_memcpy_d2d(void *dst, const void *src, size_t bytes, void *stream) {
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+    return FXC_OK;
+}
 extern "C" fxc_status fxc_stream_create(void **stream) {
     if (!stream) return fail(FXC_ERR_INVALID_ARG, "stream is null");
     hipStream_t s;

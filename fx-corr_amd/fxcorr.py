@@ -23,7 +23,8 @@ N_PRODUCTS = 5  # [ph | 11 | 22 | 12re | 12im]
 
 EXPORTED = [
     "fxc_version", "fxc_strerror", "fxc_last_error", "fxc_device_count", "fxc_dev_alloc",
-    "fxc_dev_free", "fxc_memcpy_h2d", "fxc_memcpy_d2h", "fxc_memset_d", "fxc_stream_create",
+    "fxc_dev_free", "fxc_memcpy_h2d", "fxc_memcpy_d2h", "fxc_memset_d", "fxc_memcpy_d2d",
+    "fxc_stream_create",
     "fxc_stream_destroy", "fxc_stream_sync", "fxc_event_create", "fxc_event_destroy",
     "fxc_event_record", "fxc_event_elapsed_ms", "fxc_parse_shuffle", "fxc_parse_levels",
     "fxc_quantization_mean_power", "fxc_compute_band_alignment", "fxc_sector_partition",
@@ -105,6 +106,7 @@ def lib():
             "fxc_dev_alloc": ([sz, _P(vp)], i32), "fxc_dev_free": ([vp], i32),
             "fxc_memcpy_h2d": ([vp, vp, sz, vp], i32), "fxc_memcpy_d2h": ([vp, vp, sz, vp], i32),
             "fxc_memset_d": ([vp, C.c_int, sz, vp], i32),
+            "fxc_memcpy_d2d": ([vp, vp, sz, vp], i32),
             "fxc_stream_create": ([_P(vp)], i32), "fxc_stream_destroy": ([vp], i32),
             "fxc_stream_sync": ([vp], i32), "fxc_event_create": ([_P(vp)], i32),
             "fxc_event_destroy": ([vp], i32), "fxc_event_record": ([vp, vp], i32),
@@ -353,6 +355,12 @@ class DeviceBuffer:
         check(lib().fxc_memcpy_d2h(_vp(arr), C.c_void_p(self.ptr), arr.nbytes,
                                    C.c_void_p(stream or 0)))
         return arr
+
+    def copy_from(self, src: "DeviceBuffer", nbytes: int, stream=None):
+        if nbytes > min(self.nbytes, src.nbytes):
+            raise ValueError("copy larger than either buffer")
+        check(lib().fxc_memcpy_d2d(C.c_void_p(self.ptr), C.c_void_p(src.ptr), nbytes,
+                                   C.c_void_p(stream or 0)))
 
     def free(self):
         if self.ptr:

@@ -139,6 +139,8 @@ fxc_status fxc_dev_free(void *dptr);
 fxc_status fxc_memcpy_h2d(void *dst, const void *src, size_t bytes, void *stream);
 fxc_status fxc_memcpy_d2h(void *dst, const void *src, size_t bytes, void *stream);
 fxc_status fxc_memset_d(void *dst, int value, size_t bytes, void *stream);
+/* Device-to-device copy enqueued on `stream` (NULL: the null stream). */
+fxc_status fxc_memcpy_d2d(void *dst, const void *src, size_t bytes, void *stream);
 fxc_status fxc_stream_create(void **stream);
 fxc_status fxc_stream_destroy(void *stream);
 fxc_status fxc_stream_sync(void *stream);
