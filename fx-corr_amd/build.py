@@ -14,7 +14,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfxcorr.so")
 
-SOURCES = ["fxc_device.hip", "fxc_mfma.hip", "fxc_synth.hip", "fxc_api.cpp"]
+SOURCES = ["fxc_device.hip", "fxc_mfma.hip", "fxc_phased.hip", "fxc_synth.hip", "fxc_api.cpp",
+           "fxc_io.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
@@ -38,14 +39,19 @@ def _run(cmd):
 
 
 def build(verbose: bool = True) -> str:
+    from concurrent.futures import ThreadPoolExecutor
+
     t0 = time.time()
-    objs = []
+    objs, cmds = [], []
     for src in SOURCES:
         path = os.path.join(CSRC, src)
         obj = os.path.join(CSRC, src + ".o")
         lang = ["-x", "hip"] if src.endswith(".hip") else []
-        _run([HIPCC, *COMMON, *DEVICE_FLAGS, *lang, "-c", path, "-o", obj])
+        cmds.append([HIPCC, *COMMON, *DEVICE_FLAGS, *lang, "-c", path, "-o", obj])
         objs.append(obj)
+    with ThreadPoolExecutor(max_workers=min(len(cmds), os.cpu_count() or 1, 8)) as ex:
+        for fut in [ex.submit(_run, c) for c in cmds]:
+            fut.result()
     tmp = LIB + ".tmp"
     _run([HIPCC, *COMMON, "-shared", *objs, "-o", tmp])
     os.replace(tmp, LIB)

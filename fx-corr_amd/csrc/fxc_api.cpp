@@ -30,6 +30,13 @@ fxc_status fail(fxc_status code, const char *fmt, ...) {
     return code;
 }
 
+}  // namespace
+
+// Error recording shared with the host I/O unit (fxc_io.cpp).
+extern "C" fxc_status fxc_internal_fail(fxc_status s, const char *msg) { return fail(s, "%s", msg); }
+
+namespace {
+
 #define HIP_TRY(expr)                                                                    \
     do {                                                                                 \
         hipError_t e_ = (expr);                                                          \
@@ -144,7 +151,7 @@ struct fxc_plan {
     int fft_form = 0;  // 0: FP32 Stockham, 1: FP16/FP32-MFMA four-step (N = 1024)
     // scratch
     DevBuf part, red_scratch, table_dev, d_raw[2], d_acc, d_raw_out, d_tmp_in, d_tmp_out,
-        d_shifts;
+        d_shifts, d_cb, d_tbuf;
     int max_wg = 0;
     std::vector<double> table_host;  // last uploaded geometric table
     // timing
@@ -733,6 +740,7 @@ extern "C" fxc_status fxc_plan_destroy(fxc_plan *p) {
     DevBuf *bufs[] = {&p->tw_fft, &p->tw_post, &p->lev_dev[0], &p->lev_dev[1], &p->out_lev,
                       &p->part, &p->red_scratch, &p->table_dev, &p->d_raw[0], &p->d_raw[1],
                       &p->d_acc, &p->d_raw_out, &p->d_tmp_in, &p->d_tmp_out, &p->d_shifts,
+                      &p->d_cb, &p->d_tbuf,
                       &p->mf_tab};
     for (DevBuf *b : bufs) b->release();
     if (p->stream) (void)hipStreamDestroy(p->stream);
@@ -813,19 +821,69 @@ static fxc_status collect_timing(fxc_plan *p) {
 }
 
 // Enqueue the fused kernel + deterministic reduce for one sector.
+static int64_t plan_bpf_out(const fxc_plan *p) {
+    return ((int64_t)p->N * p->desc.bit_out + 7) / 8;
+}
+
+// Phased-array sector (main.rs:2222-2302, 2391-2411): the spectrum kernel over
+// chunks of frames that reuse the same per-workgroup partial slots, and per
+// chunk, when the .raw output is requested, the inverse FFT (with the LSB
+// flip) and the quantiser into d_raw_out.
+static fxc_status enqueue_phased(fxc_plan *p, const SectorArgs &base, int64_t nf,
+                                 uint8_t *d_raw_out, double *d_acc, hipStream_t s) {
+    const bool raw = (p->desc.products & FXC_PRODUCT_RAW_OUT) && d_raw_out != nullptr;
+    const size_t half = (size_t)p->half;
+    const int64_t bpf_o = plan_bpf_out(p);
+    int64_t chunk = nf;
+    if (raw) {
+        const int64_t cap = (int64_t)((size_t)256 << 20) / (int64_t)(half * 8 > (size_t)p->N * 4
+                                                                     ? half * 8 : (size_t)p->N * 4);
+        chunk = std::max<int64_t>(1, std::min<int64_t>(nf, cap));
+        if (p->d_cb.ensure((size_t)chunk * half * 8) != hipSuccess ||
+            p->d_tbuf.ensure((size_t)chunk * p->N * 4) != hipSuccess)
+            return fail(FXC_ERR_NOMEM, "phased scratch allocation failed");
+    }
+    int64_t fpw = (chunk + p->max_wg - 1) / p->max_wg;
+    fpw = ((fpw + kFrameBatch - 1) / kFrameBatch) * kFrameBatch;
+    if (fpw < kFrameBatch) fpw = kFrameBatch;
+    const int n_wg = (int)((chunk + fpw - 1) / fpw);
+    for (int64_t c0 = 0; c0 < nf; c0 += chunk) {
+        const int64_t cn = std::min(chunk, nf - c0);
+        SectorArgs a = base;
+        a.raw1 = base.raw1 + c0 * p->st[0].bpf;
+        a.raw2 = base.raw2 + c0 * p->st[1].bpf;
+        a.first_frame = base.first_frame + c0;
+        a.nf = cn;
+        a.frames_per_wg = fpw;
+        a.part = (double *)p->part.p;
+        a.cb_out = raw ? (float2 *)p->d_cb.p : nullptr;
+        HIP_TRY(launch_phased(a, c0 > 0 ? 1 : 0, (int)((cn + fpw - 1) / fpw), s));
+        if (raw) {
+            HIP_TRY(launch_irfft_frames(a, (const float2 *)p->d_cb.p, cn, (float *)p->d_tbuf.p,
+                                        p->desc.output_lsb != 0, s));
+            HIP_TRY(launch_quantise_frames(a, (const float *)p->d_tbuf.p, cn, d_raw_out + c0 * bpf_o, s));
+        }
+    }
+    HIP_TRY(launch_reduce((const double *)p->part.p, n_wg, p->half, d_acc,
+                          (double *)p->red_scratch.p, 64, s));
+    return FXC_OK;
+}
+
 static fxc_status enqueue_sector(fxc_plan *p, const uint8_t *d_raw1, const uint8_t *d_raw2,
                                  int64_t first_frame, int64_t nf, const fxc_delay_cfg &dc,
-                                 double *d_acc, hipStream_t s) {
+                                 double *d_acc, uint8_t *d_raw_out, hipStream_t s) {
     const double *dtab = nullptr;
     fxc_status st = upload_table(p, dc, s, &dtab);
     if (st != FXC_OK) return st;
     const size_t acc_bytes = (size_t)kNumProducts * p->half * sizeof(double);
+    const bool phased = (p->desc.products & (FXC_PRODUCT_PHASED | FXC_PRODUCT_RAW_OUT)) != 0;
     if (nf <= 0 || !p->decode_ok[0] || !p->decode_ok[1]) {
-        HIP_TRY(hipMemsetAsync(d_acc, 0, acc_bytes, s));
+        // every frame fails to decode: zero products, zero .raw frames (main.rs:2226-2233)
+        HIP_TRY(launch_fill_zero(d_acc, acc_bytes, s));
+        if (phased && d_raw_out && nf > 0)
+            HIP_TRY(launch_fill_zero(d_raw_out, (size_t)nf * (size_t)plan_bpf_out(p), s));
         return FXC_OK;
     }
-    if (p->desc.products & (FXC_PRODUCT_PHASED | FXC_PRODUCT_RAW_OUT))
-        return fail(FXC_ERR_UNSUPPORTED, "phased products are not available in this build");
     SectorArgs a;
     fill_args(p, a);
     fill_delay(dc, dtab, a.dly);
@@ -833,6 +891,7 @@ static fxc_status enqueue_sector(fxc_plan *p, const uint8_t *d_raw1, const uint8
     a.raw2 = d_raw2;
     a.first_frame = first_frame;
     a.nf = nf;
+    if (phased) return enqueue_phased(p, a, nf, d_raw_out, d_acc, s);
     int64_t fpw = (nf + p->max_wg - 1) / p->max_wg;
     const int batch = p->fft_form == 1 ? 8 : kFrameBatch;   // MFMA form: whole frames per wave
     fpw = ((fpw + batch - 1) / batch) * batch;
@@ -869,9 +928,8 @@ extern "C" fxc_status fxc_sector_run_device(fxc_plan *p, const uint8_t *d_raw1,
     if (!p || !delay || !d_acc) return fail(FXC_ERR_INVALID_ARG, "null argument");
     if (nf < 0) return fail(FXC_ERR_INVALID_ARG, "nf < 0");
     if (nf > 0 && (!d_raw1 || !d_raw2)) return fail(FXC_ERR_INVALID_ARG, "null raw buffer");
-    (void)d_raw_out;
     hipStream_t s = (hipStream_t)stream;  // NULL: the HIP null stream
-    return enqueue_sector(p, d_raw1, d_raw2, first_frame, nf, *delay, d_acc, s);
+    return enqueue_sector(p, d_raw1, d_raw2, first_frame, nf, *delay, d_acc, d_raw_out, s);
 }
 
 extern "C" fxc_status fxc_sector_run(fxc_plan *p, const uint8_t *raw1, size_t len1,
@@ -885,10 +943,13 @@ extern "C" fxc_status fxc_sector_run(fxc_plan *p, const uint8_t *raw1, size_t le
     if (nf > 0 && (!raw1 || !raw2 || len1 < need1 || len2 < need2))
         return fail(FXC_ERR_INVALID_ARG, "raw buffers shorter than nf*bpf (%zu/%zu < %zu/%zu)", len1,
                     len2, need1, need2);
-    (void)raw_out;
     hipStream_t s = p->stream;
     const size_t acc_bytes = (size_t)kNumProducts * p->half * sizeof(double);
     if (p->d_acc.ensure(acc_bytes) != hipSuccess) return fail(FXC_ERR_NOMEM, "device allocation failed");
+    const bool want_raw = raw_out && (p->desc.products & FXC_PRODUCT_RAW_OUT) && nf > 0;
+    const size_t raw_bytes = want_raw ? (size_t)nf * (size_t)plan_bpf_out(p) : 0;
+    if (want_raw && p->d_raw_out.ensure(raw_bytes) != hipSuccess)
+        return fail(FXC_ERR_NOMEM, "device allocation of %zu bytes failed", raw_bytes);
     if (nf > 0) {
         if (p->d_raw[0].ensure(need1) != hipSuccess || p->d_raw[1].ensure(need2) != hipSuccess)
             return fail(FXC_ERR_NOMEM, "device allocation of %zu bytes failed", need1 + need2);
@@ -896,10 +957,12 @@ extern "C" fxc_status fxc_sector_run(fxc_plan *p, const uint8_t *raw1, size_t le
         HIP_TRY(hipMemcpyAsync(p->d_raw[1].p, raw2, need2, hipMemcpyHostToDevice, s));
     }
     fxc_status st = enqueue_sector(p, (const uint8_t *)p->d_raw[0].p, (const uint8_t *)p->d_raw[1].p,
-                                   first_frame, nf, *delay, (double *)p->d_acc.p, s);
+                                   first_frame, nf, *delay, (double *)p->d_acc.p,
+                                   want_raw ? (uint8_t *)p->d_raw_out.p : nullptr, s);
     if (st != FXC_OK) return st;
     std::vector<double> host((size_t)kNumProducts * p->half);
     HIP_TRY(hipMemcpyAsync(host.data(), p->d_acc.p, acc_bytes, hipMemcpyDeviceToHost, s));
+    if (want_raw) HIP_TRY(hipMemcpyAsync(raw_out, p->d_raw_out.p, raw_bytes, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     if (p->timing) {
         fxc_status ts = collect_timing(p);
@@ -987,7 +1050,8 @@ static fxc_status run_fft(fxc_plan *p, const float *in, int64_t nframes, float *
     SectorArgs a;
     fill_args(p, a);
     if (inverse)
-        HIP_TRY(launch_irfft_frames(a, (const float2 *)p->d_tmp_in.p, nframes, (float *)p->d_tmp_out.p, s));
+        HIP_TRY(launch_irfft_frames(a, (const float2 *)p->d_tmp_in.p, nframes, (float *)p->d_tmp_out.p,
+                                    false, s));
     else if (p->fft_form == 1)
         HIP_TRY(launch_rfft_mfma(a, (const float *)p->d_tmp_in.p, nframes, (float2 *)p->d_tmp_out.p, s));
     else

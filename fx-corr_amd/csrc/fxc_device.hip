@@ -305,7 +305,8 @@ __device__ __forceinline__ void irfft_prepare(const float2 *X, int M, const floa
 
 template <int E, int T>
 __global__ __launch_bounds__(2 * T) void k_irfft_frames(SectorArgs a, const float2 *in,
-                                                        int64_t nframes, float *out) {
+                                                        int64_t nframes, float *out,
+                                                        int negate_odd) {
     constexpr int M = E * T;
     constexpr int MP = M + M / 16 + 2;
     __shared__ float2 zbuf[2][MP];
@@ -319,21 +320,25 @@ __global__ __launch_bounds__(2 * T) void k_irfft_frames(SectorArgs a, const floa
         for (int r = 0; r < E; ++r) v[r] = make_float2(0.f, 0.f);
     fft_forward<E, T>(zbuf[s], v, a.tw_fft, t);
     const float scale = 1.0f / (float)(2 * M);
+    // odd samples are -Im of the conjugated transform, negated once more for
+    // an LSB output stream (main.rs:2290-2292)
+    const float odd_sign = negate_odd ? 1.0f : -1.0f;
     if (live)
         for (int m = t; m < M; m += T) {
             float2 z = zbuf[s][pad(m)];
             out[f * 2 * M + 2 * m] = z.x * scale;
-            out[f * 2 * M + 2 * m + 1] = -z.y * scale;
+            out[f * 2 * M + 2 * m + 1] = odd_sign * (z.y * scale);
         }
 }
 
+// mode 0: forward R2C; 1: inverse C2R; 2: inverse with odd samples negated.
 template <int E, int T>
 static hipError_t launch_fft_t(const SectorArgs &a, const void *in, int64_t nframes, void *out,
-                               bool inverse, hipStream_t s) {
+                               int mode, hipStream_t s) {
     dim3 grid((unsigned)((nframes + 1) / 2));
-    if (inverse)
+    if (mode != 0)
         hipLaunchKernelGGL((k_irfft_frames<E, T>), grid, dim3(2 * T), 0, s, a, (const float2 *)in,
-                           nframes, (float *)out);
+                           nframes, (float *)out, mode == 2 ? 1 : 0);
     else
         hipLaunchKernelGGL((k_rfft_frames<E, T>), grid, dim3(2 * T), 0, s, a, (const float *)in,
                            nframes, (float2 *)out);
@@ -341,29 +346,29 @@ static hipError_t launch_fft_t(const SectorArgs &a, const void *in, int64_t nfra
 }
 
 static hipError_t launch_fft(const SectorArgs &a, const void *in, int64_t nframes, void *out,
-                             bool inverse, hipStream_t s) {
+                             int mode, hipStream_t s) {
     if (nframes <= 0) return hipSuccess;
     switch (a.N) {
-    case 64: return launch_fft_t<2, 16>(a, in, nframes, out, inverse, s);
-    case 128: return launch_fft_t<2, 32>(a, in, nframes, out, inverse, s);
-    case 256: return launch_fft_t<2, 64>(a, in, nframes, out, inverse, s);
-    case 512: return launch_fft_t<4, 64>(a, in, nframes, out, inverse, s);
-    case 1024: return launch_fft_t<8, 64>(a, in, nframes, out, inverse, s);
-    case 2048: return launch_fft_t<8, 128>(a, in, nframes, out, inverse, s);
-    case 4096: return launch_fft_t<8, 256>(a, in, nframes, out, inverse, s);
-    case 8192: return launch_fft_t<8, 512>(a, in, nframes, out, inverse, s);
-    case 16384: return launch_fft_t<16, 512>(a, in, nframes, out, inverse, s);
+    case 64: return launch_fft_t<2, 16>(a, in, nframes, out, mode, s);
+    case 128: return launch_fft_t<2, 32>(a, in, nframes, out, mode, s);
+    case 256: return launch_fft_t<2, 64>(a, in, nframes, out, mode, s);
+    case 512: return launch_fft_t<4, 64>(a, in, nframes, out, mode, s);
+    case 1024: return launch_fft_t<8, 64>(a, in, nframes, out, mode, s);
+    case 2048: return launch_fft_t<8, 128>(a, in, nframes, out, mode, s);
+    case 4096: return launch_fft_t<8, 256>(a, in, nframes, out, mode, s);
+    case 8192: return launch_fft_t<8, 512>(a, in, nframes, out, mode, s);
+    case 16384: return launch_fft_t<16, 512>(a, in, nframes, out, mode, s);
     default: return hipErrorInvalidValue;
     }
 }
 
 hipError_t launch_rfft_frames(const SectorArgs &a, const float *in, int64_t nframes, float2 *out,
                               hipStream_t s) {
-    return launch_fft(a, in, nframes, out, false, s);
+    return launch_fft(a, in, nframes, out, 0, s);
 }
 hipError_t launch_irfft_frames(const SectorArgs &a, const float2 *in, int64_t nframes, float *out,
-                               hipStream_t s) {
-    return launch_fft(a, in, nframes, out, true, s);
+                               bool negate_odd, hipStream_t s) {
+    return launch_fft(a, in, nframes, out, negate_odd ? 2 : 1, s);
 }
 
 // Delay entries, one thread per frame.

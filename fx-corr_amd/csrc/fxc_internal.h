@@ -76,6 +76,7 @@ struct SectorArgs {
     uint8_t *raw_out;        // phased .raw frames (nf * bpf_out) or null
     double *part;            // [n_wg][kNumProducts][half]
     const void *mf_tab;      // chosen-form MFMA tables (N = 1024), layout MFT_* below
+    float2 *cb_out;          // phased: combined half spectra [frame][half] for the .raw output
 };
 
 // Chosen-form MFMA tables (fxc_mfma.hip), byte offsets into one device
@@ -103,7 +104,8 @@ hipError_t launch_decode_frames(const SectorArgs &a, int station,
 hipError_t launch_rfft_frames(const SectorArgs &a, const float *in, int64_t nframes,
                               float2 *out, hipStream_t stream);
 hipError_t launch_irfft_frames(const SectorArgs &a, const float2 *in,
-                               int64_t nframes, float *out, hipStream_t stream);
+                               int64_t nframes, float *out, bool negate_odd,
+                               hipStream_t stream);
 hipError_t launch_delay_eval(const DelayDev &d, int64_t first, int64_t n,
                              FrameDelayDev *out, hipStream_t stream);
 hipError_t launch_quantise_frames(const SectorArgs &a, const float *in,
@@ -112,6 +114,10 @@ hipError_t launch_quantise_frames(const SectorArgs &a, const float *in,
 
 hipError_t launch_sector_mfma(const SectorArgs &a, int precision, int n_wg, hipStream_t stream);
 int mfma_wg_threads();
+
+// fxc_phased.hip
+hipError_t launch_phased(const SectorArgs &a, int accumulate, int n_wg, hipStream_t stream);
+hipError_t launch_fill_zero(void *p, size_t bytes, hipStream_t stream);
 hipError_t launch_rfft_mfma(const SectorArgs &a, const float *in, int64_t nframes, float2 *out,
                             hipStream_t stream);
 

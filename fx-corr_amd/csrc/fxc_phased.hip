@@ -1,0 +1,217 @@
+// Phased-array sum of the two stations (yi-phasedarray; main.rs:2222-2302,
+// 2391-2411): per frame, decode + integer shift + real FFT of both stations
+// (the fused sector kernel's front end), each station's fractional-delay
+// rotation (utils.rs:69-124 with rate = accel = t = 0, USB sign), fringe
+// phasors, band alignment onto the output grid, cb = w1 s1 + w2 s2, |cb|^2
+// accumulated in FP64 and, for the .raw output, cb with DC/Nyquist imaginary
+// parts zeroed written out for the inverse FFT + quantise kernels.
+#include "fxc_dev.h"
+
+namespace fxc {
+
+struct PhasedParam {
+    int32_t s1, s2;        // integer shifts clamped to [-N, N]
+    float2 fr1, fr2;       // fringe phasors (main.rs:235-240, f32)
+    cd pw[2][kMaxPow];     // per station: rotation step (the reference's f32 value) ^ 2^b
+};
+
+// The reference rotates bin b by rot_b with rot_0 = 1 and rot_{b+1} =
+// rot_b * step in f32, step = from_polar(1, (-2 pi df frac) as f32).  The
+// powers step^b are evaluated here in FP64 from that same f32 step, which
+// keeps the recurrence's systematic drift (|step| != 1 in f32) and leaves out
+// only its per-bin rounding walk.
+__device__ void make_phased_param(int64_t idx, const SectorArgs &a, PhasedParam &p) {
+    FrameDelayDev d;
+    compute_frame_delay(idx, a.dly, d, nullptr, nullptr);
+    p.s1 = clamp_shift(d.int1, a.N);
+    p.s2 = clamp_shift(d.int2, a.N);
+    p.fr1 = make_float2(d.fr1[0], d.fr1[1]);
+    p.fr2 = make_float2(d.fr2[0], d.fr2[1]);
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        double phase_step;
+        {
+#pragma clang fp contract(off)
+            const double total = s ? d.frac2 : d.frac1;      // + rate*0 + 0.5*accel*0^2
+            phase_step = -1.0 * 2.0 * FXC_PI * a.df * total;  // USB sign (utils.rs:111-112)
+        }
+        const float2 st = polar1_f32((float)phase_step);
+        p.pw[s][0] = cd{(double)st.x, (double)st.y};
+        for (int b = 1; b < kMaxPow; ++b) p.pw[s][b] = d_mul(p.pw[s][b - 1], p.pw[s][b - 1]);
+    }
+}
+
+// step^e for 0 <= e < 2^kMaxPow.
+__device__ __forceinline__ cd pow_step(const cd *pw, int e) {
+    cd r = cd{1.0, 0.0};
+#pragma unroll 1
+    for (int b = 0; b < kMaxPow; ++b)
+        if ((e >> b) & 1) r = d_mul(r, pw[b]);
+    return r;
+}
+
+__device__ __forceinline__ float2 c_scale(float2 a, float w) { return make_float2(a.x * w, a.y * w); }
+
+// accumulate = 0: the workgroup's partial slot is set; 1: added to (the
+// sector is processed in chunks that reuse the same slots in order).
+template <int E, int T>
+__global__ __launch_bounds__(2 * T) void k_phased(SectorArgs a, int accumulate) {
+    constexpr int M = E * T;
+    constexpr int MP = M + M / 16 + 2;
+    constexpr int NT = 2 * T;
+    constexpr int NB = (M + NT) / NT;
+    constexpr int LOGNT = __builtin_ctz(NT);
+    static_assert(__builtin_ctz(M) < kMaxPow, "rotation power table too small");
+    __shared__ float2 zbuf[2][MP];
+    __shared__ PhasedParam pp[kFrameBatch];
+    __shared__ float lev[2][256];
+
+    const int tid = threadIdx.x;
+    const int s = half_index<T>(tid);
+    const int t = tid - s * T;
+    const int N = 2 * M;
+    const int64_t f_begin = (int64_t)blockIdx.x * a.frames_per_wg;
+    int64_t f_end = f_begin + a.frames_per_wg;
+    if (f_end > a.nf) f_end = a.nf;
+    const StationDev &st = a.st[s];
+    const uint8_t *raw = s ? a.raw2 : a.raw1;
+    if (st.bits > 2 && st.bits <= 8)
+        for (int i = t; i < 256; i += T) lev[s][i] = (i <= (int)st.code_mask) ? st.levels[i] : 0.0f;
+
+    double ph[NB];
+#pragma unroll
+    for (int i = 0; i < NB; ++i) ph[i] = 0.0;
+
+    const int gsn = a.grid;                          // station on the output grid
+    const int gs = (gsn == 0) ? a.a1s : a.a2s;
+    const int ge = gs + a.ov;
+    const int sb_other = (gsn == 0) ? (a.a2s - a.a1s) : (a.a1s - a.a2s);
+    const float w1 = (float)a.w1, w2 = (float)a.w2;
+    const bool write_cb = (a.flags & FLAG_RAW_OUT) != 0 && a.cb_out != nullptr;
+    float2 *zs = zbuf[s];
+    uint32_t *ws = reinterpret_cast<uint32_t *>(zbuf[s]);
+    const float2 *zg = zbuf[gsn];
+    const float2 *zo = zbuf[1 - gsn];
+    __syncthreads();
+
+    for (int64_t fb = f_begin; fb < f_end; fb += kFrameBatch) {
+        const int nb = (int)((f_end - fb) < kFrameBatch ? (f_end - fb) : kFrameBatch);
+        if (tid < nb) make_phased_param(a.first_frame + fb + tid, a, pp[tid]);
+        __syncthreads();
+        for (int jf = 0; jf < nb; ++jf) {
+            const int64_t f = fb + jf;
+            stage_words<T>(raw + f * st.bpf, st, ws, t);
+            __syncthreads();
+            const int sh = s ? pp[jf].s2 : pp[jf].s1;
+            float2 v[E];
+#pragma unroll
+            for (int r = 0; r < E; ++r) v[r] = decode_elem(ws, st, lev[s], t + T * r, sh, N);
+            __syncthreads();
+            fft_forward<E, T>(zs, v, a.tw_fft, t);
+
+            const PhasedParam &P = pp[jf];
+            const float2 frg = gsn == 0 ? P.fr1 : P.fr2;
+            const float2 fro = gsn == 0 ? P.fr2 : P.fr1;
+            // rotation of grid bin g: step_g^g; of the other station's bin
+            // j = g + sb: step_o^j, started exactly at this thread's first
+            // overlap bin (the overlap bins of a thread are consecutive i)
+            cd Rg = pow_step(P.pw[gsn], tid);
+            cd Ro = cd{1.0, 0.0};
+            bool have_o = false;
+            const cd Sg = P.pw[gsn][LOGNT], So = P.pw[1 - gsn][LOGNT];
+#pragma unroll
+            for (int i = 0; i < NB; ++i) {
+                const int g = tid + NT * i;
+                if (g <= M) {
+                    float2 sg = c_mul(untangle(zg, g, M, a.tw_post), make_float2((float)Rg.x, (float)Rg.y));
+                    if (g == M) sg.y = 0.0f;                  // Nyquist (utils.rs:118-123)
+                    const float2 cg = c_mul(sg, frg);
+                    float2 co = make_float2(0.0f, 0.0f);
+                    if (g >= gs && g < ge) {
+                        const int j = g + sb_other;
+                        Ro = have_o ? d_mul(Ro, So) : pow_step(P.pw[1 - gsn], j);
+                        have_o = true;
+                        float2 so = c_mul(untangle(zo, j, M, a.tw_post),
+                                          make_float2((float)Ro.x, (float)Ro.y));
+                        if (j == M) so.y = 0.0f;
+                        co = c_mul(so, fro);
+                    }
+                    const float2 c1 = gsn == 0 ? cg : co, c2 = gsn == 0 ? co : cg;
+                    float2 cb;
+                    {
+#pragma clang fp contract(off)
+                        const float2 a1 = c_scale(c1, w1), a2 = c_scale(c2, w2);
+                        cb = make_float2(a1.x + a2.x, a1.y + a2.y);
+                    }
+                    ph[i] += (double)norm_sqr(cb);
+                    if (write_cb) {
+                        if (g == 0 || g == M) cb.y = 0.0f;     // main.rs:2283-2284
+                        a.cb_out[f * (M + 1) + g] = cb;
+                    }
+                }
+                Rg = d_mul(Rg, Sg);
+            }
+            __syncthreads();
+        }
+    }
+
+    const int half = M + 1;
+    double *out = a.part + (size_t)blockIdx.x * kNumProducts * half;
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+        const int g = tid + NT * i;
+        if (g <= M) {
+            out[P_PH * half + g] = accumulate ? out[P_PH * half + g] + ph[i] : ph[i];
+            out[P_11 * half + g] = 0.0;
+            out[P_22 * half + g] = 0.0;
+            out[P_12R * half + g] = 0.0;
+            out[P_12I * half + g] = 0.0;
+        }
+    }
+}
+
+template <int E, int T>
+static hipError_t launch_phased_t(const SectorArgs &a, int accumulate, int n_wg, hipStream_t s) {
+    hipLaunchKernelGGL((k_phased<E, T>), dim3(n_wg), dim3(2 * T), 0, s, a, accumulate);
+    return hipGetLastError();
+}
+
+hipError_t launch_phased(const SectorArgs &a, int accumulate, int n_wg, hipStream_t s) {
+    if (n_wg <= 0) return hipSuccess;
+    switch (a.N) {
+    case 64: return launch_phased_t<2, 16>(a, accumulate, n_wg, s);
+    case 128: return launch_phased_t<2, 32>(a, accumulate, n_wg, s);
+    case 256: return launch_phased_t<2, 64>(a, accumulate, n_wg, s);
+    case 512: return launch_phased_t<4, 64>(a, accumulate, n_wg, s);
+    case 1024: return launch_phased_t<8, 64>(a, accumulate, n_wg, s);
+    case 2048: return launch_phased_t<8, 128>(a, accumulate, n_wg, s);
+    case 4096: return launch_phased_t<8, 256>(a, accumulate, n_wg, s);
+    case 8192: return launch_phased_t<8, 512>(a, accumulate, n_wg, s);
+    case 16384: return launch_phased_t<16, 512>(a, accumulate, n_wg, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+// Graph-safe zero fill (a kernel, not a memset node).
+__global__ void k_fill_zero(uint32_t *p, size_t nwords, uint8_t *tail, size_t ntail) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t k = i; k < nwords; k += stride) p[k] = 0u;
+    for (size_t k = i; k < ntail; k += stride) tail[k] = 0;
+}
+
+hipError_t launch_fill_zero(void *p, size_t bytes, hipStream_t s) {
+    if (bytes == 0) return hipSuccess;
+    const size_t nwords = ((uintptr_t)p % 4 == 0) ? bytes / 4 : 0;
+    uint8_t *tail = (uint8_t *)p + nwords * 4;
+    const size_t ntail = bytes - nwords * 4;
+    size_t blocks = (nwords + 255) / 256;
+    if (blocks < (ntail + 255) / 256) blocks = (ntail + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)blocks), dim3(256), 0, s, (uint32_t *)p, nwords,
+                       tail, ntail);
+    return hipGetLastError();
+}
+
+}  // namespace fxc

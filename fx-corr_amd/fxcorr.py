@@ -31,7 +31,10 @@ EXPORTED = [
     "fxc_split_delay", "fxc_plan_create", "fxc_plan_destroy", "fxc_plan_get_info",
     "fxc_sector_run", "fxc_sector_run_device", "fxc_plan_set_timing", "fxc_plan_get_timing",
     "fxc_decode_frames", "fxc_rfft_frames", "fxc_irfft_frames", "fxc_delay_eval",
-    "fxc_quantise_frames", "fxc_synth_station_device",
+    "fxc_quantise_frames", "fxc_synth_station_device", "fxc_reader_open", "fxc_reader_read",
+    "fxc_reader_close", "fxc_word_aligned_start", "fxc_normalize_sector",
+    "fxc_cor_build_header", "fxc_cor_build_sector_header", "fxc_cor_create",
+    "fxc_cor_write_sector", "fxc_cor_finalize",
 ]
 
 
@@ -88,6 +91,28 @@ class PlanInfo(C.Structure):
                 ("threads_per_frame", C.c_int32), ("device_id", C.c_int32)]
 
 
+class CorStation(C.Structure):
+    _fields_ = [("name", C.c_char_p), ("code", C.c_uint8), ("ecef_m", C.c_double * 3)]
+
+
+class CorHeader(C.Structure):
+    _fields_ = [("sampling_speed_hz", C.c_int32), ("observing_frequency_hz", C.c_double),
+                ("fft_point", C.c_int32), ("number_of_sector_hint", C.c_int32),
+                ("clock_reference_unix_sec", C.c_int64), ("source_name", C.c_char_p),
+                ("source_ra_rad", C.c_double), ("source_dec_rad", C.c_double)]
+
+
+class CorClockModel(C.Structure):
+    _fields_ = [("sec", C.c_uint32), ("nsec", C.c_uint32), ("delay", C.c_double),
+                ("rate", C.c_double), ("acel", C.c_double), ("jerk", C.c_double),
+                ("snap", C.c_double)]
+
+
+class CorSectorModel(C.Structure):
+    _fields_ = [("station1", CorClockModel), ("station2", CorClockModel),
+                ("amp", C.c_float * 2), ("phs", C.c_int16 * 2)]
+
+
 def _P(t):
     return C.POINTER(t)
 
@@ -132,6 +157,18 @@ def lib():
             "fxc_quantise_frames": ([vp, vp, i64, vp], i32),
             "fxc_synth_station_device": ([vp, i64, i32, _P(i32), i32, C.c_uint64, i32, d, i64,
                                           vp], i32),
+            "fxc_reader_open": ([C.c_char_p, C.c_uint64, i32, _P(vp)], i32),
+            "fxc_reader_read": ([vp, vp, sz, _P(sz)], i32),
+            "fxc_reader_close": ([vp], i32),
+            "fxc_word_aligned_start": ([C.c_uint64, i32, _P(C.c_uint64), _P(C.c_uint64),
+                                        _P(i64)], i32),
+            "fxc_normalize_sector": ([vp, i32, i64, d, d, d, d, vp, vp, vp, vp], i32),
+            "fxc_cor_build_header": ([_P(CorHeader), _P(CorStation), _P(CorStation), vp], i32),
+            "fxc_cor_build_sector_header": ([i64, C.c_float, _P(CorSectorModel), vp], i32),
+            "fxc_cor_create": ([C.c_char_p, _P(CorHeader), _P(CorStation), _P(CorStation),
+                                _P(vp)], i32),
+            "fxc_cor_write_sector": ([vp, i64, C.c_float, vp, i64, _P(CorSectorModel)], i32),
+            "fxc_cor_finalize": ([vp], i32),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -410,3 +447,101 @@ def synth_station_device(d_out: int, nsamples: int, bits: int, shuffle=None, lsb
     sh = (C.c_int32 * 32)(*[int(x) for x in (IDENTITY_SHUFFLE if shuffle is None else shuffle)])
     check(lib().fxc_synth_station_device(C.c_void_p(d_out), nsamples, bits, sh, int(lsb), seed,
                                          station, rho, delay_samples, C.c_void_p(stream or 0)))
+
+
+# ------------------------------------------------------------- host I/O
+class Reader:
+    """PackedSampleReader (main.rs:458-516) over the C ABI."""
+
+    def __init__(self, path: str, start_byte: int = 0, bit_offset: int = 0):
+        h = C.c_void_p()
+        check(lib().fxc_reader_open(os.fsencode(path), start_byte, bit_offset, C.byref(h)))
+        self.h = h
+
+    def read(self, n: int):
+        out = np.zeros(n, dtype=np.uint8)
+        got = C.c_size_t()
+        check(lib().fxc_reader_read(self.h, _vp(out), n, C.byref(got)))
+        return out, got.value
+
+    def close(self):
+        if self.h:
+            lib().fxc_reader_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def word_aligned_start(start_samples: int, bits: int):
+    """(actual_samples, start_byte, residual_samples), main.rs:1585-1600."""
+    a, b, r = C.c_uint64(), C.c_uint64(), C.c_int64()
+    check(lib().fxc_word_aligned_start(start_samples, bits, C.byref(a), C.byref(b), C.byref(r)))
+    return a.value, b.value, r.value
+
+
+def normalize_sector(acc, fft_len, nf, p1, p2, w1=1.0, w2=1.0):
+    """Normalised .cor spectra (main.rs:2645-2683) from a [5][half] accumulator."""
+    a = np.ascontiguousarray(acc, dtype=np.float64).reshape(-1)
+    outs = [np.zeros(fft_len, dtype=np.float32) for _ in range(4)]
+    check(lib().fxc_normalize_sector(_vp(a), fft_len, nf, p1, p2, w1, w2,
+                                     *[_vp(o) for o in outs]))
+    return {k: o.view(np.complex64) for k, o in zip(("ph", "11", "22", "12"), outs)}
+
+
+def cor_station(name: str, code: int, ecef):
+    s = CorStation()
+    s.name = name.encode()
+    s.code = code
+    s.ecef_m[:] = [float(v) for v in ecef]
+    return s
+
+
+def cor_header(fs_hz, obs_hz, fft_point, nsector_hint, clock_ref_unix, source, ra, dec):
+    h = CorHeader()
+    h.sampling_speed_hz = int(fs_hz)
+    h.observing_frequency_hz = float(obs_hz)
+    h.fft_point = fft_point
+    h.number_of_sector_hint = nsector_hint
+    h.clock_reference_unix_sec = clock_ref_unix
+    h.source_name = source.encode()
+    h.source_ra_rad = ra
+    h.source_dec_rad = dec
+    return h
+
+
+def cor_build_header(hdr: CorHeader, st1: CorStation, st2: CorStation) -> bytes:
+    out = np.zeros(256, dtype=np.uint8)
+    check(lib().fxc_cor_build_header(C.byref(hdr), C.byref(st1), C.byref(st2), _vp(out)))
+    return out.tobytes()
+
+
+def cor_build_sector_header(unix_sec: int, integ_s: float, model=None) -> bytes:
+    out = np.zeros(128, dtype=np.uint8)
+    check(lib().fxc_cor_build_sector_header(unix_sec, integ_s,
+                                            C.byref(model) if model is not None else None,
+                                            _vp(out)))
+    return out.tobytes()
+
+
+class CorWriter:
+    """CorWriter (cor.rs:206-296) over the C ABI."""
+
+    def __init__(self, path: str, hdr: CorHeader, st1: CorStation, st2: CorStation):
+        h = C.c_void_p()
+        check(lib().fxc_cor_create(os.fsencode(path), C.byref(hdr), C.byref(st1),
+                                   C.byref(st2), C.byref(h)))
+        self.h = h
+
+    def write_sector(self, unix_sec: int, integ_s: float, spectrum, model=None):
+        s = np.ascontiguousarray(spectrum, dtype=np.complex64)
+        check(lib().fxc_cor_write_sector(self.h, unix_sec, integ_s, _vp(s), s.size,
+                                         C.byref(model) if model is not None else None))
+
+    def finalize(self):
+        h, self.h = self.h, None
+        if h:
+            check(lib().fxc_cor_finalize(h))

@@ -258,6 +258,81 @@ fxc_status fxc_synth_station_device(uint8_t *d_out, int64_t nsamples, int32_t bi
                                     uint64_t seed, int32_t station, double rho,
                                     int64_t delay_samples, void *stream);
 
+/* ----------------------------------------------------------------------- */
+/* Input: packed-sample reader (replaces PackedSampleReader, main.rs:458-516, */
+/* and the word-aligned seek of main.rs:1578-1616).                          */
+/* ----------------------------------------------------------------------- */
+typedef struct fxc_reader fxc_reader;
+
+/* PackedSampleReader::open (main.rs:465-482): seek to start_byte; with a
+ * non-zero bit_offset (0..7) the first byte is held back for re-alignment. */
+fxc_status fxc_reader_open(const char *path, uint64_t start_byte, int32_t bit_offset,
+                           fxc_reader **out);
+/* read_packed_with_padding (main.rs:484-499, read_with_padding 445-456):
+ * fills len bytes, zero-padded past end of file; *got = bytes actually read
+ * from the file (may be NULL). */
+fxc_status fxc_reader_read(fxc_reader *r, uint8_t *out, size_t len, size_t *got);
+fxc_status fxc_reader_close(fxc_reader *r);
+/* main.rs:1585-1600: start sample floored to whole 32-bit words of `bits`
+ * samples; start byte = actual*bits/8; residual = actual - requested (<= 0). */
+fxc_status fxc_word_aligned_start(uint64_t start_samples, int32_t bits,
+                                  uint64_t *actual_samples, uint64_t *start_byte,
+                                  int64_t *residual_samples);
+
+/* ----------------------------------------------------------------------- */
+/* Output: sector normalisation (main.rs:2645-2683) and the .cor writer     */
+/* (CorWriter, cor.rs:152-296).                                             */
+/* ----------------------------------------------------------------------- */
+/* acc in the [ph | 11 | 22 | 12re | 12im] x (fft_len/2+1) layout of
+ * fxc_sector_run_device; each output (may be NULL) gets fft_len/2 complex
+ * f32 (re, im) values: ph / (0.5 max(w1^2 P1 + w2^2 P2, 1e-12) nf N^2),
+ * 11 / (0.5 P1 nf N^2), 22 / (0.5 P2 nf N^2), 12 / (0.5 sqrt(P1 P2) nf N^2). */
+fxc_status fxc_normalize_sector(const double *acc, int32_t fft_len, int64_t nf,
+                                double level_power1, double level_power2, double w1, double w2,
+                                float *out_ph, float *out11, float *out22, float *out12);
+
+typedef struct {            /* CorStation (cor.rs:26-31) */
+    const char *name;       /* first 16 bytes used */
+    uint8_t code;
+    double ecef_m[3];
+} fxc_cor_station;
+typedef struct {            /* CorHeaderConfig (cor.rs:33-43) */
+    int32_t sampling_speed_hz;
+    double observing_frequency_hz;
+    int32_t fft_point;
+    int32_t number_of_sector_hint;
+    int64_t clock_reference_unix_sec;
+    const char *source_name;
+    double source_ra_rad, source_dec_rad;
+} fxc_cor_header;
+typedef struct {            /* CorClockModel (cor.rs:45-54) */
+    uint32_t sec, nsec;
+    double delay, rate, acel, jerk, snap;
+} fxc_cor_clock_model;
+typedef struct {            /* CorSectorModel (cor.rs:56-62) */
+    fxc_cor_clock_model station1, station2;
+    float amp[2];
+    int16_t phs[2];
+} fxc_cor_sector_model;
+typedef struct fxc_cor_writer fxc_cor_writer;
+
+/* build_file_header (cor.rs:153-203): the 256-byte file header. */
+fxc_status fxc_cor_build_header(const fxc_cor_header *cfg, const fxc_cor_station *st1,
+                                const fxc_cor_station *st2, uint8_t out[256]);
+/* The 128-byte sector header of write_sector_with_model (cor.rs:243-276);
+ * model may be NULL. */
+fxc_status fxc_cor_build_sector_header(int64_t unix_sec, float integ_s,
+                                       const fxc_cor_sector_model *model, uint8_t out[128]);
+/* CorWriter::create / write_sector_with_model / finalize (cor.rs:206-296):
+ * spectrum = fft_point/2 complex f32 (re, im) little-endian. */
+fxc_status fxc_cor_create(const char *path, const fxc_cor_header *cfg,
+                          const fxc_cor_station *st1, const fxc_cor_station *st2,
+                          fxc_cor_writer **out);
+fxc_status fxc_cor_write_sector(fxc_cor_writer *w, int64_t unix_sec, float integ_s,
+                                const float *spectrum, int64_t n_complex,
+                                const fxc_cor_sector_model *model);
+fxc_status fxc_cor_finalize(fxc_cor_writer *w);
+
 #ifdef __cplusplus
 }
 #endif

@@ -8,7 +8,8 @@ import pytest
 
 import fxcorr as F
 import fxref as R
-from _cases import gpu_plan, gpu_run, make_case, oracle_run, spectra_err, zero_pattern_equal
+from _cases import (gpu_plan, gpu_run, make_case, oracle_run, rel_err, spectra_err,
+                    zero_pattern_equal)
 from _synth import IDENT, LEVELS_2BIT, uniform_levels
 
 pytestmark = pytest.mark.gpu
@@ -366,3 +367,70 @@ def test_mfma_needs_common_band_start():
     p = F.Plan(1024, levels=lv2, df_hz=1.0, band=(-16, 16, 512, 0),
                precision=F.PRECISION_CHOSEN)
     assert p.info.fft_form == 0
+
+
+# ------------------------------------------------------------ phased array
+def _phased_desc(c, w, bit_out, output_lsb):
+    n = c["n"]
+    return dict(fft_len=n, bits1=c["bits"][0], bits2=c["bits"][1], levels1=c["levels"][0],
+                levels2=c["levels"][1], shuffle1=c["shuffles"][0], shuffle2=c["shuffles"][1],
+                lsb1=int(c["lsb"][0]), lsb2=int(c["lsb"][1]), a1s=c["band"][1], a1e=c["band"][2],
+                a2s=c["band"][3], out_grid=c["grid"], df_hz=c["fs"] / n, need_acf=0, need_xcf=0,
+                need_phased=1, write_raw=1, w1=w[0], w2=w[1], bit_out=bit_out,
+                output_lsb=int(output_lsb), cpu_threads=8)
+
+
+@pytest.mark.parametrize("n,bits,grid,band_shift,lsb_out", [
+    (1024, (2, 2), 0, 0, False),
+    (512, (4, 2), 1, 8, True),
+    (2048, (2, 1), 0, -16, False),
+])
+def test_phased_vs_oracle(n, bits, grid, band_shift, lsb_out):
+    """yi-phasedarray sector: |w1 s1 + w2 s2|^2 within the plain bar of the
+    oracle's phased branch; the .raw output decodes to the same levels except
+    at rare quantiser decision boundaries."""
+    fs = 128e6
+    half = n // 2                                  # main.rs:360-372 with v1 = v2 = N/2
+    sb = band_shift
+    a1s = max(0, -sb)
+    band = (sb, a1s, min(half, half - sb), a1s + sb)
+    rng = np.random.default_rng(n + bits[0])
+    c = make_case(n=n, fs=fs, nf=1500, bits=bits, shuffles=(rng.permutation(32), None),
+                  lsb=(False, True), band=band, grid=grid, rho=0.4, delay_samples=3.3, seed=41,
+                  delay_user_samples=3.3, extra_delay_rate_sps=1e-9, lo1_hz=5e9, lo2_hz=5e9,
+                  products=F.PRODUCT_PHASED | F.PRODUCT_RAW_OUT)
+    w = (0.7, 1.3)
+    bit_out = max(bits)
+    p = F.Plan(n, bits=bits, levels=c["levels"], shuffles=c["shuffles"], lsb=c["lsb"],
+               band=band, df_hz=fs / n, products=F.PRODUCT_PHASED | F.PRODUCT_RAW_OUT,
+               out_grid=grid, w=w, bit_out=bit_out, output_lsb=lsb_out)
+    cfg = F.make_delay_cfg(c["table"], **c["delay"])
+    got = p.sector_run(c["raw1"], c["raw2"], 0, c["nf"], cfg)
+    ref = R.sector_run(_phased_desc(c, w, bit_out, lsb_out), c["raw1"], c["raw2"], 0, c["nf"],
+                       R.make_delay_cfg(c["table"], **c["delay"]), mode=R.MODE_F32, threads=8)
+    assert got["n_failed"] == 0 and ref["n_failed"] == 0
+    assert rel_err(got["acc_ph"], ref["acc_ph"]) < PLAIN_TOL
+    for k in ("acc11", "acc22"):
+        assert np.all(got[k] == 0)
+    assert np.all(got["acc12"] == 0)
+    bpf_o = n * bit_out // 8
+    assert got["raw_out"].size == c["nf"] * bpf_o
+    dp = R.DecodePlan(bit_out, c["shuffles"][0], c["levels"][0])
+    diff = 0
+    for f in range(0, c["nf"], 7):
+        seg = slice(f * bpf_o, (f + 1) * bpf_o)
+        _, vg = dp.decode(got["raw_out"][seg], n, False)
+        _, vr = dp.decode(ref["raw_out"][seg], n, False)
+        diff += int(np.count_nonzero(vg != vr))
+    assert diff <= 1e-4 * n * len(range(0, c["nf"], 7)) + 2, diff
+
+
+def test_phased_no_raw_out_matches_with_raw_out():
+    n, fs = 1024, 64e6
+    c = make_case(n=n, fs=fs, nf=700, rho=0.3, seed=3, lo1_hz=2e9)
+    cfg = F.make_delay_cfg(c["table"], **c["delay"])
+    res = []
+    for prods in (F.PRODUCT_PHASED, F.PRODUCT_PHASED | F.PRODUCT_RAW_OUT):
+        p = F.Plan(n, levels=c["levels"], df_hz=fs / n, products=prods, bit_out=2)
+        res.append(p.sector_run(c["raw1"], c["raw2"], 0, c["nf"], cfg)["acc_ph"])
+    assert np.array_equal(res[0], res[1])
