@@ -148,7 +148,12 @@ struct fxc_plan {
     int32_t decode_ok[2] = {1, 1};
     // device tables
     DevBuf tw_fft, tw_post, lev_dev[2], out_lev, mf_tab;
-    int fft_form = 0;  // 0: FP32 Stockham, 1: FP16/FP32-MFMA four-step (N = 1024)
+    int fft_form = 0;  // 0: FP32 Stockham, 1: FP16/FP32-MFMA four-step (N = 1024),
+                       // 2: FP32 four-step through HBM scratch (N = 32768, 65536)
+    // large-N scratch (fft_form 2): chunk of `chunk` frames
+    int M1 = 0, M2 = 0;
+    int64_t chunk = 0;
+    DevBuf tw_l1, tw_l2, tw_lm, d_x[2], d_y, d_z[2], d_params, d_lshifts;
     // scratch
     DevBuf part, red_scratch, table_dev, d_raw[2], d_acc, d_raw_out, d_tmp_in, d_tmp_out,
         d_shifts, d_cb, d_tbuf;
@@ -621,8 +626,11 @@ extern "C" fxc_status fxc_plan_create(const fxc_plan_desc *desc, fxc_plan **out)
     if (!desc || !out) return fail(FXC_ERR_INVALID_ARG, "null argument");
     *out = nullptr;
     const int N = desc->fft_len;
-    if (!is_pow2(N) || !sector_supported(N))
-        return fail(FXC_ERR_UNSUPPORTED, "fft_len %d: the GPU path supports powers of two 64..16384", N);
+    const bool large = N == 32768 || N == 65536;
+    if (!is_pow2(N) || !(sector_supported(N) || large))
+        return fail(FXC_ERR_UNSUPPORTED, "fft_len %d: the GPU path supports powers of two 64..65536", N);
+    if (large && (desc->products & (FXC_PRODUCT_PHASED | FXC_PRODUCT_RAW_OUT)))
+        return fail(FXC_ERR_UNSUPPORTED, "phased products need fft_len <= 16384");
     for (int i = 0; i < 2; ++i) {
         fxc_status s = validate_station(desc->st[i], i);
         if (s != FXC_OK) return s;
@@ -663,7 +671,7 @@ extern "C" fxc_status fxc_plan_create(const fxc_plan_desc *desc, fxc_plan **out)
     p->N = N;
     p->M = (int)M;
     p->half = (int)half;
-    p->T = sector_threads_per_frame(N);
+    p->T = large ? 512 : sector_threads_per_frame(N);
     (void)hipGetDevice(&p->device);
     for (int i = 0; i < 2; ++i) {
         p->levels[i].assign(desc->st[i].levels, desc->st[i].levels + desc->st[i].nlevels);
@@ -715,10 +723,41 @@ extern "C" fxc_status fxc_plan_create(const fxc_plan_desc *desc, fxc_plan **out)
         p->fft_form = 1;
         p->T = 64;   // one wave per frame pair
     }
+    if (large) {
+        // four-step M = M1 x M2 with FFT passes of 128/256 points; scratch for
+        // a chunk of frames (decoded samples, Y, Z per station, parameters)
+        p->fft_form = 2;
+        p->M2 = 128;
+        p->M1 = (int)M / p->M2;
+        const int64_t per_frame = 2 * (int64_t)N * 4 + 3 * (int64_t)M * 8 + 16 +
+                                  (int64_t)frame_param_bytes();
+        p->chunk = std::max<int64_t>(16, ((int64_t)768 << 20) / per_frame);
+        auto table = [](DevBuf &b, int n) {
+            std::vector<float> t(2 * (size_t)n);
+            for (int k = 0; k < n; ++k) {
+                const double ang = -2.0 * M_PI * (double)k / (double)n;
+                t[2 * k] = (float)std::cos(ang);
+                t[2 * k + 1] = (float)std::sin(ang);
+            }
+            return b.ensure(t.size() * 4) == hipSuccess &&
+                   hipMemcpy(b.p, t.data(), t.size() * 4, hipMemcpyHostToDevice) == hipSuccess;
+        };
+        const size_t C = (size_t)p->chunk;
+        if (!table(p->tw_l1, p->M1) || !table(p->tw_l2, p->M2) || !table(p->tw_lm, (int)M) ||
+            p->d_x[0].ensure(C * N * 4) != hipSuccess || p->d_x[1].ensure(C * N * 4) != hipSuccess ||
+            p->d_y.ensure(C * M * 8) != hipSuccess || p->d_z[0].ensure(C * M * 8) != hipSuccess ||
+            p->d_z[1].ensure(C * M * 8) != hipSuccess ||
+            p->d_params.ensure(C * frame_param_bytes()) != hipSuccess ||
+            p->d_lshifts.ensure(C * 16) != hipSuccess)
+            return cleanup(fail(FXC_ERR_NOMEM, "large-FFT scratch allocation failed"));
+    }
     // grid: plain form about 16 waves per CU worth of workgroups (p->T
-    // threads each); MFMA form one 8-wave workgroup per CU.  Each workgroup
-    // runs a contiguous range of frames.
-    p->max_wg = p->fft_form == 1 ? cus : std::max(64, (cus * 1024) / p->T);
+    // threads each); MFMA form one 8-wave workgroup per CU; large-N form one
+    // partial per frame group.  Each workgroup runs a contiguous range of
+    // frames.
+    p->max_wg = p->fft_form == 1 ? cus
+                : p->fft_form == 2 ? kLargeGroups
+                                   : std::max(64, (cus * 1024) / p->T);
     if (p->part.ensure((size_t)p->max_wg * kNumProducts * half * sizeof(double)) != hipSuccess ||
         p->red_scratch.ensure((size_t)64 * kNumProducts * half * sizeof(double)) != hipSuccess)
         return cleanup(fail(FXC_ERR_NOMEM, "scratch allocation failed"));
@@ -740,7 +779,8 @@ extern "C" fxc_status fxc_plan_destroy(fxc_plan *p) {
     DevBuf *bufs[] = {&p->tw_fft, &p->tw_post, &p->lev_dev[0], &p->lev_dev[1], &p->out_lev,
                       &p->part, &p->red_scratch, &p->table_dev, &p->d_raw[0], &p->d_raw[1],
                       &p->d_acc, &p->d_raw_out, &p->d_tmp_in, &p->d_tmp_out, &p->d_shifts,
-                      &p->d_cb, &p->d_tbuf,
+                      &p->d_cb, &p->d_tbuf, &p->tw_l1, &p->tw_l2, &p->tw_lm, &p->d_x[0],
+                      &p->d_x[1], &p->d_y, &p->d_z[0], &p->d_z[1], &p->d_params, &p->d_lshifts,
                       &p->mf_tab};
     for (DevBuf *b : bufs) b->release();
     if (p->stream) (void)hipStreamDestroy(p->stream);
@@ -869,6 +909,40 @@ static fxc_status enqueue_phased(fxc_plan *p, const SectorArgs &base, int64_t nf
     return FXC_OK;
 }
 
+// N = 32768 / 65536: chunks of frames through HBM scratch; the product
+// kernel's kLargeGroups frame-group partial slots accumulate over the chunks
+// in order, then one fixed-order reduce.
+static fxc_status enqueue_large(fxc_plan *p, const SectorArgs &base, int64_t nf, double *d_acc,
+                                hipStream_t s) {
+    const int M = p->M;
+    for (int64_t c0 = 0; c0 < nf; c0 += p->chunk) {
+        const int64_t cn = std::min(p->chunk, nf - c0);
+        SectorArgs a = base;
+        a.raw1 = base.raw1 + c0 * p->st[0].bpf;
+        a.raw2 = base.raw2 + c0 * p->st[1].bpf;
+        a.first_frame = base.first_frame + c0;
+        a.nf = cn;
+        a.part = (double *)p->part.p;
+        FrameParam *params = (FrameParam *)p->d_params.p;
+        int64_t *shifts = (int64_t *)p->d_lshifts.p;
+        HIP_TRY(launch_frame_params(a, p->desc.precision, params, shifts, s));
+        for (int st = 0; st < 2; ++st) {
+            HIP_TRY(launch_decode_frames(a, st, shifts + st, 2, (float *)p->d_x[st].p, s));
+            HIP_TRY(launch_fft_large((const float *)p->d_x[st].p, (float2 *)p->d_y.p,
+                                     (float2 *)p->d_z[st].p, cn, p->M1, p->M2,
+                                     (const float2 *)p->tw_l1.p, (const float2 *)p->tw_l2.p,
+                                     (const float2 *)p->tw_lm.p, s));
+        }
+        HIP_TRY(launch_mac_large(a, p->desc.precision, (const float2 *)p->d_z[0].p,
+                                 (const float2 *)p->d_z[1].p, params, kLargeGroups, c0 > 0 ? 1 : 0,
+                                 s));
+    }
+    (void)M;
+    HIP_TRY(launch_reduce((const double *)p->part.p, kLargeGroups, p->half, d_acc,
+                          (double *)p->red_scratch.p, 64, s));
+    return FXC_OK;
+}
+
 static fxc_status enqueue_sector(fxc_plan *p, const uint8_t *d_raw1, const uint8_t *d_raw2,
                                  int64_t first_frame, int64_t nf, const fxc_delay_cfg &dc,
                                  double *d_acc, uint8_t *d_raw_out, hipStream_t s) {
@@ -892,6 +966,7 @@ static fxc_status enqueue_sector(fxc_plan *p, const uint8_t *d_raw1, const uint8
     a.first_frame = first_frame;
     a.nf = nf;
     if (phased) return enqueue_phased(p, a, nf, d_raw_out, d_acc, s);
+    if (p->fft_form == 2) return enqueue_large(p, a, nf, d_acc, s);
     int64_t fpw = (nf + p->max_wg - 1) / p->max_wg;
     const int batch = p->fft_form == 1 ? 8 : kFrameBatch;   // MFMA form: whole frames per wave
     fpw = ((fpw + batch - 1) / batch) * batch;
@@ -1030,7 +1105,7 @@ extern "C" fxc_status fxc_decode_frames(fxc_plan *p, int32_t station, const uint
     a.nf = nframes;
     if (station == 0) a.raw1 = (const uint8_t *)p->d_tmp_in.p;
     else a.raw2 = (const uint8_t *)p->d_tmp_in.p;
-    HIP_TRY(launch_decode_frames(a, station, dsh, (float *)p->d_tmp_out.p, s));
+    HIP_TRY(launch_decode_frames(a, station, dsh, 1, (float *)p->d_tmp_out.p, s));
     HIP_TRY(hipMemcpyAsync(out, p->d_tmp_out.p, out_bytes, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     return FXC_OK;
@@ -1049,7 +1124,19 @@ static fxc_status run_fft(fxc_plan *p, const float *in, int64_t nframes, float *
     HIP_TRY(hipMemcpyAsync(p->d_tmp_in.p, in, in_bytes, hipMemcpyHostToDevice, s));
     SectorArgs a;
     fill_args(p, a);
-    if (inverse)
+    if (p->fft_form == 2) {
+        if (inverse) return fail(FXC_ERR_UNSUPPORTED, "inverse FFT needs fft_len <= 16384");
+        for (int64_t c0 = 0; c0 < nframes; c0 += p->chunk) {   // through the chunk scratch
+            const int64_t cn = std::min(p->chunk, nframes - c0);
+            HIP_TRY(launch_fft_large((const float *)p->d_tmp_in.p + c0 * p->N, (float2 *)p->d_y.p,
+                                     (float2 *)p->d_z[0].p, cn, p->M1, p->M2,
+                                     (const float2 *)p->tw_l1.p, (const float2 *)p->tw_l2.p,
+                                     (const float2 *)p->tw_lm.p, s));
+            HIP_TRY(launch_untangle_large((const float2 *)p->d_z[0].p, cn, p->M,
+                                          (const float2 *)p->tw_post.p,
+                                          (float2 *)p->d_tmp_out.p + c0 * p->half, s));
+        }
+    } else if (inverse)
         HIP_TRY(launch_irfft_frames(a, (const float2 *)p->d_tmp_in.p, nframes, (float *)p->d_tmp_out.p,
                                     false, s));
     else if (p->fft_form == 1)

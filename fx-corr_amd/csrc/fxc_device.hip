@@ -228,8 +228,9 @@ hipError_t launch_sector(const SectorArgs &a, int precision, int n_wg, hipStream
 // fused kernel's staging and element decode.
 template <int T>
 __global__ __launch_bounds__(T) void k_decode_frames(SectorArgs a, int station,
-                                                     const int64_t *shifts, float *out) {
-    __shared__ uint32_t words[16384 / 2 + 8];
+                                                     const int64_t *shifts, int shift_stride,
+                                                     float *out) {
+    extern __shared__ uint32_t words[];   // st.nwords
     __shared__ float lev[256];
     const StationDev &st = a.st[station];
     const int t = threadIdx.x;
@@ -239,7 +240,7 @@ __global__ __launch_bounds__(T) void k_decode_frames(SectorArgs a, int station,
     const uint8_t *raw = (station == 0 ? a.raw1 : a.raw2) + f * st.bpf;
     stage_words<T>(raw, st, words, t);
     __syncthreads();
-    const int s = shifts ? clamp_shift(shifts[f], a.N) : 0;
+    const int s = shifts ? clamp_shift(shifts[f * shift_stride], a.N) : 0;
     for (int m = t; m < a.M; m += T) {
         float2 v = decode_elem(words, st, lev, m, s, a.N);
         out[f * a.N + 2 * m] = v.x;
@@ -248,11 +249,17 @@ __global__ __launch_bounds__(T) void k_decode_frames(SectorArgs a, int station,
 }
 
 hipError_t launch_decode_frames(const SectorArgs &a, int station, const int64_t *shifts,
-                                float *out, hipStream_t s) {
+                                int shift_stride, float *out, hipStream_t s) {
     if (a.nf <= 0) return hipSuccess;
-    if (a.st[station].nwords > 16384 / 2 + 8) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_decode_frames<256>, dim3((unsigned)a.nf), dim3(256), 0, s, a, station,
-                       shifts, out);
+    const size_t lds = (size_t)a.st[station].nwords * 4;
+    if (lds > 96 * 1024) return hipErrorInvalidValue;
+    if (lds > 48 * 1024) {   // above the default dynamic-LDS limit (8-bit samples, N = 65536)
+        hipError_t e = hipFuncSetAttribute((const void *)k_decode_frames<256>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k_decode_frames<256>, dim3((unsigned)a.nf), dim3(256), lds, s, a, station,
+                       shifts, shift_stride, out);
     return hipGetLastError();
 }
 

@@ -10,7 +10,7 @@ namespace fxc {
 
 constexpr int kMaxBits = 16;       // widest sample the GPU decoder accepts
 constexpr int kFrameBatch = 8;     // frames whose delay entries one batch computes
-constexpr int kMaxPow = 14;        // step^(2^b), b < kMaxPow (covers M <= 8192)
+constexpr int kMaxPow = 16;        // step^(2^b), b < kMaxPow (exponents below 65536)
 constexpr int kNumProducts = 5;    // [ph | 11 | 22 | 12re | 12im]
 enum { P_PH = 0, P_11 = 1, P_22 = 2, P_12R = 3, P_12I = 4 };
 
@@ -99,7 +99,7 @@ int sector_threads_per_frame(int N);
 bool sector_supported(int N);
 
 hipError_t launch_decode_frames(const SectorArgs &a, int station,
-                                const int64_t *shifts, float *out,
+                                const int64_t *shifts, int shift_stride, float *out,
                                 hipStream_t stream);
 hipError_t launch_rfft_frames(const SectorArgs &a, const float *in, int64_t nframes,
                               float2 *out, hipStream_t stream);
@@ -118,6 +118,21 @@ int mfma_wg_threads();
 // fxc_phased.hip
 hipError_t launch_phased(const SectorArgs &a, int accumulate, int n_wg, hipStream_t stream);
 hipError_t launch_fill_zero(void *p, size_t bytes, hipStream_t stream);
+
+// fxc_large.hip: N = 32768, 65536 through HBM scratch (four-step M = M1 x M2)
+struct FrameParam;
+constexpr int kLargeGroups = 16;   // frame groups of the bin-owned product kernel
+hipError_t launch_frame_params(const SectorArgs &a, int precision, FrameParam *params,
+                               int64_t *shifts, hipStream_t stream);
+hipError_t launch_fft_large(const float *x, float2 *ybuf, float2 *z, int64_t nframes, int M1,
+                            int M2, const float2 *tw1, const float2 *tw2, const float2 *twm,
+                            hipStream_t stream);
+hipError_t launch_untangle_large(const float2 *z, int64_t nframes, int M, const float2 *tw_post,
+                                 float2 *out, hipStream_t stream);
+hipError_t launch_mac_large(const SectorArgs &a, int precision, const float2 *z1, const float2 *z2,
+                            const FrameParam *params, int n_groups, int accumulate,
+                            hipStream_t stream);
+size_t frame_param_bytes();
 hipError_t launch_rfft_mfma(const SectorArgs &a, const float *in, int64_t nframes, float2 *out,
                             hipStream_t stream);
 

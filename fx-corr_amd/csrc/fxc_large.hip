@@ -1,0 +1,234 @@
+// FX path for FFT lengths beyond one workgroup's LDS (N = 32768, 65536:
+// M = N/2 complex points = 128/256 KB of FP32 per station-frame).  Frames
+// are processed in chunks through HBM scratch:
+//
+//   1. k_frame_params   per-frame delay entries (main.rs:184-241) and the
+//                       XCF phase start/step (main.rs:133-149)
+//   2. k_decode_frames  decode + integer shift to FP32 (the bit-exact decode)
+//   3. k_fft_pass       four-step FFT, m = M2 m1 + m2, k = k1 + M1 k2:
+//                       column DFTs over m1 (length M1) times W_M^{m2 k1},
+//                       then row DFTs over m2 (length M2) into Z in natural
+//                       order; each pass is a batch of LDS Stockham FFTs
+//   4. k_mac_large      realfft untangle and ACF/XCF products, one output
+//                       bin per thread looping over a group of frames, FP64
+//                       accumulators (any band shift)
+#include "fxc_dev.h"
+
+namespace fxc {
+
+// ---------------------------------------------------------------- params
+template <int PREC>
+__global__ void k_frame_params(SectorArgs a, int logm, FrameParam *params, int64_t *shifts) {
+    const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= a.nf) return;
+    FrameParam p;
+    make_frame_param<PREC>(a.first_frame + f, a, logm, p);
+    params[f] = p;
+    shifts[2 * f] = p.s1;
+    shifts[2 * f + 1] = p.s2;
+}
+
+hipError_t launch_frame_params(const SectorArgs &a, int precision, FrameParam *params,
+                               int64_t *shifts, hipStream_t s) {
+    if (a.nf <= 0) return hipSuccess;
+    const int logm = __builtin_ctz((unsigned)a.M) + 1;
+    const dim3 grid((unsigned)((a.nf + 127) / 128));
+    if (precision == 0)
+        hipLaunchKernelGGL(k_frame_params<0>, grid, dim3(128), 0, s, a, logm, params, shifts);
+    else
+        hipLaunchKernelGGL(k_frame_params<1>, grid, dim3(128), 0, s, a, logm, params, shifts);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------- four-step passes
+// A batch of CB independent L-point FFTs (L = E*T), column c of the batch
+// at in[c*ics + m*ims] (m < L), result k written to out[c*ocs + k*oks],
+// multiplied by W_M^{(c0 + c) k} when `twiddle` (tw_m = W_M^j, j < M).
+// Loads and stores walk the unit-stride index across the threads.
+template <int E, int T, int CB>
+__global__ __launch_bounds__(CB *T) void k_fft_pass(const float2 *in, float2 *out, int64_t nbatch,
+                                                    int64_t cols, int64_t ics, int64_t ims,
+                                                    int64_t ocs, int64_t oks, int64_t frame_in,
+                                                    int64_t frame_out, const float2 *tw_l,
+                                                    const float2 *tw_m, int M, int twiddle) {
+    constexpr int L = E * T;
+    constexpr int LP = L + L / 16 + 2;
+    __shared__ float2 buf[CB][LP];
+    const int tid = threadIdx.x;
+    const int64_t blocks_per_frame = cols / CB;
+    const int64_t f = blockIdx.x / blocks_per_frame;
+    const int64_t c0 = (blockIdx.x % blocks_per_frame) * CB;
+    if (f >= nbatch) return;
+    const float2 *src = in + f * frame_in;
+    float2 *dst = out + f * frame_out;
+    // load: unit stride along c (ics == 1) or along m (ims == 1)
+    for (int i = tid; i < CB * L; i += CB * T) {
+        int c, m;
+        if (ics == 1) { c = i % CB; m = i / CB; }
+        else { m = i % L; c = i / L; }
+        buf[c][pad(m)] = src[(c0 + c) * ics + (int64_t)m * ims];
+    }
+    __syncthreads();
+    const int c = tid / T, t = tid % T;
+    float2 v[E];
+#pragma unroll
+    for (int r = 0; r < E; ++r) v[r] = buf[c][pad(t + T * r)];
+    __syncthreads();
+    fft_forward<E, T>(buf[c], v, tw_l, t);
+    for (int i = tid; i < CB * L; i += CB * T) {
+        int cc, k;
+        if (ocs == 1) { cc = i % CB; k = i / CB; }
+        else { k = i % L; cc = i / L; }
+        float2 y = buf[cc][pad(k)];
+        if (twiddle) y = c_mul(y, tw_m[((c0 + cc) * (int64_t)k) % M]);
+        dst[(c0 + cc) * ocs + (int64_t)k * oks] = y;
+    }
+}
+
+template <int E, int T>
+static hipError_t launch_pass_t(const float2 *in, float2 *out, int64_t nbatch, int64_t cols,
+                                int64_t ics, int64_t ims, int64_t ocs, int64_t oks,
+                                int64_t frame_in, int64_t frame_out, const float2 *tw_l,
+                                const float2 *tw_m, int M, int twiddle, hipStream_t s) {
+    constexpr int CB = 16;
+    if (cols % CB != 0) return hipErrorInvalidValue;
+    const int64_t grid = nbatch * (cols / CB);
+    hipLaunchKernelGGL((k_fft_pass<E, T, CB>), dim3((unsigned)grid), dim3(CB * T), 0, s, in, out,
+                       nbatch, cols, ics, ims, ocs, oks, frame_in, frame_out, tw_l, tw_m, M,
+                       twiddle);
+    return hipGetLastError();
+}
+
+static hipError_t launch_pass(int L, const float2 *in, float2 *out, int64_t nbatch, int64_t cols,
+                              int64_t ics, int64_t ims, int64_t ocs, int64_t oks,
+                              int64_t frame_in, int64_t frame_out, const float2 *tw_l,
+                              const float2 *tw_m, int M, int twiddle, hipStream_t s) {
+    switch (L) {
+    case 128: return launch_pass_t<4, 32>(in, out, nbatch, cols, ics, ims, ocs, oks, frame_in,
+                                          frame_out, tw_l, tw_m, M, twiddle, s);
+    case 256: return launch_pass_t<8, 32>(in, out, nbatch, cols, ics, ims, ocs, oks, frame_in,
+                                          frame_out, tw_l, tw_m, M, twiddle, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+// Z (natural order, M complex per station-frame) of nframes frames from the
+// decoded samples x (N floats per frame): x read as z[m] = x[2m] + i x[2m+1].
+hipError_t launch_fft_large(const float *x, float2 *ybuf, float2 *z, int64_t nframes, int M1,
+                            int M2, const float2 *tw1, const float2 *tw2, const float2 *twm,
+                            hipStream_t s) {
+    const int M = M1 * M2;
+    const float2 *zin = reinterpret_cast<const float2 *>(x);
+    // columns c = m2 (M2 of them): input z[m2 + M2 m1], output Y[k1][m2] * W_M^{m2 k1}
+    hipError_t e = launch_pass(M1, zin, ybuf, nframes, M2, 1, M2, 1, M2, M, M, tw1, twm, M, 1, s);
+    if (e != hipSuccess) return e;
+    // rows c = k1 (M1 of them): input Y[k1][m2], output Z[k1 + M1 k2]
+    return launch_pass(M2, ybuf, z, nframes, M1, M2, 1, 1, M1, M, M, tw2, twm, M, 0, s);
+}
+
+size_t frame_param_bytes() { return sizeof(FrameParam); }
+
+// realfft's split method on an unpadded Z (natural order in global memory).
+__device__ __forceinline__ float2 untangle_np(const float2 *zs, int k, int M, const float2 *tw_post) {
+    if (k == 0 || k == M) {
+        const float2 z0 = zs[0];
+        return make_float2(k == 0 ? z0.x + z0.y : z0.x - z0.y, 0.0f);
+    }
+    const float2 A = zs[k], Bc = zs[M - k];
+    const float ex = 0.5f * (A.x + Bc.x), ey = 0.5f * (A.y - Bc.y);
+    const float dx = 0.5f * (A.x - Bc.x), dy = 0.5f * (A.y + Bc.y);
+    const float2 w = tw_post[k];
+    return make_float2(ex + (w.x * dy - w.y * dx), ey - (w.x * dx + w.y * dy));
+}
+
+__global__ void k_untangle_large(const float2 *z, int64_t nframes, int M, const float2 *tw_post,
+                                 float2 *out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t f = i / (M + 1);
+    const int k = (int)(i % (M + 1));
+    if (f >= nframes) return;
+    out[i] = untangle_np(z + f * M, k, M, tw_post);
+}
+
+hipError_t launch_untangle_large(const float2 *z, int64_t nframes, int M, const float2 *tw_post,
+                                 float2 *out, hipStream_t s) {
+    const int64_t n = nframes * (M + 1);
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_untangle_large, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, z,
+                       nframes, M, tw_post, out);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------- untangle + MAC
+// Bin g of the output grid per thread; frames [g0, g1) of the chunk; the
+// per-frame phase terms come from k_frame_params.
+template <int PREC>
+__global__ __launch_bounds__(256) void k_mac_large(SectorArgs a, const float2 *z1, const float2 *z2,
+                                                   const FrameParam *params, int64_t fpg,
+                                                   int accumulate) {
+    const int M = a.M;
+    const int half = M + 1;
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t f0 = (int64_t)blockIdx.y * fpg;
+    int64_t f1 = f0 + fpg;
+    if (f1 > a.nf) f1 = a.nf;
+    if (g >= half) return;
+    const bool do_acf = (a.flags & FLAG_ACF) != 0;
+    const bool do_xcf = (a.flags & FLAG_XCF) != 0;
+    const bool ov_only = (a.flags & FLAG_ACF_OVERLAP) != 0;
+    const int gs = (a.grid == 0) ? a.a1s : a.a2s;
+    const bool in_ov = g >= gs && g < gs + a.ov;
+    const int sb_other = (a.grid == 0) ? (a.a2s - a.a1s) : (a.a1s - a.a2s);
+    const float2 *zg = a.grid == 0 ? z1 : z2;
+    const float2 *zo = a.grid == 0 ? z2 : z1;
+    const int kk = g - gs;   // overlap index (XCF phase exponent)
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int64_t f = f0; f < f1; ++f) {
+        const float2 Xg = untangle_np(zg + f * M, g, M, a.tw_post);
+        float2 Xo = make_float2(0.0f, 0.0f);
+        if (in_ov) Xo = untangle_np(zo + f * M, g + sb_other, M, a.tw_post);
+        if (do_acf) {
+            const float pgrid = (!ov_only || in_ov) ? norm_sqr(Xg) : 0.0f;
+            const float poth = in_ov ? norm_sqr(Xo) : 0.0f;
+            acc[0] += (double)(a.grid == 0 ? pgrid : poth);
+            acc[1] += (double)(a.grid == 0 ? poth : pgrid);
+        }
+        if (in_ov && do_xcf) {
+            const FrameParam &P = params[f];
+            cd ph = P.start;
+            for (int b = 0; b < kMaxPow; ++b)
+                if ((kk >> b) & 1) ph = d_mul(ph, P.pw[b]);
+            const float2 X1 = (a.grid == 0) ? Xg : Xo;
+            const float2 X2 = (a.grid == 0) ? Xo : Xg;
+            float2 v2 = c_mul(X1, c_conj(X2));
+            if (PREC == 0) v2 = c_mul(v2, P.frmix);
+            v2 = c_mul(v2, make_float2((float)ph.x, (float)ph.y));
+            acc[2] += (double)v2.x;
+            acc[3] += (double)v2.y;
+        }
+    }
+    double *out = a.part + (size_t)blockIdx.y * kNumProducts * half;
+    const double prev[4] = {accumulate ? out[P_11 * half + g] : 0.0,
+                            accumulate ? out[P_22 * half + g] : 0.0,
+                            accumulate ? out[P_12R * half + g] : 0.0,
+                            accumulate ? out[P_12I * half + g] : 0.0};
+    out[P_PH * half + g] = 0.0;
+    out[P_11 * half + g] = prev[0] + acc[0];
+    out[P_22 * half + g] = prev[1] + acc[1];
+    out[P_12R * half + g] = prev[2] + acc[2];
+    out[P_12I * half + g] = prev[3] + acc[3];
+}
+
+hipError_t launch_mac_large(const SectorArgs &a, int precision, const float2 *z1, const float2 *z2,
+                            const FrameParam *params, int n_groups, int accumulate,
+                            hipStream_t s) {
+    const int64_t fpg = (a.nf + n_groups - 1) / n_groups;
+    const dim3 grid((unsigned)((a.M + 1 + 255) / 256), (unsigned)n_groups);
+    if (precision == 0)
+        hipLaunchKernelGGL(k_mac_large<0>, grid, dim3(256), 0, s, a, z1, z2, params, fpg, accumulate);
+    else
+        hipLaunchKernelGGL(k_mac_large<1>, grid, dim3(256), 0, s, a, z1, z2, params, fpg, accumulate);
+    return hipGetLastError();
+}
+
+}  // namespace fxc

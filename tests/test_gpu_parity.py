@@ -434,3 +434,71 @@ def test_phased_no_raw_out_matches_with_raw_out():
         p = F.Plan(n, levels=c["levels"], df_hz=fs / n, products=prods, bit_out=2)
         res.append(p.sector_run(c["raw1"], c["raw2"], 0, c["nf"], cfg)["acc_ph"])
     assert np.array_equal(res[0], res[1])
+
+
+# --------------------------------------------- N = 32768 / 65536 (HBM four-step)
+@pytest.mark.parametrize("n", [32768, 65536])
+def test_rfft_large_vs_numpy(n):
+    p = F.Plan(n, levels=[LEVELS_2BIT, LEVELS_2BIT], df_hz=1.0)
+    assert p.info.fft_form == 2
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal((3, n)).astype(np.float32)
+    x[0] = 0.0
+    x[0, 7] = 1.0                                      # impulse: exact linear phase
+    got = p.rfft_frames(x)
+    ref = np.fft.rfft(x.astype(np.float64), axis=1)
+    err = np.sqrt(np.mean(np.abs(got - ref) ** 2, axis=1) / np.mean(np.abs(ref) ** 2, axis=1))
+    assert np.all(err < 3e-7 * np.log2(n)), err
+    assert np.all(got[:, 0].imag == 0) and np.all(got[:, -1].imag == 0)
+
+
+def test_c3_shape_4bit_1bit_band_shift():
+    """C3 shape: 512 Msps, A 4-bit (16 uniform levels) / B 1-bit, FFT 65536,
+    16 MHz centre offset -> band shift 2048 bins, overlap 30720 (main.rs:
+    360-372), both output grids' bins; 40 frames of it."""
+    n, fs = 65536, 512e6
+    band = F.band_alignment(n, fs, 8400.0, 8384.0, 256.0, 256.0)
+    assert band == (2048, 0, 30720, 2048)
+    c = make_case(n=n, fs=fs, nf=40, bits=(4, 1), band=band, rho=0.3, delay_samples=1.7,
+                  seed=13, delay_user_samples=1.7, extra_delay_rate_sps=1e-9, lo1_hz=8.4e9,
+                  lo2_hz=8.384e9)
+    o64 = oracle_run(c, R.MODE_F64)
+    o32 = oracle_run(c, R.MODE_F32)
+    g = gpu_run(c)
+    e32 = spectra_err(g, o32)
+    assert e32["acc11"] < PLAIN_TOL and e32["acc22"] < PLAIN_TOL, e32
+    assert e32["acc12"] < 2e-3, e32             # the reference's 30720-bin f32 phase walk
+    assert spectra_err(g, o64)["acc12"] < 1.2e-7 * 30720 + 1e-5
+    gc = gpu_run(c, F.PRECISION_CHOSEN)
+    _compare(gc, o64, CHOSEN_TOL)
+    for k in ("acc11", "acc22", "acc12"):
+        assert zero_pattern_equal(g[k], o64[k]), k
+
+
+def test_large_chunking_is_a_sum():
+    """A sector longer than one scratch chunk equals the sum of its halves
+    run separately (partials accumulate over chunks in order)."""
+    n, fs, nf = 32768, 256e6, 2600
+    s = F.Stream()
+    bpf = n * 2 // 8
+    r1, r2 = F.DeviceBuffer(nf * bpf), F.DeviceBuffer(nf * bpf)
+    F.synth_station_device(r1.ptr, nf * n, 2, None, False, 5, 0, 0.2, 3, s.ptr)
+    F.synth_station_device(r2.ptr, nf * n, 2, None, False, 5, 1, 0.2, 3, s.ptr)
+    s.sync()
+    p = F.Plan(n, levels=[LEVELS_2BIT, LEVELS_2BIT], df_hz=fs / n,
+               precision=F.PRECISION_CHOSEN)
+    cfg = F.make_delay_cfg(np.zeros(4), frame_dt=n / fs, fs=fs, delay_user_samples=3.2,
+                           lo1_hz=3e9)
+    h = p.half
+    acc = F.DeviceBuffer(3 * F.N_PRODUCTS * h * 8)
+    p.sector_run_device(r1.ptr, r2.ptr, 0, nf, cfg, acc.ptr, stream=s.ptr)
+    k = 1300
+    p.sector_run_device(r1.ptr, r2.ptr, 0, k, cfg, acc.ptr + F.N_PRODUCTS * h * 8, stream=s.ptr)
+    p.sector_run_device(r1.ptr + k * bpf, r2.ptr + k * bpf, k, nf - k, cfg,
+                        acc.ptr + 2 * F.N_PRODUCTS * h * 8, stream=s.ptr)
+    s.sync()
+    out = acc.download(np.zeros(3 * F.N_PRODUCTS * h)).reshape(3, F.N_PRODUCTS, h)
+    full, parts = out[0], out[1] + out[2]
+    scale = np.abs(full).max(axis=1, keepdims=True)
+    assert np.all(np.abs(full - parts) <= 1e-11 * scale + 1e-300)
+    assert np.all(full[1] > 0)
