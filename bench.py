@@ -127,6 +127,60 @@ def pmc_summary(fetch_dir, write_dir, out_json, copy_mib):
     print(json.dumps(out, indent=1, sort_keys=True))
 
 
+def serve_bench(raw1, raw2, bpf, cfg, shuffle_b, n_jobs, load, sector_ms, seed=7):
+    """Continuous-batching run: Poisson arrivals of one-second scan jobs, each
+    pushed as four 64 MB chunks per station (250,000 frames, 0.25 s at
+    1024 Msps 2-bit) from the device-resident pool, admitted under a 200 GB
+    budget; returns throughput and job-latency percentiles."""
+    rng = np.random.default_rng(seed)
+    chunk = 250_000
+    per_job = 4 * chunk
+    pool = (int(SECONDS * FS / N) // per_job)           # distinct one-second slices
+    plan_kw = dict(fft_len=N, bits=(BITS, BITS), levels=[LEVELS, LEVELS],
+                   shuffles=(None, shuffle_b), lsb=(False, True), band=(0, 0, N // 2, 0),
+                   df_hz=FS / N, products=F.PRODUCT_ACF | F.PRODUCT_XCF,
+                   precision=F.PRECISION_CHOSEN)
+    sv = F.Server(hbm_budget_bytes=200 << 30, max_active_jobs=8, n_streams=4)
+    # warm the plan cache (first plan creation uploads the tables)
+    j0 = sv.submit(plan_kw, cfg)
+    sv.push(j0, raw1.ptr, raw2.ptr, chunk)
+    sv.close_job(j0)
+    if not sv.wait(j0, 120.0):
+        raise RuntimeError("serving warm-up job did not finish")
+    sv.release(j0)
+    service_s = sector_ms * 1e-3 * per_job / 1_000_000   # fused-kernel time of one job
+    rate = load / service_s
+    jobs = []
+    t0 = time.perf_counter()
+    t_next = t0
+    for i in range(n_jobs):
+        t_next += rng.exponential(1.0 / rate)
+        while time.perf_counter() < t_next:
+            pass
+        k = i % pool
+        jid = sv.submit(plan_kw, cfg, first_frame=k * per_job)
+        for c in range(4):
+            off = k * per_job + c * chunk
+            sv.push(jid, raw1.ptr + off * bpf, raw2.ptr + off * bpf, chunk)
+        sv.close_job(jid)
+        jobs.append(jid)
+    for jid in jobs:
+        if not sv.wait(jid, 120.0):
+            raise RuntimeError(f"serving job {jid} did not finish")
+    wall = time.perf_counter() - t0
+    st = sv.stats()
+    sv.close()
+    samples = n_jobs * per_job * N
+    return {"jobs": n_jobs, "offered_load": load, "job": "1 s scan, 4 x 64 MB chunks per station",
+            "value": round(samples / wall / 1e6, 1), "unit": "Msamples/s/station",
+            "p50_latency_ms": round(st.p50_latency_ms, 3),
+            "p99_latency_ms": round(st.p99_latency_ms, 3),
+            "mean_latency_ms": round(st.mean_latency_ms, 3),
+            "max_concurrent_jobs": st.max_concurrent_jobs,
+            "peak_hbm_gb": round(st.peak_hbm_bytes / 1e9, 3),
+            "failed": st.jobs_failed}
+
+
 def load_traffic(form):
     """Bytes per launch of the form's fused kernel from the committed summary."""
     try:
@@ -153,6 +207,10 @@ def main():
                          "(reference kernel for rocprofv3 FETCH_SIZE/WRITE_SIZE scaling)")
     ap.add_argument("--no-pcie", action="store_true",
                     help="skip the PCIe-inclusive (host-buffer) sector timing")
+    ap.add_argument("--serve-jobs", type=int, default=200,
+                    help="jobs of the continuous-batching run (0: skip)")
+    ap.add_argument("--serve-load", type=float, default=0.9,
+                    help="offered load of the serving run (arrival rate x service time)")
     ap.add_argument("--pmc-summary", nargs=3, metavar=("FETCH_DIR", "WRITE_DIR", "OUT_JSON"),
                     help="summarise two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of "
                          "'bench.py --copy-mib M' into OUT_JSON and exit")
@@ -246,6 +304,15 @@ def main():
         dst.free()
 
     main_form = "chosen" if "chosen" in results else next(iter(results))
+    serving = None
+    if args.serve_jobs > 0:
+        sector_ms = results[main_form]["avg_launch_ms"] * 1_000_000 / (total_frames / len(sectors))
+        serving = serve_bench(raw1, raw2, bpf, cfg, shuffle_b, args.serve_jobs, args.serve_load,
+                              sector_ms)
+        print(f"[bench] serving: {serving['value']:,.0f} Msamples/s/station, job latency p50 "
+              f"{serving['p50_latency_ms']:.2f} ms p99 {serving['p99_latency_ms']:.2f} ms, "
+              f"up to {serving['max_concurrent_jobs']} jobs in flight", file=sys.stderr,
+              flush=True)
     h1 = h2 = None
     if not (args.no_cpu_baseline and args.no_pcie):
         nfc = min(args.cpu_frames, sectors[0])
@@ -328,6 +395,7 @@ def main():
                      "hbm_frac": round(r["gbs"] / HBM_PEAK_GBS, 4)},
         "cpu_baseline": cpu,
         "pcie_inclusive": pcie,
+        "serving": serving,
     }
     if "plain" in results and "chosen" in results:
         line["chosen_over_plain"] = round(results["chosen"]["value"] / results["plain"]["value"], 3)

@@ -788,6 +788,20 @@ extern "C" fxc_status fxc_plan_destroy(fxc_plan *p) {
     return FXC_OK;
 }
 
+extern "C" fxc_status fxc_plan_device_bytes(const fxc_plan *p, uint64_t *bytes) {
+    if (!p || !bytes) return fail(FXC_ERR_INVALID_ARG, "null argument");
+    const DevBuf *bufs[] = {&p->tw_fft, &p->tw_post, &p->lev_dev[0], &p->lev_dev[1], &p->out_lev,
+                            &p->part, &p->red_scratch, &p->table_dev, &p->d_raw[0], &p->d_raw[1],
+                            &p->d_acc, &p->d_raw_out, &p->d_tmp_in, &p->d_tmp_out, &p->d_shifts,
+                            &p->mf_tab, &p->d_cb, &p->d_tbuf, &p->tw_l1, &p->tw_l2, &p->tw_lm,
+                            &p->d_x[0], &p->d_x[1], &p->d_y, &p->d_z[0], &p->d_z[1], &p->d_params,
+                            &p->d_lshifts};
+    uint64_t t = 0;
+    for (const DevBuf *b : bufs) t += b->cap;
+    *bytes = t;
+    return FXC_OK;
+}
+
 extern "C" fxc_status fxc_plan_get_info(const fxc_plan *p, fxc_plan_info *info) {
     if (!p || !info) return fail(FXC_ERR_INVALID_ARG, "null argument");
     info->fft_len = p->N;

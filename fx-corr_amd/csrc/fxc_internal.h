@@ -118,6 +118,7 @@ int mfma_wg_threads();
 // fxc_phased.hip
 hipError_t launch_phased(const SectorArgs &a, int accumulate, int n_wg, hipStream_t stream);
 hipError_t launch_fill_zero(void *p, size_t bytes, hipStream_t stream);
+hipError_t launch_add_f64(double *dst, const double *src, size_t n, hipStream_t stream);
 
 // fxc_large.hip: N = 32768, 65536 through HBM scratch (four-step M = M1 x M2)
 struct FrameParam;

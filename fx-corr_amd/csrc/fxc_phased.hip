@@ -192,6 +192,18 @@ hipError_t launch_phased(const SectorArgs &a, int accumulate, int n_wg, hipStrea
     }
 }
 
+// dst[i] += src[i] (FP64), e.g. a job's spectra += one chunk's spectra.
+__global__ void k_add_f64(double *dst, const double *src, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] += src[i];
+}
+
+hipError_t launch_add_f64(double *dst, const double *src, size_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_add_f64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, dst, src, n);
+    return hipGetLastError();
+}
+
 // Graph-safe zero fill (a kernel, not a memset node).
 __global__ void k_fill_zero(uint32_t *p, size_t nwords, uint8_t *tail, size_t ntail) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;

@@ -263,38 +263,44 @@ IDENTITY_SHUFFLE = np.arange(32, dtype=np.int32)
 
 
 # --------------------------------------------------------------------- plan
+def _plan_desc(fft_len, bits=(2, 2), levels=None, shuffles=None, lsb=(False, False), band=None,
+               df_hz=1.0, products=PRODUCT_ACF | PRODUCT_XCF, out_grid=GRID_ANT1, w=(1.0, 1.0),
+               bit_out=None, output_lsb=False, precision=PRECISION_PLAIN_FP32, keep=None):
+    """fxc_plan_desc from keyword arguments; level arrays are appended to `keep`."""
+    d = PlanDesc()
+    d.fft_len = fft_len
+    for i in range(2):
+        lv = np.ascontiguousarray(levels[i], dtype=np.float64)
+        if keep is not None:
+            keep.append(lv)
+        sh = IDENTITY_SHUFFLE if shuffles is None or shuffles[i] is None else shuffles[i]
+        d.st[i].bits = bits[i]
+        d.st[i].lsb = int(bool(lsb[i]))
+        d.st[i].shuffle_internal[:] = [int(x) for x in sh]
+        d.st[i].levels = lv.ctypes.data_as(_P(C.c_double))
+        d.st[i].nlevels = len(lv)
+    if band is None:
+        band = (0, 0, fft_len // 2, 0)
+    d.a1s, d.a1e, d.a2s = band[1], band[2], band[3]
+    d.out_grid = out_grid
+    d.df_hz = df_hz
+    d.products = products
+    d.w1, d.w2 = w
+    d.bit_out = bit_out if bit_out is not None else max(bits)
+    d.output_lsb = int(bool(output_lsb))
+    d.precision = precision
+    return d
+
+
 class Plan:
     """One fxc_plan (per-sector constants + device tables)."""
 
-    def __init__(self, fft_len, bits=(2, 2), levels=None, shuffles=None, lsb=(False, False),
-                 band=None, df_hz=1.0, products=PRODUCT_ACF | PRODUCT_XCF, out_grid=GRID_ANT1,
-                 w=(1.0, 1.0), bit_out=None, output_lsb=False, precision=PRECISION_PLAIN_FP32):
-        d = PlanDesc()
-        d.fft_len = fft_len
+    def __init__(self, fft_len, **kw):
         self._keep = []
-        for i in range(2):
-            lv = np.ascontiguousarray(levels[i], dtype=np.float64)
-            self._keep.append(lv)
-            sh = IDENTITY_SHUFFLE if shuffles is None or shuffles[i] is None else shuffles[i]
-            d.st[i].bits = bits[i]
-            d.st[i].lsb = int(bool(lsb[i]))
-            d.st[i].shuffle_internal[:] = [int(x) for x in sh]
-            d.st[i].levels = lv.ctypes.data_as(_P(C.c_double))
-            d.st[i].nlevels = len(lv)
-        half = fft_len // 2 + 1
-        if band is None:
-            band = (0, 0, fft_len // 2, 0)
-        d.a1s, d.a1e, d.a2s = band[1], band[2], band[3]
-        d.out_grid = out_grid
-        d.df_hz = df_hz
-        d.products = products
-        d.w1, d.w2 = w
-        d.bit_out = bit_out if bit_out is not None else max(bits)
-        d.output_lsb = int(bool(output_lsb))
-        d.precision = precision
+        d = _plan_desc(fft_len, keep=self._keep, **kw)
         self.desc = d
         self.fft_len = fft_len
-        self.half = half
+        self.half = fft_len // 2 + 1
         h = C.c_void_p()
         check(lib().fxc_plan_create(C.byref(d), C.byref(h)))
         self.h = h
@@ -545,3 +551,128 @@ class CorWriter:
         h, self.h = self.h, None
         if h:
             check(lib().fxc_cor_finalize(h))
+
+
+# ------------------------------------------------------------------ serving
+class ServerCfg(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_active_jobs", C.c_int32),
+                ("n_streams", C.c_int32)]
+
+
+class JobDesc(C.Structure):
+    _fields_ = [("plan", PlanDesc), ("delay", DelayCfg), ("first_frame", C.c_int64),
+                ("input_on_device", C.c_int32)]
+
+
+class JobStats(C.Structure):
+    _fields_ = [("queue_ms", C.c_double), ("latency_ms", C.c_double), ("frames", C.c_int64),
+                ("footprint_bytes", C.c_uint64)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("jobs_done", C.c_int64), ("jobs_failed", C.c_int64),
+                ("p50_latency_ms", C.c_double), ("p99_latency_ms", C.c_double),
+                ("mean_latency_ms", C.c_double), ("max_concurrent_jobs", C.c_int32),
+                ("peak_hbm_bytes", C.c_uint64)]
+
+
+def _server_sigs(L):
+    vp, i32, i64, d, u64 = C.c_void_p, C.c_int32, C.c_int64, C.c_double, C.c_uint64
+    sig = {
+        "fxc_server_create": ([_P(ServerCfg), _P(vp)], i32),
+        "fxc_server_destroy": ([vp], i32),
+        "fxc_job_submit": ([vp, _P(JobDesc), _P(u64)], i32),
+        "fxc_job_push": ([vp, u64, vp, vp, i64], i32),
+        "fxc_job_close": ([vp, u64], i32),
+        "fxc_job_wait": ([vp, u64, d, _P(i32)], i32),
+        "fxc_job_result": ([vp, u64, vp, _P(JobStats)], i32),
+        "fxc_job_release": ([vp, u64], i32),
+        "fxc_server_get_stats": ([vp, _P(ServerStats)], i32),
+        "fxc_plan_device_bytes": ([vp, _P(u64)], i32),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+
+
+EXPORTED += ["fxc_server_create", "fxc_server_destroy", "fxc_job_submit", "fxc_job_push",
+             "fxc_job_close", "fxc_job_wait", "fxc_job_result", "fxc_job_release",
+             "fxc_server_get_stats", "fxc_plan_device_bytes"]
+
+
+def _slib():
+    L = lib()
+    if not getattr(L, "_server_sigs", False):
+        _server_sigs(L)
+        L._server_sigs = True
+    return L
+
+
+class Server:
+    """Continuous-batching job server (fxc_server_*)."""
+
+    def __init__(self, hbm_budget_bytes=200 << 30, max_active_jobs=8, n_streams=4):
+        cfg = ServerCfg(hbm_budget_bytes, max_active_jobs, n_streams)
+        h = C.c_void_p()
+        check(_slib().fxc_server_create(C.byref(cfg), C.byref(h)))
+        self.h = h
+        self._keep = {}
+
+    def submit(self, plan_kw: dict, cfg: DelayCfg, first_frame=0, input_on_device=True) -> int:
+        """plan_kw: the Plan() keyword arguments (fft_len, bits, levels, ...)."""
+        jd = JobDesc()
+        keep = []
+        jd.plan = _plan_desc(keep=keep, **plan_kw)
+        jd.delay = cfg
+        jd.first_frame = first_frame
+        jd.input_on_device = int(bool(input_on_device))
+        jid = C.c_uint64()
+        check(_slib().fxc_job_submit(self.h, C.byref(jd), C.byref(jid)))
+        self._keep[jid.value] = []
+        return jid.value
+
+    def push(self, jid: int, raw1, raw2, nframes: int):
+        """raw1/raw2: device pointers (int) or host numpy arrays kept alive here."""
+        if isinstance(raw1, np.ndarray):
+            a1, a2 = np.ascontiguousarray(raw1), np.ascontiguousarray(raw2)
+            self._keep[jid] += [a1, a2]
+            p1, p2 = _vp(a1), _vp(a2)
+        else:
+            p1, p2 = C.c_void_p(raw1), C.c_void_p(raw2)
+        check(_slib().fxc_job_push(self.h, jid, p1, p2, nframes))
+
+    def close_job(self, jid: int):
+        check(_slib().fxc_job_close(self.h, jid))
+
+    def wait(self, jid: int, timeout=-1.0) -> bool:
+        done = C.c_int32()
+        check(_slib().fxc_job_wait(self.h, jid, timeout, C.byref(done)))
+        return bool(done.value)
+
+    def result(self, jid: int, half: int):
+        acc = np.zeros(N_PRODUCTS * half)
+        st = JobStats()
+        check(_slib().fxc_job_result(self.h, jid, _vp(acc), C.byref(st)))
+        return acc, st
+
+    def release(self, jid: int):
+        check(_slib().fxc_job_release(self.h, jid))
+        self._keep.pop(jid, None)
+
+    def stats(self) -> ServerStats:
+        st = ServerStats()
+        check(_slib().fxc_server_get_stats(self.h, C.byref(st)))
+        return st
+
+    def close(self):
+        if self.h:
+            check(_slib().fxc_server_destroy(self.h))
+            self.h = None
+            self._keep.clear()
+
+
+def plan_device_bytes(plan: Plan) -> int:
+    b = C.c_uint64()
+    check(_slib().fxc_plan_device_bytes(plan.h, C.byref(b)))
+    return b.value

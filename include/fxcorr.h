@@ -74,7 +74,7 @@ typedef struct {
 
 /* Everything run_once captures for the per-sector closure (main.rs:2073-2160). */
 typedef struct {
-    int32_t fft_len;              /* power of two, 64..16384 (GPU path)             */
+    int32_t fft_len;              /* power of two, 64..65536 (GPU path)             */
     fxc_station_desc st[2];
     int64_t a1s, a1e, a2s;        /* BandAlignment (main.rs:361)                    */
     int32_t out_grid;             /* FXC_GRID_*                                     */
@@ -121,7 +121,8 @@ typedef struct {
     int32_t half;                 /* fft_len/2 + 1                                  */
     int32_t bpf1, bpf2, bpf_out;  /* bytes per station-frame (main.rs:1214-1215)    */
     int32_t decode_ok1, decode_ok2; /* 0: every frame fails decode (utils.rs:372)   */
-    int32_t fft_form;             /* 0 = FP32 Stockham, 1 = FP16-MFMA four-step     */
+    int32_t fft_form;             /* 0 = FP32 Stockham, 1 = FP16/FP32-MFMA four-step,
+                                     2 = FP32 four-step through HBM (N >= 32768)    */
     int32_t threads_per_frame;
     int32_t device_id;
 } fxc_plan_info;
@@ -257,6 +258,59 @@ fxc_status fxc_synth_station_device(uint8_t *d_out, int64_t nsamples, int32_t bi
                                     const int32_t shuffle_internal[32], int32_t lsb,
                                     uint64_t seed, int32_t station, double rho,
                                     int64_t delay_samples, void *stream);
+
+/* Device memory held by a plan (tables + scratch), bytes. */
+fxc_status fxc_plan_device_bytes(const fxc_plan *plan, uint64_t *bytes);
+
+/* ----------------------------------------------------------------------- */
+/* Serving (no reference counterpart: the reference runs one scan per       */
+/* process; this admits correlation jobs continuously).  Jobs run the same  */
+/* sector path as fxc_sector_run_device; each job's input arrives in chunks */
+/* (frames in order, e.g. 64 MB e-VLBI blocks), its delay-model frame index */
+/* and FP64 partial spectra stay resident on the device between chunks, and */
+/* jobs are admitted while others run as slots and HBM budget free up.      */
+/* ----------------------------------------------------------------------- */
+typedef struct fxc_server fxc_server;
+typedef struct {
+    uint64_t hbm_budget_bytes;  /* admission budget for plans + input windows + spectra */
+    int32_t max_active_jobs;    /* concurrent job slots */
+    int32_t n_streams;          /* HIP streams the jobs are spread over (1..4) */
+} fxc_server_cfg;
+typedef struct {
+    fxc_plan_desc plan;         /* levels pointers are copied at submit */
+    fxc_delay_cfg delay;        /* geometric table copied at submit */
+    int64_t first_frame;        /* delay-model index of the job's frame 0 */
+    int32_t input_on_device;    /* chunks point to device (1) or host (0) memory */
+} fxc_job_desc;
+typedef struct {
+    double queue_ms;            /* submit -> admission */
+    double latency_ms;          /* submit -> result on the host */
+    int64_t frames;
+    uint64_t footprint_bytes;   /* device bytes held while active */
+} fxc_job_stats;
+typedef struct {
+    int64_t jobs_done, jobs_failed;
+    double p50_latency_ms, p99_latency_ms, mean_latency_ms;
+    int32_t max_concurrent_jobs;
+    uint64_t peak_hbm_bytes;
+} fxc_server_stats;
+
+fxc_status fxc_server_create(const fxc_server_cfg *cfg, fxc_server **out);
+/* Finishes the admitted and queued work, then frees everything. */
+fxc_status fxc_server_destroy(fxc_server *s);
+fxc_status fxc_job_submit(fxc_server *s, const fxc_job_desc *job, uint64_t *job_id);
+/* Append nframes frames (raw1: nframes*bpf1 bytes, raw2: nframes*bpf2); the
+ * buffers must stay valid until the job completes. */
+fxc_status fxc_job_push(fxc_server *s, uint64_t job_id, const uint8_t *raw1,
+                        const uint8_t *raw2, int64_t nframes);
+/* No more input: the job completes after its last chunk. */
+fxc_status fxc_job_close(fxc_server *s, uint64_t job_id);
+/* Waits up to timeout_s (< 0: forever); *done = 1 when the result is ready. */
+fxc_status fxc_job_wait(fxc_server *s, uint64_t job_id, double timeout_s, int32_t *done);
+/* [ph | 11 | 22 | 12re | 12im] x (fft_len/2+1) FP64 sums over all chunks. */
+fxc_status fxc_job_result(fxc_server *s, uint64_t job_id, double *acc, fxc_job_stats *stats);
+fxc_status fxc_job_release(fxc_server *s, uint64_t job_id);
+fxc_status fxc_server_get_stats(fxc_server *s, fxc_server_stats *out);
 
 /* ----------------------------------------------------------------------- */
 /* Input: packed-sample reader (replaces PackedSampleReader, main.rs:458-516, */
