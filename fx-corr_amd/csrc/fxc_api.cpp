@@ -788,6 +788,50 @@ extern "C" fxc_status fxc_plan_destroy(fxc_plan *p) {
     return FXC_OK;
 }
 
+// ================================================================ graphs
+struct fxc_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+extern "C" fxc_status fxc_graph_begin(void *stream) {
+    if (!stream) return fail(FXC_ERR_INVALID_ARG, "graph capture needs a non-null stream");
+    HIP_TRY(hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal));
+    return FXC_OK;
+}
+
+extern "C" fxc_status fxc_graph_end(void *stream, fxc_graph **out) {
+    if (!stream || !out) return fail(FXC_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    hipGraph_t g = nullptr;
+    HIP_TRY(hipStreamEndCapture((hipStream_t)stream, &g));
+    hipGraphExec_t e = nullptr;
+    hipError_t he = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+    if (he != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return fail(FXC_ERR_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(he));
+    }
+    fxc_graph *r = new fxc_graph();
+    r->graph = g;
+    r->exec = e;
+    *out = r;
+    return FXC_OK;
+}
+
+extern "C" fxc_status fxc_graph_launch(fxc_graph *g, void *stream) {
+    if (!g) return fail(FXC_ERR_INVALID_ARG, "null graph");
+    HIP_TRY(hipGraphLaunch(g->exec, (hipStream_t)stream));
+    return FXC_OK;
+}
+
+extern "C" fxc_status fxc_graph_destroy(fxc_graph *g) {
+    if (!g) return FXC_OK;
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    delete g;
+    return FXC_OK;
+}
+
 extern "C" fxc_status fxc_plan_device_bytes(const fxc_plan *p, uint64_t *bytes) {
     if (!p || !bytes) return fail(FXC_ERR_INVALID_ARG, "null argument");
     const DevBuf *bufs[] = {&p->tw_fft, &p->tw_post, &p->lev_dev[0], &p->lev_dev[1], &p->out_lev,
@@ -850,6 +894,10 @@ static fxc_status upload_table(fxc_plan *p, const fxc_delay_cfg &c, hipStream_t 
     bool same = p->table_host.size() == n &&
                 memcmp(p->table_host.data(), c.geom_table, n * sizeof(double)) == 0;
     if (!same) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (s && hipStreamIsCapturing(s, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone)
+            return fail(FXC_ERR_INVALID_ARG,
+                        "geometric table changed during graph capture: run the call once before capturing");
         if (p->table_dev.ensure(n * sizeof(double)) != hipSuccess)
             return fail(FXC_ERR_NOMEM, "table allocation failed");
         p->table_host.assign(c.geom_table, c.geom_table + n);
@@ -990,6 +1038,9 @@ static fxc_status enqueue_sector(fxc_plan *p, const uint8_t *d_raw1, const uint8
     a.part = (double *)p->part.p;
     std::pair<hipEvent_t, hipEvent_t> ev{nullptr, nullptr};
     if (p->timing) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (s && hipStreamIsCapturing(s, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone)
+            return fail(FXC_ERR_INVALID_ARG, "plan timing is not available inside a graph capture");
         if (!p->pool.empty()) {
             ev = p->pool.back();
             p->pool.pop_back();

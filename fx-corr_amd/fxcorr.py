@@ -676,3 +676,43 @@ def plan_device_bytes(plan: Plan) -> int:
     b = C.c_uint64()
     check(_slib().fxc_plan_device_bytes(plan.h, C.byref(b)))
     return b.value
+
+
+# ------------------------------------------------------------------ graphs
+EXPORTED += ["fxc_graph_begin", "fxc_graph_end", "fxc_graph_launch", "fxc_graph_destroy"]
+
+
+def _glib():
+    L = lib()
+    if not getattr(L, "_graph_sigs", False):
+        vp, i32 = C.c_void_p, C.c_int32
+        for name, args in (("fxc_graph_begin", [vp]), ("fxc_graph_end", [vp, _P(vp)]),
+                           ("fxc_graph_launch", [vp, vp]), ("fxc_graph_destroy", [vp])):
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = i32
+        L._graph_sigs = True
+    return L
+
+
+class Graph:
+    """Capture of device-entry calls on one stream, replayable (fxc_graph_*)."""
+
+    def __init__(self, stream: Stream):
+        self.stream = stream
+        self.h = None
+        check(_glib().fxc_graph_begin(stream.ptr))
+
+    def end(self):
+        h = C.c_void_p()
+        check(_glib().fxc_graph_end(self.stream.ptr, C.byref(h)))
+        self.h = h
+        return self
+
+    def launch(self, stream=None):
+        check(_glib().fxc_graph_launch(self.h, (stream or self.stream).ptr))
+
+    def close(self):
+        if self.h:
+            _glib().fxc_graph_destroy(self.h)
+            self.h = None

@@ -262,6 +262,19 @@ fxc_status fxc_synth_station_device(uint8_t *d_out, int64_t nsamples, int32_t bi
 /* Device memory held by a plan (tables + scratch), bytes. */
 fxc_status fxc_plan_device_bytes(const fxc_plan *plan, uint64_t *bytes);
 
+/* HIP-graph capture of device-entry work (no reference counterpart: replays
+ * a fixed launch sequence, e.g. the sectors of a scan, without per-launch
+ * host cost).  Between begin and end, only fxc_sector_run_device calls on
+ * `stream` (non-NULL) may be made, with plan timing off and each plan's
+ * geometric table already uploaded by an earlier call with the same table;
+ * the captured work never synchronises or clears memory with a memset node.
+ * Replays read and write the same device buffers as the captured calls. */
+typedef struct fxc_graph fxc_graph;
+fxc_status fxc_graph_begin(void *stream);
+fxc_status fxc_graph_end(void *stream, fxc_graph **out);
+fxc_status fxc_graph_launch(fxc_graph *g, void *stream);
+fxc_status fxc_graph_destroy(fxc_graph *g);
+
 /* ----------------------------------------------------------------------- */
 /* Serving (no reference counterpart: the reference runs one scan per       */
 /* process; this admits correlation jobs continuously).  Jobs run the same  */

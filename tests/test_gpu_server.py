@@ -136,3 +136,45 @@ def test_budget_limits_concurrency_and_failed_plan_is_isolated():
     assert st.max_concurrent_jobs == 1 and st.jobs_done == 3 and st.jobs_failed == 1
     assert st.peak_hbm_bytes <= int(plan_bytes * 1.5)
     assert all(np.array_equal(res[0], r) for r in res[1:])
+
+
+# ------------------------------------------------------------------ graphs
+@pytest.mark.parametrize("n,precision", [(1024, F.PRECISION_CHOSEN), (2048, F.PRECISION_PLAIN_FP32),
+                                         (32768, F.PRECISION_CHOSEN)])
+def test_graph_replays_match_eager(n, precision):
+    """Three sectors captured once and replayed three times, with eager work
+    and host synchronisation between the replays; every replay equals the
+    eager result bit for bit (no memset nodes in the captured work)."""
+    nf_sec = 3000 if n < 32768 else 200
+    kw, cfg = _job(n, 2, precision)
+    r1, r2, bpf = _synth(n, 3 * nf_sec, 2, 31)
+    p = F.Plan(n, **{k: v for k, v in kw.items() if k != "fft_len"})
+    h = p.half
+    s = F.Stream()
+    out = F.DeviceBuffer(3 * F.N_PRODUCTS * h * 8)
+    other = F.DeviceBuffer(F.N_PRODUCTS * h * 8)
+
+    def sectors():
+        for i in range(3):
+            p.sector_run_device(r1.ptr + i * nf_sec * bpf, r2.ptr + i * nf_sec * bpf, i * nf_sec,
+                                nf_sec, cfg, out.ptr + i * F.N_PRODUCTS * h * 8, stream=s.ptr)
+
+    sectors()                                   # eager (also uploads the delay table)
+    s.sync()
+    eager = out.download(np.zeros(3 * F.N_PRODUCTS * h))
+    g = F.Graph(s)
+    sectors()
+    g.end()
+    for rep in range(3):
+        F.check(F.lib().fxc_memset_d(F.C.c_void_p(out.ptr), 0, 3 * F.N_PRODUCTS * h * 8,
+                                     F.C.c_void_p(s.ptr)))
+        s.sync()
+        g.launch()
+        s.sync()
+        got = out.download(np.zeros(3 * F.N_PRODUCTS * h))
+        assert np.array_equal(got, eager), rep
+        # eager work on the same plan between replays
+        p.sector_run_device(r1.ptr, r2.ptr, 0, nf_sec // 2, cfg, other.ptr, stream=s.ptr)
+        s.sync()
+    g.close()
+    assert np.any(eager != 0)
