@@ -55,7 +55,7 @@ def geometric_table(ra, dec, mjd0, nsec):
     return np.array(out)
 
 
-def c2_setup(seed=2025):
+def workload_setup(seed=2025):
     rng = np.random.default_rng(seed)
     ra, dec = rng.uniform(0, 2 * math.pi), math.radians(rng.uniform(-30, 80))
     table = geometric_table(ra, dec, 60000.25, int(SECONDS))
@@ -76,7 +76,7 @@ def c2_setup(seed=2025):
     return table, cfg, shuffle_b, residual
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_c2.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_1024msps_2bit_fft1024.json")
 KERNEL_KEYS = {"chosen": "k_sector_mf2", "plain": "k_sector<8, 64, 0>"}
 
 
@@ -229,7 +229,7 @@ def main():
     sectors = F.sector_partition(total_frames, N / FS)           # 8 x 1,000,000
     bpf = N * BITS // 8
     samples = total_frames * N
-    table, dcfg, shuffle_b, residual = c2_setup()
+    table, dcfg, shuffle_b, residual = workload_setup()
 
     stream = F.Stream()
     raw1 = F.DeviceBuffer(total_frames * bpf)

@@ -152,7 +152,11 @@ struct fxc_plan {
                        // 2: FP32 four-step through HBM scratch (N = 32768, 65536)
     // large-N scratch (fft_form 2): chunk of `chunk` frames
     int M1 = 0, M2 = 0;
-    int64_t chunk = 0;
+    int64_t chunk = 0;       // allocated chunk (frames)
+    int64_t chunk_use = 0;   // chunk in use (<= chunk; autotuned)
+    // autotuned choices of the Stockham form
+    int variant = 0;         // FFT pass sequence (0 default, 1 radix-16 passes)
+    int grid_div = 1;        // workgroups = max_wg / grid_div
     DevBuf tw_l1, tw_l2, tw_lm, d_x[2], d_y, d_z[2], d_params, d_lshifts;
     // scratch
     DevBuf part, red_scratch, table_dev, d_raw[2], d_acc, d_raw_out, d_tmp_in, d_tmp_out,
@@ -461,6 +465,7 @@ static void fill_args(fxc_plan *p, SectorArgs &a) {
     if (p->desc.products & FXC_PRODUCT_ACF_OVERLAP_ONLY) fl |= FLAG_ACF_OVERLAP;
     a.flags = (int32_t)fl;
     a.mf_tab = p->mf_tab.p;
+    a.variant = p->variant;
 }
 
 // IEEE binary16 bits of a float, round to nearest even (normals, subnormals,
@@ -524,7 +529,7 @@ static bool mfma_eligible(const fxc_plan *p) {
 // Tables of the MFMA four-step kernel (fxc_mfma.hip), MFT_* layout, all in
 // [idx][lane] order (lane l: g = l/16, c = l%16).
 //  B1 idx (kap*4 + t)*2 + lohi: half8 slot s of B[K][col] with K = (m1 =
-//     16 kap + 4 g + s/2, e = s%2: re|im of z), col = (k1 = c + 16(t&1),
+//     16 kap + g + 4 (s/2), e = s%2: re|im of z), col = (k1 = c + 16(t&1),
 //     part = t/2: re|im of Y); B = 1/2 * DFT-32 (the realfft split's 1/2),
 //     stored as hi + 2^-11 lo.
 //  A2 idx rt*8 + b: DFT-16 coefficient of output (k2 = c, rt: re|im) and
@@ -545,7 +550,7 @@ static std::vector<uint8_t> mfma_tables(const fxc_plan *p) {
                 for (int s = 0; s < 8; ++s) {
                     const int g = l >> 4, c = l & 15;
                     const int k1 = c + 16 * (t & 1), part = t >> 1;
-                    const int m1 = 16 * kap + 4 * g + (s >> 1), e = s & 1;
+                    const int m1 = 16 * kap + g + 4 * (s >> 1), e = s & 1;
                     const int r = (m1 * k1) % 32;
                     double cs, sn;
                     if (r % 8 == 0) {   // exact quadrant values
@@ -743,7 +748,8 @@ extern "C" fxc_status fxc_plan_create(const fxc_plan_desc *desc, fxc_plan **out)
                    hipMemcpy(b.p, t.data(), t.size() * 4, hipMemcpyHostToDevice) == hipSuccess;
         };
         const size_t C = (size_t)p->chunk;
-        if (!table(p->tw_l1, p->M1) || !table(p->tw_l2, p->M2) || !table(p->tw_lm, (int)M) ||
+        p->chunk_use = p->chunk;
+        if (!table(p->tw_l1, 128) || !table(p->tw_l2, 256) || !table(p->tw_lm, (int)M) ||
             p->d_x[0].ensure(C * N * 4) != hipSuccess || p->d_x[1].ensure(C * N * 4) != hipSuccess ||
             p->d_y.ensure(C * M * 8) != hipSuccess || p->d_z[0].ensure(C * M * 8) != hipSuccess ||
             p->d_z[1].ensure(C * M * 8) != hipSuccess ||
@@ -976,9 +982,10 @@ static fxc_status enqueue_phased(fxc_plan *p, const SectorArgs &base, int64_t nf
 // in order, then one fixed-order reduce.
 static fxc_status enqueue_large(fxc_plan *p, const SectorArgs &base, int64_t nf, double *d_acc,
                                 hipStream_t s) {
-    const int M = p->M;
-    for (int64_t c0 = 0; c0 < nf; c0 += p->chunk) {
-        const int64_t cn = std::min(p->chunk, nf - c0);
+    const float2 *twl1 = (const float2 *)(p->M1 == 128 ? p->tw_l1.p : p->tw_l2.p);
+    const float2 *twl2 = (const float2 *)(p->M2 == 128 ? p->tw_l1.p : p->tw_l2.p);
+    for (int64_t c0 = 0; c0 < nf; c0 += p->chunk_use) {
+        const int64_t cn = std::min(p->chunk_use, nf - c0);
         SectorArgs a = base;
         a.raw1 = base.raw1 + c0 * p->st[0].bpf;
         a.raw2 = base.raw2 + c0 * p->st[1].bpf;
@@ -991,15 +998,13 @@ static fxc_status enqueue_large(fxc_plan *p, const SectorArgs &base, int64_t nf,
         for (int st = 0; st < 2; ++st) {
             HIP_TRY(launch_decode_frames(a, st, shifts + st, 2, (float *)p->d_x[st].p, s));
             HIP_TRY(launch_fft_large((const float *)p->d_x[st].p, (float2 *)p->d_y.p,
-                                     (float2 *)p->d_z[st].p, cn, p->M1, p->M2,
-                                     (const float2 *)p->tw_l1.p, (const float2 *)p->tw_l2.p,
+                                     (float2 *)p->d_z[st].p, cn, p->M1, p->M2, twl1, twl2,
                                      (const float2 *)p->tw_lm.p, s));
         }
         HIP_TRY(launch_mac_large(a, p->desc.precision, (const float2 *)p->d_z[0].p,
                                  (const float2 *)p->d_z[1].p, params, kLargeGroups, c0 > 0 ? 1 : 0,
                                  s));
     }
-    (void)M;
     HIP_TRY(launch_reduce((const double *)p->part.p, kLargeGroups, p->half, d_acc,
                           (double *)p->red_scratch.p, 64, s));
     return FXC_OK;
@@ -1029,7 +1034,8 @@ static fxc_status enqueue_sector(fxc_plan *p, const uint8_t *d_raw1, const uint8
     a.nf = nf;
     if (phased) return enqueue_phased(p, a, nf, d_raw_out, d_acc, s);
     if (p->fft_form == 2) return enqueue_large(p, a, nf, d_acc, s);
-    int64_t fpw = (nf + p->max_wg - 1) / p->max_wg;
+    const int wg_cap = std::max(1, p->max_wg / p->grid_div);
+    int64_t fpw = (nf + wg_cap - 1) / wg_cap;
     const int batch = p->fft_form == 1 ? 8 : kFrameBatch;   // MFMA form: whole frames per wave
     fpw = ((fpw + batch - 1) / batch) * batch;
     if (fpw < batch) fpw = batch;
@@ -1058,6 +1064,127 @@ static fxc_status enqueue_sector(fxc_plan *p, const uint8_t *d_raw1, const uint8
     }
     HIP_TRY(launch_reduce((const double *)p->part.p, n_wg, p->half, d_acc,
                           (double *)p->red_scratch.p, 64, s));
+    return FXC_OK;
+}
+
+extern "C" fxc_status fxc_plan_autotune(fxc_plan *p, const uint8_t *d_raw1, const uint8_t *d_raw2,
+                                        int64_t nf, const fxc_delay_cfg *delay, void *stream,
+                                        fxc_tune_report *rep) {
+    if (!p || !d_raw1 || !d_raw2 || !delay || nf <= 0)
+        return fail(FXC_ERR_INVALID_ARG, "bad autotune arguments");
+    if (p->desc.products & (FXC_PRODUCT_PHASED | FXC_PRODUCT_RAW_OUT))
+        return fail(FXC_ERR_UNSUPPORTED, "autotune covers the ACF/XCF sector path");
+    hipStream_t s = (hipStream_t)stream;
+    const size_t acc_bytes = (size_t)kNumProducts * p->half * sizeof(double);
+    if (p->d_acc.ensure(acc_bytes) != hipSuccess) return fail(FXC_ERR_NOMEM, "device allocation failed");
+    struct Cand {
+        int variant, grid_div, m1;
+        int64_t chunk;
+    };
+    std::vector<Cand> cands;
+    if (p->fft_form == 2) {
+        const int M = p->M;
+        std::vector<int> m1s = {M / 128};
+        if (M / 128 != 128) m1s.push_back(128);
+        for (int m1 : m1s)
+            for (int64_t c : {p->chunk, p->chunk / 2, p->chunk / 4})
+                if (c >= 16) cands.push_back({0, 1, m1, c});
+    } else if (p->fft_form == 0) {
+        const int nv = sector_has_variant(p->N) ? 2 : 1;
+        for (int v = 0; v < nv; ++v)
+            for (int d : {1, 2, 4}) cands.push_back({v, d, 0, 0});
+    } else {
+        cands.push_back({0, 1, 0, 0});
+    }
+    const int save_variant = p->variant, save_div = p->grid_div, save_m1 = p->M1, save_m2 = p->M2;
+    const int64_t save_chunk = p->chunk_use;
+    const bool save_timing = p->timing;
+    p->timing = false;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    double best = 1e300, dflt = 0.0;
+    size_t best_i = 0;
+    fxc_status st = FXC_OK;
+    for (size_t i = 0; i < cands.size() && st == FXC_OK; ++i) {
+        const Cand &c = cands[i];
+        p->variant = c.variant;
+        p->grid_div = c.grid_div;
+        if (p->fft_form == 2) {
+            p->M1 = c.m1;
+            p->M2 = p->M / c.m1;
+            p->chunk_use = c.chunk;
+        }
+        double t[3];
+        for (int r = 0; r < 4 && st == FXC_OK; ++r) {   // run 0 warms up
+            if (hipEventRecord(e0, s) != hipSuccess) st = fail(FXC_ERR_HIP, "event record failed");
+            if (st == FXC_OK)
+                st = enqueue_sector(p, d_raw1, d_raw2, 0, nf, *delay, (double *)p->d_acc.p, nullptr, s);
+            if (st == FXC_OK && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess))
+                st = fail(FXC_ERR_HIP, "event timing failed");
+            float ms = 0.f;
+            if (st == FXC_OK && r > 0) {
+                (void)hipEventElapsedTime(&ms, e0, e1);
+                t[r - 1] = ms;
+            }
+        }
+        if (st != FXC_OK) break;
+        std::sort(t, t + 3);
+        if (i == 0) dflt = t[1];
+        if (t[1] < best) {
+            best = t[1];
+            best_i = i;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    p->timing = save_timing;
+    if (st != FXC_OK) {
+        p->variant = save_variant;
+        p->grid_div = save_div;
+        p->M1 = save_m1;
+        p->M2 = save_m2;
+        p->chunk_use = save_chunk;
+        return st;
+    }
+    const Cand &c = cands[best_i];
+    p->variant = c.variant;
+    p->grid_div = c.grid_div;
+    if (p->fft_form == 2) {
+        p->M1 = c.m1;
+        p->M2 = p->M / c.m1;
+        p->chunk_use = c.chunk;
+    }
+    if (rep) {
+        rep->candidates = (int32_t)cands.size();
+        rep->variant = p->variant;
+        rep->grid_divisor = p->grid_div;
+        rep->split_m1 = p->fft_form == 2 ? p->M1 : 0;
+        rep->chunk_frames = p->fft_form == 2 ? p->chunk_use : 0;
+        rep->best_ms = best;
+        rep->default_ms = dflt;
+    }
+    return FXC_OK;
+}
+
+extern "C" fxc_status fxc_plan_set_tuning(fxc_plan *p, const fxc_tune_report *c) {
+    if (!p || !c) return fail(FXC_ERR_INVALID_ARG, "null argument");
+    if (c->variant != 0 && !(c->variant == 1 && p->fft_form == 0 && sector_has_variant(p->N)))
+        return fail(FXC_ERR_INVALID_ARG, "variant %d not available for this plan", c->variant);
+    if (c->grid_divisor != 1 && c->grid_divisor != 2 && c->grid_divisor != 4)
+        return fail(FXC_ERR_INVALID_ARG, "grid_divisor must be 1, 2 or 4");
+    if (p->fft_form == 2) {
+        if ((c->split_m1 != 128 && c->split_m1 != 256) || p->M % c->split_m1 != 0 ||
+            (p->M / c->split_m1 != 128 && p->M / c->split_m1 != 256))
+            return fail(FXC_ERR_INVALID_ARG, "split_m1 %d invalid for M = %d", c->split_m1, p->M);
+        if (c->chunk_frames < 1 || c->chunk_frames > p->chunk)
+            return fail(FXC_ERR_INVALID_ARG, "chunk_frames outside 1..%lld", (long long)p->chunk);
+        p->M1 = c->split_m1;
+        p->M2 = p->M / c->split_m1;
+        p->chunk_use = c->chunk_frames;
+    }
+    p->variant = c->variant;
+    p->grid_div = c->grid_divisor;
     return FXC_OK;
 }
 
@@ -1195,7 +1322,8 @@ static fxc_status run_fft(fxc_plan *p, const float *in, int64_t nframes, float *
             const int64_t cn = std::min(p->chunk, nframes - c0);
             HIP_TRY(launch_fft_large((const float *)p->d_tmp_in.p + c0 * p->N, (float2 *)p->d_y.p,
                                      (float2 *)p->d_z[0].p, cn, p->M1, p->M2,
-                                     (const float2 *)p->tw_l1.p, (const float2 *)p->tw_l2.p,
+                                     (const float2 *)(p->M1 == 128 ? p->tw_l1.p : p->tw_l2.p),
+                                     (const float2 *)(p->M2 == 128 ? p->tw_l1.p : p->tw_l2.p),
                                      (const float2 *)p->tw_lm.p, s));
             HIP_TRY(launch_untangle_large((const float2 *)p->d_z[0].p, cn, p->M,
                                           (const float2 *)p->tw_post.p,

@@ -208,20 +208,28 @@ static hipError_t launch_sector_t(const SectorArgs &a, int precision, int n_wg, 
     return hipGetLastError();
 }
 
+// a.variant 1 selects radix-16 passes with half the threads per station
+// (N = 2048..8192): a different FFT split, chosen by the plan autotuner.
 hipError_t launch_sector(const SectorArgs &a, int precision, int n_wg, hipStream_t s) {
+    const bool v16 = a.variant == 1;
     switch (a.N) {
     case 64: return launch_sector_t<2, 16>(a, precision, n_wg, s);
     case 128: return launch_sector_t<2, 32>(a, precision, n_wg, s);
     case 256: return launch_sector_t<2, 64>(a, precision, n_wg, s);
     case 512: return launch_sector_t<4, 64>(a, precision, n_wg, s);
     case 1024: return launch_sector_t<8, 64>(a, precision, n_wg, s);
-    case 2048: return launch_sector_t<8, 128>(a, precision, n_wg, s);
-    case 4096: return launch_sector_t<8, 256>(a, precision, n_wg, s);
-    case 8192: return launch_sector_t<8, 512>(a, precision, n_wg, s);
+    case 2048: return v16 ? launch_sector_t<16, 64>(a, precision, n_wg, s)
+                          : launch_sector_t<8, 128>(a, precision, n_wg, s);
+    case 4096: return v16 ? launch_sector_t<16, 128>(a, precision, n_wg, s)
+                          : launch_sector_t<8, 256>(a, precision, n_wg, s);
+    case 8192: return v16 ? launch_sector_t<16, 256>(a, precision, n_wg, s)
+                          : launch_sector_t<8, 512>(a, precision, n_wg, s);
     case 16384: return launch_sector_t<16, 512>(a, precision, n_wg, s);
     default: return hipErrorInvalidValue;
     }
 }
+
+bool sector_has_variant(int N) { return N >= 2048 && N <= 8192; }
 
 // ------------------------------------------ per-function entry kernels
 // Decode (+ shift) of one frame per workgroup into out[f*N + n], through the

@@ -77,6 +77,7 @@ struct SectorArgs {
     double *part;            // [n_wg][kNumProducts][half]
     const void *mf_tab;      // chosen-form MFMA tables (N = 1024), layout MFT_* below
     float2 *cb_out;          // phased: combined half spectra [frame][half] for the .raw output
+    int32_t variant;         // FFT pass sequence of the plain kernel (autotuned; 0 default)
 };
 
 // Chosen-form MFMA tables (fxc_mfma.hip), byte offsets into one device
@@ -97,6 +98,7 @@ hipError_t launch_reduce(const double *part, int n_wg, int half, double *out,
                          double *scratch, int n_chunks, hipStream_t stream);
 int sector_threads_per_frame(int N);
 bool sector_supported(int N);
+bool sector_has_variant(int N);
 
 hipError_t launch_decode_frames(const SectorArgs &a, int station,
                                 const int64_t *shifts, int shift_stride, float *out,

@@ -179,25 +179,26 @@ __device__ __forceinline__ void mf_stage(const uint32_t w[4], int bits, int kind
 
 // Stage-1 A operands of one station-frame.  Lane l (m2 = l%16, g = l/16)
 // holds, in K-step kap, slots s = 2j + e: sample 2(16 m1 + m2) + e of the
-// shifted frame with m1 = 16 kap + 4 g + j, i.e. the FP16 pair at staged
-// position PAD + 32 m1 + 2 m2 - sh.  Positions outside the staged row read
-// the zero padding.
+// shifted frame with m1 = 16 kap + g + 4j, i.e. the FP16 pair at staged
+// position PAD + 32 m1 + 2 m2 - sh.  (The lane groups g of a half-wave then
+// read 16 dwords apart: no LDS bank conflict.)  Positions outside the
+// staged row read the zero padding.
 __device__ __forceinline__ void mf_build_a(const uint16_t *row, int sh, int l, half8 &a0, half8 &a1) {
     const uint32_t *r32 = reinterpret_cast<const uint32_t *>(row);
-    const int base = mf::PAD + 2 * (l & 15) - sh + 128 * (l >> 4);
+    const int base = mf::PAD + 2 * (l & 15) - sh + 32 * (l >> 4);
     uint32_t d[8];
     const bool near = (sh >= -mf::PAD) && (sh <= mf::PAD);
     if ((sh & 1) == 0) {
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
-            int p = base + 512 * (k >> 2) + 32 * (k & 3);
+            int p = base + 512 * (k >> 2) + 128 * (k & 3);
             if (!near) p = (p >= 0 && p < mf::SROW - 1) ? p : 0;
             d[k] = r32[p >> 1];
         }
     } else {
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
-            int p = base + 512 * (k >> 2) + 32 * (k & 3);
+            int p = base + 512 * (k >> 2) + 128 * (k & 3);
             if (!near) p = (p >= 0 && p < mf::SROW - 1) ? p : 1;
             d[k] = __builtin_amdgcn_alignbyte(r32[(p + 1) >> 1], r32[(p - 1) >> 1], 2);
         }
@@ -351,10 +352,20 @@ __global__ __launch_bounds__(512) void k_sector_mf2(SectorArgs a) {
     const int64_t f0 = wg0 + per_wave * wv;
     const int64_t f1 = (f0 + per_wave < wg1) ? f0 + per_wave : wg1;
 
-    uint32_t cur1[4] = {0, 0, 0, 0}, cur2[4] = {0, 0, 0, 0};
+    // Software pipeline: the staging rows hold frame f while its FFTs run;
+    // frame f+1 is decoded into them during frame f's products (disjoint LDS
+    // regions), and the words of frame f+2 are in flight from HBM.
+    uint32_t cur1[4] = {0, 0, 0, 0}, cur2[4] = {0, 0, 0, 0};   // words of the next frame
     if (f0 < f1) {
         mf_load(a.raw1 + f0 * bpf1, nw1, l, cur1);
         mf_load(a.raw2 + f0 * bpf2, nw2, l, cur2);
+        mf_stage(cur1, bits1, kind1, nw1, sh.slut[0], sh.dlut[0], row1 + mf::PAD, l);
+        mf_stage(cur2, bits2, kind2, nw2, sh.slut[1], sh.dlut[1], row2 + mf::PAD, l);
+        if (f0 + 1 < f1) {
+            mf_load(a.raw1 + (f0 + 1) * bpf1, nw1, l, cur1);
+            mf_load(a.raw2 + (f0 + 1) * bpf2, nw2, l, cur2);
+        }
+        wave_lds_sync();
     }
     for (int64_t fb = f0; fb < f1; fb += mf::FB) {
         const int nb = (int)((f1 - fb) < mf::FB ? (f1 - fb) : mf::FB);
@@ -363,17 +374,13 @@ __global__ __launch_bounds__(512) void k_sector_mf2(SectorArgs a) {
         for (int jf = 0; jf < nb; ++jf) {
             const int64_t f = fb + jf;
             uint32_t nxt1[4] = {0, 0, 0, 0}, nxt2[4] = {0, 0, 0, 0};
-            if (f + 1 < f1) {
-                mf_load(a.raw1 + (f + 1) * bpf1, nw1, l, nxt1);
-                mf_load(a.raw2 + (f + 1) * bpf2, nw2, l, nxt2);
+            if (f + 2 < f1) {
+                mf_load(a.raw1 + (f + 2) * bpf1, nw1, l, nxt1);
+                mf_load(a.raw2 + (f + 2) * bpf2, nw2, l, nxt2);
             }
             const int s1 = __builtin_amdgcn_readlane(pf.s1, jf);
             const int s2 = __builtin_amdgcn_readlane(pf.s2, jf);
 
-            // ---- decode both stations into their staging rows
-            mf_stage(cur1, bits1, kind1, nw1, sh.slut[0], sh.dlut[0], row1 + mf::PAD, l);
-            mf_stage(cur2, bits2, kind2, nw2, sh.slut[1], sh.dlut[1], row2 + mf::PAD, l);
-            wave_lds_sync();
             half8 a10, a11, a20, a21;
             mf_build_a(row1, s1, l, a10, a11);
             mf_build_a(row2, s2, l, a20, a21);
@@ -392,15 +399,22 @@ __global__ __launch_bounds__(512) void k_sector_mf2(SectorArgs a) {
             // ---- twiddle + stage 2, Z/2 to LDS
             mf_stage2(y1, sh.tw2, sh.a2, z1, l);
             mf_stage2(y2, sh.tw2, sh.a2, z2, l);
-            wave_lds_sync();
+            wave_lds_sync();   // Z visible; this frame's staging rows no longer read
 
-            // ---- XCF phasor of this lane's first bin (FP64 angle reduced
-            // mod 1 turn before the f32 sincos)
+            // ---- decode frame f+1 into the staging rows (overlaps the products)
+            if (f + 1 < f1) {
+                mf_stage(cur1, bits1, kind1, nw1, sh.slut[0], sh.dlut[0], row1 + mf::PAD, l);
+                mf_stage(cur2, bits2, kind2, nw2, sh.slut[1], sh.dlut[1], row2 + mf::PAD, l);
+            }
+
+            // ---- XCF phasor of this lane's first bin: FP64 angle reduced mod
+            // 1 turn, then the hardware sin/cos of 2 pi r (|r| <= 1/2 turn)
             const double phw = readlane_d(pf.phw, jf);
             const double dth = readlane_d(pf.dth, jf);
             const float2 F1 = make_float2(readlane_f(pf.F1.x, jf), readlane_f(pf.F1.y, jf));
             const float2 Cm = make_float2(readlane_f(pf.C.x, jf), readlane_f(pf.C.y, jf));
-            float2 P = sincos_turns((float)wrap_turns(fma((double)l, dth, phw)));
+            const float r0 = (float)wrap_turns(fma((double)l, dth, phw));
+            float2 P = make_float2(__builtin_amdgcn_cosf(r0), __builtin_amdgcn_sinf(r0));
 
             // ---- products on the output grid (main.rs:2491-2568)
 #pragma unroll

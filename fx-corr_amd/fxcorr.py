@@ -716,3 +716,38 @@ class Graph:
         if self.h:
             _glib().fxc_graph_destroy(self.h)
             self.h = None
+
+
+# ---------------------------------------------------------------- autotune
+class TuneReport(C.Structure):
+    _fields_ = [("candidates", C.c_int32), ("variant", C.c_int32), ("grid_divisor", C.c_int32),
+                ("split_m1", C.c_int32), ("chunk_frames", C.c_int64), ("best_ms", C.c_double),
+                ("default_ms", C.c_double)]
+
+
+EXPORTED += ["fxc_plan_autotune"]
+
+
+def plan_autotune(plan: Plan, d_raw1: int, d_raw2: int, nf: int, cfg: DelayCfg, stream=None):
+    L = lib()
+    f = L.fxc_plan_autotune
+    f.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, _P(DelayCfg), C.c_void_p,
+                  _P(TuneReport)]
+    f.restype = C.c_int32
+    rep = TuneReport()
+    check(f(plan.h, C.c_void_p(d_raw1), C.c_void_p(d_raw2), nf, C.byref(cfg),
+            C.c_void_p(stream or 0), C.byref(rep)))
+    return {k: getattr(rep, k) for k, _ in TuneReport._fields_}
+
+
+EXPORTED += ["fxc_plan_set_tuning"]
+
+
+def plan_set_tuning(plan: Plan, variant=0, grid_divisor=1, split_m1=0, chunk_frames=0):
+    L = lib()
+    f = L.fxc_plan_set_tuning
+    f.argtypes = [C.c_void_p, _P(TuneReport)]
+    f.restype = C.c_int32
+    r = TuneReport()
+    r.variant, r.grid_divisor, r.split_m1, r.chunk_frames = variant, grid_divisor, split_m1, chunk_frames
+    check(f(plan.h, C.byref(r)))

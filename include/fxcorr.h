@@ -4,8 +4,8 @@
  * Replaces the per-sector compute of M-AKIMOTOO/fx-corr (src/main.rs:2173-2591)
  * and the per-frame helpers it calls (src/utils.rs, src/main.rs:110-241,
  * src/args.rs:194-210).  The reference is Rust with no FFI; the entry points
- * below are what an FFI crate for that seam would bind (see INTEGRATION.md for
- * the Rust `extern "C"` block).  Plain pointers and sizes only.
+ * below are what an FFI crate for that seam (a Rust `extern "C"` block)
+ * would bind.  Plain pointers and sizes only.
  *
  * Conventions
  *   - Every function returns an fxc_status (0 = FXC_OK).  No exceptions cross
@@ -261,6 +261,30 @@ fxc_status fxc_synth_station_device(uint8_t *d_out, int64_t nsamples, int32_t bi
 
 /* Device memory held by a plan (tables + scratch), bytes. */
 fxc_status fxc_plan_device_bytes(const fxc_plan *plan, uint64_t *bytes);
+
+/* Autotuner (no reference counterpart).  Times the plan's candidate launch
+ * configurations on nf frames of the caller's device data (median of three
+ * runs each, HIP events on `stream`) and keeps the fastest in the plan for
+ * later calls.  Candidates: the FFT pass sequence (radix-8 or radix-16
+ * passes, N = 2048..8192) and workgroups per launch (frames per workgroup)
+ * for the Stockham form; the four-step split M1 x M2 and the chunk length
+ * for N >= 32768.  The MFMA form's shape is fixed by its LDS budget and is
+ * reported as a single candidate.  Results are the same sums up to FP64
+ * summation order. */
+typedef struct {
+    int32_t candidates;
+    int32_t variant;           /* 0: default pass sequence, 1: radix-16 passes */
+    int32_t grid_divisor;      /* workgroups = default / grid_divisor */
+    int32_t split_m1;          /* N >= 32768: column DFT length M1 (else 0) */
+    int64_t chunk_frames;      /* N >= 32768: frames per chunk (else 0) */
+    double best_ms, default_ms;
+} fxc_tune_report;
+fxc_status fxc_plan_autotune(fxc_plan *plan, const uint8_t *d_raw1, const uint8_t *d_raw2,
+                             int64_t nf, const fxc_delay_cfg *delay, void *stream,
+                             fxc_tune_report *report);
+/* Applies a configuration (variant, grid_divisor, split_m1, chunk_frames of
+ * `cfg`; other fields ignored), e.g. one saved from an earlier autotune. */
+fxc_status fxc_plan_set_tuning(fxc_plan *plan, const fxc_tune_report *cfg);
 
 /* HIP-graph capture of device-entry work (no reference counterpart: replays
  * a fixed launch sequence, e.g. the sectors of a scan, without per-launch

@@ -1,6 +1,6 @@
 """GPU parity of the HIP hot path against the oracle, called through the C ABI.
 
-Bars (north star): decode, integer shifts and delay integers bit-exact;
+Bars: decode, integer shifts and delay integers bit-exact;
 plain FP32 spectra within 1e-5 relative of the reference-faithful oracle;
 chosen form within 1e-4 relative of the FP64 oracle."""
 import numpy as np
@@ -155,8 +155,8 @@ def _compare(got, ref, tol, keys=("acc11", "acc22", "acc12")):
         assert errs[k] < tol, (k, errs[k])
 
 
-def test_c1_full_plain_and_chosen():
-    """SURVEY C1 shape in full: 64 Msps 2-bit, N=1024, 62,499 frames, USB,
+def test_64msps_2bit_fft1024_full_plain_and_chosen():
+    """64 Msps 2-bit two-station case in full: N=1024, 62,499 frames, USB,
     37.3-sample delay (seek 32 -> int1 = 5, frac1 = 0.3), fringe at 6.7 GHz."""
     fs, n = 64e6, 1024
     c = make_case(n=n, fs=fs, nf=62499, rho=0.02, delay_samples=37.3 - 32, seed=0,
@@ -202,8 +202,8 @@ def test_sector_sizes(n):
     _compare(gpu_run(c, F.PRECISION_CHOSEN), o64, CHOSEN_TOL)
 
 
-def test_c2_shape_lsb_shuffle_geometry():
-    """C2-like: 1024 Msps 2-bit N=1024, B LSB with a seeded shuffle, geometric
+def test_1024msps_2bit_fft1024_lsb_shuffle_geometry():
+    """1024 Msps 2-bit N=1024 shape: B LSB with a seeded shuffle, geometric
     table + clock 1.2 us + rate 1e-12 + residuals; a 20000-frame slice of
     sector 3 (first_frame != 0)."""
     fs, n = 1024e6, 1024
@@ -222,7 +222,7 @@ def test_c2_shape_lsb_shuffle_geometry():
 @pytest.mark.parametrize("grid", [0, 1])
 @pytest.mark.parametrize("overlap_only", [False, True])
 def test_band_shift_grids(grid, overlap_only):
-    """Mixed bits (4/1) with a band shift as in C3, scaled to N=4096."""
+    """Mixed bits (4/1) with a band shift, scaled to N=4096."""
     n, fs = 4096, 512e6
     sb = -128
     band = (sb, 128, 2048, 0) if grid == 0 else (-sb, 0, 2048 - 128, 128)
@@ -452,8 +452,8 @@ def test_rfft_large_vs_numpy(n):
     assert np.all(got[:, 0].imag == 0) and np.all(got[:, -1].imag == 0)
 
 
-def test_c3_shape_4bit_1bit_band_shift():
-    """C3 shape: 512 Msps, A 4-bit (16 uniform levels) / B 1-bit, FFT 65536,
+def test_512msps_4bit_1bit_fft65536_band_shift():
+    """512 Msps, A 4-bit (16 uniform levels) / B 1-bit, FFT 65536,
     16 MHz centre offset -> band shift 2048 bins, overlap 30720 (main.rs:
     360-372), both output grids' bins; 40 frames of it."""
     n, fs = 65536, 512e6
@@ -502,3 +502,53 @@ def test_large_chunking_is_a_sum():
     scale = np.abs(full).max(axis=1, keepdims=True)
     assert np.all(np.abs(full - parts) <= 1e-11 * scale + 1e-300)
     assert np.all(full[1] > 0)
+
+
+# ---------------------------------------------------------------- autotuner
+@pytest.mark.parametrize("n,variant,div", [(2048, 1, 1), (4096, 1, 2), (8192, 0, 4), (8192, 1, 1)])
+def test_tuning_candidates_keep_parity(n, variant, div):
+    """Every configuration the autotuner can pick gives the same spectra."""
+    c = make_case(n=n, fs=128e6, nf=max(128, 2 ** 19 // n), rho=0.5, seed=n + variant,
+                  delay_user_samples=2.3, lo1_hz=3e9)
+    o64 = oracle_run(c, R.MODE_F64)
+    for prec in (F.PRECISION_PLAIN_FP32, F.PRECISION_CHOSEN):
+        p = gpu_plan(c, prec)
+        F.plan_set_tuning(p, variant=variant, grid_divisor=div)
+        g = gpu_run(c, plan=p)
+        if prec == F.PRECISION_PLAIN_FP32:
+            _check_plain(c, g, o64=o64)
+        else:
+            _compare(g, o64, CHOSEN_TOL)
+
+
+def test_large_split_and_chunk_candidates():
+    n, fs = 65536, 512e6
+    c = make_case(n=n, fs=fs, nf=40, bits=(4, 1), rho=0.3, seed=17, delay_user_samples=1.2,
+                  lo1_hz=8e9)
+    o64 = oracle_run(c, R.MODE_F64)
+    for m1, chunk in ((256, 12), (128, 40), (128, 7)):
+        p = gpu_plan(c, F.PRECISION_CHOSEN)
+        F.plan_set_tuning(p, split_m1=m1, chunk_frames=chunk)
+        _compare(gpu_run(c, plan=p), o64, CHOSEN_TOL)
+
+
+@pytest.mark.parametrize("n,prec", [(4096, F.PRECISION_PLAIN_FP32), (1024, F.PRECISION_CHOSEN),
+                                    (65536, F.PRECISION_CHOSEN)])
+def test_autotune_picks_a_candidate(n, prec):
+    fs = 256e6
+    nf = {4096: 20000, 1024: 50000, 65536: 300}[n]
+    s = F.Stream()
+    bpf = n * 2 // 8
+    r1, r2 = F.DeviceBuffer(nf * bpf), F.DeviceBuffer(nf * bpf)
+    F.synth_station_device(r1.ptr, nf * n, 2, None, False, 9, 0, 0.2, 3, s.ptr)
+    F.synth_station_device(r2.ptr, nf * n, 2, None, False, 9, 1, 0.2, 3, s.ptr)
+    s.sync()
+    p = F.Plan(n, levels=[LEVELS_2BIT, LEVELS_2BIT], df_hz=fs / n, precision=prec)
+    cfg = F.make_delay_cfg(np.zeros(4), frame_dt=n / fs, fs=fs, delay_user_samples=3.3,
+                           lo1_hz=3e9)
+    rep = F.plan_autotune(p, r1.ptr, r2.ptr, nf, cfg, s.ptr)
+    assert rep["candidates"] >= 1 and 0 < rep["best_ms"] <= rep["default_ms"]
+    if n == 4096:
+        assert rep["candidates"] == 6
+    if n == 65536:
+        assert rep["split_m1"] in (128, 256) and rep["chunk_frames"] > 0

@@ -173,7 +173,7 @@ def test_reader_padding_and_offset(tmp_path):
 
 
 @pytest.mark.parametrize("start,bits,exp", [
-    (37, 2, (32, 8, -5)),        # C1: B seeks 37 samples -> word-aligned 32, int shift 5
+    (37, 2, (32, 8, -5)),        # B seeks 37 samples -> word-aligned 32, int shift 5
     (37, 4, (32, 16, -5)),
     (37, 1, (32, 4, -5)),
     (37, 8, (36, 36, -1)),

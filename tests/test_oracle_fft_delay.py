@@ -108,7 +108,7 @@ def test_delay_table_linear_interpolation():
 
 
 def test_delay_c1_expectation():
-    # 37.3-sample delay, word-aligned seek of 32 -> int1 = 5, frac1 = 0.3 sample
+    # 64 Msps case: 37.3-sample delay, word-aligned seek of 32 -> int1 = 5, frac1 = 0.3 sample
     fs = 64e6
     cfg = _cfg(np.zeros(2), fs=fs, frame_dt=1024 / fs, delay_user_samples=37.3, d_seek=32 / fs)
     d = R.frame_delay(0, cfg)
