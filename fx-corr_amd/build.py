@@ -14,8 +14,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfxcorr.so")
 
-SOURCES = ["fxc_device.hip", "fxc_mfma.hip", "fxc_phased.hip", "fxc_large.hip", "fxc_synth.hip",
-           "fxc_api.cpp", "fxc_server.cpp",
+SOURCES = ["fxc_device.hip", "fxc_mfma.hip", "fxc_phased.hip", "fxc_large.hip", "fxc_chain.hip",
+           "fxc_synth.hip", "fxc_api.cpp", "fxc_server.cpp",
            "fxc_io.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

@@ -161,6 +161,7 @@ struct fxc_plan {
     // scratch
     DevBuf part, red_scratch, table_dev, d_raw[2], d_acc, d_raw_out, d_tmp_in, d_tmp_out,
         d_shifts, d_cb, d_tbuf;
+    DevBuf d_chain;          // the reference's f32 phase chains of one slab/chunk of frames
     int max_wg = 0;
     std::vector<double> table_host;  // last uploaded geometric table
     // timing
@@ -845,7 +846,7 @@ extern "C" fxc_status fxc_plan_device_bytes(const fxc_plan *p, uint64_t *bytes) 
                             &p->d_acc, &p->d_raw_out, &p->d_tmp_in, &p->d_tmp_out, &p->d_shifts,
                             &p->mf_tab, &p->d_cb, &p->d_tbuf, &p->tw_l1, &p->tw_l2, &p->tw_lm,
                             &p->d_x[0], &p->d_x[1], &p->d_y, &p->d_z[0], &p->d_z[1], &p->d_params,
-                            &p->d_lshifts};
+                            &p->d_lshifts, &p->d_chain};
     uint64_t t = 0;
     for (const DevBuf *b : bufs) t += b->cap;
     *bytes = t;
@@ -937,20 +938,30 @@ static int64_t plan_bpf_out(const fxc_plan *p) {
 // chunks of frames that reuse the same per-workgroup partial slots, and per
 // chunk, when the .raw output is requested, the inverse FFT (with the LSB
 // flip) and the quantiser into d_raw_out.
+// The plain form's XCF takes the reference's f32 phase chain (k_phase_chain)
+// when the overlap exceeds kChainMinOverlap bins.
+static bool xcf_uses_chain(const fxc_plan *p) {
+    return p->desc.precision == FXC_PRECISION_PLAIN_FP32 && (p->desc.products & FXC_PRODUCT_XCF) &&
+           (p->desc.a1e - p->desc.a1s) > kChainMinOverlap;
+}
+constexpr size_t kChainSlabBytes = (size_t)256 << 20;   // chain scratch of the fused plain path
+
 static fxc_status enqueue_phased(fxc_plan *p, const SectorArgs &base, int64_t nf,
                                  uint8_t *d_raw_out, double *d_acc, hipStream_t s) {
     const bool raw = (p->desc.products & FXC_PRODUCT_RAW_OUT) && d_raw_out != nullptr;
     const size_t half = (size_t)p->half;
     const int64_t bpf_o = plan_bpf_out(p);
-    int64_t chunk = nf;
-    if (raw) {
-        const int64_t cap = (int64_t)((size_t)256 << 20) / (int64_t)(half * 8 > (size_t)p->N * 4
-                                                                     ? half * 8 : (size_t)p->N * 4);
-        chunk = std::max<int64_t>(1, std::min<int64_t>(nf, cap));
-        if (p->d_cb.ensure((size_t)chunk * half * 8) != hipSuccess ||
-            p->d_tbuf.ensure((size_t)chunk * p->N * 4) != hipSuccess)
-            return fail(FXC_ERR_NOMEM, "phased scratch allocation failed");
-    }
+    // per-frame scratch: two rotation chains, and for the .raw output the
+    // combined half spectrum and the time series.  The chunk does not depend
+    // on whether .raw is written, so |cb|^2 sums in the same order either way.
+    const int ld = chain_ld_for((int)half);
+    const size_t per_frame = 2 * (size_t)ld * 8 + half * 8 + (size_t)p->N * 4;
+    const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(nf, (int64_t)(((size_t)256 << 20) / per_frame)));
+    if (p->d_chain.ensure((size_t)chunk * 2 * ld * 8) != hipSuccess)
+        return fail(FXC_ERR_NOMEM, "phased scratch allocation failed");
+    if (raw && (p->d_cb.ensure((size_t)chunk * half * 8) != hipSuccess ||
+                p->d_tbuf.ensure((size_t)chunk * p->N * 4) != hipSuccess))
+        return fail(FXC_ERR_NOMEM, "phased scratch allocation failed");
     int64_t fpw = (chunk + p->max_wg - 1) / p->max_wg;
     fpw = ((fpw + kFrameBatch - 1) / kFrameBatch) * kFrameBatch;
     if (fpw < kFrameBatch) fpw = kFrameBatch;
@@ -965,6 +976,9 @@ static fxc_status enqueue_phased(fxc_plan *p, const SectorArgs &base, int64_t nf
         a.frames_per_wg = fpw;
         a.part = (double *)p->part.p;
         a.cb_out = raw ? (float2 *)p->d_cb.p : nullptr;
+        a.chain = (const float2 *)p->d_chain.p;
+        a.chain_ld = ld;
+        HIP_TRY(launch_phase_chain(a, CHAIN_PHASED, (float2 *)p->d_chain.p, s));
         HIP_TRY(launch_phased(a, c0 > 0 ? 1 : 0, (int)((cn + fpw - 1) / fpw), s));
         if (raw) {
             HIP_TRY(launch_irfft_frames(a, (const float2 *)p->d_cb.p, cn, (float *)p->d_tbuf.p,
@@ -984,6 +998,10 @@ static fxc_status enqueue_large(fxc_plan *p, const SectorArgs &base, int64_t nf,
                                 hipStream_t s) {
     const float2 *twl1 = (const float2 *)(p->M1 == 128 ? p->tw_l1.p : p->tw_l2.p);
     const float2 *twl2 = (const float2 *)(p->M2 == 128 ? p->tw_l1.p : p->tw_l2.p);
+    const bool chain = xcf_uses_chain(p);
+    const int ld = chain ? chain_ld_for(base.ov) : 0;
+    if (chain && p->d_chain.ensure((size_t)p->chunk * ld * 8) != hipSuccess)
+        return fail(FXC_ERR_NOMEM, "phase chain allocation failed");
     for (int64_t c0 = 0; c0 < nf; c0 += p->chunk_use) {
         const int64_t cn = std::min(p->chunk_use, nf - c0);
         SectorArgs a = base;
@@ -1000,6 +1018,11 @@ static fxc_status enqueue_large(fxc_plan *p, const SectorArgs &base, int64_t nf,
             HIP_TRY(launch_fft_large((const float *)p->d_x[st].p, (float2 *)p->d_y.p,
                                      (float2 *)p->d_z[st].p, cn, p->M1, p->M2, twl1, twl2,
                                      (const float2 *)p->tw_lm.p, s));
+        }
+        if (chain) {
+            a.chain = (const float2 *)p->d_chain.p;
+            a.chain_ld = ld;
+            HIP_TRY(launch_phase_chain(a, CHAIN_XCF, (float2 *)p->d_chain.p, s));
         }
         HIP_TRY(launch_mac_large(a, p->desc.precision, (const float2 *)p->d_z[0].p,
                                  (const float2 *)p->d_z[1].p, params, kLargeGroups, c0 > 0 ? 1 : 0,
@@ -1035,11 +1058,22 @@ static fxc_status enqueue_sector(fxc_plan *p, const uint8_t *d_raw1, const uint8
     if (phased) return enqueue_phased(p, a, nf, d_raw_out, d_acc, s);
     if (p->fft_form == 2) return enqueue_large(p, a, nf, d_acc, s);
     const int wg_cap = std::max(1, p->max_wg / p->grid_div);
-    int64_t fpw = (nf + wg_cap - 1) / wg_cap;
+    // Plain XCF over more than kChainMinOverlap bins: slabs of frames whose
+    // f32 phase chains fit the chain scratch; the slabs add into the same
+    // workgroup slots in order.
+    const bool chain = p->fft_form == 0 && xcf_uses_chain(p);
+    const int ld = chain ? chain_ld_for(a.ov) : 0;
+    int64_t slab = nf;
+    if (chain) {
+        slab = std::min<int64_t>(nf, std::max<int64_t>(kFrameBatch, (int64_t)(kChainSlabBytes / ((size_t)ld * 8))));
+        if (p->d_chain.ensure((size_t)slab * ld * 8) != hipSuccess)
+            return fail(FXC_ERR_NOMEM, "phase chain allocation failed");
+    }
+    int64_t fpw = (slab + wg_cap - 1) / wg_cap;
     const int batch = p->fft_form == 1 ? 8 : kFrameBatch;   // MFMA form: whole frames per wave
     fpw = ((fpw + batch - 1) / batch) * batch;
     if (fpw < batch) fpw = batch;
-    const int n_wg = (int)((nf + fpw - 1) / fpw);
+    const int n_wg = (int)((slab + fpw - 1) / fpw);
     a.frames_per_wg = fpw;
     a.part = (double *)p->part.p;
     std::pair<hipEvent_t, hipEvent_t> ev{nullptr, nullptr};
@@ -1056,8 +1090,23 @@ static fxc_status enqueue_sector(fxc_plan *p, const uint8_t *d_raw1, const uint8
         }
         HIP_TRY(hipEventRecord(ev.first, s));
     }
-    if (p->fft_form == 1) HIP_TRY(launch_sector_mfma(a, p->desc.precision, n_wg, s));
-    else HIP_TRY(launch_sector(a, p->desc.precision, n_wg, s));
+    for (int64_t c0 = 0; c0 < nf; c0 += slab) {
+        const int64_t cn = std::min(slab, nf - c0);
+        SectorArgs b = a;
+        b.raw1 = a.raw1 + c0 * p->st[0].bpf;
+        b.raw2 = a.raw2 + c0 * p->st[1].bpf;
+        b.first_frame = a.first_frame + c0;
+        b.nf = cn;
+        const int wgs = (int)((cn + fpw - 1) / fpw);
+        if (chain) {
+            b.chain = (const float2 *)p->d_chain.p;
+            b.chain_ld = ld;
+            b.accumulate = c0 > 0 ? 1 : 0;
+            HIP_TRY(launch_phase_chain(b, CHAIN_XCF, (float2 *)p->d_chain.p, s));
+        }
+        if (p->fft_form == 1) HIP_TRY(launch_sector_mfma(b, p->desc.precision, wgs, s));
+        else HIP_TRY(launch_sector(b, p->desc.precision, wgs, s));
+    }
     if (p->timing) {
         HIP_TRY(hipEventRecord(ev.second, s));
         p->pending.push_back(ev);

@@ -95,17 +95,22 @@ __global__ __launch_bounds__(2 * T) void k_sector(SectorArgs a) {
                     }
                     if (in_ov && do_xcf) {
                         const int k = g - gs;
-                        if (!have) {
-                            P = fp[jf].start;
-                            // first overlap bin of this thread: k < NT
-#pragma unroll 1
-                            for (int b = 0; b < LOGNT; ++b)
-                                if ((k >> b) & 1) P = d_mul(P, fp[jf].pw[b]);
-                            have = true;
+                        float2 ph;
+                        if (PREC == 0 && a.chain != nullptr) {
+                            ph = a.chain[f * a.chain_ld + k];     // the reference's own chain
                         } else {
-                            P = d_mul(P, fp[jf].pw[LOGNT]);
+                            if (!have) {
+                                P = fp[jf].start;
+                                // first overlap bin of this thread: k < NT
+#pragma unroll 1
+                                for (int b = 0; b < LOGNT; ++b)
+                                    if ((k >> b) & 1) P = d_mul(P, fp[jf].pw[b]);
+                                have = true;
+                            } else {
+                                P = d_mul(P, fp[jf].pw[LOGNT]);
+                            }
+                            ph = make_float2((float)P.x, (float)P.y);
                         }
-                        const float2 ph = make_float2((float)P.x, (float)P.y);
                         const float2 X1 = (a.grid == 0) ? Xg : Xo;
                         const float2 X2 = (a.grid == 0) ? Xo : Xg;
                         float2 v2 = c_mul(X1, c_conj(X2));
@@ -127,11 +132,12 @@ __global__ __launch_bounds__(2 * T) void k_sector(SectorArgs a) {
     for (int i = 0; i < NB; ++i) {
         const int g = tid + NT * i;
         if (g <= M) {
+            const bool add = a.accumulate != 0;
             out[P_PH * half + g] = 0.0;
-            out[P_11 * half + g] = acc[0][i];
-            out[P_22 * half + g] = acc[1][i];
-            out[P_12R * half + g] = acc[2][i];
-            out[P_12I * half + g] = acc[3][i];
+            out[P_11 * half + g] = add ? out[P_11 * half + g] + acc[0][i] : acc[0][i];
+            out[P_22 * half + g] = add ? out[P_22 * half + g] + acc[1][i] : acc[1][i];
+            out[P_12R * half + g] = add ? out[P_12R * half + g] + acc[2][i] : acc[2][i];
+            out[P_12I * half + g] = add ? out[P_12I * half + g] + acc[3][i] : acc[3][i];
         }
     }
 }

@@ -78,7 +78,22 @@ struct SectorArgs {
     const void *mf_tab;      // chosen-form MFMA tables (N = 1024), layout MFT_* below
     float2 *cb_out;          // phased: combined half spectra [frame][half] for the .raw output
     int32_t variant;         // FFT pass sequence of the plain kernel (autotuned; 0 default)
+    // Plain form: the reference's sequential f32 phase chains of this
+    // launch's frames, row (frame * chains_per_frame + c) of chain_ld float2
+    // (k_phase_chain); null where the kernels take FP64 powers of the step.
+    const float2 *chain;
+    int32_t chain_ld;
+    int32_t accumulate;      // fused plain kernel: add to the partial slots (sector in slabs)
 };
+
+// Phase chains replayed by k_phase_chain (fxc_chain.hip).
+enum { CHAIN_XCF = 0, CHAIN_PHASED = 1 };
+// The plain XCF replays the chain above this many overlap bins.  Up to it,
+// FP64 powers of the same f32 step stay within 1.4e-5 of the chain over a
+// frame's bins (2.8e-6 rms; 2.6e-5 and 6.6e-6 at 1024 bins), which keeps the
+// fused N <= 1024 kernel free of a per-frame table.
+constexpr int kChainMinOverlap = 512;
+__host__ __device__ constexpr int chain_ld_for(int len) { return (len + 1) & ~1; }
 
 // Chosen-form MFMA tables (fxc_mfma.hip), byte offsets into one device
 // buffer.  [idx][lane] order so a wave's loads are contiguous.
@@ -121,6 +136,11 @@ int mfma_wg_threads();
 hipError_t launch_phased(const SectorArgs &a, int accumulate, int n_wg, hipStream_t stream);
 hipError_t launch_fill_zero(void *p, size_t bytes, hipStream_t stream);
 hipError_t launch_add_f64(double *dst, const double *src, size_t n, hipStream_t stream);
+
+// fxc_chain.hip: one thread per chain; frames [0, a.nf) of the launch,
+// rows of a.chain_ld float2 (one XCF chain of a.ov bins, or two phased
+// chains of M+1 bins, per frame).
+hipError_t launch_phase_chain(const SectorArgs &a, int kind, float2 *out, hipStream_t stream);
 
 // fxc_large.hip: N = 32768, 65536 through HBM scratch (four-step M = M1 x M2)
 struct FrameParam;

@@ -195,14 +195,20 @@ __global__ __launch_bounds__(256) void k_mac_large(SectorArgs a, const float2 *z
         }
         if (in_ov && do_xcf) {
             const FrameParam &P = params[f];
-            cd ph = P.start;
-            for (int b = 0; b < kMaxPow; ++b)
-                if ((kk >> b) & 1) ph = d_mul(ph, P.pw[b]);
+            float2 phf;
+            if (PREC == 0 && a.chain != nullptr) {
+                phf = a.chain[f * a.chain_ld + kk];            // the reference's own chain
+            } else {
+                cd ph = P.start;
+                for (int b = 0; b < kMaxPow; ++b)
+                    if ((kk >> b) & 1) ph = d_mul(ph, P.pw[b]);
+                phf = make_float2((float)ph.x, (float)ph.y);
+            }
             const float2 X1 = (a.grid == 0) ? Xg : Xo;
             const float2 X2 = (a.grid == 0) ? Xo : Xg;
             float2 v2 = c_mul(X1, c_conj(X2));
             if (PREC == 0) v2 = c_mul(v2, P.frmix);
-            v2 = c_mul(v2, make_float2((float)ph.x, (float)ph.y));
+            v2 = c_mul(v2, phf);
             acc[2] += (double)v2.x;
             acc[3] += (double)v2.y;
         }

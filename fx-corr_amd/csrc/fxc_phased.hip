@@ -1,7 +1,8 @@
 // Phased-array sum of the two stations (yi-phasedarray; main.rs:2222-2302,
 // 2391-2411): per frame, decode + integer shift + real FFT of both stations
 // (the fused sector kernel's front end), each station's fractional-delay
-// rotation (utils.rs:69-124 with rate = accel = t = 0, USB sign), fringe
+// rotation (utils.rs:69-124 with rate = accel = t = 0, USB sign; the f32
+// rotation chain itself comes from k_phase_chain, bit for bit), fringe
 // phasors, band alignment onto the output grid, cb = w1 s1 + w2 s2, |cb|^2
 // accumulated in FP64 and, for the .raw output, cb with DC/Nyquist imaginary
 // parts zeroed written out for the inverse FFT + quantise kernels.
@@ -12,14 +13,8 @@ namespace fxc {
 struct PhasedParam {
     int32_t s1, s2;        // integer shifts clamped to [-N, N]
     float2 fr1, fr2;       // fringe phasors (main.rs:235-240, f32)
-    cd pw[2][kMaxPow];     // per station: rotation step (the reference's f32 value) ^ 2^b
 };
 
-// The reference rotates bin b by rot_b with rot_0 = 1 and rot_{b+1} =
-// rot_b * step in f32, step = from_polar(1, (-2 pi df frac) as f32).  The
-// powers step^b are evaluated here in FP64 from that same f32 step, which
-// keeps the recurrence's systematic drift (|step| != 1 in f32) and leaves out
-// only its per-bin rounding walk.
 __device__ void make_phased_param(int64_t idx, const SectorArgs &a, PhasedParam &p) {
     FrameDelayDev d;
     compute_frame_delay(idx, a.dly, d, nullptr, nullptr);
@@ -27,27 +22,6 @@ __device__ void make_phased_param(int64_t idx, const SectorArgs &a, PhasedParam 
     p.s2 = clamp_shift(d.int2, a.N);
     p.fr1 = make_float2(d.fr1[0], d.fr1[1]);
     p.fr2 = make_float2(d.fr2[0], d.fr2[1]);
-#pragma unroll
-    for (int s = 0; s < 2; ++s) {
-        double phase_step;
-        {
-#pragma clang fp contract(off)
-            const double total = s ? d.frac2 : d.frac1;      // + rate*0 + 0.5*accel*0^2
-            phase_step = -1.0 * 2.0 * FXC_PI * a.df * total;  // USB sign (utils.rs:111-112)
-        }
-        const float2 st = polar1_f32((float)phase_step);
-        p.pw[s][0] = cd{(double)st.x, (double)st.y};
-        for (int b = 1; b < kMaxPow; ++b) p.pw[s][b] = d_mul(p.pw[s][b - 1], p.pw[s][b - 1]);
-    }
-}
-
-// step^e for 0 <= e < 2^kMaxPow.
-__device__ __forceinline__ cd pow_step(const cd *pw, int e) {
-    cd r = cd{1.0, 0.0};
-#pragma unroll 1
-    for (int b = 0; b < kMaxPow; ++b)
-        if ((e >> b) & 1) r = d_mul(r, pw[b]);
-    return r;
 }
 
 __device__ __forceinline__ float2 c_scale(float2 a, float w) { return make_float2(a.x * w, a.y * w); }
@@ -60,8 +34,6 @@ __global__ __launch_bounds__(2 * T) void k_phased(SectorArgs a, int accumulate) 
     constexpr int MP = M + M / 16 + 2;
     constexpr int NT = 2 * T;
     constexpr int NB = (M + NT) / NT;
-    constexpr int LOGNT = __builtin_ctz(NT);
-    static_assert(__builtin_ctz(M) < kMaxPow, "rotation power table too small");
     __shared__ float2 zbuf[2][MP];
     __shared__ PhasedParam pp[kFrameBatch];
     __shared__ float lev[2][256];
@@ -112,27 +84,21 @@ __global__ __launch_bounds__(2 * T) void k_phased(SectorArgs a, int accumulate) 
             const PhasedParam &P = pp[jf];
             const float2 frg = gsn == 0 ? P.fr1 : P.fr2;
             const float2 fro = gsn == 0 ? P.fr2 : P.fr1;
-            // rotation of grid bin g: step_g^g; of the other station's bin
-            // j = g + sb: step_o^j, started exactly at this thread's first
-            // overlap bin (the overlap bins of a thread are consecutive i)
-            cd Rg = pow_step(P.pw[gsn], tid);
-            cd Ro = cd{1.0, 0.0};
-            bool have_o = false;
-            const cd Sg = P.pw[gsn][LOGNT], So = P.pw[1 - gsn][LOGNT];
+            // rotation of bin b: the reference's own f32 chain r_b
+            // (k_phase_chain, rows 2f + station of this chunk)
+            const float2 *chg = a.chain + (2 * f + gsn) * a.chain_ld;
+            const float2 *cho = a.chain + (2 * f + 1 - gsn) * a.chain_ld;
 #pragma unroll
             for (int i = 0; i < NB; ++i) {
                 const int g = tid + NT * i;
                 if (g <= M) {
-                    float2 sg = c_mul(untangle(zg, g, M, a.tw_post), make_float2((float)Rg.x, (float)Rg.y));
+                    float2 sg = c_mul(untangle(zg, g, M, a.tw_post), chg[g]);
                     if (g == M) sg.y = 0.0f;                  // Nyquist (utils.rs:118-123)
                     const float2 cg = c_mul(sg, frg);
                     float2 co = make_float2(0.0f, 0.0f);
                     if (g >= gs && g < ge) {
                         const int j = g + sb_other;
-                        Ro = have_o ? d_mul(Ro, So) : pow_step(P.pw[1 - gsn], j);
-                        have_o = true;
-                        float2 so = c_mul(untangle(zo, j, M, a.tw_post),
-                                          make_float2((float)Ro.x, (float)Ro.y));
+                        float2 so = c_mul(untangle(zo, j, M, a.tw_post), cho[j]);
                         if (j == M) so.y = 0.0f;
                         co = c_mul(so, fro);
                     }
@@ -149,7 +115,6 @@ __global__ __launch_bounds__(2 * T) void k_phased(SectorArgs a, int accumulate) 
                         a.cb_out[f * (M + 1) + g] = cb;
                     }
                 }
-                Rg = d_mul(Rg, Sg);
             }
             __syncthreads();
         }

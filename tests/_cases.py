@@ -98,3 +98,13 @@ def spectra_err(got, ref):
 
 def zero_pattern_equal(got, ref):
     return np.array_equal(np.asarray(got) == 0, np.asarray(ref) == 0)
+
+
+def out_decode_plan(bit_out, shuffle1, levels1):
+    """Decoder of phased .raw words: bit_out-bit codes under station 1's
+    shuffle, valued by station 1's levels (main.rs:2295).  When bit_out
+    exceeds station 1's width, codes past its levels never occur; they are
+    given distinct sentinel values so the table has 2^bit_out entries."""
+    lv = np.asarray(levels1, dtype=np.float64)
+    full = np.concatenate([lv, 1e6 + np.arange((1 << bit_out) - len(lv), dtype=np.float64)])
+    return R.DecodePlan(bit_out, shuffle1, full)
