@@ -77,7 +77,12 @@ def workload_setup(seed=2025):
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_1024msps_2bit_fft1024.json")
-KERNEL_KEYS = {"chosen": "k_sector_mf2", "plain": "k_sector<8, 64, 0>"}
+# fused kernel of each form (the MFMA form by kernel organization variant)
+KERNEL_KEYS = {("chosen", 0): "k_sector_mf2", ("chosen", 1): "k_sector_mf3",
+               ("plain", 0): "k_sector<8, 64, 0>"}
+KERNEL_LABELS = {"k_sector_mf2": "fused decode+shift+MFMA four-step FFT+MAC, whole-frame waves",
+                 "k_sector_mf3": "fused decode+shift+MFMA four-step FFT+MAC, producer/consumer waves",
+                 "k_sector<8, 64, 0>": "fused decode+shift+FFT+MAC (plain FP32)"}
 
 
 def _pmc_rows(d):
@@ -181,15 +186,15 @@ def serve_bench(raw1, raw2, bpf, cfg, shuffle_b, n_jobs, load, sector_ms, seed=7
             "failed": st.jobs_failed}
 
 
-def load_traffic(form):
-    """Bytes per launch of the form's fused kernel from the committed summary."""
+def load_traffic(key):
+    """Bytes per launch of a fused kernel from the committed summary."""
     try:
         with open(TRAFFIC_FILE) as fh:
             kern = json.load(fh)["kernels"]
     except (OSError, ValueError, KeyError):
         return None
     for name, e in kern.items():
-        if KERNEL_KEYS[form] in name:
+        if key in name:
             return e.get("traffic_bytes")
     return None
 
@@ -211,6 +216,12 @@ def main():
                     help="jobs of the continuous-batching run (0: skip)")
     ap.add_argument("--serve-load", type=float, default=0.9,
                     help="offered load of the serving run (arrival rate x service time)")
+    ap.add_argument("--tune", choices=("auto", "default"), default="auto",
+                    help="auto: run the plan autotuner on 250,000 frames of the workload "
+                         "before timing (untimed); default: the plans' default configuration")
+    ap.add_argument("--mfma-variant", type=int, choices=(0, 1), default=None,
+                    help="pin the MFMA kernel organization (0 whole-frame waves, "
+                         "1 producer/consumer waves) instead of tuning it")
     ap.add_argument("--pmc-summary", nargs=3, metavar=("FETCH_DIR", "WRITE_DIR", "OUT_JSON"),
                     help="summarise two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of "
                          "'bench.py --copy-mib M' into OUT_JSON and exit")
@@ -252,6 +263,18 @@ def main():
         plan = F.Plan(N, bits=(BITS, BITS), levels=[LEVELS, LEVELS],
                       shuffles=(None, shuffle_b), lsb=(False, True), band=(0, 0, N // 2, 0),
                       df_hz=FS / N, products=F.PRODUCT_ACF | F.PRODUCT_XCF, precision=prec)
+        tuning = {"variant": plan.info.variant, "grid_divisor": plan.info.grid_divisor,
+                  "source": "default"}
+        if form == "chosen" and args.mfma_variant is not None:
+            F.plan_set_tuning(plan, variant=args.mfma_variant)
+            tuning = {"variant": args.mfma_variant, "grid_divisor": 1, "source": "pinned"}
+        elif args.tune == "auto":
+            rep = F.plan_autotune(plan, raw1.ptr, raw2.ptr, min(250_000, sectors[0]), cfg,
+                                  stream.ptr)
+            tuning = {"variant": rep["variant"], "grid_divisor": rep["grid_divisor"],
+                      "source": f"autotuned ({rep['candidates']} candidates, 250000 frames)",
+                      "best_ms": round(rep["best_ms"], 4), "default_ms": round(rep["default_ms"], 4)}
+        kernel_key = KERNEL_KEYS[(form, tuning["variant"])]
 
         def step():
             first = 0
@@ -288,9 +311,11 @@ def main():
                              avg_launch_ms=avg_launch_ms, launches=launches,
                              tflops=flops / (avg_launch_ms * 1e-3) / 1e12,
                              gbs=bytes_in / (avg_launch_ms * 1e-3) / 1e9,
-                             acc11_sum=float(out[half:2 * half].sum()))
+                             acc11_sum=float(out[half:2 * half].sum()),
+                             tuning=tuning, kernel=kernel_key)
         print(f"[bench] {form}: {value:,.0f} Msamples/s/station, {ms_step:.3f} ms/step, "
-              f"fused kernel {avg_launch_ms:.3f} ms/launch", file=sys.stderr, flush=True)
+              f"fused kernel {avg_launch_ms:.3f} ms/launch ({kernel_key}, {tuning['source']})",
+              file=sys.stderr, flush=True)
         plan.close()
 
     if args.copy_mib > 0:
@@ -383,15 +408,14 @@ def main():
                    "sectors": len(sectors)},
         "forms": {k: {"value": round(v["value"], 1), "ms_per_step": round(v["ms_per_step"], 4),
                       "fused_kernel_ms": round(v["avg_launch_ms"], 4),
-                      "fft_tflops": round(v["tflops"], 2), "input_gbs": round(v["gbs"], 1)}
+                      "fft_tflops": round(v["tflops"], 2), "input_gbs": round(v["gbs"], 1),
+                      "kernel": v["kernel"], "tuning": v["tuning"]}
                   for k, v in results.items()},
         "roofline": {"bound": "mfma", "achieved": round(r["tflops"], 3),
                      "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(r["tflops"] / FP32_PEAK_TFLOPS, 4),
-                     "traffic": load_traffic(main_form),
-                     "kernel": ("fused decode+shift+MFMA four-step FFT+MAC (k_sector_mf2)"
-                                if main_form == "chosen" else
-                                "fused decode+shift+FFT+MAC (k_sector<8,64,0>)"),
+                     "traffic": load_traffic(r["kernel"]),
+                     "kernel": f"{KERNEL_LABELS[r['kernel']]} ({r['kernel']})",
                      "hbm_frac": round(r["gbs"] / HBM_PEAK_GBS, 4)},
         "cpu_baseline": cpu,
         "pcie_inclusive": pcie,

@@ -155,7 +155,8 @@ struct fxc_plan {
     int64_t chunk = 0;       // allocated chunk (frames)
     int64_t chunk_use = 0;   // chunk in use (<= chunk; autotuned)
     // autotuned choices of the Stockham form
-    int variant = 0;         // FFT pass sequence (0 default, 1 radix-16 passes)
+    int variant = 0;         // Stockham: FFT pass sequence (1: radix-16 passes);
+                             // MFMA: kernel organization (1: producer/consumer waves)
     int grid_div = 1;        // workgroups = max_wg / grid_div
     DevBuf tw_l1, tw_l2, tw_lm, d_x[2], d_y, d_z[2], d_params, d_lshifts;
     // scratch
@@ -728,6 +729,7 @@ extern "C" fxc_status fxc_plan_create(const fxc_plan_desc *desc, fxc_plan **out)
             return cleanup(fail(FXC_ERR_HIP, "MFMA table upload failed"));
         p->fft_form = 1;
         p->T = 64;   // one wave per frame pair
+        p->variant = 1;   // producer/consumer waves (the faster organization on gfx950)
     }
     if (large) {
         // four-step M = M1 x M2 with FFT passes of 128/256 points; scratch for
@@ -865,6 +867,8 @@ extern "C" fxc_status fxc_plan_get_info(const fxc_plan *p, fxc_plan_info *info) 
     info->fft_form = p->fft_form;
     info->threads_per_frame = p->T;
     info->device_id = p->device;
+    info->variant = p->variant;
+    info->grid_divisor = p->grid_div;
     return FXC_OK;
 }
 
@@ -1143,7 +1147,10 @@ extern "C" fxc_status fxc_plan_autotune(fxc_plan *p, const uint8_t *d_raw1, cons
         for (int v = 0; v < nv; ++v)
             for (int d : {1, 2, 4}) cands.push_back({v, d, 0, 0});
     } else {
-        cands.push_back({0, 1, 0, 0});
+        // MFMA form: one workgroup per CU; the kernel organization varies
+        // (1, the default: producer/consumer waves; 0: every wave carries
+        // whole frames).  The default is timed first.
+        for (int v : {1, 0}) cands.push_back({v, 1, 0, 0});
     }
     const int save_variant = p->variant, save_div = p->grid_div, save_m1 = p->M1, save_m2 = p->M2;
     const int64_t save_chunk = p->chunk_use;
@@ -1218,7 +1225,8 @@ extern "C" fxc_status fxc_plan_autotune(fxc_plan *p, const uint8_t *d_raw1, cons
 
 extern "C" fxc_status fxc_plan_set_tuning(fxc_plan *p, const fxc_tune_report *c) {
     if (!p || !c) return fail(FXC_ERR_INVALID_ARG, "null argument");
-    if (c->variant != 0 && !(c->variant == 1 && p->fft_form == 0 && sector_has_variant(p->N)))
+    const bool v1_ok = (p->fft_form == 0 && sector_has_variant(p->N)) || p->fft_form == 1;
+    if (c->variant != 0 && !(c->variant == 1 && v1_ok))
         return fail(FXC_ERR_INVALID_ARG, "variant %d not available for this plan", c->variant);
     if (c->grid_divisor != 1 && c->grid_divisor != 2 && c->grid_divisor != 4)
         return fail(FXC_ERR_INVALID_ARG, "grid_divisor must be 1, 2 or 4");

@@ -188,18 +188,31 @@ __device__ __forceinline__ void mf_build_a(const uint16_t *row, int sh, int l, h
     const int base = mf::PAD + 2 * (l & 15) - sh + 32 * (l >> 4);
     uint32_t d[8];
     const bool near = (sh >= -mf::PAD) && (sh <= mf::PAD);
-    if ((sh & 1) == 0) {
+    if (near && (sh & 1) == 0) {
+        // base is even: one dword pointer, constant offsets
+        const uint32_t *q = r32 + (base >> 1);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) d[k] = q[256 * (k >> 2) + 64 * (k & 3)];
+    } else if (near) {
+        // odd shift: the FP16 pair straddles two dwords
+        const uint32_t *q = r32 + ((base - 1) >> 1);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int o = 256 * (k >> 2) + 64 * (k & 3);
+            d[k] = __builtin_amdgcn_alignbyte(q[o + 1], q[o], 2);
+        }
+    } else if ((sh & 1) == 0) {
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             int p = base + 512 * (k >> 2) + 128 * (k & 3);
-            if (!near) p = (p >= 0 && p < mf::SROW - 1) ? p : 0;
+            p = (p >= 0 && p < mf::SROW - 1) ? p : 0;
             d[k] = r32[p >> 1];
         }
     } else {
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             int p = base + 512 * (k >> 2) + 128 * (k & 3);
-            if (!near) p = (p >= 0 && p < mf::SROW - 1) ? p : 1;
+            p = (p >= 0 && p < mf::SROW - 1) ? p : 1;
             d[k] = __builtin_amdgcn_alignbyte(r32[(p + 1) >> 1], r32[(p - 1) >> 1], 2);
         }
     }
@@ -498,10 +511,353 @@ __global__ __launch_bounds__(512) void k_sector_mf2(SectorArgs a) {
     }
 }
 
+// ----------------------------------------- producer/consumer variant
+// The same per-frame work split by role over 12 waves (768 threads): waves
+// 0..7 are producers, 8..11 consumers.  Wave w runs on SIMD w % 4, so each
+// SIMD holds producers c and c + 4 and their consumer 8 + c.  A producer
+// decodes its frames, builds the A operands and runs both matrix stages
+// (one station after the other), writing Z/2 into its LDS slot; the
+// consumer untangles and accumulates the products of both its producers'
+// frames, alternating between them.  Two producers per SIMD hide each
+// other's matrix-core latency and the consumer's VALU work fills the matrix
+// shadow; no wave carries the other role's registers.
+//
+// Hand-over per producer p (one slot): prod[p] counts frames published,
+// cons[p] frames whose slot its consumer has released (workgroup-scope
+// release / acquire on LDS).  Each wait is bounded (kMfSpinLimit polls); a
+// wait cannot fail while the waves run, the bound only guarantees that
+// every wave finishes.  A wave whose wait runs out stops waiting for the
+// rest of its frames and marks the workgroup broken, whose partials are
+// then written as NaN so the failure cannot pass as a result.
+namespace mf {
+constexpr int PROD = 8, CONS = 4;
+constexpr int THREADS3 = 64 * (PROD + CONS);
+}  // namespace mf
+constexpr uint32_t kMfSpinLimit = 1u << 22;
+
+struct MfShared3 {
+    uint4 b1[16 * 64];
+    float a2[16 * 64];
+    float tw2[16 * 64];
+    float twu[8 * 64];
+    double mid[mf::CONS][4];
+    uint32_t slut[2][4 * 256];
+    uint16_t dlut[2][MFT_DLUT_HALVES];
+    uint16_t stage[mf::PROD][2][mf::SROW];     // producers' staging rows
+    float2 z[mf::PROD][2][mf::M];              // Z/2 slot per producer
+    int32_t prod[mf::PROD], cons[mf::PROD];
+    int32_t broken;
+};
+
+// Waits until *flag >= v; false if the bound ran out (or an earlier wait
+// of this wave did: `ok` is cleared and later waits return at once).
+__device__ __forceinline__ bool mf_wait_ge(int32_t *flag, int32_t v, bool &ok, int32_t *broken) {
+    if (!ok) return false;
+    for (uint32_t spin = 0; spin < kMfSpinLimit; ++spin) {
+        const int32_t c = __builtin_amdgcn_readfirstlane(
+            __hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
+        if (c >= v) return true;
+        __builtin_amdgcn_s_sleep(1);
+    }
+    ok = false;
+    __hip_atomic_store(broken, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    return false;
+}
+
+__device__ __forceinline__ void mf_publish(int32_t *flag, int32_t v) {
+    __hip_atomic_store(flag, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+__device__ void mf3_producer(const SectorArgs &a, MfShared3 &sh, int p, int l, int64_t f0,
+                             int64_t f1) {
+    const int bits1 = a.st[0].bits, bits2 = a.st[1].bits;
+    const int kind1 = a.st[0].shuffle_kind, kind2 = a.st[1].shuffle_kind;
+    const int nw1 = a.st[0].nwords, nw2 = a.st[1].nwords;
+    const int64_t bpf1 = a.st[0].bpf, bpf2 = a.st[1].bpf;
+    uint16_t *row1 = sh.stage[p][0];
+    uint16_t *row2 = sh.stage[p][1];
+    uint32_t cur1[4] = {0, 0, 0, 0}, cur2[4] = {0, 0, 0, 0};
+    bool ok = true;
+    if (f0 < f1) {
+        mf_load(a.raw1 + f0 * bpf1, nw1, l, cur1);
+        mf_load(a.raw2 + f0 * bpf2, nw2, l, cur2);
+        mf_stage(cur1, bits1, kind1, nw1, sh.slut[0], sh.dlut[0], row1 + mf::PAD, l);
+        mf_stage(cur2, bits2, kind2, nw2, sh.slut[1], sh.dlut[1], row2 + mf::PAD, l);
+        if (f0 + 1 < f1) {
+            mf_load(a.raw1 + (f0 + 1) * bpf1, nw1, l, cur1);
+            mf_load(a.raw2 + (f0 + 1) * bpf2, nw2, l, cur2);
+        }
+        wave_lds_sync();
+    }
+    for (int64_t fb = f0; fb < f1; fb += mf::FB) {
+        const int nb = (int)((f1 - fb) < mf::FB ? (f1 - fb) : mf::FB);
+        MfFrame pf = {};
+        if (l < nb) pf = mf_frame(a.first_frame + fb + l, a);
+        for (int jf = 0; jf < nb; ++jf) {
+            const int64_t f = fb + jf;
+            const int32_t i = (int32_t)(f - f0);
+            uint32_t nxt1[4] = {0, 0, 0, 0}, nxt2[4] = {0, 0, 0, 0};
+            if (f + 2 < f1) {
+                mf_load(a.raw1 + (f + 2) * bpf1, nw1, l, nxt1);
+                mf_load(a.raw2 + (f + 2) * bpf2, nw2, l, nxt2);
+            }
+            const int s1 = __builtin_amdgcn_readlane(pf.s1, jf);
+            const int s2 = __builtin_amdgcn_readlane(pf.s2, jf);
+#pragma unroll 1
+            for (int st = 0; st < 2; ++st) {
+                half8 a0, a1;
+                mf_build_a(st ? row2 : row1, st ? s2 : s1, l, a0, a1);
+                f4 y[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+                    y[t] = mf_stage1(a0, a1, ld_b1(sh.b1, (0 * 4 + t) * 2 + 0, l),
+                                     ld_b1(sh.b1, (0 * 4 + t) * 2 + 1, l),
+                                     ld_b1(sh.b1, (1 * 4 + t) * 2 + 0, l),
+                                     ld_b1(sh.b1, (1 * 4 + t) * 2 + 1, l));
+                // the slot was last read by the consumer for frame i - 1
+                if (st == 0) mf_wait_ge(&sh.cons[p], i, ok, &sh.broken);
+                mf_stage2(y, sh.tw2, sh.a2, sh.z[p][st], l);
+            }
+            mf_publish(&sh.prod[p], i + 1);
+            if (f + 1 < f1) {   // decode frame f+1 (the staging rows are this wave's own)
+                wave_lds_sync();
+                mf_stage(cur1, bits1, kind1, nw1, sh.slut[0], sh.dlut[0], row1 + mf::PAD, l);
+                mf_stage(cur2, bits2, kind2, nw2, sh.slut[1], sh.dlut[1], row2 + mf::PAD, l);
+            }
+            wave_lds_sync();
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                cur1[u] = nxt1[u];
+                cur2[u] = nxt2[u];
+            }
+        }
+    }
+}
+
+// Products of one frame from Z/2 in LDS (as k_sector_mf2's epilogue).
+// FULL: the overlap is bins 0..M-1 and every ACF bin is wanted, so the only
+// bin outside the overlap is M (lane 0's slot 1): per-lane factors k11 / k22
+// / kM (0 there, else 1) replace the slot masks.
+template <bool FULL>
+__device__ __forceinline__ void mf_mac(const float2 *z1, const float2 *z2, const float *twu,
+                                       const MfFrame &pf, int jf, int l, bool do_acf, bool do_xcf,
+                                       uint32_t m11, uint32_t m22, uint32_t m_ov, float k11,
+                                       float k22, float kM, double (&acc)[8][4], double *mid) {
+    constexpr int M = mf::M;
+    const double phw = readlane_d(pf.phw, jf);
+    const double dth = readlane_d(pf.dth, jf);
+    const float2 F1 = make_float2(readlane_f(pf.F1.x, jf), readlane_f(pf.F1.y, jf));
+    const float2 Cm = make_float2(readlane_f(pf.C.x, jf), readlane_f(pf.C.y, jf));
+    const float r0 = (float)wrap_turns(fma((double)l, dth, phw));
+    float2 P = make_float2(__builtin_amdgcn_cosf(r0), __builtin_amdgcn_sinf(r0));
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int p = l + 64 * j;
+        const int pm = (M - p) & (M - 1);
+        const float2 w = make_float2(twu[(2 * j) * 64 + l], twu[(2 * j + 1) * 64 + l]);
+        float2 x1p, x1m, x2p, x2m;
+        untangle_pair(z1[p], z1[pm], w, x1p, x1m);
+        untangle_pair(z2[p], z2[pm], w, x2p, x2m);
+        const int sp = 2 * j, sm = 2 * j + 1;
+        if (do_acf) {
+            if (FULL) {
+                acc[sp][0] += (double)norm_sqr(x1p);
+                acc[sp][1] += (double)norm_sqr(x2p);
+                acc[sm][0] += (double)(j == 0 ? k11 * norm_sqr(x1m) : norm_sqr(x1m));
+                acc[sm][1] += (double)(j == 0 ? k22 * norm_sqr(x2m) : norm_sqr(x2m));
+            } else {
+                acc[sp][0] += (double)(((m11 >> sp) & 1u) ? norm_sqr(x1p) : 0.0f);
+                acc[sp][1] += (double)(((m22 >> sp) & 1u) ? norm_sqr(x2p) : 0.0f);
+                acc[sm][0] += (double)(((m11 >> sm) & 1u) ? norm_sqr(x1m) : 0.0f);
+                acc[sm][1] += (double)(((m22 >> sm) & 1u) ? norm_sqr(x2m) : 0.0f);
+            }
+        }
+        if (do_xcf) {
+            float2 Pp, Pm;
+            if (FULL) {
+                Pp = P;
+                Pm = c_mul(Cm, c_conj(P));
+                if (j == 0) Pm = make_float2(kM * Pm.x, kM * Pm.y);
+            } else {
+                Pp = ((m_ov >> sp) & 1u) ? P : make_float2(0.f, 0.f);
+                Pm = ((m_ov >> sm) & 1u) ? c_mul(Cm, c_conj(P)) : make_float2(0.f, 0.f);
+            }
+            const float2 vp = c_mul(c_mul(x1p, c_conj(x2p)), Pp);
+            const float2 vm = c_mul(c_mul(x1m, c_conj(x2m)), Pm);
+            acc[sp][2] += (double)vp.x;
+            acc[sp][3] += (double)vp.y;
+            acc[sm][2] += (double)vm.x;
+            acc[sm][3] += (double)vm.y;
+        }
+        P = c_mul(P, F1);
+    }
+    if (l == 0) {   // k = M/2 pairs with itself: X = 2 conj(Z'[M/2]); P is P(M/2)
+        const float2 u1 = z1[M / 2], u2 = z2[M / 2];
+        const float2 x1 = make_float2(2.0f * u1.x, -2.0f * u1.y);
+        const float2 x2 = make_float2(2.0f * u2.x, -2.0f * u2.y);
+        if (do_acf) {
+            mid[0] += (double)((FULL || ((m11 >> 8) & 1u)) ? norm_sqr(x1) : 0.0f);
+            mid[1] += (double)((FULL || ((m22 >> 8) & 1u)) ? norm_sqr(x2) : 0.0f);
+        }
+        if (do_xcf) {
+            const float2 Pp = (FULL || ((m_ov >> 8) & 1u)) ? P : make_float2(0.f, 0.f);
+            const float2 v = c_mul(c_mul(x1, c_conj(x2)), Pp);
+            mid[2] += (double)v.x;
+            mid[3] += (double)v.y;
+        }
+    }
+}
+
+// Consumer c: frames [fa0, fa1) of producer c and [fb0, fb1) of producer
+// c + 4, alternating (frame i of c, frame i of c + 4, frame i + 1 of c, ...).
+template <bool FULL>
+__device__ void mf3_consumer(const SectorArgs &a, MfShared3 &sh, int c, int l, int64_t fa0,
+                             int64_t fa1, int64_t fb0, int64_t fb1, double (&acc)[8][4]) {
+    const bool do_acf = (a.flags & FLAG_ACF) != 0;
+    const bool do_xcf = (a.flags & FLAG_XCF) != 0;
+    const bool ov_only = (a.flags & FLAG_ACF_OVERLAP) != 0;
+    const int gs = a.a1s, ge = a.a1s + a.ov;   // a1s == a2s on this path
+    uint32_t m_grid = 0, m_ov = 0;
+#pragma unroll
+    for (int s = 0; s < 9; ++s) {
+        const int k = mf_bin(s, l);
+        const bool in_ov = k >= gs && k < ge;
+        if (!ov_only || in_ov) m_grid |= 1u << s;
+        if (in_ov) m_ov |= 1u << s;
+    }
+    const uint32_t m11 = a.grid == 0 ? m_grid : m_ov;
+    const uint32_t m22 = a.grid == 0 ? m_ov : m_grid;
+    const float kM = (l == 0) ? 0.0f : 1.0f;   // FULL: bin M (lane 0, slot 1) is outside the overlap
+    const float k11 = a.grid == 0 ? 1.0f : kM, k22 = a.grid == 0 ? kM : 1.0f;
+    const int64_t na = fa1 - fa0, nb = fb1 - fb0;
+    const int64_t n = na > nb ? na : nb;
+    bool ok = true;
+    for (int64_t g = 0; g < n; g += mf::FB) {
+        const int cnt = (int)((n - g) < mf::FB ? (n - g) : mf::FB);
+        MfFrame pa = {}, pb = {};
+        if (l < cnt && g + l < na) pa = mf_frame(a.first_frame + fa0 + g + l, a);
+        if (l < cnt && g + l < nb) pb = mf_frame(a.first_frame + fb0 + g + l, a);
+        for (int jf = 0; jf < cnt; ++jf) {
+            const int32_t i = (int32_t)(g + jf);
+            if (i < na) {
+                mf_wait_ge(&sh.prod[c], i + 1, ok, &sh.broken);
+                mf_mac<FULL>(sh.z[c][0], sh.z[c][1], sh.twu, pa, jf, l, do_acf, do_xcf, m11, m22,
+                             m_ov, k11, k22, kM, acc, sh.mid[c]);
+                mf_publish(&sh.cons[c], i + 1);   // this slot's reads are done
+            }
+            if (i < nb) {
+                const int q = c + mf::CONS;
+                mf_wait_ge(&sh.prod[q], i + 1, ok, &sh.broken);
+                mf_mac<FULL>(sh.z[q][0], sh.z[q][1], sh.twu, pb, jf, l, do_acf, do_xcf, m11, m22,
+                             m_ov, k11, k22, kM, acc, sh.mid[c]);
+                mf_publish(&sh.cons[q], i + 1);
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(mf::THREADS3) void k_sector_mf3(SectorArgs a) {
+    __shared__ MfShared3 sh;
+    constexpr int M = mf::M;
+    constexpr int NT = mf::THREADS3;
+    const int tid = threadIdx.x;
+    const int l = tid & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const uint8_t *tab = reinterpret_cast<const uint8_t *>(a.mf_tab);
+    {
+        const uint4 *g = reinterpret_cast<const uint4 *>(tab + MFT_B1);
+        for (int i = tid; i < 16 * 64; i += NT) sh.b1[i] = g[i];
+        const uint32_t *gs = reinterpret_cast<const uint32_t *>(tab + MFT_SLUT);
+        for (int i = tid; i < 2 * 4 * 256; i += NT) (&sh.slut[0][0])[i] = gs[i];
+        const uint32_t *gd = reinterpret_cast<const uint32_t *>(tab + MFT_DLUT);
+        for (int i = tid; i < 2 * MFT_DLUT_HALVES / 2; i += NT)
+            reinterpret_cast<uint32_t *>(&sh.dlut[0][0])[i] = gd[i];
+        uint32_t *st32 = reinterpret_cast<uint32_t *>(&sh.stage[0][0][0]);
+        for (int i = tid; i < mf::PROD * 2 * mf::SROW / 2; i += NT) st32[i] = 0u;
+        const float *ga = reinterpret_cast<const float *>(tab + MFT_A2);
+        const float *gt = reinterpret_cast<const float *>(tab + MFT_TW2);
+        const float *gu = reinterpret_cast<const float *>(tab + MFT_TWU);
+        for (int i = tid; i < 16 * 64; i += NT) {
+            sh.a2[i] = ga[i];
+            sh.tw2[i] = gt[i];
+        }
+        for (int i = tid; i < 8 * 64; i += NT) sh.twu[i] = gu[i];
+        if (tid < mf::CONS * 4) (&sh.mid[0][0])[tid] = 0.0;
+        if (tid < mf::PROD) {
+            sh.prod[tid] = 0;
+            sh.cons[tid] = 0;
+        }
+        if (tid == 0) sh.broken = 0;
+    }
+    __syncthreads();
+
+    // frames of producer p: a contiguous eighth of the workgroup's range
+    const int64_t wg0 = (int64_t)blockIdx.x * a.frames_per_wg;
+    int64_t wg1 = wg0 + a.frames_per_wg;
+    if (wg1 > a.nf) wg1 = a.nf;
+    const int64_t per = (a.frames_per_wg + mf::PROD - 1) / mf::PROD;
+    auto range = [&](int p, int64_t &r0, int64_t &r1) {
+        r0 = wg0 + per * p;
+        if (r0 > wg1) r0 = wg1;
+        r1 = (r0 + per < wg1) ? r0 + per : wg1;
+    };
+
+    double acc[8][4];
+#pragma unroll
+    for (int s = 0; s < 8; ++s)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[s][c] = 0.0;
+    if (wv < mf::PROD) {
+        int64_t f0, f1;
+        range(wv, f0, f1);
+        mf3_producer(a, sh, wv, l, f0, f1);
+    } else {
+        const int c = wv - mf::PROD;
+        int64_t fa0, fa1, fb0, fb1;
+        range(c, fa0, fa1);
+        range(c + mf::CONS, fb0, fb1);
+        const bool full = (a.flags & FLAG_ACF_OVERLAP) == 0 && a.a1s == 0 && a.ov == M;
+        if (full) mf3_consumer<true>(a, sh, c, l, fa0, fa1, fb0, fb1, acc);
+        else mf3_consumer<false>(a, sh, c, l, fa0, fa1, fb0, fb1, acc);
+    }
+
+    // ---- workgroup partial: consumers add their bins in a fixed order
+    __syncthreads();
+    double *red = reinterpret_cast<double *>(&sh.z[0][0][0]);   // [4][M + 1]
+    constexpr int H = M + 1;
+    for (int w = mf::PROD; w < mf::PROD + mf::CONS; ++w) {
+        if (wv == w) {
+            const bool first = w == mf::PROD;
+#pragma unroll
+            for (int s = 0; s < 8; ++s) {
+                const int k = mf_bin(s, l);
+#pragma unroll
+                for (int c = 0; c < 4; ++c) red[c * H + k] = (first ? 0.0 : red[c * H + k]) + acc[s][c];
+            }
+            if (l == 0)
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                    red[c * H + M / 2] = (first ? 0.0 : red[c * H + M / 2]) + sh.mid[w - mf::PROD][c];
+        }
+        __syncthreads();
+    }
+    double *out = a.part + (size_t)blockIdx.x * kNumProducts * H;
+    const double bad = sh.broken ? __builtin_nan("") : 0.0;   // NaN marks a failed hand-over
+    for (int k = tid; k < H; k += NT) {
+        out[P_PH * H + k] = bad;
+        out[P_11 * H + k] = red[0 * H + k] + bad;
+        out[P_22 * H + k] = red[1 * H + k] + bad;
+        out[P_12R * H + k] = red[2 * H + k] + bad;
+        out[P_12I * H + k] = red[3 * H + k] + bad;
+    }
+}
+
 hipError_t launch_sector_mfma(const SectorArgs &a, int precision, int n_wg, hipStream_t s) {
     if (a.N != mf::N || a.mf_tab == nullptr || precision != 1 || a.a1s != a.a2s)
         return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_sector_mf2, dim3(n_wg), dim3(512), 0, s, a);
+    // variant 1: producer/consumer waves (autotuned against the default)
+    if (a.variant == 1) hipLaunchKernelGGL(k_sector_mf3, dim3(n_wg), dim3(mf::THREADS3), 0, s, a);
+    else hipLaunchKernelGGL(k_sector_mf2, dim3(n_wg), dim3(512), 0, s, a);
     return hipGetLastError();
 }
 

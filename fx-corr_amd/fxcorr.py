@@ -88,7 +88,8 @@ class PlanInfo(C.Structure):
     _fields_ = [("fft_len", C.c_int32), ("half", C.c_int32), ("bpf1", C.c_int32),
                 ("bpf2", C.c_int32), ("bpf_out", C.c_int32), ("decode_ok1", C.c_int32),
                 ("decode_ok2", C.c_int32), ("fft_form", C.c_int32),
-                ("threads_per_frame", C.c_int32), ("device_id", C.c_int32)]
+                ("threads_per_frame", C.c_int32), ("device_id", C.c_int32),
+                ("variant", C.c_int32), ("grid_divisor", C.c_int32)]
 
 
 class CorStation(C.Structure):
@@ -304,9 +305,13 @@ class Plan:
         h = C.c_void_p()
         check(lib().fxc_plan_create(C.byref(d), C.byref(h)))
         self.h = h
+
+    @property
+    def info(self) -> PlanInfo:
+        """fxc_plan_info, read at each access (tuning may change it)."""
         info = PlanInfo()
-        check(lib().fxc_plan_get_info(h, C.byref(info)))
-        self.info = info
+        check(lib().fxc_plan_get_info(self.h, C.byref(info)))
+        return info
 
     def close(self):
         if getattr(self, "h", None):

@@ -302,8 +302,9 @@ def test_mfma_rfft_vs_numpy():
     assert np.all(got[:, 0].imag == 0) and np.all(got[:, -1].imag == 0)
 
 
+@pytest.mark.parametrize("variant", [0, 1])
 @pytest.mark.parametrize("bits", [(4, 1), (1, 8), (2, 4)])
-def test_mfma_sector_mixed_bits(bits):
+def test_mfma_sector_mixed_bits(bits, variant):
     n, fs = 1024, 512e6
     rng = np.random.default_rng(sum(bits))
     c = make_case(n=n, fs=fs, nf=3000, bits=bits, shuffles=(rng.permutation(32), None),
@@ -312,12 +313,14 @@ def test_mfma_sector_mixed_bits(bits):
                   lo2_hz=8.4e9)
     p = gpu_plan(c, F.PRECISION_CHOSEN)
     assert p.info.fft_form == 1
+    F.plan_set_tuning(p, variant=variant)
     _compare(gpu_run(c, plan=p), oracle_run(c, R.MODE_F64), CHOSEN_TOL)
 
 
+@pytest.mark.parametrize("variant", [0, 1])
 @pytest.mark.parametrize("grid", [0, 1])
 @pytest.mark.parametrize("overlap_only", [False, True])
-def test_mfma_partial_band_grids(grid, overlap_only):
+def test_mfma_partial_band_grids(grid, overlap_only, variant):
     """Common band start with a partial overlap (bins 100..399): the grid
     station's ACF covers every bin, the other station's and the XCF only
     the overlap (or both ACFs only the overlap)."""
@@ -330,6 +333,7 @@ def test_mfma_partial_band_grids(grid, overlap_only):
                   lo2_hz=8e9)
     p = gpu_plan(c, F.PRECISION_CHOSEN)
     assert p.info.fft_form == 1
+    F.plan_set_tuning(p, variant=variant)
     got = gpu_run(c, plan=p)
     ref = oracle_run(c, R.MODE_F64)
     _compare(got, ref, CHOSEN_TOL)
@@ -337,7 +341,8 @@ def test_mfma_partial_band_grids(grid, overlap_only):
         assert zero_pattern_equal(got[k], ref[k]), k
 
 
-def test_mfma_large_and_odd_shifts():
+@pytest.mark.parametrize("variant", [0, 1])
+def test_mfma_large_and_odd_shifts(variant):
     """Analytic delay model sweeping the integer shift from about -1100 to
     +200 samples: odd shifts, shifts past the staging pad and whole frames
     shifted out."""
@@ -349,7 +354,32 @@ def test_mfma_large_and_odd_shifts():
     assert min(ints) < -n and max(ints) > 64 and any(i % 2 for i in ints)
     p = gpu_plan(c, F.PRECISION_CHOSEN)
     assert p.info.fft_form == 1
+    F.plan_set_tuning(p, variant=variant)
     _compare(gpu_run(c, plan=p), oracle_run(c, R.MODE_F64), CHOSEN_TOL)
+
+
+@pytest.mark.parametrize("nf", [1, 3, 5, 37, 2051])
+def test_mfma_kernel_organizations_agree(nf):
+    """Whole-frame waves and producer/consumer wave pairs agree to FP32
+    rounding (the compiler may contract the complex products of the two
+    kernels into FMAs differently) for frame counts that leave wave pairs
+    (or whole workgroups) without frames, and both match the FP64 oracle."""
+    n, fs = 1024, 1024e6
+    rng = np.random.default_rng(nf)
+    c = make_case(n=n, fs=fs, nf=nf, shuffles=(None, rng.permutation(32)), lsb=(False, True),
+                  rho=0.4, delay_samples=1.3, seed=40 + nf, delay_user_samples=1.3,
+                  extra_delay_rate_sps=1e-9, lo1_hz=6.7e9, lo2_hz=6.7e9)
+    res = []
+    for v in (0, 1):
+        p = gpu_plan(c, F.PRECISION_CHOSEN)
+        assert p.info.fft_form == 1
+        F.plan_set_tuning(p, variant=v)
+        res.append(gpu_run(c, plan=p))
+    e = spectra_err(res[1], res[0])
+    assert max(e.values()) < PLAIN_TOL, e
+    o64 = oracle_run(c, R.MODE_F64)
+    for r in res:
+        _compare(r, o64, CHOSEN_TOL)
 
 
 def test_mfma_needs_common_band_start():
@@ -543,5 +573,8 @@ def test_autotune_picks_a_candidate(n, prec):
     assert rep["candidates"] >= 1 and 0 < rep["best_ms"] <= rep["default_ms"]
     if n == 4096:
         assert rep["candidates"] == 6
+    if n == 1024:
+        assert rep["candidates"] == 2 and rep["variant"] in (0, 1)
+    assert p.info.variant == rep["variant"] and p.info.grid_divisor == rep["grid_divisor"]
     if n == 65536:
         assert rep["split_m1"] in (128, 256) and rep["chunk_frames"] > 0
