@@ -275,6 +275,112 @@ __device__ __forceinline__ void mf_stage2(const f4 y[4], const float *tw2, const
             z[(l & 15) + 16 * ct + 32 * (4 * (l >> 4) + r)] = make_float2(d[0][ct][r], d[1][ct][r]);
 }
 
+// Stage 2 on the VALU in FP32 (producer/consumer organization): the DFT-16
+// over m2 = 4g + r (g = lane group, r = this lane's rows) of each column k1
+// as radix 4 x 4 with k2 = a + 4b:
+//   Z[a + 4b] = sum_r W4^{rb} W16^{ra} sum_g W4^{ga} Y'[4g + r]
+// The DFT-4 over g crosses lane groups: two radix-2 steps through
+// v_permlane32_swap (g <-> g^2) and v_permlane16_swap (g <-> g^1).  Each
+// swap of the pair (x, y) hands one lane half x's butterfly and the other
+// half y's, so a lane may end up holding a value with its components
+// exchanged, swap(v) = i conj(v): groups 1 and 2 do after the exchanges.
+// Such lanes use conjugated twiddles (swap(u) conj(w) = swap(u w)), their
+// in-lane DFT-4 outputs b = 1 and 3 trade places, and they store the
+// components crosswise; all of this is folded into per-lane constants.
+struct MfS2 {
+    float s32, s16;        // butterfly signs: +1 on the lower half / even group, else -1
+    bool g3;               // lane group 3: the DFT-4's -i on the second difference
+    float2 w[3];           // W16^{r a}, r = 1..3 (conjugated on lanes holding swapped values)
+    int ox[4], oy[4];      // LDS byte offsets of the re / im parts of this lane's outputs
+};
+
+__device__ __forceinline__ MfS2 mf_s2_consts(int l) {
+    MfS2 c;
+    const int g = l >> 4;
+    c.s32 = (l < 32) ? 1.0f : -1.0f;
+    c.s16 = (g & 1) ? -1.0f : 1.0f;
+    c.g3 = g == 3;
+    const int a = ((g & 1) << 1) | (g >> 1);        // DFT-4 output index held by group g
+    const bool swapped = ((g ^ (g >> 1)) & 1) != 0;  // groups 1, 2
+#pragma unroll
+    for (int r = 1; r <= 3; ++r) {
+        const double th = -2.0 * FXC_PI * (double)(r * a) / 16.0;
+        c.w[r - 1] = make_float2((float)cos(th), swapped ? -(float)sin(th) : (float)sin(th));
+    }
+    const int zb = 8 * ((l & 15) + 32 * a);          // byte offset of Z[k1 + 32 a], h = 0, b = 0
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int b = swapped ? ((4 - i) & 3) : i;   // outputs 1 and 3 trade places
+        const int o = zb + 8 * 128 * b;
+        c.ox[i] = o + (swapped ? 4 : 0);
+        c.oy[i] = o + (swapped ? 0 : 4);
+    }
+    return c;
+}
+
+__device__ __forceinline__ void xlane32(float &x, float &y, float s) {
+    const auto p = __builtin_amdgcn_permlane32_swap(__builtin_bit_cast(uint32_t, x),
+                                                    __builtin_bit_cast(uint32_t, y), false, false);
+    const auto q = __builtin_amdgcn_permlane32_swap(__builtin_bit_cast(uint32_t, y),
+                                                    __builtin_bit_cast(uint32_t, x), false, false);
+    x = fmaf(__builtin_bit_cast(float, (uint32_t)p[1]), s, __builtin_bit_cast(float, (uint32_t)p[0]));
+    y = fmaf(__builtin_bit_cast(float, (uint32_t)q[1]), s, __builtin_bit_cast(float, (uint32_t)q[0]));
+}
+
+__device__ __forceinline__ void xlane16(float &x, float &y, float s) {
+    const auto p = __builtin_amdgcn_permlane16_swap(__builtin_bit_cast(uint32_t, x),
+                                                    __builtin_bit_cast(uint32_t, y), false, false);
+    const auto q = __builtin_amdgcn_permlane16_swap(__builtin_bit_cast(uint32_t, y),
+                                                    __builtin_bit_cast(uint32_t, x), false, false);
+    x = fmaf(__builtin_bit_cast(float, (uint32_t)p[1]), s, __builtin_bit_cast(float, (uint32_t)p[0]));
+    y = fmaf(__builtin_bit_cast(float, (uint32_t)q[1]), s, __builtin_bit_cast(float, (uint32_t)q[0]));
+}
+
+// Twiddle + VALU stage 2 of one station-frame; Z/2 to LDS in natural order.
+__device__ __forceinline__ void mf_stage2v(const f4 y[4], const float *tw2, float2 *z, int l,
+                                           const MfS2 &c) {
+    char *zc = reinterpret_cast<char *>(z);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        float2 v[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            v[r] = c_mul(make_float2(y[h][r], y[2 + h][r]),
+                         make_float2(tw2[((h * 4 + r) * 2) * 64 + l], tw2[((h * 4 + r) * 2 + 1) * 64 + l]));
+        // g <-> g^2: lower half S = Y_g + Y_{g+2}, upper half D = Y_{g-2} - Y_g
+        // (upper half now holds swap(D))
+#pragma unroll
+        for (int r = 0; r < 4; ++r) xlane32(v[r].x, v[r].y, c.s32);
+        // group 3: swap(D1) -> swap(-i D1) = (-D1.re, D1.im)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const float a = v[r].x, b = v[r].y;
+            v[r].x = c.g3 ? -b : a;
+            v[r].y = c.g3 ? a : b;
+        }
+        // g <-> g^1: X[0] = S0 + S1, X[2] = S0 - S1, X[1] = D0 - i D1, X[3] = D0 + i D1
+#pragma unroll
+        for (int r = 0; r < 4; ++r) xlane16(v[r].x, v[r].y, c.s16);
+        v[1] = c_mul(v[1], c.w[0]);
+        v[2] = c_mul(v[2], c.w[1]);
+        v[3] = c_mul(v[3], c.w[2]);
+        // in-lane DFT-4 over r
+        const float2 s0 = make_float2(v[0].x + v[2].x, v[0].y + v[2].y);
+        const float2 d0 = make_float2(v[0].x - v[2].x, v[0].y - v[2].y);
+        const float2 s1 = make_float2(v[1].x + v[3].x, v[1].y + v[3].y);
+        const float2 d1 = make_float2(v[1].x - v[3].x, v[1].y - v[3].y);
+        const float2 o[4] = {make_float2(s0.x + s1.x, s0.y + s1.y),
+                             make_float2(d0.x + d1.y, d0.y - d1.x),
+                             make_float2(s0.x - s1.x, s0.y - s1.y),
+                             make_float2(d0.x - d1.y, d0.y + d1.x)};
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            *reinterpret_cast<float *>(zc + c.ox[i] + 128 * h) = o[i].x;
+            *reinterpret_cast<float *>(zc + c.oy[i] + 128 * h) = o[i].y;
+        }
+    }
+}
+
 // Real-spectrum bins p and M-p from Z/2 (realfft's split method, paired):
 // E = Z'[p] + conj Z'[M-p], O = Z'[p] - conj Z'[M-p], T = W_N^p O,
 // X[p] = E - iT, X[M-p] = conj(E + iT).  p = 0 gives DC and Nyquist.
@@ -515,8 +621,11 @@ __global__ __launch_bounds__(512) void k_sector_mf2(SectorArgs a) {
 // The same per-frame work split by role over 12 waves (768 threads): waves
 // 0..7 are producers, 8..11 consumers.  Wave w runs on SIMD w % 4, so each
 // SIMD holds producers c and c + 4 and their consumer 8 + c.  A producer
-// decodes its frames, builds the A operands and runs both matrix stages
-// (one station after the other), writing Z/2 into its LDS slot; the
+// decodes its frames, builds the A operands, runs stage 1 on the matrix
+// cores and stage 2 on the VALU (mf_stage2v; on gfx950 the FP32 MFMA and
+// the VALU do not run concurrently, and the radix-4 x 4 DFT-16 takes fewer
+// cycles than the dense FP32 GEMM), one station after the other, and
+// writes Z/2 into its LDS slot; the
 // consumer untangles and accumulates the products of both its producers'
 // frames, alternating between them.  Two producers per SIMD hide each
 // other's matrix-core latency and the consumer's VALU work fills the matrix
@@ -578,6 +687,7 @@ __device__ void mf3_producer(const SectorArgs &a, MfShared3 &sh, int p, int l, i
     uint16_t *row2 = sh.stage[p][1];
     uint32_t cur1[4] = {0, 0, 0, 0}, cur2[4] = {0, 0, 0, 0};
     bool ok = true;
+    const MfS2 s2c = mf_s2_consts(l);
     if (f0 < f1) {
         mf_load(a.raw1 + f0 * bpf1, nw1, l, cur1);
         mf_load(a.raw2 + f0 * bpf2, nw2, l, cur2);
@@ -616,7 +726,7 @@ __device__ void mf3_producer(const SectorArgs &a, MfShared3 &sh, int p, int l, i
                                      ld_b1(sh.b1, (1 * 4 + t) * 2 + 1, l));
                 // the slot was last read by the consumer for frame i - 1
                 if (st == 0) mf_wait_ge(&sh.cons[p], i, ok, &sh.broken);
-                mf_stage2(y, sh.tw2, sh.a2, sh.z[p][st], l);
+                mf_stage2v(y, sh.tw2, sh.z[p][st], l, s2c);
             }
             mf_publish(&sh.prod[p], i + 1);
             if (f + 1 < f1) {   // decode frame f+1 (the staging rows are this wave's own)
@@ -891,7 +1001,8 @@ __global__ __launch_bounds__(64) void k_rfft_mfma(SectorArgs a, const float *in,
     for (int t = 0; t < 4; ++t)
         y[t] = mf_stage1(a0, a1, ld_b1(b1, t * 2, l), ld_b1(b1, t * 2 + 1, l),
                          ld_b1(b1, (4 + t) * 2, l), ld_b1(b1, (4 + t) * 2 + 1, l));
-    mf_stage2(y, gt, ga, z, l);
+    if (a.variant == 1) mf_stage2v(y, gt, z, l, mf_s2_consts(l));
+    else mf_stage2(y, gt, ga, z, l);
     __syncthreads();
     float2 *o = out + f * (mf::M + 1);
 #pragma unroll

@@ -282,10 +282,14 @@ def test_mfma_eligibility():
     assert F.Plan(1024, levels=odd, df_hz=1.0, precision=F.PRECISION_CHOSEN).info.fft_form == 0
 
 
-def test_mfma_rfft_vs_numpy():
+@pytest.mark.parametrize("variant", [0, 1])
+def test_mfma_rfft_vs_numpy(variant):
+    """Stage 2 as an FP32 MFMA GEMM (variant 0) or as the VALU radix-4 x 4
+    DFT-16 with cross-lane swaps (variant 1, the default)."""
     n = 1024
     p = F.Plan(n, levels=[LEVELS_2BIT, LEVELS_2BIT], df_hz=1.0, precision=F.PRECISION_CHOSEN)
-    assert p.info.fft_form == 1
+    assert p.info.fft_form == 1 and p.info.variant == 1
+    F.plan_set_tuning(p, variant=variant)
     rng = np.random.default_rng(77)
     # FP16-exact inputs (2-bit levels), as the decode feeds the first stage
     x = LEVELS_2BIT[rng.integers(0, 4, size=(7, n))].astype(np.float32)
@@ -360,10 +364,12 @@ def test_mfma_large_and_odd_shifts(variant):
 
 @pytest.mark.parametrize("nf", [1, 3, 5, 37, 2051])
 def test_mfma_kernel_organizations_agree(nf):
-    """Whole-frame waves and producer/consumer wave pairs agree to FP32
-    rounding (the compiler may contract the complex products of the two
-    kernels into FMAs differently) for frame counts that leave wave pairs
-    (or whole workgroups) without frames, and both match the FP64 oracle."""
+    """Whole-frame waves (FP32 MFMA stage 2) and producer/consumer waves
+    (VALU radix-4 x 4 stage 2) agree to FP32 rounding for frame counts that
+    leave producers (or whole workgroups) without frames, and both match
+    the FP64 oracle.  With a single frame a bin of tiny power turns FP32
+    rounding into a per-bin relative difference near 1e-5, hence the
+    chosen-form bar for the agreement too."""
     n, fs = 1024, 1024e6
     rng = np.random.default_rng(nf)
     c = make_case(n=n, fs=fs, nf=nf, shuffles=(None, rng.permutation(32)), lsb=(False, True),
@@ -376,7 +382,7 @@ def test_mfma_kernel_organizations_agree(nf):
         F.plan_set_tuning(p, variant=v)
         res.append(gpu_run(c, plan=p))
     e = spectra_err(res[1], res[0])
-    assert max(e.values()) < PLAIN_TOL, e
+    assert max(e.values()) < CHOSEN_TOL, e
     o64 = oracle_run(c, R.MODE_F64)
     for r in res:
         _compare(r, o64, CHOSEN_TOL)
