@@ -279,67 +279,62 @@ __device__ __forceinline__ void mf_stage2(const f4 y[4], const float *tw2, const
 // over m2 = 4g + r (g = lane group, r = this lane's rows) of each column k1
 // as radix 4 x 4 with k2 = a + 4b:
 //   Z[a + 4b] = sum_r W4^{rb} W16^{ra} sum_g W4^{ga} Y'[4g + r]
-// The DFT-4 over g crosses lane groups: two radix-2 steps through
-// v_permlane32_swap (g <-> g^2) and v_permlane16_swap (g <-> g^1).  Each
-// swap of the pair (x, y) hands one lane half x's butterfly and the other
-// half y's, so a lane may end up holding a value with its components
-// exchanged, swap(v) = i conj(v): groups 1 and 2 do after the exchanges.
-// Such lanes use conjugated twiddles (swap(u) conj(w) = swap(u w)), their
-// in-lane DFT-4 outputs b = 1 and 3 trade places, and they store the
-// components crosswise; all of this is folded into per-lane constants.
+// The DFT-4 over g crosses lane groups: two radix-2 steps, g <-> g^2 through
+// v_permlane32_swap and g <-> g^1 through v_permlane16_swap (bfly32/16),
+// the -i of its second difference on group 3 in between.  Lane group g then
+// holds output a = bitrev2(g); the W16^{ra} twiddles and the in-lane DFT-4
+// over r follow.
 struct MfS2 {
-    float s32, s16;        // butterfly signs: +1 on the lower half / even group, else -1
     bool g3;               // lane group 3: the DFT-4's -i on the second difference
-    float2 w[3];           // W16^{r a}, r = 1..3 (conjugated on lanes holding swapped values)
-    int ox[4], oy[4];      // LDS byte offsets of the re / im parts of this lane's outputs
+    float2 w[3];           // W16^{r a}, r = 1..3
+    int zi;                // Z index of this lane's output k1 + 32 a (h = 0, b = 0)
 };
 
 __device__ __forceinline__ MfS2 mf_s2_consts(int l) {
     MfS2 c;
     const int g = l >> 4;
-    c.s32 = (l < 32) ? 1.0f : -1.0f;
-    c.s16 = (g & 1) ? -1.0f : 1.0f;
     c.g3 = g == 3;
     const int a = ((g & 1) << 1) | (g >> 1);        // DFT-4 output index held by group g
-    const bool swapped = ((g ^ (g >> 1)) & 1) != 0;  // groups 1, 2
 #pragma unroll
     for (int r = 1; r <= 3; ++r) {
         const double th = -2.0 * FXC_PI * (double)(r * a) / 16.0;
-        c.w[r - 1] = make_float2((float)cos(th), swapped ? -(float)sin(th) : (float)sin(th));
+        c.w[r - 1] = make_float2((float)cos(th), (float)sin(th));
     }
-    const int zb = 8 * ((l & 15) + 32 * a);          // byte offset of Z[k1 + 32 a], h = 0, b = 0
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int b = swapped ? ((4 - i) & 3) : i;   // outputs 1 and 3 trade places
-        const int o = zb + 8 * 128 * b;
-        c.ox[i] = o + (swapped ? 4 : 0);
-        c.oy[i] = o + (swapped ? 0 : 4);
-    }
+    c.zi = (l & 15) + 32 * a;
     return c;
 }
 
-__device__ __forceinline__ void xlane32(float &x, float &y, float s) {
+// Radix-2 butterfly of the complex (x, y) between lanes l and l ^ 32: the
+// lower half gets own + partner, the upper half partner - own.  One swap
+// gives every lane both values of one component (x in the lower half, y in
+// the upper), sum and difference are formed there, and a second swap hands
+// each half the component it lacks.
+__device__ __forceinline__ void bfly32(float &x, float &y) {
     const auto p = __builtin_amdgcn_permlane32_swap(__builtin_bit_cast(uint32_t, x),
                                                     __builtin_bit_cast(uint32_t, y), false, false);
-    const auto q = __builtin_amdgcn_permlane32_swap(__builtin_bit_cast(uint32_t, y),
-                                                    __builtin_bit_cast(uint32_t, x), false, false);
-    x = fmaf(__builtin_bit_cast(float, (uint32_t)p[1]), s, __builtin_bit_cast(float, (uint32_t)p[0]));
-    y = fmaf(__builtin_bit_cast(float, (uint32_t)q[1]), s, __builtin_bit_cast(float, (uint32_t)q[0]));
+    const float p0 = __builtin_bit_cast(float, (uint32_t)p[0]);
+    const float p1 = __builtin_bit_cast(float, (uint32_t)p[1]);
+    const auto q = __builtin_amdgcn_permlane32_swap(__builtin_bit_cast(uint32_t, p0 + p1),
+                                                    __builtin_bit_cast(uint32_t, p0 - p1), false, false);
+    x = __builtin_bit_cast(float, (uint32_t)q[0]);
+    y = __builtin_bit_cast(float, (uint32_t)q[1]);
 }
 
-__device__ __forceinline__ void xlane16(float &x, float &y, float s) {
+// The same between lanes l and l ^ 16 (even lane groups get the sums).
+__device__ __forceinline__ void bfly16(float &x, float &y) {
     const auto p = __builtin_amdgcn_permlane16_swap(__builtin_bit_cast(uint32_t, x),
                                                     __builtin_bit_cast(uint32_t, y), false, false);
-    const auto q = __builtin_amdgcn_permlane16_swap(__builtin_bit_cast(uint32_t, y),
-                                                    __builtin_bit_cast(uint32_t, x), false, false);
-    x = fmaf(__builtin_bit_cast(float, (uint32_t)p[1]), s, __builtin_bit_cast(float, (uint32_t)p[0]));
-    y = fmaf(__builtin_bit_cast(float, (uint32_t)q[1]), s, __builtin_bit_cast(float, (uint32_t)q[0]));
+    const float p0 = __builtin_bit_cast(float, (uint32_t)p[0]);
+    const float p1 = __builtin_bit_cast(float, (uint32_t)p[1]);
+    const auto q = __builtin_amdgcn_permlane16_swap(__builtin_bit_cast(uint32_t, p0 + p1),
+                                                    __builtin_bit_cast(uint32_t, p0 - p1), false, false);
+    x = __builtin_bit_cast(float, (uint32_t)q[0]);
+    y = __builtin_bit_cast(float, (uint32_t)q[1]);
 }
 
 // Twiddle + VALU stage 2 of one station-frame; Z/2 to LDS in natural order.
 __device__ __forceinline__ void mf_stage2v(const f4 y[4], const float *tw2, float2 *z, int l,
                                            const MfS2 &c) {
-    char *zc = reinterpret_cast<char *>(z);
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
         float2 v[4];
@@ -347,20 +342,19 @@ __device__ __forceinline__ void mf_stage2v(const f4 y[4], const float *tw2, floa
         for (int r = 0; r < 4; ++r)
             v[r] = c_mul(make_float2(y[h][r], y[2 + h][r]),
                          make_float2(tw2[((h * 4 + r) * 2) * 64 + l], tw2[((h * 4 + r) * 2 + 1) * 64 + l]));
-        // g <-> g^2: lower half S = Y_g + Y_{g+2}, upper half D = Y_{g-2} - Y_g
-        // (upper half now holds swap(D))
+        // g <-> g^2: S = Y_g + Y_{g+2} (lower half), D = Y_{g-2} - Y_g (upper half)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) xlane32(v[r].x, v[r].y, c.s32);
-        // group 3: swap(D1) -> swap(-i D1) = (-D1.re, D1.im)
+        for (int r = 0; r < 4; ++r) bfly32(v[r].x, v[r].y);
+        // group 3: D1 -> -i D1
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const float a = v[r].x, b = v[r].y;
-            v[r].x = c.g3 ? -b : a;
-            v[r].y = c.g3 ? a : b;
+            const float re = v[r].x, im = v[r].y;
+            v[r].x = c.g3 ? im : re;
+            v[r].y = c.g3 ? -re : im;
         }
         // g <-> g^1: X[0] = S0 + S1, X[2] = S0 - S1, X[1] = D0 - i D1, X[3] = D0 + i D1
 #pragma unroll
-        for (int r = 0; r < 4; ++r) xlane16(v[r].x, v[r].y, c.s16);
+        for (int r = 0; r < 4; ++r) bfly16(v[r].x, v[r].y);
         v[1] = c_mul(v[1], c.w[0]);
         v[2] = c_mul(v[2], c.w[1]);
         v[3] = c_mul(v[3], c.w[2]);
@@ -369,15 +363,11 @@ __device__ __forceinline__ void mf_stage2v(const f4 y[4], const float *tw2, floa
         const float2 d0 = make_float2(v[0].x - v[2].x, v[0].y - v[2].y);
         const float2 s1 = make_float2(v[1].x + v[3].x, v[1].y + v[3].y);
         const float2 d1 = make_float2(v[1].x - v[3].x, v[1].y - v[3].y);
-        const float2 o[4] = {make_float2(s0.x + s1.x, s0.y + s1.y),
-                             make_float2(d0.x + d1.y, d0.y - d1.x),
-                             make_float2(s0.x - s1.x, s0.y - s1.y),
-                             make_float2(d0.x - d1.y, d0.y + d1.x)};
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            *reinterpret_cast<float *>(zc + c.ox[i] + 128 * h) = o[i].x;
-            *reinterpret_cast<float *>(zc + c.oy[i] + 128 * h) = o[i].y;
-        }
+        float2 *zo = z + c.zi + 16 * h;
+        zo[0] = make_float2(s0.x + s1.x, s0.y + s1.y);
+        zo[128] = make_float2(d0.x + d1.y, d0.y - d1.x);
+        zo[256] = make_float2(s0.x - s1.x, s0.y - s1.y);
+        zo[384] = make_float2(d0.x - d1.y, d0.y + d1.x);
     }
 }
 
