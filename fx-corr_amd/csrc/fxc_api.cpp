@@ -163,6 +163,7 @@ struct fxc_plan {
     DevBuf part, red_scratch, table_dev, d_raw[2], d_acc, d_raw_out, d_tmp_in, d_tmp_out,
         d_shifts, d_cb, d_tbuf;
     DevBuf d_chain;          // the reference's f32 phase chains of one slab/chunk of frames
+    DevBuf d_fringe;         // one chunk's products for the whole-scan cross spectrum
     int max_wg = 0;
     std::vector<double> table_host;  // last uploaded geometric table
     // timing
@@ -848,7 +849,7 @@ extern "C" fxc_status fxc_plan_device_bytes(const fxc_plan *p, uint64_t *bytes) 
                             &p->d_acc, &p->d_raw_out, &p->d_tmp_in, &p->d_tmp_out, &p->d_shifts,
                             &p->mf_tab, &p->d_cb, &p->d_tbuf, &p->tw_l1, &p->tw_l2, &p->tw_lm,
                             &p->d_x[0], &p->d_x[1], &p->d_y, &p->d_z[0], &p->d_z[1], &p->d_params,
-                            &p->d_lshifts, &p->d_chain};
+                            &p->d_lshifts, &p->d_chain, &p->d_fringe};
     uint64_t t = 0;
     for (const DevBuf *b : bufs) t += b->cap;
     *bytes = t;
@@ -1256,6 +1257,36 @@ extern "C" fxc_status fxc_sector_run_device(fxc_plan *p, const uint8_t *d_raw1,
     return enqueue_sector(p, d_raw1, d_raw2, first_frame, nf, *delay, d_acc, d_raw_out, s);
 }
 
+// One chunk of the whole-scan cross spectrum (main.rs:1760-1849: the XCF
+// of nf frames on the plan's output grid), scaled by 1/N^2 and added into
+// ac (main.rs:1751, 1850).  Frames that fail to decode add nothing.
+static fxc_status fringe_check(const fxc_plan *p) {
+    if (!(p->desc.products & FXC_PRODUCT_XCF) || (p->desc.products & (FXC_PRODUCT_PHASED | FXC_PRODUCT_RAW_OUT)))
+        return fail(FXC_ERR_UNSUPPORTED, "the whole-scan cross spectrum needs an XCF plan without phased products");
+    return FXC_OK;
+}
+
+extern "C" fxc_status fxc_fringe_accumulate_device(fxc_plan *p, const uint8_t *d_raw1,
+                                                   const uint8_t *d_raw2, int64_t first_frame,
+                                                   int64_t nf, const fxc_delay_cfg *delay,
+                                                   double *d_ac, void *stream) {
+    if (!p || !delay || !d_ac) return fail(FXC_ERR_INVALID_ARG, "null argument");
+    fxc_status st = fringe_check(p);
+    if (st != FXC_OK) return st;
+    if (nf < 0) return fail(FXC_ERR_INVALID_ARG, "nf < 0");
+    if (nf > 0 && (!d_raw1 || !d_raw2)) return fail(FXC_ERR_INVALID_ARG, "null raw buffer");
+    const size_t acc_bytes = (size_t)kNumProducts * p->half * sizeof(double);
+    if (p->d_fringe.ensure(acc_bytes) != hipSuccess) return fail(FXC_ERR_NOMEM, "device allocation failed");
+    hipStream_t s = (hipStream_t)stream;
+    double *d = (double *)p->d_fringe.p;
+    st = enqueue_sector(p, d_raw1, d_raw2, first_frame, nf, *delay, d, nullptr, s);
+    if (st != FXC_OK) return st;
+    const double inv_fft2 = 1.0 / ((double)p->N * (double)p->N);
+    HIP_TRY(launch_scale_add_xcf(d_ac, d + (size_t)P_12R * p->half, d + (size_t)P_12I * p->half, p->half,
+                                 inv_fft2, s));
+    return FXC_OK;
+}
+
 extern "C" fxc_status fxc_sector_run(fxc_plan *p, const uint8_t *raw1, size_t len1,
                                      const uint8_t *raw2, size_t len2, int64_t first_frame,
                                      int64_t nf, const fxc_delay_cfg *delay, double *acc_ph,
@@ -1302,6 +1333,26 @@ extern "C" fxc_status fxc_sector_run(fxc_plan *p, const uint8_t *raw1, size_t le
             acc12[2 * k + 1] = host[P_12I * h + k];
         }
     if (n_failed) *n_failed = (p->decode_ok[0] && p->decode_ok[1]) ? 0 : nf;
+    return FXC_OK;
+}
+
+extern "C" fxc_status fxc_fringe_accumulate(fxc_plan *p, const uint8_t *raw1, size_t len1,
+                                            const uint8_t *raw2, size_t len2, int64_t first_frame,
+                                            int64_t nf, const fxc_delay_cfg *delay, double *ac,
+                                            int64_t *n_failed) {
+    if (!p || !delay || !ac) return fail(FXC_ERR_INVALID_ARG, "null argument");
+    fxc_status st = fringe_check(p);
+    if (st != FXC_OK) return st;
+    const size_t h = (size_t)p->half;
+    std::vector<double> ph(h), a11(h), a22(h), a12(2 * h);
+    st = fxc_sector_run(p, raw1, len1, raw2, len2, first_frame, nf, delay, ph.data(), a11.data(),
+                        a12.data(), a22.data(), nullptr, n_failed);
+    if (st != FXC_OK) return st;
+    const double inv_fft2 = 1.0 / ((double)p->N * (double)p->N);
+    for (size_t k = 0; k < h; ++k) {
+        ac[2 * k] += a12[2 * k] * inv_fft2;
+        ac[2 * k + 1] += a12[2 * k + 1] * inv_fft2;
+    }
     return FXC_OK;
 }
 

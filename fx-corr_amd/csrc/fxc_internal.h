@@ -136,6 +136,8 @@ int mfma_wg_threads();
 hipError_t launch_phased(const SectorArgs &a, int accumulate, int n_wg, hipStream_t stream);
 hipError_t launch_fill_zero(void *p, size_t bytes, hipStream_t stream);
 hipError_t launch_add_f64(double *dst, const double *src, size_t n, hipStream_t stream);
+hipError_t launch_scale_add_xcf(double *ac, const double *re, const double *im, int half, double s,
+                                hipStream_t stream);
 
 // fxc_chain.hip: one thread per chain; frames [0, a.nf) of the launch,
 // rows of a.chain_ld float2 (one XCF chain of a.ov bins, or two phased

@@ -169,6 +169,24 @@ hipError_t launch_add_f64(double *dst, const double *src, size_t n, hipStream_t 
     return hipGetLastError();
 }
 
+// Whole-scan cross spectrum: ac[2k], ac[2k+1] += s * (re[k], im[k]), k < half
+// (main.rs:1850: ac[k] += bc[k] * inv_fft2).
+__global__ void k_scale_add_xcf(double *ac, const double *re, const double *im, int half, double s) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < half) {
+        ac[2 * k] += re[k] * s;
+        ac[2 * k + 1] += im[k] * s;
+    }
+}
+
+hipError_t launch_scale_add_xcf(double *ac, const double *re, const double *im, int half, double s,
+                                hipStream_t st) {
+    if (half <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_scale_add_xcf, dim3((unsigned)((half + 255) / 256)), dim3(256), 0, st, ac, re,
+                       im, half, s);
+    return hipGetLastError();
+}
+
 // Graph-safe zero fill (a kernel, not a memset node).
 __global__ void k_fill_zero(uint32_t *p, size_t nwords, uint8_t *tail, size_t ntail) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;

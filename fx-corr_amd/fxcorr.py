@@ -756,3 +756,67 @@ def plan_set_tuning(plan: Plan, variant=0, grid_divisor=1, split_m1=0, chunk_fra
     r = TuneReport()
     r.variant, r.grid_divisor, r.split_m1, r.chunk_frames = variant, grid_divisor, split_m1, chunk_frames
     check(f(plan.h, C.byref(r)))
+
+
+# ------------------------------------------------- whole-scan cross spectrum
+class CrossSpectrumResult(C.Structure):
+    _fields_ = [("has_delay", C.c_int32), ("delay_seconds", C.c_double),
+                ("slope_rad_per_hz", C.c_double), ("intercept_rad", C.c_double)]
+
+
+EXPORTED += ["fxc_fringe_accumulate", "fxc_fringe_accumulate_device",
+             "fxc_finalize_cross_spectrum"]
+
+
+def _fringe_sigs(L):
+    if not getattr(L, "_fringe_sigs", False):
+        vp, i32, i64 = C.c_void_p, C.c_int32, C.c_int64
+        L.fxc_fringe_accumulate.argtypes = [vp, vp, C.c_size_t, vp, C.c_size_t, i64, i64,
+                                            _P(DelayCfg), vp, _P(i64)]
+        L.fxc_fringe_accumulate.restype = i32
+        L.fxc_fringe_accumulate_device.argtypes = [vp, vp, vp, i64, i64, _P(DelayCfg), vp, vp]
+        L.fxc_fringe_accumulate_device.restype = i32
+        L.fxc_finalize_cross_spectrum.argtypes = [vp, i64, i32, C.c_double, C.c_double, i32,
+                                                  _P(CrossSpectrumResult), vp, vp, vp, vp]
+        L.fxc_finalize_cross_spectrum.restype = i32
+        L._fringe_sigs = True
+    return L
+
+
+def fringe_accumulate(plan: Plan, raw1, raw2, first_frame, nf, cfg: DelayCfg, ac: np.ndarray):
+    """Adds one chunk's XCF sum / N^2 into ac (complex128, plan.half bins);
+    returns the number of frames that failed to decode."""
+    L = _fringe_sigs(lib())
+    raw1 = np.ascontiguousarray(raw1, dtype=np.uint8)
+    raw2 = np.ascontiguousarray(raw2, dtype=np.uint8)
+    if ac.dtype != np.complex128 or ac.size != plan.half or not ac.flags.c_contiguous:
+        raise ValueError("ac must be a contiguous complex128 array of plan.half bins")
+    nfail = C.c_int64()
+    check(L.fxc_fringe_accumulate(plan.h, _vp(raw1), raw1.nbytes, _vp(raw2), raw2.nbytes,
+                                  first_frame, nf, C.byref(cfg), _vp(ac), C.byref(nfail)))
+    return nfail.value
+
+
+def fringe_accumulate_device(plan: Plan, d_raw1, d_raw2, first_frame, nf, cfg: DelayCfg, d_ac,
+                             stream=None):
+    L = _fringe_sigs(lib())
+    check(L.fxc_fringe_accumulate_device(plan.h, C.c_void_p(d_raw1), C.c_void_p(d_raw2),
+                                         first_frame, nf, C.byref(cfg), C.c_void_p(d_ac),
+                                         C.c_void_p(stream or 0)))
+
+
+def finalize_cross_spectrum(spec: np.ndarray, fft_len: int, sampling_rate_hz: float,
+                            sky_low_mhz: float, is_lsb: bool = False):
+    """xcf.rs:13-104 without plots.  Returns (spectrum with bins 0 and
+    fft_len/2 zeroed, dict(delay, slope, intercept, mag, phase, unwrapped,
+    fit)); delay is None when the fit is degenerate."""
+    L = _fringe_sigs(lib())
+    s = np.array(spec, dtype=np.complex128, copy=True)
+    n = s.size
+    mag, ph, unw, fit = (np.zeros(n) for _ in range(4))
+    r = CrossSpectrumResult()
+    check(L.fxc_finalize_cross_spectrum(_vp(s), n, fft_len, sampling_rate_hz, sky_low_mhz,
+                                        int(bool(is_lsb)), C.byref(r), _vp(mag), _vp(ph),
+                                        _vp(unw), _vp(fit)))
+    return s, dict(delay=r.delay_seconds if r.has_delay else None, slope=r.slope_rad_per_hz,
+                   intercept=r.intercept_rad, mag=mag, phase=ph, unwrapped=unw, fit=fit)

@@ -211,6 +211,44 @@ fxc_status fxc_sector_run_device(fxc_plan *plan, const uint8_t *d_raw1,
                                  double *d_acc, uint8_t *d_raw_out,
                                  void *stream);
 
+/* ----------------------------------------------------------------------- */
+/* Whole-scan cross spectrum (the reference's --fringe XCF pipeline,        */
+/* main.rs:1692-1947, replacing the per-chunk rayon map/reduce at           */
+/* 1760-1849 and the accumulation at 1850).  `ac` is half complex FP64      */
+/* values, re/im interleaved, zeroed by the caller before the first chunk;  */
+/* each call adds the chunk's XCF sum (the plan's output grid, frames that  */
+/* fail to decode skipped) times 1/N^2.  The plan must request XCF and no   */
+/* phased products.  The device variant is enqueued on `stream`.           */
+/* ----------------------------------------------------------------------- */
+fxc_status fxc_fringe_accumulate(fxc_plan *plan, const uint8_t *raw1, size_t len1,
+                                 const uint8_t *raw2, size_t len2, int64_t first_frame,
+                                 int64_t nf, const fxc_delay_cfg *delay, double *ac,
+                                 int64_t *n_failed);
+fxc_status fxc_fringe_accumulate_device(fxc_plan *plan, const uint8_t *d_raw1,
+                                        const uint8_t *d_raw2, int64_t first_frame,
+                                        int64_t nf, const fxc_delay_cfg *delay,
+                                        double *d_ac, void *stream);
+
+/* finalize_cross_spectrum (xcf.rs:13-104) without its plots: zeroes bins 0
+ * and fft_len/2 of `spec` (len complex values, re/im interleaved) in place,
+ * then returns per bin the magnitude (bin 0: the mean of the others halved),
+ * the phase (safe_arg, utils.rs:425), the phase unwrapped from bin 1 on
+ * (utils.rs:412-424) and the least-squares line through it against sky
+ * frequency (USB: sky_low + k df; LSB: sky_low + bw - k df) over bins
+ * 1 .. len-2 (even fft_len), whose slope gives delay = -slope / 2 pi.
+ * Output arrays may be NULL. */
+typedef struct {
+    int32_t has_delay;            /* 0 when the fit is degenerate (xcf.rs:88)  */
+    double delay_seconds;
+    double slope_rad_per_hz;
+    double intercept_rad;
+} fxc_cross_spectrum_result;
+fxc_status fxc_finalize_cross_spectrum(double *spec, int64_t len, int32_t fft_len,
+                                       double sampling_rate_hz, double sky_low_mhz,
+                                       int32_t is_lsb, fxc_cross_spectrum_result *res,
+                                       double *mag_out, double *phase_out,
+                                       double *unwrapped_out, double *fit_out);
+
 /* Kernel timing of the dominant (fused per-frame) kernel measured with HIP
  * events on the launch stream; enabled per plan. */
 fxc_status fxc_plan_set_timing(fxc_plan *plan, int32_t enable);

@@ -364,3 +364,67 @@ def sector_run(desc: dict, raw1: np.ndarray, raw2: np.ndarray, first_frame: int,
     return dict(acc_ph=acc_ph, acc11=acc11, acc22=acc22,
                 acc12=acc12.view(np.complex128), raw_out=raw_out[:nf * bpf_o],
                 n_failed=nfail.value)
+
+
+# --------------------------------------------------- whole-scan cross spectrum
+def finalize_cross_spectrum(spec, fft_len, sampling_rate_hz, sky_low_mhz, is_lsb=False):
+    """xcf.rs:13-104 (plots left out), restated in plain Python over f64.
+
+    Bins 0 and fft_len/2 (when present) are zeroed (xcf.rs:26-30); per bin
+    magnitude (hypot) and phase (safe_arg, utils.rs:425); the DC magnitude is
+    replaced by the mean of the others halved (xcf.rs:39-42); the phase of
+    bins 1.. is unwrapped among themselves (xcf.rs:44-53, utils.rs:412-424);
+    the sky frequency of bin i is sky_low + i df (USB) or sky_low + bw - i df
+    (LSB) (xcf.rs:55-63); a least-squares line through (f, phase) over bins
+    [1, len) or [1, len - 1) for even fft_len gives delay = -slope / 2 pi
+    (xcf.rs:65-101)."""
+    import math
+    s = [complex(v) for v in spec]
+    n = len(s)
+    s[0] = 0j
+    if fft_len // 2 < n:
+        s[fft_len // 2] = 0j
+    mag = [math.hypot(v.real, v.imag) for v in s]
+    ph = [0.0 if (v.real == 0.0 and v.imag == 0.0) else math.atan2(v.imag, v.real) for v in s]
+    if n > 1:
+        rest = 0.0
+        for v in mag[1:]:
+            rest += v
+        mag[0] = rest / (n - 1) / 2.0
+    unw = list(ph)
+    offset = 0.0
+    for i in range(2, n):
+        d = ph[i] - ph[i - 1]
+        if d > math.pi:
+            offset -= 2.0 * math.pi
+        elif d < -math.pi:
+            offset += 2.0 * math.pi
+        unw[i] = ph[i] + offset
+    bw_mhz = sampling_rate_hz / 2.0 / 1e6
+    df_hz = sampling_rate_hz / fft_len
+    fhz = []
+    for i in range(n):
+        fm = (sky_low_mhz + bw_mhz) - (i * df_hz) / 1e6 if is_lsb else sky_low_mhz + (i * df_hz) / 1e6
+        fhz.append(fm * 1.0e6)
+    delay = slope = icpt = None
+    if n > 1:
+        end = n - 1 if (fft_len % 2 == 0 and n > 1) else n
+        if end > 1:
+            xs, ys = fhz[1:end], unw[1:end]
+            cnt = float(len(xs))
+            sx = sy = sxy = sx2 = 0.0
+            for x in xs:
+                sx += x
+            for y in ys:
+                sy += y
+            for x, y in zip(xs, ys):
+                sxy += x * y
+            for x in xs:
+                sx2 += x * x
+            den = cnt * sx2 - sx * sx
+            if abs(den) > 1e-9:
+                slope = (cnt * sxy - sx * sy) / den
+                icpt = (sy - slope * sx) / cnt
+                delay = -slope / (2.0 * math.pi)
+    return (np.array(s), dict(delay=delay, slope=slope, intercept=icpt, mag=np.array(mag),
+                              phase=np.array(ph), unwrapped=np.array(unw), fhz=np.array(fhz)))
