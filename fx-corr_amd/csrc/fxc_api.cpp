@@ -1537,3 +1537,126 @@ extern "C" fxc_status fxc_synth_station_device(uint8_t *d_out, int64_t nsamples,
                          delay_samples, (hipStream_t)stream));
     return FXC_OK;
 }
+
+// ============================================================ host pipeline
+// Streaming host input (the reader-thread + per-sector loop of
+// main.rs:2098-2213): sectors submitted from host memory are copied on a
+// copy stream into one of `depth` device slots while earlier sectors run on
+// the compute stream, and each sector's spectra come back into the caller's
+// memory.  With page-locked input (fxc_host_alloc) the copies run at PCIe
+// rate and overlap the kernels.
+extern "C" fxc_status fxc_host_alloc(size_t bytes, void **out) {
+    if (!out) return fail(FXC_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    HIP_TRY(hipHostMalloc(out, bytes ? bytes : 1, hipHostMallocDefault));
+    return FXC_OK;
+}
+
+extern "C" fxc_status fxc_host_free(void *p) {
+    if (p) HIP_TRY(hipHostFree(p));
+    return FXC_OK;
+}
+
+struct fxc_pipeline {
+    fxc_plan *plan = nullptr;
+    int64_t max_frames = 0;
+    int depth = 0;
+    hipStream_t copy = nullptr, compute = nullptr;
+    struct Slot {
+        DevBuf raw[2], acc;
+        hipEvent_t copied = nullptr, done = nullptr;
+        int64_t ticket = -1;
+    };
+    std::vector<Slot> slots;
+    int64_t next_ticket = 0;
+};
+
+static void pipeline_release(fxc_pipeline *pl) {
+    if (!pl) return;
+    if (pl->copy) (void)hipStreamSynchronize(pl->copy);
+    if (pl->compute) (void)hipStreamSynchronize(pl->compute);
+    for (auto &s : pl->slots) {
+        s.raw[0].release();
+        s.raw[1].release();
+        s.acc.release();
+        if (s.copied) (void)hipEventDestroy(s.copied);
+        if (s.done) (void)hipEventDestroy(s.done);
+    }
+    if (pl->copy) (void)hipStreamDestroy(pl->copy);
+    if (pl->compute) (void)hipStreamDestroy(pl->compute);
+    delete pl;
+}
+
+extern "C" fxc_status fxc_pipeline_create(fxc_plan *p, int64_t max_frames, int32_t depth,
+                                          fxc_pipeline **out) {
+    if (!p || !out || max_frames < 1 || depth < 1 || depth > 8)
+        return fail(FXC_ERR_INVALID_ARG, "pipeline needs a plan, max_frames >= 1 and depth 1..8");
+    if (p->desc.products & FXC_PRODUCT_RAW_OUT)
+        return fail(FXC_ERR_UNSUPPORTED, "the pipeline returns spectra; .raw output uses fxc_sector_run");
+    *out = nullptr;
+    auto *pl = new fxc_pipeline();
+    pl->plan = p;
+    pl->max_frames = max_frames;
+    pl->depth = depth;
+    pl->slots.resize((size_t)depth);
+    auto bail = [&](fxc_status st) { pipeline_release(pl); return st; };
+    if (hipStreamCreateWithFlags(&pl->copy, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&pl->compute, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(FXC_ERR_HIP, "stream creation failed"));
+    const size_t acc_bytes = (size_t)kNumProducts * p->half * sizeof(double);
+    for (auto &s : pl->slots) {
+        if (s.raw[0].ensure((size_t)max_frames * p->st[0].bpf) != hipSuccess ||
+            s.raw[1].ensure((size_t)max_frames * p->st[1].bpf) != hipSuccess ||
+            s.acc.ensure(acc_bytes) != hipSuccess)
+            return bail(fail(FXC_ERR_NOMEM, "pipeline slot allocation failed"));
+        if (hipEventCreateWithFlags(&s.copied, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&s.done, hipEventDisableTiming) != hipSuccess)
+            return bail(fail(FXC_ERR_HIP, "event creation failed"));
+    }
+    *out = pl;
+    return FXC_OK;
+}
+
+extern "C" fxc_status fxc_pipeline_submit(fxc_pipeline *pl, const uint8_t *raw1, const uint8_t *raw2,
+                                          int64_t first_frame, int64_t nf, const fxc_delay_cfg *delay,
+                                          double *acc_out, int64_t *ticket) {
+    if (!pl || !delay || !acc_out || !ticket) return fail(FXC_ERR_INVALID_ARG, "null argument");
+    if (nf < 0 || nf > pl->max_frames)
+        return fail(FXC_ERR_INVALID_ARG, "nf %lld outside 0..%lld", (long long)nf, (long long)pl->max_frames);
+    if (nf > 0 && (!raw1 || !raw2)) return fail(FXC_ERR_INVALID_ARG, "null raw buffer");
+    fxc_plan *p = pl->plan;
+    const int64_t t = pl->next_ticket;
+    auto &s = pl->slots[(size_t)(t % pl->depth)];
+    // the slot's previous sector must have finished reading its input
+    if (s.ticket >= 0) HIP_TRY(hipStreamWaitEvent(pl->copy, s.done, 0));
+    const size_t n1 = (size_t)nf * p->st[0].bpf, n2 = (size_t)nf * p->st[1].bpf;
+    if (nf > 0) {
+        HIP_TRY(hipMemcpyAsync(s.raw[0].p, raw1, n1, hipMemcpyHostToDevice, pl->copy));
+        HIP_TRY(hipMemcpyAsync(s.raw[1].p, raw2, n2, hipMemcpyHostToDevice, pl->copy));
+    }
+    HIP_TRY(hipEventRecord(s.copied, pl->copy));
+    HIP_TRY(hipStreamWaitEvent(pl->compute, s.copied, 0));
+    fxc_status st = enqueue_sector(p, (const uint8_t *)s.raw[0].p, (const uint8_t *)s.raw[1].p,
+                                   first_frame, nf, *delay, (double *)s.acc.p, nullptr, pl->compute);
+    if (st != FXC_OK) return st;
+    HIP_TRY(hipMemcpyAsync(acc_out, s.acc.p, (size_t)kNumProducts * p->half * sizeof(double),
+                           hipMemcpyDeviceToHost, pl->compute));
+    HIP_TRY(hipEventRecord(s.done, pl->compute));
+    s.ticket = t;
+    pl->next_ticket = t + 1;
+    *ticket = t;
+    return FXC_OK;
+}
+
+extern "C" fxc_status fxc_pipeline_wait(fxc_pipeline *pl, int64_t ticket) {
+    if (!pl || ticket < 0 || ticket >= pl->next_ticket) return fail(FXC_ERR_INVALID_ARG, "unknown ticket");
+    auto &s = pl->slots[(size_t)(ticket % pl->depth)];
+    // a slot reused since then: its later sector completes after this one
+    HIP_TRY(hipEventSynchronize(s.done));
+    return FXC_OK;
+}
+
+extern "C" fxc_status fxc_pipeline_destroy(fxc_pipeline *pl) {
+    pipeline_release(pl);
+    return FXC_OK;
+}

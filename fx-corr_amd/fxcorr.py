@@ -820,3 +820,86 @@ def finalize_cross_spectrum(spec: np.ndarray, fft_len: int, sampling_rate_hz: fl
                                         _vp(unw), _vp(fit)))
     return s, dict(delay=r.delay_seconds if r.has_delay else None, slope=r.slope_rad_per_hz,
                    intercept=r.intercept_rad, mag=mag, phase=ph, unwrapped=unw, fit=fit)
+
+
+# ------------------------------------------------------ streaming host input
+EXPORTED += ["fxc_host_alloc", "fxc_host_free", "fxc_pipeline_create", "fxc_pipeline_submit",
+             "fxc_pipeline_wait", "fxc_pipeline_destroy"]
+
+
+def _pipe_sigs(L):
+    if not getattr(L, "_pipe_sigs", False):
+        vp, i32, i64 = C.c_void_p, C.c_int32, C.c_int64
+        L.fxc_host_alloc.argtypes = [C.c_size_t, _P(vp)]
+        L.fxc_host_alloc.restype = i32
+        L.fxc_host_free.argtypes = [vp]
+        L.fxc_host_free.restype = i32
+        L.fxc_pipeline_create.argtypes = [vp, i64, i32, _P(vp)]
+        L.fxc_pipeline_create.restype = i32
+        L.fxc_pipeline_submit.argtypes = [vp, vp, vp, i64, i64, _P(DelayCfg), vp, _P(i64)]
+        L.fxc_pipeline_submit.restype = i32
+        L.fxc_pipeline_wait.argtypes = [vp, i64]
+        L.fxc_pipeline_wait.restype = i32
+        L.fxc_pipeline_destroy.argtypes = [vp]
+        L.fxc_pipeline_destroy.restype = i32
+        L._pipe_sigs = True
+    return L
+
+
+class HostBuffer:
+    """Page-locked host memory (fxc_host_alloc) viewed as a numpy array."""
+
+    def __init__(self, nbytes: int):
+        L = _pipe_sigs(lib())
+        p = C.c_void_p()
+        check(L.fxc_host_alloc(nbytes, C.byref(p)))
+        self.ptr, self.nbytes = p.value, nbytes
+        self.array = np.ctypeslib.as_array((C.c_uint8 * nbytes).from_address(self.ptr))
+
+    def view(self, dtype):
+        return self.array.view(dtype)
+
+    def free(self):
+        if self.ptr:
+            _pipe_sigs(lib()).fxc_host_free(C.c_void_p(self.ptr))
+            self.ptr = 0
+            self.array = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Pipeline:
+    """fxc_pipeline: sectors from host memory, copies overlapping kernels."""
+
+    def __init__(self, plan: Plan, max_frames: int, depth: int = 2):
+        L = _pipe_sigs(lib())
+        h = C.c_void_p()
+        check(L.fxc_pipeline_create(plan.h, max_frames, depth, C.byref(h)))
+        self.h, self.plan = h, plan
+
+    def submit(self, raw1_ptr: int, raw2_ptr: int, first_frame: int, nf: int, cfg: DelayCfg,
+               acc_out_ptr: int) -> int:
+        t = C.c_int64()
+        check(_pipe_sigs(lib()).fxc_pipeline_submit(self.h, C.c_void_p(raw1_ptr),
+                                                    C.c_void_p(raw2_ptr), first_frame, nf,
+                                                    C.byref(cfg), C.c_void_p(acc_out_ptr),
+                                                    C.byref(t)))
+        return t.value
+
+    def wait(self, ticket: int):
+        check(_pipe_sigs(lib()).fxc_pipeline_wait(self.h, ticket))
+
+    def close(self):
+        if getattr(self, "h", None):
+            _pipe_sigs(lib()).fxc_pipeline_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

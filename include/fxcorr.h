@@ -249,6 +249,30 @@ fxc_status fxc_finalize_cross_spectrum(double *spec, int64_t len, int32_t fft_le
                                        double *mag_out, double *phase_out,
                                        double *unwrapped_out, double *fit_out);
 
+/* ----------------------------------------------------------------------- */
+/* Streaming host input (replaces the reader thread + per-sector loop of    */
+/* main.rs:2098-2213 for callers whose sector bytes are in host memory).   */
+/* fxc_host_alloc returns page-locked memory: a reader filling such buffers */
+/* lets the copies run at PCIe rate.  A pipeline owns `depth` device slots  */
+/* of max_frames frames, a copy stream and a compute stream: submit copies  */
+/* a sector into the next slot (after that slot's previous sector has read  */
+/* it), runs it on the compute stream behind the sectors submitted before,  */
+/* and copies its spectra (5 x half f64, the fxc_sector_run_device layout)  */
+/* into acc_out; it returns at once with a ticket.  raw1/raw2 must stay     */
+/* unchanged, and acc_out is only valid, once fxc_pipeline_wait(ticket)     */
+/* has returned.  One pipeline per plan; plans without .raw output.        */
+/* ----------------------------------------------------------------------- */
+fxc_status fxc_host_alloc(size_t bytes, void **out);
+fxc_status fxc_host_free(void *p);
+typedef struct fxc_pipeline fxc_pipeline;
+fxc_status fxc_pipeline_create(fxc_plan *plan, int64_t max_frames, int32_t depth,
+                               fxc_pipeline **out);
+fxc_status fxc_pipeline_submit(fxc_pipeline *pl, const uint8_t *raw1, const uint8_t *raw2,
+                               int64_t first_frame, int64_t nf, const fxc_delay_cfg *delay,
+                               double *acc_out, int64_t *ticket);
+fxc_status fxc_pipeline_wait(fxc_pipeline *pl, int64_t ticket);
+fxc_status fxc_pipeline_destroy(fxc_pipeline *pl);
+
 /* Kernel timing of the dominant (fused per-frame) kernel measured with HIP
  * events on the launch stream; enabled per plan. */
 fxc_status fxc_plan_set_timing(fxc_plan *plan, int32_t enable);
