@@ -132,58 +132,118 @@ def pmc_summary(fetch_dir, write_dir, out_json, copy_mib):
     print(json.dumps(out, indent=1, sort_keys=True))
 
 
-def serve_bench(raw1, raw2, bpf, cfg, shuffle_b, n_jobs, load, sector_ms, seed=7):
-    """Continuous-batching run: Poisson arrivals of one-second scan jobs, each
-    pushed as four 64 MB chunks per station (250,000 frames, 0.25 s at
-    1024 Msps 2-bit) from the device-resident pool, admitted under a 200 GB
-    budget; returns throughput and job-latency percentiles."""
+def _mix_types(stream, seed=5, n_types=4, pool_bytes=256 << 20):
+    """Seeded job types of the mixed-scan set (FFT 256/1024/4096/16384, 1/2/4/8
+    bits per station, 128/256/512/1024 Msps), each with a device pool of
+    synthetic correlated data (pool_bytes per station at most)."""
     rng = np.random.default_rng(seed)
-    chunk = 250_000
-    per_job = 4 * chunk
-    pool = (int(SECONDS * FS / N) // per_job)           # distinct one-second slices
-    plan_kw = dict(fft_len=N, bits=(BITS, BITS), levels=[LEVELS, LEVELS],
-                   shuffles=(None, shuffle_b), lsb=(False, True), band=(0, 0, N // 2, 0),
-                   df_hz=FS / N, products=F.PRODUCT_ACF | F.PRODUCT_XCF,
-                   precision=F.PRECISION_CHOSEN)
-    sv = F.Server(hbm_budget_bytes=200 << 30, max_active_jobs=8, n_streams=4)
-    # warm the plan cache (first plan creation uploads the tables)
-    j0 = sv.submit(plan_kw, cfg)
-    sv.push(j0, raw1.ptr, raw2.ptr, chunk)
-    sv.close_job(j0)
-    if not sv.wait(j0, 120.0):
-        raise RuntimeError("serving warm-up job did not finish")
-    sv.release(j0)
-    service_s = sector_ms * 1e-3 * per_job / 1_000_000   # fused-kernel time of one job
-    rate = load / service_s
+    out = []
+    for _ in range(n_types):
+        n = int(rng.choice([256, 1024, 4096, 16384]))
+        bits = (int(rng.choice([1, 2, 4, 8])), int(rng.choice([1, 2, 4, 8])))
+        fs = float(rng.choice([128e6, 256e6, 512e6, 1024e6]))
+        bpf = (n * bits[0] // 8, n * bits[1] // 8)
+        nf = pool_bytes // max(bpf)
+        d = (F.DeviceBuffer(nf * bpf[0]), F.DeviceBuffer(nf * bpf[1]))
+        for st in range(2):
+            F.synth_station_device(d[st].ptr, nf * n, bits[st], None, False, 11, st, 0.01, 3,
+                                   stream.ptr)
+        lv = [np.arange(1 << b, dtype=np.float64) - ((1 << b) - 1) / 2.0 for b in bits]
+        plan_kw = dict(fft_len=n, bits=bits, levels=lv, df_hz=fs / n,
+                       products=F.PRODUCT_ACF | F.PRODUCT_XCF)
+        cfg = F.make_delay_cfg(np.zeros(4), frame_dt=n / fs, fs=fs, delay_user_samples=3.2,
+                               extra_delay_rate_sps=1e-9, lo1_hz=6.7e9, lo2_hz=6.7e9)
+        out.append(dict(n=n, bits=bits, fs=fs, bpf=bpf, nf=nf, dev=d, plan_kw=plan_kw, cfg=cfg))
+    stream.sync()
+    return out
+
+
+def serve_bench(raw1, raw2, bpf, cfg, shuffle_b, n_jobs, load, seed=7):
+    """Served correlation stream.  Jobs arrive as a Poisson process; 80 % are
+    1024 Msps 2-bit FFT-1024 scans (the device-resident workload pool), 20 %
+    are drawn from four seeded types of the mixed-scan set.  Durations are
+    log-normal (median 4 s of data, at most 60 s).  Each station's data is
+    pushed in 64 MB chunks, admission is bounded by a 200 GB HBM budget, and
+    every job runs the chosen form.  The same job list is then run with the
+    plain FP32 form one job at a time, for the throughput comparison."""
+    stream = F.Stream()
+    mix = _mix_types(stream)
+    rng = np.random.default_rng(seed)
+    main_pool = int(SECONDS * FS / N)
+    main = dict(n=N, fs=FS, bpf=(bpf, bpf), nf=main_pool, dev=(raw1, raw2), cfg=cfg,
+                plan_kw=dict(fft_len=N, bits=(BITS, BITS), levels=[LEVELS, LEVELS],
+                             shuffles=(None, shuffle_b), lsb=(False, True),
+                             band=(0, 0, N // 2, 0), df_hz=FS / N,
+                             products=F.PRODUCT_ACF | F.PRODUCT_XCF))
+    # job list: type, duration (s of data) -> frames
+    sigma = math.log(15.0) / 3.0                          # 60 s at about +3 sigma of 4 s
     jobs = []
-    t0 = time.perf_counter()
-    t_next = t0
-    for i in range(n_jobs):
-        t_next += rng.exponential(1.0 / rate)
-        while time.perf_counter() < t_next:
-            pass
-        k = i % pool
-        jid = sv.submit(plan_kw, cfg, first_frame=k * per_job)
-        for c in range(4):
-            off = k * per_job + c * chunk
-            sv.push(jid, raw1.ptr + off * bpf, raw2.ptr + off * bpf, chunk)
-        sv.close_job(jid)
-        jobs.append(jid)
-    for jid in jobs:
-        if not sv.wait(jid, 120.0):
-            raise RuntimeError(f"serving job {jid} did not finish")
-    wall = time.perf_counter() - t0
-    st = sv.stats()
-    sv.close()
-    samples = n_jobs * per_job * N
-    return {"jobs": n_jobs, "offered_load": load, "job": "1 s scan, 4 x 64 MB chunks per station",
+    for _ in range(n_jobs):
+        t = main if rng.random() < 0.8 else mix[int(rng.integers(len(mix)))]
+        dur = min(60.0, 4.0 * math.exp(sigma * rng.standard_normal()))
+        jobs.append((t, max(1, int(dur * t["fs"] / t["n"]))))
+
+    def run(precision, max_active, arrivals):
+        sv = F.Server(hbm_budget_bytes=200 << 30, max_active_jobs=max_active, n_streams=4)
+        for t in [main] + mix:                            # warm the plan cache
+            kw = dict(t["plan_kw"], precision=precision)
+            j = sv.submit(kw, t["cfg"])
+            sv.push(j, t["dev"][0].ptr, t["dev"][1].ptr, min(1024, t["nf"]))
+            sv.close_job(j)
+            if not sv.wait(j, 120.0):
+                raise RuntimeError("serving warm-up job did not finish")
+            sv.release(j)
+        ids, samples = [], 0
+        t0 = time.perf_counter()
+        t_next = t0
+        for (t, nfj), gap in zip(jobs, arrivals):
+            t_next += gap
+            while time.perf_counter() < t_next:
+                pass
+            kw = dict(t["plan_kw"], precision=precision)
+            jid = sv.submit(kw, t["cfg"])
+            chunk = (64 << 20) // max(t["bpf"])
+            done = 0
+            while done < nfj:
+                k = min(chunk, nfj - done)
+                off = done % max(1, t["nf"] - k)          # cycle through the data pool
+                sv.push(jid, t["dev"][0].ptr + off * t["bpf"][0], t["dev"][1].ptr + off * t["bpf"][1], k)
+                done += k
+            sv.close_job(jid)
+            ids.append(jid)
+            samples += nfj * t["n"]
+        for jid in ids:
+            if not sv.wait(jid, 600.0):
+                raise RuntimeError(f"serving job {jid} did not finish")
+        wall = time.perf_counter() - t0
+        st = sv.stats()
+        sv.close()
+        return samples, wall, st
+
+    # service rate of the chosen form, from one closed-loop pass (no arrivals)
+    samples, wall, _ = run(F.PRECISION_CHOSEN, 8, [0.0] * n_jobs)
+    print(f"[bench] serving: closed-loop pass {samples / wall / 1e6:,.0f} Msamples/s/station "
+          f"({wall:.1f} s)", file=sys.stderr, flush=True)
+    rate = load * n_jobs / wall
+    gaps = list(rng.exponential(1.0 / rate, size=n_jobs))
+    samples, wall, st = run(F.PRECISION_CHOSEN, 8, gaps)
+    print(f"[bench] serving: Poisson pass done ({wall:.1f} s)", file=sys.stderr, flush=True)
+    p_samples, p_wall, p_st = run(F.PRECISION_PLAIN_FP32, 1, [0.0] * n_jobs)
+    return {"jobs": n_jobs, "offered_load": load,
+            "job": "80% 1024 Msps 2-bit FFT-1024, 20% from 4 mixed-scan types; log-normal "
+                   "durations (median 4 s, max 60 s of data); 64 MB chunks per station",
             "value": round(samples / wall / 1e6, 1), "unit": "Msamples/s/station",
             "p50_latency_ms": round(st.p50_latency_ms, 3),
             "p99_latency_ms": round(st.p99_latency_ms, 3),
             "mean_latency_ms": round(st.mean_latency_ms, 3),
             "max_concurrent_jobs": st.max_concurrent_jobs,
             "peak_hbm_gb": round(st.peak_hbm_bytes / 1e9, 3),
-            "failed": st.jobs_failed}
+            "failed": st.jobs_failed,
+            "plain_one_at_a_time": {"value": round(p_samples / p_wall / 1e6, 1),
+                                    "p50_latency_ms": round(p_st.p50_latency_ms, 3),
+                                    "p99_latency_ms": round(p_st.p99_latency_ms, 3)},
+            "mix_types": [f"FFT {t['n']}, {t['bits'][0]}/{t['bits'][1]}-bit, "
+                          f"{t['fs'] / 1e6:.0f} Msps" for t in mix]}
 
 
 def load_traffic(key):
@@ -331,13 +391,11 @@ def main():
     main_form = "chosen" if "chosen" in results else next(iter(results))
     serving = None
     if args.serve_jobs > 0:
-        sector_ms = results[main_form]["avg_launch_ms"] * 1_000_000 / (total_frames / len(sectors))
-        serving = serve_bench(raw1, raw2, bpf, cfg, shuffle_b, args.serve_jobs, args.serve_load,
-                              sector_ms)
+        serving = serve_bench(raw1, raw2, bpf, cfg, shuffle_b, args.serve_jobs, args.serve_load)
         print(f"[bench] serving: {serving['value']:,.0f} Msamples/s/station, job latency p50 "
               f"{serving['p50_latency_ms']:.2f} ms p99 {serving['p99_latency_ms']:.2f} ms, "
-              f"up to {serving['max_concurrent_jobs']} jobs in flight", file=sys.stderr,
-              flush=True)
+              f"up to {serving['max_concurrent_jobs']} jobs in flight; plain one job at a time "
+              f"{serving['plain_one_at_a_time']['value']:,.0f}", file=sys.stderr, flush=True)
     h1 = h2 = None
     if not (args.no_cpu_baseline and args.no_pcie):
         nfc = min(args.cpu_frames, sectors[0])
