@@ -4,9 +4,9 @@
 Workload: two stations, 1024 Msps, 2-bit, FFT 1024, 8 s each (2.048 GB packed
 per station, 8,000,000 frames, 8 one-second sectors), station B LSB with a
 seeded 32-entry bit shuffle, geometric delay from two ECEF positions ~108 m
-apart toward a seeded RA/Dec + clock 1.2 us / 1e-12 s/s + residual 0.2
-samples / 0.05 Hz, correlated Gaussian signal (rho = 0.005) generated on the
-device.  One step = the whole 8-sector job (ACF + XCF) with the input already
+apart toward a seeded J2000 RA/Dec (the library's 1-s geometric table), clock
+1.2 us / 1e-12 s/s, residual 0.2 samples / 0.05 Hz, correlated Gaussian
+signal (rho = 0.005) generated on the device.  One step = the whole 8-sector job (ACF + XCF) with the input already
 resident in HBM.  Both arithmetic forms are timed; `value` is the chosen form.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
@@ -37,28 +37,18 @@ OBS_HZ = 6.7e9
 LEVELS = np.array([-3.0, -1.0, 1.0, 3.0])
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
-C_LIGHT = 299792458.0
 ECEF1 = np.array([-3.50e6, 3.66e6, 3.50e6])
 ECEF2 = ECEF1 + np.array([-22.989, -80.501, 67.923])
-
-
-def geometric_table(ra, dec, mjd0, nsec):
-    """1-s baseline delay table -(B . s)/c, s from Earth rotation angle (input only)."""
-    b = ECEF2 - ECEF1
-    out = []
-    for sec in range(nsec + 1):
-        jd = mjd0 + sec / 86400.0 + 2400000.5
-        era = 2 * math.pi * ((0.7790572732640 + 1.00273781191135448 * (jd - 2451545.0)) % 1.0)
-        ha = era - ra
-        s = np.array([math.cos(dec) * math.cos(ha), -math.cos(dec) * math.sin(ha), math.sin(dec)])
-        out.append(-(b @ s) / C_LIGHT)
-    return np.array(out)
+MJD0 = 60000.25
 
 
 def workload_setup(seed=2025):
     rng = np.random.default_rng(seed)
     ra, dec = rng.uniform(0, 2 * math.pi), math.radians(rng.uniform(-30, 80))
-    table = geometric_table(ra, dec, 60000.25, int(SECONDS))
+    # J2000 source precessed to the date; 1-s table over the 8 s scan (main.rs:1628-1642)
+    ra, dec = F.precess_j2000_to_date(ra, dec, MJD0)
+    n_points = F.geometric_table_points(int(SECONDS * FS / N), N / FS)
+    table = F.geometric_delay_table(ECEF1, ECEF2, ra, dec, MJD0, 0.0, n_points)
     clock2, clock2_rate, user, rate_user = 1.2e-6, 1e-12, 0.2, 0.05
     net_d0 = table[0] + user / FS + clock2                       # main.rs:1183-1184
     init = int(round(net_d0 * FS))                               # main.rs:1279

@@ -903,3 +903,96 @@ class Pipeline:
             self.close()
         except Exception:
             pass
+
+
+# ------------------------------------------------------------ station geometry
+EXPORTED += ["fxc_parse_ra", "fxc_parse_dec", "fxc_parse_epoch_mjd", "fxc_precess_j2000_to_date",
+             "fxc_sidereal_time", "fxc_geometric_delay", "fxc_geometric_delay_table",
+             "fxc_geometric_table_points"]
+
+
+class GeometricDelay(C.Structure):
+    _fields_ = [("delay1_s", C.c_double), ("delay2_s", C.c_double), ("delay_s", C.c_double),
+                ("rate_sps", C.c_double), ("accel_sps2", C.c_double)]
+
+
+def _geom_sigs(L):
+    if not getattr(L, "_geom_sigs", False):
+        d, pd, i64, i32 = C.c_double, _P(C.c_double), C.c_int64, C.c_int32
+        for name in ("fxc_parse_ra", "fxc_parse_dec", "fxc_parse_epoch_mjd"):
+            getattr(L, name).argtypes = [C.c_char_p, pd]
+            getattr(L, name).restype = i32
+        L.fxc_precess_j2000_to_date.argtypes = [d, d, d, pd, pd]
+        L.fxc_precess_j2000_to_date.restype = i32
+        L.fxc_sidereal_time.argtypes = [d, pd, pd]
+        L.fxc_sidereal_time.restype = i32
+        L.fxc_geometric_delay.argtypes = [pd, pd, d, d, d, _P(GeometricDelay)]
+        L.fxc_geometric_delay.restype = i32
+        L.fxc_geometric_delay_table.argtypes = [pd, pd, d, d, d, d, i64, pd]
+        L.fxc_geometric_delay_table.restype = i32
+        L.fxc_geometric_table_points.argtypes = [i64, d]
+        L.fxc_geometric_table_points.restype = i64
+        L._geom_sigs = True
+    return L
+
+
+def _parse(name, text):
+    out = C.c_double()
+    check(getattr(_geom_sigs(lib()), name)(text.encode(), C.byref(out)))
+    return out.value
+
+
+def parse_ra(text: str) -> float:
+    """geom.rs:86-111: radians."""
+    return _parse("fxc_parse_ra", text)
+
+
+def parse_dec(text: str) -> float:
+    """geom.rs:115-148: radians."""
+    return _parse("fxc_parse_dec", text)
+
+
+def parse_epoch_mjd(text: str) -> float:
+    """geom.rs:226-257: MJD."""
+    return _parse("fxc_parse_epoch_mjd", text)
+
+
+def precess_j2000_to_date(ra: float, dec: float, mjd: float):
+    """geom.rs:265-277: (ra, dec) of date."""
+    r, d = C.c_double(), C.c_double()
+    check(_geom_sigs(lib()).fxc_precess_j2000_to_date(ra, dec, mjd, C.byref(r), C.byref(d)))
+    return r.value, d.value
+
+
+def sidereal_time(mjd: float):
+    """(GMST, GAST) in radians (geom.rs:279-297)."""
+    g, a = C.c_double(), C.c_double()
+    check(_geom_sigs(lib()).fxc_sidereal_time(mjd, C.byref(g), C.byref(a)))
+    return g.value, a.value
+
+
+def _ecef(x):
+    a = (C.c_double * 3)(*[float(v) for v in x])
+    return a
+
+
+def geometric_delay(ant1_ecef, ant2_ecef, ra: float, dec: float, mjd: float):
+    """geom.rs:346-384: (delay1, delay2, delay, rate, accel)."""
+    r = GeometricDelay()
+    check(_geom_sigs(lib()).fxc_geometric_delay(_ecef(ant1_ecef), _ecef(ant2_ecef), ra, dec, mjd,
+                                                C.byref(r)))
+    return r.delay1_s, r.delay2_s, r.delay_s, r.rate_sps, r.accel_sps2
+
+
+def geometric_table_points(total_frames: int, frame_dt: float) -> int:
+    return int(_geom_sigs(lib()).fxc_geometric_table_points(total_frames, frame_dt))
+
+
+def geometric_delay_table(ant1_ecef, ant2_ecef, ra: float, dec: float, mjd: float,
+                          time_offset_s: float, n_points: int) -> np.ndarray:
+    """main.rs:1628-1642: baseline delay at each whole second of the process."""
+    out = np.zeros(n_points)
+    check(_geom_sigs(lib()).fxc_geometric_delay_table(
+        _ecef(ant1_ecef), _ecef(ant2_ecef), ra, dec, mjd, time_offset_s, n_points,
+        out.ctypes.data_as(_P(C.c_double))))
+    return out

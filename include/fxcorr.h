@@ -489,6 +489,52 @@ fxc_status fxc_cor_write_sector(fxc_cor_writer *w, int64_t unix_sec, float integ
                                 const fxc_cor_sector_model *model);
 fxc_status fxc_cor_finalize(fxc_cor_writer *w);
 
+/* ----------------------------------------------------------------------- */
+/* Station geometry (geom.rs) and the 1-second geometric-delay table of    */
+/* the delay model (main.rs:1628-1642).  Host functions.  Sidereal time is */
+/* IAU 2006 GMST plus the equation of the equinoxes with the IAU 2000B     */
+/* nutation series (within 1 mas of the 2000A series behind ERFA's         */
+/* eraGst06a); precession is ERFA's eraPmat06 model (IAU 2006 with frame   */
+/* bias).  MJDs stand for both UT1 and TT, as in geom.rs:282-285.          */
+/* ----------------------------------------------------------------------- */
+typedef struct {
+    double delay1_s, delay2_s;   /* -(x_i . s)/c per antenna at the epoch           */
+    double delay_s;              /* baseline delay -((x2 - x1) . s)/c                */
+    double rate_sps;             /* central difference over +-1 s                    */
+    double accel_sps2;           /* second difference over +-1 s                     */
+} fxc_geometric_delay_result;
+
+/* parse_ra (geom.rs:86-111): "16h42m58.8s", "16:42:58.8", packed 164258.8
+ * (|v| >= 10000) are hours; a plain number below 10000 is degrees. */
+fxc_status fxc_parse_ra(const char *text, double *radians);
+/* parse_dec (geom.rs:115-148): "+39d48m36.0s", "39:48:36", packed 394836.0,
+ * or decimal degrees; a leading '-' negates. */
+fxc_status fxc_parse_dec(const char *text, double *radians);
+/* parse_epoch_to_mjd (geom.rs:226-257): ISO "2024-02-12T15:52:00Z",
+ * "YYYY/DDD [HH:MM:SS]", an MJD in [40000, 100000) or a Julian year in
+ * [1800, 3000) (optionally prefixed J). */
+fxc_status fxc_parse_epoch_mjd(const char *text, double *mjd);
+/* precess_j2000_to_mean_of_date (geom.rs:265-277). */
+fxc_status fxc_precess_j2000_to_date(double ra, double dec, double mjd, double *ra_date,
+                                     double *dec_date);
+/* mjd_to_gmst (geom.rs:279-287) and the apparent sidereal time of
+ * source_vector_itrs (geom.rs:289-297), radians in [0, 2 pi); either
+ * output may be NULL. */
+fxc_status fxc_sidereal_time(double mjd, double *gmst, double *gast);
+/* calculate_geometric_delay_and_derivatives (geom.rs:346-384): ra/dec of
+ * date (radians), ECEF positions in metres. */
+fxc_status fxc_geometric_delay(const double ant1_ecef[3], const double ant2_ecef[3], double ra,
+                               double dec, double mjd, fxc_geometric_delay_result *out);
+/* The 1-second table (main.rs:1628-1642): table[k] = baseline delay at
+ * mjd + (time_offset_s + k)/86400 for k < n_points; the delay model reads
+ * it as fxc_delay_cfg.geom_table. */
+fxc_status fxc_geometric_delay_table(const double ant1_ecef[3], const double ant2_ecef[3],
+                                     double ra, double dec, double mjd, double time_offset_s,
+                                     int64_t n_points, double *table);
+/* Table length for a process of total_frames frames of frame_dt seconds:
+ * max(ceil(duration), 1) + 1 (main.rs:1630-1634). */
+int64_t fxc_geometric_table_points(int64_t total_frames, double frame_dt);
+
 #ifdef __cplusplus
 }
 #endif

@@ -195,16 +195,25 @@ def test_sector_sizes(n):
 def test_1024msps_2bit_fft1024_lsb_shuffle_geometry():
     """1024 Msps 2-bit N=1024 shape: B LSB with a seeded shuffle, geometric
     table + clock 1.2 us + rate 1e-12 + residuals; a 20000-frame slice of
-    sector 3 (first_frame != 0)."""
+    sector 3 (first_frame != 0).  The table is the library's own 1-s
+    geometric table for the reference's default stations (geom.rs:7-8)
+    toward a J2000 source precessed to the epoch (main.rs:1128-1135)."""
     fs, n = 1024e6, 1024
     rng = np.random.default_rng(2)
     sh = rng.permutation(32).astype(np.int32)
-    table = 2.9e-7 + 1.5e-9 * np.arange(10)
+    mjd = F.parse_epoch_mjd("2024/043 15:52:00")
+    ra, dec = F.precess_j2000_to_date(F.parse_ra("16h42m58.8s"), F.parse_dec("+39d48m36.0s"), mjd)
+    table = F.geometric_delay_table([-3502544.587, 3950966.235, 3566381.192],
+                                    [-3502567.576, 3950885.734, 3566449.115], ra, dec, mjd, 0.0,
+                                    F.geometric_table_points(8_000_000, n / fs))
+    assert table.size == 9 and np.all(np.abs(table) < 107.81 / 299792458.0)
+    init = int(round((table[0] + 0.2 / fs + 1.2e-6) * fs))           # main.rs:1279
+    d_seek = (max(init, 0) // 16 * 16) / fs                            # 2-bit word-aligned seek
     c = make_case(n=n, fs=fs, nf=20000, shuffles=(None, sh), lsb=(False, True), rho=0.005,
                   delay_samples=3.0, seed=3, table=table, first_frame=3_000_000,
                   time_offset_s=0.0, clock2_delay_s=1.2e-6, clock2_rate_sps=1e-12,
                   delay_user_samples=0.2, extra_delay_rate_sps=0.05 / 6.7e9,
-                  d_seek=(1526 // 16 * 16) / fs, lo1_hz=6.7e9, lo2_hz=6.7e9)
+                  d_seek=d_seek, lo1_hz=6.7e9, lo2_hz=6.7e9)
     _compare(gpu_run(c), oracle_run(c), PLAIN_TOL)
     _compare(gpu_run(c, F.PRECISION_CHOSEN), oracle_run(c, R.MODE_F64), CHOSEN_TOL)
 
