@@ -16,7 +16,8 @@ LIB = os.path.join(HERE, "libfxcorr.so")
 
 SOURCES = ["fxc_device.hip", "fxc_mfma.hip", "fxc_phased.hip", "fxc_large.hip", "fxc_chain.hip",
            "fxc_synth.hip", "fxc_api.cpp", "fxc_server.cpp",
-           "fxc_io.cpp", "fxc_fringe.cpp", "fxc_geom.cpp"]
+           "fxc_io.cpp", "fxc_fringe.cpp", "fxc_geom.cpp",
+           "fxc_auto.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 

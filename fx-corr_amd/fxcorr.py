@@ -996,3 +996,55 @@ def geometric_delay_table(ant1_ecef, ant2_ecef, ra: float, dec: float, mjd: floa
         _ecef(ant1_ecef), _ecef(ant2_ecef), ra, dec, mjd, time_offset_s, n_points,
         out.ctypes.data_as(_P(C.c_double))))
     return out
+
+
+# ------------------------------------------------ auto spectra and phased plots
+EXPORTED += ["fxc_finalize_auto_spectrum", "fxc_phased_plot_series"]
+
+
+class PhasedPlot(C.Structure):
+    _fields_ = [("freqs_mhz", _P(C.c_double)), ("phased_power", _P(C.c_double)),
+                ("amp_ph", _P(C.c_double)), ("amp11", _P(C.c_double)), ("amp22", _P(C.c_double)),
+                ("acf_shifted", _P(C.c_double)), ("spec_bins", C.c_int64)]
+
+
+def _auto_sigs(L):
+    if not getattr(L, "_auto_sigs", False):
+        pd = _P(C.c_double)
+        L.fxc_finalize_auto_spectrum.argtypes = [pd, C.c_int64, C.c_double, pd]
+        L.fxc_finalize_auto_spectrum.restype = C.c_int32
+        L.fxc_phased_plot_series.argtypes = [pd, pd, pd, C.c_int64, C.c_int32, C.c_int64,
+                                             C.c_double, C.c_double, _P(PhasedPlot)]
+        L.fxc_phased_plot_series.restype = C.c_int32
+        L._auto_sigs = True
+    return L
+
+
+def _dp(a):
+    return a.ctypes.data_as(_P(C.c_double))
+
+
+def finalize_auto_spectrum(acc, power_norm: float) -> np.ndarray:
+    """acf.rs:3-23."""
+    a = np.ascontiguousarray(acc, dtype=np.float64)
+    out = np.zeros_like(a)
+    check(_auto_sigs(lib()).fxc_finalize_auto_spectrum(_dp(a), a.size, power_norm, _dp(out)))
+    return out
+
+
+def phased_plot_series(acc_ph, acc11, acc22, fft_len: int, emitted_frames: int, obs_mhz: float,
+                       df_hz: float) -> dict:
+    """main.rs:2727-2768 without drawing (see fxc_phased_plot_series)."""
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (acc_ph, acc11, acc22)]
+    h = fft_len // 2 + 1
+    outs = {k: np.zeros(h) for k in ("freqs_mhz", "phased_power", "amp_ph", "amp11", "amp22")}
+    acf = np.zeros(fft_len)
+    pp = PhasedPlot(*[_dp(outs[k]) for k in ("freqs_mhz", "phased_power", "amp_ph", "amp11",
+                                             "amp22")], _dp(acf), 0)
+    check(_auto_sigs(lib()).fxc_phased_plot_series(_dp(arrs[0]), _dp(arrs[1]), _dp(arrs[2]),
+                                                   arrs[0].size, fft_len, emitted_frames,
+                                                   obs_mhz, df_hz, C.byref(pp)))
+    b = pp.spec_bins
+    res = {k: v[:b] for k, v in outs.items()}
+    res["acf_shifted"] = acf
+    return res

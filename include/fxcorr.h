@@ -249,6 +249,29 @@ fxc_status fxc_finalize_cross_spectrum(double *spec, int64_t len, int32_t fft_le
                                        double *mag_out, double *phase_out,
                                        double *unwrapped_out, double *fit_out);
 
+/* finalize_auto_spectrum (acf.rs:3-23): out[i] = acc[i] / power_norm (acc[i]
+ * itself when power_norm <= 0), then out[0] = 0.  Host function. */
+fxc_status fxc_finalize_auto_spectrum(const double *acc, int64_t len, double power_norm,
+                                      double *out);
+
+/* The numbers behind yi-phasedarray's plots (main.rs:2727-2768), host
+ * function: the phased, station-1 and station-2 auto spectra of a scan
+ * finalised with power_norm = max(emitted_frames * fft_len, 1); the first
+ * spec_bins bins (fft_len/2 for even fft_len, the whole half spectrum for
+ * odd) on the axis obs_mhz + i df; the phased power and the three
+ * amplitudes sqrt(power); and the phased autocorrelation: the f32 spectrum
+ * mirrored into fft_len bins, inverse DFT x 1/fft_len, real part, rotated
+ * so that acf_shifted[fft_len/2] is lag 0 (lags -fft_len/2 ..).  The caller
+ * provides arrays of fft_len/2+1 (acf_shifted: fft_len) doubles. */
+typedef struct {
+    double *freqs_mhz, *phased_power, *amp_ph, *amp11, *amp22;
+    double *acf_shifted;
+    int64_t spec_bins;            /* set by the call */
+} fxc_phased_plot;
+fxc_status fxc_phased_plot_series(const double *acc_ph, const double *acc11, const double *acc22,
+                                  int64_t len, int32_t fft_len, int64_t emitted_frames,
+                                  double obs_mhz, double df_hz, fxc_phased_plot *out);
+
 /* ----------------------------------------------------------------------- */
 /* Streaming host input (replaces the reader thread + per-sector loop of    */
 /* main.rs:2098-2213 for callers whose sector bytes are in host memory).   */

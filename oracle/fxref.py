@@ -428,3 +428,32 @@ def finalize_cross_spectrum(spec, fft_len, sampling_rate_hz, sky_low_mhz, is_lsb
                 delay = -slope / (2.0 * math.pi)
     return (np.array(s), dict(delay=delay, slope=slope, intercept=icpt, mag=np.array(mag),
                               phase=np.array(ph), unwrapped=np.array(unw), fhz=np.array(fhz)))
+
+
+# ------------------------------------------------- auto spectra and phased plots
+def finalize_auto_spectrum(acc, power_norm):
+    """acf.rs:3-23: acc / power_norm (acc when power_norm <= 0), bin 0 zeroed."""
+    out = np.array(acc, dtype=np.float64) / power_norm if power_norm > 0 else np.array(acc, float)
+    if out.size:
+        out[0] = 0.0
+    return out
+
+
+def phased_plot_series(acc_ph, acc11, acc22, fft_len, emitted, obs_mhz, df_hz):
+    """main.rs:2727-2768 without drawing: dict(freqs_mhz, phased_power, amp_ph,
+    amp11, amp22, acf_shifted).  The inverse C2C of rustfft x 1/N (utils.rs:30-36)
+    is numpy's ifft, which carries the 1/N."""
+    norm = max(emitted * fft_len, 1.0)
+    ph, a11, a22 = (finalize_auto_spectrum(a, norm) for a in (acc_ph, acc11, acc22))
+    bins = fft_len // 2 if fft_len % 2 == 0 else ph.size
+    full = np.zeros(fft_len, dtype=np.complex128)
+    for i in range(bins):
+        v = float(np.float32(ph[i]))
+        full[i] = v
+        if 0 < i < fft_len // 2:
+            full[fft_len - i] = v
+    acf = np.fft.ifft(full).real.astype(np.float32).astype(np.float64)
+    df = df_hz / 1e6
+    return dict(freqs_mhz=obs_mhz + np.arange(bins) * df, phased_power=ph[:bins],
+                amp_ph=np.sqrt(ph[:bins]), amp11=np.sqrt(a11[:bins]), amp22=np.sqrt(a22[:bins]),
+                acf_shifted=np.roll(acf, -(fft_len // 2)))
