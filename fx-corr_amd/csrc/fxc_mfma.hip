@@ -275,66 +275,22 @@ __device__ __forceinline__ void mf_stage2(const f4 y[4], const float *tw2, const
             z[(l & 15) + 16 * ct + 32 * (4 * (l >> 4) + r)] = make_float2(d[0][ct][r], d[1][ct][r]);
 }
 
-// Stage 2 on the VALU in FP32 (producer/consumer organization): the DFT-16
-// over m2 = 4g + r (g = lane group, r = this lane's rows) of each column k1
-// as radix 4 x 4 with k2 = a + 4b:
-//   Z[a + 4b] = sum_r W4^{rb} W16^{ra} sum_g W4^{ga} Y'[4g + r]
-// The DFT-4 over g crosses lane groups: two radix-2 steps, g <-> g^2 through
-// v_permlane32_swap and g <-> g^1 through v_permlane16_swap (bfly32/16),
-// the -i of its second difference on group 3 in between.  Lane group g then
-// holds output a = bitrev2(g); the W16^{ra} twiddles and the in-lane DFT-4
-// over r follow.
-struct MfS2 {
-    bool g3;               // lane group 3: the DFT-4's -i on the second difference
-    float2 w[3];           // W16^{r a}, r = 1..3
-    int zi;                // Z index of this lane's output k1 + 32 a (h = 0, b = 0)
-};
+// Stage 2 through a transposed copy in the Z slot (producer/consumer
+// organization).  Each station's twiddled stage-1 output Y'[m2][k1] is
+// stored column by column (the 16 values m2 = 0..15 of column k1 in a row
+// of 128 bytes, 16-byte pairs XOR-swizzled by k1 & 7 so that a lane group's
+// stores and the column reads spread over the banks); then lane l takes
+// column k1 = l & 31 of station l >> 5 and runs the whole DFT-16 in its own
+// registers (radix 4 x 4, no cross-lane traffic), writing Z/2 over the same
+// slot in natural order.
+__device__ __forceinline__ int mf_tidx(int s, int k1, int pair) {
+    return s * mf::M + k1 * 16 + 2 * (pair ^ (k1 & 7));
+}
 
-__device__ __forceinline__ MfS2 mf_s2_consts(int l) {
-    MfS2 c;
+// Twiddle W512^{m2 k1} of station s's stage-1 output and its transposed store.
+__device__ __forceinline__ void mf_twiddle_store(const f4 y[4], const float *tw2, float2 *slot, int s,
+                                                 int l) {
     const int g = l >> 4;
-    c.g3 = g == 3;
-    const int a = ((g & 1) << 1) | (g >> 1);        // DFT-4 output index held by group g
-#pragma unroll
-    for (int r = 1; r <= 3; ++r) {
-        const double th = -2.0 * FXC_PI * (double)(r * a) / 16.0;
-        c.w[r - 1] = make_float2((float)cos(th), (float)sin(th));
-    }
-    c.zi = (l & 15) + 32 * a;
-    return c;
-}
-
-// Radix-2 butterfly of the complex (x, y) between lanes l and l ^ 32: the
-// lower half gets own + partner, the upper half partner - own.  One swap
-// gives every lane both values of one component (x in the lower half, y in
-// the upper), sum and difference are formed there, and a second swap hands
-// each half the component it lacks.
-__device__ __forceinline__ void bfly32(float &x, float &y) {
-    const auto p = __builtin_amdgcn_permlane32_swap(__builtin_bit_cast(uint32_t, x),
-                                                    __builtin_bit_cast(uint32_t, y), false, false);
-    const float p0 = __builtin_bit_cast(float, (uint32_t)p[0]);
-    const float p1 = __builtin_bit_cast(float, (uint32_t)p[1]);
-    const auto q = __builtin_amdgcn_permlane32_swap(__builtin_bit_cast(uint32_t, p0 + p1),
-                                                    __builtin_bit_cast(uint32_t, p0 - p1), false, false);
-    x = __builtin_bit_cast(float, (uint32_t)q[0]);
-    y = __builtin_bit_cast(float, (uint32_t)q[1]);
-}
-
-// The same between lanes l and l ^ 16 (even lane groups get the sums).
-__device__ __forceinline__ void bfly16(float &x, float &y) {
-    const auto p = __builtin_amdgcn_permlane16_swap(__builtin_bit_cast(uint32_t, x),
-                                                    __builtin_bit_cast(uint32_t, y), false, false);
-    const float p0 = __builtin_bit_cast(float, (uint32_t)p[0]);
-    const float p1 = __builtin_bit_cast(float, (uint32_t)p[1]);
-    const auto q = __builtin_amdgcn_permlane16_swap(__builtin_bit_cast(uint32_t, p0 + p1),
-                                                    __builtin_bit_cast(uint32_t, p0 - p1), false, false);
-    x = __builtin_bit_cast(float, (uint32_t)q[0]);
-    y = __builtin_bit_cast(float, (uint32_t)q[1]);
-}
-
-// Twiddle + VALU stage 2 of one station-frame; Z/2 to LDS in natural order.
-__device__ __forceinline__ void mf_stage2v(const f4 y[4], const float *tw2, float2 *z, int l,
-                                           const MfS2 &c) {
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
         float2 v[4];
@@ -342,32 +298,70 @@ __device__ __forceinline__ void mf_stage2v(const f4 y[4], const float *tw2, floa
         for (int r = 0; r < 4; ++r)
             v[r] = c_mul(make_float2(y[h][r], y[2 + h][r]),
                          make_float2(tw2[((h * 4 + r) * 2) * 64 + l], tw2[((h * 4 + r) * 2 + 1) * 64 + l]));
-        // g <-> g^2: S = Y_g + Y_{g+2} (lower half), D = Y_{g-2} - Y_g (upper half)
+        const int k1 = (l & 15) + 16 * h;   // rows m2 = 4g + r: pairs 2g and 2g + 1
+        *reinterpret_cast<float4 *>(slot + mf_tidx(s, k1, 2 * g)) = make_float4(v[0].x, v[0].y, v[1].x, v[1].y);
+        *reinterpret_cast<float4 *>(slot + mf_tidx(s, k1, 2 * g + 1)) =
+            make_float4(v[2].x, v[2].y, v[3].x, v[3].y);
+    }
+}
+
+// x * W16^n for the constant twiddles of the DFT-16 (W16 = exp(-2 pi i / 16)).
+template <int n>
+__device__ __forceinline__ float2 mf_w16(float2 x) {
+    constexpr float c1 = 0.92387953251128674f, s1 = 0.38268343236508978f, c2 = 0.70710678118654752f;
+    if constexpr (n == 4) {
+        return make_float2(x.y, -x.x);
+    } else {
+        constexpr float wr = n == 1 ? c1 : n == 2 ? c2 : n == 3 ? s1 : n == 6 ? -c2 : -c1;
+        constexpr float wi = n == 1 ? -s1 : n == 2 ? -c2 : n == 3 ? -c1 : n == 6 ? -c2 : s1;
+        return make_float2(x.x * wr - x.y * wi, x.x * wi + x.y * wr);
+    }
+}
+
+// DFT-4 of (x0, x1, x2, x3) with W4 = -i, outputs in natural order.
+__device__ __forceinline__ void mf_dft4(float2 x0, float2 x1, float2 x2, float2 x3, float2 &o0,
+                                        float2 &o1, float2 &o2, float2 &o3) {
+    const float2 a0 = make_float2(x0.x + x2.x, x0.y + x2.y), a1 = make_float2(x0.x - x2.x, x0.y - x2.y);
+    const float2 b0 = make_float2(x1.x + x3.x, x1.y + x3.y), b1 = make_float2(x1.x - x3.x, x1.y - x3.y);
+    o0 = make_float2(a0.x + b0.x, a0.y + b0.y);
+    o2 = make_float2(a0.x - b0.x, a0.y - b0.y);
+    o1 = make_float2(a1.x + b1.y, a1.y - b1.x);   // a1 - i b1
+    o3 = make_float2(a1.x - b1.y, a1.y + b1.x);   // a1 + i b1
+}
+
+// Column DFT-16 of both stations: reads the transposed Y', writes Z/2.
+__device__ __forceinline__ void mf_columns_dft16(float2 *slot, int l) {
+    const int s = l >> 5, k1 = l & 31;
+    float2 x[16];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) bfly32(v[r].x, v[r].y);
-        // group 3: D1 -> -i D1
+    for (int j = 0; j < 8; ++j) {
+        const float4 q = *reinterpret_cast<const float4 *>(slot + mf_tidx(s, k1, j));
+        x[2 * j] = make_float2(q.x, q.y);
+        x[2 * j + 1] = make_float2(q.z, q.w);
+    }
+    wave_lds_sync();   // every lane's column is read before Z overwrites the slot
+    // m2 = 4g + r, k2 = a + 4b: Z[a + 4b] = sum_r W4^{rb} W16^{ra} sum_g W4^{ga} x[4g + r]
+    float2 u[4][4];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const float re = v[r].x, im = v[r].y;
-            v[r].x = c.g3 ? im : re;
-            v[r].y = c.g3 ? -re : im;
-        }
-        // g <-> g^1: X[0] = S0 + S1, X[2] = S0 - S1, X[1] = D0 - i D1, X[3] = D0 + i D1
+    for (int r = 0; r < 4; ++r) mf_dft4(x[r], x[4 + r], x[8 + r], x[12 + r], u[0][r], u[1][r], u[2][r], u[3][r]);
+    u[1][1] = mf_w16<1>(u[1][1]);
+    u[1][2] = mf_w16<2>(u[1][2]);
+    u[1][3] = mf_w16<3>(u[1][3]);
+    u[2][1] = mf_w16<2>(u[2][1]);
+    u[2][2] = mf_w16<4>(u[2][2]);
+    u[2][3] = mf_w16<6>(u[2][3]);
+    u[3][1] = mf_w16<3>(u[3][1]);
+    u[3][2] = mf_w16<6>(u[3][2]);
+    u[3][3] = mf_w16<9>(u[3][3]);
+    float2 *zo = slot + s * mf::M + k1;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) bfly16(v[r].x, v[r].y);
-        v[1] = c_mul(v[1], c.w[0]);
-        v[2] = c_mul(v[2], c.w[1]);
-        v[3] = c_mul(v[3], c.w[2]);
-        // in-lane DFT-4 over r
-        const float2 s0 = make_float2(v[0].x + v[2].x, v[0].y + v[2].y);
-        const float2 d0 = make_float2(v[0].x - v[2].x, v[0].y - v[2].y);
-        const float2 s1 = make_float2(v[1].x + v[3].x, v[1].y + v[3].y);
-        const float2 d1 = make_float2(v[1].x - v[3].x, v[1].y - v[3].y);
-        float2 *zo = z + c.zi + 16 * h;
-        zo[0] = make_float2(s0.x + s1.x, s0.y + s1.y);
-        zo[128] = make_float2(d0.x + d1.y, d0.y - d1.x);
-        zo[256] = make_float2(s0.x - s1.x, s0.y - s1.y);
-        zo[384] = make_float2(d0.x - d1.y, d0.y + d1.x);
+    for (int a = 0; a < 4; ++a) {
+        float2 o0, o1, o2, o3;
+        mf_dft4(u[a][0], u[a][1], u[a][2], u[a][3], o0, o1, o2, o3);
+        zo[32 * a] = o0;
+        zo[32 * (a + 4)] = o1;
+        zo[32 * (a + 8)] = o2;
+        zo[32 * (a + 12)] = o3;
     }
 }
 
@@ -611,12 +605,12 @@ __global__ __launch_bounds__(512) void k_sector_mf2(SectorArgs a) {
 // The same per-frame work split by role over 12 waves (768 threads): waves
 // 0..7 are producers, 8..11 consumers.  Wave w runs on SIMD w % 4, so each
 // SIMD holds producers c and c + 4 and their consumer 8 + c.  A producer
-// decodes its frames, builds the A operands, runs stage 1 on the matrix
-// cores and stage 2 on the VALU (mf_stage2v; on gfx950 the FP32 MFMA and
-// the VALU do not run concurrently, and the radix-4 x 4 DFT-16 takes fewer
-// cycles than the dense FP32 GEMM), one station after the other, and
-// writes Z/2 into its LDS slot; the
-// consumer untangles and accumulates the products of both its producers'
+// decodes its frames, builds the A operands and runs stage 1 on the matrix
+// cores for both stations, storing the twiddled result transposed in its
+// LDS slot; then each lane runs one column's DFT-16 on the VALU
+// (mf_columns_dft16; on gfx950 the FP32 MFMA and the VALU do not run
+// concurrently, and the radix-4 x 4 DFT-16 takes fewer cycles than the
+// dense FP32 GEMM) and writes Z/2 over the slot; the consumer untangles and accumulates the products of both its producers'
 // frames, alternating between them.  Two producers per SIMD hide each
 // other's matrix-core latency and the consumer's VALU work fills the matrix
 // shadow; no wave carries the other role's registers.
@@ -643,7 +637,7 @@ struct MfShared3 {
     uint32_t slut[2][4 * 256];
     uint16_t dlut[2][MFT_DLUT_HALVES];
     uint16_t stage[mf::PROD][2][mf::SROW];     // producers' staging rows
-    float2 z[mf::PROD][2][mf::M];              // Z/2 slot per producer
+    alignas(16) float2 z[mf::PROD][2][mf::M];  // Z/2 slot per producer (16-byte stores)
     int32_t prod[mf::PROD], cons[mf::PROD];
     int32_t broken;
 };
@@ -677,7 +671,6 @@ __device__ void mf3_producer(const SectorArgs &a, MfShared3 &sh, int p, int l, i
     uint16_t *row2 = sh.stage[p][1];
     uint32_t cur1[4] = {0, 0, 0, 0}, cur2[4] = {0, 0, 0, 0};
     bool ok = true;
-    const MfS2 s2c = mf_s2_consts(l);
     if (f0 < f1) {
         mf_load(a.raw1 + f0 * bpf1, nw1, l, cur1);
         mf_load(a.raw2 + f0 * bpf2, nw2, l, cur2);
@@ -716,8 +709,10 @@ __device__ void mf3_producer(const SectorArgs &a, MfShared3 &sh, int p, int l, i
                                      ld_b1(sh.b1, (1 * 4 + t) * 2 + 1, l));
                 // the slot was last read by the consumer for frame i - 1
                 if (st == 0) mf_wait_ge(&sh.cons[p], i, ok, &sh.broken);
-                mf_stage2v(y, sh.tw2, sh.z[p][st], l, s2c);
+                mf_twiddle_store(y, sh.tw2, &sh.z[p][0][0], st, l);
             }
+            wave_lds_sync();
+            mf_columns_dft16(&sh.z[p][0][0], l);
             mf_publish(&sh.prod[p], i + 1);
             if (f + 1 < f1) {   // decode frame f+1 (the staging rows are this wave's own)
                 wave_lds_sync();
@@ -967,51 +962,68 @@ int mfma_wg_threads() { return 512; }
 // one wave per frame.  Inputs must be exact in FP16.
 __global__ __launch_bounds__(64) void k_rfft_mfma(SectorArgs a, const float *in, int64_t nframes,
                                                   float2 *out) {
+    // variant 1: two frames per workgroup through the column DFT-16 (the
+    // producer/consumer kernel's stage 2, the frames in the station slots)
     __shared__ uint4 b1[16 * 64];
-    __shared__ uint16_t row[mf::SROW];
-    __shared__ float2 z[mf::M];
+    __shared__ uint16_t row[2][mf::SROW];
+    __shared__ alignas(16) float2 z[2][mf::M];
     const int l = threadIdx.x;
-    const int64_t f = blockIdx.x;
+    const int per = a.variant == 1 ? 2 : 1;
+    const int64_t f0 = (int64_t)blockIdx.x * per;
     const uint8_t *tab = reinterpret_cast<const uint8_t *>(a.mf_tab);
     const uint4 *g = reinterpret_cast<const uint4 *>(tab + MFT_B1);
     for (int i = l; i < 16 * 64; i += 64) b1[i] = g[i];
-    for (int i = l; i < mf::SROW; i += 64) {
-        const int n = i - mf::PAD;
-        const float v = (n >= 0 && n < mf::N) ? in[f * mf::N + n] : 0.0f;
-        row[i] = __builtin_bit_cast(uint16_t, (_Float16)v);
-    }
+    for (int s = 0; s < per; ++s)
+        for (int i = l; i < mf::SROW; i += 64) {
+            const int n = i - mf::PAD;
+            const bool in_frame = n >= 0 && n < mf::N && f0 + s < nframes;
+            const float v = in_frame ? in[(f0 + s) * mf::N + n] : 0.0f;
+            row[s][i] = __builtin_bit_cast(uint16_t, (_Float16)v);
+        }
     const float *ga = reinterpret_cast<const float *>(tab + MFT_A2);
     const float *gt = reinterpret_cast<const float *>(tab + MFT_TW2);
     const float *gu = reinterpret_cast<const float *>(tab + MFT_TWU);
     __syncthreads();
-    half8 a0, a1;
-    mf_build_a(row, 0, l, a0, a1);
-    f4 y[4];
+    for (int s = 0; s < per; ++s) {
+        half8 a0, a1;
+        mf_build_a(row[s], 0, l, a0, a1);
+        f4 y[4];
 #pragma unroll
-    for (int t = 0; t < 4; ++t)
-        y[t] = mf_stage1(a0, a1, ld_b1(b1, t * 2, l), ld_b1(b1, t * 2 + 1, l),
-                         ld_b1(b1, (4 + t) * 2, l), ld_b1(b1, (4 + t) * 2 + 1, l));
-    if (a.variant == 1) mf_stage2v(y, gt, z, l, mf_s2_consts(l));
-    else mf_stage2(y, gt, ga, z, l);
-    __syncthreads();
-    float2 *o = out + f * (mf::M + 1);
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const int p = l + 64 * j;
-        float2 xp, xm;
-        untangle_pair(z[p], z[(mf::M - p) & (mf::M - 1)],
-                      make_float2(gu[(2 * j) * 64 + l], gu[(2 * j + 1) * 64 + l]), xp, xm);
-        o[p] = xp;
-        o[mf::M - p] = xm;
+        for (int t = 0; t < 4; ++t)
+            y[t] = mf_stage1(a0, a1, ld_b1(b1, t * 2, l), ld_b1(b1, t * 2 + 1, l),
+                             ld_b1(b1, (4 + t) * 2, l), ld_b1(b1, (4 + t) * 2 + 1, l));
+        if (a.variant == 1) mf_twiddle_store(y, gt, &z[0][0], s, l);
+        else mf_stage2(y, gt, ga, z[0], l);
     }
-    if (l == 0) o[mf::M / 2] = make_float2(2.0f * z[mf::M / 2].x, -2.0f * z[mf::M / 2].y);
+    if (a.variant == 1) {
+        wave_lds_sync();
+        mf_columns_dft16(&z[0][0], l);
+    }
+    __syncthreads();
+    for (int s = 0; s < per; ++s) {
+        if (f0 + s >= nframes) break;
+        float2 *o = out + (f0 + s) * (mf::M + 1);
+        const float2 *zs = z[s];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int p = l + 64 * j;
+            float2 xp, xm;
+            untangle_pair(zs[p], zs[(mf::M - p) & (mf::M - 1)],
+                          make_float2(gu[(2 * j) * 64 + l], gu[(2 * j + 1) * 64 + l]), xp, xm);
+            o[p] = xp;
+            o[mf::M - p] = xm;
+        }
+        if (l == 0) o[mf::M / 2] = make_float2(2.0f * zs[mf::M / 2].x, -2.0f * zs[mf::M / 2].y);
+    }
 }
 
 hipError_t launch_rfft_mfma(const SectorArgs &a, const float *in, int64_t nframes, float2 *out,
                             hipStream_t s) {
     if (a.N != mf::N || a.mf_tab == nullptr) return hipErrorInvalidValue;
     if (nframes <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_rfft_mfma, dim3((unsigned)nframes), dim3(64), 0, s, a, in, nframes, out);
+    const int64_t per = a.variant == 1 ? 2 : 1;
+    hipLaunchKernelGGL(k_rfft_mfma, dim3((unsigned)((nframes + per - 1) / per)), dim3(64), 0, s, a, in,
+                       nframes, out);
     return hipGetLastError();
 }
 
