@@ -278,30 +278,42 @@ __device__ __forceinline__ void mf_stage2(const f4 y[4], const float *tw2, const
 // Stage 2 through a transposed copy in the Z slot (producer/consumer
 // organization).  Each station's twiddled stage-1 output Y'[m2][k1] is
 // stored column by column (the 16 values m2 = 0..15 of column k1 in a row
-// of 128 bytes, 16-byte pairs XOR-swizzled by k1 & 7 so that a lane group's
-// stores and the column reads spread over the banks); then lane l takes
+// of 128 bytes, 16-byte pairs XOR-swizzled by (k1 >> 1) & 7, so that any 16
+// consecutive columns -- one pass of a 16-byte store or load -- fall on 16
+// different 16-byte bank groups); then lane l takes
 // column k1 = l & 31 of station l >> 5 and runs the whole DFT-16 in its own
 // registers (radix 4 x 4, no cross-lane traffic), writing Z/2 over the same
 // slot in natural order.
 __device__ __forceinline__ int mf_tidx(int s, int k1, int pair) {
-    return s * mf::M + k1 * 16 + 2 * (pair ^ (k1 & 7));
+    return s * mf::M + k1 * 16 + 2 * (pair ^ ((k1 >> 1) & 7));
 }
 
-// Twiddle W512^{m2 k1} of station s's stage-1 output and its transposed store.
+// W512^{m2 k1} of this lane's rows m2 = 4g + r in column half h.
+__device__ __forceinline__ void mf_load_tw(const float *tw2, int h, int l, float2 tw[4]) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+        tw[r] = make_float2(tw2[((h * 4 + r) * 2) * 64 + l], tw2[((h * 4 + r) * 2 + 1) * 64 + l]);
+}
+
+// Twiddle of column half h (k1 = l%16 + 16h; re / im from stage-1 tiles h
+// and 2 + h) of station s, stored transposed.
+__device__ __forceinline__ void mf_twiddle_store_half(f4 re, f4 im, const float2 tw[4], float2 *slot,
+                                                      int s, int h, int l) {
+    float2 v[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) v[r] = c_mul(make_float2(re[r], im[r]), tw[r]);
+    const int g = l >> 4, k1 = (l & 15) + 16 * h;   // rows m2 = 4g + r: pairs 2g and 2g + 1
+    *reinterpret_cast<float4 *>(slot + mf_tidx(s, k1, 2 * g)) = make_float4(v[0].x, v[0].y, v[1].x, v[1].y);
+    *reinterpret_cast<float4 *>(slot + mf_tidx(s, k1, 2 * g + 1)) = make_float4(v[2].x, v[2].y, v[3].x, v[3].y);
+}
+
 __device__ __forceinline__ void mf_twiddle_store(const f4 y[4], const float *tw2, float2 *slot, int s,
                                                  int l) {
-    const int g = l >> 4;
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-        float2 v[4];
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-            v[r] = c_mul(make_float2(y[h][r], y[2 + h][r]),
-                         make_float2(tw2[((h * 4 + r) * 2) * 64 + l], tw2[((h * 4 + r) * 2 + 1) * 64 + l]));
-        const int k1 = (l & 15) + 16 * h;   // rows m2 = 4g + r: pairs 2g and 2g + 1
-        *reinterpret_cast<float4 *>(slot + mf_tidx(s, k1, 2 * g)) = make_float4(v[0].x, v[0].y, v[1].x, v[1].y);
-        *reinterpret_cast<float4 *>(slot + mf_tidx(s, k1, 2 * g + 1)) =
-            make_float4(v[2].x, v[2].y, v[3].x, v[3].y);
+        float2 tw[4];
+        mf_load_tw(tw2, h, l, tw);
+        mf_twiddle_store_half(y[h], y[2 + h], tw, slot, s, h, l);
     }
 }
 
@@ -696,20 +708,28 @@ __device__ void mf3_producer(const SectorArgs &a, MfShared3 &sh, int p, int l, i
             }
             const int s1 = __builtin_amdgcn_readlane(pf.s1, jf);
             const int s2 = __builtin_amdgcn_readlane(pf.s2, jf);
+            // stage 1 of both stations by column half: each B operand read
+            // from LDS feeds two MFMAs, each twiddle load two stations
+            half8 a10, a11, a20, a21;
+            mf_build_a(row1, s1, l, a10, a11);
+            mf_build_a(row2, s2, l, a20, a21);
 #pragma unroll 1
-            for (int st = 0; st < 2; ++st) {
-                half8 a0, a1;
-                mf_build_a(st ? row2 : row1, st ? s2 : s1, l, a0, a1);
-                f4 y[4];
+            for (int h = 0; h < 2; ++h) {
+                f4 y1[2], y2[2];   // tiles h (re) and 2 + h (im)
 #pragma unroll
-                for (int t = 0; t < 4; ++t)
-                    y[t] = mf_stage1(a0, a1, ld_b1(sh.b1, (0 * 4 + t) * 2 + 0, l),
-                                     ld_b1(sh.b1, (0 * 4 + t) * 2 + 1, l),
-                                     ld_b1(sh.b1, (1 * 4 + t) * 2 + 0, l),
-                                     ld_b1(sh.b1, (1 * 4 + t) * 2 + 1, l));
+                for (int q = 0; q < 2; ++q) {
+                    const int t = h + 2 * q;
+                    const half8 bh0 = ld_b1(sh.b1, t * 2 + 0, l), bl0 = ld_b1(sh.b1, t * 2 + 1, l);
+                    const half8 bh1 = ld_b1(sh.b1, (4 + t) * 2 + 0, l), bl1 = ld_b1(sh.b1, (4 + t) * 2 + 1, l);
+                    y1[q] = mf_stage1(a10, a11, bh0, bl0, bh1, bl1);
+                    y2[q] = mf_stage1(a20, a21, bh0, bl0, bh1, bl1);
+                }
+                float2 tw[4];
+                mf_load_tw(sh.tw2, h, l, tw);
                 // the slot was last read by the consumer for frame i - 1
-                if (st == 0) mf_wait_ge(&sh.cons[p], i, ok, &sh.broken);
-                mf_twiddle_store(y, sh.tw2, &sh.z[p][0][0], st, l);
+                if (h == 0) mf_wait_ge(&sh.cons[p], i, ok, &sh.broken);
+                mf_twiddle_store_half(y1[0], y1[1], tw, &sh.z[p][0][0], 0, h, l);
+                mf_twiddle_store_half(y2[0], y2[1], tw, &sh.z[p][0][0], 1, h, l);
             }
             wave_lds_sync();
             mf_columns_dft16(&sh.z[p][0][0], l);
