@@ -534,7 +534,8 @@ static bool mfma_eligible(const fxc_plan *p) {
 //  B1 idx (kap*4 + t)*2 + lohi: half8 slot s of B[K][col] with K = (m1 =
 //     16 kap + g + 4 (s/2), e = s%2: re|im of z), col = (k1 = c + 16(t&1),
 //     part = t/2: re|im of Y); B = 1/2 * DFT-32 (the realfft split's 1/2),
-//     stored as hi + 2^-11 lo.
+//     stored as hi + lo, lo = B - hi rounded to FP16 (subnormal for the
+//     smaller coefficients: hi + lo is within 2^-25 of B).
 //  A2 idx rt*8 + b: DFT-16 coefficient of output (k2 = c, rt: re|im) and
 //     input (m2 = 4 g + b/2, b%2: re|im).
 //  TW2 idx (ct*4 + i)*2 + part: W512^{m2 k1}, m2 = 4 g + i, k1 = c + 16 ct.
@@ -566,7 +567,7 @@ static std::vector<uint8_t> mfma_tables(const fxc_plan *p) {
                     }
                     const double v = 0.5 * (part == 0 ? (e == 0 ? cs : sn) : (e == 0 ? -sn : cs));
                     const uint16_t hi = f32_to_f16_bits((float)v);
-                    const uint16_t lo = f32_to_f16_bits((float)((v - (double)f16_bits_to_f32(hi)) * 2048.0));
+                    const uint16_t lo = f32_to_f16_bits((float)(v - (double)f16_bits_to_f32(hi)));
                     const size_t idx_hi = (size_t)((kap * 4 + t) * 2 + 0), idx_lo = idx_hi + 1;
                     put16(MFT_B1 + ((idx_hi * 64 + l) * 8 + s) * 2, hi);
                     put16(MFT_B1 + ((idx_lo * 64 + l) * 8 + s) * 2, lo);

@@ -1,16 +1,19 @@
 // Chosen-form fused kernel for N = 1024 (M = 512 complex points per
-// station-frame, z[m] = x[2m] + i x[2m+1]): a four-step FFT whose two
-// stages both run on the matrix cores.
+// station-frame, z[m] = x[2m] + i x[2m+1]): a four-step FFT whose first
+// stage runs on the matrix cores.
 //
 //   m = 16 m1 + m2,  k = k1 + 32 k2
 //   stage 1 (FP16 MFMA, FP32 acc): Y[m2][k1]  = sum_m1 z[16 m1 + m2] W32^{m1 k1}
 //   twiddle (FP32):                Y'[m2][k1] = Y[m2][k1] W512^{m2 k1}
-//   stage 2 (FP32 MFMA):           Z[k1 + 32 k2] = sum_m2 Y'[m2][k1] W16^{m2 k2}
+//   stage 2 (FP32):                Z[k1 + 32 k2] = sum_m2 Y'[m2][k1] W16^{m2 k2}
+//     k_sector_mf2: an FP32 MFMA GEMM; k_sector_mf3: one column per lane,
+//     radix 4 x 4 on the VALU
 //
 // Stage 1 is the real GEMM D[m2][(k1,re|im)] = sum_(m1,re|im) A B with
 // A = the samples (exact in FP16 for 1/2/4/8-bit levels) and B the DFT-32
-// coefficients split as B = B_hi + 2^-11 B_lo (both FP16) with separate
-// FP32 accumulators.  Its accumulator layout (lane l: rows m2 = 4(l/16)+r,
+// coefficients split as B = B_hi + B_lo (both FP16, B_lo = B - B_hi
+// rounded, subnormal where small: B_hi + B_lo is within 2^-25 of B),
+// both products accumulated in FP32.  Its accumulator layout (lane l: rows m2 = 4(l/16)+r,
 // column l%16) is exactly the B-operand layout stage 2 needs (K = (m2,
 // re|im), column k1), so the data never leaves registers between stages.
 //
@@ -33,7 +36,6 @@ constexpr int WAVES = 8;                 // waves per workgroup
 constexpr int PAD = 64;                  // zero FP16 samples each side of a staged frame
 constexpr int SROW = N + 2 * PAD;        // halves per staged station-frame
 constexpr int FB = 64;                   // frames whose parameters one batch computes
-constexpr float LO_SCALE = 1.0f / 2048.0f;
 }  // namespace mf
 
 struct MfShared {
@@ -223,17 +225,16 @@ __device__ __forceinline__ void mf_build_a(const uint16_t *row, int sh, int l, h
 // ------------------------------------------------------------- FFT
 // Stage 1 for one station-frame, column tile t (t&1: k1 half, t>>1: re|im):
 // y[r] = Y[m2 = 4(l/16) + r][k1 = l%16 + 16(t&1)] (re or im).
+// The lo products (B - B_hi, FP16 subnormals included) go first into the
+// one FP32 accumulator, the hi products after them.
 __device__ __forceinline__ f4 mf_stage1(half8 a0, half8 a1, half8 bh0, half8 bl0, half8 bh1,
                                         half8 bl1) {
-    f4 dh = {0.f, 0.f, 0.f, 0.f}, dl = {0.f, 0.f, 0.f, 0.f};
-    dh = __builtin_amdgcn_mfma_f32_16x16x32_f16(a0, bh0, dh, 0, 0, 0);
-    dl = __builtin_amdgcn_mfma_f32_16x16x32_f16(a0, bl0, dl, 0, 0, 0);
-    dh = __builtin_amdgcn_mfma_f32_16x16x32_f16(a1, bh1, dh, 0, 0, 0);
-    dl = __builtin_amdgcn_mfma_f32_16x16x32_f16(a1, bl1, dl, 0, 0, 0);
-    f4 y;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) y[r] = fmaf(dl[r], mf::LO_SCALE, dh[r]);
-    return y;
+    f4 d = {0.f, 0.f, 0.f, 0.f};
+    d = __builtin_amdgcn_mfma_f32_16x16x32_f16(a0, bl0, d, 0, 0, 0);
+    d = __builtin_amdgcn_mfma_f32_16x16x32_f16(a1, bl1, d, 0, 0, 0);
+    d = __builtin_amdgcn_mfma_f32_16x16x32_f16(a0, bh0, d, 0, 0, 0);
+    d = __builtin_amdgcn_mfma_f32_16x16x32_f16(a1, bh1, d, 0, 0, 0);
+    return d;
 }
 
 __device__ __forceinline__ half8 ld_b1(const uint4 *b1, int idx, int l) {
@@ -278,14 +279,14 @@ __device__ __forceinline__ void mf_stage2(const f4 y[4], const float *tw2, const
 // Stage 2 through a transposed copy in the Z slot (producer/consumer
 // organization).  Each station's twiddled stage-1 output Y'[m2][k1] is
 // stored column by column (the 16 values m2 = 0..15 of column k1 in a row
-// of 128 bytes, 16-byte pairs XOR-swizzled by (k1 >> 1) & 7, so that any 16
-// consecutive columns -- one pass of a 16-byte store or load -- fall on 16
-// different 16-byte bank groups); then lane l takes
+// of 128 bytes, 16-byte pairs XOR-swizzled by k1 & 7, so that any 8
+// consecutive columns -- one pass of a 16-byte store or load over the 32
+// LDS banks -- fall on 8 different 16-byte bank groups); then lane l takes
 // column k1 = l & 31 of station l >> 5 and runs the whole DFT-16 in its own
 // registers (radix 4 x 4, no cross-lane traffic), writing Z/2 over the same
 // slot in natural order.
 __device__ __forceinline__ int mf_tidx(int s, int k1, int pair) {
-    return s * mf::M + k1 * 16 + 2 * (pair ^ ((k1 >> 1) & 7));
+    return s * mf::M + k1 * 16 + 2 * (pair ^ (k1 & 7));
 }
 
 // W512^{m2 k1} of this lane's rows m2 = 4g + r in column half h.

@@ -293,8 +293,10 @@ def test_mfma_eligibility():
 
 @pytest.mark.parametrize("variant", [0, 1])
 def test_mfma_rfft_vs_numpy(variant):
-    """Stage 2 as an FP32 MFMA GEMM (variant 0) or as the VALU radix-4 x 4
-    DFT-16 with cross-lane swaps (variant 1, the default)."""
+    """Stage 2 as an FP32 MFMA GEMM (variant 0) or as the column DFT-16 in
+    registers after a transposed LDS store (variant 1, the default).  The
+    RMS bar also pins the FP16 hi + lo split of the DFT-32 coefficients: a
+    lost lo part (e.g. FP16 subnormals flushed) gives errors near 1e-4."""
     n = 1024
     p = F.Plan(n, levels=[LEVELS_2BIT, LEVELS_2BIT], df_hz=1.0, precision=F.PRECISION_CHOSEN)
     assert p.info.fft_form == 1 and p.info.variant == 1
