@@ -663,7 +663,7 @@ __device__ __forceinline__ bool mf_wait_ge(int32_t *flag, int32_t v, bool &ok, i
         const int32_t c = __builtin_amdgcn_readfirstlane(
             __hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
         if (c >= v) return true;
-        __builtin_amdgcn_s_sleep(1);
+        __builtin_amdgcn_s_sleep(3);
     }
     ok = false;
     __hip_atomic_store(broken, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
