@@ -646,7 +646,6 @@ struct MfShared3 {
     float a2[16 * 64];
     float tw2[16 * 64];
     float twu[8 * 64];
-    double mid[mf::CONS][4];
     uint32_t slut[2][4 * 256];
     uint16_t dlut[2][MFT_DLUT_HALVES];
     uint16_t stage[mf::PROD][2][mf::SROW];     // producers' staging rows
@@ -758,7 +757,7 @@ template <bool FULL>
 __device__ __forceinline__ void mf_mac(const float2 *z1, const float2 *z2, const float *twu,
                                        const MfFrame &pf, int jf, int l, bool do_acf, bool do_xcf,
                                        uint32_t m11, uint32_t m22, uint32_t m_ov, float k11,
-                                       float k22, float kM, double (&acc)[8][4], double *mid) {
+                                       float k22, float kM, double (&acc)[8][4]) {
     constexpr int M = mf::M;
     const double phw = readlane_d(pf.phw, jf);
     const double dth = readlane_d(pf.dth, jf);
@@ -807,20 +806,27 @@ __device__ __forceinline__ void mf_mac(const float2 *z1, const float2 *z2, const
         }
         P = c_mul(P, F1);
     }
-    if (l == 0) {   // k = M/2 pairs with itself: X = 2 conj(Z'[M/2]); P is P(M/2)
-        const float2 u1 = z1[M / 2], u2 = z2[M / 2];
-        const float2 x1 = make_float2(2.0f * u1.x, -2.0f * u1.y);
-        const float2 x2 = make_float2(2.0f * u2.x, -2.0f * u2.y);
-        if (do_acf) {
-            mid[0] += (double)((FULL || ((m11 >> 8) & 1u)) ? norm_sqr(x1) : 0.0f);
-            mid[1] += (double)((FULL || ((m22 >> 8) & 1u)) ? norm_sqr(x2) : 0.0f);
-        }
-        if (do_xcf) {
-            const float2 Pp = (FULL || ((m_ov >> 8) & 1u)) ? P : make_float2(0.f, 0.f);
-            const float2 v = c_mul(c_mul(x1, c_conj(x2)), Pp);
-            mid[2] += (double)v.x;
-            mid[3] += (double)v.y;
-        }
+}
+
+// Bin k = M/2 pairs with itself: X = 2 conj(Z'[M/2]).  Its products are
+// formed once per batch of frames, lane l taking the frame whose
+// parameters it holds (u1 / u2: that frame's Z'[M/2]), and added to this
+// lane's FP64 sums; the lanes' sums are added at the end of the kernel.
+__device__ __forceinline__ void mf_mid(const MfFrame &pf, float2 u1, float2 u2, bool do_acf,
+                                       bool do_xcf, bool in11, bool in22, bool in_ov,
+                                       double (&macc)[4]) {
+    const float r = (float)wrap_turns(fma((double)(mf::M / 2), pf.dth, pf.phw));
+    const float2 P = make_float2(__builtin_amdgcn_cosf(r), __builtin_amdgcn_sinf(r));
+    const float2 x1 = make_float2(2.0f * u1.x, -2.0f * u1.y);
+    const float2 x2 = make_float2(2.0f * u2.x, -2.0f * u2.y);
+    if (do_acf) {
+        macc[0] += (double)(in11 ? norm_sqr(x1) : 0.0f);
+        macc[1] += (double)(in22 ? norm_sqr(x2) : 0.0f);
+    }
+    if (do_xcf) {
+        const float2 v = c_mul(c_mul(x1, c_conj(x2)), in_ov ? P : make_float2(0.f, 0.f));
+        macc[2] += (double)v.x;
+        macc[3] += (double)v.y;
     }
 }
 
@@ -828,7 +834,8 @@ __device__ __forceinline__ void mf_mac(const float2 *z1, const float2 *z2, const
 // c + 4, alternating (frame i of c, frame i of c + 4, frame i + 1 of c, ...).
 template <bool FULL>
 __device__ void mf3_consumer(const SectorArgs &a, MfShared3 &sh, int c, int l, int64_t fa0,
-                             int64_t fa1, int64_t fb0, int64_t fb1, double (&acc)[8][4]) {
+                             int64_t fa1, int64_t fb0, int64_t fb1, double (&acc)[8][4],
+                             double (&macc)[4]) {
     const bool do_acf = (a.flags & FLAG_ACF) != 0;
     const bool do_xcf = (a.flags & FLAG_XCF) != 0;
     const bool ov_only = (a.flags & FLAG_ACF_OVERLAP) != 0;
@@ -845,30 +852,44 @@ __device__ void mf3_consumer(const SectorArgs &a, MfShared3 &sh, int c, int l, i
     const uint32_t m22 = a.grid == 0 ? m_ov : m_grid;
     const float kM = (l == 0) ? 0.0f : 1.0f;   // FULL: bin M (lane 0, slot 1) is outside the overlap
     const float k11 = a.grid == 0 ? 1.0f : kM, k22 = a.grid == 0 ? kM : 1.0f;
+    const bool in11 = (m11 >> 8) & 1u, in22 = (m22 >> 8) & 1u, in_ov = (m_ov >> 8) & 1u;  // bin M/2
     const int64_t na = fa1 - fa0, nb = fb1 - fb0;
     const int64_t n = na > nb ? na : nb;
+    const int q = c + mf::CONS;
     bool ok = true;
     for (int64_t g = 0; g < n; g += mf::FB) {
         const int cnt = (int)((n - g) < mf::FB ? (n - g) : mf::FB);
         MfFrame pa = {}, pb = {};
         if (l < cnt && g + l < na) pa = mf_frame(a.first_frame + fa0 + g + l, a);
         if (l < cnt && g + l < nb) pb = mf_frame(a.first_frame + fb0 + g + l, a);
+        float2 ua1 = make_float2(0.f, 0.f), ua2 = ua1, ub1 = ua1, ub2 = ua1;   // Z'[M/2] of frame g + l
         for (int jf = 0; jf < cnt; ++jf) {
             const int32_t i = (int32_t)(g + jf);
             if (i < na) {
                 mf_wait_ge(&sh.prod[c], i + 1, ok, &sh.broken);
                 mf_mac<FULL>(sh.z[c][0], sh.z[c][1], sh.twu, pa, jf, l, do_acf, do_xcf, m11, m22,
-                             m_ov, k11, k22, kM, acc, sh.mid[c]);
+                             m_ov, k11, k22, kM, acc);
+                const float2 u1 = sh.z[c][0][mf::M / 2], u2 = sh.z[c][1][mf::M / 2];
+                if (l == jf) {
+                    ua1 = u1;
+                    ua2 = u2;
+                }
                 mf_publish(&sh.cons[c], i + 1);   // this slot's reads are done
             }
             if (i < nb) {
-                const int q = c + mf::CONS;
                 mf_wait_ge(&sh.prod[q], i + 1, ok, &sh.broken);
                 mf_mac<FULL>(sh.z[q][0], sh.z[q][1], sh.twu, pb, jf, l, do_acf, do_xcf, m11, m22,
-                             m_ov, k11, k22, kM, acc, sh.mid[c]);
+                             m_ov, k11, k22, kM, acc);
+                const float2 u1 = sh.z[q][0][mf::M / 2], u2 = sh.z[q][1][mf::M / 2];
+                if (l == jf) {
+                    ub1 = u1;
+                    ub2 = u2;
+                }
                 mf_publish(&sh.cons[q], i + 1);
             }
         }
+        if (l < cnt && g + l < na) mf_mid(pa, ua1, ua2, do_acf, do_xcf, in11, in22, in_ov, macc);
+        if (l < cnt && g + l < nb) mf_mid(pb, ub1, ub2, do_acf, do_xcf, in11, in22, in_ov, macc);
     }
 }
 
@@ -898,7 +919,6 @@ __global__ __launch_bounds__(mf::THREADS3) void k_sector_mf3(SectorArgs a) {
             sh.tw2[i] = gt[i];
         }
         for (int i = tid; i < 8 * 64; i += NT) sh.twu[i] = gu[i];
-        if (tid < mf::CONS * 4) (&sh.mid[0][0])[tid] = 0.0;
         if (tid < mf::PROD) {
             sh.prod[tid] = 0;
             sh.cons[tid] = 0;
@@ -918,7 +938,7 @@ __global__ __launch_bounds__(mf::THREADS3) void k_sector_mf3(SectorArgs a) {
         r1 = (r0 + per < wg1) ? r0 + per : wg1;
     };
 
-    double acc[8][4];
+    double acc[8][4], macc[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int s = 0; s < 8; ++s)
 #pragma unroll
@@ -933,14 +953,19 @@ __global__ __launch_bounds__(mf::THREADS3) void k_sector_mf3(SectorArgs a) {
         range(c, fa0, fa1);
         range(c + mf::CONS, fb0, fb1);
         const bool full = (a.flags & FLAG_ACF_OVERLAP) == 0 && a.a1s == 0 && a.ov == M;
-        if (full) mf3_consumer<true>(a, sh, c, l, fa0, fa1, fb0, fb1, acc);
-        else mf3_consumer<false>(a, sh, c, l, fa0, fa1, fb0, fb1, acc);
+        if (full) mf3_consumer<true>(a, sh, c, l, fa0, fa1, fb0, fb1, acc, macc);
+        else mf3_consumer<false>(a, sh, c, l, fa0, fa1, fb0, fb1, acc, macc);
     }
 
     // ---- workgroup partial: consumers add their bins in a fixed order
     __syncthreads();
     double *red = reinterpret_cast<double *>(&sh.z[0][0][0]);   // [4][M + 1]
+    double *mids = reinterpret_cast<double *>(&sh.z[4][0][0]);  // [CONS][4][64] lanes' bin-M/2 sums
     constexpr int H = M + 1;
+    if (wv >= mf::PROD)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) mids[((wv - mf::PROD) * 4 + c) * 64 + l] = macc[c];
+    __syncthreads();
     for (int w = mf::PROD; w < mf::PROD + mf::CONS; ++w) {
         if (wv == w) {
             const bool first = w == mf::PROD;
@@ -952,8 +977,12 @@ __global__ __launch_bounds__(mf::THREADS3) void k_sector_mf3(SectorArgs a) {
             }
             if (l == 0)
 #pragma unroll
-                for (int c = 0; c < 4; ++c)
-                    red[c * H + M / 2] = (first ? 0.0 : red[c * H + M / 2]) + sh.mid[w - mf::PROD][c];
+                for (int c = 0; c < 4; ++c) {
+                    const double *mc = mids + ((w - mf::PROD) * 4 + c) * 64;
+                    double sum = 0.0;
+                    for (int i = 0; i < 64; ++i) sum += mc[i];   // lanes in order
+                    red[c * H + M / 2] = (first ? 0.0 : red[c * H + M / 2]) + sum;
+                }
         }
         __syncthreads();
     }
