@@ -25,6 +25,17 @@ __device__ __forceinline__ cd d_mul(cd a, cd b) { return cd{a.x * b.x - a.y * b.
 // stride-E writes of the first pass free of bank conflicts.
 __device__ __forceinline__ int pad(int i) { return i + (i >> 4); }
 
+// The value v, opaque to the optimizer from here on.  A frame loop that
+// takes its thread indices through this forms the LDS addresses and
+// twiddle loads derived from them inside the loop; otherwise the compiler
+// hoists all of them out of the loop and keeps them in registers for the
+// whole kernel, which doubles the Stockham kernels' register count and
+// halves their occupancy (216 -> 120 VGPRs at N = 1024).
+__device__ __forceinline__ int loop_local(int v) {
+    asm volatile("" : "+v"(v));
+    return v;
+}
+
 // ------------------------------------------------------------ delay model
 // Rust `v as i64`: truncation with saturation, NaN -> 0.
 __device__ __forceinline__ int64_t f64_to_i64_sat(double v) {

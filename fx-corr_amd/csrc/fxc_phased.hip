@@ -38,9 +38,10 @@ __global__ __launch_bounds__(2 * T) void k_phased(SectorArgs a, int accumulate) 
     __shared__ PhasedParam pp[kFrameBatch];
     __shared__ float lev[2][256];
 
-    const int tid = threadIdx.x;
-    const int s = half_index<T>(tid);
-    const int t = tid - s * T;
+    const int tid0 = threadIdx.x;
+    const int s = half_index<T>(tid0);
+    const int t0 = tid0 - s * T;
+    const int tid = tid0, t = t0;
     const int N = 2 * M;
     const int64_t f_begin = (int64_t)blockIdx.x * a.frames_per_wg;
     int64_t f_end = f_begin + a.frames_per_wg;
@@ -72,6 +73,7 @@ __global__ __launch_bounds__(2 * T) void k_phased(SectorArgs a, int accumulate) 
         __syncthreads();
         for (int jf = 0; jf < nb; ++jf) {
             const int64_t f = fb + jf;
+            const int tid = loop_local(tid0), t = loop_local(t0);   // see loop_local
             stage_words<T>(raw + f * st.bpf, st, ws, t);
             __syncthreads();
             const int sh = s ? pp[jf].s2 : pp[jf].s1;
