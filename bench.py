@@ -6,10 +6,12 @@ per station, 8,000,000 frames, 8 one-second sectors), station B LSB with a
 seeded 32-entry bit shuffle, geometric delay from two ECEF positions ~108 m
 apart toward a seeded J2000 RA/Dec (the library's 1-s geometric table), clock
 1.2 us / 1e-12 s/s, residual 0.2 samples / 0.05 Hz, correlated Gaussian
-signal (rho = 0.005) generated on the device.  One step = the whole 8-sector job (ACF + XCF) with the input already
-resident in HBM.  Both arithmetic forms are timed; `value` is the chosen form.
+signal (rho = 0.005) generated on the device.  One step = the whole 8-sector
+job (ACF + XCF) with the input already resident in HBM.  Both arithmetic
+forms are timed; `value` is the chosen form.  `--sizes` adds both forms'
+throughput at FFT sizes 256..65536.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--sizes]
 """
 from __future__ import annotations
 
@@ -249,6 +251,49 @@ def load_traffic(key):
     return None
 
 
+FORM_NAMES = {0: "FP32 Stockham", 1: "FP16-MFMA four-step", 2: "FP32 four-step through HBM"}
+
+
+def size_sweep(stream, samples_per_station=1 << 28, reps=5):
+    """Throughput of both forms at FFT sizes 256..65536: two 2-bit stations at
+    1024 Msps, device-resident synthetic input (station B LSB), the plans'
+    default configuration, `reps` timed sector runs after one warm-up."""
+    rows = {}
+    fs = 1024e6
+    for n in [256, 512, 1024, 2048, 4096, 8192, 16384, 32768, 65536]:
+        nf = samples_per_station // n
+        bpf = n * 2 // 8
+        d1, d2 = F.DeviceBuffer(nf * bpf), F.DeviceBuffer(nf * bpf)
+        F.synth_station_device(d1.ptr, nf * n, 2, None, False, 11, 0, 0.01, 3, stream.ptr)
+        F.synth_station_device(d2.ptr, nf * n, 2, None, True, 11, 1, 0.01, 3, stream.ptr)
+        stream.sync()
+        cfg = F.make_delay_cfg(np.zeros(2 + int(nf * n / fs)), frame_dt=n / fs, fs=fs,
+                               delay_user_samples=2.3, lo1_hz=OBS_HZ, lo2_hz=OBS_HZ)
+        row = {"frames": nf}
+        for name, prec in (("plain", F.PRECISION_PLAIN_FP32), ("chosen", F.PRECISION_CHOSEN)):
+            p = F.Plan(n, levels=[LEVELS, LEVELS], lsb=(False, True), df_hz=fs / n, precision=prec)
+            acc = F.DeviceBuffer(p.half * 5 * 8)
+            p.sector_run_device(d1.ptr, d2.ptr, 0, nf, cfg, acc.ptr, stream=stream.ptr)
+            stream.sync()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                p.sector_run_device(d1.ptr, d2.ptr, 0, nf, cfg, acc.ptr, stream=stream.ptr)
+            stream.sync()
+            el = (time.perf_counter() - t0) / reps
+            row[name] = round(nf * n / el / 1e6, 1)
+            row[name + "_fft"] = FORM_NAMES[p.info.fft_form]
+            acc.free()
+            p.close()
+        d1.free()
+        d2.free()
+        rows[str(n)] = row
+        print(f"[bench] FFT {n:6d}: plain {row['plain']:,.0f}, chosen {row['chosen']:,.0f} "
+              f"Msamples/s/station ({row['chosen_fft']})", file=sys.stderr, flush=True)
+    return {"workload": f"two 2-bit stations at 1024 Msps, {samples_per_station} samples per "
+                        f"station per size, input resident in HBM, default configuration, "
+                        f"mean of {reps} sector runs", "unit": "Msamples/s/station", "sizes": rows}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -272,6 +317,8 @@ def main():
     ap.add_argument("--mfma-variant", type=int, choices=(0, 1), default=None,
                     help="pin the MFMA kernel organization (0 whole-frame waves, "
                          "1 producer/consumer waves) instead of tuning it")
+    ap.add_argument("--sizes", action="store_true",
+                    help="also time both forms at FFT sizes 256..65536 (adds 'fft_sizes')")
     ap.add_argument("--pmc-summary", nargs=3, metavar=("FETCH_DIR", "WRITE_DIR", "OUT_JSON"),
                     help="summarise two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of "
                          "'bench.py --copy-mib M' into OUT_JSON and exit")
@@ -471,6 +518,8 @@ def main():
         print(f"[bench] cpu baseline: {cpu['value']:.1f} Msamples/s/station "
               f"({threads} threads, {el:.1f} s)", file=sys.stderr, flush=True)
 
+    sizes = size_sweep(stream) if args.sizes else None
+
     r = results[main_form]
     line = {
         "metric": "Msamples/s/station @2-bit FFT1024",
@@ -505,6 +554,8 @@ def main():
         "pcie_inclusive": pcie,
         "serving": serving,
     }
+    if sizes is not None:
+        line["fft_sizes"] = sizes
     if "plain" in results and "chosen" in results:
         line["chosen_over_plain"] = round(results["chosen"]["value"] / results["plain"]["value"], 3)
     print(json.dumps(line), flush=True)
