@@ -1000,14 +1000,26 @@ static fxc_status enqueue_phased(fxc_plan *p, const SectorArgs &base, int64_t nf
 // N = 32768 / 65536: chunks of frames through HBM scratch; the product
 // kernel's kLargeGroups frame-group partial slots accumulate over the chunks
 // in order, then one fixed-order reduce.
+// Chains of the large-N plain path are computed for a span of whole chunks
+// at a time (up to this much table): each chain is one thread's sequential
+// walk, so the frames of one chunk alone (~600) would leave the GPU idle.
+constexpr size_t kLargeChainBytes = (size_t)4 << 30;
+
 static fxc_status enqueue_large(fxc_plan *p, const SectorArgs &base, int64_t nf, double *d_acc,
                                 hipStream_t s) {
     const float2 *twl1 = (const float2 *)(p->M1 == 128 ? p->tw_l1.p : p->tw_l2.p);
     const float2 *twl2 = (const float2 *)(p->M2 == 128 ? p->tw_l1.p : p->tw_l2.p);
     const bool chain = xcf_uses_chain(p);
     const int ld = chain ? chain_ld_for(base.ov) : 0;
-    if (chain && p->d_chain.ensure((size_t)p->chunk * ld * 8) != hipSuccess)
-        return fail(FXC_ERR_NOMEM, "phase chain allocation failed");
+    int64_t span = 0;      // frames whose chains one launch computes: whole chunks
+    if (chain) {
+        const int64_t fit = std::max<int64_t>(1, (int64_t)(kLargeChainBytes / ((size_t)ld * 8)) / p->chunk_use);
+        const int64_t need = (nf + p->chunk_use - 1) / p->chunk_use;
+        span = std::min(fit, need) * p->chunk_use;
+        if (p->d_chain.ensure((size_t)span * ld * 8) != hipSuccess)
+            return fail(FXC_ERR_NOMEM, "phase chain allocation failed");
+    }
+    int64_t span0 = -1;    // first frame (of this sector) of the computed chains
     for (int64_t c0 = 0; c0 < nf; c0 += p->chunk_use) {
         const int64_t cn = std::min(p->chunk_use, nf - c0);
         SectorArgs a = base;
@@ -1026,9 +1038,15 @@ static fxc_status enqueue_large(fxc_plan *p, const SectorArgs &base, int64_t nf,
                                      (const float2 *)p->tw_lm.p, s));
         }
         if (chain) {
-            a.chain = (const float2 *)p->d_chain.p;
+            if (span0 < 0 || c0 >= span0 + span) {
+                SectorArgs ca = a;
+                ca.nf = std::min(span, nf - c0);
+                ca.chain_ld = ld;
+                HIP_TRY(launch_phase_chain(ca, CHAIN_XCF, (float2 *)p->d_chain.p, s));
+                span0 = c0;
+            }
+            a.chain = (const float2 *)p->d_chain.p + (size_t)(c0 - span0) * ld;
             a.chain_ld = ld;
-            HIP_TRY(launch_phase_chain(a, CHAIN_XCF, (float2 *)p->d_chain.p, s));
         }
         HIP_TRY(launch_mac_large(a, p->desc.precision, (const float2 *)p->d_z[0].p,
                                  (const float2 *)p->d_z[1].p, params, kLargeGroups, c0 > 0 ? 1 : 0,

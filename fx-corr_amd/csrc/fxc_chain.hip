@@ -15,6 +15,8 @@
 // angles) moves a chain by at most ~5e-7 over 30720 links.
 //
 // One thread per chain writes its row in pairs of links (16-byte stores).
+// Blocks of one wave, so that the few long chains of a launch spread over
+// the compute units (each chain's walk is latency-bound).
 #include "fxc_dev.h"
 
 namespace fxc {
@@ -28,7 +30,7 @@ __device__ __forceinline__ float2 c_mul_rust(float2 a, float2 b) {
     return make_float2(rr - ii, ri + ir);
 }
 
-__global__ __launch_bounds__(256) void k_phase_chain(SectorArgs a, int kind, float2 *out) {
+__global__ __launch_bounds__(64) void k_phase_chain(SectorArgs a, int kind, float2 *out) {
     const int nch = kind == CHAIN_XCF ? 1 : 2;
     const int len = kind == CHAIN_XCF ? a.ov : a.M + 1;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -67,7 +69,7 @@ hipError_t launch_phase_chain(const SectorArgs &a, int kind, float2 *out, hipStr
     if (n <= 0) return hipSuccess;
     const int len = kind == CHAIN_XCF ? a.ov : a.M + 1;
     if (a.chain_ld < chain_ld_for(len) || (a.chain_ld & 1)) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_phase_chain, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, kind,
+    hipLaunchKernelGGL(k_phase_chain, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s, a, kind,
                        out);
     return hipGetLastError();
 }
