@@ -952,6 +952,25 @@ static bool xcf_uses_chain(const fxc_plan *p) {
 }
 constexpr size_t kChainSlabBytes = (size_t)256 << 20;   // chain scratch of the fused plain path
 
+// Long phase chains of the chunked paths (phased, large-N plain) are
+// computed for a span of whole chunks at a time, up to this much table: each
+// chain is one thread's sequential walk, so the frames of one chunk alone (a
+// few hundred to a few thousand) would leave most of the GPU idle for the
+// length of the walk.  Chains shorter than kChainSpanMinLen stay per chunk:
+// their walks are short and a chunk's table stays in the last-level cache
+// between the chain and product kernels.
+constexpr size_t kChainSpanBytes = (size_t)4 << 30;
+constexpr int kChainSpanMinLen = 4096;
+
+// Frames (whole chunks, at least one) whose chains (`len` links, `row_bytes`
+// per frame) one launch computes for a sector of nf frames.
+static int64_t chain_span(int64_t nf, int64_t chunk, size_t row_bytes, int len) {
+    if (len < kChainSpanMinLen) return chunk;
+    const int64_t fit = std::max<int64_t>(1, (int64_t)(kChainSpanBytes / row_bytes) / chunk);
+    const int64_t need = (nf + chunk - 1) / chunk;
+    return std::min(fit, need) * chunk;
+}
+
 static fxc_status enqueue_phased(fxc_plan *p, const SectorArgs &base, int64_t nf,
                                  uint8_t *d_raw_out, double *d_acc, hipStream_t s) {
     const bool raw = (p->desc.products & FXC_PRODUCT_RAW_OUT) && d_raw_out != nullptr;
@@ -963,7 +982,8 @@ static fxc_status enqueue_phased(fxc_plan *p, const SectorArgs &base, int64_t nf
     const int ld = chain_ld_for((int)half);
     const size_t per_frame = 2 * (size_t)ld * 8 + half * 8 + (size_t)p->N * 4;
     const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(nf, (int64_t)(((size_t)256 << 20) / per_frame)));
-    if (p->d_chain.ensure((size_t)chunk * 2 * ld * 8) != hipSuccess)
+    const int64_t span = chain_span(nf, chunk, 2 * (size_t)ld * 8, (int)half);
+    if (p->d_chain.ensure((size_t)span * 2 * ld * 8) != hipSuccess)
         return fail(FXC_ERR_NOMEM, "phased scratch allocation failed");
     if (raw && (p->d_cb.ensure((size_t)chunk * half * 8) != hipSuccess ||
                 p->d_tbuf.ensure((size_t)chunk * p->N * 4) != hipSuccess))
@@ -972,6 +992,7 @@ static fxc_status enqueue_phased(fxc_plan *p, const SectorArgs &base, int64_t nf
     fpw = ((fpw + kFrameBatch - 1) / kFrameBatch) * kFrameBatch;
     if (fpw < kFrameBatch) fpw = kFrameBatch;
     const int n_wg = (int)((chunk + fpw - 1) / fpw);
+    int64_t span0 = -1;    // first frame (of this sector) of the computed chains
     for (int64_t c0 = 0; c0 < nf; c0 += chunk) {
         const int64_t cn = std::min(chunk, nf - c0);
         SectorArgs a = base;
@@ -982,9 +1003,15 @@ static fxc_status enqueue_phased(fxc_plan *p, const SectorArgs &base, int64_t nf
         a.frames_per_wg = fpw;
         a.part = (double *)p->part.p;
         a.cb_out = raw ? (float2 *)p->d_cb.p : nullptr;
-        a.chain = (const float2 *)p->d_chain.p;
+        if (span0 < 0 || c0 >= span0 + span) {
+            SectorArgs ca = a;
+            ca.nf = std::min(span, nf - c0);
+            ca.chain_ld = ld;
+            HIP_TRY(launch_phase_chain(ca, CHAIN_PHASED, (float2 *)p->d_chain.p, s));
+            span0 = c0;
+        }
+        a.chain = (const float2 *)p->d_chain.p + (size_t)(c0 - span0) * 2 * ld;   // rows 2f + station
         a.chain_ld = ld;
-        HIP_TRY(launch_phase_chain(a, CHAIN_PHASED, (float2 *)p->d_chain.p, s));
         HIP_TRY(launch_phased(a, c0 > 0 ? 1 : 0, (int)((cn + fpw - 1) / fpw), s));
         if (raw) {
             HIP_TRY(launch_irfft_frames(a, (const float2 *)p->d_cb.p, cn, (float *)p->d_tbuf.p,
@@ -1000,11 +1027,6 @@ static fxc_status enqueue_phased(fxc_plan *p, const SectorArgs &base, int64_t nf
 // N = 32768 / 65536: chunks of frames through HBM scratch; the product
 // kernel's kLargeGroups frame-group partial slots accumulate over the chunks
 // in order, then one fixed-order reduce.
-// Chains of the large-N plain path are computed for a span of whole chunks
-// at a time (up to this much table): each chain is one thread's sequential
-// walk, so the frames of one chunk alone (~600) would leave the GPU idle.
-constexpr size_t kLargeChainBytes = (size_t)4 << 30;
-
 static fxc_status enqueue_large(fxc_plan *p, const SectorArgs &base, int64_t nf, double *d_acc,
                                 hipStream_t s) {
     const float2 *twl1 = (const float2 *)(p->M1 == 128 ? p->tw_l1.p : p->tw_l2.p);
@@ -1013,9 +1035,7 @@ static fxc_status enqueue_large(fxc_plan *p, const SectorArgs &base, int64_t nf,
     const int ld = chain ? chain_ld_for(base.ov) : 0;
     int64_t span = 0;      // frames whose chains one launch computes: whole chunks
     if (chain) {
-        const int64_t fit = std::max<int64_t>(1, (int64_t)(kLargeChainBytes / ((size_t)ld * 8)) / p->chunk_use);
-        const int64_t need = (nf + p->chunk_use - 1) / p->chunk_use;
-        span = std::min(fit, need) * p->chunk_use;
+        span = chain_span(nf, p->chunk_use, (size_t)ld * 8, base.ov);
         if (p->d_chain.ensure((size_t)span * ld * 8) != hipSuccess)
             return fail(FXC_ERR_NOMEM, "phase chain allocation failed");
     }
