@@ -9,9 +9,10 @@ apart toward a seeded J2000 RA/Dec (the library's 1-s geometric table), clock
 signal (rho = 0.005) generated on the device.  One step = the whole 8-sector
 job (ACF + XCF) with the input already resident in HBM.  Both arithmetic
 forms are timed; `value` is the chosen form.  `--sizes` adds both forms'
-throughput at FFT sizes 256..65536.
+throughput at FFT sizes 256..65536, `--phased` the phased-array path's.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--sizes]
+       [--phased]
 """
 from __future__ import annotations
 
@@ -294,6 +295,52 @@ def size_sweep(stream, samples_per_station=1 << 28, reps=5):
                         f"mean of {reps} sector runs", "unit": "Msamples/s/station", "sizes": rows}
 
 
+def phased_sweep(stream, samples_per_station=1 << 28, reps=3):
+    """Phased-array path (yi-phasedarray): two 2-bit stations at 1024 Msps
+    summed into the phased power spectrum, without and with the 2-bit .raw
+    output, at FFT sizes 1024, 4096 and 16384; device-resident input, mean of
+    `reps` sector runs after one warm-up."""
+    rows = {}
+    fs = 1024e6
+    for n in [1024, 4096, 16384]:
+        nf = samples_per_station // n
+        bpf = n * 2 // 8
+        d1, d2 = F.DeviceBuffer(nf * bpf), F.DeviceBuffer(nf * bpf)
+        F.synth_station_device(d1.ptr, nf * n, 2, None, False, 13, 0, 0.3, 2, stream.ptr)
+        F.synth_station_device(d2.ptr, nf * n, 2, None, False, 13, 1, 0.3, 2, stream.ptr)
+        stream.sync()
+        cfg = F.make_delay_cfg(np.zeros(2 + int(nf * n / fs)), frame_dt=n / fs, fs=fs,
+                               delay_user_samples=2.3, lo1_hz=OBS_HZ, lo2_hz=OBS_HZ)
+        row = {"frames": nf}
+        for name, raw in (("power", False), ("power_and_raw", True)):
+            prods = F.PRODUCT_PHASED | (F.PRODUCT_RAW_OUT if raw else 0)
+            p = F.Plan(n, levels=[LEVELS, LEVELS], df_hz=fs / n, products=prods, bit_out=2)
+            acc = F.DeviceBuffer(p.half * 5 * 8)
+            out = F.DeviceBuffer(nf * p.info.bpf_out) if raw else None
+            optr = out.ptr if raw else None
+            p.sector_run_device(d1.ptr, d2.ptr, 0, nf, cfg, acc.ptr, optr, stream=stream.ptr)
+            stream.sync()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                p.sector_run_device(d1.ptr, d2.ptr, 0, nf, cfg, acc.ptr, optr, stream=stream.ptr)
+            stream.sync()
+            el = (time.perf_counter() - t0) / reps
+            row[name] = round(nf * n / el / 1e6, 1)
+            acc.free()
+            if out is not None:
+                out.free()
+            p.close()
+        d1.free()
+        d2.free()
+        rows[str(n)] = row
+        print(f"[bench] phased FFT {n:6d}: power {row['power']:,.0f}, power + .raw "
+              f"{row['power_and_raw']:,.0f} Msamples/s/station", file=sys.stderr, flush=True)
+    return {"workload": f"two 2-bit stations at 1024 Msps, weights 1/1, 2-bit .raw output, "
+                        f"{samples_per_station} samples per station per size, input resident "
+                        f"in HBM, mean of {reps} sector runs", "unit": "Msamples/s/station",
+            "sizes": rows}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -319,6 +366,8 @@ def main():
                          "1 producer/consumer waves) instead of tuning it")
     ap.add_argument("--sizes", action="store_true",
                     help="also time both forms at FFT sizes 256..65536 (adds 'fft_sizes')")
+    ap.add_argument("--phased", action="store_true",
+                    help="also time the phased-array path at FFT 1024/4096/16384 (adds 'phased')")
     ap.add_argument("--pmc-summary", nargs=3, metavar=("FETCH_DIR", "WRITE_DIR", "OUT_JSON"),
                     help="summarise two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of "
                          "'bench.py --copy-mib M' into OUT_JSON and exit")
@@ -519,6 +568,7 @@ def main():
               f"({threads} threads, {el:.1f} s)", file=sys.stderr, flush=True)
 
     sizes = size_sweep(stream) if args.sizes else None
+    phased = phased_sweep(stream) if args.phased else None
 
     r = results[main_form]
     line = {
@@ -556,6 +606,8 @@ def main():
     }
     if sizes is not None:
         line["fft_sizes"] = sizes
+    if phased is not None:
+        line["phased"] = phased
     if "plain" in results and "chosen" in results:
         line["chosen_over_plain"] = round(results["chosen"]["value"] / results["plain"]["value"], 3)
     print(json.dumps(line), flush=True)
