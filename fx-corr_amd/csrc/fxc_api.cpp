@@ -510,17 +510,20 @@ static float f16_bits_to_f32(uint16_t h) {
     return v;
 }
 
-// The MFMA four-step form applies when N = 1024, the chosen precision is
-// requested, ACF/XCF products are computed on a common band start
-// (a1s == a2s, so a bin and its mirror M-k share one lane), and every level
-// (as the f32 the decode produces, utils.rs:186) is exact in FP16.
+// The MFMA four-step form applies when N = 1024 or 2048, the chosen
+// precision is requested, ACF/XCF products are computed on a common band
+// start (a1s == a2s, so a bin and its mirror M-k share one lane), every
+// level (as the f32 the decode produces, utils.rs:186) is exact in FP16, and
+// a station-frame is at most 256 packed words (1/2/4/8-bit at N = 1024,
+// 1/2/4-bit at N = 2048).
 static bool mfma_eligible(const fxc_plan *p) {
-    if (p->N != 1024 || p->desc.precision != FXC_PRECISION_CHOSEN) return false;
+    if ((p->N != 1024 && p->N != 2048) || p->desc.precision != FXC_PRECISION_CHOSEN) return false;
     if (p->desc.products & (FXC_PRODUCT_PHASED | FXC_PRODUCT_RAW_OUT)) return false;
     if (p->desc.a1s != p->desc.a2s) return false;
     for (int i = 0; i < 2; ++i) {
         const int b = p->desc.st[i].bits;
         if (b != 1 && b != 2 && b != 4 && b != 8) return false;
+        if (p->N * b / 32 > 256) return false;
         for (double lv : p->levels[i]) {
             const float f = (float)lv;
             if (!std::isfinite(f) || f16_bits_to_f32(f32_to_f16_bits(f)) != f) return false;
@@ -540,11 +543,18 @@ static bool mfma_eligible(const fxc_plan *p) {
 //     input (m2 = 4 g + b/2, b%2: re|im).
 //  TW2 idx (ct*4 + i)*2 + part: W512^{m2 k1}, m2 = 4 g + i, k1 = c + 16 ct.
 //  TWU idx 2j + part: W1024^p, p = l + 64 j.
+// N = 2048 (MFT4_*): the same B1, DLUT and SLUT; no A2;
+//  TW2 idx ((rt*2 + h)*4 + i)*2 + part: W1024^{m2 k1}, m2 = 16 rt + 4 g + i,
+//     k1 = c + 16 h;
+//  TWU idx 2j + part: W2048^p, p = l + 64 j, j < 8.
 //  DLUT per station: FP16 samples of each byte value (code = low bits
 //     first, odd samples negated for LSB stations); 8-bit keeps an odd-byte
 //     copy at +256.  SLUT per station: generic shuffle as 4 byte tables.
 static std::vector<uint8_t> mfma_tables(const fxc_plan *p) {
-    std::vector<uint8_t> tab(MFT_BYTES, 0);
+    const bool wide = p->N == 2048;
+    const size_t o_b1 = wide ? MFT4_B1 : MFT_B1, o_dlut = wide ? MFT4_DLUT : MFT_DLUT;
+    const size_t o_slut = wide ? MFT4_SLUT : MFT_SLUT;
+    std::vector<uint8_t> tab(wide ? MFT4_BYTES : MFT_BYTES, 0);
     auto put16 = [&](size_t off, uint16_t v) { memcpy(&tab[off], &v, 2); };
     auto putf = [&](size_t off, float v) { memcpy(&tab[off], &v, 4); };
     auto put32 = [&](size_t off, uint32_t v) { memcpy(&tab[off], &v, 4); };
@@ -569,10 +579,28 @@ static std::vector<uint8_t> mfma_tables(const fxc_plan *p) {
                     const uint16_t hi = f32_to_f16_bits((float)v);
                     const uint16_t lo = f32_to_f16_bits((float)(v - (double)f16_bits_to_f32(hi)));
                     const size_t idx_hi = (size_t)((kap * 4 + t) * 2 + 0), idx_lo = idx_hi + 1;
-                    put16(MFT_B1 + ((idx_hi * 64 + l) * 8 + s) * 2, hi);
-                    put16(MFT_B1 + ((idx_lo * 64 + l) * 8 + s) * 2, lo);
+                    put16(o_b1 + ((idx_hi * 64 + l) * 8 + s) * 2, hi);
+                    put16(o_b1 + ((idx_lo * 64 + l) * 8 + s) * 2, lo);
                 }
-    for (int rt = 0; rt < 2; ++rt)
+    if (wide) {
+        for (int rt = 0; rt < 2; ++rt)
+            for (int h = 0; h < 2; ++h)
+                for (int i = 0; i < 4; ++i)
+                    for (int l = 0; l < 64; ++l) {
+                        const int m2 = 16 * rt + 4 * (l >> 4) + i, k1 = (l & 15) + 16 * h;
+                        const double th = -2.0 * M_PI * (double)((m2 * k1) % 1024) / 1024.0;
+                        const size_t idx = (size_t)((rt * 2 + h) * 4 + i) * 2;
+                        putf(MFT4_TW2 + (idx * 64 + l) * 4, (float)std::cos(th));
+                        putf(MFT4_TW2 + ((idx + 1) * 64 + l) * 4, (float)std::sin(th));
+                    }
+        for (int j = 0; j < 8; ++j)
+            for (int l = 0; l < 64; ++l) {
+                const double th = -2.0 * M_PI * (double)(l + 64 * j) / 2048.0;
+                putf(MFT4_TWU + ((size_t)(2 * j) * 64 + l) * 4, (float)std::cos(th));
+                putf(MFT4_TWU + ((size_t)(2 * j + 1) * 64 + l) * 4, (float)std::sin(th));
+            }
+    }
+    for (int rt = 0; rt < 2 && !wide; ++rt)
         for (int b = 0; b < 8; ++b)
             for (int l = 0; l < 64; ++l) {
                 const int k2 = l & 15, m2 = 4 * (l >> 4) + (b >> 1), pin = b & 1;
@@ -581,7 +609,7 @@ static std::vector<uint8_t> mfma_tables(const fxc_plan *p) {
                 const double v = rt == 0 ? (pin == 0 ? cs : sn) : (pin == 0 ? -sn : cs);
                 putf(MFT_A2 + ((size_t)(rt * 8 + b) * 64 + l) * 4, (float)v);
             }
-    for (int ct = 0; ct < 2; ++ct)
+    for (int ct = 0; ct < 2 && !wide; ++ct)
         for (int i = 0; i < 4; ++i)
             for (int l = 0; l < 64; ++l) {
                 const int m2 = 4 * (l >> 4) + i, k1 = (l & 15) + 16 * ct;
@@ -590,7 +618,7 @@ static std::vector<uint8_t> mfma_tables(const fxc_plan *p) {
                 putf(MFT_TW2 + (idx * 64 + l) * 4, (float)std::cos(th));
                 putf(MFT_TW2 + ((idx + 1) * 64 + l) * 4, (float)std::sin(th));
             }
-    for (int j = 0; j < 4; ++j)
+    for (int j = 0; j < 4 && !wide; ++j)
         for (int l = 0; l < 64; ++l) {
             const double th = -2.0 * M_PI * (double)(l + 64 * j) / 1024.0;
             putf(MFT_TWU + ((size_t)(2 * j) * 64 + l) * 4, (float)std::cos(th));
@@ -600,7 +628,7 @@ static std::vector<uint8_t> mfma_tables(const fxc_plan *p) {
         const int bits = p->desc.st[st].bits;
         const bool lsb = p->desc.st[st].lsb != 0;
         const uint32_t mask = (1u << bits) - 1u;
-        const size_t base = MFT_DLUT + (size_t)st * MFT_DLUT_HALVES * 2;
+        const size_t base = o_dlut + (size_t)st * MFT_DLUT_HALVES * 2;
         auto lev = [&](uint32_t code, bool neg) {
             uint16_t h = f32_to_f16_bits((float)p->levels[st][code]);
             return (uint16_t)(neg ? (h ^ 0x8000u) : h);
@@ -617,7 +645,7 @@ static std::vector<uint8_t> mfma_tables(const fxc_plan *p) {
                     put16(base + (size_t)(v * spb + i) * 2,
                           lev(((uint32_t)v >> (bits * i)) & mask, lsb && (i & 1)));
         }
-        const size_t sbase = MFT_SLUT + (size_t)st * 4 * 256 * 4;
+        const size_t sbase = o_slut + (size_t)st * 4 * 256 * 4;
         for (int bp = 0; bp < 4; ++bp)
             for (int v = 0; v < 256; ++v) {
                 uint32_t o = 0;
@@ -1187,10 +1215,14 @@ extern "C" fxc_status fxc_plan_autotune(fxc_plan *p, const uint8_t *d_raw1, cons
         for (int v = 0; v < nv; ++v)
             for (int d : {1, 2, 4}) cands.push_back({v, d, 0, 0});
     } else {
-        // MFMA form: one workgroup per CU; the kernel organization varies
-        // (1, the default: producer/consumer waves; 0: every wave carries
-        // whole frames).  The default is timed first.
-        for (int v : {1, 0}) cands.push_back({v, 1, 0, 0});
+        // MFMA form: one workgroup per CU; at N = 1024 the kernel
+        // organization varies (1, the default: producer/consumer waves; 0:
+        // every wave carries whole frames).  The default is timed first.
+        // N = 2048 has one organization.
+        if (p->N == 1024)
+            for (int v : {1, 0}) cands.push_back({v, 1, 0, 0});
+        else
+            cands.push_back({1, 1, 0, 0});
     }
     const int save_variant = p->variant, save_div = p->grid_div, save_m1 = p->M1, save_m2 = p->M2;
     const int64_t save_chunk = p->chunk_use;
@@ -1266,7 +1298,8 @@ extern "C" fxc_status fxc_plan_autotune(fxc_plan *p, const uint8_t *d_raw1, cons
 extern "C" fxc_status fxc_plan_set_tuning(fxc_plan *p, const fxc_tune_report *c) {
     if (!p || !c) return fail(FXC_ERR_INVALID_ARG, "null argument");
     const bool v1_ok = (p->fft_form == 0 && sector_has_variant(p->N)) || p->fft_form == 1;
-    if (c->variant != 0 && !(c->variant == 1 && v1_ok))
+    const bool v0_ok = !(p->fft_form == 1 && p->N != 1024);
+    if ((c->variant != 0 && !(c->variant == 1 && v1_ok)) || (c->variant == 0 && !v0_ok))
         return fail(FXC_ERR_INVALID_ARG, "variant %d not available for this plan", c->variant);
     if (c->grid_divisor != 1 && c->grid_divisor != 2 && c->grid_divisor != 4)
         return fail(FXC_ERR_INVALID_ARG, "grid_divisor must be 1, 2 or 4");
@@ -1479,7 +1512,7 @@ static fxc_status run_fft(fxc_plan *p, const float *in, int64_t nframes, float *
     } else if (inverse)
         HIP_TRY(launch_irfft_frames(a, (const float2 *)p->d_tmp_in.p, nframes, (float *)p->d_tmp_out.p,
                                     false, s));
-    else if (p->fft_form == 1)
+    else if (p->fft_form == 1 && p->N == 1024)
         HIP_TRY(launch_rfft_mfma(a, (const float *)p->d_tmp_in.p, nframes, (float2 *)p->d_tmp_out.p, s));
     else
         HIP_TRY(launch_rfft_frames(a, (const float *)p->d_tmp_in.p, nframes, (float2 *)p->d_tmp_out.p, s));

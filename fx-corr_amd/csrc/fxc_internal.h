@@ -75,7 +75,7 @@ struct SectorArgs {
     int32_t bpf_out;
     uint8_t *raw_out;        // phased .raw frames (nf * bpf_out) or null
     double *part;            // [n_wg][kNumProducts][half]
-    const void *mf_tab;      // chosen-form MFMA tables (N = 1024), layout MFT_* below
+    const void *mf_tab;      // chosen-form MFMA tables, layout MFT_* (N = 1024) or MFT4_* (N = 2048)
     float2 *cb_out;          // phased: combined half spectra [frame][half] for the .raw output
     int32_t variant;         // FFT pass sequence of the plain kernel (autotuned; 0 default)
     // Plain form: the reference's sequential f32 phase chains of this
@@ -105,6 +105,14 @@ constexpr int MFT_DLUT_HALVES = 2048;          // per station: byte -> FP16 samp
 constexpr int MFT_DLUT = MFT_TWU + 8 * 64 * 4;
 constexpr int MFT_SLUT = MFT_DLUT + 2 * MFT_DLUT_HALVES * 2;  // 2 x 4 x 256 u32
 constexpr int MFT_BYTES = MFT_SLUT + 2 * 4 * 256 * 4;
+// The same tables for N = 2048 (k_sector_mf4): the DFT-32 operands are
+// those of N = 1024; the twiddles cover 32 rows m2 and 8 untangle pairs.
+constexpr int MFT4_B1 = 0;                       // 16 x 64 uint4
+constexpr int MFT4_TW2 = MFT4_B1 + 16 * 64 * 16; // 32 x 64 f32: W1024^{m2 k1}
+constexpr int MFT4_TWU = MFT4_TW2 + 32 * 64 * 4; // 16 x 64 f32: W2048^p
+constexpr int MFT4_DLUT = MFT4_TWU + 16 * 64 * 4;
+constexpr int MFT4_SLUT = MFT4_DLUT + 2 * MFT_DLUT_HALVES * 2;
+constexpr int MFT4_BYTES = MFT4_SLUT + 2 * 4 * 256 * 4;
 
 // Launchers implemented in fxc_device.hip.  All enqueue on `stream`.
 hipError_t launch_sector(const SectorArgs &a, int precision, int n_wg,

@@ -100,7 +100,7 @@ __device__ __forceinline__ double wrap_turns(double x) { return x - rint(x); }
 // per-bin XCF phase step dth = -df (frac1 - frac2), so that the XCF phasor
 // of grid bin k is exp(2 pi i (phw + k dth)) when both stations share the
 // band start (a1s == a2s).  F1 = exp(2 pi i 64 dth), C = exp(2 pi i (2 phw
-// + 512 dth)) turn the per-lane phasor into the phasors of its other bins.
+// + M dth)) turn the per-lane phasor into the phasors of its other bins.
 struct MfFrame {
     int32_t s1, s2;
     double phw, dth;
@@ -115,14 +115,14 @@ __device__ MfFrame mf_frame(int64_t idx, const SectorArgs &a) {
     double fr1, fr2;
     split_delay(tau1, a.dly.fs, n1, fr1);
     split_delay(tau2, a.dly.fs, n2, fr2);
-    o.s1 = clamp_shift(n1, mf::N);
-    o.s2 = clamp_shift(n2, mf::N);
+    o.s1 = clamp_shift(n1, a.N);
+    o.s2 = clamp_shift(n2, a.N);
     const double w1 = a.dly.lo1 == 0.0 ? 0.0 : wrap_turns(-a.dly.lo1 * tf1);
     const double w2 = a.dly.lo2 == 0.0 ? 0.0 : wrap_turns(-a.dly.lo2 * tf2);
     o.phw = w1 - w2;
     o.dth = -a.df * (fr1 - fr2);
     o.F1 = sincos_turns((float)wrap_turns(64.0 * o.dth));
-    o.C = sincos_turns((float)wrap_turns(2.0 * o.phw + 512.0 * o.dth));
+    o.C = sincos_turns((float)wrap_turns(2.0 * o.phw + (double)a.M * o.dth));
     return o;
 }
 
@@ -185,41 +185,50 @@ __device__ __forceinline__ void mf_stage(const uint32_t w[4], int bits, int kind
 // position PAD + 32 m1 + 2 m2 - sh.  (The lane groups g of a half-wave then
 // read 16 dwords apart: no LDS bank conflict.)  Positions outside the
 // staged row read the zero padding.
-__device__ __forceinline__ void mf_build_a(const uint16_t *row, int sh, int l, half8 &a0, half8 &a1) {
+// R2 rows m2 per station-frame (M = 32 R2, m = R2 m1 + m2); row tile rt
+// covers m2 = 16 rt + (l % 16).  SROW: halves per staged row.
+template <int R2, int SROW>
+__device__ __forceinline__ void mf_build_a_t(const uint16_t *row, int sh, int l, int rt, half8 &a0,
+                                             half8 &a1) {
+    constexpr int OK = 16 * R2, OJ = 4 * R2;   // dword offsets of K-step kap and slot pair j
     const uint32_t *r32 = reinterpret_cast<const uint32_t *>(row);
-    const int base = mf::PAD + 2 * (l & 15) - sh + 32 * (l >> 4);
+    const int base = mf::PAD + 2 * ((l & 15) + 16 * rt) - sh + 2 * R2 * (l >> 4);
     uint32_t d[8];
     const bool near = (sh >= -mf::PAD) && (sh <= mf::PAD);
     if (near && (sh & 1) == 0) {
         // base is even: one dword pointer, constant offsets
         const uint32_t *q = r32 + (base >> 1);
 #pragma unroll
-        for (int k = 0; k < 8; ++k) d[k] = q[256 * (k >> 2) + 64 * (k & 3)];
+        for (int k = 0; k < 8; ++k) d[k] = q[OK * (k >> 2) + OJ * (k & 3)];
     } else if (near) {
         // odd shift: the FP16 pair straddles two dwords
         const uint32_t *q = r32 + ((base - 1) >> 1);
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
-            const int o = 256 * (k >> 2) + 64 * (k & 3);
+            const int o = OK * (k >> 2) + OJ * (k & 3);
             d[k] = __builtin_amdgcn_alignbyte(q[o + 1], q[o], 2);
         }
     } else if ((sh & 1) == 0) {
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
-            int p = base + 512 * (k >> 2) + 128 * (k & 3);
-            p = (p >= 0 && p < mf::SROW - 1) ? p : 0;
+            int p = base + 2 * OK * (k >> 2) + 2 * OJ * (k & 3);
+            p = (p >= 0 && p < SROW - 1) ? p : 0;
             d[k] = r32[p >> 1];
         }
     } else {
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
-            int p = base + 512 * (k >> 2) + 128 * (k & 3);
-            p = (p >= 0 && p < mf::SROW - 1) ? p : 1;
+            int p = base + 2 * OK * (k >> 2) + 2 * OJ * (k & 3);
+            p = (p >= 0 && p < SROW - 1) ? p : 1;
             d[k] = __builtin_amdgcn_alignbyte(r32[(p + 1) >> 1], r32[(p - 1) >> 1], 2);
         }
     }
     a0 = __builtin_bit_cast(half8, make_uint4(d[0], d[1], d[2], d[3]));
     a1 = __builtin_bit_cast(half8, make_uint4(d[4], d[5], d[6], d[7]));
+}
+
+__device__ __forceinline__ void mf_build_a(const uint16_t *row, int sh, int l, half8 &a0, half8 &a1) {
+    mf_build_a_t<16, mf::SROW>(row, sh, l, 0, a0, a1);
 }
 
 // ------------------------------------------------------------- FFT
@@ -342,18 +351,10 @@ __device__ __forceinline__ void mf_dft4(float2 x0, float2 x1, float2 x2, float2 
     o3 = make_float2(a1.x - b1.y, a1.y + b1.x);   // a1 + i b1
 }
 
-// Column DFT-16 of both stations: reads the transposed Y', writes Z/2.
-__device__ __forceinline__ void mf_columns_dft16(float2 *slot, int l) {
-    const int s = l >> 5, k1 = l & 31;
-    float2 x[16];
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-        const float4 q = *reinterpret_cast<const float4 *>(slot + mf_tidx(s, k1, j));
-        x[2 * j] = make_float2(q.x, q.y);
-        x[2 * j + 1] = make_float2(q.z, q.w);
-    }
-    wave_lds_sync();   // every lane's column is read before Z overwrites the slot
-    // m2 = 4g + r, k2 = a + 4b: Z[a + 4b] = sum_r W4^{rb} W16^{ra} sum_g W4^{ga} x[4g + r]
+// DFT-16 of x[0..15] in registers (radix 4 x 4: input n = 4g + r, output
+// k = a + 4b, X[a + 4b] = sum_r W4^{rb} W16^{ra} sum_g W4^{ga} x[4g + r]);
+// X[k] is written to zo[stride * k].
+__device__ __forceinline__ void mf_dft16_out(const float2 (&x)[16], float2 *zo, int stride) {
     float2 u[4][4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) mf_dft4(x[r], x[4 + r], x[8 + r], x[12 + r], u[0][r], u[1][r], u[2][r], u[3][r]);
@@ -366,16 +367,29 @@ __device__ __forceinline__ void mf_columns_dft16(float2 *slot, int l) {
     u[3][1] = mf_w16<3>(u[3][1]);
     u[3][2] = mf_w16<6>(u[3][2]);
     u[3][3] = mf_w16<9>(u[3][3]);
-    float2 *zo = slot + s * mf::M + k1;
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
         float2 o0, o1, o2, o3;
         mf_dft4(u[a][0], u[a][1], u[a][2], u[a][3], o0, o1, o2, o3);
-        zo[32 * a] = o0;
-        zo[32 * (a + 4)] = o1;
-        zo[32 * (a + 8)] = o2;
-        zo[32 * (a + 12)] = o3;
+        zo[stride * a] = o0;
+        zo[stride * (a + 4)] = o1;
+        zo[stride * (a + 8)] = o2;
+        zo[stride * (a + 12)] = o3;
     }
+}
+
+// Column DFT-16 of both stations: reads the transposed Y', writes Z/2.
+__device__ __forceinline__ void mf_columns_dft16(float2 *slot, int l) {
+    const int s = l >> 5, k1 = l & 31;
+    float2 x[16];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const float4 q = *reinterpret_cast<const float4 *>(slot + mf_tidx(s, k1, j));
+        x[2 * j] = make_float2(q.x, q.y);
+        x[2 * j + 1] = make_float2(q.z, q.w);
+    }
+    wave_lds_sync();   // every lane's column is read before Z overwrites the slot
+    mf_dft16_out(x, slot + s * mf::M + k1, 32);
 }
 
 // Real-spectrum bins p and M-p from Z/2 (realfft's split method, paired):
@@ -814,8 +828,8 @@ __device__ __forceinline__ void mf_mac(const float2 *z1, const float2 *z2, const
 // lane's FP64 sums; the lanes' sums are added at the end of the kernel.
 __device__ __forceinline__ void mf_mid(const MfFrame &pf, float2 u1, float2 u2, bool do_acf,
                                        bool do_xcf, bool in11, bool in22, bool in_ov,
-                                       double (&macc)[4]) {
-    const float r = (float)wrap_turns(fma((double)(mf::M / 2), pf.dth, pf.phw));
+                                       double (&macc)[4], int kmid = mf::M / 2) {
+    const float r = (float)wrap_turns(fma((double)kmid, pf.dth, pf.phw));
     const float2 P = make_float2(__builtin_amdgcn_cosf(r), __builtin_amdgcn_sinf(r));
     const float2 x1 = make_float2(2.0f * u1.x, -2.0f * u1.y);
     const float2 x2 = make_float2(2.0f * u2.x, -2.0f * u2.y);
@@ -997,9 +1011,407 @@ __global__ __launch_bounds__(mf::THREADS3) void k_sector_mf3(SectorArgs a) {
     }
 }
 
+// ------------------------------------------- N = 2048: k_sector_mf4
+// The producer/consumer organization for M = 1024 = 32 x 32 (m = 32 m1 + m2,
+// k = k1 + 32 k2).  Eight producers, each one station of one of four frame
+// streams (producer p: stream p % 4, station p / 4), so that a producer's
+// slot holds one station-frame (8 KB), as at N = 1024:
+//   stage 1: the DFT-32 over m1 as FP16 MFMA GEMMs on two 16-row tiles
+//            (m2 = 16 rt + l % 16), each B operand read feeding both tiles;
+//   twiddle: W1024^{m2 k1}, stored transposed (column k1: the 32 values m2
+//            in 256 bytes, 16-byte pairs XOR-swizzled by k1 & 7);
+//   stage 2: lanes l and l + 32 share column k1 = l % 32; each reads the
+//            whole column and forms its half h2 = l / 32 of the DFT-32 by
+//            decimation in frequency (x[t] + x[t+16], or (x[t] - x[t+16])
+//            W32^t), then that half's DFT-16 in registers, giving
+//            Z/2[k1 + 32 (2 k' + h2)].
+// Four consumers split the bins: consumer c takes the pairs (p, M - p),
+// p = l + 64 j for j = 2c and 2c + 1, of every frame of every stream (32
+// FP64 accumulator registers each); consumer 0 also bin M/2.  A slot is
+// released when all four consumers have read it: cons[p] counts four per
+// frame.
+namespace mf4 {
+constexpr int N = 2048, M = 1024, R2 = 32;
+constexpr int STREAMS = 4, PROD = 8, CONS = 4;
+constexpr int THREADS = 64 * (PROD + CONS);
+constexpr int SROW = N + 2 * mf::PAD;
+}  // namespace mf4
+
+struct MfShared4 {
+    uint4 b1[16 * 64];
+    float tw2[32 * 64];
+    float twu[16 * 64];
+    uint32_t slut[2][4 * 256];
+    uint16_t dlut[2][MFT_DLUT_HALVES];
+    uint16_t stage[mf4::PROD][mf4::SROW];      // each producer's station row
+    alignas(16) float2 z[mf4::PROD][mf4::M];   // one station-frame per producer
+    float2 mid_u[mf4::STREAMS][mf::FB][2];     // consumer 0: Z'[M/2] per stream, batch frame, station
+    int32_t prod[mf4::PROD], cons[mf4::PROD];
+    int32_t broken;
+};
+
+__device__ __forceinline__ int mf4_tidx(int k1, int pair) { return k1 * 32 + 2 * (pair ^ (k1 & 7)); }
+
+// W1024^{m2 k1} for rows m2 = 16 rt + 4g + r of column half h.
+__device__ __forceinline__ void mf4_load_tw(const float *tw2, int rt, int h, int l, float2 tw[4]) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int idx = ((rt * 2 + h) * 4 + r) * 2;
+        tw[r] = make_float2(tw2[idx * 64 + l], tw2[(idx + 1) * 64 + l]);
+    }
+}
+
+__device__ __forceinline__ void mf4_twiddle_store(f4 re, f4 im, const float2 tw[4], float2 *slot,
+                                                  int rt, int h, int l) {
+    float2 v[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) v[r] = c_mul(make_float2(re[r], im[r]), tw[r]);
+    const int k1 = (l & 15) + 16 * h, pair = 2 * (l >> 4) + 8 * rt;   // m2 = 16 rt + 4g + r
+    *reinterpret_cast<float4 *>(slot + mf4_tidx(k1, pair)) = make_float4(v[0].x, v[0].y, v[1].x, v[1].y);
+    *reinterpret_cast<float4 *>(slot + mf4_tidx(k1, pair + 1)) = make_float4(v[2].x, v[2].y, v[3].x, v[3].y);
+}
+
+// cos(pi t / 16), t = 0..8.
+__device__ constexpr float kCos16[9] = {1.0f, 0.98078528040323044913f, 0.92387953251128675613f,
+                                        0.83146961230254523708f, 0.70710678118654752440f,
+                                        0.55557023301960222474f, 0.38268343236508977173f,
+                                        0.19509032201612826785f, 0.0f};
+
+// x * W32^t, W32 = exp(-2 pi i / 32), t = 1..15.
+template <int t>
+__device__ __forceinline__ float2 mf_w32(float2 x) {
+    if constexpr (t == 8) {
+        return make_float2(x.y, -x.x);
+    } else {
+        constexpr float wr = t <= 8 ? kCos16[t] : -kCos16[16 - t];
+        constexpr float wi = t <= 8 ? -kCos16[8 - t] : -kCos16[t - 8];
+        return make_float2(x.x * wr - x.y * wi, x.x * wi + x.y * wr);
+    }
+}
+
+template <int t>
+__device__ __forceinline__ void mf_w32_step(float2 (&d)[16]) {
+    if constexpr (t < 16) {
+        d[t] = mf_w32<t>(d[t]);
+        mf_w32_step<t + 1>(d);
+    }
+}
+
+// Stage 2 of one station-frame (M = 1024) from the transposed slot.
+__device__ __forceinline__ void mf4_columns(float2 *slot, int l) {
+    const int k1 = l & 31, h2 = l >> 5;
+    const float sgn = h2 ? -1.0f : 1.0f;
+    float2 d[16];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {   // pairs j (m2 = 2j, 2j + 1) and j + 8 (m2 + 16)
+        const float4 qa = *reinterpret_cast<const float4 *>(slot + mf4_tidx(k1, j));
+        const float4 qb = *reinterpret_cast<const float4 *>(slot + mf4_tidx(k1, j + 8));
+        d[2 * j] = make_float2(fmaf(sgn, qb.x, qa.x), fmaf(sgn, qb.y, qa.y));
+        d[2 * j + 1] = make_float2(fmaf(sgn, qb.z, qa.z), fmaf(sgn, qb.w, qa.w));
+    }
+    wave_lds_sync();   // every lane's column is read before Z overwrites the slot
+    if (h2) mf_w32_step<1>(d);
+    mf_dft16_out(d, slot + k1 + 32 * h2, 64);
+}
+
+__device__ void mf4_producer(const SectorArgs &a, MfShared4 &sh, int p, int l, int64_t f0, int64_t f1) {
+    const int st = p / mf4::STREAMS;
+    const int bits = a.st[st].bits, kind = a.st[st].shuffle_kind, nw = a.st[st].nwords;
+    const int64_t bpf = a.st[st].bpf;
+    const uint8_t *raw = st ? a.raw2 : a.raw1;
+    uint16_t *row = sh.stage[p];
+    float2 *slot = sh.z[p];
+    uint32_t cur[4] = {0, 0, 0, 0};
+    bool ok = true;
+    if (f0 < f1) {
+        mf_load(raw + f0 * bpf, nw, l, cur);
+        mf_stage(cur, bits, kind, nw, sh.slut[st], sh.dlut[st], row + mf::PAD, l);
+        if (f0 + 1 < f1) mf_load(raw + (f0 + 1) * bpf, nw, l, cur);
+        wave_lds_sync();
+    }
+    for (int64_t fb = f0; fb < f1; fb += mf::FB) {
+        const int nb = (int)((f1 - fb) < mf::FB ? (f1 - fb) : mf::FB);
+        MfFrame pf = {};
+        if (l < nb) pf = mf_frame(a.first_frame + fb + l, a);
+        for (int jf = 0; jf < nb; ++jf) {
+            const int64_t f = fb + jf;
+            const int32_t i = (int32_t)(f - f0);
+            uint32_t nxt[4] = {0, 0, 0, 0};
+            if (f + 2 < f1) mf_load(raw + (f + 2) * bpf, nw, l, nxt);
+            const int sh_int = __builtin_amdgcn_readlane(st ? pf.s2 : pf.s1, jf);
+            half8 a00, a01, a10, a11;   // row tiles 0 and 1
+            mf_build_a_t<mf4::R2, mf4::SROW>(row, sh_int, l, 0, a00, a01);
+            mf_build_a_t<mf4::R2, mf4::SROW>(row, sh_int, l, 1, a10, a11);
+#pragma unroll 1
+            for (int h = 0; h < 2; ++h) {
+                f4 y0[2], y1[2];   // tiles h (re) and 2 + h (im) of row tiles 0 and 1
+#pragma unroll
+                for (int q = 0; q < 2; ++q) {
+                    const int t = h + 2 * q;
+                    const half8 bh0 = ld_b1(sh.b1, t * 2 + 0, l), bl0 = ld_b1(sh.b1, t * 2 + 1, l);
+                    const half8 bh1 = ld_b1(sh.b1, (4 + t) * 2 + 0, l), bl1 = ld_b1(sh.b1, (4 + t) * 2 + 1, l);
+                    y0[q] = mf_stage1(a00, a01, bh0, bl0, bh1, bl1);
+                    y1[q] = mf_stage1(a10, a11, bh0, bl0, bh1, bl1);
+                }
+                float2 tw0[4], tw1[4];
+                mf4_load_tw(sh.tw2, 0, h, l, tw0);
+                mf4_load_tw(sh.tw2, 1, h, l, tw1);
+                // the slot was last read by the four consumers for frame i - 1
+                if (h == 0) mf_wait_ge(&sh.cons[p], mf4::CONS * i, ok, &sh.broken);
+                mf4_twiddle_store(y0[0], y0[1], tw0, slot, 0, h, l);
+                mf4_twiddle_store(y1[0], y1[1], tw1, slot, 1, h, l);
+            }
+            wave_lds_sync();
+            mf4_columns(slot, l);
+            mf_publish(&sh.prod[p], i + 1);
+            if (f + 1 < f1) {   // decode frame f+1 (the staging row is this wave's own)
+                wave_lds_sync();
+                mf_stage(cur, bits, kind, nw, sh.slut[st], sh.dlut[st], row + mf::PAD, l);
+            }
+            wave_lds_sync();
+#pragma unroll
+            for (int u = 0; u < 4; ++u) cur[u] = nxt[u];
+        }
+    }
+}
+
+// Consumer c's two bin pairs of one frame (slots z1, z2: the two stations).
+// Local slot s: j = 2c + s/2, p = l + 64 j, bin p (s even) or M - p (odd).
+template <bool FULL>
+__device__ __forceinline__ void mf4_mac(const float2 *z1, const float2 *z2, const float *twu,
+                                        const MfFrame &pf, int jf, int c, int l, bool do_acf,
+                                        bool do_xcf, uint32_t m11, uint32_t m22, uint32_t m_ov,
+                                        float k11, float k22, float kM, double (&acc)[4][4]) {
+    constexpr int M = mf4::M;
+    const double phw = readlane_d(pf.phw, jf);
+    const double dth = readlane_d(pf.dth, jf);
+    const float2 F1 = make_float2(readlane_f(pf.F1.x, jf), readlane_f(pf.F1.y, jf));
+    const float2 Cm = make_float2(readlane_f(pf.C.x, jf), readlane_f(pf.C.y, jf));
+    const float r0 = (float)wrap_turns(fma((double)(l + 128 * c), dth, phw));
+    float2 P = make_float2(__builtin_amdgcn_cosf(r0), __builtin_amdgcn_sinf(r0));
+#pragma unroll
+    for (int jj = 0; jj < 2; ++jj) {
+        const int j = 2 * c + jj;
+        const int p = l + 64 * j;
+        const int pm = (M - p) & (M - 1);
+        const float2 w = make_float2(twu[(2 * j) * 64 + l], twu[(2 * j + 1) * 64 + l]);
+        float2 x1p, x1m, x2p, x2m;
+        untangle_pair(z1[p], z1[pm], w, x1p, x1m);
+        untangle_pair(z2[p], z2[pm], w, x2p, x2m);
+        const int sp = 2 * jj, sm = 2 * jj + 1;
+        const bool j0 = jj == 0 && c == 0;   // lane 0's slot 1 is bin M
+        if (do_acf) {
+            if (FULL) {
+                acc[sp][0] += (double)norm_sqr(x1p);
+                acc[sp][1] += (double)norm_sqr(x2p);
+                acc[sm][0] += (double)(j0 ? k11 * norm_sqr(x1m) : norm_sqr(x1m));
+                acc[sm][1] += (double)(j0 ? k22 * norm_sqr(x2m) : norm_sqr(x2m));
+            } else {
+                acc[sp][0] += (double)(((m11 >> sp) & 1u) ? norm_sqr(x1p) : 0.0f);
+                acc[sp][1] += (double)(((m22 >> sp) & 1u) ? norm_sqr(x2p) : 0.0f);
+                acc[sm][0] += (double)(((m11 >> sm) & 1u) ? norm_sqr(x1m) : 0.0f);
+                acc[sm][1] += (double)(((m22 >> sm) & 1u) ? norm_sqr(x2m) : 0.0f);
+            }
+        }
+        if (do_xcf) {
+            float2 Pp, Pm;
+            if (FULL) {
+                Pp = P;
+                Pm = c_mul(Cm, c_conj(P));
+                if (j0) Pm = make_float2(kM * Pm.x, kM * Pm.y);
+            } else {
+                Pp = ((m_ov >> sp) & 1u) ? P : make_float2(0.f, 0.f);
+                Pm = ((m_ov >> sm) & 1u) ? c_mul(Cm, c_conj(P)) : make_float2(0.f, 0.f);
+            }
+            const float2 vp = c_mul(c_mul(x1p, c_conj(x2p)), Pp);
+            const float2 vm = c_mul(c_mul(x1m, c_conj(x2m)), Pm);
+            acc[sp][2] += (double)vp.x;
+            acc[sp][3] += (double)vp.y;
+            acc[sm][2] += (double)vm.x;
+            acc[sm][3] += (double)vm.y;
+        }
+        P = c_mul(P, F1);
+    }
+}
+
+__device__ __forceinline__ int mf4_bin(int c, int s, int l) {
+    const int p = l + 64 * (2 * c + (s >> 1));
+    return (s & 1) ? mf4::M - p : p;
+}
+
+// One lane of the wave adds 1 to *flag (release: this wave's reads of the
+// slot are complete).
+__device__ __forceinline__ void mf_release_add(int32_t *flag, int l) {
+    if (l == 0) __hip_atomic_fetch_add(flag, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+template <bool FULL>
+__device__ void mf4_consumer(const SectorArgs &a, MfShared4 &sh, int c, int l, const int64_t (&fs0)[4],
+                             const int64_t (&fs1)[4], double (&acc)[4][4], double (&macc)[4]) {
+    const bool do_acf = (a.flags & FLAG_ACF) != 0;
+    const bool do_xcf = (a.flags & FLAG_XCF) != 0;
+    const bool ov_only = (a.flags & FLAG_ACF_OVERLAP) != 0;
+    const int gs = a.a1s, ge = a.a1s + a.ov;   // a1s == a2s on this path
+    uint32_t m_grid = 0, m_ov = 0;
+#pragma unroll
+    for (int s = 0; s < 5; ++s) {   // local slots 0..3, bit 4: bin M/2
+        const int k = s == 4 ? mf4::M / 2 : mf4_bin(c, s, l);
+        const bool in_ov = k >= gs && k < ge;
+        if (!ov_only || in_ov) m_grid |= 1u << s;
+        if (in_ov) m_ov |= 1u << s;
+    }
+    const uint32_t m11 = a.grid == 0 ? m_grid : m_ov;
+    const uint32_t m22 = a.grid == 0 ? m_ov : m_grid;
+    const float kM = (l == 0) ? 0.0f : 1.0f;   // FULL: bin M (lane 0, consumer 0, slot 1)
+    const float k11 = a.grid == 0 ? 1.0f : kM, k22 = a.grid == 0 ? kM : 1.0f;
+    const bool in11 = (m11 >> 4) & 1u, in22 = (m22 >> 4) & 1u, in_ovm = (m_ov >> 4) & 1u;
+    int64_t len[4], n = 0;
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        len[s] = fs1[s] - fs0[s];
+        n = len[s] > n ? len[s] : n;
+    }
+    bool ok = true;
+    for (int64_t g = 0; g < n; g += mf::FB) {
+        const int cnt = (int)((n - g) < mf::FB ? (n - g) : mf::FB);
+        MfFrame pf[4];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            pf[s] = MfFrame{};
+            if (l < cnt && g + l < len[s]) pf[s] = mf_frame(a.first_frame + fs0[s] + g + l, a);
+        }
+        for (int jf = 0; jf < cnt; ++jf) {
+            const int32_t i = (int32_t)(g + jf);
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                if (i >= len[s]) continue;
+                mf_wait_ge(&sh.prod[s], i + 1, ok, &sh.broken);
+                mf_wait_ge(&sh.prod[s + 4], i + 1, ok, &sh.broken);
+                mf4_mac<FULL>(sh.z[s], sh.z[s + 4], sh.twu, pf[s], jf, c, l, do_acf, do_xcf, m11, m22,
+                              m_ov, k11, k22, kM, acc);
+                if (c == 0 && l == 0) {   // bin M/2 of this frame, formed after the batch
+                    sh.mid_u[s][jf][0] = sh.z[s][mf4::M / 2];
+                    sh.mid_u[s][jf][1] = sh.z[s + 4][mf4::M / 2];
+                }
+                mf_release_add(&sh.cons[s], l);       // this wave's reads of both slots are done
+                mf_release_add(&sh.cons[s + 4], l);
+            }
+        }
+        if (c == 0) {
+            wave_lds_sync();   // lane 0's stores of the batch before the lanes' loads
+#pragma unroll
+            for (int s = 0; s < 4; ++s)
+                if (l < cnt && g + l < len[s])
+                    mf_mid(pf[s], sh.mid_u[s][l][0], sh.mid_u[s][l][1], do_acf, do_xcf, FULL || in11,
+                           FULL || in22, FULL || in_ovm, macc, mf4::M / 2);
+            wave_lds_sync();   // and those loads before the next batch's stores
+        }
+    }
+}
+
+__global__ __launch_bounds__(mf4::THREADS) void k_sector_mf4(SectorArgs a) {
+    __shared__ MfShared4 sh;
+    constexpr int M = mf4::M;
+    constexpr int NT = mf4::THREADS;
+    const int tid = threadIdx.x;
+    const int l = tid & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const uint8_t *tab = reinterpret_cast<const uint8_t *>(a.mf_tab);
+    {
+        const uint4 *g = reinterpret_cast<const uint4 *>(tab + MFT4_B1);
+        for (int i = tid; i < 16 * 64; i += NT) sh.b1[i] = g[i];
+        const uint32_t *gs = reinterpret_cast<const uint32_t *>(tab + MFT4_SLUT);
+        for (int i = tid; i < 2 * 4 * 256; i += NT) (&sh.slut[0][0])[i] = gs[i];
+        const uint32_t *gd = reinterpret_cast<const uint32_t *>(tab + MFT4_DLUT);
+        for (int i = tid; i < 2 * MFT_DLUT_HALVES / 2; i += NT)
+            reinterpret_cast<uint32_t *>(&sh.dlut[0][0])[i] = gd[i];
+        uint32_t *st32 = reinterpret_cast<uint32_t *>(&sh.stage[0][0]);
+        for (int i = tid; i < mf4::PROD * mf4::SROW / 2; i += NT) st32[i] = 0u;
+        const float *gt = reinterpret_cast<const float *>(tab + MFT4_TW2);
+        const float *gu = reinterpret_cast<const float *>(tab + MFT4_TWU);
+        for (int i = tid; i < 32 * 64; i += NT) sh.tw2[i] = gt[i];
+        for (int i = tid; i < 16 * 64; i += NT) sh.twu[i] = gu[i];
+        if (tid < mf4::PROD) {
+            sh.prod[tid] = 0;
+            sh.cons[tid] = 0;
+        }
+        if (tid == 0) sh.broken = 0;
+    }
+    __syncthreads();
+
+    // frames of stream s: a contiguous quarter of the workgroup's range
+    const int64_t wg0 = (int64_t)blockIdx.x * a.frames_per_wg;
+    int64_t wg1 = wg0 + a.frames_per_wg;
+    if (wg1 > a.nf) wg1 = a.nf;
+    const int64_t per = (a.frames_per_wg + mf4::STREAMS - 1) / mf4::STREAMS;
+    auto range = [&](int s, int64_t &r0, int64_t &r1) {
+        r0 = wg0 + per * s;
+        if (r0 > wg1) r0 = wg1;
+        r1 = (r0 + per < wg1) ? r0 + per : wg1;
+    };
+
+    double acc[4][4], macc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[s][c] = 0.0;
+    if (wv < mf4::PROD) {
+        int64_t f0, f1;
+        range(wv % mf4::STREAMS, f0, f1);
+        mf4_producer(a, sh, wv, l, f0, f1);
+    } else {
+        const int c = wv - mf4::PROD;
+        int64_t fs0[4], fs1[4];
+        for (int s = 0; s < 4; ++s) range(s, fs0[s], fs1[s]);
+        const bool full = (a.flags & FLAG_ACF_OVERLAP) == 0 && a.a1s == 0 && a.ov == M;
+        if (full) mf4_consumer<true>(a, sh, c, l, fs0, fs1, acc, macc);
+        else mf4_consumer<false>(a, sh, c, l, fs0, fs1, acc, macc);
+    }
+
+    // ---- workgroup partial: every bin belongs to one consumer lane
+    __syncthreads();
+    constexpr int H = M + 1;
+    double *red = reinterpret_cast<double *>(&sh.z[0][0]);    // [4][H]
+    double *mids = reinterpret_cast<double *>(&sh.z[5][0]);   // [4][64] consumer 0's lane sums
+    if (wv >= mf4::PROD) {
+        const int c = wv - mf4::PROD;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const int k = mf4_bin(c, s, l);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) red[q * H + k] = acc[s][q];
+        }
+        if (c == 0)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) mids[q * 64 + l] = macc[q];
+    }
+    __syncthreads();
+    if (tid == 0)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            double sum = 0.0;
+            for (int i = 0; i < 64; ++i) sum += mids[q * 64 + i];   // lanes in order
+            red[q * H + M / 2] = sum;
+        }
+    __syncthreads();
+    double *out = a.part + (size_t)blockIdx.x * kNumProducts * H;
+    const double bad = sh.broken ? __builtin_nan("") : 0.0;   // NaN marks a failed hand-over
+    for (int k = tid; k < H; k += NT) {
+        out[P_PH * H + k] = bad;
+        out[P_11 * H + k] = red[0 * H + k] + bad;
+        out[P_22 * H + k] = red[1 * H + k] + bad;
+        out[P_12R * H + k] = red[2 * H + k] + bad;
+        out[P_12I * H + k] = red[3 * H + k] + bad;
+    }
+}
+
 hipError_t launch_sector_mfma(const SectorArgs &a, int precision, int n_wg, hipStream_t s) {
-    if (a.N != mf::N || a.mf_tab == nullptr || precision != 1 || a.a1s != a.a2s)
+    if ((a.N != mf::N && a.N != mf4::N) || a.mf_tab == nullptr || precision != 1 || a.a1s != a.a2s)
         return hipErrorInvalidValue;
+    if (a.N == mf4::N) {   // one organization at N = 2048
+        hipLaunchKernelGGL(k_sector_mf4, dim3(n_wg), dim3(mf4::THREADS), 0, s, a);
+        return hipGetLastError();
+    }
     // variant 1: producer/consumer waves (autotuned against the default)
     if (a.variant == 1) hipLaunchKernelGGL(k_sector_mf3, dim3(n_wg), dim3(mf::THREADS3), 0, s, a);
     else hipLaunchKernelGGL(k_sector_mf2, dim3(n_wg), dim3(512), 0, s, a);

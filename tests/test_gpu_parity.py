@@ -286,7 +286,15 @@ def test_mfma_eligibility():
     lv2 = [LEVELS_2BIT, LEVELS_2BIT]
     assert F.Plan(1024, levels=lv2, df_hz=1.0, precision=F.PRECISION_CHOSEN).info.fft_form == 1
     assert F.Plan(1024, levels=lv2, df_hz=1.0).info.fft_form == 0          # plain: FP32 FFT
-    assert F.Plan(2048, levels=lv2, df_hz=1.0, precision=F.PRECISION_CHOSEN).info.fft_form == 0
+    assert F.Plan(2048, levels=lv2, df_hz=1.0, precision=F.PRECISION_CHOSEN).info.fft_form == 1
+    assert F.Plan(4096, levels=lv2, df_hz=1.0, precision=F.PRECISION_CHOSEN).info.fft_form == 0
+    lv8 = [np.arange(256, dtype=np.float64) - 127.5] * 2
+    assert F.Plan(1024, bits=(8, 8), levels=lv8, df_hz=1.0, precision=F.PRECISION_CHOSEN).info.fft_form == 1
+    # a 2048-sample 8-bit frame is 512 words, past the staging of one frame
+    assert F.Plan(2048, bits=(8, 8), levels=lv8, df_hz=1.0, precision=F.PRECISION_CHOSEN).info.fft_form == 0
+    p2 = F.Plan(2048, levels=lv2, df_hz=1.0, precision=F.PRECISION_CHOSEN)
+    with pytest.raises(F.FxcError):
+        F.plan_set_tuning(p2, variant=0)            # no whole-frame organization at N = 2048
     odd = [np.array([-1.1, -0.3, 0.3, 1.1])] * 2                           # not exact in FP16
     assert F.Plan(1024, levels=odd, df_hz=1.0, precision=F.PRECISION_CHOSEN).info.fft_form == 0
 
@@ -397,6 +405,70 @@ def test_mfma_kernel_organizations_agree(nf):
     o64 = oracle_run(c, R.MODE_F64)
     for r in res:
         _compare(r, o64, CHOSEN_TOL)
+
+
+# N = 2048: 32 x 32 four-step (one station-frame per producer slot, column
+# DFT-32 split over lane pairs); producer/consumer organization only.
+@pytest.mark.parametrize("bits", [(4, 1), (1, 2), (2, 4), (2, 2)])
+def test_mfma2048_sector_mixed_bits(bits):
+    n, fs = 2048, 512e6
+    rng = np.random.default_rng(100 + sum(bits))
+    c = make_case(n=n, fs=fs, nf=1700, bits=bits, shuffles=(rng.permutation(32), None),
+                  lsb=(True, False), rho=0.5, delay_samples=-3.4, seed=23,
+                  delay_user_samples=-3.4, extra_delay_rate_sps=2e-9, lo1_hz=8.4e9,
+                  lo2_hz=8.4e9)
+    p = gpu_plan(c, F.PRECISION_CHOSEN)
+    assert p.info.fft_form == 1 and p.info.variant == 1
+    _compare(gpu_run(c, plan=p), oracle_run(c, R.MODE_F64), CHOSEN_TOL)
+
+
+@pytest.mark.parametrize("grid", [0, 1])
+@pytest.mark.parametrize("overlap_only", [False, True])
+def test_mfma2048_partial_band_grids(grid, overlap_only):
+    n, fs = 2048, 256e6
+    prods = F.PRODUCT_ACF | F.PRODUCT_XCF
+    if overlap_only:
+        prods |= F.PRODUCT_ACF_OVERLAP_ONLY
+    c = make_case(n=n, fs=fs, nf=1200, band=(0, 200, 700, 200), grid=grid, products=prods,
+                  rho=0.5, delay_samples=2.6, seed=33, delay_user_samples=2.6, lo1_hz=8e9,
+                  lo2_hz=8e9)
+    p = gpu_plan(c, F.PRECISION_CHOSEN)
+    assert p.info.fft_form == 1
+    got = gpu_run(c, plan=p)
+    ref = oracle_run(c, R.MODE_F64)
+    _compare(got, ref, CHOSEN_TOL)
+    for k in ("acc11", "acc22", "acc12"):
+        assert zero_pattern_equal(got[k], ref[k]), k
+
+
+def test_mfma2048_large_and_odd_shifts():
+    n, fs = 2048, 128e6
+    c = make_case(n=n, fs=fs, nf=1000, no_table=True, net_d1_base=-17.2e-6,
+                  total_rate1_base=1.2e-3, lo1_hz=2e9, seed=7, rho=0.3)
+    d = R.make_delay_cfg(None, **c["delay"])
+    ints = [R.frame_delay(i, d).int1 for i in range(c["nf"])]
+    assert min(ints) < -n and max(ints) > 64 and any(i % 2 for i in ints)
+    p = gpu_plan(c, F.PRECISION_CHOSEN)
+    assert p.info.fft_form == 1
+    _compare(gpu_run(c, plan=p), oracle_run(c, R.MODE_F64), CHOSEN_TOL)
+
+
+@pytest.mark.parametrize("nf", [1, 3, 5, 37, 2051])
+def test_mfma2048_ragged_frame_counts(nf):
+    """Frame counts that leave producer slots, streams or whole workgroups
+    without frames; the MFMA kernel against the FP32 Stockham kernel (the
+    plain form) and both against the FP64 oracle."""
+    n, fs = 2048, 1024e6
+    rng = np.random.default_rng(nf)
+    c = make_case(n=n, fs=fs, nf=nf, shuffles=(None, rng.permutation(32)), lsb=(False, True),
+                  rho=0.4, delay_samples=1.3, seed=50 + nf, delay_user_samples=1.3,
+                  extra_delay_rate_sps=1e-9, lo1_hz=6.7e9, lo2_hz=6.7e9)
+    p = gpu_plan(c, F.PRECISION_CHOSEN)
+    assert p.info.fft_form == 1
+    got = gpu_run(c, plan=p)
+    o64 = oracle_run(c, R.MODE_F64)
+    _compare(got, o64, CHOSEN_TOL)
+    _check_plain(c, gpu_run(c, plan=gpu_plan(c, F.PRECISION_PLAIN_FP32)))
 
 
 def test_mfma_needs_common_band_start():
