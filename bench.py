@@ -255,6 +255,13 @@ def load_traffic(key):
 FORM_NAMES = {0: "FP32 Stockham", 1: "FP16-MFMA four-step", 2: "FP32 four-step through HBM"}
 
 
+def form_name(info):
+    """The FFT a plan runs (fxc_plan_info.fft_form / fft_mfma)."""
+    if info.fft_form == 2 and info.fft_mfma:
+        return "four-step through HBM, FP16-MFMA DFT-32 first pass"
+    return FORM_NAMES[info.fft_form]
+
+
 def size_sweep(stream, samples_per_station=1 << 28, reps=5):
     """Throughput of both forms at FFT sizes 256..65536: two 2-bit stations at
     1024 Msps, device-resident synthetic input (station B LSB), the plans'
@@ -282,7 +289,7 @@ def size_sweep(stream, samples_per_station=1 << 28, reps=5):
             stream.sync()
             el = (time.perf_counter() - t0) / reps
             row[name] = round(nf * n / el / 1e6, 1)
-            row[name + "_fft"] = FORM_NAMES[p.info.fft_form]
+            row[name + "_fft"] = form_name(p.info)
             acc.free()
             p.close()
         d1.free()

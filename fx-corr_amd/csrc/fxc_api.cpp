@@ -148,8 +148,9 @@ struct fxc_plan {
     int32_t decode_ok[2] = {1, 1};
     // device tables
     DevBuf tw_fft, tw_post, lev_dev[2], out_lev, mf_tab;
-    int fft_form = 0;  // 0: FP32 Stockham, 1: FP16/FP32-MFMA four-step (N = 1024),
-                       // 2: FP32 four-step through HBM scratch (N = 32768, 65536)
+    int fft_form = 0;  // 0: FP32 Stockham, 1: FP16/FP32-MFMA four-step (N = 1024, 2048),
+                       // 2: four-step through HBM scratch (N = 32768, 65536)
+    bool large_mfma = false;   // fft_form 2: first pass's DFT-32 on FP16 MFMA (mf_tab)
     // large-N scratch (fft_form 2): chunk of `chunk` frames
     int M1 = 0, M2 = 0;
     int64_t chunk = 0;       // allocated chunk (frames)
@@ -516,6 +517,17 @@ static float f16_bits_to_f32(uint16_t h) {
 // level (as the f32 the decode produces, utils.rs:186) is exact in FP16, and
 // a station-frame is at most 256 packed words (1/2/4/8-bit at N = 1024,
 // 1/2/4-bit at N = 2048).
+// Every level of both stations exact in FP16: the decoded samples then feed
+// an FP16 MFMA operand without rounding.
+static bool levels_fp16_exact(const fxc_plan *p) {
+    for (int i = 0; i < 2; ++i)
+        for (double lv : p->levels[i]) {
+            const float f = (float)lv;
+            if (!std::isfinite(f) || f16_bits_to_f32(f32_to_f16_bits(f)) != f) return false;
+        }
+    return true;
+}
+
 static bool mfma_eligible(const fxc_plan *p) {
     if ((p->N != 1024 && p->N != 2048) || p->desc.precision != FXC_PRECISION_CHOSEN) return false;
     if (p->desc.products & (FXC_PRODUCT_PHASED | FXC_PRODUCT_RAW_OUT)) return false;
@@ -524,12 +536,16 @@ static bool mfma_eligible(const fxc_plan *p) {
         const int b = p->desc.st[i].bits;
         if (b != 1 && b != 2 && b != 4 && b != 8) return false;
         if (p->N * b / 32 > 256) return false;
-        for (double lv : p->levels[i]) {
-            const float f = (float)lv;
-            if (!std::isfinite(f) || f16_bits_to_f32(f32_to_f16_bits(f)) != f) return false;
-        }
     }
-    return true;
+    return levels_fp16_exact(p);
+}
+
+// N >= 32768 (HBM four-step): the chosen form runs the first pass's DFT-32
+// on FP16 MFMA when the levels are exact in FP16 (any band alignment).
+static bool large_mfma_eligible(const fxc_plan *p) {
+    if (p->desc.precision != FXC_PRECISION_CHOSEN) return false;
+    if (p->desc.products & (FXC_PRODUCT_PHASED | FXC_PRODUCT_RAW_OUT)) return false;
+    return levels_fp16_exact(p);
 }
 
 // Tables of the MFMA four-step kernel (fxc_mfma.hip), MFT_* layout, all in
@@ -550,14 +566,10 @@ static bool mfma_eligible(const fxc_plan *p) {
 //  DLUT per station: FP16 samples of each byte value (code = low bits
 //     first, odd samples negated for LSB stations); 8-bit keeps an odd-byte
 //     copy at +256.  SLUT per station: generic shuffle as 4 byte tables.
-static std::vector<uint8_t> mfma_tables(const fxc_plan *p) {
-    const bool wide = p->N == 2048;
-    const size_t o_b1 = wide ? MFT4_B1 : MFT_B1, o_dlut = wide ? MFT4_DLUT : MFT_DLUT;
-    const size_t o_slut = wide ? MFT4_SLUT : MFT_SLUT;
-    std::vector<uint8_t> tab(wide ? MFT4_BYTES : MFT_BYTES, 0);
+// The DFT-32 B operands (scale * DFT-32 as FP16 hi + lo) at tab[o_b1...],
+// 16 x 64 half8 in the MFT_B1 layout.
+static void put_dft32_b1(std::vector<uint8_t> &tab, size_t o_b1, double scale) {
     auto put16 = [&](size_t off, uint16_t v) { memcpy(&tab[off], &v, 2); };
-    auto putf = [&](size_t off, float v) { memcpy(&tab[off], &v, 4); };
-    auto put32 = [&](size_t off, uint32_t v) { memcpy(&tab[off], &v, 4); };
     for (int kap = 0; kap < 2; ++kap)
         for (int t = 0; t < 4; ++t)
             for (int l = 0; l < 64; ++l)
@@ -575,13 +587,24 @@ static std::vector<uint8_t> mfma_tables(const fxc_plan *p) {
                         cs = std::cos(2.0 * M_PI * r / 32.0);
                         sn = std::sin(2.0 * M_PI * r / 32.0);
                     }
-                    const double v = 0.5 * (part == 0 ? (e == 0 ? cs : sn) : (e == 0 ? -sn : cs));
+                    const double v = scale * (part == 0 ? (e == 0 ? cs : sn) : (e == 0 ? -sn : cs));
                     const uint16_t hi = f32_to_f16_bits((float)v);
                     const uint16_t lo = f32_to_f16_bits((float)(v - (double)f16_bits_to_f32(hi)));
                     const size_t idx_hi = (size_t)((kap * 4 + t) * 2 + 0), idx_lo = idx_hi + 1;
                     put16(o_b1 + ((idx_hi * 64 + l) * 8 + s) * 2, hi);
                     put16(o_b1 + ((idx_lo * 64 + l) * 8 + s) * 2, lo);
                 }
+}
+
+static std::vector<uint8_t> mfma_tables(const fxc_plan *p) {
+    const bool wide = p->N == 2048;
+    const size_t o_b1 = wide ? MFT4_B1 : MFT_B1, o_dlut = wide ? MFT4_DLUT : MFT_DLUT;
+    const size_t o_slut = wide ? MFT4_SLUT : MFT_SLUT;
+    std::vector<uint8_t> tab(wide ? MFT4_BYTES : MFT_BYTES, 0);
+    auto put16 = [&](size_t off, uint16_t v) { memcpy(&tab[off], &v, 2); };
+    auto putf = [&](size_t off, float v) { memcpy(&tab[off], &v, 4); };
+    auto put32 = [&](size_t off, uint32_t v) { memcpy(&tab[off], &v, 4); };
+    put_dft32_b1(tab, o_b1, 0.5);   // the realfft split's 1/2 folded in
     if (wide) {
         for (int rt = 0; rt < 2; ++rt)
             for (int h = 0; h < 2; ++h)
@@ -789,6 +812,14 @@ extern "C" fxc_status fxc_plan_create(const fxc_plan_desc *desc, fxc_plan **out)
             p->d_params.ensure(C * frame_param_bytes()) != hipSuccess ||
             p->d_lshifts.ensure(C * 16) != hipSuccess)
             return cleanup(fail(FXC_ERR_NOMEM, "large-FFT scratch allocation failed"));
+        if (large_mfma_eligible(p)) {
+            std::vector<uint8_t> tab(MFT_A2 - MFT_B1, 0);
+            put_dft32_b1(tab, 0, 1.0);
+            if (p->mf_tab.ensure(tab.size()) != hipSuccess ||
+                hipMemcpy(p->mf_tab.p, tab.data(), tab.size(), hipMemcpyHostToDevice) != hipSuccess)
+                return cleanup(fail(FXC_ERR_HIP, "MFMA table upload failed"));
+            p->large_mfma = true;
+        }
     }
     // grid: plain form about 16 waves per CU worth of workgroups (p->T
     // threads each); MFMA form one 8-wave workgroup per CU; large-N form one
@@ -899,6 +930,7 @@ extern "C" fxc_status fxc_plan_get_info(const fxc_plan *p, fxc_plan_info *info) 
     info->device_id = p->device;
     info->variant = p->variant;
     info->grid_divisor = p->grid_div;
+    info->fft_mfma = (p->fft_form == 1 || p->large_mfma) ? 1 : 0;
     return FXC_OK;
 }
 
@@ -1083,7 +1115,8 @@ static fxc_status enqueue_large(fxc_plan *p, const SectorArgs &base, int64_t nf,
             HIP_TRY(launch_decode_frames(a, st, shifts + st, 2, (float *)p->d_x[st].p, s));
             HIP_TRY(launch_fft_large((const float *)p->d_x[st].p, (float2 *)p->d_y.p,
                                      (float2 *)p->d_z[st].p, cn, p->M1, p->M2, twl1, twl2,
-                                     (const float2 *)p->tw_lm.p, s));
+                                     (const float2 *)p->tw_lm.p,
+                                     p->large_mfma ? p->mf_tab.p : nullptr, s));
         }
         if (chain) {
             if (span0 < 0 || c0 >= span0 + span) {
@@ -1504,7 +1537,8 @@ static fxc_status run_fft(fxc_plan *p, const float *in, int64_t nframes, float *
                                      (float2 *)p->d_z[0].p, cn, p->M1, p->M2,
                                      (const float2 *)(p->M1 == 128 ? p->tw_l1.p : p->tw_l2.p),
                                      (const float2 *)(p->M2 == 128 ? p->tw_l1.p : p->tw_l2.p),
-                                     (const float2 *)p->tw_lm.p, s));
+                                     (const float2 *)p->tw_lm.p,
+                                     p->large_mfma ? p->mf_tab.p : nullptr, s));
             HIP_TRY(launch_untangle_large((const float2 *)p->d_z[0].p, cn, p->M,
                                           (const float2 *)p->tw_post.p,
                                           (float2 *)p->d_tmp_out.p + c0 * p->half, s));

@@ -157,9 +157,13 @@ struct FrameParam;
 constexpr int kLargeGroups = 16;   // frame groups of the bin-owned product kernel
 hipError_t launch_frame_params(const SectorArgs &a, int precision, FrameParam *params,
                                int64_t *shifts, hipStream_t stream);
+// mf_b1: the FP16 hi/lo DFT-32 operand table for a first pass on MFMA
+// (chosen form, inputs exact in FP16), or null for the FP32 pass.
 hipError_t launch_fft_large(const float *x, float2 *ybuf, float2 *z, int64_t nframes, int M1,
                             int M2, const float2 *tw1, const float2 *tw2, const float2 *twm,
-                            hipStream_t stream);
+                            const void *mf_b1, hipStream_t stream);
+hipError_t launch_fft_pass1_mfma(const float2 *in, float2 *out, int64_t nbatch, int M1, int M2,
+                                 const void *b1, const float2 *tw_m, hipStream_t stream);
 hipError_t launch_untangle_large(const float2 *z, int64_t nframes, int M, const float2 *tw_post,
                                  float2 *out, hipStream_t stream);
 hipError_t launch_mac_large(const SectorArgs &a, int precision, const float2 *z1, const float2 *z2,

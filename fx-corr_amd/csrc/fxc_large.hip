@@ -114,13 +114,15 @@ static hipError_t launch_pass(int L, const float2 *in, float2 *out, int64_t nbat
 
 // Z (natural order, M complex per station-frame) of nframes frames from the
 // decoded samples x (N floats per frame): x read as z[m] = x[2m] + i x[2m+1].
+// With mf_b1 the first pass's DFT-32 runs on FP16 MFMA (fxc_mfma.hip).
 hipError_t launch_fft_large(const float *x, float2 *ybuf, float2 *z, int64_t nframes, int M1,
                             int M2, const float2 *tw1, const float2 *tw2, const float2 *twm,
-                            hipStream_t s) {
+                            const void *mf_b1, hipStream_t s) {
     const int M = M1 * M2;
     const float2 *zin = reinterpret_cast<const float2 *>(x);
     // columns c = m2 (M2 of them): input z[m2 + M2 m1], output Y[k1][m2] * W_M^{m2 k1}
-    hipError_t e = launch_pass(M1, zin, ybuf, nframes, M2, 1, M2, 1, M2, M, M, tw1, twm, M, 1, s);
+    hipError_t e = mf_b1 ? launch_fft_pass1_mfma(zin, ybuf, nframes, M1, M2, mf_b1, twm, s)
+                         : launch_pass(M1, zin, ybuf, nframes, M2, 1, M2, 1, M2, M, M, tw1, twm, M, 1, s);
     if (e != hipSuccess) return e;
     // rows c = k1 (M1 of them): input Y[k1][m2], output Z[k1 + M1 k2]
     return launch_pass(M2, ybuf, z, nframes, M1, M2, 1, 1, M1, M, M, tw2, twm, M, 0, s);

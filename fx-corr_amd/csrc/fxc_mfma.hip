@@ -1489,4 +1489,91 @@ hipError_t launch_rfft_mfma(const SectorArgs &a, const float *in, int64_t nframe
     return hipGetLastError();
 }
 
+// ------------------------------- N >= 32768: first pass of the HBM four-step
+// The column DFTs of length L = 32 R of the first pass (m = M2 m1 + m2, one
+// column per m2, L = M1), split m1 = v + R u, k1 = k' + 32 k'':
+//   A_v[k'] = sum_u W32^{u k'} x[v + R u]     FP16 hi + lo MFMA GEMM: rows =
+//                                              16 columns at one v, K = u
+//   Y[k' + 32 k''] = sum_v W_R^{v k''} (W_L^{v k'} A_v[k'])      FP32
+// and Y[k1] W_M^{m2 k1} stored at out[m2 + M2 k1], as the FP32 pass does.
+// Wave v forms A_v of the workgroup's 16 columns; the DFT-R runs after an
+// LDS exchange, one (column, k') pair per thread.  The inputs are decoded
+// levels, exact in FP16 (the plan checks this), so the conversion is exact.
+template <int R>
+__global__ __launch_bounds__(64 * R) void k_fft_pass1_mfma(const float2 *in, float2 *out,
+                                                           int64_t nbatch, int M2,
+                                                           const uint4 *b1g, const float2 *tw_m,
+                                                           int M) {
+    constexpr int KP = 33;   // padded k' row: lanes of consecutive columns hit distinct banks
+    __shared__ uint4 b1[16 * 64];
+    __shared__ float2 sb[R][16][KP];
+    const int tid = threadIdx.x;
+    const int l = tid & 63;
+    const int v = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int64_t blocks_per_frame = M2 / 16;
+    const int64_t f = blockIdx.x / blocks_per_frame;
+    const int c0 = (int)(blockIdx.x % blocks_per_frame) * 16;
+    if (f >= nbatch) return;   // uniform per workgroup
+    const float2 *src = in + f * (int64_t)M;
+    float2 *dst = out + f * (int64_t)M;
+    for (int i = tid; i < 16 * 64; i += 64 * R) b1[i] = b1g[i];
+    // A operand of K-step kap, lane l: row r = l % 16 (column c0 + r),
+    // slots (re, im) of u = 16 kap + g + 4 q, g = l / 16, q = 0..3
+    const int r = l & 15, g = l >> 4;
+    uint32_t d[8];
+#pragma unroll
+    for (int kap = 0; kap < 2; ++kap)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int u = 16 * kap + g + 4 * q;
+            const float2 x = src[(c0 + r) + (int64_t)M2 * (v + R * u)];
+            d[kap * 4 + q] = __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pkrtz(x.x, x.y));
+        }
+    const half8 a0 = __builtin_bit_cast(half8, make_uint4(d[0], d[1], d[2], d[3]));
+    const half8 a1 = __builtin_bit_cast(half8, make_uint4(d[4], d[5], d[6], d[7]));
+    __syncthreads();   // B table in LDS
+    f4 y[4];           // tile t: k' = l % 16 + 16 (t & 1), re (t < 2) | im; rows 4 g + i
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+        y[t] = mf_stage1(a0, a1, ld_b1(b1, t * 2 + 0, l), ld_b1(b1, t * 2 + 1, l),
+                         ld_b1(b1, (4 + t) * 2 + 0, l), ld_b1(b1, (4 + t) * 2 + 1, l));
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) sb[v][4 * g + i][r + 16 * h] = make_float2(y[h][i], y[2 + h][i]);
+    __syncthreads();
+    for (int pi = tid; pi < 16 * 32; pi += 64 * R) {
+        const int rr = pi & 15, kp = pi >> 4;
+        float2 a[R];
+#pragma unroll
+        for (int vv = 0; vv < R; ++vv) {
+            a[vv] = sb[vv][rr][kp];
+            if (vv > 0) a[vv] = c_mul(a[vv], tw_m[vv * kp * M2]);   // W_L^{v k'} = W_M^{v k' M2}
+        }
+        dftR<R>(a);
+        const int64_t m2 = c0 + rr;
+#pragma unroll
+        for (int k2 = 0; k2 < R; ++k2) {
+            const int64_t k1 = kp + 32 * k2;
+            dst[m2 + k1 * M2] = c_mul(a[k2], tw_m[(m2 * k1) % M]);
+        }
+    }
+}
+
+hipError_t launch_fft_pass1_mfma(const float2 *in, float2 *out, int64_t nbatch, int M1, int M2,
+                                 const void *b1, const float2 *tw_m, hipStream_t s) {
+    const int M = M1 * M2;
+    if (b1 == nullptr || M2 % 16 != 0 || (M1 != 128 && M1 != 256)) return hipErrorInvalidValue;
+    if (nbatch <= 0) return hipSuccess;
+    const int64_t grid = nbatch * (M2 / 16);
+    const uint4 *b = reinterpret_cast<const uint4 *>(b1);
+    if (M1 == 128)
+        hipLaunchKernelGGL(k_fft_pass1_mfma<4>, dim3((unsigned)grid), dim3(256), 0, s, in, out, nbatch,
+                           M2, b, tw_m, M);
+    else
+        hipLaunchKernelGGL(k_fft_pass1_mfma<8>, dim3((unsigned)grid), dim3(512), 0, s, in, out, nbatch,
+                           M2, b, tw_m, M);
+    return hipGetLastError();
+}
+
 }  // namespace fxc

@@ -89,7 +89,7 @@ class PlanInfo(C.Structure):
                 ("bpf2", C.c_int32), ("bpf_out", C.c_int32), ("decode_ok1", C.c_int32),
                 ("decode_ok2", C.c_int32), ("fft_form", C.c_int32),
                 ("threads_per_frame", C.c_int32), ("device_id", C.c_int32),
-                ("variant", C.c_int32), ("grid_divisor", C.c_int32)]
+                ("variant", C.c_int32), ("grid_divisor", C.c_int32), ("fft_mfma", C.c_int32)]
 
 
 class CorStation(C.Structure):

@@ -121,12 +121,17 @@ typedef struct {
     int32_t half;                 /* fft_len/2 + 1                                  */
     int32_t bpf1, bpf2, bpf_out;  /* bytes per station-frame (main.rs:1214-1215)    */
     int32_t decode_ok1, decode_ok2; /* 0: every frame fails decode (utils.rs:372)   */
-    int32_t fft_form;             /* 0 = FP32 Stockham, 1 = FP16/FP32-MFMA four-step,
-                                     2 = FP32 four-step through HBM (N >= 32768)    */
+    int32_t fft_form;             /* 0 = FP32 Stockham, 1 = FP16/FP32-MFMA four-step
+                                     (N = 1024, 2048), 2 = four-step through HBM
+                                     (N >= 32768)                                   */
     int32_t threads_per_frame;
     int32_t device_id;
     int32_t variant;              /* launch configuration in use (fxc_tune_report) */
     int32_t grid_divisor;
+    int32_t fft_mfma;             /* 1: the FFT's first DFT (32 points) runs as an
+                                     FP16 hi/lo MFMA GEMM (chosen form, levels exact
+                                     in FP16: fft_form 1, or fft_form 2 when the plan
+                                     has no phased-array products)                  */
 } fxc_plan_info;
 
 /* ----------------------------------------------------------------------- */
@@ -315,7 +320,9 @@ fxc_status fxc_decode_frames(fxc_plan *plan, int32_t station,
                              const int64_t *shifts, float *out,
                              int64_t *n_failed);
 /* FftHelper::forward_r2c_process (utils.rs:37-59) on nframes real frames:
- * in nframes*fft_len floats, out nframes*(half) interleaved complex. */
+ * in nframes*fft_len floats, out nframes*(half) interleaved complex.  When
+ * fxc_plan_info.fft_mfma is 1 the samples enter the FFT as FP16: exact for
+ * the plan's levels (what the decode feeds the FFT), rounded otherwise. */
 fxc_status fxc_rfft_frames(fxc_plan *plan, const float *in, int64_t nframes,
                            float *out);
 /* FftHelper::inverse_c2r_process (utils.rs:60-66): in nframes*half complex,

@@ -297,6 +297,13 @@ def test_mfma_eligibility():
         F.plan_set_tuning(p2, variant=0)            # no whole-frame organization at N = 2048
     odd = [np.array([-1.1, -0.3, 0.3, 1.1])] * 2                           # not exact in FP16
     assert F.Plan(1024, levels=odd, df_hz=1.0, precision=F.PRECISION_CHOSEN).info.fft_form == 0
+    # fft_mfma: the first DFT-32 on FP16 MFMA, also in the HBM four-step (N >= 32768)
+    assert F.Plan(1024, levels=lv2, df_hz=1.0, precision=F.PRECISION_CHOSEN).info.fft_mfma == 1
+    assert F.Plan(4096, levels=lv2, df_hz=1.0, precision=F.PRECISION_CHOSEN).info.fft_mfma == 0
+    big = F.Plan(65536, levels=lv2, df_hz=1.0, precision=F.PRECISION_CHOSEN).info
+    assert big.fft_form == 2 and big.fft_mfma == 1
+    assert F.Plan(65536, levels=lv2, df_hz=1.0).info.fft_mfma == 0
+    assert F.Plan(32768, levels=odd, df_hz=1.0, precision=F.PRECISION_CHOSEN).info.fft_mfma == 0
 
 
 @pytest.mark.parametrize("variant", [0, 1])
@@ -561,6 +568,27 @@ def test_rfft_large_vs_numpy(n):
     ref = np.fft.rfft(x.astype(np.float64), axis=1)
     err = np.sqrt(np.mean(np.abs(got - ref) ** 2, axis=1) / np.mean(np.abs(ref) ** 2, axis=1))
     assert np.all(err < 3e-7 * np.log2(n)), err
+    assert np.all(got[:, 0].imag == 0) and np.all(got[:, -1].imag == 0)
+
+
+@pytest.mark.parametrize("n,m1", [(32768, 128), (65536, 128), (65536, 256)])
+def test_rfft_large_mfma_vs_numpy(n, m1):
+    """Chosen form, N >= 32768: the first pass's column DFTs (M1 = 32 R) as
+    an FP16 hi + lo DFT-32 MFMA GEMM, the W_M1 twiddles and DFT-R in FP32.
+    FP16-exact inputs as the decode feeds it; the RMS bar pins the lo part
+    (a lost lo part gives errors near 1e-4)."""
+    p = F.Plan(n, levels=[LEVELS_2BIT, LEVELS_2BIT], df_hz=1.0, precision=F.PRECISION_CHOSEN)
+    assert p.info.fft_form == 2 and p.info.fft_mfma == 1
+    F.plan_set_tuning(p, split_m1=m1, chunk_frames=3)
+    rng = np.random.default_rng(n + m1)
+    x = LEVELS_2BIT[rng.integers(0, 4, size=(4, n))].astype(np.float32)
+    x[0] = 0.0
+    x[0, 7] = 3.0                                      # impulse: exact linear phase
+    got = p.rfft_frames(x)
+    ref = np.fft.rfft(x.astype(np.float64), axis=1)
+    err = np.sqrt(np.mean(np.abs(got - ref) ** 2, axis=1) / np.mean(np.abs(ref) ** 2, axis=1))
+    assert np.all(err < 3e-7 * np.log2(n)), err
+    assert np.max(np.abs(got[0] - ref[0])) < 3e-6
     assert np.all(got[:, 0].imag == 0) and np.all(got[:, -1].imag == 0)
 
 
