@@ -259,6 +259,8 @@ def form_name(info):
     """The FFT a plan runs (fxc_plan_info.fft_form / fft_mfma)."""
     if info.fft_form == 2 and info.fft_mfma:
         return "four-step through HBM, FP16-MFMA DFT-32 first pass"
+    if info.fft_form == 0 and info.fft_mfma:
+        return "Stockham, FP16-MFMA DFT-32 first pass"
     return FORM_NAMES[info.fft_form]
 
 

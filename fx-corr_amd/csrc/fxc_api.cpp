@@ -150,7 +150,8 @@ struct fxc_plan {
     DevBuf tw_fft, tw_post, lev_dev[2], out_lev, mf_tab;
     int fft_form = 0;  // 0: FP32 Stockham, 1: FP16/FP32-MFMA four-step (N = 1024, 2048),
                        // 2: four-step through HBM scratch (N = 32768, 65536)
-    bool large_mfma = false;   // fft_form 2: first pass's DFT-32 on FP16 MFMA (mf_tab)
+    bool mf_b1 = false;   // fft_form 0 / 2: mf_tab holds the DFT-32 operand table (scale 1)
+                          // and the FFT's first pass runs on FP16 MFMA
     // large-N scratch (fft_form 2): chunk of `chunk` frames
     int M1 = 0, M2 = 0;
     int64_t chunk = 0;       // allocated chunk (frames)
@@ -468,7 +469,9 @@ static void fill_args(fxc_plan *p, SectorArgs &a) {
     if (p->desc.products & FXC_PRODUCT_RAW_OUT) fl |= FLAG_RAW_OUT;
     if (p->desc.products & FXC_PRODUCT_ACF_OVERLAP_ONLY) fl |= FLAG_ACF_OVERLAP;
     a.flags = (int32_t)fl;
-    a.mf_tab = p->mf_tab.p;
+    // fused MFMA kernels: their table layout; Stockham / HBM four-step: the
+    // DFT-32 operand table when the first pass runs on MFMA, else none
+    a.mf_tab = (p->fft_form == 1 || p->mf_b1) ? p->mf_tab.p : nullptr;
     a.variant = p->variant;
 }
 
@@ -540,11 +543,14 @@ static bool mfma_eligible(const fxc_plan *p) {
     return levels_fp16_exact(p);
 }
 
-// N >= 32768 (HBM four-step): the chosen form runs the first pass's DFT-32
-// on FP16 MFMA when the levels are exact in FP16 (any band alignment).
-static bool large_mfma_eligible(const fxc_plan *p) {
+// Chosen form outside the fused MFMA kernels (fft_form 0 with whole waves of
+// 32-point DFT rows, N = 1024..16384; fft_form 2, N >= 32768): the FFT's
+// first pass runs its DFT-32 on FP16 MFMA when the levels are exact in FP16
+// (any band alignment).
+static bool first_pass_mfma_eligible(const fxc_plan *p) {
     if (p->desc.precision != FXC_PRECISION_CHOSEN) return false;
     if (p->desc.products & (FXC_PRODUCT_PHASED | FXC_PRODUCT_RAW_OUT)) return false;
+    if (!(p->fft_form == 2 || (p->fft_form == 0 && p->N >= 1024 && p->N <= 16384))) return false;
     return levels_fp16_exact(p);
 }
 
@@ -812,14 +818,14 @@ extern "C" fxc_status fxc_plan_create(const fxc_plan_desc *desc, fxc_plan **out)
             p->d_params.ensure(C * frame_param_bytes()) != hipSuccess ||
             p->d_lshifts.ensure(C * 16) != hipSuccess)
             return cleanup(fail(FXC_ERR_NOMEM, "large-FFT scratch allocation failed"));
-        if (large_mfma_eligible(p)) {
-            std::vector<uint8_t> tab(MFT_A2 - MFT_B1, 0);
-            put_dft32_b1(tab, 0, 1.0);
-            if (p->mf_tab.ensure(tab.size()) != hipSuccess ||
-                hipMemcpy(p->mf_tab.p, tab.data(), tab.size(), hipMemcpyHostToDevice) != hipSuccess)
-                return cleanup(fail(FXC_ERR_HIP, "MFMA table upload failed"));
-            p->large_mfma = true;
-        }
+    }
+    if (first_pass_mfma_eligible(p)) {
+        std::vector<uint8_t> tab(MFT_A2 - MFT_B1, 0);
+        put_dft32_b1(tab, 0, 1.0);
+        if (p->mf_tab.ensure(tab.size()) != hipSuccess ||
+            hipMemcpy(p->mf_tab.p, tab.data(), tab.size(), hipMemcpyHostToDevice) != hipSuccess)
+            return cleanup(fail(FXC_ERR_HIP, "MFMA table upload failed"));
+        p->mf_b1 = true;
     }
     // grid: plain form about 16 waves per CU worth of workgroups (p->T
     // threads each); MFMA form one 8-wave workgroup per CU; large-N form one
@@ -930,7 +936,7 @@ extern "C" fxc_status fxc_plan_get_info(const fxc_plan *p, fxc_plan_info *info) 
     info->device_id = p->device;
     info->variant = p->variant;
     info->grid_divisor = p->grid_div;
-    info->fft_mfma = (p->fft_form == 1 || p->large_mfma) ? 1 : 0;
+    info->fft_mfma = (p->fft_form == 1 || p->mf_b1) ? 1 : 0;
     return FXC_OK;
 }
 
@@ -1116,7 +1122,7 @@ static fxc_status enqueue_large(fxc_plan *p, const SectorArgs &base, int64_t nf,
             HIP_TRY(launch_fft_large((const float *)p->d_x[st].p, (float2 *)p->d_y.p,
                                      (float2 *)p->d_z[st].p, cn, p->M1, p->M2, twl1, twl2,
                                      (const float2 *)p->tw_lm.p,
-                                     p->large_mfma ? p->mf_tab.p : nullptr, s));
+                                     p->mf_b1 ? p->mf_tab.p : nullptr, s));
         }
         if (chain) {
             if (span0 < 0 || c0 >= span0 + span) {
@@ -1538,7 +1544,7 @@ static fxc_status run_fft(fxc_plan *p, const float *in, int64_t nframes, float *
                                      (const float2 *)(p->M1 == 128 ? p->tw_l1.p : p->tw_l2.p),
                                      (const float2 *)(p->M2 == 128 ? p->tw_l1.p : p->tw_l2.p),
                                      (const float2 *)p->tw_lm.p,
-                                     p->large_mfma ? p->mf_tab.p : nullptr, s));
+                                     p->mf_b1 ? p->mf_tab.p : nullptr, s));
             HIP_TRY(launch_untangle_large((const float2 *)p->d_z[0].p, cn, p->M,
                                           (const float2 *)p->tw_post.p,
                                           (float2 *)p->d_tmp_out.p + c0 * p->half, s));
@@ -1548,8 +1554,10 @@ static fxc_status run_fft(fxc_plan *p, const float *in, int64_t nframes, float *
                                     false, s));
     else if (p->fft_form == 1 && p->N == 1024)
         HIP_TRY(launch_rfft_mfma(a, (const float *)p->d_tmp_in.p, nframes, (float2 *)p->d_tmp_out.p, s));
-    else
+    else {
+        if (p->fft_form == 1) a.mf_tab = nullptr;   // N = 2048: its table is the fused kernel's
         HIP_TRY(launch_rfft_frames(a, (const float *)p->d_tmp_in.p, nframes, (float2 *)p->d_tmp_out.p, s));
+    }
     HIP_TRY(hipMemcpyAsync(out, p->d_tmp_out.p, out_bytes, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     return FXC_OK;

@@ -359,6 +359,78 @@ __device__ __forceinline__ void stockham_first(float2 *z, float2 *v, int t) {
     __syncthreads();
 }
 
+// ------------------------------------------- FP16 MFMA DFT-32 (chosen form)
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+// One 16-row tile of a DFT-32 GEMM, output column tile t (t&1: k1 half,
+// t>>1: re|im): y[r] = Y[row 4(l/16) + r][k1 = l%16 + 16(t&1)] (re or im).
+// A operands a0/a1 (K-steps 0/1) hold, for lane l, row l%16 and the re|im
+// pairs of m1 = 16 kap + l/16 + 4q (q = 0..3); B the matching hi / lo FP16
+// parts of the DFT-32 coefficients.  The lo products (FP16 subnormals
+// included) go first into the one FP32 accumulator, the hi products after.
+__device__ __forceinline__ f4 mf_stage1(half8 a0, half8 a1, half8 bh0, half8 bl0, half8 bh1,
+                                        half8 bl1) {
+    f4 d = {0.f, 0.f, 0.f, 0.f};
+    d = __builtin_amdgcn_mfma_f32_16x16x32_f16(a0, bl0, d, 0, 0, 0);
+    d = __builtin_amdgcn_mfma_f32_16x16x32_f16(a1, bl1, d, 0, 0, 0);
+    d = __builtin_amdgcn_mfma_f32_16x16x32_f16(a0, bh0, d, 0, 0, 0);
+    d = __builtin_amdgcn_mfma_f32_16x16x32_f16(a1, bh1, d, 0, 0, 0);
+    return d;
+}
+
+// The first Stockham pass (NS = 1, radix 32) of this half's M-point buffer
+// as FP16 MFMA GEMMs: z[m2*32 + k1] = sum_m1 W32^{m1 k1} z[m2 + M2 m1],
+// M2 = M/32 rows of 32-point DFTs in 16-row tiles, E/8 tiles per wave.  The
+// inputs are decoded levels, exact in FP16 (the plan checks this), so the
+// conversion is exact.  b1: the hi/lo operand table in global memory
+// (16 x 64 uint4, [(kap*4 + t)*2 + lohi][lane]).  Workgroup-wide barriers:
+// every thread of the workgroup calls this.
+template <int E, int T>
+__device__ __forceinline__ void stockham_first_mfma(float2 *z, const uint4 *b1, int t) {
+    constexpr int M = E * T, M2 = M / 32, TPW = E / 8;
+    static_assert(T % 64 == 0 && TPW >= 1 && TPW * (T / 64) * 16 == M2, "tile layout");
+    const int l = t & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
+    const int c = l & 15, g = l >> 4;
+    half8 A[TPW][2];
+#pragma unroll
+    for (int tp = 0; tp < TPW; ++tp) {
+        const int m2 = 16 * (wv * TPW + tp) + c;
+        uint32_t d[8];
+#pragma unroll
+        for (int kap = 0; kap < 2; ++kap)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const float2 x = z[pad(m2 + M2 * (16 * kap + g + 4 * q))];
+                d[kap * 4 + q] = __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pkrtz(x.x, x.y));
+            }
+        A[tp][0] = __builtin_bit_cast(half8, make_uint4(d[0], d[1], d[2], d[3]));
+        A[tp][1] = __builtin_bit_cast(half8, make_uint4(d[4], d[5], d[6], d[7]));
+    }
+    f4 y[TPW][4];
+#pragma unroll
+    for (int tt = 0; tt < 4; ++tt) {
+        const half8 bh0 = __builtin_bit_cast(half8, b1[(tt * 2 + 0) * 64 + l]);
+        const half8 bl0 = __builtin_bit_cast(half8, b1[(tt * 2 + 1) * 64 + l]);
+        const half8 bh1 = __builtin_bit_cast(half8, b1[((4 + tt) * 2 + 0) * 64 + l]);
+        const half8 bl1 = __builtin_bit_cast(half8, b1[((4 + tt) * 2 + 1) * 64 + l]);
+#pragma unroll
+        for (int tp = 0; tp < TPW; ++tp) y[tp][tt] = mf_stage1(A[tp][0], A[tp][1], bh0, bl0, bh1, bl1);
+    }
+    __syncthreads();   // every wave's reads of z are done
+#pragma unroll
+    for (int tp = 0; tp < TPW; ++tp)
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int m2 = 16 * (wv * TPW + tp) + 4 * g + i;
+                z[pad(m2 * 32 + c + 16 * h)] = make_float2(y[tp][h][i], y[tp][2 + h][i]);
+            }
+    __syncthreads();
+}
+
 // Forward complex FFT of M = E*T points held as v[r] = z[t + T*r]; result
 // in natural order in z.
 template <int E, int T>

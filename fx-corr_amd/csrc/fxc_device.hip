@@ -14,7 +14,9 @@ namespace fxc {
 // The workgroup holds 2T threads: threads [0,T) decode and transform
 // station 1, [T,2T) station 2 (M = E*T complex points each); then every
 // thread owns output-grid bins g = tid + 2T*i for the products.
-template <int E, int T, int PREC>
+// MF (chosen form, levels exact in FP16): the FFT's first pass (radix 32)
+// runs as FP16 hi/lo MFMA GEMMs, the remaining Stockham passes in FP32.
+template <int E, int T, int PREC, bool MF = false>
 __global__ __launch_bounds__(2 * T) void k_sector(SectorArgs a) {
     constexpr int M = E * T;
     constexpr int MP = M + M / 16 + 2;
@@ -26,6 +28,10 @@ __global__ __launch_bounds__(2 * T) void k_sector(SectorArgs a) {
     __shared__ float2 zbuf[2][MP];
     __shared__ FrameParam fp[kFrameBatch];
     __shared__ float lev[2][256];
+    // MF: the DFT-32 operand table in LDS (16 KB; at M = 8192 the kernel's
+    // LDS is then 160,032 of 163,840 bytes)
+    constexpr bool B1_LDS = MF;
+    __shared__ uint4 b1s[B1_LDS ? 16 * 64 : 1];
 
     const int tid0 = threadIdx.x;
     const int s = half_index<T>(tid0);   // station handled by this half of the workgroup
@@ -40,6 +46,9 @@ __global__ __launch_bounds__(2 * T) void k_sector(SectorArgs a) {
 
     if (st.bits > 2 && st.bits <= 8)
         for (int i = t; i < 256; i += T) lev[s][i] = (i <= (int)st.code_mask) ? st.levels[i] : 0.0f;
+    if constexpr (B1_LDS)
+        for (int i = tid0; i < 16 * 64; i += NT) b1s[i] = reinterpret_cast<const uint4 *>(a.mf_tab)[i];
+    const uint4 *b1 = B1_LDS ? b1s : reinterpret_cast<const uint4 *>(a.mf_tab);
 
     double acc[4][NB];
 #pragma unroll
@@ -73,7 +82,15 @@ __global__ __launch_bounds__(2 * T) void k_sector(SectorArgs a) {
 #pragma unroll
             for (int r = 0; r < E; ++r) v[r] = decode_elem(ws, st, lev[s], t + T * r, sh, N);
             __syncthreads();
-            fft_forward<E, T>(zs, v, a.tw_fft, t);
+            if constexpr (MF) {
+#pragma unroll
+                for (int r = 0; r < E; ++r) zs[pad(t + T * r)] = v[r];
+                __syncthreads();
+                stockham_first_mfma<E, T>(zs, b1, t);
+                stockham_rest<E, T, 32>(zs, a.tw_fft, t);
+            } else {
+                fft_forward<E, T>(zs, v, a.tw_fft, t);
+            }
 
             // ---- products on the output grid (main.rs:2491-2568)
             const float2 frmix = fp[jf].frmix;
@@ -209,10 +226,19 @@ int sector_threads_per_frame(int N) {
 
 template <int E, int T>
 static hipError_t launch_sector_t(const SectorArgs &a, int precision, int n_wg, hipStream_t s) {
-    if (precision == 0)
+    if (precision == 0) {
         hipLaunchKernelGGL((k_sector<E, T, 0>), dim3(n_wg), dim3(2 * T), 0, s, a);
-    else
-        hipLaunchKernelGGL((k_sector<E, T, 1>), dim3(n_wg), dim3(2 * T), 0, s, a);
+        return hipGetLastError();
+    }
+    // a.mf_tab: the plan's first FFT pass runs on FP16 MFMA (whole waves of
+    // 32-point DFT rows: E >= 8, T a multiple of 64)
+    if constexpr (E >= 8 && T % 64 == 0) {
+        if (a.mf_tab != nullptr) {
+            hipLaunchKernelGGL((k_sector<E, T, 1, true>), dim3(n_wg), dim3(2 * T), 0, s, a);
+            return hipGetLastError();
+        }
+    }
+    hipLaunchKernelGGL((k_sector<E, T, 1>), dim3(n_wg), dim3(2 * T), 0, s, a);
     return hipGetLastError();
 }
 
@@ -280,8 +306,8 @@ hipError_t launch_decode_frames(const SectorArgs &a, int station, const int64_t 
 }
 
 // Forward R2C with the fused kernel's FFT code: each half of the workgroup
-// transforms one frame.
-template <int E, int T>
+// transforms one frame (MF: first pass on FP16 MFMA, as k_sector).
+template <int E, int T, bool MF = false>
 __global__ __launch_bounds__(2 * T) void k_rfft_frames(SectorArgs a, const float *in,
                                                        int64_t nframes, float2 *out) {
     constexpr int M = E * T;
@@ -296,7 +322,15 @@ __global__ __launch_bounds__(2 * T) void k_rfft_frames(SectorArgs a, const float
     for (int r = 0; r < E; ++r)
         v[r] = live ? reinterpret_cast<const float2 *>(in + f * 2 * M)[t + T * r]
                     : make_float2(0.f, 0.f);
-    fft_forward<E, T>(zbuf[s], v, a.tw_fft, t);
+    if constexpr (MF) {
+#pragma unroll
+        for (int r = 0; r < E; ++r) zbuf[s][pad(t + T * r)] = v[r];
+        __syncthreads();
+        stockham_first_mfma<E, T>(zbuf[s], reinterpret_cast<const uint4 *>(a.mf_tab), t);
+        stockham_rest<E, T, 32>(zbuf[s], a.tw_fft, t);
+    } else {
+        fft_forward<E, T>(zbuf[s], v, a.tw_fft, t);
+    }
     if (live)
         for (int k = t; k <= M; k += T) out[f * (M + 1) + k] = untangle(zbuf[s], k, M, a.tw_post);
 }
@@ -362,9 +396,17 @@ static hipError_t launch_fft_t(const SectorArgs &a, const void *in, int64_t nfra
     if (mode != 0)
         hipLaunchKernelGGL((k_irfft_frames<E, T>), grid, dim3(2 * T), 0, s, a, (const float2 *)in,
                            nframes, (float *)out, mode == 2 ? 1 : 0);
-    else
+    else {
+        if constexpr (E >= 8 && T % 64 == 0) {
+            if (a.mf_tab != nullptr) {
+                hipLaunchKernelGGL((k_rfft_frames<E, T, true>), grid, dim3(2 * T), 0, s, a,
+                                   (const float *)in, nframes, (float2 *)out);
+                return hipGetLastError();
+            }
+        }
         hipLaunchKernelGGL((k_rfft_frames<E, T>), grid, dim3(2 * T), 0, s, a, (const float *)in,
                            nframes, (float2 *)out);
+    }
     return hipGetLastError();
 }
 

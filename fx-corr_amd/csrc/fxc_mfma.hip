@@ -27,9 +27,6 @@
 
 namespace fxc {
 
-typedef _Float16 half8 __attribute__((ext_vector_type(8)));
-typedef float f4 __attribute__((ext_vector_type(4)));
-
 namespace mf {
 constexpr int N = 1024, M = 512;
 constexpr int WAVES = 8;                 // waves per workgroup
@@ -232,20 +229,6 @@ __device__ __forceinline__ void mf_build_a(const uint16_t *row, int sh, int l, h
 }
 
 // ------------------------------------------------------------- FFT
-// Stage 1 for one station-frame, column tile t (t&1: k1 half, t>>1: re|im):
-// y[r] = Y[m2 = 4(l/16) + r][k1 = l%16 + 16(t&1)] (re or im).
-// The lo products (B - B_hi, FP16 subnormals included) go first into the
-// one FP32 accumulator, the hi products after them.
-__device__ __forceinline__ f4 mf_stage1(half8 a0, half8 a1, half8 bh0, half8 bl0, half8 bh1,
-                                        half8 bl1) {
-    f4 d = {0.f, 0.f, 0.f, 0.f};
-    d = __builtin_amdgcn_mfma_f32_16x16x32_f16(a0, bl0, d, 0, 0, 0);
-    d = __builtin_amdgcn_mfma_f32_16x16x32_f16(a1, bl1, d, 0, 0, 0);
-    d = __builtin_amdgcn_mfma_f32_16x16x32_f16(a0, bh0, d, 0, 0, 0);
-    d = __builtin_amdgcn_mfma_f32_16x16x32_f16(a1, bh1, d, 0, 0, 0);
-    return d;
-}
-
 __device__ __forceinline__ half8 ld_b1(const uint4 *b1, int idx, int l) {
     return __builtin_bit_cast(half8, b1[idx * 64 + l]);
 }

@@ -236,6 +236,23 @@ def test_band_shift_grids(grid, overlap_only):
     _compare(gpu_run(c, F.PRECISION_CHOSEN), o64, CHOSEN_TOL)
 
 
+@pytest.mark.parametrize("n", [1024, 2048])
+def test_stockham_mfma_band_shift(n):
+    """A band shift keeps N = 1024/2048 off the fused MFMA kernels: the
+    chosen form runs the Stockham kernel with its MFMA first pass."""
+    fs = 512e6
+    c = make_case(n=n, fs=fs, nf=1500, bits=(2, 4), band=(-64, 64, n // 2, 0), rho=0.5,
+                  delay_samples=-2.3, seed=19, delay_user_samples=-2.3,
+                  extra_delay_rate_sps=2e-9, lo1_hz=8.4e9, lo2_hz=8.4e9)
+    p = gpu_plan(c, F.PRECISION_CHOSEN)
+    assert p.info.fft_form == 0 and p.info.fft_mfma == 1
+    o64 = oracle_run(c, R.MODE_F64)
+    got = gpu_run(c, plan=p)
+    _compare(got, o64, CHOSEN_TOL)
+    for k in ("acc11", "acc22", "acc12"):
+        assert zero_pattern_equal(got[k], o64[k]), k
+
+
 def test_analytic_delay_branch_and_large_shifts():
     # no geometric table: quadratic delay model; shifts up to beyond +-N
     n, fs = 512, 128e6
@@ -299,7 +316,10 @@ def test_mfma_eligibility():
     assert F.Plan(1024, levels=odd, df_hz=1.0, precision=F.PRECISION_CHOSEN).info.fft_form == 0
     # fft_mfma: the first DFT-32 on FP16 MFMA, also in the HBM four-step (N >= 32768)
     assert F.Plan(1024, levels=lv2, df_hz=1.0, precision=F.PRECISION_CHOSEN).info.fft_mfma == 1
-    assert F.Plan(4096, levels=lv2, df_hz=1.0, precision=F.PRECISION_CHOSEN).info.fft_mfma == 0
+    mid = F.Plan(4096, levels=lv2, df_hz=1.0, precision=F.PRECISION_CHOSEN).info
+    assert mid.fft_form == 0 and mid.fft_mfma == 1                      # Stockham, MFMA first pass
+    assert F.Plan(512, levels=lv2, df_hz=1.0, precision=F.PRECISION_CHOSEN).info.fft_mfma == 0
+    assert F.Plan(4096, levels=lv2, df_hz=1.0).info.fft_mfma == 0
     big = F.Plan(65536, levels=lv2, df_hz=1.0, precision=F.PRECISION_CHOSEN).info
     assert big.fft_form == 2 and big.fft_mfma == 1
     assert F.Plan(65536, levels=lv2, df_hz=1.0).info.fft_mfma == 0
@@ -482,7 +502,32 @@ def test_mfma_needs_common_band_start():
     lv2 = [LEVELS_2BIT, LEVELS_2BIT]
     p = F.Plan(1024, levels=lv2, df_hz=1.0, band=(-16, 16, 512, 0),
                precision=F.PRECISION_CHOSEN)
-    assert p.info.fft_form == 0
+    assert p.info.fft_form == 0 and p.info.fft_mfma == 1   # Stockham with the MFMA first pass
+
+
+@pytest.mark.parametrize("n,variant", [(1024, 0), (2048, 0), (2048, 1), (4096, 0), (4096, 1),
+                                       (8192, 1), (16384, 0)])
+def test_rfft_stockham_mfma_vs_numpy(n, variant):
+    """Chosen form on the Stockham kernel: the first pass (radix 32) as FP16
+    hi + lo DFT-32 MFMA GEMMs, the rest in FP32.  FP16-exact inputs as the
+    decode feeds it; the RMS bar pins the lo part."""
+    lv2 = [LEVELS_2BIT, LEVELS_2BIT]
+    band = (-16, 16, n // 2, 0) if n <= 2048 else None   # keeps N = 1024/2048 off the fused kernels
+    kw = dict(band=band) if band else {}
+    p = F.Plan(n, levels=lv2, df_hz=1.0, precision=F.PRECISION_CHOSEN, **kw)
+    assert p.info.fft_form == 0 and p.info.fft_mfma == 1
+    if variant:
+        F.plan_set_tuning(p, variant=variant)
+    rng = np.random.default_rng(n + variant)
+    x = LEVELS_2BIT[rng.integers(0, 4, size=(5, n))].astype(np.float32)
+    x[0] = 0.0
+    x[0, 5] = 3.0                                      # impulse: exact linear phase
+    got = p.rfft_frames(x)
+    ref = np.fft.rfft(x.astype(np.float64), axis=1)
+    err = np.sqrt(np.mean(np.abs(got - ref) ** 2, axis=1) / np.mean(np.abs(ref) ** 2, axis=1))
+    assert np.all(err < 3e-7 * np.log2(n)), err
+    assert np.max(np.abs(got[0] - ref[0])) < 3e-6
+    assert np.all(got[:, 0].imag == 0) and np.all(got[:, -1].imag == 0)
 
 
 # ------------------------------------------------------------ phased array
