@@ -695,8 +695,6 @@ extern "C" fxc_status fxc_plan_create(const fxc_plan_desc *desc, fxc_plan **out)
     const bool large = N == 32768 || N == 65536;
     if (!is_pow2(N) || !(sector_supported(N) || large))
         return fail(FXC_ERR_UNSUPPORTED, "fft_len %d: the GPU path supports powers of two 64..65536", N);
-    if (large && (desc->products & (FXC_PRODUCT_PHASED | FXC_PRODUCT_RAW_OUT)))
-        return fail(FXC_ERR_UNSUPPORTED, "phased products need fft_len <= 16384");
     for (int i = 0; i < 2; ++i) {
         fxc_status s = validate_station(desc->st[i], i);
         if (s != FXC_OK) return s;
@@ -815,7 +813,7 @@ extern "C" fxc_status fxc_plan_create(const fxc_plan_desc *desc, fxc_plan **out)
             p->d_x[0].ensure(C * N * 4) != hipSuccess || p->d_x[1].ensure(C * N * 4) != hipSuccess ||
             p->d_y.ensure(C * M * 8) != hipSuccess || p->d_z[0].ensure(C * M * 8) != hipSuccess ||
             p->d_z[1].ensure(C * M * 8) != hipSuccess ||
-            p->d_params.ensure(C * frame_param_bytes()) != hipSuccess ||
+            p->d_params.ensure(C * std::max(frame_param_bytes(), phased_param_bytes())) != hipSuccess ||
             p->d_lshifts.ensure(C * 16) != hipSuccess)
             return cleanup(fail(FXC_ERR_NOMEM, "large-FFT scratch allocation failed"));
     }
@@ -1090,6 +1088,68 @@ static fxc_status enqueue_phased(fxc_plan *p, const SectorArgs &base, int64_t nf
     return FXC_OK;
 }
 
+// Phased-array sum at N = 32768 / 65536: the large path's chunks (decode and
+// HBM four-step FFT of both stations), the stations' rotation chains over
+// whole chunks, the bin-per-thread phased kernel into kLargeGroups partial
+// slots and, for the .raw output, the inverse FFT through the same scratch
+// and the quantiser.
+static fxc_status enqueue_phased_large(fxc_plan *p, const SectorArgs &base, int64_t nf,
+                                       uint8_t *d_raw_out, double *d_acc, hipStream_t s) {
+    const bool raw = (p->desc.products & FXC_PRODUCT_RAW_OUT) && d_raw_out != nullptr;
+    const size_t half = (size_t)p->half;
+    const int64_t bpf_o = plan_bpf_out(p);
+    const float2 *twl1 = (const float2 *)(p->M1 == 128 ? p->tw_l1.p : p->tw_l2.p);
+    const float2 *twl2 = (const float2 *)(p->M2 == 128 ? p->tw_l1.p : p->tw_l2.p);
+    const float2 *twm = (const float2 *)p->tw_lm.p;
+    const int ld = chain_ld_for((int)half);
+    const int64_t chunk = p->chunk_use;
+    const int64_t span = chain_span(nf, chunk, 2 * (size_t)ld * 8, (int)half);
+    if (p->d_chain.ensure((size_t)span * 2 * ld * 8) != hipSuccess ||
+        (raw && p->d_cb.ensure((size_t)chunk * half * 8) != hipSuccess))
+        return fail(FXC_ERR_NOMEM, "phased scratch allocation failed");
+    int64_t span0 = -1;    // first frame (of this sector) of the computed chains
+    for (int64_t c0 = 0; c0 < nf; c0 += chunk) {
+        const int64_t cn = std::min(chunk, nf - c0);
+        SectorArgs a = base;
+        a.raw1 = base.raw1 + c0 * p->st[0].bpf;
+        a.raw2 = base.raw2 + c0 * p->st[1].bpf;
+        a.first_frame = base.first_frame + c0;
+        a.nf = cn;
+        a.part = (double *)p->part.p;
+        a.cb_out = raw ? (float2 *)p->d_cb.p : nullptr;
+        int64_t *shifts = (int64_t *)p->d_lshifts.p;
+        HIP_TRY(launch_phased_params(a, p->d_params.p, shifts, s));
+        for (int st = 0; st < 2; ++st) {
+            HIP_TRY(launch_decode_frames(a, st, shifts + st, 2, (float *)p->d_x[st].p, s));
+            HIP_TRY(launch_fft_large((const float *)p->d_x[st].p, (float2 *)p->d_y.p,
+                                     (float2 *)p->d_z[st].p, cn, p->M1, p->M2, twl1, twl2, twm,
+                                     nullptr, s));
+        }
+        if (span0 < 0 || c0 >= span0 + span) {
+            SectorArgs ca = a;
+            ca.nf = std::min(span, nf - c0);
+            ca.chain_ld = ld;
+            HIP_TRY(launch_phase_chain(ca, CHAIN_PHASED, (float2 *)p->d_chain.p, s));
+            span0 = c0;
+        }
+        a.chain = (const float2 *)p->d_chain.p + (size_t)(c0 - span0) * 2 * ld;   // rows 2f + station
+        a.chain_ld = ld;
+        HIP_TRY(launch_phased_large(a, (const float2 *)p->d_z[0].p, (const float2 *)p->d_z[1].p,
+                                    p->d_params.p, kLargeGroups, c0 > 0 ? 1 : 0, s));
+        if (raw) {
+            // d_x[0]: conj(Z'), d_z[0]: its FFT, d_x[1]: the time series
+            HIP_TRY(launch_irfft_large((const float2 *)p->d_cb.p, cn, p->M1, p->M2, twl1, twl2, twm,
+                                       (const float2 *)p->tw_post.p, (float2 *)p->d_x[0].p,
+                                       (float2 *)p->d_y.p, (float2 *)p->d_z[0].p,
+                                       p->desc.output_lsb != 0, (float *)p->d_x[1].p, s));
+            HIP_TRY(launch_quantise_frames(a, (const float *)p->d_x[1].p, cn, d_raw_out + c0 * bpf_o, s));
+        }
+    }
+    HIP_TRY(launch_reduce((const double *)p->part.p, kLargeGroups, p->half, d_acc,
+                          (double *)p->red_scratch.p, 64, s));
+    return FXC_OK;
+}
+
 // N = 32768 / 65536: chunks of frames through HBM scratch; the product
 // kernel's kLargeGroups frame-group partial slots accumulate over the chunks
 // in order, then one fixed-order reduce.
@@ -1166,6 +1226,7 @@ static fxc_status enqueue_sector(fxc_plan *p, const uint8_t *d_raw1, const uint8
     a.raw2 = d_raw2;
     a.first_frame = first_frame;
     a.nf = nf;
+    if (phased && p->fft_form == 2) return enqueue_phased_large(p, a, nf, d_raw_out, d_acc, s);
     if (phased) return enqueue_phased(p, a, nf, d_raw_out, d_acc, s);
     if (p->fft_form == 2) return enqueue_large(p, a, nf, d_acc, s);
     const int wg_cap = std::max(1, p->max_wg / p->grid_div);
@@ -1535,8 +1596,17 @@ static fxc_status run_fft(fxc_plan *p, const float *in, int64_t nframes, float *
     HIP_TRY(hipMemcpyAsync(p->d_tmp_in.p, in, in_bytes, hipMemcpyHostToDevice, s));
     SectorArgs a;
     fill_args(p, a);
-    if (p->fft_form == 2) {
-        if (inverse) return fail(FXC_ERR_UNSUPPORTED, "inverse FFT needs fft_len <= 16384");
+    if (p->fft_form == 2 && inverse) {
+        for (int64_t c0 = 0; c0 < nframes; c0 += p->chunk) {   // through the chunk scratch
+            const int64_t cn = std::min(p->chunk, nframes - c0);
+            HIP_TRY(launch_irfft_large((const float2 *)p->d_tmp_in.p + c0 * p->half, cn, p->M1, p->M2,
+                                       (const float2 *)(p->M1 == 128 ? p->tw_l1.p : p->tw_l2.p),
+                                       (const float2 *)(p->M2 == 128 ? p->tw_l1.p : p->tw_l2.p),
+                                       (const float2 *)p->tw_lm.p, (const float2 *)p->tw_post.p,
+                                       (float2 *)p->d_x[0].p, (float2 *)p->d_y.p, (float2 *)p->d_z[0].p,
+                                       false, (float *)p->d_tmp_out.p + c0 * p->N, s));
+        }
+    } else if (p->fft_form == 2) {
         for (int64_t c0 = 0; c0 < nframes; c0 += p->chunk) {   // through the chunk scratch
             const int64_t cn = std::min(p->chunk, nframes - c0);
             HIP_TRY(launch_fft_large((const float *)p->d_tmp_in.p + c0 * p->N, (float2 *)p->d_y.p,

@@ -455,6 +455,40 @@ __device__ __forceinline__ float2 untangle(const float2 *zs, int k, int M, const
     return make_float2(ex + (w.x * dy - w.y * dx), ey - (w.x * dx + w.y * dy));
 }
 
+// realfft's split method on an unpadded Z (natural order in global memory).
+__device__ __forceinline__ float2 untangle_np(const float2 *zs, int k, int M, const float2 *tw_post) {
+    if (k == 0 || k == M) {
+        const float2 z0 = zs[0];
+        return make_float2(k == 0 ? z0.x + z0.y : z0.x - z0.y, 0.0f);
+    }
+    const float2 A = zs[k], Bc = zs[M - k];
+    const float ex = 0.5f * (A.x + Bc.x), ey = 0.5f * (A.y - Bc.y);
+    const float dx = 0.5f * (A.x - Bc.x), dy = 0.5f * (A.y + Bc.y);
+    const float2 w = tw_post[k];
+    return make_float2(ex + (w.x * dy - w.y * dx), ey - (w.x * dx + w.y * dy));
+}
+
+// The inverse real FFT's input element k (< M) of the half spectrum X
+// (M + 1 bins): Z'[k] = E' + i O' (the split method run backwards), returned
+// conjugated for IFFT = conj(FFT(conj(.))).  DC/Nyquist imaginary parts are
+// ignored (the reference zeroes them first).
+__device__ __forceinline__ float2 irfft_pre_elem(const float2 *X, int k, int M,
+                                                 const float2 *tw_post) {
+    float2 z;
+    if (k == 0) {
+        z = make_float2(X[0].x + X[M].x, X[0].x - X[M].x);
+    } else {
+        const float2 A = X[k], Bc = X[M - k];
+        const float er = A.x + Bc.x, ei = A.y - Bc.y;
+        const float dr = A.x - Bc.x, di = A.y + Bc.y;
+        const float2 w = tw_post[k];
+        const float orr = dr * w.x - di * w.y;
+        const float oi = dr * w.y + di * w.x;
+        z = make_float2(er - oi, ei + orr);
+    }
+    return c_conj(z);
+}
+
 __device__ __forceinline__ float norm_sqr(float2 a) { return a.x * a.x + a.y * a.y; }
 
 // Which half [0,T) / [T,2T) of the workgroup thread `tid` is in; made

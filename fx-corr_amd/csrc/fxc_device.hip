@@ -342,22 +342,7 @@ template <int E, int T>
 __device__ __forceinline__ void irfft_prepare(const float2 *X, int M, const float2 *tw_post,
                                               float2 *v, int t) {
 #pragma unroll
-    for (int r = 0; r < E; ++r) {
-        const int k = t + T * r;
-        float2 z;
-        if (k == 0) {
-            z = make_float2(X[0].x + X[M].x, X[0].x - X[M].x);
-        } else {
-            float2 A = X[k], Bc = X[M - k];
-            float er = A.x + Bc.x, ei = A.y - Bc.y;
-            float dr = A.x - Bc.x, di = A.y + Bc.y;
-            float2 w = tw_post[k];
-            float orr = dr * w.x - di * w.y;
-            float oi = dr * w.y + di * w.x;
-            z = make_float2(er - oi, ei + orr);
-        }
-        v[r] = c_conj(z);
-    }
+    for (int r = 0; r < E; ++r) v[r] = irfft_pre_elem(X, t + T * r, M, tw_post);
 }
 
 template <int E, int T>

@@ -164,6 +164,18 @@ hipError_t launch_fft_large(const float *x, float2 *ybuf, float2 *z, int64_t nfr
                             const void *mf_b1, hipStream_t stream);
 hipError_t launch_fft_pass1_mfma(const float2 *in, float2 *out, int64_t nbatch, int M1, int M2,
                                  const void *b1, const float2 *tw_m, hipStream_t stream);
+// inverse real FFT of nframes half spectra X through the HBM four-step:
+// zc (M complex per frame), ybuf, z scratch; out nframes*N floats.
+hipError_t launch_irfft_large(const float2 *X, int64_t nframes, int M1, int M2, const float2 *tw1,
+                              const float2 *tw2, const float2 *twm, const float2 *tw_post,
+                              float2 *zc, float2 *ybuf, float2 *z, bool negate_odd, float *out,
+                              hipStream_t stream);
+// phased-array sum at N >= 32768 (fxc_phased.hip); pp: phased_param_bytes()
+// per frame of the chunk
+size_t phased_param_bytes();
+hipError_t launch_phased_params(const SectorArgs &a, void *pp, int64_t *shifts, hipStream_t stream);
+hipError_t launch_phased_large(const SectorArgs &a, const float2 *z1, const float2 *z2,
+                               const void *pp, int n_groups, int accumulate, hipStream_t stream);
 hipError_t launch_untangle_large(const float2 *z, int64_t nframes, int M, const float2 *tw_post,
                                  float2 *out, hipStream_t stream);
 hipError_t launch_mac_large(const SectorArgs &a, int precision, const float2 *z1, const float2 *z2,

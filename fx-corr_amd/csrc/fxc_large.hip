@@ -130,19 +130,6 @@ hipError_t launch_fft_large(const float *x, float2 *ybuf, float2 *z, int64_t nfr
 
 size_t frame_param_bytes() { return sizeof(FrameParam); }
 
-// realfft's split method on an unpadded Z (natural order in global memory).
-__device__ __forceinline__ float2 untangle_np(const float2 *zs, int k, int M, const float2 *tw_post) {
-    if (k == 0 || k == M) {
-        const float2 z0 = zs[0];
-        return make_float2(k == 0 ? z0.x + z0.y : z0.x - z0.y, 0.0f);
-    }
-    const float2 A = zs[k], Bc = zs[M - k];
-    const float ex = 0.5f * (A.x + Bc.x), ey = 0.5f * (A.y - Bc.y);
-    const float dx = 0.5f * (A.x - Bc.x), dy = 0.5f * (A.y + Bc.y);
-    const float2 w = tw_post[k];
-    return make_float2(ex + (w.x * dy - w.y * dx), ey - (w.x * dx + w.y * dy));
-}
-
 __global__ void k_untangle_large(const float2 *z, int64_t nframes, int M, const float2 *tw_post,
                                  float2 *out) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -158,6 +145,50 @@ hipError_t launch_untangle_large(const float2 *z, int64_t nframes, int M, const 
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_untangle_large, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, z,
                        nframes, M, tw_post, out);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------ inverse real FFT, N >= 32768
+// inverse_c2r_process (utils.rs:60-66) through the HBM four-step: conj(Z')
+// of each frame's half spectrum, a forward FFT, then the real samples
+// interleaved and scaled 1/N (odd samples negated once more for an LSB
+// output stream, main.rs:2290-2292), as k_irfft_frames does in LDS.
+__global__ void k_irfft_prepare_large(const float2 *X, int64_t nframes, int M, const float2 *tw_post,
+                                      float2 *zc) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t f = i / M;
+    const int k = (int)(i - f * M);
+    if (f >= nframes) return;
+    zc[i] = irfft_pre_elem(X + f * (M + 1), k, M, tw_post);
+}
+
+__global__ void k_irfft_finish_large(const float2 *z, int64_t nframes, int M, int negate_odd,
+                                     float *out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nframes * M) return;
+    const float scale = 1.0f / (float)(2 * M);
+    const float odd_sign = negate_odd ? 1.0f : -1.0f;
+    const float2 v = z[i];
+    out[2 * i] = v.x * scale;
+    out[2 * i + 1] = odd_sign * (v.y * scale);
+}
+
+hipError_t launch_irfft_large(const float2 *X, int64_t nframes, int M1, int M2, const float2 *tw1,
+                              const float2 *tw2, const float2 *twm, const float2 *tw_post,
+                              float2 *zc, float2 *ybuf, float2 *z, bool negate_odd, float *out,
+                              hipStream_t s) {
+    const int M = M1 * M2;
+    const int64_t n = nframes * M;
+    if (n <= 0) return hipSuccess;
+    const dim3 grid((unsigned)((n + 255) / 256));
+    hipLaunchKernelGGL(k_irfft_prepare_large, grid, dim3(256), 0, s, X, nframes, M, tw_post, zc);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    e = launch_fft_large(reinterpret_cast<const float *>(zc), ybuf, z, nframes, M1, M2, tw1, tw2, twm,
+                         nullptr, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_irfft_finish_large, grid, dim3(256), 0, s, z, nframes, M, negate_odd ? 1 : 0,
+                       out);
     return hipGetLastError();
 }
 

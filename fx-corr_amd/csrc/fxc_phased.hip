@@ -159,6 +159,94 @@ hipError_t launch_phased(const SectorArgs &a, int accumulate, int n_wg, hipStrea
     }
 }
 
+// ------------------------------------------------ N = 32768 / 65536
+// The same sum through HBM scratch: per-frame parameters (and the decode's
+// integer shifts), then, after the HBM four-step FFTs of both stations, one
+// output-grid bin per thread over a group of frames (blockIdx.y), with the
+// arithmetic of k_phased's epilogue; partial slot per frame group.
+__global__ void k_phased_params(SectorArgs a, PhasedParam *pp, int64_t *shifts) {
+    const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= a.nf) return;
+    PhasedParam p;
+    make_phased_param(a.first_frame + f, a, p);
+    pp[f] = p;
+    shifts[2 * f] = p.s1;
+    shifts[2 * f + 1] = p.s2;
+}
+
+__global__ __launch_bounds__(256) void k_phased_large(SectorArgs a, const float2 *z1, const float2 *z2,
+                                                      const PhasedParam *pp, int64_t fpg,
+                                                      int accumulate) {
+    const int M = a.M;
+    const int half = M + 1;
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t f0 = (int64_t)blockIdx.y * fpg;
+    int64_t f1 = f0 + fpg;
+    if (f1 > a.nf) f1 = a.nf;
+    if (g >= half) return;
+    const int gsn = a.grid;
+    const int gs = (gsn == 0) ? a.a1s : a.a2s;
+    const bool in_ov = g >= gs && g < gs + a.ov;
+    const int j = g + ((gsn == 0) ? (a.a2s - a.a1s) : (a.a1s - a.a2s));   // other station's bin
+    const float w1 = (float)a.w1, w2 = (float)a.w2;
+    const bool write_cb = (a.flags & FLAG_RAW_OUT) != 0 && a.cb_out != nullptr;
+    const float2 *zg = gsn == 0 ? z1 : z2;
+    const float2 *zo = gsn == 0 ? z2 : z1;
+    double ph = 0.0;
+    for (int64_t f = f0; f < f1; ++f) {
+        const PhasedParam P = pp[f];
+        const float2 frg = gsn == 0 ? P.fr1 : P.fr2;
+        const float2 fro = gsn == 0 ? P.fr2 : P.fr1;
+        const float2 *chg = a.chain + (2 * f + gsn) * a.chain_ld;
+        const float2 *cho = a.chain + (2 * f + 1 - gsn) * a.chain_ld;
+        float2 sg = c_mul(untangle_np(zg + f * M, g, M, a.tw_post), chg[g]);
+        if (g == M) sg.y = 0.0f;                          // Nyquist (utils.rs:118-123)
+        const float2 cg = c_mul(sg, frg);
+        float2 co = make_float2(0.0f, 0.0f);
+        if (in_ov) {
+            float2 so = c_mul(untangle_np(zo + f * M, j, M, a.tw_post), cho[j]);
+            if (j == M) so.y = 0.0f;
+            co = c_mul(so, fro);
+        }
+        const float2 c1 = gsn == 0 ? cg : co, c2 = gsn == 0 ? co : cg;
+        float2 cb;
+        {
+#pragma clang fp contract(off)
+            const float2 a1 = c_scale(c1, w1), a2 = c_scale(c2, w2);
+            cb = make_float2(a1.x + a2.x, a1.y + a2.y);
+        }
+        ph += (double)norm_sqr(cb);
+        if (write_cb) {
+            if (g == 0 || g == M) cb.y = 0.0f;            // main.rs:2283-2284
+            a.cb_out[f * half + g] = cb;
+        }
+    }
+    double *out = a.part + (size_t)blockIdx.y * kNumProducts * half;
+    out[P_PH * half + g] = accumulate ? out[P_PH * half + g] + ph : ph;
+    out[P_11 * half + g] = 0.0;
+    out[P_22 * half + g] = 0.0;
+    out[P_12R * half + g] = 0.0;
+    out[P_12I * half + g] = 0.0;
+}
+
+size_t phased_param_bytes() { return sizeof(PhasedParam); }
+
+hipError_t launch_phased_params(const SectorArgs &a, void *pp, int64_t *shifts, hipStream_t s) {
+    if (a.nf <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_phased_params, dim3((unsigned)((a.nf + 127) / 128)), dim3(128), 0, s, a,
+                       reinterpret_cast<PhasedParam *>(pp), shifts);
+    return hipGetLastError();
+}
+
+hipError_t launch_phased_large(const SectorArgs &a, const float2 *z1, const float2 *z2,
+                               const void *pp, int n_groups, int accumulate, hipStream_t s) {
+    const int64_t fpg = (a.nf + n_groups - 1) / n_groups;
+    const dim3 grid((unsigned)((a.M + 1 + 255) / 256), (unsigned)n_groups);
+    hipLaunchKernelGGL(k_phased_large, grid, dim3(256), 0, s, a, z1, z2,
+                       reinterpret_cast<const PhasedParam *>(pp), fpg, accumulate);
+    return hipGetLastError();
+}
+
 // dst[i] += src[i] (FP64), e.g. a job's spectra += one chunk's spectra.
 __global__ void k_add_f64(double *dst, const double *src, size_t n) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;

@@ -128,6 +128,22 @@ def test_irfft_matches_oracle():
         assert np.allclose(got[i], R.irfft_f32(spec[i], n), atol=2e-6)
 
 
+@pytest.mark.parametrize("n", [32768, 65536])
+def test_irfft_large_matches_oracle(n):
+    """inverse_c2r_process through the HBM four-step (the .raw path of the
+    phased sum at N >= 32768)."""
+    p = _plan(n)
+    assert p.info.fft_form == 2
+    rng = np.random.default_rng(n + 4)
+    spec = (rng.standard_normal((3, n // 2 + 1)) + 1j * rng.standard_normal((3, n // 2 + 1)))
+    spec = spec.astype(np.complex64)
+    got = p.irfft_frames(spec)
+    for i in range(3):
+        ref = R.irfft_f32(spec[i], n)
+        rms = np.sqrt(np.mean(ref.astype(np.float64) ** 2))
+        assert np.max(np.abs(got[i] - ref)) < 2e-5 * rms, i
+
+
 # --------------------------------------------------------------- quantise
 @pytest.mark.parametrize("bits", [1, 2, 4, 8])
 def test_quantise_bit_exact(bits):
@@ -547,6 +563,8 @@ def _phased_desc(c, w, bit_out, output_lsb):
     (2048, (2, 1), 0, -16, False),
     (4096, (8, 2), 0, 64, True),
     (16384, (1, 8), 1, -32, False),
+    (32768, (4, 1), 0, 2048, True),
+    (65536, (2, 2), 1, -64, False),
 ])
 def test_phased_vs_oracle(n, bits, grid, band_shift, lsb_out):
     """yi-phasedarray sector: |w1 s1 + w2 s2|^2 within the plain bar of the
@@ -559,7 +577,8 @@ def test_phased_vs_oracle(n, bits, grid, band_shift, lsb_out):
     a1s = max(0, -sb)
     band = (sb, a1s, min(half, half - sb), a1s + sb)
     rng = np.random.default_rng(n + bits[0])
-    c = make_case(n=n, fs=fs, nf=1500, bits=bits, shuffles=(rng.permutation(32), None),
+    nf = 1500 if n <= 16384 else 48               # N >= 32768: the HBM four-step path
+    c = make_case(n=n, fs=fs, nf=nf, bits=bits, shuffles=(rng.permutation(32), None),
                   lsb=(False, True), band=band, grid=grid, rho=0.4, delay_samples=3.3, seed=41,
                   delay_user_samples=3.3, extra_delay_rate_sps=1e-9, lo1_hz=5e9, lo2_hz=5e9,
                   products=F.PRODUCT_PHASED | F.PRODUCT_RAW_OUT)
@@ -589,9 +608,10 @@ def test_phased_vs_oracle(n, bits, grid, band_shift, lsb_out):
     assert diff <= 1e-4 * n * len(range(0, c["nf"], 7)) + 2, diff
 
 
-def test_phased_no_raw_out_matches_with_raw_out():
-    n, fs = 1024, 64e6
-    c = make_case(n=n, fs=fs, nf=700, rho=0.3, seed=3, lo1_hz=2e9)
+@pytest.mark.parametrize("n,nf", [(1024, 700), (32768, 40)])
+def test_phased_no_raw_out_matches_with_raw_out(n, nf):
+    fs = 64e6
+    c = make_case(n=n, fs=fs, nf=nf, rho=0.3, seed=3, lo1_hz=2e9)
     cfg = F.make_delay_cfg(c["table"], **c["delay"])
     res = []
     for prods in (F.PRODUCT_PHASED, F.PRODUCT_PHASED | F.PRODUCT_RAW_OUT):
