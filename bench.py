@@ -307,11 +307,12 @@ def size_sweep(stream, samples_per_station=1 << 28, reps=5):
 def phased_sweep(stream, samples_per_station=1 << 28, reps=3):
     """Phased-array path (yi-phasedarray): two 2-bit stations at 1024 Msps
     summed into the phased power spectrum, without and with the 2-bit .raw
-    output, at FFT sizes 1024, 4096 and 16384; device-resident input, mean of
-    `reps` sector runs after one warm-up."""
+    output, at FFT sizes 1024, 4096, 16384 (fused LDS kernel) and 65536 (HBM
+    four-step); device-resident input, mean of `reps` sector runs after one
+    warm-up."""
     rows = {}
     fs = 1024e6
-    for n in [1024, 4096, 16384]:
+    for n in [1024, 4096, 16384, 65536]:
         nf = samples_per_station // n
         bpf = n * 2 // 8
         d1, d2 = F.DeviceBuffer(nf * bpf), F.DeviceBuffer(nf * bpf)
@@ -376,7 +377,7 @@ def main():
     ap.add_argument("--sizes", action="store_true",
                     help="also time both forms at FFT sizes 256..65536 (adds 'fft_sizes')")
     ap.add_argument("--phased", action="store_true",
-                    help="also time the phased-array path at FFT 1024/4096/16384 (adds 'phased')")
+                    help="also time the phased-array path at FFT 1024/4096/16384/65536 (adds 'phased')")
     ap.add_argument("--pmc-summary", nargs=3, metavar=("FETCH_DIR", "WRITE_DIR", "OUT_JSON"),
                     help="summarise two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of "
                          "'bench.py --copy-mib M' into OUT_JSON and exit")
