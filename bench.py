@@ -351,6 +351,113 @@ def phased_sweep(stream, samples_per_station=1 << 28, reps=3):
             "sizes": rows}
 
 
+def _uniform_levels(bits):
+    return np.arange(1 << bits, dtype=np.float64) - ((1 << bits) - 1) / 2.0
+
+
+def batch_bench(stream, scale=0.125, reps=3, seed=48):
+    """Batched uneven scans in one graph: 48 seeded scans, FFT sizes from
+    {256, 1024, 4096, 16384}, per-station bits from {1, 2, 4, 8}, sampling
+    rates from {128, 256, 512, 1024} Msps, lengths uniform in 0.5..20 s
+    scaled by `scale` (device-generated input), each with its own delay
+    model and clock terms; ACF+XCF of every scan, and 12 scans also phased
+    into an 8-bit series.  Per form, every ACF/XCF plan is autotuned on its
+    own scan, all sector calls are captured into one graph, and the graph is
+    replayed `reps` times after one warm-up replay; the phased plans run the
+    plain form in both."""
+    rng = np.random.default_rng(seed)
+    scans, total = [], 0
+    for i in range(48):
+        n = int(rng.choice([256, 1024, 4096, 16384]))
+        bits = [int(rng.choice([1, 2, 4, 8])), int(rng.choice([1, 2, 4, 8]))]
+        phased = i < 12
+        if phased and max(bits) < 8:
+            bits[int(rng.integers(2))] = 8
+        fs = float(rng.choice([128e6, 256e6, 512e6, 1024e6]))
+        nf = max(16, int(float(rng.uniform(0.5, 20.0)) * scale * fs / n))
+        delay = float(rng.uniform(-12.0, 12.0))
+        lo = float(rng.uniform(2e9, 9e9))
+        sh1 = rng.permutation(32).astype(np.int32) if rng.integers(2) else None
+        lsb2 = bool(rng.integers(2))
+        w = (float(rng.uniform(0.5, 1.5)), float(rng.uniform(0.5, 1.5)))
+        d1 = F.DeviceBuffer(nf * n * bits[0] // 8)
+        d2 = F.DeviceBuffer(nf * n * bits[1] // 8)
+        F.synth_station_device(d1.ptr, nf * n, bits[0], sh1, False, 500 + i, 0, 0.3, 3, stream.ptr)
+        F.synth_station_device(d2.ptr, nf * n, bits[1], None, lsb2, 500 + i, 1, 0.3, 3, stream.ptr)
+        cfg = F.make_delay_cfg(np.zeros(2 + int(nf * n / fs)), frame_dt=n / fs, fs=fs,
+                               delay_user_samples=delay,
+                               extra_delay_rate_sps=float(rng.uniform(-5e-9, 5e-9)),
+                               clock1_delay_s=float(rng.uniform(-2, 2)) / fs,
+                               clock1_rate_sps=float(rng.uniform(-1e-12, 1e-12)),
+                               clock2_delay_s=float(rng.uniform(-2, 2)) / fs,
+                               clock2_rate_sps=float(rng.uniform(-1e-12, 1e-12)),
+                               lo1_hz=lo, lo2_hz=lo)
+        scans.append(dict(n=n, bits=tuple(bits), fs=fs, nf=nf, d1=d1, d2=d2, cfg=cfg, sh1=sh1,
+                          lsb=(False, lsb2), phased=phased, w=w,
+                          output_lsb=bool(rng.integers(2))))
+        total += nf * n
+    stream.sync()
+    gbytes = sum(sc["d1"].nbytes + sc["d2"].nbytes for sc in scans) / 1e9
+    out = {"workload": f"48 scans (12 also phased, 8-bit series), FFT 256/1024/4096/16384, "
+                       f"1/2/4/8-bit, 128..1024 Msps, lengths 0.5..20 s x {scale}, "
+                       f"device-generated, one graph per form",
+           "samples_per_station": total, "input_gb": round(gbytes, 2), "unit": "Msamples/s/station"}
+    for form, prec in (("plain", F.PRECISION_PLAIN_FP32), ("chosen", F.PRECISION_CHOSEN)):
+        jobs = []
+        for sc in scans:
+            n, lv = sc["n"], [_uniform_levels(b) for b in sc["bits"]]
+            p = F.Plan(n, bits=sc["bits"], levels=lv, shuffles=(sc["sh1"], None), lsb=sc["lsb"],
+                       df_hz=sc["fs"] / n, products=F.PRODUCT_ACF | F.PRODUCT_XCF, precision=prec)
+            F.plan_autotune(p, sc["d1"].ptr, sc["d2"].ptr, sc["nf"], sc["cfg"], stream.ptr)
+            job = dict(sc=sc, plan=p, acc=F.DeviceBuffer(F.N_PRODUCTS * (n // 2 + 1) * 8))
+            if sc["phased"]:
+                pp = F.Plan(n, bits=sc["bits"], levels=lv, shuffles=(sc["sh1"], None),
+                            lsb=sc["lsb"], df_hz=sc["fs"] / n,
+                            products=F.PRODUCT_PHASED | F.PRODUCT_RAW_OUT, w=sc["w"],
+                            bit_out=max(sc["bits"]), output_lsb=sc["output_lsb"])
+                job.update(pplan=pp, pacc=F.DeviceBuffer(F.N_PRODUCTS * (n // 2 + 1) * 8),
+                           praw=F.DeviceBuffer(sc["nf"] * pp.info.bpf_out))
+            jobs.append(job)
+
+        def enqueue_all():
+            for j in jobs:
+                sc = j["sc"]
+                j["plan"].sector_run_device(sc["d1"].ptr, sc["d2"].ptr, 0, sc["nf"], sc["cfg"],
+                                            j["acc"].ptr, stream=stream.ptr)
+                if "pplan" in j:
+                    j["pplan"].sector_run_device(sc["d1"].ptr, sc["d2"].ptr, 0, sc["nf"], sc["cfg"],
+                                                 j["pacc"].ptr, j["praw"].ptr, stream=stream.ptr)
+
+        enqueue_all()                       # eager: delay tables, scratch sizes
+        stream.sync()
+        g = F.Graph(stream)
+        enqueue_all()
+        g.end()
+        g.launch()
+        stream.sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            g.launch()
+        stream.sync()
+        el = (time.perf_counter() - t0) / reps
+        out[form] = {"value": round(total / el / 1e6, 1), "ms_per_batch": round(el * 1e3, 3)}
+        print(f"[bench] batch {form}: {total / el / 1e6:,.0f} Msamples/s/station "
+              f"({el * 1e3:.1f} ms per 48-scan graph)", file=sys.stderr, flush=True)
+        g.close()
+        for j in jobs:
+            for k in ("acc", "pacc", "praw"):
+                if k in j:
+                    j[k].free()
+            j["plan"].close()
+            if "pplan" in j:
+                j["pplan"].close()
+    for sc in scans:
+        sc["d1"].free()
+        sc["d2"].free()
+    out["chosen_over_plain"] = round(out["chosen"]["value"] / out["plain"]["value"], 3)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -378,6 +485,10 @@ def main():
                     help="also time both forms at FFT sizes 256..65536 (adds 'fft_sizes')")
     ap.add_argument("--phased", action="store_true",
                     help="also time the phased-array path at FFT 1024/4096/16384/65536 (adds 'phased')")
+    ap.add_argument("--batch", action="store_true",
+                    help="also time the 48-scan uneven batch captured in one graph (adds 'batch')")
+    ap.add_argument("--batch-scale", type=float, default=0.125,
+                    help="scan lengths (0.5..20 s) are scaled by this factor")
     ap.add_argument("--pmc-summary", nargs=3, metavar=("FETCH_DIR", "WRITE_DIR", "OUT_JSON"),
                     help="summarise two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of "
                          "'bench.py --copy-mib M' into OUT_JSON and exit")
@@ -579,6 +690,7 @@ def main():
 
     sizes = size_sweep(stream) if args.sizes else None
     phased = phased_sweep(stream) if args.phased else None
+    batch = batch_bench(stream, args.batch_scale) if args.batch else None
 
     r = results[main_form]
     line = {
@@ -618,6 +730,8 @@ def main():
         line["fft_sizes"] = sizes
     if phased is not None:
         line["phased"] = phased
+    if batch is not None:
+        line["batch"] = batch
     if "plain" in results and "chosen" in results:
         line["chosen_over_plain"] = round(results["chosen"]["value"] / results["plain"]["value"], 3)
     print(json.dumps(line), flush=True)

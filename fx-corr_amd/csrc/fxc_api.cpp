@@ -461,6 +461,9 @@ static void fill_args(fxc_plan *p, SectorArgs &a) {
     for (int k = 0; k < 32; ++k) a.out_shifts[k] = p->desc.st[0].shuffle_internal[k];
     a.out_levels = (const float *)p->out_lev.p;
     a.out_nlevels = p->desc.st[0].nlevels;
+    a.out_sorted = 1;
+    for (int k = 1; k < (int)p->levels[0].size(); ++k)
+        if (!((float)p->levels[0][k - 1] <= (float)p->levels[0][k])) a.out_sorted = 0;
     a.bpf_out = (int32_t)(((int64_t)p->N * p->desc.bit_out + 7) / 8);
     uint32_t fl = 0;
     if (p->desc.products & FXC_PRODUCT_ACF) fl |= FLAG_ACF;

@@ -455,6 +455,39 @@ __device__ __forceinline__ uint32_t quant_code(float x, const float *lv, int nl)
     return best;
 }
 
+// The same code for non-decreasing levels, by bisection.  |fl(x - l_k)| is
+// non-increasing in k up to j, the last level <= x, and non-decreasing after
+// it (rounding is monotone), so the scan's first minimum is j + 1 when that
+// error is strictly smaller, else the first of the run of errors equal to
+// level j's.  As in the scan, an error that does not compare below FLT_MAX
+// (x NaN or infinite) leaves code 0.
+__device__ __forceinline__ uint32_t quant_code_sorted(float x, const float *lv, int nl) {
+    if (!(x == x)) return 0u;
+    int lo = -1, hi = nl;   // lv[lo] <= x < lv[hi] (lo = -1, hi = nl: none)
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (lv[mid] <= x) lo = mid;
+        else hi = mid;
+    }
+    int best;
+    float eb;
+    if (lo < 0) {
+        best = 0;
+        eb = fabsf(x - lv[0]);
+    } else {
+        best = lo;
+        eb = fabsf(x - lv[lo]);
+        const float eh = hi < nl ? fabsf(x - lv[hi]) : eb;
+        if (eh < eb) {
+            best = hi;
+            eb = eh;
+        } else {
+            while (best > 0 && fabsf(x - lv[best - 1]) == eb) --best;
+        }
+    }
+    return eb < 3.40282347e+38f ? (uint32_t)best : 0u;
+}
+
 __device__ __forceinline__ uint32_t shuffle_out_word(uint32_t w, const int32_t *sh) {
     uint32_t o = 0;
 #pragma unroll
@@ -464,12 +497,13 @@ __device__ __forceinline__ uint32_t shuffle_out_word(uint32_t w, const int32_t *
 
 // One thread per output word of one frame.
 __device__ __forceinline__ uint32_t quant_word(const float *x, int N, int w, int bits,
-                                               const float *lv, int nl, uint32_t mask) {
+                                               const float *lv, int nl, uint32_t mask, bool sorted) {
     uint64_t acc = 0;
     const int64_t b0 = (int64_t)w * 32, b1 = b0 + 32;
     int64_t s = b0 / bits;
     for (; s * bits < b1 && s < N; ++s) {
-        uint64_t code = (uint64_t)(quant_code(x[s], lv, nl) & mask);
+        const uint32_t q = sorted ? quant_code_sorted(x[s], lv, nl) : quant_code(x[s], lv, nl);
+        uint64_t code = (uint64_t)(q & mask);
         int64_t pos = s * bits - b0;  // may be negative for the straddling first sample
         if (pos >= 0) acc |= code << pos;
         else acc |= code >> (-pos);
@@ -486,7 +520,8 @@ __global__ void k_quantise_frames(SectorArgs a, const float *in, int64_t nframes
     const uint32_t mask = (a.bit_out >= 32) ? 0xFFFFFFFFu : ((1u << a.bit_out) - 1u);
     const int64_t f = blockIdx.x;
     for (int w = threadIdx.x; w < nwo; w += blockDim.x) {
-        uint32_t word = quant_word(in + f * a.N, a.N, w, a.bit_out, lv, nl < 256 ? nl : 256, mask);
+        uint32_t word = quant_word(in + f * a.N, a.N, w, a.bit_out, lv, nl < 256 ? nl : 256, mask,
+                                   a.out_sorted != 0);
         uint32_t sh = shuffle_out_word(word, a.out_shifts);
         reinterpret_cast<uint32_t *>(out + f * a.bpf_out)[w] = sh;
     }

@@ -72,6 +72,7 @@ struct SectorArgs {
     int32_t out_shifts[32];  // station-1 shuffle used on output words
     const float *out_levels; // station-1 levels (f32 as the reference compares)
     int32_t out_nlevels;
+    int32_t out_sorted;      // 1: out_levels non-decreasing (nearest level by bisection)
     int32_t bpf_out;
     uint8_t *raw_out;        // phased .raw frames (nf * bpf_out) or null
     double *part;            // [n_wg][kNumProducts][half]

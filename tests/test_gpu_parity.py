@@ -161,6 +161,35 @@ def test_quantise_bit_exact(bits):
     p.close()
 
 
+@pytest.mark.parametrize("kind", ["uniform", "gaussian", "duplicates", "unsorted"])
+def test_quantise_nearest_level_edges(kind):
+    """Ascending levels take the bisection path, others the linear scan; both
+    give the reference's first minimum of |x - level| (f32) bit for bit at
+    midpoints (ties go to the lower index), at the levels, one ulp either
+    side of the midpoints, beyond the outer levels and for NaN / +-inf."""
+    rng = np.random.default_rng(len(kind))
+    bits, n = 8, 1024
+    lv = {"uniform": uniform_levels(8) * 0.5,
+          "gaussian": np.sort(rng.standard_normal(256)) * 3.0,
+          "duplicates": np.repeat(np.sort(rng.standard_normal(64)), 4),
+          "unsorted": rng.standard_normal(256) * 2.0}[kind]
+    lv32 = lv.astype(np.float32)
+    srt = np.sort(lv32).astype(np.float64)
+    mids = ((srt[:-1] + srt[1:]) / 2).astype(np.float32)
+    pool = np.concatenate([mids, lv32, np.nextafter(mids, np.float32(np.inf)),
+                           np.nextafter(mids, np.float32(-np.inf)),
+                           np.float32([np.nan, np.inf, -np.inf, 1e30, -1e30, 0.0, -0.0]),
+                           (rng.standard_normal(4000) * 4).astype(np.float32)])
+    x = rng.choice(pool, size=(6, n)).astype(np.float32)
+    sh = np.arange(32, dtype=np.int32)                  # identity output bit map
+    p = F.Plan(n, bits=(bits, bits), levels=[lv, lv], shuffles=(sh, None), df_hz=1.0,
+               products=F.PRODUCT_ACF, bit_out=bits)
+    got = p.quantise_frames(x)
+    for f in range(6):
+        assert np.array_equal(got[f], R.quantise_frame(x[f], bits, lv, sh)), f
+    p.close()
+
+
 # ------------------------------------------------------------- sector MAC
 def _compare(got, ref, tol, keys=("acc11", "acc22", "acc12")):
     # ACF: per-bin relative error.  XCF: per-bin relative on bins >= 0.1x the
