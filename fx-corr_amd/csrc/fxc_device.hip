@@ -211,6 +211,9 @@ static void shape_for(int N, int &E, int &T) {
     } else if (M == 256) {
         E = 4;
         T = 64;
+    } else if (M == 128) {   // one wave per frame pair: radix-4 passes (129k vs 101k
+        E = 4;               // Msamples/s/station with two waves of radix-2 passes)
+        T = 32;
     } else {
         E = 2;
         T = M / 2;
@@ -249,7 +252,7 @@ hipError_t launch_sector(const SectorArgs &a, int precision, int n_wg, hipStream
     switch (a.N) {
     case 64: return launch_sector_t<2, 16>(a, precision, n_wg, s);
     case 128: return launch_sector_t<2, 32>(a, precision, n_wg, s);
-    case 256: return launch_sector_t<2, 64>(a, precision, n_wg, s);
+    case 256: return launch_sector_t<4, 32>(a, precision, n_wg, s);
     case 512: return launch_sector_t<4, 64>(a, precision, n_wg, s);
     case 1024: return launch_sector_t<8, 64>(a, precision, n_wg, s);
     case 2048: return v16 ? launch_sector_t<16, 64>(a, precision, n_wg, s)
@@ -401,7 +404,7 @@ static hipError_t launch_fft(const SectorArgs &a, const void *in, int64_t nframe
     switch (a.N) {
     case 64: return launch_fft_t<2, 16>(a, in, nframes, out, mode, s);
     case 128: return launch_fft_t<2, 32>(a, in, nframes, out, mode, s);
-    case 256: return launch_fft_t<2, 64>(a, in, nframes, out, mode, s);
+    case 256: return launch_fft_t<4, 32>(a, in, nframes, out, mode, s);
     case 512: return launch_fft_t<4, 64>(a, in, nframes, out, mode, s);
     case 1024: return launch_fft_t<8, 64>(a, in, nframes, out, mode, s);
     case 2048: return launch_fft_t<8, 128>(a, in, nframes, out, mode, s);

@@ -148,7 +148,7 @@ hipError_t launch_phased(const SectorArgs &a, int accumulate, int n_wg, hipStrea
     switch (a.N) {
     case 64: return launch_phased_t<2, 16>(a, accumulate, n_wg, s);
     case 128: return launch_phased_t<2, 32>(a, accumulate, n_wg, s);
-    case 256: return launch_phased_t<2, 64>(a, accumulate, n_wg, s);
+    case 256: return launch_phased_t<4, 32>(a, accumulate, n_wg, s);
     case 512: return launch_phased_t<4, 64>(a, accumulate, n_wg, s);
     case 1024: return launch_phased_t<8, 64>(a, accumulate, n_wg, s);
     case 2048: return launch_phased_t<8, 128>(a, accumulate, n_wg, s);
