@@ -637,6 +637,37 @@ def test_phased_vs_oracle(n, bits, grid, band_shift, lsb_out):
     assert diff <= 1e-4 * n * len(range(0, c["nf"], 7)) + 2, diff
 
 
+@pytest.mark.parametrize("nf,chunk", [(5, 0), (23, 7)])
+def test_phased_large_chunks_and_few_frames(nf, chunk):
+    """N = 32768 phased sum with fewer frames than frame groups, and over
+    several scratch chunks (partials accumulated in order, rotation chains
+    computed for whole chunks), against the oracle's phased branch."""
+    n, fs, bits = 32768, 256e6, (2, 4)
+    band = (16, 0, n // 2 - 16, 16)
+    c = make_case(n=n, fs=fs, nf=nf, bits=bits, lsb=(True, False), band=band, rho=0.4,
+                  delay_samples=-2.7, seed=61 + nf, delay_user_samples=-2.7, lo1_hz=4e9,
+                  lo2_hz=4e9, products=F.PRODUCT_PHASED | F.PRODUCT_RAW_OUT)
+    w = (1.1, 0.8)
+    p = F.Plan(n, bits=bits, levels=c["levels"], shuffles=c["shuffles"], lsb=c["lsb"], band=band,
+               df_hz=fs / n, products=F.PRODUCT_PHASED | F.PRODUCT_RAW_OUT, w=w, bit_out=4)
+    if chunk:
+        F.plan_set_tuning(p, split_m1=128, chunk_frames=chunk)
+    cfg = F.make_delay_cfg(c["table"], **c["delay"])
+    got = p.sector_run(c["raw1"], c["raw2"], 0, nf, cfg)
+    ref = R.sector_run(_phased_desc(c, w, 4, False), c["raw1"], c["raw2"], 0, nf,
+                       R.make_delay_cfg(c["table"], **c["delay"]), mode=R.MODE_F32, threads=8)
+    assert rel_err(got["acc_ph"], ref["acc_ph"]) < PLAIN_TOL
+    dp = out_decode_plan(4, c["shuffles"][0], c["levels"][0])
+    bpf_o = n * 4 // 8
+    diff = 0
+    for f in range(nf):
+        seg = slice(f * bpf_o, (f + 1) * bpf_o)
+        _, vg = dp.decode(got["raw_out"][seg], n, False)
+        _, vr = dp.decode(ref["raw_out"][seg], n, False)
+        diff += int(np.count_nonzero(vg != vr))
+    assert diff <= 1e-4 * n * nf + 2, diff
+
+
 @pytest.mark.parametrize("n,nf", [(1024, 700), (32768, 40)])
 def test_phased_no_raw_out_matches_with_raw_out(n, nf):
     fs = 64e6
