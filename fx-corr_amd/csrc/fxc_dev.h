@@ -143,11 +143,16 @@ __device__ void compute_frame_delay(int64_t idx, const DelayDev &c, FrameDelayDe
 
 // Per-frame values every thread of the workgroup needs, computed once per
 // frame (one thread per frame) and broadcast through LDS.
+// x reduced to [-1/2, 1/2] turns
+__device__ __forceinline__ double wrap_turns(double x) { return x - rint(x); }
+
 struct FrameParam {
     int32_t s1, s2;      // integer shifts clamped to [-N, N]
     float2 frmix;        // fr_lo1 * conj(fr_lo2) in f32 (plain form)
-    cd start;            // XCF phasor at overlap index 0
-    cd pw[kMaxPow];      // XCF phase step^(2^b)
+    cd start;            // XCF phasor at overlap index 0 (plain form)
+    cd pw[kMaxPow];      // XCF phase step^(2^b) (plain form)
+    double phw, dth;     // chosen form: XCF phase at overlap index 0 (fringe
+                         // difference included) and its step per index, turns
 };
 
 __device__ __forceinline__ int32_t clamp_shift(int64_t s, int n) {
@@ -157,10 +162,23 @@ __device__ __forceinline__ int32_t clamp_shift(int64_t s, int n) {
 template <int PREC>
 __device__ void make_frame_param(int64_t idx, const SectorArgs &a, int logm, FrameParam &p) {
     FrameDelayDev d;
-    cd fr1, fr2;
-    compute_frame_delay(idx, a.dly, d, &fr1, &fr2);
+    compute_frame_delay(idx, a.dly, d, nullptr, nullptr);
     p.s1 = clamp_shift(d.int1, a.N);
     p.s2 = clamp_shift(d.int2, a.N);
+    if (PREC != 0) {
+        // chosen form: main.rs:133-149 and the fringe difference restated in
+        // FP64 turns: w1 - w2 (-lo tau reduced mod 1) plus
+        // -df (frac1 (a1s + k) - frac2 (a2s + k)) at overlap index k
+        double tau1, tau2, tf1, tf2;
+        frame_taus(idx, a.dly, tau1, tau2, tf1, tf2);
+        const double w1 = a.dly.lo1 == 0.0 ? 0.0 : wrap_turns(-a.dly.lo1 * tf1);
+        const double w2 = a.dly.lo2 == 0.0 ? 0.0 : wrap_turns(-a.dly.lo2 * tf2);
+        p.frmix = make_float2(1.0f, 0.0f);
+        p.start = cd{1.0, 0.0};
+        p.dth = -a.df * (d.frac1 - d.frac2);
+        p.phw = wrap_turns(w1 - w2 - a.df * (d.frac1 * (double)a.a1s - d.frac2 * (double)a.a2s));
+        return;
+    }
     cd step;
     {
 #pragma clang fp contract(off)
@@ -170,21 +188,16 @@ __device__ void make_frame_param(int64_t idx, const SectorArgs &a, int logm, Fra
         double step2 = two_pi_df * d.frac2;
         double ang0 = step1 * (double)a.a1s - step2 * (double)a.a2s;
         double angs = step1 - step2;
-        if (PREC == 0) {
-            float2 f1 = make_float2(d.fr1[0], d.fr1[1]);
-            float2 f2 = make_float2(d.fr2[0], d.fr2[1]);
-            p.frmix = c_mul(f1, c_conj(f2));
-            float2 st = polar1_f32((float)ang0);
-            float2 sp = polar1_f32((float)angs);
-            p.start = cd{(double)st.x, (double)st.y};
-            step = cd{(double)sp.x, (double)sp.y};
-        } else {
-            cd mix = d_mul(fr1, cd{fr2.x, -fr2.y});
-            p.frmix = make_float2(1.0f, 0.0f);
-            p.start = d_mul(mix, cd{cos(ang0), sin(ang0)});
-            step = cd{cos(angs), sin(angs)};
-        }
+        float2 f1 = make_float2(d.fr1[0], d.fr1[1]);
+        float2 f2 = make_float2(d.fr2[0], d.fr2[1]);
+        p.frmix = c_mul(f1, c_conj(f2));
+        float2 st = polar1_f32((float)ang0);
+        float2 sp = polar1_f32((float)angs);
+        p.start = cd{(double)st.x, (double)st.y};
+        step = cd{(double)sp.x, (double)sp.y};
     }
+    p.phw = 0.0;
+    p.dth = 0.0;
     p.pw[0] = step;
     for (int b = 1; b < logm; ++b) p.pw[b] = d_mul(p.pw[b - 1], p.pw[b - 1]);
 }

@@ -96,6 +96,7 @@ __global__ __launch_bounds__(2 * T) void k_sector(SectorArgs a) {
             const float2 frmix = fp[jf].frmix;
             bool have = false;
             cd P = cd{0.0, 0.0};
+            float2 Pf = make_float2(0.f, 0.f), Ff = Pf;
 #pragma unroll
             for (int i = 0; i < NB; ++i) {
                 const int g = tid + NT * i;
@@ -117,6 +118,21 @@ __global__ __launch_bounds__(2 * T) void k_sector(SectorArgs a) {
                         float2 ph;
                         if (PREC == 0 && a.chain != nullptr) {
                             ph = a.chain[f * a.chain_ld + k];     // the reference's own chain
+                        } else if (PREC != 0) {
+                            // chosen form: the first overlap bin's phasor from
+                            // the FP64 phase in turns (reduced mod 1, then the
+                            // f32 sincos), the thread's later bins (k + NT)
+                            // by FP32 steps of exp(2 pi i NT dth)
+                            if (!have) {
+                                const float r = (float)wrap_turns(fma((double)k, fp[jf].dth, fp[jf].phw));
+                                Pf = make_float2(__builtin_amdgcn_cosf(r), __builtin_amdgcn_sinf(r));
+                                const float rs = (float)wrap_turns((double)NT * fp[jf].dth);
+                                Ff = make_float2(__builtin_amdgcn_cosf(rs), __builtin_amdgcn_sinf(rs));
+                                have = true;
+                            } else {
+                                Pf = c_mul(Pf, Ff);
+                            }
+                            ph = Pf;
                         } else {
                             if (!have) {
                                 P = fp[jf].start;

@@ -231,6 +231,10 @@ __global__ __launch_bounds__(256) void k_mac_large(SectorArgs a, const float2 *z
             float2 phf;
             if (PREC == 0 && a.chain != nullptr) {
                 phf = a.chain[f * a.chain_ld + kk];            // the reference's own chain
+            } else if (PREC != 0) {
+                // chosen form: the FP64 phase in turns, reduced mod 1, then the f32 sincos
+                const float r = (float)wrap_turns(fma((double)kk, P.dth, P.phw));
+                phf = make_float2(__builtin_amdgcn_cosf(r), __builtin_amdgcn_sinf(r));
             } else {
                 cd ph = P.start;
                 for (int b = 0; b < kMaxPow; ++b)

@@ -90,7 +90,6 @@ __device__ __forceinline__ float2 sincos_turns(float r) {
     return make_float2(co, si);
 }
 
-__device__ __forceinline__ double wrap_turns(double x) { return x - rint(x); }
 
 // Per-frame values of the chosen form (main.rs:184-241, 133-149 restated in
 // turns): integer shifts, the fringe difference phw = w1 - w2 and the
