@@ -536,4 +536,57 @@ __device__ __forceinline__ float2 decode_elem(const uint32_t *words, const Stati
     return make_float2(in0 ? x0 : 0.0f, in1 ? x1 : 0.0f);
 }
 
+
+// E consecutive elements m0 .. m0 + E - 1 (samples 2 m0 .. 2 m0 + 2E - 1 of
+// the shifted frame) of one station, with decode_elem's values: the run's
+// words are read once and funnel-shifted into a bit stream whose codes sit
+// at fixed offsets (BITS = 1, 2, 4, 8); samples outside the frame are zero,
+// and with the LSB flip the samples of odd index in the unshifted frame are
+// negated.  Per-sample range checks run only for runs that cross an edge.
+template <int E, int BITS>
+__device__ __forceinline__ void decode_run_t(const uint32_t *words, const StationDev &st,
+                                             const float *lev_lds, int m0, int shift, int N,
+                                             float2 (&v)[E]) {
+    constexpr int NS = 2 * E, NW = (NS * BITS + 31) / 32;
+    const int s0 = 2 * m0 - shift;                 // index of the run's first sample
+    const int bp = s0 * BITS;                      // may be negative (arithmetic shift floors)
+    const int w0 = bp >> 5;
+    const uint32_t off = (uint32_t)bp & 31u;
+    uint32_t w[NW + 1];
+#pragma unroll
+    for (int i = 0; i <= NW; ++i) {
+        const int wi = w0 + i;
+        w[i] = (wi >= 0 && wi < st.nwords) ? words[wi] : 0u;
+    }
+    uint32_t d[NW];
+#pragma unroll
+    for (int i = 0; i < NW; ++i) d[i] = __builtin_amdgcn_alignbit(w[i + 1], w[i], off);
+    const bool inside = s0 >= 0 && s0 + NS <= N;
+    const bool flip = st.lsb != 0;
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+        const uint32_t code = (d[(j * BITS) >> 5] >> ((j * BITS) & 31)) & st.code_mask;
+        float x = level_of(code, st, lev_lds);
+        if (flip && ((s0 + j) & 1)) x = -x;
+        if (!inside && (unsigned)(s0 + j) >= (unsigned)N) x = 0.0f;
+        if (j & 1) v[j >> 1].y = x;
+        else v[j >> 1].x = x;
+    }
+}
+
+template <int E>
+__device__ __forceinline__ void decode_run(const uint32_t *words, const StationDev &st,
+                                           const float *lev_lds, int m0, int shift, int N,
+                                           float2 (&v)[E]) {
+    switch (st.bits) {
+    case 1: decode_run_t<E, 1>(words, st, lev_lds, m0, shift, N, v); break;
+    case 2: decode_run_t<E, 2>(words, st, lev_lds, m0, shift, N, v); break;
+    case 4: decode_run_t<E, 4>(words, st, lev_lds, m0, shift, N, v); break;
+    case 8: decode_run_t<E, 8>(words, st, lev_lds, m0, shift, N, v); break;
+    default:
+#pragma unroll
+        for (int r = 0; r < E; ++r) v[r] = decode_elem(words, st, lev_lds, m0 + r, shift, N);
+    }
+}
+
 }  // namespace fxc

@@ -16,8 +16,12 @@ namespace fxc {
 // thread owns output-grid bins g = tid + 2T*i for the products.
 // MF (chosen form, levels exact in FP16): the FFT's first pass (radix 32)
 // runs as FP16 hi/lo MFMA GEMMs, the remaining Stockham passes in FP32.
+// The MF instance at E = 8, T = 256 is held to 128 VGPRs (four waves per
+// SIMD, two 512-thread workgroups per CU); unconstrained it takes 130 and
+// one workgroup per CU.
 template <int E, int T, int PREC, bool MF = false>
-__global__ __launch_bounds__(2 * T) void k_sector(SectorArgs a) {
+__global__ __launch_bounds__(2 * T)
+__attribute__((amdgpu_waves_per_eu(MF && E == 8 && T == 256 ? 4 : 1))) void k_sector(SectorArgs a) {
     constexpr int M = E * T;
     constexpr int MP = M + M / 16 + 2;
     constexpr int NT = 2 * T;
@@ -79,16 +83,20 @@ __global__ __launch_bounds__(2 * T) void k_sector(SectorArgs a) {
             __syncthreads();
             const int sh = s ? fp[jf].s2 : fp[jf].s1;
             float2 v[E];
-#pragma unroll
-            for (int r = 0; r < E; ++r) v[r] = decode_elem(ws, st, lev[s], t + T * r, sh, N);
-            __syncthreads();
             if constexpr (MF) {
+                // the frame goes to LDS before the first pass: each thread
+                // decodes a run of E consecutive elements
+                decode_run<E>(ws, st, lev[s], t * E, sh, N, v);
+                __syncthreads();
 #pragma unroll
-                for (int r = 0; r < E; ++r) zs[pad(t + T * r)] = v[r];
+                for (int r = 0; r < E; ++r) zs[pad(t * E + r)] = v[r];
                 __syncthreads();
                 stockham_first_mfma<E, T>(zs, b1, t);
                 stockham_rest<E, T, 32>(zs, a.tw_fft, t);
             } else {
+#pragma unroll
+                for (int r = 0; r < E; ++r) v[r] = decode_elem(ws, st, lev[s], t + T * r, sh, N);
+                __syncthreads();
                 fft_forward<E, T>(zs, v, a.tw_fft, t);
             }
 

@@ -309,6 +309,23 @@ def test_analytic_delay_branch_and_large_shifts():
     _compare(gpu_run(c), oracle_run(c), PLAIN_TOL)
 
 
+@pytest.mark.parametrize("bits", [(1, 8), (4, 2), (2, 2)])
+def test_stockham_mfma_large_and_odd_shifts(bits):
+    """Chosen form on the Stockham kernel (run decode + MFMA first pass) with
+    the quadratic delay model sweeping the integer shift past -N and +N:
+    odd shifts, runs crossing the frame edges, frames shifted wholly out."""
+    n, fs = 4096, 256e6
+    c = make_case(n=n, fs=fs, nf=400, bits=bits, lsb=(True, False), no_table=True,
+                  net_d1_base=-20.0e-6, total_rate1_base=6.5e-3, total_accel1_base=2e-3,
+                  lo1_hz=3e9, seed=9 + bits[0], rho=0.4)
+    d = R.make_delay_cfg(None, **c["delay"])
+    ints = [R.frame_delay(i, d).int1 for i in range(c["nf"])]
+    assert min(ints) < -n and max(ints) > n and any(i % 2 for i in ints)
+    p = gpu_plan(c, F.PRECISION_CHOSEN)
+    assert p.info.fft_form == 0 and p.info.fft_mfma == 1
+    _compare(gpu_run(c, plan=p), oracle_run(c, R.MODE_F64), CHOSEN_TOL)
+
+
 def test_device_entry_matches_host_entry_and_is_deterministic():
     c = make_case(n=1024, fs=64e6, nf=3000, seed=8, delay_user_samples=4.2, lo1_hz=1e9)
     p = gpu_plan(c)
