@@ -1123,8 +1123,8 @@ static fxc_status enqueue_phased_large(fxc_plan *p, const SectorArgs &base, int6
         int64_t *shifts = (int64_t *)p->d_lshifts.p;
         HIP_TRY(launch_phased_params(a, p->d_params.p, shifts, s));
         for (int st = 0; st < 2; ++st) {
-            HIP_TRY(launch_decode_frames(a, st, shifts + st, 2, (float *)p->d_x[st].p, s));
-            HIP_TRY(launch_fft_large((const float *)p->d_x[st].p, (float2 *)p->d_y.p,
+            HIP_TRY(launch_decode_frames(a, st, shifts + st, 2, p->d_x[st].p, false, s));
+            HIP_TRY(launch_fft_large(p->d_x[st].p, false, (float2 *)p->d_y.p,
                                      (float2 *)p->d_z[st].p, cn, p->M1, p->M2, twl1, twl2, twm,
                                      nullptr, s));
         }
@@ -1181,8 +1181,10 @@ static fxc_status enqueue_large(fxc_plan *p, const SectorArgs &base, int64_t nf,
         int64_t *shifts = (int64_t *)p->d_lshifts.p;
         HIP_TRY(launch_frame_params(a, p->desc.precision, params, shifts, s));
         for (int st = 0; st < 2; ++st) {
-            HIP_TRY(launch_decode_frames(a, st, shifts + st, 2, (float *)p->d_x[st].p, s));
-            HIP_TRY(launch_fft_large((const float *)p->d_x[st].p, (float2 *)p->d_y.p,
+            // chosen form with the MFMA first pass: the decode writes FP16 pairs
+            // (exact for these levels), half the bytes of FP32
+            HIP_TRY(launch_decode_frames(a, st, shifts + st, 2, p->d_x[st].p, p->mf_b1, s));
+            HIP_TRY(launch_fft_large(p->d_x[st].p, p->mf_b1, (float2 *)p->d_y.p,
                                      (float2 *)p->d_z[st].p, cn, p->M1, p->M2, twl1, twl2,
                                      (const float2 *)p->tw_lm.p,
                                      p->mf_b1 ? p->mf_tab.p : nullptr, s));
@@ -1580,7 +1582,7 @@ extern "C" fxc_status fxc_decode_frames(fxc_plan *p, int32_t station, const uint
     a.nf = nframes;
     if (station == 0) a.raw1 = (const uint8_t *)p->d_tmp_in.p;
     else a.raw2 = (const uint8_t *)p->d_tmp_in.p;
-    HIP_TRY(launch_decode_frames(a, station, dsh, 1, (float *)p->d_tmp_out.p, s));
+    HIP_TRY(launch_decode_frames(a, station, dsh, 1, p->d_tmp_out.p, false, s));
     HIP_TRY(hipMemcpyAsync(out, p->d_tmp_out.p, out_bytes, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     return FXC_OK;
@@ -1612,7 +1614,7 @@ static fxc_status run_fft(fxc_plan *p, const float *in, int64_t nframes, float *
     } else if (p->fft_form == 2) {
         for (int64_t c0 = 0; c0 < nframes; c0 += p->chunk) {   // through the chunk scratch
             const int64_t cn = std::min(p->chunk, nframes - c0);
-            HIP_TRY(launch_fft_large((const float *)p->d_tmp_in.p + c0 * p->N, (float2 *)p->d_y.p,
+            HIP_TRY(launch_fft_large((const float *)p->d_tmp_in.p + c0 * p->N, false, (float2 *)p->d_y.p,
                                      (float2 *)p->d_z[0].p, cn, p->M1, p->M2,
                                      (const float2 *)(p->M1 == 128 ? p->tw_l1.p : p->tw_l2.p),
                                      (const float2 *)(p->M2 == 128 ? p->tw_l1.p : p->tw_l2.p),

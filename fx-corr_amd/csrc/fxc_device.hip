@@ -294,11 +294,13 @@ bool sector_has_variant(int N) { return N >= 2048 && N <= 8192; }
 
 // ------------------------------------------ per-function entry kernels
 // Decode (+ shift) of one frame per workgroup into out[f*N + n], through the
-// fused kernel's staging and element decode.
-template <int T>
+// fused kernel's staging and element decode.  H: each element (x[2m],
+// x[2m+1]) as an FP16 pair in one dword (exact for levels that are exact in
+// FP16), else as two floats.
+template <int T, bool H>
 __global__ __launch_bounds__(T) void k_decode_frames(SectorArgs a, int station,
                                                      const int64_t *shifts, int shift_stride,
-                                                     float *out) {
+                                                     void *out) {
     extern __shared__ uint32_t words[];   // st.nwords
     __shared__ float lev[256];
     const StationDev &st = a.st[station];
@@ -311,25 +313,35 @@ __global__ __launch_bounds__(T) void k_decode_frames(SectorArgs a, int station,
     __syncthreads();
     const int s = shifts ? clamp_shift(shifts[f * shift_stride], a.N) : 0;
     for (int m = t; m < a.M; m += T) {
-        float2 v = decode_elem(words, st, lev, m, s, a.N);
-        out[f * a.N + 2 * m] = v.x;
-        out[f * a.N + 2 * m + 1] = v.y;
+        const float2 v = decode_elem(words, st, lev, m, s, a.N);
+        if constexpr (H)
+            reinterpret_cast<uint32_t *>(out)[f * a.M + m] =
+                __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pkrtz(v.x, v.y));
+        else
+            reinterpret_cast<float2 *>(out)[f * a.M + m] = v;
     }
 }
 
-hipError_t launch_decode_frames(const SectorArgs &a, int station, const int64_t *shifts,
-                                int shift_stride, float *out, hipStream_t s) {
-    if (a.nf <= 0) return hipSuccess;
+template <bool H>
+static hipError_t launch_decode_frames_t(const SectorArgs &a, int station, const int64_t *shifts,
+                                         int shift_stride, void *out, hipStream_t s) {
     const size_t lds = (size_t)a.st[station].nwords * 4;
     if (lds > 96 * 1024) return hipErrorInvalidValue;
     if (lds > 48 * 1024) {   // above the default dynamic-LDS limit (8-bit samples, N = 65536)
-        hipError_t e = hipFuncSetAttribute((const void *)k_decode_frames<256>,
+        hipError_t e = hipFuncSetAttribute((const void *)k_decode_frames<256, H>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(k_decode_frames<256>, dim3((unsigned)a.nf), dim3(256), lds, s, a, station,
-                       shifts, shift_stride, out);
+    hipLaunchKernelGGL((k_decode_frames<256, H>), dim3((unsigned)a.nf), dim3(256), lds, s, a,
+                       station, shifts, shift_stride, out);
     return hipGetLastError();
+}
+
+hipError_t launch_decode_frames(const SectorArgs &a, int station, const int64_t *shifts,
+                                int shift_stride, void *out, bool fp16, hipStream_t s) {
+    if (a.nf <= 0) return hipSuccess;
+    return fp16 ? launch_decode_frames_t<true>(a, station, shifts, shift_stride, out, s)
+                : launch_decode_frames_t<false>(a, station, shifts, shift_stride, out, s);
 }
 
 // Forward R2C with the fused kernel's FFT code: each half of the workgroup

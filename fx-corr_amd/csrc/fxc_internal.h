@@ -124,8 +124,9 @@ int sector_threads_per_frame(int N);
 bool sector_supported(int N);
 bool sector_has_variant(int N);
 
+// out: nframes x N floats, or with fp16 nframes x N/2 FP16 pairs (dwords)
 hipError_t launch_decode_frames(const SectorArgs &a, int station,
-                                const int64_t *shifts, int shift_stride, float *out,
+                                const int64_t *shifts, int shift_stride, void *out, bool fp16,
                                 hipStream_t stream);
 hipError_t launch_rfft_frames(const SectorArgs &a, const float *in, int64_t nframes,
                               float2 *out, hipStream_t stream);
@@ -160,11 +161,12 @@ hipError_t launch_frame_params(const SectorArgs &a, int precision, FrameParam *p
                                int64_t *shifts, hipStream_t stream);
 // mf_b1: the FP16 hi/lo DFT-32 operand table for a first pass on MFMA
 // (chosen form, inputs exact in FP16), or null for the FP32 pass.
-hipError_t launch_fft_large(const float *x, float2 *ybuf, float2 *z, int64_t nframes, int M1,
-                            int M2, const float2 *tw1, const float2 *tw2, const float2 *twm,
-                            const void *mf_b1, hipStream_t stream);
-hipError_t launch_fft_pass1_mfma(const float2 *in, float2 *out, int64_t nbatch, int M1, int M2,
-                                 const void *b1, const float2 *tw_m, hipStream_t stream);
+// x_fp16: x holds FP16 sample pairs (N/2 dwords per frame; only with mf_b1)
+hipError_t launch_fft_large(const void *x, bool x_fp16, float2 *ybuf, float2 *z, int64_t nframes,
+                            int M1, int M2, const float2 *tw1, const float2 *tw2,
+                            const float2 *twm, const void *mf_b1, hipStream_t stream);
+hipError_t launch_fft_pass1_mfma(const void *in, bool in_fp16, float2 *out, int64_t nbatch, int M1,
+                                 int M2, const void *b1, const float2 *tw_m, hipStream_t stream);
 // inverse real FFT of nframes half spectra X through the HBM four-step:
 // zc (M complex per frame), ybuf, z scratch; out nframes*N floats.
 hipError_t launch_irfft_large(const float2 *X, int64_t nframes, int M1, int M2, const float2 *tw1,

@@ -115,13 +115,14 @@ static hipError_t launch_pass(int L, const float2 *in, float2 *out, int64_t nbat
 // Z (natural order, M complex per station-frame) of nframes frames from the
 // decoded samples x (N floats per frame): x read as z[m] = x[2m] + i x[2m+1].
 // With mf_b1 the first pass's DFT-32 runs on FP16 MFMA (fxc_mfma.hip).
-hipError_t launch_fft_large(const float *x, float2 *ybuf, float2 *z, int64_t nframes, int M1,
-                            int M2, const float2 *tw1, const float2 *tw2, const float2 *twm,
-                            const void *mf_b1, hipStream_t s) {
+hipError_t launch_fft_large(const void *x, bool x_fp16, float2 *ybuf, float2 *z, int64_t nframes,
+                            int M1, int M2, const float2 *tw1, const float2 *tw2,
+                            const float2 *twm, const void *mf_b1, hipStream_t s) {
     const int M = M1 * M2;
+    if (x_fp16 && !mf_b1) return hipErrorInvalidValue;   // FP16 input feeds the MFMA pass only
     const float2 *zin = reinterpret_cast<const float2 *>(x);
     // columns c = m2 (M2 of them): input z[m2 + M2 m1], output Y[k1][m2] * W_M^{m2 k1}
-    hipError_t e = mf_b1 ? launch_fft_pass1_mfma(zin, ybuf, nframes, M1, M2, mf_b1, twm, s)
+    hipError_t e = mf_b1 ? launch_fft_pass1_mfma(x, x_fp16, ybuf, nframes, M1, M2, mf_b1, twm, s)
                          : launch_pass(M1, zin, ybuf, nframes, M2, 1, M2, 1, M2, M, M, tw1, twm, M, 1, s);
     if (e != hipSuccess) return e;
     // rows c = k1 (M1 of them): input Y[k1][m2], output Z[k1 + M1 k2]
@@ -184,8 +185,7 @@ hipError_t launch_irfft_large(const float2 *X, int64_t nframes, int M1, int M2, 
     hipLaunchKernelGGL(k_irfft_prepare_large, grid, dim3(256), 0, s, X, nframes, M, tw_post, zc);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    e = launch_fft_large(reinterpret_cast<const float *>(zc), ybuf, z, nframes, M1, M2, tw1, tw2, twm,
-                         nullptr, s);
+    e = launch_fft_large(zc, false, ybuf, z, nframes, M1, M2, tw1, tw2, twm, nullptr, s);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_irfft_finish_large, grid, dim3(256), 0, s, z, nframes, M, negate_odd ? 1 : 0,
                        out);

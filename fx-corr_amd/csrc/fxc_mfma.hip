@@ -1480,9 +1480,10 @@ hipError_t launch_rfft_mfma(const SectorArgs &a, const float *in, int64_t nframe
 // and Y[k1] W_M^{m2 k1} stored at out[m2 + M2 k1], as the FP32 pass does.
 // Wave v forms A_v of the workgroup's 16 columns; the DFT-R runs after an
 // LDS exchange, one (column, k') pair per thread.  The inputs are decoded
-// levels, exact in FP16 (the plan checks this), so the conversion is exact.
-template <int R>
-__global__ __launch_bounds__(64 * R) void k_fft_pass1_mfma(const float2 *in, float2 *out,
+// levels, exact in FP16 (the plan checks this): FP32 pairs converted exactly
+// (IN = float2), or FP16 pairs written so by the decode (IN = uint32_t).
+template <int R, typename IN>
+__global__ __launch_bounds__(64 * R) void k_fft_pass1_mfma(const IN *in, float2 *out,
                                                            int64_t nbatch, int M2,
                                                            const uint4 *b1g, const float2 *tw_m,
                                                            int M) {
@@ -1496,7 +1497,7 @@ __global__ __launch_bounds__(64 * R) void k_fft_pass1_mfma(const float2 *in, flo
     const int64_t f = blockIdx.x / blocks_per_frame;
     const int c0 = (int)(blockIdx.x % blocks_per_frame) * 16;
     if (f >= nbatch) return;   // uniform per workgroup
-    const float2 *src = in + f * (int64_t)M;
+    const IN *src = in + f * (int64_t)M;
     float2 *dst = out + f * (int64_t)M;
     for (int i = tid; i < 16 * 64; i += 64 * R) b1[i] = b1g[i];
     // A operand of K-step kap, lane l: row r = l % 16 (column c0 + r),
@@ -1508,8 +1509,11 @@ __global__ __launch_bounds__(64 * R) void k_fft_pass1_mfma(const float2 *in, flo
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int u = 16 * kap + g + 4 * q;
-            const float2 x = src[(c0 + r) + (int64_t)M2 * (v + R * u)];
-            d[kap * 4 + q] = __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pkrtz(x.x, x.y));
+            const IN x = src[(c0 + r) + (int64_t)M2 * (v + R * u)];
+            if constexpr (sizeof(IN) == 4)
+                d[kap * 4 + q] = x;
+            else
+                d[kap * 4 + q] = __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pkrtz(x.x, x.y));
         }
     const half8 a0 = __builtin_bit_cast(half8, make_uint4(d[0], d[1], d[2], d[3]));
     const half8 a1 = __builtin_bit_cast(half8, make_uint4(d[4], d[5], d[6], d[7]));
@@ -1542,19 +1546,28 @@ __global__ __launch_bounds__(64 * R) void k_fft_pass1_mfma(const float2 *in, flo
     }
 }
 
-hipError_t launch_fft_pass1_mfma(const float2 *in, float2 *out, int64_t nbatch, int M1, int M2,
-                                 const void *b1, const float2 *tw_m, hipStream_t s) {
+template <typename IN>
+static void launch_pass1_t(const IN *in, float2 *out, int64_t nbatch, int M1, int M2,
+                           const uint4 *b, const float2 *tw_m, int64_t grid, hipStream_t s) {
     const int M = M1 * M2;
+    if (M1 == 128)
+        hipLaunchKernelGGL((k_fft_pass1_mfma<4, IN>), dim3((unsigned)grid), dim3(256), 0, s, in, out,
+                           nbatch, M2, b, tw_m, M);
+    else
+        hipLaunchKernelGGL((k_fft_pass1_mfma<8, IN>), dim3((unsigned)grid), dim3(512), 0, s, in, out,
+                           nbatch, M2, b, tw_m, M);
+}
+
+hipError_t launch_fft_pass1_mfma(const void *in, bool in_fp16, float2 *out, int64_t nbatch, int M1,
+                                 int M2, const void *b1, const float2 *tw_m, hipStream_t s) {
     if (b1 == nullptr || M2 % 16 != 0 || (M1 != 128 && M1 != 256)) return hipErrorInvalidValue;
     if (nbatch <= 0) return hipSuccess;
     const int64_t grid = nbatch * (M2 / 16);
     const uint4 *b = reinterpret_cast<const uint4 *>(b1);
-    if (M1 == 128)
-        hipLaunchKernelGGL(k_fft_pass1_mfma<4>, dim3((unsigned)grid), dim3(256), 0, s, in, out, nbatch,
-                           M2, b, tw_m, M);
+    if (in_fp16)
+        launch_pass1_t(reinterpret_cast<const uint32_t *>(in), out, nbatch, M1, M2, b, tw_m, grid, s);
     else
-        hipLaunchKernelGGL(k_fft_pass1_mfma<8>, dim3((unsigned)grid), dim3(512), 0, s, in, out, nbatch,
-                           M2, b, tw_m, M);
+        launch_pass1_t(reinterpret_cast<const float2 *>(in), out, nbatch, M1, M2, b, tw_m, grid, s);
     return hipGetLastError();
 }
 
