@@ -1043,17 +1043,18 @@ static fxc_status enqueue_phased(fxc_plan *p, const SectorArgs &base, int64_t nf
     const bool raw = (p->desc.products & FXC_PRODUCT_RAW_OUT) && d_raw_out != nullptr;
     const size_t half = (size_t)p->half;
     const int64_t bpf_o = plan_bpf_out(p);
-    // per-frame scratch: two rotation chains, and for the .raw output the
-    // combined half spectrum and the time series.  The chunk does not depend
-    // on whether .raw is written, so |cb|^2 sums in the same order either way.
+    // per-frame scratch: two rotation chains and, for the .raw output, the
+    // combined half spectrum (the inverse FFT quantises its time series from
+    // LDS; the N * 4 term keeps the chunking of the staged form).  The chunk
+    // does not depend on whether .raw is written, so |cb|^2 sums in the same
+    // order either way.
     const int ld = chain_ld_for((int)half);
     const size_t per_frame = 2 * (size_t)ld * 8 + half * 8 + (size_t)p->N * 4;
     const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(nf, (int64_t)(((size_t)256 << 20) / per_frame)));
     const int64_t span = chain_span(nf, chunk, 2 * (size_t)ld * 8, (int)half);
     if (p->d_chain.ensure((size_t)span * 2 * ld * 8) != hipSuccess)
         return fail(FXC_ERR_NOMEM, "phased scratch allocation failed");
-    if (raw && (p->d_cb.ensure((size_t)chunk * half * 8) != hipSuccess ||
-                p->d_tbuf.ensure((size_t)chunk * p->N * 4) != hipSuccess))
+    if (raw && p->d_cb.ensure((size_t)chunk * half * 8) != hipSuccess)
         return fail(FXC_ERR_NOMEM, "phased scratch allocation failed");
     int64_t fpw = (chunk + p->max_wg - 1) / p->max_wg;
     fpw = ((fpw + kFrameBatch - 1) / kFrameBatch) * kFrameBatch;
@@ -1081,9 +1082,8 @@ static fxc_status enqueue_phased(fxc_plan *p, const SectorArgs &base, int64_t nf
         a.chain_ld = ld;
         HIP_TRY(launch_phased(a, c0 > 0 ? 1 : 0, (int)((cn + fpw - 1) / fpw), s));
         if (raw) {
-            HIP_TRY(launch_irfft_frames(a, (const float2 *)p->d_cb.p, cn, (float *)p->d_tbuf.p,
-                                        p->desc.output_lsb != 0, s));
-            HIP_TRY(launch_quantise_frames(a, (const float *)p->d_tbuf.p, cn, d_raw_out + c0 * bpf_o, s));
+            HIP_TRY(launch_irfft_quantise(a, (const float2 *)p->d_cb.p, cn, d_raw_out + c0 * bpf_o,
+                                          p->desc.output_lsb != 0, s));
         }
     }
     HIP_TRY(launch_reduce((const double *)p->part.p, n_wg, p->half, d_acc,

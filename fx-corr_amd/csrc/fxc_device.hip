@@ -534,14 +534,16 @@ __device__ __forceinline__ uint32_t shuffle_out_word(uint32_t w, const int32_t *
     return o;
 }
 
-// One thread per output word of one frame.
-__device__ __forceinline__ uint32_t quant_word(const float *x, int N, int w, int bits,
-                                               const float *lv, int nl, uint32_t mask, bool sorted) {
+// One thread per output word of one frame; x(s) gives sample s.
+template <typename X>
+__device__ __forceinline__ uint32_t quant_word(X x, int N, int w, int bits, const float *lv, int nl,
+                                               uint32_t mask, bool sorted) {
     uint64_t acc = 0;
     const int64_t b0 = (int64_t)w * 32, b1 = b0 + 32;
     int64_t s = b0 / bits;
     for (; s * bits < b1 && s < N; ++s) {
-        const uint32_t q = sorted ? quant_code_sorted(x[s], lv, nl) : quant_code(x[s], lv, nl);
+        const float xs = x((int)s);
+        const uint32_t q = sorted ? quant_code_sorted(xs, lv, nl) : quant_code(xs, lv, nl);
         uint64_t code = (uint64_t)(q & mask);
         int64_t pos = s * bits - b0;  // may be negative for the straddling first sample
         if (pos >= 0) acc |= code << pos;
@@ -558,11 +560,76 @@ __global__ void k_quantise_frames(SectorArgs a, const float *in, int64_t nframes
     const int nwo = a.bpf_out / 4;
     const uint32_t mask = (a.bit_out >= 32) ? 0xFFFFFFFFu : ((1u << a.bit_out) - 1u);
     const int64_t f = blockIdx.x;
+    const float *x = in + f * a.N;
     for (int w = threadIdx.x; w < nwo; w += blockDim.x) {
-        uint32_t word = quant_word(in + f * a.N, a.N, w, a.bit_out, lv, nl < 256 ? nl : 256, mask,
-                                   a.out_sorted != 0);
+        uint32_t word = quant_word([x](int i) { return x[i]; }, a.N, w, a.bit_out, lv,
+                                   nl < 256 ? nl : 256, mask, a.out_sorted != 0);
         uint32_t sh = shuffle_out_word(word, a.out_shifts);
         reinterpret_cast<uint32_t *>(out + f * a.bpf_out)[w] = sh;
+    }
+}
+
+// The inverse FFT and the quantiser fused for the phased .raw output: the
+// time series never leaves LDS.  Its samples are the floats k_irfft_frames
+// writes (z.x * scale, odd_sign * (z.y * scale)), so the words are the same.
+template <int E, int T>
+__global__ __launch_bounds__(2 * T) void k_irfft_quantise(SectorArgs a, const float2 *in,
+                                                          int64_t nframes, uint8_t *out,
+                                                          int negate_odd) {
+    constexpr int M = E * T;
+    constexpr int MP = M + M / 16 + 2;
+    __shared__ float2 zbuf[2][MP];
+    __shared__ float lv[256];
+    const int s = half_index<T>(threadIdx.x);
+    const int t = threadIdx.x - s * T;
+    const int64_t f = 2 * (int64_t)blockIdx.x + s;
+    const bool live = f < nframes;
+    const int nl = a.out_nlevels < 256 ? a.out_nlevels : 256;
+    for (int i = threadIdx.x; i < nl; i += 2 * T) lv[i] = a.out_levels[i];
+    float2 v[E];
+    if (live) irfft_prepare<E, T>(in + f * (M + 1), M, a.tw_post, v, t);
+    else
+        for (int r = 0; r < E; ++r) v[r] = make_float2(0.f, 0.f);
+    fft_forward<E, T>(zbuf[s], v, a.tw_fft, t);   // ends with a barrier: zbuf and lv complete
+    const float scale = 1.0f / (float)(2 * M);
+    const float odd_sign = negate_odd ? 1.0f : -1.0f;   // main.rs:2290-2292
+    const float2 *zs = zbuf[s];
+    const int nwo = a.bpf_out / 4;
+    const uint32_t mask = (a.bit_out >= 32) ? 0xFFFFFFFFu : ((1u << a.bit_out) - 1u);
+    if (live)
+        for (int w = t; w < nwo; w += T) {
+            const uint32_t word = quant_word(
+                [zs, scale, odd_sign](int i) {
+                    const float2 z = zs[pad(i >> 1)];
+                    return (i & 1) ? odd_sign * (z.y * scale) : z.x * scale;
+                },
+                2 * M, w, a.bit_out, lv, nl, mask, a.out_sorted != 0);
+            reinterpret_cast<uint32_t *>(out + f * a.bpf_out)[w] = shuffle_out_word(word, a.out_shifts);
+        }
+}
+
+template <int E, int T>
+static hipError_t launch_irfft_quantise_t(const SectorArgs &a, const float2 *in, int64_t nframes,
+                                          uint8_t *out, bool negate_odd, hipStream_t s) {
+    hipLaunchKernelGGL((k_irfft_quantise<E, T>), dim3((unsigned)((nframes + 1) / 2)), dim3(2 * T), 0,
+                       s, a, in, nframes, out, negate_odd ? 1 : 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_irfft_quantise(const SectorArgs &a, const float2 *in, int64_t nframes,
+                                 uint8_t *out, bool negate_odd, hipStream_t s) {
+    if (nframes <= 0) return hipSuccess;
+    switch (a.N) {
+    case 64: return launch_irfft_quantise_t<2, 16>(a, in, nframes, out, negate_odd, s);
+    case 128: return launch_irfft_quantise_t<2, 32>(a, in, nframes, out, negate_odd, s);
+    case 256: return launch_irfft_quantise_t<4, 32>(a, in, nframes, out, negate_odd, s);
+    case 512: return launch_irfft_quantise_t<4, 64>(a, in, nframes, out, negate_odd, s);
+    case 1024: return launch_irfft_quantise_t<8, 64>(a, in, nframes, out, negate_odd, s);
+    case 2048: return launch_irfft_quantise_t<8, 128>(a, in, nframes, out, negate_odd, s);
+    case 4096: return launch_irfft_quantise_t<8, 256>(a, in, nframes, out, negate_odd, s);
+    case 8192: return launch_irfft_quantise_t<8, 512>(a, in, nframes, out, negate_odd, s);
+    case 16384: return launch_irfft_quantise_t<16, 512>(a, in, nframes, out, negate_odd, s);
+    default: return hipErrorInvalidValue;
     }
 }
 

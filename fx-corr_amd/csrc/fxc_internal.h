@@ -135,6 +135,9 @@ hipError_t launch_irfft_frames(const SectorArgs &a, const float2 *in,
                                hipStream_t stream);
 hipError_t launch_delay_eval(const DelayDev &d, int64_t first, int64_t n,
                              FrameDelayDev *out, hipStream_t stream);
+// Inverse FFT and quantiser in one kernel (phased .raw output, N <= 16384).
+hipError_t launch_irfft_quantise(const SectorArgs &a, const float2 *in, int64_t nframes,
+                                 uint8_t *out, bool negate_odd, hipStream_t s);
 hipError_t launch_quantise_frames(const SectorArgs &a, const float *in,
                                   int64_t nframes, uint8_t *out,
                                   hipStream_t stream);
