@@ -1140,12 +1140,13 @@ static fxc_status enqueue_phased_large(fxc_plan *p, const SectorArgs &base, int6
         HIP_TRY(launch_phased_large(a, (const float2 *)p->d_z[0].p, (const float2 *)p->d_z[1].p,
                                     p->d_params.p, kLargeGroups, c0 > 0 ? 1 : 0, s));
         if (raw) {
-            // d_x[0]: conj(Z'), d_z[0]: its FFT, d_x[1]: the time series
+            // d_x[0]: conj(Z'), d_z[0]: its FFT, quantised straight from there
             HIP_TRY(launch_irfft_large((const float2 *)p->d_cb.p, cn, p->M1, p->M2, twl1, twl2, twm,
                                        (const float2 *)p->tw_post.p, (float2 *)p->d_x[0].p,
                                        (float2 *)p->d_y.p, (float2 *)p->d_z[0].p,
-                                       p->desc.output_lsb != 0, (float *)p->d_x[1].p, s));
-            HIP_TRY(launch_quantise_frames(a, (const float *)p->d_x[1].p, cn, d_raw_out + c0 * bpf_o, s));
+                                       p->desc.output_lsb != 0, nullptr, s));
+            HIP_TRY(launch_quantise_cplx(a, (const float2 *)p->d_z[0].p, cn, p->desc.output_lsb != 0,
+                                         d_raw_out + c0 * bpf_o, s));
         }
     }
     HIP_TRY(launch_reduce((const double *)p->part.p, kLargeGroups, p->half, d_acc,

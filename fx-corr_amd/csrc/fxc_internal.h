@@ -138,6 +138,9 @@ hipError_t launch_delay_eval(const DelayDev &d, int64_t first, int64_t n,
 // Inverse FFT and quantiser in one kernel (phased .raw output, N <= 16384).
 hipError_t launch_irfft_quantise(const SectorArgs &a, const float2 *in, int64_t nframes,
                                  uint8_t *out, bool negate_odd, hipStream_t s);
+// Quantiser over the HBM four-step inverse's complex output (N >= 32768).
+hipError_t launch_quantise_cplx(const SectorArgs &a, const float2 *z, int64_t nframes,
+                                bool negate_odd, uint8_t *out, hipStream_t s);
 hipError_t launch_quantise_frames(const SectorArgs &a, const float *in,
                                   int64_t nframes, uint8_t *out,
                                   hipStream_t stream);
@@ -171,7 +174,8 @@ hipError_t launch_fft_large(const void *x, bool x_fp16, float2 *ybuf, float2 *z,
 hipError_t launch_fft_pass1_mfma(const void *in, bool in_fp16, float2 *out, int64_t nbatch, int M1,
                                  int M2, const void *b1, const float2 *tw_m, hipStream_t stream);
 // inverse real FFT of nframes half spectra X through the HBM four-step:
-// zc (M complex per frame), ybuf, z scratch; out nframes*N floats.
+// zc (M complex per frame), ybuf, z scratch; out nframes*N floats, or null
+// to leave the unscaled transform (conj-FFT input, M per frame) in z.
 hipError_t launch_irfft_large(const float2 *X, int64_t nframes, int M1, int M2, const float2 *tw1,
                               const float2 *tw2, const float2 *twm, const float2 *tw_post,
                               float2 *zc, float2 *ybuf, float2 *z, bool negate_odd, float *out,

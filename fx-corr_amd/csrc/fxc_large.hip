@@ -186,7 +186,7 @@ hipError_t launch_irfft_large(const float2 *X, int64_t nframes, int M1, int M2, 
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     e = launch_fft_large(zc, false, ybuf, z, nframes, M1, M2, tw1, tw2, twm, nullptr, s);
-    if (e != hipSuccess) return e;
+    if (e != hipSuccess || out == nullptr) return e;   // no out: the caller reads z
     hipLaunchKernelGGL(k_irfft_finish_large, grid, dim3(256), 0, s, z, nframes, M, negate_odd ? 1 : 0,
                        out);
     return hipGetLastError();
