@@ -34,6 +34,8 @@ __global__ __launch_bounds__(2 * T) void k_phased(SectorArgs a, int accumulate) 
     constexpr int MP = M + M / 16 + 2;
     constexpr int NT = 2 * T;
     constexpr int NB = (M + NT) / NT;
+    // chain prefetch: fits in registers (the one-wave N = 256 form spills)
+    constexpr bool PF = NB <= 5 && T >= 64;
     __shared__ float2 zbuf[2][MP];
     __shared__ PhasedParam pp[kFrameBatch];
     __shared__ float lev[2][256];
@@ -74,6 +76,20 @@ __global__ __launch_bounds__(2 * T) void k_phased(SectorArgs a, int accumulate) 
         for (int jf = 0; jf < nb; ++jf) {
             const int64_t f = fb + jf;
             const int tid = loop_local(tid0), t = loop_local(t0);   // see loop_local
+            // rotation of bin b: the reference's own f32 chain r_b
+            // (k_phase_chain, rows 2f + station of this chunk).  Loaded
+            // before the decode and FFT so their latency overlaps them.
+            const float2 *chg = a.chain + (2 * f + gsn) * a.chain_ld;
+            const float2 *cho = a.chain + (2 * f + 1 - gsn) * a.chain_ld;
+            float2 rg[PF ? NB : 1], ro[PF ? NB : 1];
+            if constexpr (PF) {
+#pragma unroll
+                for (int i = 0; i < NB; ++i) {
+                    const int g = tid + NT * i;
+                    rg[i] = g <= M ? chg[g] : make_float2(0.f, 0.f);
+                    ro[i] = (g <= M && g >= gs && g < ge) ? cho[g + sb_other] : make_float2(0.f, 0.f);
+                }
+            }
             stage_words<T>(raw + f * st.bpf, st, ws, t);
             __syncthreads();
             const int sh = s ? pp[jf].s2 : pp[jf].s1;
@@ -86,21 +102,17 @@ __global__ __launch_bounds__(2 * T) void k_phased(SectorArgs a, int accumulate) 
             const PhasedParam &P = pp[jf];
             const float2 frg = gsn == 0 ? P.fr1 : P.fr2;
             const float2 fro = gsn == 0 ? P.fr2 : P.fr1;
-            // rotation of bin b: the reference's own f32 chain r_b
-            // (k_phase_chain, rows 2f + station of this chunk)
-            const float2 *chg = a.chain + (2 * f + gsn) * a.chain_ld;
-            const float2 *cho = a.chain + (2 * f + 1 - gsn) * a.chain_ld;
 #pragma unroll
             for (int i = 0; i < NB; ++i) {
                 const int g = tid + NT * i;
                 if (g <= M) {
-                    float2 sg = c_mul(untangle(zg, g, M, a.tw_post), chg[g]);
+                    float2 sg = c_mul(untangle(zg, g, M, a.tw_post), PF ? rg[PF ? i : 0] : chg[g]);
                     if (g == M) sg.y = 0.0f;                  // Nyquist (utils.rs:118-123)
                     const float2 cg = c_mul(sg, frg);
                     float2 co = make_float2(0.0f, 0.0f);
                     if (g >= gs && g < ge) {
                         const int j = g + sb_other;
-                        float2 so = c_mul(untangle(zo, j, M, a.tw_post), cho[j]);
+                        float2 so = c_mul(untangle(zo, j, M, a.tw_post), PF ? ro[PF ? i : 0] : cho[j]);
                         if (j == M) so.y = 0.0f;
                         co = c_mul(so, fro);
                     }
@@ -128,11 +140,15 @@ __global__ __launch_bounds__(2 * T) void k_phased(SectorArgs a, int accumulate) 
     for (int i = 0; i < NB; ++i) {
         const int g = tid + NT * i;
         if (g <= M) {
-            out[P_PH * half + g] = accumulate ? out[P_PH * half + g] + ph[i] : ph[i];
-            out[P_11 * half + g] = 0.0;
-            out[P_22 * half + g] = 0.0;
-            out[P_12R * half + g] = 0.0;
-            out[P_12I * half + g] = 0.0;
+            if (accumulate) {
+                out[P_PH * half + g] += ph[i];
+            } else {   // the other products stay zero over the later chunks
+                out[P_PH * half + g] = ph[i];
+                out[P_11 * half + g] = 0.0;
+                out[P_22 * half + g] = 0.0;
+                out[P_12R * half + g] = 0.0;
+                out[P_12I * half + g] = 0.0;
+            }
         }
     }
 }
