@@ -36,6 +36,10 @@ __attribute__((amdgpu_waves_per_eu(MF && E == 8 && T == 256 ? 4 : 1))) void k_se
     // LDS is then 160,032 of 163,840 bytes)
     constexpr bool B1_LDS = MF;
     __shared__ uint4 b1s[B1_LDS ? 16 * 64 : 1];
+    // plain-form chain prefetch at N = 8192, where it also relieves the
+    // register spills; at N = 2048/4096 its registers cost a wave per SIMD
+    // and the kernel got slower
+    constexpr bool CPF = PREC == 0 && NB <= 5 && T >= 512;
 
     const int tid0 = threadIdx.x;
     const int s = half_index<T>(tid0);   // station handled by this half of the workgroup
@@ -82,6 +86,20 @@ __attribute__((amdgpu_waves_per_eu(MF && E == 8 && T == 256 ? 4 : 1))) void k_se
             stage_words<T>(raw + f * st.bpf, st, ws, t);
             __syncthreads();
             const int sh = s ? fp[jf].s2 : fp[jf].s1;
+            // plain form: the frame's chain values (the reference's own f32
+            // XCF phasors) loaded before the decode and FFT so their latency
+            // overlaps them
+            float2 chv[CPF ? NB : 1];
+            if constexpr (CPF) {
+                if (a.chain != nullptr) {
+#pragma unroll
+                    for (int i = 0; i < NB; ++i) {
+                        const int g = tid + NT * i;
+                        chv[i] = (g <= M && g >= gs && g < ge) ? a.chain[f * a.chain_ld + (g - gs)]
+                                                               : make_float2(0.f, 0.f);
+                    }
+                }
+            }
             float2 v[E];
             if constexpr (MF) {
                 // the frame goes to LDS before the first pass: each thread
@@ -125,7 +143,7 @@ __attribute__((amdgpu_waves_per_eu(MF && E == 8 && T == 256 ? 4 : 1))) void k_se
                         const int k = g - gs;
                         float2 ph;
                         if (PREC == 0 && a.chain != nullptr) {
-                            ph = a.chain[f * a.chain_ld + k];     // the reference's own chain
+                            ph = CPF ? chv[CPF ? i : 0] : a.chain[f * a.chain_ld + k];   // the reference's own chain
                         } else if (PREC != 0) {
                             // chosen form: the first overlap bin's phasor from
                             // the FP64 phase in turns (reduced mod 1, then the
