@@ -655,34 +655,39 @@ hipError_t launch_irfft_quantise(const SectorArgs &a, const float2 *in, int64_t 
 // per frame, natural order) of the HBM four-step path: sample 2m is
 // z[m].x * scale, sample 2m + 1 is odd_sign * (z[m].y * scale), the floats
 // the staged finish kernel writes, so the words match.
-__global__ void k_quantise_cplx(SectorArgs a, const float2 *z, int64_t nframes, int negate_odd,
-                                uint8_t *out) {
+// One word per thread over a (frame, 256-word tile) grid: a frame of
+// N = 65536 samples has up to 16,384 words, and a chunk's few hundred frames
+// alone would leave the GPU at about two waves per SIMD.
+__global__ __launch_bounds__(256) void k_quantise_cplx(SectorArgs a, const float2 *z,
+                                                       int64_t nframes, int negate_odd,
+                                                       uint8_t *out) {
     __shared__ float lv[256];
     const int nl = a.out_nlevels < 256 ? a.out_nlevels : 256;
     for (int i = threadIdx.x; i < nl; i += blockDim.x) lv[i] = a.out_levels[i];
     __syncthreads();
     const int nwo = a.bpf_out / 4;
+    const int w = (int)blockIdx.y * (int)blockDim.x + (int)threadIdx.x;
+    if (w >= nwo) return;
     const uint32_t mask = (a.bit_out >= 32) ? 0xFFFFFFFFu : ((1u << a.bit_out) - 1u);
     const int64_t f = blockIdx.x;
     const float2 *zf = z + f * (a.N / 2);
     const float scale = 1.0f / (float)a.N;
     const float odd_sign = negate_odd ? 1.0f : -1.0f;   // main.rs:2290-2292
-    for (int w = threadIdx.x; w < nwo; w += blockDim.x) {
-        const uint32_t word = quant_word(
-            [zf, scale, odd_sign](int i) {
-                const float2 v = zf[i >> 1];
-                return (i & 1) ? odd_sign * (v.y * scale) : v.x * scale;
-            },
-            a.N, w, a.bit_out, lv, nl, mask, a.out_sorted != 0);
-        reinterpret_cast<uint32_t *>(out + f * a.bpf_out)[w] = shuffle_out_word(word, a.out_shifts);
-    }
+    const uint32_t word = quant_word(
+        [zf, scale, odd_sign](int i) {
+            const float2 v = zf[i >> 1];
+            return (i & 1) ? odd_sign * (v.y * scale) : v.x * scale;
+        },
+        a.N, w, a.bit_out, lv, nl, mask, a.out_sorted != 0);
+    reinterpret_cast<uint32_t *>(out + f * a.bpf_out)[w] = shuffle_out_word(word, a.out_shifts);
 }
 
 hipError_t launch_quantise_cplx(const SectorArgs &a, const float2 *z, int64_t nframes,
                                 bool negate_odd, uint8_t *out, hipStream_t s) {
     if (nframes <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_quantise_cplx, dim3((unsigned)nframes), dim3(256), 0, s, a, z, nframes,
-                       negate_odd ? 1 : 0, out);
+    const int nwo = a.bpf_out / 4;
+    hipLaunchKernelGGL(k_quantise_cplx, dim3((unsigned)nframes, (unsigned)((nwo + 255) / 256)), dim3(256),
+                       0, s, a, z, nframes, negate_odd ? 1 : 0, out);
     return hipGetLastError();
 }
 
