@@ -340,26 +340,36 @@ __global__ __launch_bounds__(T) void k_decode_frames(SectorArgs a, int station,
     }
 }
 
-template <bool H>
+template <int T, bool H>
 static hipError_t launch_decode_frames_t(const SectorArgs &a, int station, const int64_t *shifts,
                                          int shift_stride, void *out, hipStream_t s) {
     const size_t lds = (size_t)a.st[station].nwords * 4;
     if (lds > 96 * 1024) return hipErrorInvalidValue;
     if (lds > 48 * 1024) {   // above the default dynamic-LDS limit (8-bit samples, N = 65536)
-        hipError_t e = hipFuncSetAttribute((const void *)k_decode_frames<256, H>,
+        hipError_t e = hipFuncSetAttribute((const void *)k_decode_frames<T, H>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL((k_decode_frames<256, H>), dim3((unsigned)a.nf), dim3(256), lds, s, a,
-                       station, shifts, shift_stride, out);
+    hipLaunchKernelGGL((k_decode_frames<T, H>), dim3((unsigned)a.nf), dim3(T), lds, s, a, station,
+                       shifts, shift_stride, out);
     return hipGetLastError();
+}
+
+// N >= 16384: 1024 threads per frame.  The HBM four-step path decodes chunks
+// of a few hundred frames, and 256-thread workgroups left about two waves per
+// SIMD.
+template <bool H>
+static hipError_t launch_decode_frames_h(const SectorArgs &a, int station, const int64_t *shifts,
+                                         int shift_stride, void *out, hipStream_t s) {
+    return a.M >= 8192 ? launch_decode_frames_t<1024, H>(a, station, shifts, shift_stride, out, s)
+                       : launch_decode_frames_t<256, H>(a, station, shifts, shift_stride, out, s);
 }
 
 hipError_t launch_decode_frames(const SectorArgs &a, int station, const int64_t *shifts,
                                 int shift_stride, void *out, bool fp16, hipStream_t s) {
     if (a.nf <= 0) return hipSuccess;
-    return fp16 ? launch_decode_frames_t<true>(a, station, shifts, shift_stride, out, s)
-                : launch_decode_frames_t<false>(a, station, shifts, shift_stride, out, s);
+    return fp16 ? launch_decode_frames_h<true>(a, station, shifts, shift_stride, out, s)
+                : launch_decode_frames_h<false>(a, station, shifts, shift_stride, out, s);
 }
 
 // Forward R2C with the fused kernel's FFT code: each half of the workgroup
