@@ -166,6 +166,7 @@ struct fxc_plan {
         d_shifts, d_cb, d_tbuf;
     DevBuf d_chain;          // the reference's f32 phase chains of one slab/chunk of frames
     DevBuf d_fringe;         // one chunk's products for the whole-scan cross spectrum
+    DevBuf d_err;            // device error word: bit 0 set by a workgroup whose hand-over wait ran out
     int max_wg = 0;
     std::vector<double> table_host;  // last uploaded geometric table
     // timing
@@ -173,6 +174,22 @@ struct fxc_plan {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pending, pool;
     double timed_ms = 0.0;
     int64_t timed_launches = 0;
+    // Graphs that captured work of this plan hold its device buffers by
+    // address: while any is alive the scratch may not move and the delay
+    // table may not change, and fxc_plan_destroy defers the release to the
+    // last graph's destruction.
+    int live_graphs = 0;
+    bool handle_released = false;
+
+    // Every device buffer of the plan (release, byte count).
+    template <class Fn> void each_buf(Fn fn) {
+        DevBuf *bufs[] = {&tw_fft, &tw_post, &lev_dev[0], &lev_dev[1], &out_lev, &mf_tab,
+                          &tw_l1, &tw_l2, &tw_lm, &d_x[0], &d_x[1], &d_y, &d_z[0], &d_z[1],
+                          &d_params, &d_lshifts, &part, &red_scratch, &table_dev, &d_raw[0],
+                          &d_raw[1], &d_acc, &d_raw_out, &d_tmp_in, &d_tmp_out, &d_shifts,
+                          &d_cb, &d_tbuf, &d_chain, &d_fringe, &d_err};
+        for (DevBuf *b : bufs) fn(*b);
+    }
 };
 
 // ====================================================================== misc
@@ -203,6 +220,15 @@ extern "C" fxc_status fxc_device_count(int32_t *n) {
         return fail(FXC_ERR_HIP, "hipGetDeviceCount: %s", hipGetErrorString(e));
     }
     *n = c;
+    return FXC_OK;
+}
+
+extern "C" fxc_status fxc_device_mem_info(uint64_t *free_bytes, uint64_t *total_bytes) {
+    if (!free_bytes || !total_bytes) return fail(FXC_ERR_INVALID_ARG, "null argument");
+    size_t f = 0, t = 0;
+    HIP_TRY(hipMemGetInfo(&f, &t));
+    *free_bytes = f;
+    *total_bytes = t;
     return FXC_OK;
 }
 
@@ -476,6 +502,7 @@ static void fill_args(fxc_plan *p, SectorArgs &a) {
     // DFT-32 operand table when the first pass runs on MFMA, else none
     a.mf_tab = (p->fft_form == 1 || p->mf_b1) ? p->mf_tab.p : nullptr;
     a.variant = p->variant;
+    a.err = (int32_t *)p->d_err.p;
 }
 
 // IEEE binary16 bits of a float, round to nearest even (normals, subnormals,
@@ -836,14 +863,22 @@ extern "C" fxc_status fxc_plan_create(const fxc_plan_desc *desc, fxc_plan **out)
                 : p->fft_form == 2 ? kLargeGroups
                                    : std::max(64, (cus * 1024) / p->T);
     if (p->part.ensure((size_t)p->max_wg * kNumProducts * half * sizeof(double)) != hipSuccess ||
-        p->red_scratch.ensure((size_t)64 * kNumProducts * half * sizeof(double)) != hipSuccess)
+        p->red_scratch.ensure((size_t)64 * kNumProducts * half * sizeof(double)) != hipSuccess ||
+        p->d_err.ensure(sizeof(int32_t)) != hipSuccess)
         return cleanup(fail(FXC_ERR_NOMEM, "scratch allocation failed"));
+    if (hipMemset(p->d_err.p, 0, sizeof(int32_t)) != hipSuccess)
+        return cleanup(fail(FXC_ERR_HIP, "error word initialisation failed"));
     *out = p;
     return FXC_OK;
 }
 
-extern "C" fxc_status fxc_plan_destroy(fxc_plan *p) {
-    if (!p) return FXC_OK;
+// Plans whose release waits for graphs, and the plans each capturing stream
+// has enqueued work of (collected by enqueue_sector, handed to the graph by
+// fxc_graph_end).
+static std::mutex g_graph_mu;
+static std::vector<std::pair<hipStream_t, fxc_plan *>> g_captured;
+
+static void plan_free(fxc_plan *p) {
     if (p->stream) (void)hipStreamSynchronize(p->stream);
     for (auto &e : p->pending) {
         (void)hipEventDestroy(e.first);
@@ -853,26 +888,86 @@ extern "C" fxc_status fxc_plan_destroy(fxc_plan *p) {
         (void)hipEventDestroy(e.first);
         (void)hipEventDestroy(e.second);
     }
-    DevBuf *bufs[] = {&p->tw_fft, &p->tw_post, &p->lev_dev[0], &p->lev_dev[1], &p->out_lev,
-                      &p->part, &p->red_scratch, &p->table_dev, &p->d_raw[0], &p->d_raw[1],
-                      &p->d_acc, &p->d_raw_out, &p->d_tmp_in, &p->d_tmp_out, &p->d_shifts,
-                      &p->d_cb, &p->d_tbuf, &p->tw_l1, &p->tw_l2, &p->tw_lm, &p->d_x[0],
-                      &p->d_x[1], &p->d_y, &p->d_z[0], &p->d_z[1], &p->d_params, &p->d_lshifts,
-                      &p->mf_tab};
-    for (DevBuf *b : bufs) b->release();
+    p->each_buf([](DevBuf &b) { b.release(); });
     if (p->stream) (void)hipStreamDestroy(p->stream);
     delete p;
+}
+
+extern "C" fxc_status fxc_plan_destroy(fxc_plan *p) {
+    if (!p) return FXC_OK;
+    {
+        std::lock_guard<std::mutex> lk(g_graph_mu);
+        if (p->live_graphs > 0) {   // the last graph that captured it frees it
+            p->handle_released = true;
+            return FXC_OK;
+        }
+    }
+    plan_free(p);
     return FXC_OK;
 }
+
+static bool stream_capturing(hipStream_t s) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    return s && hipStreamIsCapturing(s, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone;
+}
+
+// Grows a plan buffer that device-entry work reads or writes.  Refused
+// while a live graph holds the old address, and during a capture (the
+// captured nodes before it would keep the freed one).
+static fxc_status grow(fxc_plan *p, DevBuf &b, size_t bytes, hipStream_t s, const char *what) {
+    if (bytes <= b.cap) return FXC_OK;
+    if (p->live_graphs > 0)
+        return fail(FXC_ERR_INVALID_ARG,
+                    "%s: the plan's scratch would grow to %zu bytes, but %d live graph(s) captured "
+                    "its current buffers; destroy them first, or make the largest call before capturing",
+                    what, bytes, p->live_graphs);
+    if (stream_capturing(s))
+        return fail(FXC_ERR_INVALID_ARG,
+                    "%s: the plan's scratch must grow to %zu bytes during a graph capture; run the "
+                    "call once before capturing", what, bytes);
+    if (b.ensure(bytes) != hipSuccess)
+        return fail(FXC_ERR_NOMEM, "%s: device allocation of %zu bytes failed", what, bytes);
+    return FXC_OK;
+}
+#define GROW(p, buf, bytes, s, what)                                  \
+    do {                                                              \
+        fxc_status g_ = grow((p), (buf), (bytes), (s), (what));       \
+        if (g_ != FXC_OK) return g_;                                  \
+    } while (0)
 
 // ================================================================ graphs
 struct fxc_graph {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    std::vector<fxc_plan *> plans;   // plans whose buffers the graph captured
 };
+
+static std::vector<fxc_plan *> take_captured(hipStream_t s) {
+    std::vector<fxc_plan *> out;
+    for (auto it = g_captured.begin(); it != g_captured.end();) {
+        if (it->first == s) {
+            if (std::find(out.begin(), out.end(), it->second) == out.end()) out.push_back(it->second);
+            it = g_captured.erase(it);
+        } else {
+            ++it;
+        }
+    }
+    return out;
+}
+
+static void note_captured(fxc_plan *p, hipStream_t s) {
+    std::lock_guard<std::mutex> lk(g_graph_mu);
+    for (auto &e : g_captured)
+        if (e.first == s && e.second == p) return;
+    g_captured.emplace_back(s, p);
+}
 
 extern "C" fxc_status fxc_graph_begin(void *stream) {
     if (!stream) return fail(FXC_ERR_INVALID_ARG, "graph capture needs a non-null stream");
+    {
+        std::lock_guard<std::mutex> lk(g_graph_mu);
+        (void)take_captured((hipStream_t)stream);   // left over from an abandoned capture
+    }
     HIP_TRY(hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal));
     return FXC_OK;
 }
@@ -880,6 +975,11 @@ extern "C" fxc_status fxc_graph_begin(void *stream) {
 extern "C" fxc_status fxc_graph_end(void *stream, fxc_graph **out) {
     if (!stream || !out) return fail(FXC_ERR_INVALID_ARG, "null argument");
     *out = nullptr;
+    std::vector<fxc_plan *> plans;
+    {
+        std::lock_guard<std::mutex> lk(g_graph_mu);
+        plans = take_captured((hipStream_t)stream);
+    }
     hipGraph_t g = nullptr;
     HIP_TRY(hipStreamEndCapture((hipStream_t)stream, &g));
     hipGraphExec_t e = nullptr;
@@ -891,6 +991,11 @@ extern "C" fxc_status fxc_graph_end(void *stream, fxc_graph **out) {
     fxc_graph *r = new fxc_graph();
     r->graph = g;
     r->exec = e;
+    r->plans = plans;
+    {
+        std::lock_guard<std::mutex> lk(g_graph_mu);
+        for (fxc_plan *p : plans) p->live_graphs++;
+    }
     *out = r;
     return FXC_OK;
 }
@@ -905,22 +1010,60 @@ extern "C" fxc_status fxc_graph_destroy(fxc_graph *g) {
     if (!g) return FXC_OK;
     if (g->exec) (void)hipGraphExecDestroy(g->exec);
     if (g->graph) (void)hipGraphDestroy(g->graph);
+    std::vector<fxc_plan *> to_free;
+    {
+        std::lock_guard<std::mutex> lk(g_graph_mu);
+        for (fxc_plan *p : g->plans)
+            if (--p->live_graphs == 0 && p->handle_released) to_free.push_back(p);
+    }
+    for (fxc_plan *p : to_free) plan_free(p);
     delete g;
+    return FXC_OK;
+}
+
+extern "C" fxc_status fxc_plan_live_graphs(const fxc_plan *p, int32_t *n) {
+    if (!p || !n) return fail(FXC_ERR_INVALID_ARG, "null argument");
+    std::lock_guard<std::mutex> lk(g_graph_mu);
+    *n = p->live_graphs;
     return FXC_OK;
 }
 
 extern "C" fxc_status fxc_plan_device_bytes(const fxc_plan *p, uint64_t *bytes) {
     if (!p || !bytes) return fail(FXC_ERR_INVALID_ARG, "null argument");
-    const DevBuf *bufs[] = {&p->tw_fft, &p->tw_post, &p->lev_dev[0], &p->lev_dev[1], &p->out_lev,
-                            &p->part, &p->red_scratch, &p->table_dev, &p->d_raw[0], &p->d_raw[1],
-                            &p->d_acc, &p->d_raw_out, &p->d_tmp_in, &p->d_tmp_out, &p->d_shifts,
-                            &p->mf_tab, &p->d_cb, &p->d_tbuf, &p->tw_l1, &p->tw_l2, &p->tw_lm,
-                            &p->d_x[0], &p->d_x[1], &p->d_y, &p->d_z[0], &p->d_z[1], &p->d_params,
-                            &p->d_lshifts, &p->d_chain, &p->d_fringe};
     uint64_t t = 0;
-    for (const DevBuf *b : bufs) t += b->cap;
+    const_cast<fxc_plan *>(p)->each_buf([&t](DevBuf &b) { t += b.cap; });
     *bytes = t;
     return FXC_OK;
+}
+
+// Reads and clears the plan's device error word (after the plan's work has
+// completed).  A set bit 0 means a workgroup's bounded hand-over wait ran
+// out and its partial sums are not valid.
+static fxc_status read_error_word(fxc_plan *p, hipStream_t s) {
+    if (!p->d_err.p) return FXC_OK;
+    int32_t w = 0;
+    HIP_TRY(hipMemcpyAsync(&w, p->d_err.p, sizeof w, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (w != 0) {
+        HIP_TRY(hipMemsetAsync(p->d_err.p, 0, sizeof w, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        return fail(FXC_ERR_HIP,
+                    "a sector kernel workgroup's bounded hand-over wait ran out (error word 0x%x); "
+                    "the spectra of the affected calls are not valid", (unsigned)w);
+    }
+    return FXC_OK;
+}
+
+// The server copies the word behind each job's result (fxc_server.cpp).
+extern "C" const int32_t *fxc_internal_plan_error_word(const fxc_plan *p) {
+    return (const int32_t *)p->d_err.p;
+}
+
+extern "C" fxc_status fxc_plan_check_device_error(fxc_plan *p, void *stream) {
+    if (!p) return fail(FXC_ERR_INVALID_ARG, "null plan");
+    if (stream_capturing((hipStream_t)stream))
+        return fail(FXC_ERR_INVALID_ARG, "the error word cannot be read inside a graph capture");
+    return read_error_word(p, (hipStream_t)stream);
 }
 
 extern "C" fxc_status fxc_plan_get_info(const fxc_plan *p, fxc_plan_info *info) {
@@ -974,12 +1117,14 @@ static fxc_status upload_table(fxc_plan *p, const fxc_delay_cfg &c, hipStream_t 
     bool same = p->table_host.size() == n &&
                 memcmp(p->table_host.data(), c.geom_table, n * sizeof(double)) == 0;
     if (!same) {
-        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-        if (s && hipStreamIsCapturing(s, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone)
+        if (stream_capturing(s))
             return fail(FXC_ERR_INVALID_ARG,
                         "geometric table changed during graph capture: run the call once before capturing");
-        if (p->table_dev.ensure(n * sizeof(double)) != hipSuccess)
-            return fail(FXC_ERR_NOMEM, "table allocation failed");
+        if (p->live_graphs > 0)
+            return fail(FXC_ERR_INVALID_ARG,
+                        "geometric table differs from the one %d live graph(s) of this plan captured; "
+                        "destroy them first, or use another plan", p->live_graphs);
+        GROW(p, p->table_dev, n * sizeof(double), s, "geometric table");
         p->table_host.assign(c.geom_table, c.geom_table + n);
         HIP_TRY(hipMemcpyAsync(p->table_dev.p, p->table_host.data(), n * sizeof(double),
                                hipMemcpyHostToDevice, s));
@@ -1052,10 +1197,8 @@ static fxc_status enqueue_phased(fxc_plan *p, const SectorArgs &base, int64_t nf
     const size_t per_frame = 2 * (size_t)ld * 8 + half * 8 + (size_t)p->N * 4;
     const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(nf, (int64_t)(((size_t)256 << 20) / per_frame)));
     const int64_t span = chain_span(nf, chunk, 2 * (size_t)ld * 8, (int)half);
-    if (p->d_chain.ensure((size_t)span * 2 * ld * 8) != hipSuccess)
-        return fail(FXC_ERR_NOMEM, "phased scratch allocation failed");
-    if (raw && p->d_cb.ensure((size_t)chunk * half * 8) != hipSuccess)
-        return fail(FXC_ERR_NOMEM, "phased scratch allocation failed");
+    GROW(p, p->d_chain, (size_t)span * 2 * ld * 8, s, "phased rotation chains");
+    if (raw) GROW(p, p->d_cb, (size_t)chunk * half * 8, s, "phased spectra");
     int64_t fpw = (chunk + p->max_wg - 1) / p->max_wg;
     fpw = ((fpw + kFrameBatch - 1) / kFrameBatch) * kFrameBatch;
     if (fpw < kFrameBatch) fpw = kFrameBatch;
@@ -1107,9 +1250,8 @@ static fxc_status enqueue_phased_large(fxc_plan *p, const SectorArgs &base, int6
     const int ld = chain_ld_for((int)half);
     const int64_t chunk = p->chunk_use;
     const int64_t span = chain_span(nf, chunk, 2 * (size_t)ld * 8, (int)half);
-    if (p->d_chain.ensure((size_t)span * 2 * ld * 8) != hipSuccess ||
-        (raw && p->d_cb.ensure((size_t)chunk * half * 8) != hipSuccess))
-        return fail(FXC_ERR_NOMEM, "phased scratch allocation failed");
+    GROW(p, p->d_chain, (size_t)span * 2 * ld * 8, s, "phased rotation chains");
+    if (raw) GROW(p, p->d_cb, (size_t)chunk * half * 8, s, "phased spectra");
     int64_t span0 = -1;    // first frame (of this sector) of the computed chains
     for (int64_t c0 = 0; c0 < nf; c0 += chunk) {
         const int64_t cn = std::min(chunk, nf - c0);
@@ -1166,8 +1308,7 @@ static fxc_status enqueue_large(fxc_plan *p, const SectorArgs &base, int64_t nf,
     int64_t span = 0;      // frames whose chains one launch computes: whole chunks
     if (chain) {
         span = chain_span(nf, p->chunk_use, (size_t)ld * 8, base.ov);
-        if (p->d_chain.ensure((size_t)span * ld * 8) != hipSuccess)
-            return fail(FXC_ERR_NOMEM, "phase chain allocation failed");
+        GROW(p, p->d_chain, (size_t)span * ld * 8, s, "phase chains");
     }
     int64_t span0 = -1;    // first frame (of this sector) of the computed chains
     for (int64_t c0 = 0; c0 < nf; c0 += p->chunk_use) {
@@ -1216,6 +1357,7 @@ static fxc_status enqueue_sector(fxc_plan *p, const uint8_t *d_raw1, const uint8
     const double *dtab = nullptr;
     fxc_status st = upload_table(p, dc, s, &dtab);
     if (st != FXC_OK) return st;
+    if (stream_capturing(s)) note_captured(p, s);
     const size_t acc_bytes = (size_t)kNumProducts * p->half * sizeof(double);
     const bool phased = (p->desc.products & (FXC_PRODUCT_PHASED | FXC_PRODUCT_RAW_OUT)) != 0;
     if (nf <= 0 || !p->decode_ok[0] || !p->decode_ok[1]) {
@@ -1244,8 +1386,7 @@ static fxc_status enqueue_sector(fxc_plan *p, const uint8_t *d_raw1, const uint8
     int64_t slab = nf;
     if (chain) {
         slab = std::min<int64_t>(nf, std::max<int64_t>(kFrameBatch, (int64_t)(kChainSlabBytes / ((size_t)ld * 8))));
-        if (p->d_chain.ensure((size_t)slab * ld * 8) != hipSuccess)
-            return fail(FXC_ERR_NOMEM, "phase chain allocation failed");
+        GROW(p, p->d_chain, (size_t)slab * ld * 8, s, "phase chains");
     }
     int64_t fpw = (slab + wg_cap - 1) / wg_cap;
     const int batch = p->fft_form == 1 ? 8 : kFrameBatch;   // MFMA form: whole frames per wave
@@ -1303,6 +1444,7 @@ extern "C" fxc_status fxc_plan_autotune(fxc_plan *p, const uint8_t *d_raw1, cons
         return fail(FXC_ERR_UNSUPPORTED, "autotune covers the ACF/XCF sector path");
     hipStream_t s = (hipStream_t)stream;
     const size_t acc_bytes = (size_t)kNumProducts * p->half * sizeof(double);
+    if (stream_capturing(s)) return fail(FXC_ERR_INVALID_ARG, "autotune cannot run inside a graph capture");
     if (p->d_acc.ensure(acc_bytes) != hipSuccess) return fail(FXC_ERR_NOMEM, "device allocation failed");
     struct Cand {
         int variant, grid_div, m1;
@@ -1454,8 +1596,8 @@ extern "C" fxc_status fxc_fringe_accumulate_device(fxc_plan *p, const uint8_t *d
     if (nf < 0) return fail(FXC_ERR_INVALID_ARG, "nf < 0");
     if (nf > 0 && (!d_raw1 || !d_raw2)) return fail(FXC_ERR_INVALID_ARG, "null raw buffer");
     const size_t acc_bytes = (size_t)kNumProducts * p->half * sizeof(double);
-    if (p->d_fringe.ensure(acc_bytes) != hipSuccess) return fail(FXC_ERR_NOMEM, "device allocation failed");
     hipStream_t s = (hipStream_t)stream;
+    GROW(p, p->d_fringe, acc_bytes, s, "whole-scan spectra");
     double *d = (double *)p->d_fringe.p;
     st = enqueue_sector(p, d_raw1, d_raw2, first_frame, nf, *delay, d, nullptr, s);
     if (st != FXC_OK) return st;
@@ -1501,6 +1643,8 @@ extern "C" fxc_status fxc_sector_run(fxc_plan *p, const uint8_t *raw1, size_t le
         fxc_status ts = collect_timing(p);
         if (ts != FXC_OK) return ts;
     }
+    st = read_error_word(p, s);
+    if (st != FXC_OK) return st;
     const int h = p->half;
     if (acc_ph) memcpy(acc_ph, host.data() + P_PH * h, h * sizeof(double));
     if (acc11) memcpy(acc11, host.data() + P_11 * h, h * sizeof(double));
@@ -1755,6 +1899,7 @@ struct fxc_pipeline {
     struct Slot {
         DevBuf raw[2], acc;
         hipEvent_t copied = nullptr, done = nullptr;
+        int32_t *err = nullptr;   // page-locked copy of the plan's error word after this sector
         int64_t ticket = -1;
     };
     std::vector<Slot> slots;
@@ -1771,6 +1916,7 @@ static void pipeline_release(fxc_pipeline *pl) {
         s.acc.release();
         if (s.copied) (void)hipEventDestroy(s.copied);
         if (s.done) (void)hipEventDestroy(s.done);
+        if (s.err) (void)hipHostFree(s.err);
     }
     if (pl->copy) (void)hipStreamDestroy(pl->copy);
     if (pl->compute) (void)hipStreamDestroy(pl->compute);
@@ -1800,8 +1946,10 @@ extern "C" fxc_status fxc_pipeline_create(fxc_plan *p, int64_t max_frames, int32
             s.acc.ensure(acc_bytes) != hipSuccess)
             return bail(fail(FXC_ERR_NOMEM, "pipeline slot allocation failed"));
         if (hipEventCreateWithFlags(&s.copied, hipEventDisableTiming) != hipSuccess ||
-            hipEventCreateWithFlags(&s.done, hipEventDisableTiming) != hipSuccess)
-            return bail(fail(FXC_ERR_HIP, "event creation failed"));
+            hipEventCreateWithFlags(&s.done, hipEventDisableTiming) != hipSuccess ||
+            hipHostMalloc((void **)&s.err, sizeof(int32_t), hipHostMallocDefault) != hipSuccess)
+            return bail(fail(FXC_ERR_HIP, "event or host allocation failed"));
+        *s.err = 0;
     }
     *out = pl;
     return FXC_OK;
@@ -1831,6 +1979,7 @@ extern "C" fxc_status fxc_pipeline_submit(fxc_pipeline *pl, const uint8_t *raw1,
     if (st != FXC_OK) return st;
     HIP_TRY(hipMemcpyAsync(acc_out, s.acc.p, (size_t)kNumProducts * p->half * sizeof(double),
                            hipMemcpyDeviceToHost, pl->compute));
+    HIP_TRY(hipMemcpyAsync(s.err, p->d_err.p, sizeof(int32_t), hipMemcpyDeviceToHost, pl->compute));
     HIP_TRY(hipEventRecord(s.done, pl->compute));
     s.ticket = t;
     pl->next_ticket = t + 1;
@@ -1843,6 +1992,9 @@ extern "C" fxc_status fxc_pipeline_wait(fxc_pipeline *pl, int64_t ticket) {
     auto &s = pl->slots[(size_t)(ticket % pl->depth)];
     // a slot reused since then: its later sector completes after this one
     HIP_TRY(hipEventSynchronize(s.done));
+    if (*s.err != 0)
+        return fail(FXC_ERR_HIP, "a sector kernel workgroup's bounded hand-over wait ran out "
+                                 "(error word 0x%x); the spectra are not valid", (unsigned)*s.err);
     return FXC_OK;
 }
 

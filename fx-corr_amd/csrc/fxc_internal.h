@@ -85,6 +85,7 @@ struct SectorArgs {
     const float2 *chain;
     int32_t chain_ld;
     int32_t accumulate;      // fused plain kernel: add to the partial slots (sector in slabs)
+    int32_t *err;            // plan's device error word (bit 0: a bounded hand-over wait ran out)
 };
 
 // Phase chains replayed by k_phase_chain (fxc_chain.hip).

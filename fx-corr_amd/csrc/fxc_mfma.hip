@@ -984,6 +984,8 @@ __global__ __launch_bounds__(mf::THREADS3) void k_sector_mf3(SectorArgs a) {
     }
     double *out = a.part + (size_t)blockIdx.x * kNumProducts * H;
     const double bad = sh.broken ? __builtin_nan("") : 0.0;   // NaN marks a failed hand-over
+    if (sh.broken && tid == 0 && a.err)   // and the plan's error word reports it to the host
+        __hip_atomic_fetch_or(a.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     for (int k = tid; k < H; k += NT) {
         out[P_PH * H + k] = bad;
         out[P_11 * H + k] = red[0 * H + k] + bad;
@@ -1378,6 +1380,8 @@ __global__ __launch_bounds__(mf4::THREADS) void k_sector_mf4(SectorArgs a) {
     __syncthreads();
     double *out = a.part + (size_t)blockIdx.x * kNumProducts * H;
     const double bad = sh.broken ? __builtin_nan("") : 0.0;   // NaN marks a failed hand-over
+    if (sh.broken && tid == 0 && a.err)   // and the plan's error word reports it to the host
+        __hip_atomic_fetch_or(a.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     for (int k = tid; k < H; k += NT) {
         out[P_PH * H + k] = bad;
         out[P_11 * H + k] = red[0 * H + k] + bad;

@@ -26,6 +26,7 @@
 #include <vector>
 
 extern "C" fxc_status fxc_internal_fail(fxc_status s, const char *msg);
+extern "C" const int32_t *fxc_internal_plan_error_word(const fxc_plan *p);
 
 using fxc::kNumProducts;
 using fxc::launch_add_f64;
@@ -243,7 +244,7 @@ struct fxc_server {
         }
         if (hipMalloc(&j.d_acc, acc_bytes) != hipSuccess ||
             hipMalloc(&j.d_chunk, acc_bytes) != hipSuccess ||
-            hipHostMalloc(&j.h_result, acc_bytes, 0) != hipSuccess ||
+            hipHostMalloc(&j.h_result, acc_bytes + sizeof(double), 0) != hipSuccess ||
             hipEventCreateWithFlags(&j.done_ev, hipEventDisableTiming) != hipSuccess) {
             free_job_device(j);
             fxc_plan_destroy(plan);
@@ -327,7 +328,12 @@ struct fxc_server {
         }
         if (j.closed && !j.final_enqueued) {
             const size_t bytes = (size_t)kNumProducts * j.half * sizeof(double);
+            // the plan's device error word lands behind the spectra
+            int32_t *err = reinterpret_cast<int32_t *>(j.h_result + (size_t)kNumProducts * j.half);
+            *err = 0;
             if (hipMemcpyAsync(j.h_result, j.d_acc, bytes, hipMemcpyDeviceToHost, s) != hipSuccess ||
+                hipMemcpyAsync(err, fxc_internal_plan_error_word(j.plan), sizeof(int32_t),
+                               hipMemcpyDeviceToHost, s) != hipSuccess ||
                 hipEventRecord(j.done_ev, s) != hipSuccess) {
                 fail_active(j, "result copy failed");
                 return true;
@@ -368,6 +374,11 @@ struct fxc_server {
         if (q == hipErrorNotReady) return false;
         if (q != hipSuccess) {
             fail_active(j, std::string("device error: ") + hipGetErrorString(q));
+            return true;
+        }
+        const int32_t err = *reinterpret_cast<const int32_t *>(j.h_result + (size_t)kNumProducts * j.half);
+        if (err != 0) {   // a workgroup's bounded hand-over wait ran out: the spectra are not valid
+            fail_active(j, "device error word set: a sector kernel's hand-over wait ran out");
             return true;
         }
         j.result.assign(j.h_result, j.h_result + (size_t)kNumProducts * j.half);

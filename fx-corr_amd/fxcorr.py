@@ -22,7 +22,8 @@ PRECISION_PLAIN_FP32, PRECISION_CHOSEN = 0, 1
 N_PRODUCTS = 5  # [ph | 11 | 22 | 12re | 12im]
 
 EXPORTED = [
-    "fxc_version", "fxc_strerror", "fxc_last_error", "fxc_device_count", "fxc_dev_alloc",
+    "fxc_version", "fxc_strerror", "fxc_last_error", "fxc_device_count", "fxc_device_mem_info",
+    "fxc_dev_alloc",
     "fxc_dev_free", "fxc_memcpy_h2d", "fxc_memcpy_d2h", "fxc_memset_d", "fxc_memcpy_d2d",
     "fxc_stream_create",
     "fxc_stream_destroy", "fxc_stream_sync", "fxc_event_create", "fxc_event_destroy",
@@ -129,6 +130,7 @@ def lib():
         sig = {
             "fxc_version": ([], C.c_char_p), "fxc_strerror": ([i32], C.c_char_p),
             "fxc_last_error": ([], C.c_char_p), "fxc_device_count": ([_P(i32)], i32),
+            "fxc_device_mem_info": ([_P(C.c_uint64), _P(C.c_uint64)], i32),
             "fxc_dev_alloc": ([sz, _P(vp)], i32), "fxc_dev_free": ([vp], i32),
             "fxc_memcpy_h2d": ([vp, vp, sz, vp], i32), "fxc_memcpy_d2h": ([vp, vp, sz, vp], i32),
             "fxc_memset_d": ([vp, C.c_int, sz, vp], i32),
@@ -237,6 +239,13 @@ def device_count() -> int:
     n = C.c_int32()
     st = lib().fxc_device_count(C.byref(n))
     return n.value if st == OK else 0
+
+
+def device_mem_info():
+    """(free, total) bytes of the current device."""
+    f, t = C.c_uint64(), C.c_uint64()
+    check(lib().fxc_device_mem_info(C.byref(f), C.byref(t)))
+    return f.value, t.value
 
 
 def make_delay_cfg(table=None, **kw) -> DelayCfg:
@@ -684,7 +693,8 @@ def plan_device_bytes(plan: Plan) -> int:
 
 
 # ------------------------------------------------------------------ graphs
-EXPORTED += ["fxc_graph_begin", "fxc_graph_end", "fxc_graph_launch", "fxc_graph_destroy"]
+EXPORTED += ["fxc_graph_begin", "fxc_graph_end", "fxc_graph_launch", "fxc_graph_destroy",
+             "fxc_plan_live_graphs", "fxc_plan_check_device_error"]
 
 
 def _glib():
@@ -692,7 +702,9 @@ def _glib():
     if not getattr(L, "_graph_sigs", False):
         vp, i32 = C.c_void_p, C.c_int32
         for name, args in (("fxc_graph_begin", [vp]), ("fxc_graph_end", [vp, _P(vp)]),
-                           ("fxc_graph_launch", [vp, vp]), ("fxc_graph_destroy", [vp])):
+                           ("fxc_graph_launch", [vp, vp]), ("fxc_graph_destroy", [vp]),
+                           ("fxc_plan_live_graphs", [vp, _P(i32)]),
+                           ("fxc_plan_check_device_error", [vp, vp])):
             f = getattr(L, name)
             f.argtypes = args
             f.restype = i32
@@ -721,6 +733,23 @@ class Graph:
         if self.h:
             _glib().fxc_graph_destroy(self.h)
             self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def plan_live_graphs(plan: "Plan") -> int:
+    n = C.c_int32()
+    check(_glib().fxc_plan_live_graphs(plan.h, C.byref(n)))
+    return n.value
+
+
+def plan_check_device_error(plan: "Plan", stream=None):
+    """Raises FxcError when a fused kernel's bounded hand-over wait ran out."""
+    check(_glib().fxc_plan_check_device_error(plan.h, stream.ptr if stream else None))
 
 
 # ---------------------------------------------------------------- autotune

@@ -142,6 +142,8 @@ const char *fxc_version(void);
 const char *fxc_strerror(fxc_status s);
 const char *fxc_last_error(void);
 fxc_status fxc_device_count(int32_t *n);
+/* Free and total memory of the current device (hipMemGetInfo). */
+fxc_status fxc_device_mem_info(uint64_t *free_bytes, uint64_t *total_bytes);
 fxc_status fxc_dev_alloc(size_t bytes, void **dptr);
 fxc_status fxc_dev_free(void *dptr);
 fxc_status fxc_memcpy_h2d(void *dst, const void *src, size_t bytes, void *stream);
@@ -389,12 +391,31 @@ fxc_status fxc_plan_set_tuning(fxc_plan *plan, const fxc_tune_report *cfg);
  * `stream` (non-NULL) may be made, with plan timing off and each plan's
  * geometric table already uploaded by an earlier call with the same table;
  * the captured work never synchronises or clears memory with a memset node.
- * Replays read and write the same device buffers as the captured calls. */
+ * Replays read and write the same device buffers as the captured calls:
+ * the caller's inputs and outputs, and the scratch of every plan whose work
+ * was captured.  While a graph that captured a plan is alive:
+ *   - a call on that plan that would grow its scratch, or that brings a
+ *     different geometric table, fails with FXC_ERR_INVALID_ARG before any
+ *     launch (size the scratch with the largest call before capturing);
+ *   - fxc_plan_destroy returns at once and the plan's memory is released
+ *     when the last such graph is destroyed.
+ * A call that would grow plan scratch inside a capture fails the same way. */
 typedef struct fxc_graph fxc_graph;
 fxc_status fxc_graph_begin(void *stream);
 fxc_status fxc_graph_end(void *stream, fxc_graph **out);
 fxc_status fxc_graph_launch(fxc_graph *g, void *stream);
 fxc_status fxc_graph_destroy(fxc_graph *g);
+/* Live graphs holding the plan's buffers. */
+fxc_status fxc_plan_live_graphs(const fxc_plan *plan, int32_t *n);
+
+/* Device error word of a plan (no reference counterpart).  The fused MFMA
+ * kernels bound every wait between their producer and consumer waves; a
+ * workgroup whose wait runs out writes NaN partial sums and sets the plan's
+ * error word.  fxc_sector_run, fxc_pipeline_wait and the server's job
+ * completion read it and fail with FXC_ERR_HIP; callers of
+ * fxc_sector_run_device read and clear it with this call once their work
+ * on `stream` is done (it synchronises `stream`). */
+fxc_status fxc_plan_check_device_error(fxc_plan *plan, void *stream);
 
 /* ----------------------------------------------------------------------- */
 /* Serving (no reference counterpart: the reference runs one scan per       */
