@@ -15,7 +15,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfxcorr.so")
 
 SOURCES = ["fxc_device.hip", "fxc_mfma.hip", "fxc_phased.hip", "fxc_large.hip", "fxc_chain.hip",
-           "fxc_synth.hip", "fxc_api.cpp", "fxc_server.cpp",
+           "fxc_synth.hip", "fxc_noalloc.hip", "fxc_api.cpp", "fxc_server.cpp",
            "fxc_io.cpp", "fxc_fringe.cpp", "fxc_geom.cpp",
            "fxc_auto.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

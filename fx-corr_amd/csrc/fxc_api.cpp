@@ -1577,6 +1577,87 @@ extern "C" fxc_status fxc_sector_run_device(fxc_plan *p, const uint8_t *d_raw1,
     return enqueue_sector(p, d_raw1, d_raw2, first_frame, nf, *delay, d_acc, d_raw_out, s);
 }
 
+// The sector path of fxc_sector_run_device with every scratch buffer in
+// caller-given device memory (fxc_noalloc.hip passes its static array):
+// [geometric table | reduction chunk sums | workgroup partial slots].  Only
+// the paths whose whole scratch is the partial slots run here: ACF/XCF plans
+// of the fused kernels without the plain form's phase-chain table.  The
+// grid is the plan's, capped by the slots that fit.  Allocates nothing and
+// never synchronises.
+extern "C" fxc_status fxc_internal_sector_noalloc(fxc_plan *p, const uint8_t *d_raw1,
+                                                  const uint8_t *d_raw2, int64_t first_frame,
+                                                  int64_t nf, const fxc_delay_cfg *delay,
+                                                  double *d_acc, void *stream, uint8_t *scratch,
+                                                  size_t scratch_bytes, size_t table_bytes,
+                                                  int device) {
+    if (!p || !delay || !d_acc || !scratch) return fail(FXC_ERR_INVALID_ARG, "null argument");
+    if (nf < 0) return fail(FXC_ERR_INVALID_ARG, "nf < 0");
+    if (nf > 0 && (!d_raw1 || !d_raw2)) return fail(FXC_ERR_INVALID_ARG, "null raw buffer");
+    if (device != p->device)
+        return fail(FXC_ERR_INVALID_ARG, "the plan was created on device %d but device %d is current",
+                    p->device, device);
+    if (p->desc.products & (FXC_PRODUCT_PHASED | FXC_PRODUCT_RAW_OUT))
+        return fail(FXC_ERR_UNSUPPORTED,
+                    "the allocation-free entry runs ACF/XCF plans; phased plans keep per-chunk "
+                    "spectra and rotation chains in plan scratch (use fxc_sector_run_device)");
+    if (p->fft_form == 2)
+        return fail(FXC_ERR_UNSUPPORTED,
+                    "fft_len %d runs its four-step FFT through per-plan chunk scratch "
+                    "(use fxc_sector_run_device)", p->N);
+    if (xcf_uses_chain(p))
+        return fail(FXC_ERR_UNSUPPORTED,
+                    "the plain form's XCF over %lld overlap bins replays the reference's f32 phase "
+                    "chains through a per-plan table (use fxc_sector_run_device, or the chosen form)",
+                    (long long)(p->desc.a1e - p->desc.a1s));
+    hipStream_t s = (hipStream_t)stream;
+    const double *dtab = nullptr;
+    if (delay->geom_table && delay->geom_table_len > 0) {
+        const size_t tb = (size_t)delay->geom_table_len * sizeof(double);
+        if (tb > table_bytes)
+            return fail(FXC_ERR_INVALID_ARG,
+                        "geometric table of %lld entries exceeds the allocation-free entry's limit "
+                        "of %zu entries", (long long)delay->geom_table_len, table_bytes / sizeof(double));
+        if (stream_capturing(s))
+            return fail(FXC_ERR_INVALID_ARG,
+                        "the allocation-free entry copies the geometric table from host memory on "
+                        "each call, which a graph capture cannot hold; capture fxc_sector_run_device");
+        HIP_TRY(hipMemcpyAsync(scratch, delay->geom_table, tb, hipMemcpyHostToDevice, s));
+        dtab = (const double *)scratch;
+    }
+    const size_t acc_bytes = (size_t)kNumProducts * p->half * sizeof(double);
+    if (nf <= 0 || !p->decode_ok[0] || !p->decode_ok[1]) {
+        HIP_TRY(launch_fill_zero(d_acc, acc_bytes, s));
+        return FXC_OK;
+    }
+    const int kReduceChunks = 64;
+    const size_t slot = acc_bytes;
+    const int64_t slots = (int64_t)((scratch_bytes - table_bytes) / slot) - kReduceChunks;
+    if (slots < 1)
+        return fail(FXC_ERR_UNSUPPORTED, "fft_len %d: %zu-byte partial slots do not fit the %zu-byte scratch",
+                    p->N, slot, scratch_bytes);
+    double *red = (double *)(scratch + table_bytes);
+    double *part = red + (size_t)kReduceChunks * kNumProducts * p->half;
+    SectorArgs a;
+    fill_args(p, a);
+    fill_delay(*delay, dtab, a.dly);
+    a.raw1 = d_raw1;
+    a.raw2 = d_raw2;
+    a.first_frame = first_frame;
+    a.nf = nf;
+    const int64_t wg_cap = std::min<int64_t>(std::max(1, p->max_wg / p->grid_div), slots);
+    int64_t fpw = (nf + wg_cap - 1) / wg_cap;
+    const int batch = p->fft_form == 1 ? 8 : kFrameBatch;
+    fpw = ((fpw + batch - 1) / batch) * batch;
+    if (fpw < batch) fpw = batch;
+    const int n_wg = (int)((nf + fpw - 1) / fpw);
+    a.frames_per_wg = fpw;
+    a.part = part;
+    if (p->fft_form == 1) HIP_TRY(launch_sector_mfma(a, p->desc.precision, n_wg, s));
+    else HIP_TRY(launch_sector(a, p->desc.precision, n_wg, s));
+    HIP_TRY(launch_reduce(part, n_wg, p->half, d_acc, red, kReduceChunks, s));
+    return FXC_OK;
+}
+
 // One chunk of the whole-scan cross spectrum (main.rs:1760-1849: the XCF
 // of nf frames on the plan's output grid), scaled by 1/N^2 and added into
 // ac (main.rs:1751, 1850).  Frames that fail to decode add nothing.

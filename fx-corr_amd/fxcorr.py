@@ -30,7 +30,8 @@ EXPORTED = [
     "fxc_event_record", "fxc_event_elapsed_ms", "fxc_parse_shuffle", "fxc_parse_levels",
     "fxc_quantization_mean_power", "fxc_compute_band_alignment", "fxc_sector_partition",
     "fxc_split_delay", "fxc_plan_create", "fxc_plan_destroy", "fxc_plan_get_info",
-    "fxc_sector_run", "fxc_sector_run_device", "fxc_plan_set_timing", "fxc_plan_get_timing",
+    "fxc_sector_run", "fxc_sector_run_device", "fxc_sector_run_device_noalloc",
+    "fxc_plan_set_timing", "fxc_plan_get_timing",
     "fxc_decode_frames", "fxc_rfft_frames", "fxc_irfft_frames", "fxc_delay_eval",
     "fxc_quantise_frames", "fxc_synth_station_device", "fxc_reader_open", "fxc_reader_read",
     "fxc_reader_close", "fxc_word_aligned_start", "fxc_normalize_sector",
@@ -151,6 +152,7 @@ def lib():
             "fxc_sector_run": ([vp, vp, sz, vp, sz, i64, i64, _P(DelayCfg), vp, vp, vp, vp, vp,
                                 _P(i64)], i32),
             "fxc_sector_run_device": ([vp, vp, vp, i64, i64, _P(DelayCfg), vp, vp, vp], i32),
+            "fxc_sector_run_device_noalloc": ([vp, vp, vp, i64, i64, _P(DelayCfg), vp, vp], i32),
             "fxc_plan_set_timing": ([vp, i32], i32),
             "fxc_plan_get_timing": ([vp, _P(d), _P(i64)], i32),
             "fxc_decode_frames": ([vp, i32, vp, sz, i64, vp, vp, _P(i64)], i32),
@@ -353,6 +355,12 @@ class Plan:
         check(lib().fxc_sector_run_device(self.h, C.c_void_p(d_raw1), C.c_void_p(d_raw2),
                                           first_frame, nf, C.byref(cfg), C.c_void_p(d_acc),
                                           C.c_void_p(d_raw_out or 0), C.c_void_p(stream or 0)))
+
+    def sector_run_device_noalloc(self, d_raw1, d_raw2, first_frame, nf, cfg, d_acc, stream=None):
+        """fxc_sector_run_device_noalloc: scratch in the library's static device array."""
+        check(lib().fxc_sector_run_device_noalloc(self.h, C.c_void_p(d_raw1), C.c_void_p(d_raw2),
+                                                  first_frame, nf, C.byref(cfg), C.c_void_p(d_acc),
+                                                  C.c_void_p(stream or 0)))
 
     def set_timing(self, on: bool):
         check(lib().fxc_plan_set_timing(self.h, int(on)))

@@ -218,6 +218,29 @@ fxc_status fxc_sector_run_device(fxc_plan *plan, const uint8_t *d_raw1,
                                  double *d_acc, uint8_t *d_raw_out,
                                  void *stream);
 
+/* Allocation-free variant of fxc_sector_run_device.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The call allocates and frees nothing and never synchronises;
+ * its scratch (geometric table, workgroup partial sums, reduction sums) is
+ * one static device array of FXC_NOALLOC_SCRATCH_BYTES per device, held
+ * until the process ends.  Same arguments and d_acc layout as
+ * fxc_sector_run_device, for ACF/XCF plans of fft_len <= 16384 whose XCF
+ * needs no phase-chain table (the chosen form, or the plain form with at
+ * most 512 overlap bins); other plans fail with FXC_ERR_UNSUPPORTED, a
+ * geometric table longer than FXC_NOALLOC_MAX_TABLE entries with
+ * FXC_ERR_INVALID_ARG.  The table is copied from host memory on each call,
+ * so a call with a table cannot be captured into a graph.
+ * Every call on one device shares that device's array: calls that overlap
+ * on one device (other streams, other host threads) race and must be
+ * serialised by the caller.  The plan's current device must be current. */
+#define FXC_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+#define FXC_NOALLOC_MAX_TABLE 8192
+fxc_status fxc_sector_run_device_noalloc(fxc_plan *plan, const uint8_t *d_raw1,
+                                         const uint8_t *d_raw2, int64_t first_frame,
+                                         int64_t nf, const fxc_delay_cfg *delay,
+                                         double *d_acc, void *stream);
+
 /* ----------------------------------------------------------------------- */
 /* Whole-scan cross spectrum (the reference's --fringe XCF pipeline,        */
 /* main.rs:1692-1947, replacing the per-chunk rayon map/reduce at           */
