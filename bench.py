@@ -458,6 +458,190 @@ def batch_bench(stream, scale=0.125, reps=3, seed=48):
     return out
 
 
+def host_identity(threads_used):
+    """CPU model and logical core count of the machine the baseline ran on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = None
+    return {"cpu_model": model, "logical_cores": os.cpu_count(), "cores_available": affinity,
+            "threads_used": threads_used}
+
+
+def amp_phase(got, ref):
+    """Per-channel amplitude error ||got|-|ref||/|ref| and phase error
+    |arg(got conj(ref))| (radians) over channels with ref != 0: max and RMS,
+    and the max over channels >= 0.1x the reference's RMS magnitude."""
+    got = np.asarray(got).astype(np.complex128)
+    ref = np.asarray(ref).astype(np.complex128)
+    m = np.abs(ref) > 0
+    g, r = got[m], ref[m]
+    amp = np.abs(np.abs(g) - np.abs(r)) / np.abs(r)
+    ph = np.abs(np.angle(g * np.conj(r)))
+    strong = np.abs(r) >= 0.1 * np.sqrt(np.mean(np.abs(r) ** 2))
+    return {"amp_max": float(amp.max()), "amp_rms": float(np.sqrt(np.mean(amp ** 2))),
+            "phase_max_rad": float(ph.max()), "phase_rms_rad": float(np.sqrt(np.mean(ph ** 2))),
+            "amp_max_strong": float(amp[strong].max()),
+            "phase_max_strong_rad": float(ph[strong].max())}
+
+
+def sector_parity(acc, ref, half, tol):
+    """One sector's device sums ([ph|11|22|12re|12im] x half) against an
+    oracle result dict: per-channel amplitude/phase errors of ACF1, ACF2, XCF."""
+    acc = np.asarray(acc).reshape(F.N_PRODUCTS, half)
+    got = {"acf1": acc[1], "acf2": acc[2], "xcf": acc[3] + 1j * acc[4]}
+    want = {"acf1": ref["acc11"], "acf2": ref["acc22"], "xcf": ref["acc12"]}
+    out = {k: {kk: float(f"{vv:.3e}") for kk, vv in amp_phase(got[k], want[k]).items()}
+           for k in got}
+    out["tol"] = tol
+    out["within_tol"] = all(v["amp_max_strong"] < tol and v["phase_max_strong_rad"] < tol
+                            for k, v in out.items() if isinstance(v, dict))
+    return out
+
+
+def c1_cpu_legs(stream, threads, seed=0):
+    """The 64 Msps 2-bit FFT-1024 one-second two-station case on the host
+    cores: compute only (input in RAM) and file-inclusive (both stations'
+    .raw files read from disk at their word-aligned starts, then the same
+    sector).  Input: device-generated correlated 2-bit pair (rho 0.02, B
+    delayed 37 samples), written to a temporary directory first."""
+    import tempfile
+
+    import fxref as R
+
+    fs, n, bits = 64e6, 1024, 2
+    samples = int(fs)
+    bpf = n * bits // 8
+    d1, d2 = F.DeviceBuffer(samples * bits // 8), F.DeviceBuffer(samples * bits // 8)
+    F.synth_station_device(d1.ptr, samples, bits, None, False, seed, 0, 0.02, 37, stream.ptr)
+    F.synth_station_device(d2.ptr, samples, bits, None, False, seed, 1, 0.02, 37, stream.ptr)
+    stream.sync()
+    h1 = d1.download(np.zeros(samples * bits // 8, dtype=np.uint8))
+    h2 = d2.download(np.zeros(samples * bits // 8, dtype=np.uint8))
+    d1.free()
+    d2.free()
+    start2 = (37 // 16) * 16                                       # word-aligned seek: 32
+    nf = (samples - start2) // n                                   # 62,499 frames
+    desc = dict(fft_len=n, bits1=bits, bits2=bits, levels1=LEVELS, levels2=LEVELS,
+                shuffle1=np.arange(32, dtype=np.int32), shuffle2=np.arange(32, dtype=np.int32),
+                lsb1=0, lsb2=0, a1s=0, a1e=n // 2, a2s=0, df_hz=fs / n, need_acf=1, need_xcf=1,
+                cpu_threads=max(1, threads - 2))
+    rcfg = R.make_delay_cfg(np.zeros(3), frame_dt=n / fs, fs=fs, delay_user_samples=37.3,
+                            d_seek=start2 / fs, lo1_hz=6.7e9, lo2_hz=6.7e9)
+    o1 = h1[:nf * bpf]
+    o2 = h2[start2 * bits // 8:start2 * bits // 8 + nf * bpf]
+    t0 = time.perf_counter()
+    R.sector_run(desc, o1, o2, 0, nf, rcfg, mode=R.MODE_F32, threads=threads)
+    el_c = time.perf_counter() - t0
+    with tempfile.TemporaryDirectory() as tmp:
+        p1, p2 = os.path.join(tmp, "A.raw"), os.path.join(tmp, "B.raw")
+        h1.tofile(p1)
+        h2.tofile(p2)
+        t0 = time.perf_counter()
+        r1 = np.fromfile(p1, dtype=np.uint8, count=nf * bpf, offset=0)
+        r2 = np.fromfile(p2, dtype=np.uint8, count=nf * bpf, offset=start2 * bits // 8)
+        R.sector_run(desc, r1, r2, 0, nf, rcfg, mode=R.MODE_F32, threads=threads)
+        el_f = time.perf_counter() - t0
+    return {"compute_only": round(nf * n / el_c / 1e6, 1),
+            "file_inclusive": round(nf * n / el_f / 1e6, 1), "unit": "Msamples/s/station",
+            "sample": f"64 Msps 2-bit FFT 1024, {nf} frames (1 s, seek 32), oracle FP32 sector "
+                      f"runner, {threads} threads; file-inclusive reads both .raw files (just "
+                      f"written, so from the page cache) with the sector; "
+                      f"{el_c:.2f} s / {el_f:.2f} s wall"}
+
+
+def c3_bench(stream, reps=2, seed=2303):
+    """Precision-stress workload: 512 Msps, A 4-bit / B 1-bit, FFT 65536,
+    16 MHz centre offset (band shift 2048, overlap 30,720), continuum
+    coherence 1e-3 plus a 3-channel Gaussian line of peak coherence 0.05 at a
+    seeded channel, 30 s = 234,375 frames in 30 sectors, generated on the
+    device.  Times the 30-sector job (ACF + XCF) in both forms with the input
+    resident in HBM, and checks each form's integrated spectrum: the line is
+    the strongest channel with the expected coherence, the continuum has the
+    expected mean coherence, and the forms agree."""
+    n, fs, bits = 65536, 512e6, (4, 1)
+    total = int(30 * fs // n)
+    sectors = F.sector_partition(total, n / fs)
+    band = F.band_alignment(n, fs, 8400.0, 8384.0, 256.0, 256.0)
+    line_bin = int(np.random.default_rng(seed).integers(2000, 28000))
+    lv = (_uniform_levels(4), _uniform_levels(1))
+    bpf = (n * bits[0] // 8, n * bits[1] // 8)
+    d1, d2 = F.DeviceBuffer(total * bpf[0]), F.DeviceBuffer(total * bpf[1])
+    t0 = time.perf_counter()
+    F.synth_spectral_pair_device(d1.ptr, d2.ptr, n, 0, total, bits=bits, shift_bins=band[0],
+                                 rho_continuum=1e-3, rho_line=0.05, line_bin=line_bin,
+                                 line_fwhm_bins=3.0, clip_sigma=4.0, seed=seed, stream=stream.ptr)
+    gen_s = time.perf_counter() - t0
+    cfg = F.make_delay_cfg(None, frame_dt=n / fs, fs=fs)
+    half = n // 2 + 1
+    acc = F.DeviceBuffer(len(sectors) * F.N_PRODUCTS * half * 8)
+    out, tot = {}, {}
+    for form, prec in (("plain", F.PRECISION_PLAIN_FP32), ("chosen", F.PRECISION_CHOSEN)):
+        plan = F.Plan(n, bits=bits, levels=list(lv), band=band, df_hz=fs / n,
+                      products=F.PRODUCT_ACF | F.PRODUCT_XCF, precision=prec)
+
+        def job():
+            first = 0
+            for i, nf in enumerate(sectors):
+                plan.sector_run_device(d1.ptr + first * bpf[0], d2.ptr + first * bpf[1], first, nf,
+                                       cfg, acc.ptr + i * F.N_PRODUCTS * half * 8, stream=stream.ptr)
+                first += nf
+
+        job()
+        stream.sync()
+        plan.set_timing(True)
+        e0, e1 = F.Event(), F.Event()
+        e0.record(stream)
+        for _ in range(reps):
+            job()
+        e1.record(stream)
+        stream.sync()
+        ms = e0.elapsed_ms(e1) / reps
+        kms, launches = plan.get_timing()
+        plan.set_timing(False)
+        F.plan_check_device_error(plan, stream)
+        a = acc.download(np.zeros(len(sectors) * F.N_PRODUCTS * half)).reshape(
+            len(sectors), F.N_PRODUCTS, half).sum(axis=0)
+        tot[form] = a
+        out[form] = {"value": round(total * n / (ms * 1e-3) / 1e6, 1), "ms_per_job": round(ms, 3),
+                     "sector_ms": round(kms / max(launches, 1), 3)}
+        plan.close()
+        print(f"[bench] C3 {form}: {out[form]['value']:,.0f} Msamples/s/station, "
+              f"{ms:.1f} ms per 30-sector job", file=sys.stderr, flush=True)
+    acc.free()
+    d1.free()
+    d2.free()
+    lo, hi = band[1], band[2]
+    far = np.abs(np.arange(lo, hi) - line_bin) > 20
+    checks = {}
+    for form, a in tot.items():
+        coh = (a[3, lo:hi] + 1j * a[4, lo:hi]) / np.sqrt(a[1, lo:hi] * a[2, lo:hi])
+        checks[form] = {"peak_channel": int(np.argmax(np.abs(coh))) + lo,
+                        "line_coherence": round(float(coh[line_bin - lo].real), 5),
+                        "continuum_coherence": float(f"{np.mean(coh[far].real):.4e}")}
+    x = [tot[f][3, lo:hi] + 1j * tot[f][4, lo:hi] for f in ("chosen", "plain")]
+    rms = np.sqrt(np.mean(np.abs(x[1]) ** 2))
+    agree = amp_phase(x[0], x[1])
+    return {"workload": "512 Msps, A 4-bit / B 1-bit, FFT 65536, 16 MHz offset (shift 2048, "
+                        "overlap 30720), continuum coherence 1e-3 + 3-channel line of 0.05, "
+                        f"30 s = {total} frames in 30 sectors, input resident in HBM",
+            "forms": out, "chosen_over_plain": round(out["chosen"]["value"] / out["plain"]["value"], 3),
+            "line_channel": line_bin, "checks": checks,
+            "chosen_vs_plain_xcf": {"max_abs_over_rms": float(f"{np.max(np.abs(x[0] - x[1])) / rms:.3e}"),
+                                    "amp_max_strong": float(f"{agree['amp_max_strong']:.3e}"),
+                                    "phase_max_strong_rad": float(f"{agree['phase_max_strong_rad']:.3e}")},
+            "generation_s": round(gen_s, 2), "reps": reps}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -465,6 +649,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-frames", type=int, default=1_000_000)
+    ap.add_argument("--no-c1", action="store_true",
+                    help="skip the 64 Msps (C1) compute-only and file-inclusive CPU legs")
     ap.add_argument("--forms", default="plain,chosen")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="after timing, one device-to-device copy of this many MiB "
@@ -485,6 +671,9 @@ def main():
                     help="also time both forms at FFT sizes 256..65536 (adds 'fft_sizes')")
     ap.add_argument("--phased", action="store_true",
                     help="also time the phased-array path at FFT 1024/4096/16384/65536 (adds 'phased')")
+    ap.add_argument("--c3", action="store_true",
+                    help="also time the 512 Msps 4/1-bit FFT-65536 30-sector precision-stress "
+                         "job with its line + continuum signal (adds 'c3')")
     ap.add_argument("--batch", action="store_true",
                     help="also time the 48-scan uneven batch captured in one graph (adds 'batch')")
     ap.add_argument("--batch-scale", type=float, default=0.125,
@@ -579,6 +768,7 @@ def main():
                              tflops=flops / (avg_launch_ms * 1e-3) / 1e12,
                              gbs=bytes_in / (avg_launch_ms * 1e-3) / 1e9,
                              acc11_sum=float(out[half:2 * half].sum()),
+                             sector1=out[:F.N_PRODUCTS * half].copy(),
                              tuning=tuning, kernel=kernel_key)
         print(f"[bench] {form}: {value:,.0f} Msamples/s/station, {ms_step:.3f} ms/step, "
               f"fused kernel {avg_launch_ms:.3f} ms/launch ({kernel_key}, {tuning['source']})",
@@ -667,6 +857,7 @@ def main():
         plan.close()
 
     cpu = None
+    parity = None
     if not args.no_cpu_baseline:
         import fxref as R
 
@@ -678,19 +869,36 @@ def main():
                     cpu_threads=max(1, threads - 2))
         rcfg = R.make_delay_cfg(table, **dcfg)
         t0 = time.perf_counter()
-        R.sector_run(desc, h1, h2, 0, nfc, rcfg, mode=R.MODE_F32, threads=threads)
+        o32 = R.sector_run(desc, h1, h2, 0, nfc, rcfg, mode=R.MODE_F32, threads=threads)
         el = time.perf_counter() - t0
         cpu = {"value": nfc * N / el / 1e6, "unit": "Msamples/s/station", "cores": threads,
                "kind": "port",
                "sample": f"first {nfc} frames (sector 1, {nfc * N / FS:.3f} s) of the same "
                          f"workload, oracle FP32 sector runner, {threads} threads, "
-                         f"{el:.1f} s wall"}
+                         f"{el:.1f} s wall, input in RAM (compute only)",
+               "host": host_identity(threads)}
         print(f"[bench] cpu baseline: {cpu['value']:.1f} Msamples/s/station "
               f"({threads} threads, {el:.1f} s)", file=sys.stderr, flush=True)
+        if nfc == sectors[0]:
+            # The same oracle runs check the timed runs' own sector-1 sums:
+            # plain against the reference-faithful FP32 mode, chosen against FP64.
+            o64 = R.sector_run(desc, h1, h2, 0, nfc, rcfg, mode=R.MODE_F64, threads=threads)
+            parity = {}
+            for form, ref, tol in (("plain", o32, 1e-5), ("chosen", o64, 1e-4)):
+                if form in results:
+                    parity[form] = sector_parity(results[form]["sector1"], ref, half, tol)
+                    parity[form]["oracle"] = "FP32 (reference-faithful)" if form == "plain" else "FP64"
+            parity["sample"] = (f"sector 1 ({nfc} frames) of each form's timed run against the "
+                                f"oracle on the same bytes")
+            print(f"[bench] parity: {json.dumps(parity)}", file=sys.stderr, flush=True)
+        if not args.no_c1:
+            cpu["c1"] = c1_cpu_legs(stream, threads)
+            print(f"[bench] cpu baseline C1: {json.dumps(cpu['c1'])}", file=sys.stderr, flush=True)
 
     sizes = size_sweep(stream) if args.sizes else None
     phased = phased_sweep(stream) if args.phased else None
     batch = batch_bench(stream, args.batch_scale) if args.batch else None
+    c3 = c3_bench(stream) if args.c3 else None
 
     r = results[main_form]
     line = {
@@ -723,6 +931,7 @@ def main():
                      "kernel": f"{KERNEL_LABELS[r['kernel']]} ({r['kernel']})",
                      "hbm_frac": round(r["gbs"] / HBM_PEAK_GBS, 4)},
         "cpu_baseline": cpu,
+        "parity": parity,
         "pcie_inclusive": pcie,
         "serving": serving,
     }
@@ -732,6 +941,8 @@ def main():
         line["phased"] = phased
     if batch is not None:
         line["batch"] = batch
+    if c3 is not None:
+        line["c3"] = c3
     if "plain" in results and "chosen" in results:
         line["chosen_over_plain"] = round(results["chosen"]["value"] / results["plain"]["value"], 3)
     print(json.dumps(line), flush=True)

@@ -1953,6 +1953,99 @@ extern "C" fxc_status fxc_synth_station_device(uint8_t *d_out, int64_t nsamples,
     return FXC_OK;
 }
 
+extern "C" fxc_status fxc_synth_spectral_pair_device(const fxc_synth_spectral_desc *d,
+                                                     int64_t first_frame, int64_t nframes,
+                                                     uint8_t *d_raw1, uint8_t *d_raw2,
+                                                     void *stream) {
+    if (!d || !d_raw1 || !d_raw2 || nframes < 0 || first_frame < 0)
+        return fail(FXC_ERR_INVALID_ARG, "bad spectral synth arguments");
+    const int N = d->fft_len;
+    if (!is_pow2(N) || N < 64 || N > 65536)
+        return fail(FXC_ERR_UNSUPPORTED, "spectral synth: fft_len %d is not a power of two 64..65536", N);
+    for (int i = 0; i < 2; ++i) {
+        const int b = d->bits[i];
+        if (b != 1 && b != 2 && b != 4 && b != 8)
+            return fail(FXC_ERR_UNSUPPORTED, "spectral synth: bits must be 1, 2, 4 or 8");
+        int seen[32] = {0};
+        for (int k = 0; k < 32; ++k)
+            if (d->shuffle_internal[i][k] < 0 || d->shuffle_internal[i][k] > 31 ||
+                seen[d->shuffle_internal[i][k]]++)
+                return fail(FXC_ERR_INVALID_ARG, "shuffle is not a permutation of 0..31");
+    }
+    if (!(d->rho_continuum >= 0.0 && d->rho_continuum < 1.0) || !(d->rho_line < 1.0) ||
+        std::llabs(d->shift_bins) > N / 2 || (d->rho_line > 0.0 && !(d->line_fwhm_bins > 0.0)))
+        return fail(FXC_ERR_INVALID_ARG, "spectral synth: coherence outside [0, 1), shift beyond "
+                                         "the half spectrum or line without width");
+    if ((first_frame + nframes) > ((int64_t)1 << 38))
+        return fail(FXC_ERR_INVALID_ARG, "spectral synth: frame index beyond 2^38");
+    if (nframes == 0) return FXC_OK;
+    hipStream_t s = (hipStream_t)stream;
+    // The inverse FFT runs through a private plan of this length (its
+    // twiddles and, at N >= 32768, its chunk scratch).
+    fxc_plan_desc pd;
+    memset(&pd, 0, sizeof pd);
+    pd.fft_len = N;
+    static const double lv1[2] = {-1.0, 1.0};
+    for (int i = 0; i < 2; ++i) {
+        pd.st[i].bits = 1;
+        for (int k = 0; k < 32; ++k) pd.st[i].shuffle_internal[k] = k;
+        pd.st[i].levels = lv1;
+        pd.st[i].nlevels = 2;
+    }
+    pd.a1s = 0;
+    pd.a1e = N / 2;
+    pd.a2s = 0;
+    pd.df_hz = 1.0;
+    pd.products = FXC_PRODUCT_ACF;
+    pd.w1 = pd.w2 = 1.0;
+    pd.bit_out = 1;
+    pd.precision = FXC_PRECISION_PLAIN_FP32;
+    fxc_plan *p = nullptr;
+    fxc_status st = fxc_plan_create(&pd, &p);
+    if (st != FXC_OK) return st;
+    struct Guard {
+        fxc_plan *p;
+        hipStream_t s;
+        ~Guard() {
+            (void)hipStreamSynchronize(s);
+            fxc_plan_destroy(p);
+        }
+    } guard{p, s};
+    const int64_t chunk = p->fft_form == 2
+                              ? p->chunk
+                              : std::max<int64_t>(1, ((int64_t)256 << 20) / ((int64_t)N * 4));
+    if (p->d_tmp_in.ensure((size_t)chunk * p->half * 8) != hipSuccess ||
+        p->d_tmp_out.ensure((size_t)chunk * N * 4) != hipSuccess)
+        return fail(FXC_ERR_NOMEM, "spectral synth: FFT scratch allocation failed");
+    SectorArgs a;
+    fill_args(p, a);
+    uint8_t *outs[2] = {d_raw1, d_raw2};
+    for (int stn = 0; stn < 2; ++stn) {
+        const int64_t bpf = (int64_t)N * d->bits[stn] / 8;
+        for (int64_t c0 = 0; c0 < nframes; c0 += chunk) {
+            const int64_t cn = std::min(chunk, nframes - c0);
+            float2 *spec = (float2 *)p->d_tmp_in.p;
+            float *x = (float *)p->d_tmp_out.p;
+            HIP_TRY(launch_synth_spectrum(spec, first_frame + c0, cn, p->M, stn, d->shift_bins, d->seed,
+                                          d->rho_continuum, d->rho_line > 0.0 ? d->rho_line : 0.0,
+                                          d->line_bin, d->line_fwhm_bins, s));
+            if (p->fft_form == 2)
+                HIP_TRY(launch_irfft_large(spec, cn, p->M1, p->M2,
+                                           (const float2 *)(p->M1 == 128 ? p->tw_l1.p : p->tw_l2.p),
+                                           (const float2 *)(p->M2 == 128 ? p->tw_l1.p : p->tw_l2.p),
+                                           (const float2 *)p->tw_lm.p, (const float2 *)p->tw_post.p,
+                                           (float2 *)p->d_x[0].p, (float2 *)p->d_y.p,
+                                           (float2 *)p->d_z[0].p, false, x, s));
+            else
+                HIP_TRY(launch_irfft_frames(a, spec, cn, x, false, s));
+            HIP_TRY(launch_synth_pack(x, cn * N, d->bits[stn], d->lsb[stn], d->shuffle_internal[stn],
+                                      d->thresh_sigma, d->clip_sigma, outs[stn] + c0 * bpf, s));
+        }
+    }
+    HIP_TRY(hipStreamSynchronize(s));
+    return FXC_OK;
+}
+
 // ============================================================ host pipeline
 // Streaming host input (the reader-thread + per-sector loop of
 // main.rs:2098-2213): sectors submitted from host memory are copied on a

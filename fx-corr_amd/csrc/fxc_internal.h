@@ -199,5 +199,12 @@ hipError_t launch_rfft_mfma(const SectorArgs &a, const float *in, int64_t nframe
 hipError_t launch_synth(uint8_t *d_out, int64_t nsamples, int bits, const int32_t *shuffle,
                         int lsb, uint64_t seed, int station, double rho, int64_t delay,
                         hipStream_t stream);
+hipError_t launch_synth_spectrum(float2 *out, int64_t first_frame, int64_t nframes, int M,
+                                 int station, int64_t shift, uint64_t seed, double rho_c,
+                                 double rho_l, double line_bin, double line_fwhm,
+                                 hipStream_t stream);
+hipError_t launch_synth_pack(const float *x, int64_t nsamples, int bits, int lsb,
+                             const int32_t *shuffle, double thresh, double clip, uint8_t *out,
+                             hipStream_t stream);
 
 }  // namespace fxc

@@ -33,7 +33,7 @@ EXPORTED = [
     "fxc_sector_run", "fxc_sector_run_device", "fxc_sector_run_device_noalloc",
     "fxc_plan_set_timing", "fxc_plan_get_timing",
     "fxc_decode_frames", "fxc_rfft_frames", "fxc_irfft_frames", "fxc_delay_eval",
-    "fxc_quantise_frames", "fxc_synth_station_device", "fxc_reader_open", "fxc_reader_read",
+    "fxc_quantise_frames", "fxc_synth_station_device", "fxc_synth_spectral_pair_device", "fxc_reader_open", "fxc_reader_read",
     "fxc_reader_close", "fxc_word_aligned_start", "fxc_normalize_sector",
     "fxc_cor_build_header", "fxc_cor_build_sector_header", "fxc_cor_create",
     "fxc_cor_write_sector", "fxc_cor_finalize",
@@ -468,6 +468,41 @@ class Event:
         ms = C.c_float()
         check(lib().fxc_event_elapsed_ms(C.c_void_p(self.ptr), C.c_void_p(end.ptr), C.byref(ms)))
         return ms.value
+
+
+class SynthSpectralDesc(C.Structure):
+    _fields_ = [("fft_len", C.c_int32), ("bits", C.c_int32 * 2), ("lsb", C.c_int32 * 2),
+                ("shuffle_internal", (C.c_int32 * 32) * 2), ("shift_bins", C.c_int64),
+                ("rho_continuum", C.c_double), ("rho_line", C.c_double), ("line_bin", C.c_double),
+                ("line_fwhm_bins", C.c_double), ("thresh_sigma", C.c_double),
+                ("clip_sigma", C.c_double), ("seed", C.c_uint64)]
+
+
+def synth_spectral_pair_device(d_raw1: int, d_raw2: int, fft_len: int, first_frame: int,
+                               nframes: int, bits=(2, 2), lsb=(False, False), shuffles=(None, None),
+                               shift_bins=0, rho_continuum=1e-3, rho_line=0.0, line_bin=0.0,
+                               line_fwhm_bins=3.0, thresh_sigma=0.98, clip_sigma=4.0, seed=0,
+                               stream=None):
+    """fxc_synth_spectral_pair_device: seeded pair with a coherence profile
+    (continuum + Gaussian line) and a band shift, frame by frame."""
+    d = SynthSpectralDesc()
+    d.fft_len = fft_len
+    for i in range(2):
+        d.bits[i] = bits[i]
+        d.lsb[i] = int(bool(lsb[i]))
+        sh = IDENTITY_SHUFFLE if shuffles[i] is None else shuffles[i]
+        d.shuffle_internal[i][:] = [int(x) for x in sh]
+    d.shift_bins = shift_bins
+    d.rho_continuum, d.rho_line = rho_continuum, rho_line
+    d.line_bin, d.line_fwhm_bins = line_bin, line_fwhm_bins
+    d.thresh_sigma, d.clip_sigma = thresh_sigma, clip_sigma
+    d.seed = seed
+    L = lib()
+    f = L.fxc_synth_spectral_pair_device
+    f.argtypes = [_P(SynthSpectralDesc), C.c_int64, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]
+    f.restype = C.c_int32
+    check(f(C.byref(d), first_frame, nframes, C.c_void_p(d_raw1), C.c_void_p(d_raw2),
+            C.c_void_p(stream or 0)))
 
 
 def synth_station_device(d_out: int, nsamples: int, bits: int, shuffle=None, lsb=False, seed=0,

@@ -378,6 +378,34 @@ fxc_status fxc_synth_station_device(uint8_t *d_out, int64_t nsamples, int32_t bi
                                     uint64_t seed, int32_t station, double rho,
                                     int64_t delay_samples, void *stream);
 
+/* A seeded two-station pair with a spectral coherence profile, built frame
+ * by frame in the frequency domain (frames are the correlator's independent
+ * units) and inverse-FFT'd on the device: in sky channel q (station-1 grid)
+ * the stations share a common signal with coherence
+ *   rho(q) = rho_continuum + (rho_line - rho_continuum) exp(-(q - line_bin)^2 / 2s^2),
+ * s = line_fwhm_bins / 2.3548 (no line when rho_line <= 0); station 2's
+ * channel k carries sky channel k - shift_bins (the band alignment's
+ * a2s - a1s).  Unit-variance samples are quantised per station (1-bit at 0,
+ * 2-bit at +-thresh_sigma, wider bits uniformly over +-clip_sigma with the
+ * levels l_c = c - (2^bits - 1)/2), odd samples negated when lsb, and packed
+ * as fxc_synth_station_device packs.  Writes frames first_frame ..
+ * first_frame + nframes - 1 (nframes * fft_len * bits / 8 bytes per
+ * station); the same frame always gets the same samples.  Allocates its
+ * FFT scratch and synchronises `stream` before returning. */
+typedef struct {
+    int32_t fft_len;
+    int32_t bits[2];
+    int32_t lsb[2];
+    int32_t shuffle_internal[2][32];
+    int64_t shift_bins;
+    double rho_continuum, rho_line, line_bin, line_fwhm_bins;
+    double thresh_sigma, clip_sigma;
+    uint64_t seed;
+} fxc_synth_spectral_desc;
+fxc_status fxc_synth_spectral_pair_device(const fxc_synth_spectral_desc *desc, int64_t first_frame,
+                                          int64_t nframes, uint8_t *d_raw1, uint8_t *d_raw2,
+                                          void *stream);
+
 /* Device memory held by a plan (tables + scratch), bytes. */
 fxc_status fxc_plan_device_bytes(const fxc_plan *plan, uint64_t *bytes);
 

@@ -96,6 +96,43 @@ def spectra_err(got, ref):
             "acc12": xcf_err(got["acc12"], ref["acc12"])}
 
 
+def amp_phase_err(got, ref):
+    """Per-channel amplitude and phase error of a spectrum against a reference
+    spectrum (the products written to .cor, main.rs:2660-2683).  Over channels
+    with ref != 0: amplitude error ||got|-|ref||/|ref| and phase error
+    |arg(got * conj(ref))| in radians, as max and RMS; "_strong" repeats the
+    max over channels at least 0.1x the reference's RMS magnitude (a channel
+    whose frame sum nearly cancels magnifies rounding noise that is
+    uncorrelated across frames)."""
+    got = np.asarray(got).astype(np.complex128)
+    ref = np.asarray(ref).astype(np.complex128)
+    m = np.abs(ref) > 0
+    if not np.any(m):
+        return dict(amp_max=0.0, amp_rms=0.0, phase_max=0.0, phase_rms=0.0, amp_max_strong=0.0,
+                    phase_max_strong=0.0, channels=0)
+    g, r = got[m], ref[m]
+    amp = np.abs(np.abs(g) - np.abs(r)) / np.abs(r)
+    ph = np.abs(np.angle(g * np.conj(r)))
+    rms = np.sqrt(np.mean(np.abs(r) ** 2))
+    s = np.abs(r) >= 0.1 * rms
+    return dict(amp_max=float(amp.max()), amp_rms=float(np.sqrt(np.mean(amp ** 2))),
+                phase_max=float(ph.max()), phase_rms=float(np.sqrt(np.mean(ph ** 2))),
+                amp_max_strong=float(amp[s].max()), phase_max_strong=float(ph[s].max()),
+                channels=int(m.sum()))
+
+
+def spectra_amp_phase(got, ref):
+    """amp_phase_err per product of two result dicts (acc11, acc22, acc12)."""
+    return {k: amp_phase_err(got[k], ref[k]) for k in ("acc11", "acc22", "acc12")}
+
+
+def fmt_amp_phase(d):
+    """One line per product: amplitude max/rms (relative), phase max/rms (rad)."""
+    return "; ".join(f"{k}: amp {v['amp_max']:.2e}/{v['amp_rms']:.2e} "
+                     f"phase {v['phase_max']:.2e}/{v['phase_rms']:.2e} rad"
+                     for k, v in d.items())
+
+
 def zero_pattern_equal(got, ref):
     return np.array_equal(np.asarray(got) == 0, np.asarray(ref) == 0)
 

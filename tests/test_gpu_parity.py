@@ -9,7 +9,8 @@ import pytest
 import fxcorr as F
 import fxref as R
 from _cases import (gpu_plan, gpu_run, make_case, oracle_run, out_decode_plan, rel_err,
-                    spectra_err, zero_pattern_equal)
+                    spectra_err, zero_pattern_equal, spectra_amp_phase,
+                    fmt_amp_phase)
 from _synth import IDENT, LEVELS_2BIT, uniform_levels
 
 pytestmark = pytest.mark.gpu
@@ -194,10 +195,15 @@ def test_quantise_nearest_level_edges(kind):
 def _compare(got, ref, tol, keys=("acc11", "acc22", "acc12")):
     # ACF: per-bin relative error.  XCF: per-bin relative on bins >= 0.1x the
     # spectrum RMS plus |error| <= tol x RMS everywhere (see _cases.xcf_err).
+    # Per-channel amplitude and phase errors are printed for every comparison
+    # and held to the same bar on channels >= 0.1x the RMS magnitude.
     errs = spectra_err(got, ref)
+    ap = spectra_amp_phase(got, ref)
+    print(f"[parity] tol {tol:.0e}: {fmt_amp_phase(ap)}", flush=True)
     for k in keys:
         assert zero_pattern_equal(got[k], ref[k]), k
         assert errs[k] < tol, (k, errs[k])
+        assert ap[k]["amp_max_strong"] < tol and ap[k]["phase_max_strong"] < tol, (k, ap[k])
 
 
 def test_64msps_2bit_fft1024_full_plain_and_chosen():
