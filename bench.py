@@ -72,9 +72,11 @@ def workload_setup(seed=2025):
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_1024msps_2bit_fft1024.json")
 # fused kernel of each form (the MFMA form by kernel organization variant)
 KERNEL_KEYS = {("chosen", 0): "k_sector_mf2", ("chosen", 1): "k_sector_mf3",
-               ("plain", 0): "k_sector<8, 64, 0>"}
+               ("chosen", 2): "k_sector_mf3<FP32 batches>", ("plain", 0): "k_sector<8, 64, 0>"}
 KERNEL_LABELS = {"k_sector_mf2": "fused decode+shift+MFMA four-step FFT+MAC, whole-frame waves",
                  "k_sector_mf3": "fused decode+shift+MFMA four-step FFT+MAC, producer/consumer waves",
+                 "k_sector_mf3<FP32 batches>": "fused decode+shift+MFMA four-step FFT+MAC, "
+                                               "producer/consumer waves, FP32 batch sums",
                  "k_sector<8, 64, 0>": "fused decode+shift+FFT+MAC (plain FP32)"}
 
 
@@ -696,7 +698,7 @@ def main():
     ap.add_argument("--tune", choices=("auto", "default"), default="auto",
                     help="auto: run the plan autotuner on 250,000 frames of the workload "
                          "before timing (untimed); default: the plans' default configuration")
-    ap.add_argument("--mfma-variant", type=int, choices=(0, 1), default=None,
+    ap.add_argument("--mfma-variant", type=int, choices=(0, 1, 2), default=None,
                     help="pin the MFMA kernel organization (0 whole-frame waves, "
                          "1 producer/consumer waves) instead of tuning it")
     ap.add_argument("--sizes", action="store_true",

@@ -167,6 +167,7 @@ struct fxc_plan {
     DevBuf d_chain;          // the reference's f32 phase chains of one slab/chunk of frames
     DevBuf d_fringe;         // one chunk's products for the whole-scan cross spectrum
     DevBuf d_err;            // device error word: bit 0 set by a workgroup whose hand-over wait ran out
+    DevBuf d_cacc;           // fused N = 1024 form: per-consumer FP64 totals (kMfConsumerAccBytes per workgroup)
     int max_wg = 0;
     std::vector<double> table_host;  // last uploaded geometric table
     // timing
@@ -187,7 +188,7 @@ struct fxc_plan {
                           &tw_l1, &tw_l2, &tw_lm, &d_x[0], &d_x[1], &d_y, &d_z[0], &d_z[1],
                           &d_params, &d_lshifts, &part, &red_scratch, &table_dev, &d_raw[0],
                           &d_raw[1], &d_acc, &d_raw_out, &d_tmp_in, &d_tmp_out, &d_shifts,
-                          &d_cb, &d_tbuf, &d_chain, &d_fringe, &d_err};
+                          &d_cb, &d_tbuf, &d_chain, &d_fringe, &d_err, &d_cacc};
         for (DevBuf *b : bufs) fn(*b);
     }
 };
@@ -503,6 +504,7 @@ static void fill_args(fxc_plan *p, SectorArgs &a) {
     a.mf_tab = (p->fft_form == 1 || p->mf_b1) ? p->mf_tab.p : nullptr;
     a.variant = p->variant;
     a.err = (int32_t *)p->d_err.p;
+    a.cons_acc = (double *)p->d_cacc.p;
 }
 
 // IEEE binary16 bits of a float, round to nearest even (normals, subnormals,
@@ -862,7 +864,8 @@ extern "C" fxc_status fxc_plan_create(const fxc_plan_desc *desc, fxc_plan **out)
     p->max_wg = p->fft_form == 1 ? cus
                 : p->fft_form == 2 ? kLargeGroups
                                    : std::max(64, (cus * 1024) / p->T);
-    if (p->part.ensure((size_t)p->max_wg * kNumProducts * half * sizeof(double)) != hipSuccess ||
+    if ((p->fft_form == 1 && p->d_cacc.ensure((size_t)p->max_wg * kMfConsumerAccBytes) != hipSuccess) ||
+        p->part.ensure((size_t)p->max_wg * kNumProducts * half * sizeof(double)) != hipSuccess ||
         p->red_scratch.ensure((size_t)64 * kNumProducts * half * sizeof(double)) != hipSuccess ||
         p->d_err.ensure(sizeof(int32_t)) != hipSuccess)
         return cleanup(fail(FXC_ERR_NOMEM, "scratch allocation failed"));
@@ -1468,7 +1471,7 @@ extern "C" fxc_status fxc_plan_autotune(fxc_plan *p, const uint8_t *d_raw1, cons
         // every wave carries whole frames).  The default is timed first.
         // N = 2048 has one organization.
         if (p->N == 1024)
-            for (int v : {1, 0}) cands.push_back({v, 1, 0, 0});
+            for (int v : {1, 2, 0}) cands.push_back({v, 1, 0, 0});
         else
             cands.push_back({1, 1, 0, 0});
     }
@@ -1547,7 +1550,9 @@ extern "C" fxc_status fxc_plan_set_tuning(fxc_plan *p, const fxc_tune_report *c)
     if (!p || !c) return fail(FXC_ERR_INVALID_ARG, "null argument");
     const bool v1_ok = (p->fft_form == 0 && sector_has_variant(p->N)) || p->fft_form == 1;
     const bool v0_ok = !(p->fft_form == 1 && p->N != 1024);
-    if ((c->variant != 0 && !(c->variant == 1 && v1_ok)) || (c->variant == 0 && !v0_ok))
+    const bool v2_ok = p->fft_form == 1 && p->N == 1024;
+    if ((c->variant == 1 && !v1_ok) || (c->variant == 0 && !v0_ok) || (c->variant == 2 && !v2_ok) ||
+        c->variant < 0 || c->variant > 2)
         return fail(FXC_ERR_INVALID_ARG, "variant %d not available for this plan", c->variant);
     if (c->grid_divisor != 1 && c->grid_divisor != 2 && c->grid_divisor != 4)
         return fail(FXC_ERR_INVALID_ARG, "grid_divisor must be 1, 2 or 4");
@@ -1631,14 +1636,21 @@ extern "C" fxc_status fxc_internal_sector_noalloc(fxc_plan *p, const uint8_t *d_
     }
     const int kReduceChunks = 64;
     const size_t slot = acc_bytes;
-    const int64_t slots = (int64_t)((scratch_bytes - table_bytes) / slot) - kReduceChunks;
+    // per workgroup: its partial slot and, for the fused N = 1024 / 2048
+    // form, its consumers' FP64 totals
+    const size_t per_wg = slot + (p->fft_form == 1 ? kMfConsumerAccBytes : 0);
+    const size_t head = table_bytes + (size_t)kReduceChunks * slot + 256;   // + alignment of the totals
+    const int64_t slots = scratch_bytes > head ? (int64_t)((scratch_bytes - head) / per_wg) : 0;
     if (slots < 1)
         return fail(FXC_ERR_UNSUPPORTED, "fft_len %d: %zu-byte partial slots do not fit the %zu-byte scratch",
                     p->N, slot, scratch_bytes);
     double *red = (double *)(scratch + table_bytes);
     double *part = red + (size_t)kReduceChunks * kNumProducts * p->half;
+    const uintptr_t cacc_at = ((uintptr_t)(part + (size_t)slots * kNumProducts * p->half) + 255) & ~(uintptr_t)255;
+    double *cacc = (double *)cacc_at;
     SectorArgs a;
     fill_args(p, a);
+    a.cons_acc = p->fft_form == 1 ? cacc : nullptr;
     fill_delay(*delay, dtab, a.dly);
     a.raw1 = d_raw1;
     a.raw2 = d_raw2;

@@ -86,7 +86,12 @@ struct SectorArgs {
     int32_t chain_ld;
     int32_t accumulate;      // fused plain kernel: add to the partial slots (sector in slabs)
     int32_t *err;            // plan's device error word (bit 0: a bounded hand-over wait ran out)
+    // N = 1024 producer/consumer kernel: per-consumer FP64 totals,
+    // [n_wg][kMfConsumers][16][64] double2 (kMfConsumerAccBytes per workgroup)
+    double *cons_acc;
 };
+constexpr int kMfConsumers = 4;
+constexpr size_t kMfConsumerAccBytes = (size_t)kMfConsumers * 32 * 64 * sizeof(double);
 
 // Phase chains replayed by k_phase_chain (fxc_chain.hip).
 enum { CHAIN_XCF = 0, CHAIN_PHASED = 1 };

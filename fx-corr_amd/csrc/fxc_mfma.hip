@@ -804,6 +804,109 @@ __device__ __forceinline__ void mf_mac(const float2 *z1, const float2 *z2, const
     }
 }
 
+// The same products accumulated in FP32 over a batch of frames, each
+// product folded into its accumulator by fused multiply-adds (|x|^2 =
+// fma(x.x, x.x, fma(x.y, x.y, acc)); (u P).re / .im likewise).  The batch
+// sums go to FP64 totals once per batch (mf_flush).
+template <bool FULL>
+__device__ __forceinline__ void mf_mac32(const float2 *z1, const float2 *z2, const float *twu,
+                                         const MfFrame &pf, int jf, int l, bool do_acf, bool do_xcf,
+                                         uint32_t m11, uint32_t m22, uint32_t m_ov, float k11,
+                                         float k22, float kM, float (&acc)[8][4]) {
+    constexpr int M = mf::M;
+    const double phw = readlane_d(pf.phw, jf);
+    const double dth = readlane_d(pf.dth, jf);
+    const float2 F1 = make_float2(readlane_f(pf.F1.x, jf), readlane_f(pf.F1.y, jf));
+    const float2 Cm = make_float2(readlane_f(pf.C.x, jf), readlane_f(pf.C.y, jf));
+    const float r0 = (float)wrap_turns(fma((double)l, dth, phw));
+    float2 P = make_float2(__builtin_amdgcn_cosf(r0), __builtin_amdgcn_sinf(r0));
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int p = l + 64 * j;
+        const int pm = (M - p) & (M - 1);
+        const float2 w = make_float2(twu[(2 * j) * 64 + l], twu[(2 * j + 1) * 64 + l]);
+        float2 x1p, x1m, x2p, x2m;
+        untangle_pair(z1[p], z1[pm], w, x1p, x1m);
+        untangle_pair(z2[p], z2[pm], w, x2p, x2m);
+        const int sp = 2 * j, sm = 2 * j + 1;
+        if (do_acf) {
+            float2 y1m = x1m, y2m = x2m;
+            bool a1p = true, a2p = true, a1m = true, a2m = true;
+            if (FULL) {
+                if (j == 0) {   // bin M (lane 0) on the grid of the other station
+                    y1m = make_float2(k11 * x1m.x, k11 * x1m.y);
+                    y2m = make_float2(k22 * x2m.x, k22 * x2m.y);
+                }
+            } else {
+                a1p = (m11 >> sp) & 1u;
+                a2p = (m22 >> sp) & 1u;
+                a1m = (m11 >> sm) & 1u;
+                a2m = (m22 >> sm) & 1u;
+            }
+            const float n1p = fmaf(x1p.x, x1p.x, fmaf(x1p.y, x1p.y, acc[sp][0]));
+            const float n2p = fmaf(x2p.x, x2p.x, fmaf(x2p.y, x2p.y, acc[sp][1]));
+            const float n1m = fmaf(y1m.x, x1m.x, fmaf(y1m.y, x1m.y, acc[sm][0]));
+            const float n2m = fmaf(y2m.x, x2m.x, fmaf(y2m.y, x2m.y, acc[sm][1]));
+            acc[sp][0] = a1p ? n1p : acc[sp][0];
+            acc[sp][1] = a2p ? n2p : acc[sp][1];
+            acc[sm][0] = a1m ? n1m : acc[sm][0];
+            acc[sm][1] = a2m ? n2m : acc[sm][1];
+        }
+        if (do_xcf) {
+            float2 Pp, Pm;
+            if (FULL) {
+                Pp = P;
+                Pm = c_mul(Cm, c_conj(P));
+                if (j == 0) Pm = make_float2(kM * Pm.x, kM * Pm.y);
+            } else {
+                Pp = ((m_ov >> sp) & 1u) ? P : make_float2(0.f, 0.f);
+                Pm = ((m_ov >> sm) & 1u) ? c_mul(Cm, c_conj(P)) : make_float2(0.f, 0.f);
+            }
+            const float2 up = c_mul(x1p, c_conj(x2p));
+            const float2 um = c_mul(x1m, c_conj(x2m));
+            acc[sp][2] = fmaf(up.x, Pp.x, fmaf(-up.y, Pp.y, acc[sp][2]));
+            acc[sp][3] = fmaf(up.x, Pp.y, fmaf(up.y, Pp.x, acc[sp][3]));
+            acc[sm][2] = fmaf(um.x, Pm.x, fmaf(-um.y, Pm.y, acc[sm][2]));
+            acc[sm][3] = fmaf(um.x, Pm.y, fmaf(um.y, Pm.x, acc[sm][3]));
+        }
+        P = c_mul(P, F1);
+    }
+}
+
+// This consumer lane's FP64 totals: 32 values as [16 pairs][64 lanes]
+// double2 in global scratch (each wave access is one coalesced 1 KB line
+// set; the 16 KB per consumer stays in L2).  Adds the batch's FP32 sums in
+// a fixed order (batch after batch) and clears them; `first` writes
+// instead of adding.
+__device__ __forceinline__ void mf_flush(double *g, float (&acc)[8][4], bool first, int l) {
+    double2 *g2 = reinterpret_cast<double2 *>(g) + l;
+#pragma unroll
+    for (int v0 = 0; v0 < 16; v0 += 4) {   // four 16-byte values in flight at a time
+        double2 t[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) t[u] = first ? make_double2(0.0, 0.0) : g2[(v0 + u) * 64];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int v = v0 + u, s = v >> 1, c = 2 * (v & 1);
+            t[u].x += (double)acc[s][c];
+            t[u].y += (double)acc[s][c + 1];
+            g2[v * 64] = t[u];
+            acc[s][c] = 0.0f;
+            acc[s][c + 1] = 0.0f;
+        }
+    }
+}
+
+__device__ __forceinline__ void mf_load_totals(const double *g, bool none, double (&acc)[8][4], int l) {
+    const double2 *g2 = reinterpret_cast<const double2 *>(g);
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+        const double2 t = none ? make_double2(0.0, 0.0) : g2[v * 64 + l];
+        acc[v >> 1][2 * (v & 1)] = t.x;
+        acc[v >> 1][2 * (v & 1) + 1] = t.y;
+    }
+}
+
 // Bin k = M/2 pairs with itself: X = 2 conj(Z'[M/2]).  Its products are
 // formed once per batch of frames, lane l taking the frame whose
 // parameters it holds (u1 / u2: that frame's Z'[M/2]), and added to this
@@ -828,7 +931,10 @@ __device__ __forceinline__ void mf_mid(const MfFrame &pf, float2 u1, float2 u2, 
 
 // Consumer c: frames [fa0, fa1) of producer c and [fb0, fb1) of producer
 // c + 4, alternating (frame i of c, frame i of c + 4, frame i + 1 of c, ...).
-template <bool FULL>
+// ACC32: each 64-iteration batch (128 frames) is summed in FP32 registers
+// (mf_mac32) and added to FP64 totals in global scratch at the batch's end;
+// otherwise every frame's products are added to FP64 registers (mf_mac).
+template <bool FULL, bool ACC32>
 __device__ void mf3_consumer(const SectorArgs &a, MfShared3 &sh, int c, int l, int64_t fa0,
                              int64_t fa1, int64_t fb0, int64_t fb1, double (&acc)[8][4],
                              double (&macc)[4]) {
@@ -853,6 +959,13 @@ __device__ void mf3_consumer(const SectorArgs &a, MfShared3 &sh, int c, int l, i
     const int64_t n = na > nb ? na : nb;
     const int q = c + mf::CONS;
     bool ok = true;
+    double *gacc = ACC32 ? a.cons_acc + ((size_t)blockIdx.x * mf::CONS + c) * 32 * 64 : nullptr;
+    float acc32[8][4];
+#pragma unroll
+    for (int s = 0; s < 8; ++s)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) acc32[s][k] = 0.0f;
+    bool first = true;
     for (int64_t g = 0; g < n; g += mf::FB) {
         const int cnt = (int)((n - g) < mf::FB ? (n - g) : mf::FB);
         MfFrame pa = {}, pb = {};
@@ -863,8 +976,12 @@ __device__ void mf3_consumer(const SectorArgs &a, MfShared3 &sh, int c, int l, i
             const int32_t i = (int32_t)(g + jf);
             if (i < na) {
                 mf_wait_ge(&sh.prod[c], i + 1, ok, &sh.broken);
-                mf_mac<FULL>(sh.z[c][0], sh.z[c][1], sh.twu, pa, jf, l, do_acf, do_xcf, m11, m22,
-                             m_ov, k11, k22, kM, acc);
+                if constexpr (ACC32)
+                    mf_mac32<FULL>(sh.z[c][0], sh.z[c][1], sh.twu, pa, jf, l, do_acf, do_xcf, m11,
+                                   m22, m_ov, k11, k22, kM, acc32);
+                else
+                    mf_mac<FULL>(sh.z[c][0], sh.z[c][1], sh.twu, pa, jf, l, do_acf, do_xcf, m11, m22,
+                                 m_ov, k11, k22, kM, acc);
                 const float2 u1 = sh.z[c][0][mf::M / 2], u2 = sh.z[c][1][mf::M / 2];
                 if (l == jf) {
                     ua1 = u1;
@@ -874,8 +991,12 @@ __device__ void mf3_consumer(const SectorArgs &a, MfShared3 &sh, int c, int l, i
             }
             if (i < nb) {
                 mf_wait_ge(&sh.prod[q], i + 1, ok, &sh.broken);
-                mf_mac<FULL>(sh.z[q][0], sh.z[q][1], sh.twu, pb, jf, l, do_acf, do_xcf, m11, m22,
-                             m_ov, k11, k22, kM, acc);
+                if constexpr (ACC32)
+                    mf_mac32<FULL>(sh.z[q][0], sh.z[q][1], sh.twu, pb, jf, l, do_acf, do_xcf, m11,
+                                   m22, m_ov, k11, k22, kM, acc32);
+                else
+                    mf_mac<FULL>(sh.z[q][0], sh.z[q][1], sh.twu, pb, jf, l, do_acf, do_xcf, m11, m22,
+                                 m_ov, k11, k22, kM, acc);
                 const float2 u1 = sh.z[q][0][mf::M / 2], u2 = sh.z[q][1][mf::M / 2];
                 if (l == jf) {
                     ub1 = u1;
@@ -886,9 +1007,13 @@ __device__ void mf3_consumer(const SectorArgs &a, MfShared3 &sh, int c, int l, i
         }
         if (l < cnt && g + l < na) mf_mid(pa, ua1, ua2, do_acf, do_xcf, in11, in22, in_ov, macc);
         if (l < cnt && g + l < nb) mf_mid(pb, ub1, ub2, do_acf, do_xcf, in11, in22, in_ov, macc);
+        if constexpr (ACC32) mf_flush(gacc, acc32, first, l);
+        first = false;
     }
+    if constexpr (ACC32) mf_load_totals(gacc, first, acc, l);
 }
 
+template <bool ACC32>
 __global__ __launch_bounds__(mf::THREADS3) void k_sector_mf3(SectorArgs a) {
     __shared__ MfShared3 sh;
     constexpr int M = mf::M;
@@ -934,11 +1059,15 @@ __global__ __launch_bounds__(mf::THREADS3) void k_sector_mf3(SectorArgs a) {
         r1 = (r0 + per < wg1) ? r0 + per : wg1;
     };
 
+    // consumers: FP64 totals of their bins (ACC32: loaded from the batch
+    // totals when the consumer finishes)
     double acc[8][4], macc[4] = {0.0, 0.0, 0.0, 0.0};
+    if constexpr (!ACC32) {
 #pragma unroll
-    for (int s = 0; s < 8; ++s)
+        for (int s = 0; s < 8; ++s)
 #pragma unroll
-        for (int c = 0; c < 4; ++c) acc[s][c] = 0.0;
+            for (int c = 0; c < 4; ++c) acc[s][c] = 0.0;
+    }
     if (wv < mf::PROD) {
         int64_t f0, f1;
         range(wv, f0, f1);
@@ -949,8 +1078,8 @@ __global__ __launch_bounds__(mf::THREADS3) void k_sector_mf3(SectorArgs a) {
         range(c, fa0, fa1);
         range(c + mf::CONS, fb0, fb1);
         const bool full = (a.flags & FLAG_ACF_OVERLAP) == 0 && a.a1s == 0 && a.ov == M;
-        if (full) mf3_consumer<true>(a, sh, c, l, fa0, fa1, fb0, fb1, acc, macc);
-        else mf3_consumer<false>(a, sh, c, l, fa0, fa1, fb0, fb1, acc, macc);
+        if (full) mf3_consumer<true, ACC32>(a, sh, c, l, fa0, fa1, fb0, fb1, acc, macc);
+        else mf3_consumer<false, ACC32>(a, sh, c, l, fa0, fa1, fb0, fb1, acc, macc);
     }
 
     // ---- workgroup partial: consumers add their bins in a fixed order
@@ -1398,9 +1527,13 @@ hipError_t launch_sector_mfma(const SectorArgs &a, int precision, int n_wg, hipS
         hipLaunchKernelGGL(k_sector_mf4, dim3(n_wg), dim3(mf4::THREADS), 0, s, a);
         return hipGetLastError();
     }
-    // variant 1: producer/consumer waves (autotuned against the default)
-    if (a.variant == 1) hipLaunchKernelGGL(k_sector_mf3, dim3(n_wg), dim3(mf::THREADS3), 0, s, a);
-    else hipLaunchKernelGGL(k_sector_mf2, dim3(n_wg), dim3(512), 0, s, a);
+    // variant 1: producer/consumer waves; 2: the same with FP32 batch sums
+    // in the consumers (needs a.cons_acc); 0: whole-frame waves
+    if (a.variant == 1) hipLaunchKernelGGL(k_sector_mf3<false>, dim3(n_wg), dim3(mf::THREADS3), 0, s, a);
+    else if (a.variant == 2) {
+        if (a.cons_acc == nullptr) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(k_sector_mf3<true>, dim3(n_wg), dim3(mf::THREADS3), 0, s, a);
+    } else hipLaunchKernelGGL(k_sector_mf2, dim3(n_wg), dim3(512), 0, s, a);
     return hipGetLastError();
 }
 

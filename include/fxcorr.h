@@ -416,14 +416,17 @@ fxc_status fxc_plan_device_bytes(const fxc_plan *plan, uint64_t *bytes);
  * passes, N = 2048..8192) and workgroups per launch (frames per workgroup)
  * for the Stockham form; the four-step split M1 x M2 and the chunk length
  * for N >= 32768; the kernel organization for the MFMA form at N = 1024
- * (every wave carrying whole frames, or producer/consumer waves; N = 2048
- * has the producer/consumer organization only).  Results are the
- * same sums up to FP64 summation order. */
+ * (every wave carrying whole frames, producer/consumer waves, or
+ * producer/consumer waves with FP32 batch sums; N = 2048 has the
+ * producer/consumer organization only).  Results are the same sums up to
+ * FP64 summation order, or FP32 rounding of the batch sums (about 1e-7 of
+ * the spectrum's scale) for the FP32-batch organization. */
 typedef struct {
     int32_t candidates;
     int32_t variant;           /* Stockham: 0 default passes, 1 radix-16 passes;
-                                  MFMA: 1 producer/consumer (default), 0 whole-frame waves
-                                  (N = 1024 only) */
+                                  MFMA: 1 producer/consumer (default), 2 producer/consumer
+                                  with FP32 sums over batches of 128 frames added to FP64
+                                  totals, 0 whole-frame waves (2 and 0: N = 1024 only) */
     int32_t grid_divisor;      /* workgroups = default / grid_divisor */
     int32_t split_m1;          /* N >= 32768: column DFT length M1 (else 0) */
     int64_t chunk_frames;      /* N >= 32768: frames per chunk (else 0) */

@@ -420,7 +420,7 @@ def test_mfma_rfft_vs_numpy(variant):
     assert np.all(got[:, 0].imag == 0) and np.all(got[:, -1].imag == 0)
 
 
-@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("variant", [0, 1, 2])
 @pytest.mark.parametrize("bits", [(4, 1), (1, 8), (2, 4)])
 def test_mfma_sector_mixed_bits(bits, variant):
     n, fs = 1024, 512e6
@@ -435,7 +435,7 @@ def test_mfma_sector_mixed_bits(bits, variant):
     _compare(gpu_run(c, plan=p), oracle_run(c, R.MODE_F64), CHOSEN_TOL)
 
 
-@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("variant", [0, 1, 2])
 @pytest.mark.parametrize("grid", [0, 1])
 @pytest.mark.parametrize("overlap_only", [False, True])
 def test_mfma_partial_band_grids(grid, overlap_only, variant):
@@ -459,7 +459,7 @@ def test_mfma_partial_band_grids(grid, overlap_only, variant):
         assert zero_pattern_equal(got[k], ref[k]), k
 
 
-@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("variant", [0, 1, 2])
 def test_mfma_large_and_odd_shifts(variant):
     """Analytic delay model sweeping the integer shift from about -1100 to
     +200 samples: odd shifts, shifts past the staging pad and whole frames
@@ -478,8 +478,9 @@ def test_mfma_large_and_odd_shifts(variant):
 
 @pytest.mark.parametrize("nf", [1, 3, 5, 37, 2051])
 def test_mfma_kernel_organizations_agree(nf):
-    """Whole-frame waves (FP32 MFMA stage 2) and producer/consumer waves
-    (VALU radix-4 x 4 stage 2) agree to FP32 rounding for frame counts that
+    """Whole-frame waves (FP32 MFMA stage 2), producer/consumer waves (VALU
+    radix-4 x 4 stage 2) and the same with FP32 batch sums in the consumers
+    agree to FP32 rounding for frame counts that
     leave producers (or whole workgroups) without frames, and both match
     the FP64 oracle.  With a single frame a bin of tiny power turns FP32
     rounding into a per-bin relative difference near 1e-5, hence the
@@ -490,13 +491,14 @@ def test_mfma_kernel_organizations_agree(nf):
                   rho=0.4, delay_samples=1.3, seed=40 + nf, delay_user_samples=1.3,
                   extra_delay_rate_sps=1e-9, lo1_hz=6.7e9, lo2_hz=6.7e9)
     res = []
-    for v in (0, 1):
+    for v in (0, 1, 2):
         p = gpu_plan(c, F.PRECISION_CHOSEN)
         assert p.info.fft_form == 1
         F.plan_set_tuning(p, variant=v)
         res.append(gpu_run(c, plan=p))
-    e = spectra_err(res[1], res[0])
-    assert max(e.values()) < CHOSEN_TOL, e
+    for r in res[1:]:
+        e = spectra_err(r, res[0])
+        assert max(e.values()) < CHOSEN_TOL, e
     o64 = oracle_run(c, R.MODE_F64)
     for r in res:
         _compare(r, o64, CHOSEN_TOL)

@@ -3,8 +3,8 @@
 A job's spectra must not depend on what else is admitted with it: the same
 job run alone and in a mix (other FFT sizes, forms, bit depths, host and
 device input, interleaved chunk arrival) gives bit-identical FP64 sums; and
-the chunked sum equals a single sector call over the same frames to FP64
-summation-order rounding."""
+the chunked sum equals a single sector call over the same frames to the
+rounding of the kernel's partial sums."""
 import numpy as np
 import pytest
 
@@ -103,7 +103,11 @@ def test_job_alone_equals_job_in_mix_and_single_call():
     one = acc.download(np.zeros(F.N_PRODUCTS * half)).reshape(F.N_PRODUCTS, half)
     got = alone.reshape(F.N_PRODUCTS, half)
     scale = np.abs(one).max(axis=1, keepdims=True) + 1e-300
-    assert np.all(np.abs(got - one) <= 1e-11 * scale)
+    # The fused N = 1024 chosen kernel sums each consumer's frames in FP32
+    # batches before its FP64 totals; chunking the job regroups frames into
+    # other batches, which moves the FP32 rounding (about 1e-7 of the
+    # spectrum's scale), far inside the chosen form's 1e-4 bar.
+    assert np.all(np.abs(got - one) <= 1e-6 * scale)
     # job B (host input) vs a direct call
     pB = F.Plan(4096, **{k: v for k, v in kwB.items() if k != "fft_len"})
     directB = pB.sector_run(hB1, hB2, 0, 3000, cfgB)
