@@ -614,6 +614,21 @@ extern "C" fxc_status fxc_geometric_delay_table(const double ant1_ecef[3], const
     return FXC_OK;
 }
 
+extern "C" fxc_status fxc_geometric_model_bound(const double ant1_ecef[3], const double ant2_ecef[3],
+                                                double obs_hz, double *delay_s, double *phase_rad) {
+    if (!ant1_ecef || !ant2_ecef || !std::isfinite(obs_hz) || obs_hz < 0.0)
+        return bad("fxc_geometric_model_bound: bad argument");
+    // IAU 2000B vs 2000A nutation: at most 1 mas (published accuracy of the
+    // 2000B series, 1995-2050) in the equation of the equinoxes
+    const double dtheta = 1e-3 / 3600.0 * (M_PI / 180.0);
+    const double bx = ant2_ecef[0] - ant1_ecef[0], by = ant2_ecef[1] - ant1_ecef[1],
+                 bz = ant2_ecef[2] - ant1_ecef[2];
+    const double d = std::sqrt(bx * bx + by * by + bz * bz) * dtheta / 299792458.0;
+    if (delay_s) *delay_s = d;
+    if (phase_rad) *phase_rad = 2.0 * M_PI * obs_hz * d;
+    return FXC_OK;
+}
+
 extern "C" int64_t fxc_geometric_table_points(int64_t total_frames, double frame_dt) {
     const double dur = (double)total_frames * frame_dt;   // main.rs:1630-1634
     const double secs = std::fmax(std::ceil(dur), 1.0);

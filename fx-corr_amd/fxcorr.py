@@ -980,7 +980,7 @@ class Pipeline:
 # ------------------------------------------------------------ station geometry
 EXPORTED += ["fxc_parse_ra", "fxc_parse_dec", "fxc_parse_epoch_mjd", "fxc_precess_j2000_to_date",
              "fxc_sidereal_time", "fxc_geometric_delay", "fxc_geometric_delay_table",
-             "fxc_geometric_table_points"]
+             "fxc_geometric_table_points", "fxc_geometric_model_bound"]
 
 
 class GeometricDelay(C.Structure):
@@ -1004,6 +1004,8 @@ def _geom_sigs(L):
         L.fxc_geometric_delay_table.restype = i32
         L.fxc_geometric_table_points.argtypes = [i64, d]
         L.fxc_geometric_table_points.restype = i64
+        L.fxc_geometric_model_bound.argtypes = [pd, pd, d, pd, pd]
+        L.fxc_geometric_model_bound.restype = i32
         L._geom_sigs = True
     return L
 
@@ -1120,3 +1122,12 @@ def phased_plot_series(acc_ph, acc11, acc22, fft_len: int, emitted_frames: int, 
     res = {k: v[:b] for k, v in outs.items()}
     res["acf_shifted"] = acf
     return res
+
+
+def geometric_model_bound(ecef1, ecef2, obs_hz: float):
+    """(delay_s, phase_rad): how far the 2000B-based geometry may move the
+    fringe phase from the reference's 2000A-based one on this baseline."""
+    dl, ph = C.c_double(), C.c_double()
+    check(_geom_sigs(lib()).fxc_geometric_model_bound(_ecef(ecef1), _ecef(ecef2), obs_hz,
+                                                      C.byref(dl), C.byref(ph)))
+    return dl.value, ph.value

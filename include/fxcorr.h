@@ -641,6 +641,17 @@ fxc_status fxc_geometric_delay_table(const double ant1_ecef[3], const double ant
 /* Table length for a process of total_frames frames of frame_dt seconds:
  * max(ceil(duration), 1) + 1 (main.rs:1630-1634). */
 int64_t fxc_geometric_table_points(int64_t total_frames, double frame_dt);
+/* How far this library's geometric delays may move the fringe phase from
+ * the reference's (no reference counterpart: a check for callers on long
+ * baselines).  Sidereal time here uses the IAU 2000B nutation series where
+ * the reference's eraGst06a uses 2000A; 2000B is within 1 mas of 2000A over
+ * 1995-2050 (its published accuracy), so the hour angle differs by at most
+ * 4.85e-9 rad and the delay by |ant2 - ant1| * 4.85e-9 / c.  Returns that
+ * delay bound (s) and the fringe-phase bound 2 pi obs_hz delay (rad); the
+ * plain form's 1e-5 relative .cor bar holds end to end for baselines whose
+ * phase bound is below 1e-5 (about 15 m at 6.7 GHz; 7.4e-5 rad at 108 m). */
+fxc_status fxc_geometric_model_bound(const double ant1_ecef[3], const double ant2_ecef[3],
+                                     double obs_hz, double *delay_s, double *phase_rad);
 
 #ifdef __cplusplus
 }

@@ -209,3 +209,22 @@ def test_parsers_match_oracle(kind):
             continue
         got = lib_fn(s)
         assert got == want or (math.isnan(got) and math.isnan(want)), s
+
+
+def test_geometric_model_bound():
+    """The fringe-phase bound of the 2000B-for-2000A substitution: |B| x
+    4.85e-9 rad / c of delay (1 mas of hour angle), at the reference's
+    default stations (geom.rs:7-8, 108 m) 7.4e-5 rad at 6.7 GHz; it covers
+    the offset measured at 2006.0 against eraGst06a (2.0e-9 rad)."""
+    a = [-3502544.587, 3950966.235, 3566381.192]
+    b = [-3502567.576, 3950885.734, 3566449.115]
+    d, ph = F.geometric_model_bound(a, b, 6.7e9)
+    blen = math.dist(a, b)
+    assert 107.0 < blen < 109.0
+    assert d == pytest.approx(blen * math.radians(1e-3 / 3600) / 299792458.0, rel=1e-12)
+    assert ph == pytest.approx(2 * math.pi * 6.7e9 * d, rel=1e-12)
+    assert 7.0e-5 < ph < 7.8e-5
+    assert d >= blen * 2.0e-9 / 299792458.0
+    assert F.geometric_model_bound(a, a, 6.7e9) == (0.0, 0.0)
+    with pytest.raises(F.FxcError):
+        F.geometric_model_bound(a, b, float("nan"))
