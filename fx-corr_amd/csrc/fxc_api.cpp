@@ -720,6 +720,8 @@ static std::vector<uint8_t> mfma_tables(const fxc_plan *p) {
     return tab;
 }
 
+static bool large_fused_possible(const fxc_plan *p);
+
 extern "C" fxc_status fxc_plan_create(const fxc_plan_desc *desc, fxc_plan **out) {
     if (!desc || !out) return fail(FXC_ERR_INVALID_ARG, "null argument");
     *out = nullptr;
@@ -861,6 +863,11 @@ extern "C" fxc_status fxc_plan_create(const fxc_plan_desc *desc, fxc_plan **out)
     // threads each); MFMA form one 8-wave workgroup per CU; large-N form one
     // partial per frame group.  Each workgroup runs a contiguous range of
     // frames.
+    if (large_fused_possible(p)) {   // the fused four-step is the large form's default
+        p->variant = 1;
+        p->M1 = 128;
+        p->M2 = (int)M / 128;
+    }
     p->max_wg = p->fft_form == 1 ? cus
                 : p->fft_form == 2 ? kLargeGroups
                                    : std::max(64, (cus * 1024) / p->T);
@@ -1302,10 +1309,48 @@ static fxc_status enqueue_phased_large(fxc_plan *p, const SectorArgs &base, int6
 // N = 32768 / 65536: chunks of frames through HBM scratch; the product
 // kernel's kLargeGroups frame-group partial slots accumulate over the chunks
 // in order, then one fixed-order reduce.
+// The chosen form's fused four-step (decode in the first pass; the second
+// pass, untangle and products in one kernel) applies when both stations'
+// bins of a product lie on the same rows (band shift a multiple of M1) and
+// each station's frame fits the first pass's LDS word stage.
+// The fused four-step runs at M1 = 128 (variant 1 of the large form).
+static bool large_fused_possible(const fxc_plan *p) {
+    const int64_t sb = p->desc.a2s - p->desc.a1s;
+    return p->fft_form == 2 && p->desc.precision == FXC_PRECISION_CHOSEN && p->mf_b1 &&
+           (p->M / 128 == 128 || p->M / 128 == 256) && (sb % 128) == 0 &&
+           (size_t)p->st[0].nwords * 4 <= kLargeFusedMaxWordBytes &&
+           (size_t)p->st[1].nwords * 4 <= kLargeFusedMaxWordBytes;
+}
+
+static bool large_fused_ok(const fxc_plan *p) {
+    return p->variant == 1 && p->M1 == 128 && large_fused_possible(p);
+}
+
 static fxc_status enqueue_large(fxc_plan *p, const SectorArgs &base, int64_t nf, double *d_acc,
                                 hipStream_t s) {
     const float2 *twl1 = (const float2 *)(p->M1 == 128 ? p->tw_l1.p : p->tw_l2.p);
     const float2 *twl2 = (const float2 *)(p->M2 == 128 ? p->tw_l1.p : p->tw_l2.p);
+    if (large_fused_ok(p)) {
+        for (int64_t c0 = 0; c0 < nf; c0 += p->chunk_use) {
+            const int64_t cn = std::min(p->chunk_use, nf - c0);
+            SectorArgs a = base;
+            a.raw1 = base.raw1 + c0 * p->st[0].bpf;
+            a.raw2 = base.raw2 + c0 * p->st[1].bpf;
+            a.first_frame = base.first_frame + c0;
+            a.nf = cn;
+            a.part = (double *)p->part.p;
+            FrameParam *params = (FrameParam *)p->d_params.p;
+            int64_t *shifts = (int64_t *)p->d_lshifts.p;
+            HIP_TRY(launch_frame_params(a, p->desc.precision, params, shifts, s));
+            HIP_TRY(launch_large_pass1_fused(a, shifts, (float2 *)p->d_z[0].p, (float2 *)p->d_z[1].p,
+                                             p->M1, p->M2, (const float2 *)p->tw_lm.p, s));
+            HIP_TRY(launch_large_pass2_mac(a, (const float2 *)p->d_z[0].p, (const float2 *)p->d_z[1].p,
+                                           params, p->M1, p->M2, twl2, kLargeGroups, c0 > 0 ? 1 : 0, s));
+        }
+        HIP_TRY(launch_reduce((const double *)p->part.p, kLargeGroups, p->half, d_acc,
+                              (double *)p->red_scratch.p, 64, s));
+        return FXC_OK;
+    }
     const bool chain = xcf_uses_chain(p);
     const int ld = chain ? chain_ld_for(base.ov) : 0;
     int64_t span = 0;      // frames whose chains one launch computes: whole chunks
@@ -1455,7 +1500,13 @@ extern "C" fxc_status fxc_plan_autotune(fxc_plan *p, const uint8_t *d_raw1, cons
     };
     std::vector<Cand> cands;
     if (p->fft_form == 2) {
+        // variant 1: the fused four-step (M1 = 128), the default where it
+        // applies and so timed first; variant 0: the four-step through HBM
+        // at either split
         const int M = p->M;
+        if (large_fused_possible(p))
+            for (int64_t c : {p->chunk, p->chunk / 2, p->chunk / 4})
+                if (c >= 16) cands.push_back({1, 1, 128, c});
         std::vector<int> m1s = {M / 128};
         if (M / 128 != 128) m1s.push_back(128);
         for (int m1 : m1s)
@@ -1548,7 +1599,8 @@ extern "C" fxc_status fxc_plan_autotune(fxc_plan *p, const uint8_t *d_raw1, cons
 
 extern "C" fxc_status fxc_plan_set_tuning(fxc_plan *p, const fxc_tune_report *c) {
     if (!p || !c) return fail(FXC_ERR_INVALID_ARG, "null argument");
-    const bool v1_ok = (p->fft_form == 0 && sector_has_variant(p->N)) || p->fft_form == 1;
+    const bool v1_ok = (p->fft_form == 0 && sector_has_variant(p->N)) || p->fft_form == 1 ||
+                       (large_fused_possible(p) && c->split_m1 == 128);
     const bool v0_ok = !(p->fft_form == 1 && p->N != 1024);
     const bool v2_ok = p->fft_form == 1 && p->N == 1024;
     if ((c->variant == 1 && !v1_ok) || (c->variant == 0 && !v0_ok) || (c->variant == 2 && !v2_ok) ||

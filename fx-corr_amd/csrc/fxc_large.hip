@@ -262,6 +262,223 @@ __global__ __launch_bounds__(256) void k_mac_large(SectorArgs a, const float2 *z
     out[P_12I * half + g] = prev[3] + acc[3];
 }
 
+// Wave-synchronous Stockham FFT of L = 64 E points in LDS (one wave, no
+// workgroup barrier): v[r] = z[lane + 64 r] on entry, natural order in z on
+// return.  Radix E passes (E = 2 or 4), then a radix-2 pass when needed.
+__device__ __forceinline__ void wave_sync_lds() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int L, int R, int NS>
+__device__ __forceinline__ void wave_stockham_pass(float2 *z, const float2 *tw, int lane) {
+    if constexpr (NS < L) {
+        constexpr int RR = (L / NS < R) ? (L / NS) : R;
+        constexpr int BPT = L / (64 * RR);   // butterflies per lane
+        float2 a[BPT][RR];
+#pragma unroll
+        for (int u = 0; u < BPT; ++u) {
+            const int j = lane + 64 * u;
+#pragma unroll
+            for (int r = 0; r < RR; ++r) a[u][r] = z[pad(j + r * (L / RR))];
+            if constexpr (NS > 1) {
+                const int jm = j % NS;
+#pragma unroll
+                for (int r = 1; r < RR; ++r) a[u][r] = c_mul(a[u][r], tw[r * jm * (L / (NS * RR))]);
+            }
+            dftR<RR>(a[u]);
+        }
+        wave_sync_lds();
+#pragma unroll
+        for (int u = 0; u < BPT; ++u) {
+            const int j = lane + 64 * u;
+            const int base = (j / NS) * NS * RR + (j % NS);
+#pragma unroll
+            for (int r = 0; r < RR; ++r) z[pad(base + r * NS)] = a[u][r];
+        }
+        wave_sync_lds();
+        wave_stockham_pass<L, R, NS * RR>(z, tw, lane);
+    }
+}
+
+// ------------------------------------ second pass fused with the products
+// Chosen form, both stations' bins on the same rows of the four-step (the
+// band shift a multiple of M1).  Z[k1 + M1 k2] is row k1's DFT over m2 of
+// Y[k1][m2], and bin M - k lies in row M1 - k1 (row 0 and row M1/2 are their
+// own mirrors), so the untangle of a bin needs only its row pair.
+// Workgroup (pair p, frame group): rows ra = p, rb = M1 - p (p = 0: rows 0
+// and M1/2) of both stations; the pair's post twiddles W_N^{-g} and the
+// DFT-M2 twiddles are staged in LDS once.  Per frame, wave q loads row q of
+// [grid ra, grid rb, other ra, other rb] (the next two frames' rows are
+// loaded into registers while this one is worked on) and runs its DFT-M2 in LDS
+// with wave-level syncs; after one workgroup barrier each thread takes bins
+// g = row + M1 k2 of the pair (and bin M in pair 0): the split-method
+// untangle of both stations, the ACF/XCF products with the XCF phasor from
+// the frame's FP64 phase (k_mac_large's arithmetic) and FP64 sums.  The
+// rows are double-buffered in LDS, so a frame needs only that one barrier:
+// a wave rewrites a buffer two frames later, after every thread has passed
+// the next frame's barrier and so finished this frame's bins.
+__device__ __forceinline__ float2 untangle_ab(float2 A, float2 B, int k, int M, float2 w) {
+    if (k == 0 || k == M) return make_float2(k == 0 ? A.x + A.y : A.x - A.y, 0.0f);
+    const float ex = 0.5f * (A.x + B.x), ey = 0.5f * (A.y - B.y);
+    const float dx = 0.5f * (A.x - B.x), dy = 0.5f * (A.y + B.y);
+    return make_float2(ex + (w.x * dy - w.y * dx), ey - (w.x * dx + w.y * dy));
+}
+
+template <int M2>
+__global__ __launch_bounds__(256) void k_large_pass2_mac(SectorArgs a, const float2 *y1,
+                                                         const float2 *y2, const FrameParam *params,
+                                                         int M1, const float2 *tw_m2, int64_t fpg,
+                                                         int accumulate) {
+    constexpr int E = M2 / 64;
+    constexpr int LP = M2 + M2 / 16 + 2;
+    constexpr int BPT = (2 * M2) / 256;   // bins per thread (plus bin M for thread 0 of pair 0)
+    __shared__ float2 zr[2][4][LP];       // [buffer][grid ra, grid rb, other ra, other rb]
+    __shared__ float2 twm[M2];
+    __shared__ float2 twp[2][M2];         // tw_post[row + M1 k2], rows ra / rb
+    const int M = a.M;
+    const int half = M + 1;
+    const int tid = threadIdx.x;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int lane = tid & 63;
+    const int p = blockIdx.x;
+    const int ra = p, rb = p == 0 ? M1 / 2 : M1 - p;
+    for (int i = tid; i < M2; i += 256) twm[i] = tw_m2[i];
+    for (int i = tid; i < 2 * M2; i += 256)
+        twp[i / M2][i % M2] = a.tw_post[(i < M2 ? ra : rb) + M1 * (i % M2)];
+    const int64_t g0 = (int64_t)blockIdx.y * fpg;
+    int64_t g1 = g0 + fpg;
+    if (g1 > a.nf) g1 = a.nf;
+    const bool do_acf = (a.flags & FLAG_ACF) != 0;
+    const bool do_xcf = (a.flags & FLAG_XCF) != 0;
+    const bool ov_only = (a.flags & FLAG_ACF_OVERLAP) != 0;
+    const int gs = (a.grid == 0) ? a.a1s : a.a2s;
+    const int sbo = (a.grid == 0) ? (a.a2s - a.a1s) : (a.a1s - a.a2s);
+    const int jo = sbo / M1;   // the other station's k2 offset (sbo is a multiple of M1)
+    // this wave's row: q = w
+    const float2 *src_row = ((w & 2) ? (a.grid == 0 ? y2 : y1) : (a.grid == 0 ? y1 : y2)) +
+                            (int64_t)((w & 1) ? rb : ra) * M2;
+    const bool nyq = p == 0 && tid == 0;
+    double acc[BPT + 1][4];
+#pragma unroll
+    for (int i = 0; i <= BPT; ++i)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[i][c] = 0.0;
+    // rows of frames f (cur), f + 1 and f + 2 in flight
+    float2 cur[E], nx1[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        cur[e] = g0 < g1 ? src_row[g0 * (int64_t)M + lane + 64 * e] : make_float2(0.f, 0.f);
+        nx1[e] = g0 + 1 < g1 ? src_row[(g0 + 1) * (int64_t)M + lane + 64 * e] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();   // twiddles staged
+    int b = 0;
+    for (int64_t f = g0; f < g1; ++f, b ^= 1) {
+        float2 nx2[E];
+        const bool more = f + 2 < g1;
+#pragma unroll
+        for (int e = 0; e < E; ++e)
+            nx2[e] = more ? src_row[(f + 2) * (int64_t)M + lane + 64 * e] : make_float2(0.f, 0.f);
+        float2 *z = zr[b][w];
+        dftR<E>(cur);
+#pragma unroll
+        for (int r = 0; r < E; ++r) z[pad(lane * E + r)] = cur[r];
+        wave_sync_lds();
+        wave_stockham_pass<M2, E, E>(z, twm, lane);
+        __syncthreads();   // the four rows of frame f are ready
+        const double dth = params[f].dth, phw = params[f].phw;
+        const float2(*zb)[LP] = zr[b];
+#pragma unroll
+        for (int i = 0; i <= BPT; ++i) {
+            int g, row, k2;
+            if (i < BPT) {
+                const int j = tid + 256 * i;
+                row = j < M2 ? ra : rb;
+                k2 = j % M2;
+                g = row + M1 * k2;
+            } else {
+                if (!nyq) continue;
+                row = 0;
+                k2 = M2;
+                g = M;
+            }
+            const int bg = row == ra ? 0 : 1;
+            // mirror M - g: row 0 -> row 0, k2' = M2 - k2; else row M1 - row, k2' = M2 - 1 - k2
+            const int bm = (row == 0 ? row : M1 - row) == ra ? 0 : 1;
+            const int km = row == 0 ? (M2 - k2) & (M2 - 1) : M2 - 1 - k2;
+            const float2 Zg = zb[bg][pad(k2 & (M2 - 1))], Zm = zb[bm][pad(km)];
+            const float2 Xg = untangle_ab(Zg, Zm, g, M, twp[bg][k2 & (M2 - 1)]);
+            const bool in_ov = g >= gs && g < gs + a.ov;
+            float2 Xo = make_float2(0.0f, 0.0f);
+            if (in_ov) {
+                const int h = g + sbo, kh = k2 + jo;   // same row, k2 shifted
+                const int khm = row == 0 ? (M2 - kh) & (M2 - 1) : M2 - 1 - kh;
+                const float2 Oh = zb[2 + bg][pad(kh & (M2 - 1))];
+                const float2 Om = zb[2 + bm][pad(khm & (M2 - 1))];
+                Xo = untangle_ab(Oh, Om, h, M, twp[bg][kh & (M2 - 1)]);
+            }
+            if (do_acf) {
+                const float pgrid = (!ov_only || in_ov) ? norm_sqr(Xg) : 0.0f;
+                const float poth = in_ov ? norm_sqr(Xo) : 0.0f;
+                acc[i][0] += (double)(a.grid == 0 ? pgrid : poth);
+                acc[i][1] += (double)(a.grid == 0 ? poth : pgrid);
+            }
+            if (in_ov && do_xcf) {
+                const float r = (float)wrap_turns(fma((double)(g - gs), dth, phw));
+                const float2 phf = make_float2(__builtin_amdgcn_cosf(r), __builtin_amdgcn_sinf(r));
+                const float2 X1 = (a.grid == 0) ? Xg : Xo;
+                const float2 X2 = (a.grid == 0) ? Xo : Xg;
+                const float2 v2 = c_mul(c_mul(X1, c_conj(X2)), phf);
+                acc[i][2] += (double)v2.x;
+                acc[i][3] += (double)v2.y;
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            cur[e] = nx1[e];
+            nx1[e] = nx2[e];
+        }
+    }
+    double *out = a.part + (size_t)blockIdx.y * kNumProducts * half;
+#pragma unroll
+    for (int i = 0; i <= BPT; ++i) {
+        int g;
+        if (i < BPT) {
+            const int j = tid + 256 * i;
+            g = (j < M2 ? ra : rb) + M1 * (j % M2);
+        } else {
+            if (!nyq) continue;
+            g = M;
+        }
+        const double prev[4] = {accumulate ? out[P_11 * half + g] : 0.0,
+                                accumulate ? out[P_22 * half + g] : 0.0,
+                                accumulate ? out[P_12R * half + g] : 0.0,
+                                accumulate ? out[P_12I * half + g] : 0.0};
+        out[P_PH * half + g] = 0.0;
+        out[P_11 * half + g] = prev[0] + acc[i][0];
+        out[P_22 * half + g] = prev[1] + acc[i][1];
+        out[P_12R * half + g] = prev[2] + acc[i][2];
+        out[P_12I * half + g] = prev[3] + acc[i][3];
+    }
+}
+
+hipError_t launch_large_pass2_mac(const SectorArgs &a, const float2 *y1, const float2 *y2,
+                                  const FrameParam *params, int M1, int M2, const float2 *tw_m2,
+                                  int n_groups, int accumulate, hipStream_t s) {
+    const int sbo = (a.grid == 0) ? (a.a2s - a.a1s) : (a.a1s - a.a2s);
+    if (M1 * M2 != a.M || sbo % M1 != 0 || (M2 != 128 && M2 != 256)) return hipErrorInvalidValue;
+    const int64_t fpg = (a.nf + n_groups - 1) / n_groups;
+    const dim3 grid((unsigned)(M1 / 2), (unsigned)n_groups);
+    if (M2 == 128)
+        hipLaunchKernelGGL(k_large_pass2_mac<128>, grid, dim3(256), 0, s, a, y1, y2, params, M1, tw_m2,
+                           fpg, accumulate);
+    else
+        hipLaunchKernelGGL(k_large_pass2_mac<256>, grid, dim3(256), 0, s, a, y1, y2, params, M1, tw_m2,
+                           fpg, accumulate);
+    return hipGetLastError();
+}
+
 hipError_t launch_mac_large(const SectorArgs &a, int precision, const float2 *z1, const float2 *z2,
                             const FrameParam *params, int n_groups, int accumulate,
                             hipStream_t s) {

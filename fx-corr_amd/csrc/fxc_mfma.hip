@@ -1678,9 +1678,201 @@ __global__ __launch_bounds__(64 * R) void k_fft_pass1_mfma(const IN *in, float2 
 #pragma unroll
         for (int k2 = 0; k2 < R; ++k2) {
             const int64_t k1 = kp + 32 * k2;
-            dst[m2 + k1 * M2] = c_mul(a[k2], tw_m[(m2 * k1) % M]);
+            dst[m2 + k1 * M2] = c_mul(a[k2], tw_m[(m2 * k1) & (M - 1)]);   // M: a power of two
         }
     }
+}
+
+// decode_elem with the code width fixed at compile time (1, 2, 4 or 8
+// bits; 0: the run-time width of decode_elem), branch-free: the two words
+// are read at indexes clamped into the frame (a sample inside the frame
+// never takes bits from outside it; samples outside are zeroed), and the
+// LSB flip is a sign mask per sample parity (fm0 for sample 2m - shift,
+// fm1 for the next; the parity of 2m - shift is the shift's).
+template <int BITS>
+__device__ __forceinline__ float2 decode_elem_t(const uint32_t *words, const StationDev &st,
+                                                const float *lev_lds, int m, int shift, int N,
+                                                uint32_t fm0, uint32_t fm1) {
+    if constexpr (BITS == 0) {
+        return decode_elem(words, st, lev_lds, m, shift, N);
+    } else {
+        constexpr uint32_t mask = (1u << BITS) - 1u;
+        const int src0 = 2 * m - shift;
+        const int bp = src0 * BITS;             // may be negative (arithmetic shift floors)
+        const int w0 = bp >> 5;
+        const int hiw = st.nwords - 1;
+        const uint32_t lo = words[min(max(w0, 0), hiw)];
+        const uint32_t hi = words[min(max(w0 + 1, 0), hiw)];
+        const uint32_t v = __builtin_amdgcn_alignbit(hi, lo, (uint32_t)bp & 31u);
+        float x0, x1;
+        if constexpr (BITS == 1) {
+            x0 = (v & 1u) ? st.lev4[1] : st.lev4[0];
+            x1 = (v & 2u) ? st.lev4[1] : st.lev4[0];
+        } else if constexpr (BITS == 2) {
+            x0 = level_of(v & mask, st, lev_lds);
+            x1 = level_of((v >> 2) & mask, st, lev_lds);
+        } else {
+            x0 = lev_lds[v & mask];
+            x1 = lev_lds[(v >> BITS) & mask];
+        }
+        x0 = __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, x0) ^ fm0);
+        x1 = __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, x1) ^ fm1);
+        return make_float2((unsigned)src0 < (unsigned)N ? x0 : 0.0f,
+                           (unsigned)(src0 + 1) < (unsigned)N ? x1 : 0.0f);
+    }
+}
+
+// The same first pass with the frame decode fused in (chosen form, sector
+// path, M1 = 128 = 32 x 4).  A workgroup stages one station-frame's packed
+// words in LDS once (shuffle applied) and its four waves take 16-column
+// blocks on their own, with no workgroup barrier after the stage:
+//  - for each v = 0..3 the wave decodes its A_v operand straight from the
+//    words (decode_elem: levels, integer shift with zero fill, LSB flip) and
+//    runs the DFT-32 GEMM with the operands in swapped roles (table x data),
+//    so lane (r = l % 16, g = l / 16) receives column c0 + r at
+//    k' = 4 g + i (+ 16 for the second k' tile) for every v;
+//  - the DFT-4 over v is then lane-local (no LDS exchange), and each store
+//    instruction writes 16 consecutive columns of 4 rows k1 = k' + 32 k2.
+// Y[k1][m2] W_M^{m2 k1} lands at out[m2 + M2 k1] as in k_fft_pass1_mfma; the
+// output twiddle is W_M^{m2 k'} W_M^{32 m2 k2} (two table values).
+// Grid: frames x parts, one launch per station; wave w of part p takes
+// column block 4 p + w.
+__device__ __forceinline__ f4 mf_stage1_t(half8 a0, half8 a1, half8 bh0, half8 bl0, half8 bh1,
+                                          half8 bl1) {
+    f4 d = {0.f, 0.f, 0.f, 0.f};
+    d = __builtin_amdgcn_mfma_f32_16x16x32_f16(bl0, a0, d, 0, 0, 0);
+    d = __builtin_amdgcn_mfma_f32_16x16x32_f16(bl1, a1, d, 0, 0, 0);
+    d = __builtin_amdgcn_mfma_f32_16x16x32_f16(bh0, a0, d, 0, 0, 0);
+    d = __builtin_amdgcn_mfma_f32_16x16x32_f16(bh1, a1, d, 0, 0, 0);
+    return d;
+}
+
+template <int BITS>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_large_pass1_fused(SectorArgs a, int station,
+                                                           const int64_t *shifts, float2 *out,
+                                                           int M2, int parts, const float2 *tw_m) {
+    constexpr int R = 4;
+    extern __shared__ uint32_t words[];   // the frame's st.nwords words
+    __shared__ uint4 b1[16 * 64];
+    __shared__ float2 twv[R][32];         // W_128^{v k'}
+    __shared__ float lev[256];
+    const StationDev &st = a.st[station];
+    const int M = a.M;
+    const int tid = threadIdx.x;
+    const int l = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int64_t f = blockIdx.x / parts;
+    const int part = blockIdx.x % parts;
+    if (f >= a.nf) return;   // uniform per workgroup
+    const uint4 *b1g = reinterpret_cast<const uint4 *>(a.mf_tab);
+    for (int i = tid; i < 16 * 64; i += 256) b1[i] = b1g[i];
+    if (tid < R * 32) twv[tid >> 5][tid & 31] = tw_m[(tid >> 5) * (tid & 31) * M2];
+    if (st.bits > 2 && st.bits <= 8)
+        for (int i = tid; i < 256; i += 256) lev[i] = (i <= (int)st.code_mask) ? st.levels[i] : 0.0f;
+    stage_words<256>((station == 0 ? a.raw1 : a.raw2) + f * st.bpf, st, words, tid);
+    __syncthreads();
+    const int sh = clamp_shift(shifts[f * 2 + station], a.N);
+    // LSB flip (odd samples of the unshifted frame): sample 2m - sh is odd iff sh is
+    const uint32_t fm0 = (st.lsb && (sh & 1)) ? 0x80000000u : 0u;
+    const uint32_t fm1 = (st.lsb && !(sh & 1)) ? 0x80000000u : 0u;
+    const int r = l & 15, g = l >> 4;
+    float2 *dst = out + f * (int64_t)M;
+    {
+        const int cb = part * 4 + w;
+        const int c0 = cb * 16;
+        const int m2 = c0 + r;
+        // output twiddles: W_M^{m2 k'} for this lane's 8 k', W_M^{32 m2 k2}
+        float2 wk[2][4], w32[R];
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) wk[h][i] = tw_m[(m2 * (4 * g + i + 16 * h)) & (M - 1)];
+#pragma unroll
+        for (int k2 = 1; k2 < R; ++k2) w32[k2] = tw_m[(32 * m2 * k2) & (M - 1)];
+        // the A_v operands of all four v (decoded once), then per k' half h
+        // the two tiles (re, im) of every v, their DFT-4 over v and stores
+        half8 av0[R], av1[R];
+#pragma unroll
+        for (int v = 0; v < R; ++v) {
+            uint32_t d[8];
+#pragma unroll
+            for (int kap = 0; kap < 2; ++kap)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int u = 16 * kap + g + 4 * q;
+                    const float2 x = decode_elem_t<BITS>(words, st, lev, m2 + M2 * (v + R * u), sh, a.N,
+                                                                 fm0, fm1);
+                    d[kap * 4 + q] = __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pkrtz(x.x, x.y));
+                }
+            av0[v] = __builtin_bit_cast(half8, make_uint4(d[0], d[1], d[2], d[3]));
+            av1[v] = __builtin_bit_cast(half8, make_uint4(d[4], d[5], d[6], d[7]));
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            f4 yr[R], yi[R];
+#pragma unroll
+            for (int v = 0; v < R; ++v) {
+                yr[v] = mf_stage1_t(av0[v], av1[v], ld_b1(b1, h * 2 + 0, l), ld_b1(b1, h * 2 + 1, l),
+                                    ld_b1(b1, (4 + h) * 2 + 0, l), ld_b1(b1, (4 + h) * 2 + 1, l));
+                yi[v] = mf_stage1_t(av0[v], av1[v], ld_b1(b1, (2 + h) * 2 + 0, l),
+                                    ld_b1(b1, (2 + h) * 2 + 1, l), ld_b1(b1, (6 + h) * 2 + 0, l),
+                                    ld_b1(b1, (6 + h) * 2 + 1, l));
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int kp = 4 * g + i + 16 * h;
+                float2 c[R];
+#pragma unroll
+                for (int v = 0; v < R; ++v) {
+                    c[v] = make_float2(yr[v][i], yi[v][i]);
+                    if (v > 0) c[v] = c_mul(c[v], twv[v][kp]);
+                }
+                dftR<R>(c);
+#pragma unroll
+                for (int k2 = 0; k2 < R; ++k2) {
+                    const float2 tw = k2 == 0 ? wk[h][i] : c_mul(wk[h][i], w32[k2]);
+                    dst[m2 + (kp + 32 * k2) * M2] = c_mul(c[k2], tw);
+                }
+            }
+        }
+    }
+}
+
+// Shifts: frame f's integer shifts at shifts[2 f + station]; one launch
+// per station, the kernel instance fixed by its code width.
+hipError_t launch_large_pass1_fused(const SectorArgs &a, const int64_t *shifts, float2 *out0,
+                                    float2 *out1, int M1, int M2, const float2 *tw_m,
+                                    hipStream_t s) {
+    if (a.mf_tab == nullptr || M1 != 128 || (M2 != 128 && M2 != 256) || M1 * M2 != a.M)
+        return hipErrorInvalidValue;
+    if (a.nf <= 0) return hipSuccess;
+    // 16-column blocks per frame: M2 / 16; one per wave, four waves per part
+    const int parts = M2 / 64;
+    for (int st = 0; st < 2; ++st) {
+        const size_t lds = (size_t)a.st[st].nwords * 4;
+        if (lds > kLargeFusedMaxWordBytes) return hipErrorInvalidValue;
+        auto go = [&](auto kern) -> hipError_t {
+            if (lds > 48 * 1024) {
+                hipError_t e = hipFuncSetAttribute((const void *)kern,
+                                                   hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                   (int)kLargeFusedMaxWordBytes);
+                if (e != hipSuccess) return e;
+            }
+            hipLaunchKernelGGL(kern, dim3((unsigned)(a.nf * parts)), dim3(256), lds, s, a, st, shifts,
+                               st == 0 ? out0 : out1, M2, parts, tw_m);
+            return hipGetLastError();
+        };
+        hipError_t e;
+        switch (a.st[st].bits) {
+        case 1: e = go(k_large_pass1_fused<1>); break;
+        case 2: e = go(k_large_pass1_fused<2>); break;
+        case 4: e = go(k_large_pass1_fused<4>); break;
+        case 8: e = go(k_large_pass1_fused<8>); break;
+        default: e = go(k_large_pass1_fused<0>); break;
+        }
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 template <typename IN>

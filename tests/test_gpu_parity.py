@@ -812,14 +812,61 @@ def test_tuning_candidates_keep_parity(n, variant, div):
 
 
 def test_large_split_and_chunk_candidates():
+    """Every large-form configuration the autotuner can pick: the fused
+    four-step (variant 1, M1 = 128) and the four-step through HBM (variant
+    0) at both splits, over one or several chunks."""
     n, fs = 65536, 512e6
     c = make_case(n=n, fs=fs, nf=40, bits=(4, 1), rho=0.3, seed=17, delay_user_samples=1.2,
                   lo1_hz=8e9)
     o64 = oracle_run(c, R.MODE_F64)
-    for m1, chunk in ((256, 12), (128, 40), (128, 7)):
+    p = gpu_plan(c, F.PRECISION_CHOSEN)
+    assert p.info.variant == 1                        # the fused form is the default
+    for v, m1, chunk in ((1, 128, 40), (1, 128, 7), (0, 256, 12), (0, 128, 40), (0, 128, 7)):
         p = gpu_plan(c, F.PRECISION_CHOSEN)
-        F.plan_set_tuning(p, split_m1=m1, chunk_frames=chunk)
+        F.plan_set_tuning(p, variant=v, split_m1=m1, chunk_frames=chunk)
         _compare(gpu_run(c, plan=p), o64, CHOSEN_TOL)
+    with pytest.raises(F.FxcError):                   # the fused form needs M1 = 128
+        F.plan_set_tuning(gpu_plan(c, F.PRECISION_CHOSEN), variant=1, split_m1=256,
+                          chunk_frames=7)
+
+
+@pytest.mark.parametrize("n,bits,sb,grid,prods,lsb,delay,nf,chunk", [
+    (32768, (2, 2), 0, 0, "xa", (False, False), 5.3, 40, 0),
+    (32768, (1, 8), 128, 0, "xa", (True, False), -3.6, 5, 0),        # fewer frames than groups
+    (32768, (4, 2), -256, 1, "xa", (False, True), 17.2, 40, 7),      # several chunks
+    (65536, (4, 1), 2048, 0, "ov", (True, True), 2.5, 24, 0),        # ACF on the overlap only
+    (65536, (2, 4), -128, 1, "x", (False, False), -40.7, 24, 5),     # XCF only, odd shifts
+])
+def test_large_fused_form(n, bits, sb, grid, prods, lsb, delay, nf, chunk):
+    """The fused four-step (decode in the first pass; the row DFTs, untangle
+    and products in the second) against the FP64 oracle and against the
+    four-step through HBM on the same plan: mixed code widths, band shifts
+    (multiples of M1 = 128) on both output grids, LSB flips, integer shifts
+    of either sign, fewer frames than frame groups and several chunks."""
+    fs = 512e6
+    band = (0, 0, n // 2, 0)
+    if sb:
+        band = (sb, 0, n // 2 - abs(sb), abs(sb)) if sb > 0 else (sb, abs(sb), n // 2, 0)
+    products = {"xa": F.PRODUCT_ACF | F.PRODUCT_XCF, "x": F.PRODUCT_XCF,
+                "ov": F.PRODUCT_ACF | F.PRODUCT_ACF_OVERLAP_ONLY | F.PRODUCT_XCF}[prods]
+    c = make_case(n=n, fs=fs, nf=nf, bits=bits, lsb=lsb, band=band, grid=grid,
+                  products=products, rho=0.4, delay_samples=delay, seed=n + nf + abs(sb),
+                  delay_user_samples=delay, extra_delay_rate_sps=2e-9, lo1_hz=8.4e9,
+                  lo2_hz=8.4e9 - sb * fs / n)
+    o64 = oracle_run(c, R.MODE_F64)
+    keys = ("acc11", "acc22", "acc12") if products & F.PRODUCT_ACF else ("acc12",)
+    p = gpu_plan(c, F.PRECISION_CHOSEN)
+    assert p.info.fft_form == 2 and p.info.variant == 1
+    if chunk:
+        F.plan_set_tuning(p, variant=1, split_m1=128, chunk_frames=chunk)
+    fused = gpu_run(c, plan=p)
+    _compare(fused, o64, CHOSEN_TOL, keys)
+    q = gpu_plan(c, F.PRECISION_CHOSEN)
+    F.plan_set_tuning(q, variant=0, split_m1=n // 2 // 128, chunk_frames=chunk or nf)
+    hbm = gpu_run(c, plan=q)
+    _compare(fused, hbm, CHOSEN_TOL, keys)
+    for k in keys:
+        assert zero_pattern_equal(fused[k], hbm[k]), k
 
 
 @pytest.mark.parametrize("n,prec", [(4096, F.PRECISION_PLAIN_FP32), (1024, F.PRECISION_CHOSEN),
@@ -844,4 +891,8 @@ def test_autotune_picks_a_candidate(n, prec):
         assert rep["candidates"] == 3 and rep["variant"] in (0, 1, 2)
     assert p.info.variant == rep["variant"] and p.info.grid_divisor == rep["grid_divisor"]
     if n == 65536:
+        # the fused form (variant 1) at three chunk sizes, then the four-step
+        # through HBM at two splits and three chunk sizes
+        assert rep["candidates"] == 9 and rep["variant"] in (0, 1)
         assert rep["split_m1"] in (128, 256) and rep["chunk_frames"] > 0
+        assert rep["variant"] == 0 or rep["split_m1"] == 128
