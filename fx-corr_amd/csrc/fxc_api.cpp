@@ -1312,14 +1312,13 @@ static fxc_status enqueue_phased_large(fxc_plan *p, const SectorArgs &base, int6
 // The chosen form's fused four-step (decode in the first pass; the second
 // pass, untangle and products in one kernel) applies when both stations'
 // bins of a product lie on the same rows (band shift a multiple of M1) and
-// each station's frame fits the first pass's LDS word stage.
+// both code widths are at most 16 bits (two codes per funnel shift).
 // The fused four-step runs at M1 = 128 (variant 1 of the large form).
 static bool large_fused_possible(const fxc_plan *p) {
     const int64_t sb = p->desc.a2s - p->desc.a1s;
     return p->fft_form == 2 && p->desc.precision == FXC_PRECISION_CHOSEN && p->mf_b1 &&
            (p->M / 128 == 128 || p->M / 128 == 256) && (sb % 128) == 0 &&
-           (size_t)p->st[0].nwords * 4 <= kLargeFusedMaxWordBytes &&
-           (size_t)p->st[1].nwords * 4 <= kLargeFusedMaxWordBytes;
+           p->st[0].bits <= 16 && p->st[1].bits <= 16;
 }
 
 static bool large_fused_ok(const fxc_plan *p) {
@@ -1347,8 +1346,8 @@ static fxc_status enqueue_large(fxc_plan *p, const SectorArgs &base, int64_t nf,
             HIP_TRY(launch_large_pass2_mac(a, (const float2 *)p->d_z[0].p, (const float2 *)p->d_z[1].p,
                                            params, p->M1, p->M2, twl2, kLargeGroups, c0 > 0 ? 1 : 0, s));
         }
-        HIP_TRY(launch_reduce((const double *)p->part.p, kLargeGroups, p->half, d_acc,
-                              (double *)p->red_scratch.p, 64, s));
+        HIP_TRY(launch_reduce_pairs((const double *)p->part.p, kLargeGroups, p->half, p->M1, p->M2,
+                                    d_acc, (double *)p->red_scratch.p, 64, s));
         return FXC_OK;
     }
     const bool chain = xcf_uses_chain(p);

@@ -239,6 +239,40 @@ hipError_t launch_reduce(const double *part, int n_wg, int half, double *out, do
     return hipGetLastError();
 }
 
+// The same reduce over partial slots held in the fused four-step's
+// pair-major order (index p 2 M2 + j: row p for j < M2, row M1 - p, or
+// M1 / 2 for p = 0, otherwise; k2 = j mod M2; bin M last), written to the
+// bins g = row + M1 k2.
+__global__ void k_reduce_final_pairs(const double *scratch, int n_chunks, int half, int M1,
+                                     int M2, double *out) {
+    const int col = blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= kNumProducts * half) return;
+    double s = 0.0;
+    for (int c = 0; c < n_chunks; ++c) s += scratch[(size_t)c * kNumProducts * half + col];
+    const int prod = col / half, idx = col - prod * half;
+    int g = idx;
+    if (idx < half - 1) {
+        const int p = idx / (2 * M2), j = idx - p * 2 * M2;
+        const int row = j < M2 ? p : (p == 0 ? M1 / 2 : M1 - p);
+        g = row + M1 * (j % M2);
+    }
+    out[(size_t)prod * half + g] = s;
+}
+
+hipError_t launch_reduce_pairs(const double *part, int n_wg, int half, int M1, int M2, double *out,
+                               double *scratch, int n_chunks, hipStream_t stream) {
+    const int cols = kNumProducts * half;
+    if (n_chunks < 1) n_chunks = 1;
+    if (n_chunks > n_wg) n_chunks = n_wg;
+    const int chunk = (n_wg + n_chunks - 1) / n_chunks;
+    n_chunks = (n_wg + chunk - 1) / chunk;
+    dim3 g1((cols + 255) / 256, n_chunks);
+    hipLaunchKernelGGL(k_reduce_chunks, g1, dim3(256), 0, stream, part, n_wg, cols, chunk, scratch);
+    hipLaunchKernelGGL(k_reduce_final_pairs, dim3((cols + 255) / 256), dim3(256), 0, stream, scratch,
+                       n_chunks, half, M1, M2, out);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------- dispatch
 // Elements per thread E and threads per station T for a given N
 // (M = N/2 = E*T; the workgroup holds 2T threads).

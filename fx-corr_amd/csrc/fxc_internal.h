@@ -203,11 +203,12 @@ hipError_t launch_rfft_mfma(const SectorArgs &a, const float *in, int64_t nframe
 
 // N >= 32768, chosen form: the four-step's first pass with the decode fused
 // (fxc_mfma.hip) and its second pass with the untangle and products fused
-// (fxc_large.hip).  The fused first pass stages a whole frame's words in LDS.
-constexpr size_t kLargeFusedMaxWordBytes = 64 * 1024;
+// (fxc_large.hip).
 hipError_t launch_large_pass1_fused(const SectorArgs &a, const int64_t *shifts, float2 *out0,
                                     float2 *out1, int M1, int M2, const float2 *tw_m,
                                     hipStream_t s);
+hipError_t launch_reduce_pairs(const double *part, int n_wg, int half, int M1, int M2, double *out,
+                               double *scratch, int n_chunks, hipStream_t stream);
 hipError_t launch_large_pass2_mac(const SectorArgs &a, const float2 *y1, const float2 *y2,
                                   const FrameParam *params, int M1, int M2, const float2 *tw_m2,
                                   int n_groups, int accumulate, hipStream_t s);

@@ -318,7 +318,9 @@ __device__ __forceinline__ void wave_stockham_pass(float2 *z, const float2 *tw, 
 // the frame's FP64 phase (k_mac_large's arithmetic) and FP64 sums.  The
 // rows are double-buffered in LDS, so a frame needs only that one barrier:
 // a wave rewrites a buffer two frames later, after every thread has passed
-// the next frame's barrier and so finished this frame's bins.
+// the next frame's barrier and so finished this frame's bins.  The frame
+// group's partial slot holds the pair's bins contiguously (coalesced
+// stores); the reduce over the slots puts them back in bin order.
 __device__ __forceinline__ float2 untangle_ab(float2 A, float2 B, int k, int M, float2 w) {
     if (k == 0 || k == M) return make_float2(k == 0 ? A.x + A.y : A.x - A.y, 0.0f);
     const float ex = 0.5f * (A.x + B.x), ey = 0.5f * (A.y - B.y);
@@ -440,13 +442,15 @@ __global__ __launch_bounds__(256) void k_large_pass2_mac(SectorArgs a, const flo
             nx1[e] = nx2[e];
         }
     }
+    // partial slot in pair-major order: index p 2 M2 + j (j < M2: row ra,
+    // k2 = j; else row rb, k2 = j - M2), bin M last; launch_reduce_pairs
+    // restores bin order
     double *out = a.part + (size_t)blockIdx.y * kNumProducts * half;
 #pragma unroll
     for (int i = 0; i <= BPT; ++i) {
         int g;
         if (i < BPT) {
-            const int j = tid + 256 * i;
-            g = (j < M2 ? ra : rb) + M1 * (j % M2);
+            g = p * 2 * M2 + tid + 256 * i;
         } else {
             if (!nyq) continue;
             g = M;

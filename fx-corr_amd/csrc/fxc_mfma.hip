@@ -1683,60 +1683,74 @@ __global__ __launch_bounds__(64 * R) void k_fft_pass1_mfma(const IN *in, float2 
     }
 }
 
-// decode_elem with the code width fixed at compile time (1, 2, 4 or 8
-// bits; 0: the run-time width of decode_elem), branch-free: the two words
-// are read at indexes clamped into the frame (a sample inside the frame
-// never takes bits from outside it; samples outside are zeroed), and the
-// LSB flip is a sign mask per sample parity (fm0 for sample 2m - shift,
-// fm1 for the next; the parity of 2m - shift is the shift's).
+// One element (two consecutive samples, the first at src0 = 2m - shift of
+// the shifted frame) from words staged per four-step row: lw is the local
+// index of the word holding bit src0 * bits and off its bit offset.  The
+// code width is fixed at compile time for 1, 2, 4 and 8 bits (0: the
+// station's run-time width, at most 16 so that both codes fit one funnel
+// shift).  Branch-free: the LSB flip is a sign mask per sample parity (fm0
+// for the first sample, fm1 for the second; the parity of src0 is the
+// shift's), and samples outside the frame are zero.
 template <int BITS>
-__device__ __forceinline__ float2 decode_elem_t(const uint32_t *words, const StationDev &st,
-                                                const float *lev_lds, int m, int shift, int N,
-                                                uint32_t fm0, uint32_t fm1) {
-    if constexpr (BITS == 0) {
-        return decode_elem(words, st, lev_lds, m, shift, N);
-    } else {
+__device__ __forceinline__ float2 decode_pair_local(const uint32_t *words, const StationDev &st,
+                                                    const float *lev_lds, int src0, int lw,
+                                                    uint32_t off, int N, uint32_t fm0,
+                                                    uint32_t fm1) {
+    const uint32_t v = __builtin_amdgcn_alignbit(words[lw + 1], words[lw], off);
+    float x0, x1;
+    if constexpr (BITS == 1) {
+        x0 = (v & 1u) ? st.lev4[1] : st.lev4[0];
+        x1 = (v & 2u) ? st.lev4[1] : st.lev4[0];
+    } else if constexpr (BITS == 2) {
+        x0 = level_of(v & 3u, st, lev_lds);
+        x1 = level_of((v >> 2) & 3u, st, lev_lds);
+    } else if constexpr (BITS == 4 || BITS == 8) {
         constexpr uint32_t mask = (1u << BITS) - 1u;
-        const int src0 = 2 * m - shift;
-        const int bp = src0 * BITS;             // may be negative (arithmetic shift floors)
-        const int w0 = bp >> 5;
-        const int hiw = st.nwords - 1;
-        const uint32_t lo = words[min(max(w0, 0), hiw)];
-        const uint32_t hi = words[min(max(w0 + 1, 0), hiw)];
-        const uint32_t v = __builtin_amdgcn_alignbit(hi, lo, (uint32_t)bp & 31u);
-        float x0, x1;
-        if constexpr (BITS == 1) {
-            x0 = (v & 1u) ? st.lev4[1] : st.lev4[0];
-            x1 = (v & 2u) ? st.lev4[1] : st.lev4[0];
-        } else if constexpr (BITS == 2) {
-            x0 = level_of(v & mask, st, lev_lds);
-            x1 = level_of((v >> 2) & mask, st, lev_lds);
-        } else {
-            x0 = lev_lds[v & mask];
-            x1 = lev_lds[(v >> BITS) & mask];
-        }
-        x0 = __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, x0) ^ fm0);
-        x1 = __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, x1) ^ fm1);
-        return make_float2((unsigned)src0 < (unsigned)N ? x0 : 0.0f,
-                           (unsigned)(src0 + 1) < (unsigned)N ? x1 : 0.0f);
+        x0 = lev_lds[v & mask];
+        x1 = lev_lds[(v >> BITS) & mask];
+    } else {
+        x0 = level_of(v & st.code_mask, st, lev_lds);
+        x1 = level_of((v >> st.bits) & st.code_mask, st, lev_lds);
     }
+    x0 = __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, x0) ^ fm0);
+    x1 = __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, x1) ^ fm1);
+    return make_float2((unsigned)src0 < (unsigned)N ? x0 : 0.0f,
+                       (unsigned)(src0 + 1) < (unsigned)N ? x1 : 0.0f);
+}
+
+// Word gw of a frame (little-endian, shuffle applied); zero outside it.
+__device__ __forceinline__ uint32_t frame_word(const uint8_t *frame, const StationDev &st, int gw) {
+    if (gw < 0 || gw >= st.nwords) return 0u;
+    uint32_t v = 0;
+    if (st.aligned) {
+        v = reinterpret_cast<const uint32_t *>(frame)[gw];
+    } else {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int b = 4 * gw + j;
+            if (b < st.bpf) v |= (uint32_t)frame[b] << (8 * j);
+        }
+    }
+    return shuffle_word(v, st);
 }
 
 // The same first pass with the frame decode fused in (chosen form, sector
-// path, M1 = 128 = 32 x 4).  A workgroup stages one station-frame's packed
-// words in LDS once (shuffle applied) and its four waves take 16-column
-// blocks on their own, with no workgroup barrier after the stage:
-//  - for each v = 0..3 the wave decodes its A_v operand straight from the
-//    words (decode_elem: levels, integer shift with zero fill, LSB flip) and
-//    runs the DFT-32 GEMM with the operands in swapped roles (table x data),
-//    so lane (r = l % 16, g = l / 16) receives column c0 + r at
-//    k' = 4 g + i (+ 16 for the second k' tile) for every v;
+// path, M1 = 128 = 32 x 4).  Workgroup (frame, part p) takes the columns
+// m2 in [64 p, 64 p + 64) of all 128 rows m1 (element m = m2 + M2 m1): it
+// stages in LDS, per row, the K = 4 bits + 1 packed words that hold those
+// columns' samples (shuffle applied), and its four waves take one 16-column
+// block each, with no workgroup barrier after the stage:
+//  - for each v = 0..3 the wave decodes its A_v operand from the staged
+//    words (levels, integer shift with zero fill, LSB flip) and runs the
+//    DFT-32 GEMM with the operands in swapped roles (table x data), so lane
+//    (r = l % 16, g = l / 16) receives column c0 + r at k' = 4 g + i (+ 16
+//    for the second k' tile) for every v;
 //  - the DFT-4 over v is then lane-local (no LDS exchange), and each store
 //    instruction writes 16 consecutive columns of 4 rows k1 = k' + 32 k2.
 // Y[k1][m2] W_M^{m2 k1} lands at out[m2 + M2 k1] as in k_fft_pass1_mfma; the
 // output twiddle is W_M^{m2 k'} W_M^{32 m2 k2} (two table values).
-// Grid: frames x parts, one launch per station; wave w of part p takes
-// column block 4 p + w.
+// Row m1's words start at word m1 rw + base0 of the frame, rw = M2 bits / 16
+// (the row's 2 M2 samples), base0 = floor((128 p - shift) bits / 32).
 __device__ __forceinline__ f4 mf_stage1_t(half8 a0, half8 a1, half8 bh0, half8 bl0, half8 bh1,
                                           half8 bl1) {
     f4 d = {0.f, 0.f, 0.f, 0.f};
@@ -1748,15 +1762,17 @@ __device__ __forceinline__ f4 mf_stage1_t(half8 a0, half8 a1, half8 bh0, half8 b
 }
 
 template <int BITS>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_large_pass1_fused(SectorArgs a, int station,
-                                                           const int64_t *shifts, float2 *out,
-                                                           int M2, int parts, const float2 *tw_m) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_large_pass1_fused(
+    SectorArgs a, int station, const int64_t *shifts, float2 *out, int M2, int parts,
+    const float2 *tw_m) {
     constexpr int R = 4;
-    extern __shared__ uint32_t words[];   // the frame's st.nwords words
+    extern __shared__ uint32_t words[];   // 128 rows x K words, one zero word after them
     __shared__ uint4 b1[16 * 64];
     __shared__ float2 twv[R][32];         // W_128^{v k'}
     __shared__ float lev[256];
     const StationDev &st = a.st[station];
+    const int bits = BITS ? BITS : st.bits;
+    const int K = 4 * bits + 1, rw = M2 * bits / 16;
     const int M = a.M;
     const int tid = threadIdx.x;
     const int l = tid & 63;
@@ -1769,9 +1785,28 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
     if (tid < R * 32) twv[tid >> 5][tid & 31] = tw_m[(tid >> 5) * (tid & 31) * M2];
     if (st.bits > 2 && st.bits <= 8)
         for (int i = tid; i < 256; i += 256) lev[i] = (i <= (int)st.code_mask) ? st.levels[i] : 0.0f;
-    stage_words<256>((station == 0 ? a.raw1 : a.raw2) + f * st.bpf, st, words, tid);
-    __syncthreads();
     const int sh = clamp_shift(shifts[f * 2 + station], a.N);
+    const int base0 = ((128 * part - sh) * bits) >> 5;   // arithmetic shift: floor
+    {
+        const uint8_t *frame = (station == 0 ? a.raw1 : a.raw2) + f * st.bpf;
+        const int nst = 128 * K;
+        for (int i0 = 0; i0 < nst; i0 += 256 * 8) {   // eight loads in flight per thread
+            uint32_t v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int idx = i0 + tid + 256 * u;
+                const int m1 = idx / K;
+                v[u] = idx < nst ? frame_word(frame, st, m1 * rw + base0 + (idx - m1 * K)) : 0u;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int idx = i0 + tid + 256 * u;
+                if (idx < nst) words[idx] = v[u];
+            }
+        }
+        if (tid == 0) words[nst] = 0u;
+    }
+    __syncthreads();
     // LSB flip (odd samples of the unshifted frame): sample 2m - sh is odd iff sh is
     const uint32_t fm0 = (st.lsb && (sh & 1)) ? 0x80000000u : 0u;
     const uint32_t fm1 = (st.lsb && !(sh & 1)) ? 0x80000000u : 0u;
@@ -1800,8 +1835,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     const int u = 16 * kap + g + 4 * q;
-                    const float2 x = decode_elem_t<BITS>(words, st, lev, m2 + M2 * (v + R * u), sh, a.N,
-                                                                 fm0, fm1);
+                    const int m1 = v + R * u;
+                    const int src0 = 2 * (m2 + M2 * m1) - sh;
+                    const int bp = src0 * bits;
+                    const float2 x = decode_pair_local<BITS>(words, st, lev, src0,
+                                                             (bp >> 5) - base0 + m1 * (K - rw),
+                                                             (uint32_t)bp & 31u, a.N, fm0, fm1);
                     d[kap * 4 + q] = __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pkrtz(x.x, x.y));
                 }
             av0[v] = __builtin_bit_cast(half8, make_uint4(d[0], d[1], d[2], d[3]));
@@ -1839,7 +1878,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
 }
 
 // Shifts: frame f's integer shifts at shifts[2 f + station]; one launch
-// per station, the kernel instance fixed by its code width.
+// per station, the kernel instance fixed by its code width (1..16 bits).
 hipError_t launch_large_pass1_fused(const SectorArgs &a, const int64_t *shifts, float2 *out0,
                                     float2 *out1, int M1, int M2, const float2 *tw_m,
                                     hipStream_t s) {
@@ -1849,15 +1888,9 @@ hipError_t launch_large_pass1_fused(const SectorArgs &a, const int64_t *shifts, 
     // 16-column blocks per frame: M2 / 16; one per wave, four waves per part
     const int parts = M2 / 64;
     for (int st = 0; st < 2; ++st) {
-        const size_t lds = (size_t)a.st[st].nwords * 4;
-        if (lds > kLargeFusedMaxWordBytes) return hipErrorInvalidValue;
+        if (a.st[st].bits < 1 || a.st[st].bits > 16) return hipErrorInvalidValue;
+        const size_t lds = (size_t)(128 * (4 * a.st[st].bits + 1) + 1) * 4;   // <= 33,284 bytes
         auto go = [&](auto kern) -> hipError_t {
-            if (lds > 48 * 1024) {
-                hipError_t e = hipFuncSetAttribute((const void *)kern,
-                                                   hipFuncAttributeMaxDynamicSharedMemorySize,
-                                                   (int)kLargeFusedMaxWordBytes);
-                if (e != hipSuccess) return e;
-            }
             hipLaunchKernelGGL(kern, dim3((unsigned)(a.nf * parts)), dim3(256), lds, s, a, st, shifts,
                                st == 0 ? out0 : out1, M2, parts, tw_m);
             return hipGetLastError();
