@@ -1325,11 +1325,20 @@ static bool large_fused_ok(const fxc_plan *p) {
     return p->variant == 1 && p->M1 == 128 && large_fused_possible(p);
 }
 
+// Frame groups of the fused second pass: enough workgroups (M1 / 2 row
+// pairs x groups) for one resident round at the kernel's occupancy (three
+// workgroups per CU at M2 = 256, five at M2 = 128), at most kLargeGroups.
+static int large_fused_groups(const fxc_plan *p) {
+    const int per_cu = p->M2 == 256 ? 3 : 5;
+    return std::max(1, std::min(kLargeGroups, num_cus() * per_cu / (p->M1 / 2)));
+}
+
 static fxc_status enqueue_large(fxc_plan *p, const SectorArgs &base, int64_t nf, double *d_acc,
                                 hipStream_t s) {
     const float2 *twl1 = (const float2 *)(p->M1 == 128 ? p->tw_l1.p : p->tw_l2.p);
     const float2 *twl2 = (const float2 *)(p->M2 == 128 ? p->tw_l1.p : p->tw_l2.p);
     if (large_fused_ok(p)) {
+        const int ng = large_fused_groups(p);
         for (int64_t c0 = 0; c0 < nf; c0 += p->chunk_use) {
             const int64_t cn = std::min(p->chunk_use, nf - c0);
             SectorArgs a = base;
@@ -1344,10 +1353,10 @@ static fxc_status enqueue_large(fxc_plan *p, const SectorArgs &base, int64_t nf,
             HIP_TRY(launch_large_pass1_fused(a, shifts, (float2 *)p->d_z[0].p, (float2 *)p->d_z[1].p,
                                              p->M1, p->M2, (const float2 *)p->tw_lm.p, s));
             HIP_TRY(launch_large_pass2_mac(a, (const float2 *)p->d_z[0].p, (const float2 *)p->d_z[1].p,
-                                           params, p->M1, p->M2, twl2, kLargeGroups, c0 > 0 ? 1 : 0, s));
+                                           params, p->M1, p->M2, twl2, ng, c0 > 0 ? 1 : 0, s));
         }
-        HIP_TRY(launch_reduce_pairs((const double *)p->part.p, kLargeGroups, p->half, p->M1, p->M2,
-                                    d_acc, (double *)p->red_scratch.p, 64, s));
+        HIP_TRY(launch_reduce_pairs((const double *)p->part.p, ng, p->half, p->M1, p->M2, d_acc,
+                                    (double *)p->red_scratch.p, 64, s));
         return FXC_OK;
     }
     const bool chain = xcf_uses_chain(p);

@@ -467,6 +467,165 @@ __global__ __launch_bounds__(256) void k_large_pass2_mac(SectorArgs a, const flo
     }
 }
 
+// The same second pass at M2 = 256 with two radix-16 passes per row: each
+// wave transforms one frame's four rows (lane = 16 q + j: row q, butterfly
+// j) -- the first pass from registers, its outputs multiplied by the second
+// pass's twiddles W_256^{j r} (held per lane) and transposed through LDS,
+// the second pass writing X[j + 16 r] -- so a frame costs 48 LDS accesses
+// per lane instead of the radix-4 passes' 112 over four waves.  The four
+// waves transform four frames, then after a barrier every thread forms its
+// bins of all four (k_large_pass2_mac's bin code), and a second barrier
+// frees the rows.
+__device__ __forceinline__ void pair_bins(const float2 (*zb)[274], const float2 (*twp)[256],
+                                          int tid, int M, int M1, int ra, int rb, int gs, int ov,
+                                          int sbo, int jo, int grid, bool do_acf, bool do_xcf,
+                                          bool ov_only, bool nyq, double dth, double phw,
+                                          double (&acc)[3][4]) {
+    constexpr int M2 = 256;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        int g, row, k2;
+        if (i < 2) {
+            row = i == 0 ? ra : rb;
+            k2 = tid;
+            g = row + M1 * k2;
+        } else {
+            if (!nyq) continue;
+            row = 0;
+            k2 = M2;
+            g = M;
+        }
+        const int bg = row == ra ? 0 : 1;
+        const int bm = (row == 0 ? row : M1 - row) == ra ? 0 : 1;
+        const int km = row == 0 ? (M2 - k2) & (M2 - 1) : M2 - 1 - k2;
+        const float2 Zg = zb[bg][pad(k2 & (M2 - 1))], Zm = zb[bm][pad(km)];
+        const float2 Xg = untangle_ab(Zg, Zm, g, M, twp[bg][k2 & (M2 - 1)]);
+        const bool in_ov = g >= gs && g < gs + ov;
+        float2 Xo = make_float2(0.0f, 0.0f);
+        if (in_ov) {
+            const int h = g + sbo, kh = k2 + jo;
+            const int khm = row == 0 ? (M2 - kh) & (M2 - 1) : M2 - 1 - kh;
+            const float2 Oh = zb[2 + bg][pad(kh & (M2 - 1))];
+            const float2 Om = zb[2 + bm][pad(khm & (M2 - 1))];
+            Xo = untangle_ab(Oh, Om, h, M, twp[bg][kh & (M2 - 1)]);
+        }
+        if (do_acf) {
+            const float pgrid = (!ov_only || in_ov) ? norm_sqr(Xg) : 0.0f;
+            const float poth = in_ov ? norm_sqr(Xo) : 0.0f;
+            acc[i][0] += (double)(grid == 0 ? pgrid : poth);
+            acc[i][1] += (double)(grid == 0 ? poth : pgrid);
+        }
+        if (in_ov && do_xcf) {
+            const float r = (float)wrap_turns(fma((double)(g - gs), dth, phw));
+            const float2 phf = make_float2(__builtin_amdgcn_cosf(r), __builtin_amdgcn_sinf(r));
+            const float2 X1 = (grid == 0) ? Xg : Xo;
+            const float2 X2 = (grid == 0) ? Xo : Xg;
+            const float2 v2 = c_mul(c_mul(X1, c_conj(X2)), phf);
+            acc[i][2] += (double)v2.x;
+            acc[i][3] += (double)v2.y;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_large_pass2_r16(SectorArgs a, const float2 *y1,
+                                                         const float2 *y2, const FrameParam *params,
+                                                         int M1, const float2 *tw_m2, int64_t fpg,
+                                                         int accumulate) {
+    constexpr int M2 = 256, LP = M2 + M2 / 16 + 2;
+    __shared__ float2 zr[4][4][LP];   // [frame of the iteration = wave][grid ra, grid rb, other ra, other rb]
+    __shared__ float2 twp[2][M2];     // tw_post[row + M1 k2], rows ra / rb
+    static_assert(LP == 274, "pair_bins row pitch");
+    const int M = a.M;
+    const int half = M + 1;
+    const int tid = threadIdx.x;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int lane = tid & 63;
+    const int q = lane >> 4, j = lane & 15;
+    const int p = blockIdx.x;
+    const int ra = p, rb = p == 0 ? M1 / 2 : M1 - p;
+    for (int i = tid; i < 2 * M2; i += 256)
+        twp[i / M2][i % M2] = a.tw_post[(i < M2 ? ra : rb) + M1 * (i % M2)];
+    float2 tw[16];   // W_256^{j r}
+#pragma unroll
+    for (int r = 1; r < 16; ++r) tw[r] = tw_m2[(j * r) & (M2 - 1)];
+    const int64_t g0 = (int64_t)blockIdx.y * fpg;
+    int64_t g1 = g0 + fpg;
+    if (g1 > a.nf) g1 = a.nf;
+    const bool do_acf = (a.flags & FLAG_ACF) != 0;
+    const bool do_xcf = (a.flags & FLAG_XCF) != 0;
+    const bool ov_only = (a.flags & FLAG_ACF_OVERLAP) != 0;
+    const int gs = (a.grid == 0) ? a.a1s : a.a2s;
+    const int sbo = (a.grid == 0) ? (a.a2s - a.a1s) : (a.a1s - a.a2s);
+    const int jo = sbo / M1;
+    const float2 *src_row = ((q & 2) ? (a.grid == 0 ? y2 : y1) : (a.grid == 0 ? y1 : y2)) +
+                            (int64_t)((q & 1) ? rb : ra) * M2 + j;
+    const bool nyq = p == 0 && tid == 0;
+    double acc[3][4];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[i][c] = 0.0;
+    float2 cur[16];
+    {
+        const int64_t f = g0 + w;
+#pragma unroll
+        for (int r = 0; r < 16; ++r)
+            cur[r] = f < g1 ? src_row[f * (int64_t)M + 16 * r] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();   // twiddles staged
+    float2 *z = zr[w][q];
+    for (int64_t it = g0; it < g1; it += 4) {
+        const int64_t f = it + w, fn = f + 4;
+        float2 nxt[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r)
+            nxt[r] = fn < g1 ? src_row[fn * (int64_t)M + 16 * r] : make_float2(0.f, 0.f);
+        if (f < g1) {
+            dft16(cur);                                  // pass 1: inputs j + 16 r
+#pragma unroll
+            for (int r = 1; r < 16; ++r) cur[r] = c_mul(cur[r], tw[r]);
+#pragma unroll
+            for (int r = 0; r < 16; ++r) z[pad(16 * j + r)] = cur[r];
+            wave_sync_lds();
+#pragma unroll
+            for (int r = 0; r < 16; ++r) cur[r] = z[pad(j + 16 * r)];
+            dft16(cur);                                  // pass 2
+            wave_sync_lds();
+#pragma unroll
+            for (int r = 0; r < 16; ++r) z[pad(j + 16 * r)] = cur[r];   // X[j + 16 r]
+        }
+        __syncthreads();   // the iteration's rows are ready
+        const int nfr = (int)((g1 - it) < 4 ? (g1 - it) : 4);
+        for (int s2 = 0; s2 < nfr; ++s2)
+            pair_bins(zr[s2], twp, tid, M, M1, ra, rb, gs, a.ov, sbo, jo, a.grid, do_acf, do_xcf,
+                      ov_only, nyq, params[it + s2].dth, params[it + s2].phw, acc);
+        __syncthreads();   // every thread is done with the rows
+#pragma unroll
+        for (int r = 0; r < 16; ++r) cur[r] = nxt[r];
+    }
+    // partial slot in pair-major order (as k_large_pass2_mac)
+    double *out = a.part + (size_t)blockIdx.y * kNumProducts * half;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        int g;
+        if (i < 2) {
+            g = p * 2 * M2 + tid + 256 * i;
+        } else {
+            if (!nyq) continue;
+            g = M;
+        }
+        const double prev[4] = {accumulate ? out[P_11 * half + g] : 0.0,
+                                accumulate ? out[P_22 * half + g] : 0.0,
+                                accumulate ? out[P_12R * half + g] : 0.0,
+                                accumulate ? out[P_12I * half + g] : 0.0};
+        out[P_PH * half + g] = 0.0;
+        out[P_11 * half + g] = prev[0] + acc[i][0];
+        out[P_22 * half + g] = prev[1] + acc[i][1];
+        out[P_12R * half + g] = prev[2] + acc[i][2];
+        out[P_12I * half + g] = prev[3] + acc[i][3];
+    }
+}
+
 hipError_t launch_large_pass2_mac(const SectorArgs &a, const float2 *y1, const float2 *y2,
                                   const FrameParam *params, int M1, int M2, const float2 *tw_m2,
                                   int n_groups, int accumulate, hipStream_t s) {
@@ -478,7 +637,7 @@ hipError_t launch_large_pass2_mac(const SectorArgs &a, const float2 *y1, const f
         hipLaunchKernelGGL(k_large_pass2_mac<128>, grid, dim3(256), 0, s, a, y1, y2, params, M1, tw_m2,
                            fpg, accumulate);
     else
-        hipLaunchKernelGGL(k_large_pass2_mac<256>, grid, dim3(256), 0, s, a, y1, y2, params, M1, tw_m2,
+        hipLaunchKernelGGL(k_large_pass2_r16, grid, dim3(256), 0, s, a, y1, y2, params, M1, tw_m2,
                            fpg, accumulate);
     return hipGetLastError();
 }
