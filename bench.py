@@ -593,7 +593,7 @@ def c1_cpu_legs(stream, threads, seed=0):
                       f"{el_c:.2f} s / {el_f:.2f} s wall"}
 
 
-def c3_bench(stream, reps=2, seed=2303):
+def c3_bench(stream, reps=2, seed=2303, forms=("plain", "chosen"), n_sectors=30):
     """Precision-stress workload: 512 Msps, A 4-bit / B 1-bit, FFT 65536,
     16 MHz centre offset (band shift 2048, overlap 30,720), continuum
     coherence 1e-3 plus a 3-channel Gaussian line of peak coherence 0.05 at a
@@ -601,10 +601,12 @@ def c3_bench(stream, reps=2, seed=2303):
     device.  Times the 30-sector job (ACF + XCF) in both forms with the input
     resident in HBM, and checks each form's integrated spectrum: the line is
     the strongest channel with the expected coherence, the continuum has the
-    expected mean coherence, and the forms agree."""
+    expected mean coherence, and the forms agree.  A profile may run the
+    first n_sectors sectors of one form only."""
     n, fs, bits = 65536, 512e6, (4, 1)
     total = int(30 * fs // n)
-    sectors = F.sector_partition(total, n / fs)
+    sectors = F.sector_partition(total, n / fs)[:n_sectors]
+    total = int(sum(sectors))
     band = F.band_alignment(n, fs, 8400.0, 8384.0, 256.0, 256.0)
     line_bin = int(np.random.default_rng(seed).integers(2000, 28000))
     lv = (_uniform_levels(4), _uniform_levels(1))
@@ -620,6 +622,8 @@ def c3_bench(stream, reps=2, seed=2303):
     acc = F.DeviceBuffer(len(sectors) * F.N_PRODUCTS * half * 8)
     out, tot = {}, {}
     for form, prec in (("plain", F.PRECISION_PLAIN_FP32), ("chosen", F.PRECISION_CHOSEN)):
+        if form not in forms:
+            continue
         plan = F.Plan(n, bits=bits, levels=list(lv), band=band, df_hz=fs / n,
                       products=F.PRODUCT_ACF | F.PRODUCT_XCF, precision=prec)
 
@@ -632,6 +636,7 @@ def c3_bench(stream, reps=2, seed=2303):
 
         job()
         stream.sync()
+        print(f"[bench] C3 {form}: warm-up job done", file=sys.stderr, flush=True)
         plan.set_timing(True)
         e0, e1 = F.Event(), F.Event()
         e0.record(stream)
@@ -650,7 +655,7 @@ def c3_bench(stream, reps=2, seed=2303):
                      "sector_ms": round(kms / max(launches, 1), 3)}
         plan.close()
         print(f"[bench] C3 {form}: {out[form]['value']:,.0f} Msamples/s/station, "
-              f"{ms:.1f} ms per 30-sector job", file=sys.stderr, flush=True)
+              f"{ms:.1f} ms per {len(sectors)}-sector job", file=sys.stderr, flush=True)
     acc.free()
     d1.free()
     d2.free()
@@ -662,18 +667,22 @@ def c3_bench(stream, reps=2, seed=2303):
         checks[form] = {"peak_channel": int(np.argmax(np.abs(coh))) + lo,
                         "line_coherence": round(float(coh[line_bin - lo].real), 5),
                         "continuum_coherence": float(f"{np.mean(coh[far].real):.4e}")}
-    x = [tot[f][3, lo:hi] + 1j * tot[f][4, lo:hi] for f in ("chosen", "plain")]
-    rms = np.sqrt(np.mean(np.abs(x[1]) ** 2))
-    agree = amp_phase(x[0], x[1])
-    return {"workload": "512 Msps, A 4-bit / B 1-bit, FFT 65536, 16 MHz offset (shift 2048, "
-                        "overlap 30720), continuum coherence 1e-3 + 3-channel line of 0.05, "
-                        f"30 s = {total} frames in 30 sectors, input resident in HBM",
-            "forms": out, "chosen_over_plain": round(out["chosen"]["value"] / out["plain"]["value"], 3),
-            "line_channel": line_bin, "checks": checks,
-            "chosen_vs_plain_xcf": {"max_abs_over_rms": float(f"{np.max(np.abs(x[0] - x[1])) / rms:.3e}"),
-                                    "amp_max_strong": float(f"{agree['amp_max_strong']:.3e}"),
-                                    "phase_max_strong_rad": float(f"{agree['phase_max_strong_rad']:.3e}")},
-            "generation_s": round(gen_s, 2), "reps": reps}
+    res = {"workload": "512 Msps, A 4-bit / B 1-bit, FFT 65536, 16 MHz offset (shift 2048, "
+                       "overlap 30720), continuum coherence 1e-3 + 3-channel line of 0.05, "
+                       f"{total * n / fs:.0f} s = {total} frames in {len(sectors)} sectors, "
+                       "input resident in HBM",
+           "forms": out, "line_channel": line_bin, "checks": checks,
+           "generation_s": round(gen_s, 2), "reps": reps}
+    if "chosen" in tot and "plain" in tot:
+        x = [tot[f][3, lo:hi] + 1j * tot[f][4, lo:hi] for f in ("chosen", "plain")]
+        rms = np.sqrt(np.mean(np.abs(x[1]) ** 2))
+        agree = amp_phase(x[0], x[1])
+        res["chosen_over_plain"] = round(out["chosen"]["value"] / out["plain"]["value"], 3)
+        res["chosen_vs_plain_xcf"] = {
+            "max_abs_over_rms": float(f"{np.max(np.abs(x[0] - x[1])) / rms:.3e}"),
+            "amp_max_strong": float(f"{agree['amp_max_strong']:.3e}"),
+            "phase_max_strong_rad": float(f"{agree['phase_max_strong_rad']:.3e}")}
+    return res
 
 
 def main():
@@ -708,6 +717,11 @@ def main():
     ap.add_argument("--c3", action="store_true",
                     help="also time the 512 Msps 4/1-bit FFT-65536 30-sector precision-stress "
                          "job with its line + continuum signal (adds 'c3')")
+    ap.add_argument("--c3-forms", default="plain,chosen",
+                    help="forms the --c3 leg runs (a profile may run one)")
+    ap.add_argument("--c3-sectors", type=int, default=30,
+                    help="first sectors of the 30-sector --c3 job to run (a profile may run a few)")
+    ap.add_argument("--c3-reps", type=int, default=2, help="timed --c3 jobs per form")
     ap.add_argument("--batch", action="store_true",
                     help="also time the 48-scan uneven batch captured in one graph (adds 'batch')")
     ap.add_argument("--batch-scale", type=float, default=0.125,
@@ -933,7 +947,10 @@ def main():
     sizes = size_sweep(stream) if args.sizes else None
     phased = phased_sweep(stream) if args.phased else None
     batch = batch_bench(stream, args.batch_scale) if args.batch else None
-    c3 = c3_bench(stream) if args.c3 else None
+    c3 = None
+    if args.c3:
+        c3 = c3_bench(stream, reps=args.c3_reps, n_sectors=args.c3_sectors,
+                      forms=[f.strip() for f in args.c3_forms.split(",") if f.strip()])
 
     r = results[main_form]
     line = {

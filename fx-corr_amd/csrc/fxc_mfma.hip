@@ -1777,8 +1777,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
     const int tid = threadIdx.x;
     const int l = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int64_t f = blockIdx.x / parts;
-    const int part = blockIdx.x % parts;
+    // workgroups are dealt to the 8 XCDs in turn: give each XCD a contiguous
+    // run of (frame, part) so a frame's parts share one L2 (their rows'
+    // words share cache lines)
+    const int nb = (int)gridDim.x, xcd = (int)blockIdx.x & 7, k = (int)blockIdx.x >> 3;
+    const int lb = xcd * (nb >> 3) + min(xcd, nb & 7) + k;
+    const int64_t f = lb / parts;
+    const int part = lb % parts;
     if (f >= a.nf) return;   // uniform per workgroup
     const uint4 *b1g = reinterpret_cast<const uint4 *>(a.mf_tab);
     for (int i = tid; i < 16 * 64; i += 256) b1[i] = b1g[i];
