@@ -652,7 +652,9 @@ def c3_bench(stream, reps=2, seed=2303, forms=("plain", "chosen"), n_sectors=30)
             len(sectors), F.N_PRODUCTS, half).sum(axis=0)
         tot[form] = a
         out[form] = {"value": round(total * n / (ms * 1e-3) / 1e6, 1), "ms_per_job": round(ms, 3),
-                     "sector_ms": round(kms / max(launches, 1), 3)}
+                     "ms_per_sector": round(ms / len(sectors), 3),
+                     "kernel": "fused four-step" if plan.info.fft_form == 2 and plan.info.variant == 1
+                               else "four-step through HBM"}
         plan.close()
         print(f"[bench] C3 {form}: {out[form]['value']:,.0f} Msamples/s/station, "
               f"{ms:.1f} ms per {len(sectors)}-sector job", file=sys.stderr, flush=True)
