@@ -70,6 +70,14 @@ def workload_setup(seed=2025):
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_1024msps_2bit_fft1024.json")
+# Issue counts per frame pair of the fused sector kernels, from the counter
+# pass in profiles/sector_kernels_1024msps_2bit_fft1024_kernel_stats.md:
+# (VALU instructions, matrix-busy cycles).  A wave64 VALU instruction takes
+# 4 cycles of its SIMD; MI355X has 1,024 SIMDs at up to 2.4 GHz.
+ISSUE_PER_FRAME = {"k_sector_mf3": (664.8, 512.0), "k_sector_mf3<FP32 batches>": (618.0, 512.0),
+                   "k_sector<8, 64, 0>": (2973.0, 0.0)}
+SIMDS, CLOCK_HZ = 1024, 2.4e9
+COUNTS_FILE = "profiles/sector_kernels_1024msps_2bit_fft1024_kernel_stats.md"
 # fused kernel of each form (the MFMA form by kernel organization variant)
 KERNEL_KEYS = {("chosen", 0): "k_sector_mf2", ("chosen", 1): "k_sector_mf3",
                ("chosen", 2): "k_sector_mf3<FP32 batches>", ("plain", 0): "k_sector<8, 64, 0>"}
@@ -271,6 +279,22 @@ def serve_bench(raw1, raw2, bpf, cfg, shuffle_b, n_jobs, load, seed=7,
                 "mix_types": [f"FFT {t['n']}, {t['bits'][0]}/{t['bits'][1]}-bit, "
                               f"{t['fs'] / 1e6:.0f} Msps" for t in mix]})
     return out
+
+
+def issue_fractions(key, frames, launch_ms):
+    """The compute bound of a fused kernel made explicit: the kernels issue
+    VALU and MFMA work from the same SIMDs (compute-bound, the FFT flops
+    priced against the FP32 vector peak), so report the share of the SIMDs'
+    cycles that its VALU instructions and its matrix-busy cycles take at the
+    measured launch time."""
+    if key not in ISSUE_PER_FRAME:
+        return {}
+    valu, mfma = ISSUE_PER_FRAME[key]
+    cycles = SIMDS * CLOCK_HZ * launch_ms * 1e-3
+    return {"issue": "VALU + MFMA issue on the same SIMDs (peak: FP32 vector)",
+            "valu_issue_frac": round(frames * valu * 4 / cycles, 4),
+            "mfma_busy_frac": round(frames * mfma / cycles, 4),
+            "issue_counts_source": COUNTS_FILE}
 
 
 def load_traffic(key):
@@ -982,8 +1006,10 @@ def main():
                      "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(r["tflops"] / FP32_PEAK_TFLOPS, 4),
                      "traffic": load_traffic(r["kernel"]),
+                     "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT),
                      "kernel": f"{KERNEL_LABELS[r['kernel']]} ({r['kernel']})",
-                     "hbm_frac": round(r["gbs"] / HBM_PEAK_GBS, 4)},
+                     "hbm_frac": round(r["gbs"] / HBM_PEAK_GBS, 4),
+                     **issue_fractions(r["kernel"], total_frames / len(sectors), r["avg_launch_ms"])},
         "cpu_baseline": cpu,
         "parity": parity,
         "pcie_inclusive": pcie,
