@@ -315,6 +315,8 @@ FORM_NAMES = {0: "FP32 Stockham", 1: "FP16-MFMA four-step", 2: "FP32 four-step t
 
 def form_name(info):
     """The FFT a plan runs (fxc_plan_info.fft_form / fft_mfma)."""
+    if info.fft_form == 2 and info.fft_mfma and info.variant == 1:
+        return "fused four-step: decode + FP16-MFMA DFT-128 columns, row DFTs + products"
     if info.fft_form == 2 and info.fft_mfma:
         return "four-step through HBM, FP16-MFMA DFT-32 first pass"
     if info.fft_form == 0 and info.fft_mfma:

@@ -217,41 +217,12 @@ __device__ __forceinline__ uint32_t shuffle_word(uint32_t w, const StationDev &s
 
 // Stage the frame's packed words (little-endian, shuffle applied) in LDS;
 // T threads, t = this thread's index among them.
-// A 16-byte-aligned frame is read with 16-byte loads, eight in flight per
-// thread before any is stored, so a large frame costs one or two load
-// round trips rather than one per word.
 template <int T>
 __device__ __forceinline__ void stage_words(const uint8_t *frame, const StationDev &st,
                                             uint32_t *words, int t) {
     if (st.aligned) {
         const uint32_t *src = reinterpret_cast<const uint32_t *>(frame);
-        const int nw = st.nwords;
-        if ((reinterpret_cast<uintptr_t>(frame) & 15u) == 0) {
-            typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-            const u32x4 *s4 = reinterpret_cast<const u32x4 *>(frame);
-            const int n4 = nw >> 2;
-            for (int b = 0; b < n4; b += 8 * T) {
-                u32x4 v[8];
-#pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    const int i = b + t + T * u;
-                    if (i < n4) v[u] = __builtin_nontemporal_load(s4 + i);
-                }
-#pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    const int i = b + t + T * u;
-                    if (i < n4) {
-                        words[4 * i + 0] = shuffle_word(v[u].x, st);
-                        words[4 * i + 1] = shuffle_word(v[u].y, st);
-                        words[4 * i + 2] = shuffle_word(v[u].z, st);
-                        words[4 * i + 3] = shuffle_word(v[u].w, st);
-                    }
-                }
-            }
-            for (int w = 4 * n4 + t; w < nw; w += T) words[w] = shuffle_word(src[w], st);
-            return;
-        }
-        for (int w = t; w < nw; w += T) words[w] = shuffle_word(__builtin_nontemporal_load(src + w), st);
+        for (int w = t; w < st.nwords; w += T) words[w] = shuffle_word(__builtin_nontemporal_load(src + w), st);
     } else {
         for (int w = t; w < st.nwords; w += T) {
             uint32_t v = 0;
