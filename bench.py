@@ -424,7 +424,7 @@ def batch_bench(stream, scale=0.125, reps=3, seed=48):
     into an 8-bit series.  Per form, every ACF/XCF plan is autotuned on its
     own scan, all sector calls are captured into one graph, and the graph is
     replayed `reps` times after one warm-up replay; the phased plans run the
-    plain form in both."""
+    form's own arithmetic (chosen: rotation phases in FP64, no f32 chains)."""
     rng = np.random.default_rng(seed)
     scans, total = [], 0
     for i in range(48):
@@ -474,7 +474,7 @@ def batch_bench(stream, scale=0.125, reps=3, seed=48):
                 pp = F.Plan(n, bits=sc["bits"], levels=lv, shuffles=(sc["sh1"], None),
                             lsb=sc["lsb"], df_hz=sc["fs"] / n,
                             products=F.PRODUCT_PHASED | F.PRODUCT_RAW_OUT, w=sc["w"],
-                            bit_out=max(sc["bits"]), output_lsb=sc["output_lsb"])
+                            bit_out=max(sc["bits"]), output_lsb=sc["output_lsb"], precision=prec)
                 job.update(pplan=pp, pacc=F.DeviceBuffer(F.N_PRODUCTS * (n // 2 + 1) * 8),
                            praw=F.DeviceBuffer(sc["nf"] * pp.info.bpf_out))
             jobs.append(job)

@@ -1203,11 +1203,13 @@ static fxc_status enqueue_phased(fxc_plan *p, const SectorArgs &base, int64_t nf
     // LDS; the N * 4 term keeps the chunking of the staged form).  The chunk
     // does not depend on whether .raw is written, so |cb|^2 sums in the same
     // order either way.
+    // chosen form: rotations from FP64 phases in the kernel, no chain table
+    const int prec = p->desc.precision == FXC_PRECISION_CHOSEN ? 1 : 0;
     const int ld = chain_ld_for((int)half);
     const size_t per_frame = 2 * (size_t)ld * 8 + half * 8 + (size_t)p->N * 4;
     const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(nf, (int64_t)(((size_t)256 << 20) / per_frame)));
     const int64_t span = chain_span(nf, chunk, 2 * (size_t)ld * 8, (int)half);
-    GROW(p, p->d_chain, (size_t)span * 2 * ld * 8, s, "phased rotation chains");
+    if (!prec) GROW(p, p->d_chain, (size_t)span * 2 * ld * 8, s, "phased rotation chains");
     if (raw) GROW(p, p->d_cb, (size_t)chunk * half * 8, s, "phased spectra");
     int64_t fpw = (chunk + p->max_wg - 1) / p->max_wg;
     fpw = ((fpw + kFrameBatch - 1) / kFrameBatch) * kFrameBatch;
@@ -1224,16 +1226,20 @@ static fxc_status enqueue_phased(fxc_plan *p, const SectorArgs &base, int64_t nf
         a.frames_per_wg = fpw;
         a.part = (double *)p->part.p;
         a.cb_out = raw ? (float2 *)p->d_cb.p : nullptr;
-        if (span0 < 0 || c0 >= span0 + span) {
-            SectorArgs ca = a;
-            ca.nf = std::min(span, nf - c0);
-            ca.chain_ld = ld;
-            HIP_TRY(launch_phase_chain(ca, CHAIN_PHASED, (float2 *)p->d_chain.p, s));
-            span0 = c0;
+        if (!prec) {
+            if (span0 < 0 || c0 >= span0 + span) {
+                SectorArgs ca = a;
+                ca.nf = std::min(span, nf - c0);
+                ca.chain_ld = ld;
+                HIP_TRY(launch_phase_chain(ca, CHAIN_PHASED, (float2 *)p->d_chain.p, s));
+                span0 = c0;
+            }
+            a.chain = (const float2 *)p->d_chain.p + (size_t)(c0 - span0) * 2 * ld;   // rows 2f + station
+            a.chain_ld = ld;
+        } else {
+            a.chain = nullptr;
         }
-        a.chain = (const float2 *)p->d_chain.p + (size_t)(c0 - span0) * 2 * ld;   // rows 2f + station
-        a.chain_ld = ld;
-        HIP_TRY(launch_phased(a, c0 > 0 ? 1 : 0, (int)((cn + fpw - 1) / fpw), s));
+        HIP_TRY(launch_phased(a, prec, c0 > 0 ? 1 : 0, (int)((cn + fpw - 1) / fpw), s));
         if (raw) {
             HIP_TRY(launch_irfft_quantise(a, (const float2 *)p->d_cb.p, cn, d_raw_out + c0 * bpf_o,
                                           p->desc.output_lsb != 0, s));
@@ -1257,10 +1263,11 @@ static fxc_status enqueue_phased_large(fxc_plan *p, const SectorArgs &base, int6
     const float2 *twl1 = (const float2 *)(p->M1 == 128 ? p->tw_l1.p : p->tw_l2.p);
     const float2 *twl2 = (const float2 *)(p->M2 == 128 ? p->tw_l1.p : p->tw_l2.p);
     const float2 *twm = (const float2 *)p->tw_lm.p;
+    const int prec = p->desc.precision == FXC_PRECISION_CHOSEN ? 1 : 0;
     const int ld = chain_ld_for((int)half);
     const int64_t chunk = p->chunk_use;
     const int64_t span = chain_span(nf, chunk, 2 * (size_t)ld * 8, (int)half);
-    GROW(p, p->d_chain, (size_t)span * 2 * ld * 8, s, "phased rotation chains");
+    if (!prec) GROW(p, p->d_chain, (size_t)span * 2 * ld * 8, s, "phased rotation chains");
     if (raw) GROW(p, p->d_cb, (size_t)chunk * half * 8, s, "phased spectra");
     int64_t span0 = -1;    // first frame (of this sector) of the computed chains
     for (int64_t c0 = 0; c0 < nf; c0 += chunk) {
@@ -1273,23 +1280,27 @@ static fxc_status enqueue_phased_large(fxc_plan *p, const SectorArgs &base, int6
         a.part = (double *)p->part.p;
         a.cb_out = raw ? (float2 *)p->d_cb.p : nullptr;
         int64_t *shifts = (int64_t *)p->d_lshifts.p;
-        HIP_TRY(launch_phased_params(a, p->d_params.p, shifts, s));
+        HIP_TRY(launch_phased_params(a, prec, p->d_params.p, shifts, s));
         for (int st = 0; st < 2; ++st) {
             HIP_TRY(launch_decode_frames(a, st, shifts + st, 2, p->d_x[st].p, false, s));
             HIP_TRY(launch_fft_large(p->d_x[st].p, false, (float2 *)p->d_y.p,
                                      (float2 *)p->d_z[st].p, cn, p->M1, p->M2, twl1, twl2, twm,
                                      nullptr, s));
         }
-        if (span0 < 0 || c0 >= span0 + span) {
-            SectorArgs ca = a;
-            ca.nf = std::min(span, nf - c0);
-            ca.chain_ld = ld;
-            HIP_TRY(launch_phase_chain(ca, CHAIN_PHASED, (float2 *)p->d_chain.p, s));
-            span0 = c0;
+        if (!prec) {
+            if (span0 < 0 || c0 >= span0 + span) {
+                SectorArgs ca = a;
+                ca.nf = std::min(span, nf - c0);
+                ca.chain_ld = ld;
+                HIP_TRY(launch_phase_chain(ca, CHAIN_PHASED, (float2 *)p->d_chain.p, s));
+                span0 = c0;
+            }
+            a.chain = (const float2 *)p->d_chain.p + (size_t)(c0 - span0) * 2 * ld;   // rows 2f + station
+            a.chain_ld = ld;
+        } else {
+            a.chain = nullptr;
         }
-        a.chain = (const float2 *)p->d_chain.p + (size_t)(c0 - span0) * 2 * ld;   // rows 2f + station
-        a.chain_ld = ld;
-        HIP_TRY(launch_phased_large(a, (const float2 *)p->d_z[0].p, (const float2 *)p->d_z[1].p,
+        HIP_TRY(launch_phased_large(a, prec, (const float2 *)p->d_z[0].p, (const float2 *)p->d_z[1].p,
                                     p->d_params.p, kLargeGroups, c0 > 0 ? 1 : 0, s));
         if (raw) {
             // d_x[0]: conj(Z'), d_z[0]: its FFT, quantised straight from there

@@ -155,7 +155,9 @@ hipError_t launch_sector_mfma(const SectorArgs &a, int precision, int n_wg, hipS
 int mfma_wg_threads();
 
 // fxc_phased.hip
-hipError_t launch_phased(const SectorArgs &a, int accumulate, int n_wg, hipStream_t stream);
+// precision: 0 plain (reference-faithful f32 chains), 1 chosen (FP64 phases, no chains)
+hipError_t launch_phased(const SectorArgs &a, int precision, int accumulate, int n_wg,
+                         hipStream_t stream);
 hipError_t launch_fill_zero(void *p, size_t bytes, hipStream_t stream);
 hipError_t launch_add_f64(double *dst, const double *src, size_t n, hipStream_t stream);
 hipError_t launch_scale_add_xcf(double *ac, const double *re, const double *im, int half, double s,
@@ -189,9 +191,11 @@ hipError_t launch_irfft_large(const float2 *X, int64_t nframes, int M1, int M2, 
 // phased-array sum at N >= 32768 (fxc_phased.hip); pp: phased_param_bytes()
 // per frame of the chunk
 size_t phased_param_bytes();
-hipError_t launch_phased_params(const SectorArgs &a, void *pp, int64_t *shifts, hipStream_t stream);
-hipError_t launch_phased_large(const SectorArgs &a, const float2 *z1, const float2 *z2,
-                               const void *pp, int n_groups, int accumulate, hipStream_t stream);
+hipError_t launch_phased_params(const SectorArgs &a, int precision, void *pp, int64_t *shifts,
+                                hipStream_t stream);
+hipError_t launch_phased_large(const SectorArgs &a, int precision, const float2 *z1,
+                               const float2 *z2, const void *pp, int n_groups, int accumulate,
+                               hipStream_t stream);
 hipError_t launch_untangle_large(const float2 *z, int64_t nframes, int M, const float2 *tw_post,
                                  float2 *out, hipStream_t stream);
 hipError_t launch_mac_large(const SectorArgs &a, int precision, const float2 *z1, const float2 *z2,

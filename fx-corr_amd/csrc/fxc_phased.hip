@@ -1,41 +1,66 @@
 // Phased-array sum of the two stations (yi-phasedarray; main.rs:2222-2302,
 // 2391-2411): per frame, decode + integer shift + real FFT of both stations
 // (the fused sector kernel's front end), each station's fractional-delay
-// rotation (utils.rs:69-124 with rate = accel = t = 0, USB sign; the f32
-// rotation chain itself comes from k_phase_chain, bit for bit), fringe
+// rotation (utils.rs:69-124 with rate = accel = t = 0, USB sign), fringe
 // phasors, band alignment onto the output grid, cb = w1 s1 + w2 s2, |cb|^2
 // accumulated in FP64 and, for the .raw output, cb with DC/Nyquist imaginary
 // parts zeroed written out for the inverse FFT + quantise kernels.
+// PREC 0 (plain, reference-faithful): the f32 rotation chain itself comes
+// from k_phase_chain, bit for bit, and the fringe phasors are the
+// reference's f32 values.  PREC 1 (chosen): bin b's rotation exp(2 pi i b th)
+// (th = -df frac, turns per bin) from the FP64 phase, reduced mod 1, with
+// f32 sincos for a thread's first bin and FP32 steps after it, and fringe
+// phasors from the FP64 phase -lo tau in turns; no chain table.
 #include "fxc_dev.h"
 
 namespace fxc {
 
 struct PhasedParam {
     int32_t s1, s2;        // integer shifts clamped to [-N, N]
-    float2 fr1, fr2;       // fringe phasors (main.rs:235-240, f32)
+    float2 fr1, fr2;       // fringe phasors (plain: main.rs:235-240, f32; chosen: FP64 phase)
+    double th1, th2;       // chosen: rotation per bin, turns (-df frac)
 };
 
+template <int PREC>
 __device__ void make_phased_param(int64_t idx, const SectorArgs &a, PhasedParam &p) {
     FrameDelayDev d;
     compute_frame_delay(idx, a.dly, d, nullptr, nullptr);
     p.s1 = clamp_shift(d.int1, a.N);
     p.s2 = clamp_shift(d.int2, a.N);
-    p.fr1 = make_float2(d.fr1[0], d.fr1[1]);
-    p.fr2 = make_float2(d.fr2[0], d.fr2[1]);
+    if constexpr (PREC == 0) {
+        p.fr1 = make_float2(d.fr1[0], d.fr1[1]);
+        p.fr2 = make_float2(d.fr2[0], d.fr2[1]);
+        p.th1 = p.th2 = 0.0;
+    } else {
+        double tau1, tau2, tf1, tf2;
+        frame_taus(idx, a.dly, tau1, tau2, tf1, tf2);
+        const float w1 = a.dly.lo1 == 0.0 ? 0.0f : (float)wrap_turns(-a.dly.lo1 * tf1);
+        const float w2 = a.dly.lo2 == 0.0 ? 0.0f : (float)wrap_turns(-a.dly.lo2 * tf2);
+        p.fr1 = make_float2(__builtin_amdgcn_cosf(w1), __builtin_amdgcn_sinf(w1));
+        p.fr2 = make_float2(__builtin_amdgcn_cosf(w2), __builtin_amdgcn_sinf(w2));
+        p.th1 = -a.df * d.frac1;
+        p.th2 = -a.df * d.frac2;
+    }
+}
+
+// exp(2 pi i x) for x in turns (reduced mod 1 in FP64, f32 sincos)
+__device__ __forceinline__ float2 phasor_turns(double x) {
+    const float r = (float)wrap_turns(x);
+    return make_float2(__builtin_amdgcn_cosf(r), __builtin_amdgcn_sinf(r));
 }
 
 __device__ __forceinline__ float2 c_scale(float2 a, float w) { return make_float2(a.x * w, a.y * w); }
 
 // accumulate = 0: the workgroup's partial slot is set; 1: added to (the
 // sector is processed in chunks that reuse the same slots in order).
-template <int E, int T>
+template <int E, int T, int PREC>
 __global__ __launch_bounds__(2 * T) void k_phased(SectorArgs a, int accumulate) {
     constexpr int M = E * T;
     constexpr int MP = M + M / 16 + 2;
     constexpr int NT = 2 * T;
     constexpr int NB = (M + NT) / NT;
     // chain prefetch: fits in registers (the one-wave N = 256 form spills)
-    constexpr bool PF = NB <= 5 && T >= 64;
+    constexpr bool PF = PREC == 0 && NB <= 5 && T >= 64;
     __shared__ float2 zbuf[2][MP];
     __shared__ PhasedParam pp[kFrameBatch];
     __shared__ float lev[2][256];
@@ -71,7 +96,7 @@ __global__ __launch_bounds__(2 * T) void k_phased(SectorArgs a, int accumulate) 
 
     for (int64_t fb = f_begin; fb < f_end; fb += kFrameBatch) {
         const int nb = (int)((f_end - fb) < kFrameBatch ? (f_end - fb) : kFrameBatch);
-        if (tid < nb) make_phased_param(a.first_frame + fb + tid, a, pp[tid]);
+        if (tid < nb) make_phased_param<PREC>(a.first_frame + fb + tid, a, pp[tid]);
         __syncthreads();
         for (int jf = 0; jf < nb; ++jf) {
             const int64_t f = fb + jf;
@@ -79,8 +104,8 @@ __global__ __launch_bounds__(2 * T) void k_phased(SectorArgs a, int accumulate) 
             // rotation of bin b: the reference's own f32 chain r_b
             // (k_phase_chain, rows 2f + station of this chunk).  Loaded
             // before the decode and FFT so their latency overlaps them.
-            const float2 *chg = a.chain + (2 * f + gsn) * a.chain_ld;
-            const float2 *cho = a.chain + (2 * f + 1 - gsn) * a.chain_ld;
+            const float2 *chg = PREC ? nullptr : a.chain + (2 * f + gsn) * a.chain_ld;
+            const float2 *cho = PREC ? nullptr : a.chain + (2 * f + 1 - gsn) * a.chain_ld;
             float2 rg[PF ? NB : 1], ro[PF ? NB : 1];
             if constexpr (PF) {
 #pragma unroll
@@ -102,17 +127,32 @@ __global__ __launch_bounds__(2 * T) void k_phased(SectorArgs a, int accumulate) 
             const PhasedParam &P = pp[jf];
             const float2 frg = gsn == 0 ? P.fr1 : P.fr2;
             const float2 fro = gsn == 0 ? P.fr2 : P.fr1;
+            // chosen: the thread's rotations, bin tid + NT i = first * step^i
+            float2 Rg = make_float2(1.f, 0.f), Ro = Rg, Sg = Rg, So = Rg;
+            if constexpr (PREC != 0) {
+                const double thg = gsn == 0 ? P.th1 : P.th2, tho = gsn == 0 ? P.th2 : P.th1;
+                Rg = phasor_turns((double)tid * thg);
+                Ro = phasor_turns((double)(tid + sb_other) * tho);
+                Sg = phasor_turns((double)NT * thg);
+                So = phasor_turns((double)NT * tho);
+            }
 #pragma unroll
             for (int i = 0; i < NB; ++i) {
                 const int g = tid + NT * i;
+                if (PREC != 0 && i > 0) {
+                    Rg = c_mul(Rg, Sg);
+                    Ro = c_mul(Ro, So);
+                }
                 if (g <= M) {
-                    float2 sg = c_mul(untangle(zg, g, M, a.tw_post), PF ? rg[PF ? i : 0] : chg[g]);
+                    float2 sg = c_mul(untangle(zg, g, M, a.tw_post),
+                                      PREC ? Rg : (PF ? rg[PF ? i : 0] : chg[g]));
                     if (g == M) sg.y = 0.0f;                  // Nyquist (utils.rs:118-123)
                     const float2 cg = c_mul(sg, frg);
                     float2 co = make_float2(0.0f, 0.0f);
                     if (g >= gs && g < ge) {
                         const int j = g + sb_other;
-                        float2 so = c_mul(untangle(zo, j, M, a.tw_post), PF ? ro[PF ? i : 0] : cho[j]);
+                        float2 so = c_mul(untangle(zo, j, M, a.tw_post),
+                                          PREC ? Ro : (PF ? ro[PF ? i : 0] : cho[j]));
                         if (j == M) so.y = 0.0f;
                         co = c_mul(so, fro);
                     }
@@ -154,23 +194,28 @@ __global__ __launch_bounds__(2 * T) void k_phased(SectorArgs a, int accumulate) 
 }
 
 template <int E, int T>
-static hipError_t launch_phased_t(const SectorArgs &a, int accumulate, int n_wg, hipStream_t s) {
-    hipLaunchKernelGGL((k_phased<E, T>), dim3(n_wg), dim3(2 * T), 0, s, a, accumulate);
+static hipError_t launch_phased_t(const SectorArgs &a, int precision, int accumulate, int n_wg,
+                                  hipStream_t s) {
+    if (precision == 0)
+        hipLaunchKernelGGL((k_phased<E, T, 0>), dim3(n_wg), dim3(2 * T), 0, s, a, accumulate);
+    else
+        hipLaunchKernelGGL((k_phased<E, T, 1>), dim3(n_wg), dim3(2 * T), 0, s, a, accumulate);
     return hipGetLastError();
 }
 
-hipError_t launch_phased(const SectorArgs &a, int accumulate, int n_wg, hipStream_t s) {
+hipError_t launch_phased(const SectorArgs &a, int precision, int accumulate, int n_wg,
+                         hipStream_t s) {
     if (n_wg <= 0) return hipSuccess;
     switch (a.N) {
-    case 64: return launch_phased_t<2, 16>(a, accumulate, n_wg, s);
-    case 128: return launch_phased_t<2, 32>(a, accumulate, n_wg, s);
-    case 256: return launch_phased_t<4, 32>(a, accumulate, n_wg, s);
-    case 512: return launch_phased_t<4, 64>(a, accumulate, n_wg, s);
-    case 1024: return launch_phased_t<8, 64>(a, accumulate, n_wg, s);
-    case 2048: return launch_phased_t<8, 128>(a, accumulate, n_wg, s);
-    case 4096: return launch_phased_t<8, 256>(a, accumulate, n_wg, s);
-    case 8192: return launch_phased_t<8, 512>(a, accumulate, n_wg, s);
-    case 16384: return launch_phased_t<16, 512>(a, accumulate, n_wg, s);
+    case 64: return launch_phased_t<2, 16>(a, precision, accumulate, n_wg, s);
+    case 128: return launch_phased_t<2, 32>(a, precision, accumulate, n_wg, s);
+    case 256: return launch_phased_t<4, 32>(a, precision, accumulate, n_wg, s);
+    case 512: return launch_phased_t<4, 64>(a, precision, accumulate, n_wg, s);
+    case 1024: return launch_phased_t<8, 64>(a, precision, accumulate, n_wg, s);
+    case 2048: return launch_phased_t<8, 128>(a, precision, accumulate, n_wg, s);
+    case 4096: return launch_phased_t<8, 256>(a, precision, accumulate, n_wg, s);
+    case 8192: return launch_phased_t<8, 512>(a, precision, accumulate, n_wg, s);
+    case 16384: return launch_phased_t<16, 512>(a, precision, accumulate, n_wg, s);
     default: return hipErrorInvalidValue;
     }
 }
@@ -180,16 +225,18 @@ hipError_t launch_phased(const SectorArgs &a, int accumulate, int n_wg, hipStrea
 // integer shifts), then, after the HBM four-step FFTs of both stations, one
 // output-grid bin per thread over a group of frames (blockIdx.y), with the
 // arithmetic of k_phased's epilogue; partial slot per frame group.
+template <int PREC>
 __global__ void k_phased_params(SectorArgs a, PhasedParam *pp, int64_t *shifts) {
     const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= a.nf) return;
     PhasedParam p;
-    make_phased_param(a.first_frame + f, a, p);
+    make_phased_param<PREC>(a.first_frame + f, a, p);
     pp[f] = p;
     shifts[2 * f] = p.s1;
     shifts[2 * f + 1] = p.s2;
 }
 
+template <int PREC>
 __global__ __launch_bounds__(256) void k_phased_large(SectorArgs a, const float2 *z1, const float2 *z2,
                                                       const PhasedParam *pp, int64_t fpg,
                                                       int accumulate) {
@@ -213,14 +260,20 @@ __global__ __launch_bounds__(256) void k_phased_large(SectorArgs a, const float2
         const PhasedParam P = pp[f];
         const float2 frg = gsn == 0 ? P.fr1 : P.fr2;
         const float2 fro = gsn == 0 ? P.fr2 : P.fr1;
-        const float2 *chg = a.chain + (2 * f + gsn) * a.chain_ld;
-        const float2 *cho = a.chain + (2 * f + 1 - gsn) * a.chain_ld;
-        float2 sg = c_mul(untangle_np(zg + f * M, g, M, a.tw_post), chg[g]);
+        float2 rg, ro = make_float2(1.f, 0.f);
+        if constexpr (PREC == 0) {
+            rg = a.chain[(2 * f + gsn) * a.chain_ld + g];
+            if (in_ov) ro = a.chain[(2 * f + 1 - gsn) * a.chain_ld + j];
+        } else {
+            rg = phasor_turns((double)g * (gsn == 0 ? P.th1 : P.th2));
+            if (in_ov) ro = phasor_turns((double)j * (gsn == 0 ? P.th2 : P.th1));
+        }
+        float2 sg = c_mul(untangle_np(zg + f * M, g, M, a.tw_post), rg);
         if (g == M) sg.y = 0.0f;                          // Nyquist (utils.rs:118-123)
         const float2 cg = c_mul(sg, frg);
         float2 co = make_float2(0.0f, 0.0f);
         if (in_ov) {
-            float2 so = c_mul(untangle_np(zo + f * M, j, M, a.tw_post), cho[j]);
+            float2 so = c_mul(untangle_np(zo + f * M, j, M, a.tw_post), ro);
             if (j == M) so.y = 0.0f;
             co = c_mul(so, fro);
         }
@@ -247,19 +300,29 @@ __global__ __launch_bounds__(256) void k_phased_large(SectorArgs a, const float2
 
 size_t phased_param_bytes() { return sizeof(PhasedParam); }
 
-hipError_t launch_phased_params(const SectorArgs &a, void *pp, int64_t *shifts, hipStream_t s) {
+hipError_t launch_phased_params(const SectorArgs &a, int precision, void *pp, int64_t *shifts,
+                                hipStream_t s) {
     if (a.nf <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_phased_params, dim3((unsigned)((a.nf + 127) / 128)), dim3(128), 0, s, a,
-                       reinterpret_cast<PhasedParam *>(pp), shifts);
+    if (precision == 0)
+        hipLaunchKernelGGL(k_phased_params<0>, dim3((unsigned)((a.nf + 127) / 128)), dim3(128), 0, s,
+                           a, reinterpret_cast<PhasedParam *>(pp), shifts);
+    else
+        hipLaunchKernelGGL(k_phased_params<1>, dim3((unsigned)((a.nf + 127) / 128)), dim3(128), 0, s,
+                           a, reinterpret_cast<PhasedParam *>(pp), shifts);
     return hipGetLastError();
 }
 
-hipError_t launch_phased_large(const SectorArgs &a, const float2 *z1, const float2 *z2,
-                               const void *pp, int n_groups, int accumulate, hipStream_t s) {
+hipError_t launch_phased_large(const SectorArgs &a, int precision, const float2 *z1,
+                               const float2 *z2, const void *pp, int n_groups, int accumulate,
+                               hipStream_t s) {
     const int64_t fpg = (a.nf + n_groups - 1) / n_groups;
     const dim3 grid((unsigned)((a.M + 1 + 255) / 256), (unsigned)n_groups);
-    hipLaunchKernelGGL(k_phased_large, grid, dim3(256), 0, s, a, z1, z2,
-                       reinterpret_cast<const PhasedParam *>(pp), fpg, accumulate);
+    if (precision == 0)
+        hipLaunchKernelGGL(k_phased_large<0>, grid, dim3(256), 0, s, a, z1, z2,
+                           reinterpret_cast<const PhasedParam *>(pp), fpg, accumulate);
+    else
+        hipLaunchKernelGGL(k_phased_large<1>, grid, dim3(256), 0, s, a, z1, z2,
+                           reinterpret_cast<const PhasedParam *>(pp), fpg, accumulate);
     return hipGetLastError();
 }
 

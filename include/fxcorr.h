@@ -58,7 +58,10 @@ enum {
     FXC_PRECISION_PLAIN_FP32 = 0,
     /* FP64 fringe and per-bin fractional-delay phase (reduced mod 2*pi before
      * the cast), FP64 integration; FFT as the plain form unless the plan is
-     * eligible for the FP16-MFMA four-step first stage (see fxc_plan_info). */
+     * eligible for the FP16-MFMA four-step first stage (see fxc_plan_info).
+     * Phased plans: each station's rotation and fringe phasor from FP64
+     * phases instead of the reference's f32 recurrence; the .raw codes can
+     * then differ from the reference's at quantiser decision boundaries. */
     FXC_PRECISION_CHOSEN = 1
 };
 

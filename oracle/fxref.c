@@ -538,6 +538,31 @@ int fxr_irfft_f32(int n, const float *in, float *out) {
     return 0;
 }
 
+/* The same c2r in FP64 (the FP64 oracle's phased .raw branch). */
+int fxr_irfft_f64(int n, const double *in, double *out) {
+    if (n < 4 || !is_pow2(n)) return -1;
+    int m = n / 2;
+    double *z = (double *)malloc(sizeof(double) * 2 * (size_t)m);
+    z[0] = in[0] + in[2 * m];
+    z[1] = in[0] - in[2 * m];
+    for (int k = 1; k < m; ++k) {
+        double ar = in[2 * k], ai = in[2 * k + 1];
+        double br = in[2 * (m - k)], bi = -in[2 * (m - k) + 1];
+        double er = ar + br, ei = ai + bi;
+        double dr = ar - br, di = ai - bi;
+        double ang = 2.0 * FXR_PI * (double)k / (double)n;
+        double c = cos(ang), s = sin(ang);
+        double orr = dr * c - di * s;
+        double oi = dr * s + di * c;
+        z[2 * k] = er - oi;
+        z[2 * k + 1] = ei + orr;
+    }
+    cfft_f64(z, m, +1);
+    for (int i = 0; i < n; ++i) out[i] = z[i] / (double)n;
+    free(z);
+    return 0;
+}
+
 /* utils.rs:93-124 */
 void fxr_apply_delay_and_rate_range(float *spec, size_t len, size_t fft_len,
                                     double df, double delay_s, double rate,
@@ -1006,6 +1031,88 @@ static int frame_phased_f32(sector_ctx *c, int64_t i, float *f1, float *f2, floa
     return 0;
 }
 
+/* The phased branch in FP64 (the chosen form's reference): FP64 FFTs, each
+ * bin's fractional-delay rotation exp(-2 pi i df frac b) and the fringe
+ * phasors evaluated in FP64 (no f32 recurrence), the weighted sum and |cb|^2
+ * in FP64; for .raw output the FP64 inverse FFT, the output LSB flip, and
+ * the quantiser (utils.rs:378-410) on the samples rounded to f32. */
+static int frame_phased_f64(sector_ctx *c, int64_t i, float *f1, float *f2, double *x1,
+                            double *x2, double *s1, double *s2, double *s1c, double *s2a,
+                            double *cb, double *out_t, double *acc) {
+    const fxr_sector_desc *d = c->d;
+    int n = d->fft_len;
+    size_t half = (size_t)n / 2 + 1;
+    uint8_t *out_f = c->raw_out ? c->raw_out + (size_t)i * c->bpf_o : NULL;
+    fxr_frame_delay fd;
+    double fr1[2], fr2[2];
+    fxr_compute_frame_delay_f64(c->first_frame + i, c->cfg, &fd, fr1, fr2);
+    if (fxr_decode_block(c->raw1 + (size_t)i * c->bpf1, c->bpf1, (size_t)n, c->dp1, f1,
+                         (size_t)n, d->lsb1) != 0 ||
+        fxr_decode_block(c->raw2 + (size_t)i * c->bpf2, c->bpf2, (size_t)n, c->dp2, f2,
+                         (size_t)n, d->lsb2) != 0) {
+        if (out_f) memset(out_f, 0, c->bpf_o);
+        return -1;
+    }
+    for (int k = 0; k < n; ++k) {
+        x1[k] = (double)f1[k];
+        x2[k] = (double)f2[k];
+    }
+    fxr_shift_zerofill_f64(x1, (size_t)n, fd.int1);
+    fxr_shift_zerofill_f64(x2, (size_t)n, fd.int2);
+    fxr_rfft_f64(n, x1, s1);
+    fxr_rfft_f64(n, x2, s2);
+    /* utils.rs:93-124 with rate = accel = t = 0, USB sign, from bin 0 */
+    double *sp[2] = {s1, s2};
+    double fr[2] = {fd.frac1, fd.frac2};
+    for (int st = 0; st < 2; ++st) {
+        double step = -2.0 * FXR_PI * d->df_hz * fr[st];
+        for (size_t b = 0; b < half; ++b) {
+            double r[2] = {cos(step * (double)b), sin(step * (double)b)}, v[2];
+            cmul_f64(sp[st] + 2 * b, r, v);
+            sp[st][2 * b] = v[0];
+            sp[st][2 * b + 1] = v[1];
+        }
+        if (n % 2 == 0) sp[st][2 * ((size_t)n / 2) + 1] = 0.0;
+    }
+    size_t ov = (size_t)(d->a1e - d->a1s);
+    double w1 = d->w1, w2 = d->w2;
+    memset(s2a, 0, sizeof(double) * 2 * half);
+    if (d->out_grid == FXR_GRID_ANT1) {
+        for (size_t k = 0; k < ov; ++k)
+            cmul_f64(s2 + 2 * ((size_t)d->a2s + k), fr2, s2a + 2 * ((size_t)d->a1s + k));
+        for (size_t k = 0; k < half; ++k) cmul_f64(s1 + 2 * k, fr1, s1c + 2 * k);
+        for (size_t k = 0; k < half; ++k) {
+            cb[2 * k] = s1c[2 * k] * w1 + s2a[2 * k] * w2;
+            cb[2 * k + 1] = s1c[2 * k + 1] * w1 + s2a[2 * k + 1] * w2;
+        }
+    } else {
+        for (size_t k = 0; k < ov; ++k)
+            cmul_f64(s1 + 2 * ((size_t)d->a1s + k), fr1, s2a + 2 * ((size_t)d->a2s + k));
+        for (size_t k = 0; k < half; ++k) cmul_f64(s2 + 2 * k, fr2, s1c + 2 * k);
+        for (size_t k = 0; k < half; ++k) {
+            cb[2 * k] = s2a[2 * k] * w1 + s1c[2 * k] * w2;
+            cb[2 * k + 1] = s2a[2 * k + 1] * w1 + s1c[2 * k + 1] * w2;
+        }
+    }
+    double *a_ph = acc + P_PH * half;
+    for (size_t k = 0; k < half; ++k) a_ph[k] += cb[2 * k] * cb[2 * k] + cb[2 * k + 1] * cb[2 * k + 1];
+    if (out_f) {
+        cb[1] = 0.0;
+        if (n % 2 == 0) cb[2 * ((size_t)n / 2) + 1] = 0.0;
+        fxr_irfft_f64(n, cb, out_t);
+        if (d->output_lsb)
+            for (int k = 1; k < n; k += 2) out_t[k] = -out_t[k];
+        for (int k = 0; k < n; ++k) f1[k] = (float)out_t[k];
+        long wr = fxr_quantise_frame(f1, (size_t)n, d->bit_out, d->levels1, d->nlevels1,
+                                     d->shuffle1, out_f, c->bpf_o);
+        if (wr != (long)c->bpf_o) {
+            memset(out_f, 0, c->bpf_o);
+            return -1;
+        }
+    }
+    return 0;
+}
+
 static void *sector_worker(void *arg) {
     sector_ctx *c = (sector_ctx *)arg;
     const fxr_sector_desc *d = c->d;
@@ -1023,6 +1130,11 @@ static void *sector_worker(void *arg) {
     double *x2 = (double *)malloc(sizeof(double) * (size_t)n);
     double *d1 = (double *)malloc(sizeof(double) * 2 * half);
     double *d2 = (double *)malloc(sizeof(double) * 2 * half);
+    int f64ph = d->write_raw && c->mode == FXR_MODE_F64;
+    double *p1c = f64ph ? (double *)malloc(sizeof(double) * 2 * half) : NULL;
+    double *p2a = f64ph ? (double *)malloc(sizeof(double) * 2 * half) : NULL;
+    double *pcb = f64ph ? (double *)malloc(sizeof(double) * 2 * half) : NULL;
+    double *pout = f64ph ? (double *)malloc(sizeof(double) * (size_t)n) : NULL;
     for (;;) {
         pthread_mutex_lock(&c->mu);
         int64_t job = c->next_job++;
@@ -1035,7 +1147,9 @@ static void *sector_worker(void *arg) {
         int64_t fails = 0;
         for (int64_t i = start; i < end; ++i) {
             int rc;
-            if (d->write_raw)
+            if (d->write_raw && c->mode == FXR_MODE_F64)
+                rc = frame_phased_f64(c, i, f1, f2, x1, x2, d1, d2, p1c, p2a, pcb, pout, acc);
+            else if (d->write_raw)
                 rc = frame_phased_f32(c, i, f1, f2, s1, s2, s1c, s2a, cb, out_t, acc);
             else if (c->mode == FXR_MODE_F64)
                 rc = frame_acc_f64(c, i, f1, f2, x1, x2, d1, d2, acc);
@@ -1047,6 +1161,7 @@ static void *sector_worker(void *arg) {
     }
     free(f1); free(f2); free(s1); free(s2); free(s1c); free(s2a); free(cb); free(out_t);
     free(x1); free(x2); free(d1); free(d2);
+    free(p1c); free(p2a); free(pcb); free(pout);
     return NULL;
 }
 

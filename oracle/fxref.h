@@ -12,7 +12,9 @@
  *                  (built with -ffp-contract=off, as Rust never contracts),
  *                  f32 phase recurrences, f64 accumulators.
  *   FXR_MODE_F64 — FP64 oracle: exact decode, FP64 FFT, per-bin FP64 phases,
- *                  FP64 fringe phasor, FP64 products.
+ *                  FP64 fringe phasor, FP64 products; in the phased branch
+ *                  FP64 rotations and weighted sum, FP64 inverse FFT, the
+ *                  quantiser on the samples rounded to f32.
  *
  * The reference's FFT is the third-party crate realfft 3.5.0 over rustfft 6.4.0
  * (not vendored, not buildable here).  fxr_rfft_f32 restates realfft's
@@ -29,6 +31,7 @@
  *   band alignment                             pinned (closed forms)
  *   xcf phase start/step, delay rotation       pinned (closed forms)
  *   sector MAC                                 pinned (invariants, numpy re-run)
+ *   phased branch (both modes)                 pinned (closed forms, numpy re-run)
  *   quantise                                   pinned (round trip, nearest level)
  */
 #ifndef FXREF_H
@@ -74,6 +77,7 @@ int fxr_rfft_f32(int n, const float *in, float *out);
 int fxr_rfft_f64(int n, const double *in, double *out);
 /* utils.rs:60-66 (realfft c2r, then *= 1/n in f32).  in: (n/2+1) complex. */
 int fxr_irfft_f32(int n, const float *in, float *out);
+int fxr_irfft_f64(int n, const double *in, double *out);
 
 /* utils.rs:93-124 */
 void fxr_apply_delay_and_rate_range(float *spec, size_t len, size_t fft_len,

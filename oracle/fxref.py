@@ -116,6 +116,8 @@ def _declare(L):
         _p(C.c_float), C.c_size_t, C.c_size_t, C.c_double, C.c_double, C.c_double,
         C.c_double, C.c_double, C.c_int, C.c_size_t, C.c_size_t]
     L.fxr_compute_frame_delay.argtypes = [C.c_int64, _p(DelayCfg), _p(FrameDelay)]
+    L.fxr_compute_frame_delay_f64.argtypes = [C.c_int64, _p(DelayCfg), _p(FrameDelay),
+                                              _p(C.c_double), _p(C.c_double)]
     L.fxr_split_delay.argtypes = [C.c_double, C.c_double, _p(C.c_int64), _p(C.c_double)]
     L.fxr_carrier_phase.argtypes = [C.c_double, C.c_double, _p(C.c_float)]
     L.fxr_xcf_phase_start_step.argtypes = [C.c_double, C.c_double, C.c_double, C.c_size_t,
@@ -293,6 +295,14 @@ def frame_delay(idx: int, cfg: DelayCfg) -> FrameDelay:
     out = FrameDelay()
     lib().fxr_compute_frame_delay(idx, C.byref(cfg), C.byref(out))
     return out
+
+
+def frame_delay_f64(idx: int, cfg: DelayCfg):
+    """frame_delay plus the fringe phasors in FP64: (FrameDelay, fr1, fr2)."""
+    out = FrameDelay()
+    a, b = (C.c_double * 2)(), (C.c_double * 2)()
+    lib().fxr_compute_frame_delay_f64(idx, C.byref(cfg), C.byref(out), a, b)
+    return out, complex(a[0], a[1]), complex(b[0], b[1])
 
 
 def band_alignment(fft, fs, c1_mhz, c2_mhz, bw1_mhz, bw2_mhz):

@@ -611,7 +611,7 @@ def _phased_desc(c, w, bit_out, output_lsb):
                 output_lsb=int(output_lsb), cpu_threads=8)
 
 
-@pytest.mark.parametrize("n,bits,grid,band_shift,lsb_out", [
+PHASED_CASES = [
     (1024, (2, 2), 0, 0, False),
     (512, (4, 2), 1, 8, True),
     (2048, (2, 1), 0, -16, False),
@@ -619,12 +619,10 @@ def _phased_desc(c, w, bit_out, output_lsb):
     (16384, (1, 8), 1, -32, False),
     (32768, (4, 1), 0, 2048, True),
     (65536, (2, 2), 1, -64, False),
-])
-def test_phased_vs_oracle(n, bits, grid, band_shift, lsb_out):
-    """yi-phasedarray sector: |w1 s1 + w2 s2|^2 within the plain bar of the
-    oracle's phased branch at every N (each station's rotation is the
-    reference's own f32 chain over all N/2+1 bins); the .raw output decodes
-    to the same levels except at rare quantiser decision boundaries."""
+]
+
+
+def _phased_case(n, bits, grid, band_shift, lsb_out, precision):
     fs = 128e6
     half = n // 2                                  # main.rs:360-372 with v1 = v2 = N/2
     sb = band_shift
@@ -640,13 +638,17 @@ def test_phased_vs_oracle(n, bits, grid, band_shift, lsb_out):
     bit_out = max(bits)
     p = F.Plan(n, bits=bits, levels=c["levels"], shuffles=c["shuffles"], lsb=c["lsb"],
                band=band, df_hz=fs / n, products=F.PRODUCT_PHASED | F.PRODUCT_RAW_OUT,
-               out_grid=grid, w=w, bit_out=bit_out, output_lsb=lsb_out)
+               out_grid=grid, w=w, bit_out=bit_out, output_lsb=lsb_out, precision=precision)
     cfg = F.make_delay_cfg(c["table"], **c["delay"])
     got = p.sector_run(c["raw1"], c["raw2"], 0, c["nf"], cfg)
+    mode, tol = ((R.MODE_F32, PLAIN_TOL) if precision == F.PRECISION_PLAIN_FP32
+                 else (R.MODE_F64, CHOSEN_TOL))
     ref = R.sector_run(_phased_desc(c, w, bit_out, lsb_out), c["raw1"], c["raw2"], 0, c["nf"],
-                       R.make_delay_cfg(c["table"], **c["delay"]), mode=R.MODE_F32, threads=8)
+                       R.make_delay_cfg(c["table"], **c["delay"]), mode=mode, threads=8)
     assert got["n_failed"] == 0 and ref["n_failed"] == 0
-    assert rel_err(got["acc_ph"], ref["acc_ph"]) < PLAIN_TOL
+    e = rel_err(got["acc_ph"], ref["acc_ph"])
+    print(f"[phased] N={n} precision {precision}: acc_ph {e:.2e}", flush=True)
+    assert e < tol
     for k in ("acc11", "acc22"):
         assert np.all(got[k] == 0)
     assert np.all(got["acc12"] == 0)
@@ -660,6 +662,24 @@ def test_phased_vs_oracle(n, bits, grid, band_shift, lsb_out):
         _, vr = dp.decode(ref["raw_out"][seg], n, False)
         diff += int(np.count_nonzero(vg != vr))
     assert diff <= 1e-4 * n * len(range(0, c["nf"], 7)) + 2, diff
+
+
+@pytest.mark.parametrize("n,bits,grid,band_shift,lsb_out", PHASED_CASES)
+def test_phased_vs_oracle(n, bits, grid, band_shift, lsb_out):
+    """yi-phasedarray sector: |w1 s1 + w2 s2|^2 within the plain bar of the
+    oracle's phased branch at every N (each station's rotation is the
+    reference's own f32 chain over all N/2+1 bins); the .raw output decodes
+    to the same levels except at rare quantiser decision boundaries."""
+    _phased_case(n, bits, grid, band_shift, lsb_out, F.PRECISION_PLAIN_FP32)
+
+
+@pytest.mark.parametrize("n,bits,grid,band_shift,lsb_out", PHASED_CASES)
+def test_phased_chosen_vs_fp64_oracle(n, bits, grid, band_shift, lsb_out):
+    """The chosen phased form (rotations and fringe phasors from FP64
+    phases, no f32 chain table) against the oracle's FP64 phased branch:
+    |cb|^2 within the chosen bar, .raw codes equal except at rare quantiser
+    decision boundaries."""
+    _phased_case(n, bits, grid, band_shift, lsb_out, F.PRECISION_CHOSEN)
 
 
 @pytest.mark.parametrize("nf,chunk", [(5, 0), (23, 7)])
