@@ -368,8 +368,9 @@ def phased_sweep(stream, samples_per_station=1 << 28, reps=3):
     """Phased-array path (yi-phasedarray): two 2-bit stations at 1024 Msps
     summed into the phased power spectrum, without and with the 2-bit .raw
     output, at FFT sizes 1024, 4096, 16384 (fused LDS kernel) and 65536 (HBM
-    four-step); device-resident input, mean of `reps` sector runs after one
-    warm-up."""
+    four-step), in both forms (plain: the reference's f32 rotation chains;
+    chosen: FP64 rotation phases and the FP16-MFMA first FFT pass);
+    device-resident input, mean of `reps` sector runs after one warm-up."""
     rows = {}
     fs = 1024e6
     for n in [1024, 4096, 16384, 65536]:
@@ -382,9 +383,13 @@ def phased_sweep(stream, samples_per_station=1 << 28, reps=3):
         cfg = F.make_delay_cfg(np.zeros(2 + int(nf * n / fs)), frame_dt=n / fs, fs=fs,
                                delay_user_samples=2.3, lo1_hz=OBS_HZ, lo2_hz=OBS_HZ)
         row = {"frames": nf}
-        for name, raw in (("power", False), ("power_and_raw", True)):
+        for name, raw, prec in (("power", False, F.PRECISION_PLAIN_FP32),
+                                ("power_and_raw", True, F.PRECISION_PLAIN_FP32),
+                                ("chosen_power", False, F.PRECISION_CHOSEN),
+                                ("chosen_power_and_raw", True, F.PRECISION_CHOSEN)):
             prods = F.PRODUCT_PHASED | (F.PRODUCT_RAW_OUT if raw else 0)
-            p = F.Plan(n, levels=[LEVELS, LEVELS], df_hz=fs / n, products=prods, bit_out=2)
+            p = F.Plan(n, levels=[LEVELS, LEVELS], df_hz=fs / n, products=prods, bit_out=2,
+                       precision=prec)
             acc = F.DeviceBuffer(p.half * 5 * 8)
             out = F.DeviceBuffer(nf * p.info.bpf_out) if raw else None
             optr = out.ptr if raw else None
@@ -404,7 +409,8 @@ def phased_sweep(stream, samples_per_station=1 << 28, reps=3):
         d2.free()
         rows[str(n)] = row
         print(f"[bench] phased FFT {n:6d}: power {row['power']:,.0f}, power + .raw "
-              f"{row['power_and_raw']:,.0f} Msamples/s/station", file=sys.stderr, flush=True)
+              f"{row['power_and_raw']:,.0f}; chosen {row['chosen_power']:,.0f} / "
+              f"{row['chosen_power_and_raw']:,.0f} Msamples/s/station", file=sys.stderr, flush=True)
     return {"workload": f"two 2-bit stations at 1024 Msps, weights 1/1, 2-bit .raw output, "
                         f"{samples_per_station} samples per station per size, input resident "
                         f"in HBM, mean of {reps} sector runs", "unit": "Msamples/s/station",

@@ -581,7 +581,6 @@ static bool mfma_eligible(const fxc_plan *p) {
 // (any band alignment).
 static bool first_pass_mfma_eligible(const fxc_plan *p) {
     if (p->desc.precision != FXC_PRECISION_CHOSEN) return false;
-    if (p->desc.products & (FXC_PRODUCT_PHASED | FXC_PRODUCT_RAW_OUT)) return false;
     if (!(p->fft_form == 2 || (p->fft_form == 0 && p->N >= 1024 && p->N <= 16384))) return false;
     return levels_fp16_exact(p);
 }
@@ -1281,11 +1280,13 @@ static fxc_status enqueue_phased_large(fxc_plan *p, const SectorArgs &base, int6
         a.cb_out = raw ? (float2 *)p->d_cb.p : nullptr;
         int64_t *shifts = (int64_t *)p->d_lshifts.p;
         HIP_TRY(launch_phased_params(a, prec, p->d_params.p, shifts, s));
+        // chosen: FP16 decode and the MFMA first pass, as the four-step through HBM
+        const bool mf = prec && p->mf_b1;
         for (int st = 0; st < 2; ++st) {
-            HIP_TRY(launch_decode_frames(a, st, shifts + st, 2, p->d_x[st].p, false, s));
-            HIP_TRY(launch_fft_large(p->d_x[st].p, false, (float2 *)p->d_y.p,
+            HIP_TRY(launch_decode_frames(a, st, shifts + st, 2, p->d_x[st].p, mf, s));
+            HIP_TRY(launch_fft_large(p->d_x[st].p, mf, (float2 *)p->d_y.p,
                                      (float2 *)p->d_z[st].p, cn, p->M1, p->M2, twl1, twl2, twm,
-                                     nullptr, s));
+                                     mf ? p->mf_tab.p : nullptr, s));
         }
         if (!prec) {
             if (span0 < 0 || c0 >= span0 + span) {
@@ -1328,6 +1329,7 @@ static fxc_status enqueue_phased_large(fxc_plan *p, const SectorArgs &base, int6
 static bool large_fused_possible(const fxc_plan *p) {
     const int64_t sb = p->desc.a2s - p->desc.a1s;
     return p->fft_form == 2 && p->desc.precision == FXC_PRECISION_CHOSEN && p->mf_b1 &&
+           !(p->desc.products & (FXC_PRODUCT_PHASED | FXC_PRODUCT_RAW_OUT)) &&
            (p->M / 128 == 128 || p->M / 128 == 256) && (sb % 128) == 0 &&
            p->st[0].bits <= 16 && p->st[1].bits <= 16;
 }

@@ -53,7 +53,9 @@ __device__ __forceinline__ float2 c_scale(float2 a, float w) { return make_float
 
 // accumulate = 0: the workgroup's partial slot is set; 1: added to (the
 // sector is processed in chunks that reuse the same slots in order).
-template <int E, int T, int PREC>
+// MF (chosen form, levels exact in FP16): the FFT's first pass (radix 32)
+// as FP16 hi/lo MFMA GEMMs with the operand table in LDS, as k_sector's.
+template <int E, int T, int PREC, bool MF = false>
 __global__ __launch_bounds__(2 * T) void k_phased(SectorArgs a, int accumulate) {
     constexpr int M = E * T;
     constexpr int MP = M + M / 16 + 2;
@@ -64,6 +66,7 @@ __global__ __launch_bounds__(2 * T) void k_phased(SectorArgs a, int accumulate) 
     __shared__ float2 zbuf[2][MP];
     __shared__ PhasedParam pp[kFrameBatch];
     __shared__ float lev[2][256];
+    __shared__ uint4 b1s[MF ? 16 * 64 : 1];
 
     const int tid0 = threadIdx.x;
     const int s = half_index<T>(tid0);
@@ -77,6 +80,8 @@ __global__ __launch_bounds__(2 * T) void k_phased(SectorArgs a, int accumulate) 
     const uint8_t *raw = s ? a.raw2 : a.raw1;
     if (st.bits > 2 && st.bits <= 8)
         for (int i = t; i < 256; i += T) lev[s][i] = (i <= (int)st.code_mask) ? st.levels[i] : 0.0f;
+    if constexpr (MF)
+        for (int i = tid0; i < 16 * 64; i += NT) b1s[i] = reinterpret_cast<const uint4 *>(a.mf_tab)[i];
 
     double ph[NB];
 #pragma unroll
@@ -119,10 +124,21 @@ __global__ __launch_bounds__(2 * T) void k_phased(SectorArgs a, int accumulate) 
             __syncthreads();
             const int sh = s ? pp[jf].s2 : pp[jf].s1;
             float2 v[E];
+            if constexpr (MF) {
+                // each thread decodes a run of E consecutive elements into LDS
+                decode_run<E>(ws, st, lev[s], t * E, sh, N, v);
+                __syncthreads();
 #pragma unroll
-            for (int r = 0; r < E; ++r) v[r] = decode_elem(ws, st, lev[s], t + T * r, sh, N);
-            __syncthreads();
-            fft_forward<E, T>(zs, v, a.tw_fft, t);
+                for (int r = 0; r < E; ++r) zs[pad(t * E + r)] = v[r];
+                __syncthreads();
+                stockham_first_mfma<E, T>(zs, b1s, t);
+                stockham_rest<E, T, 32>(zs, a.tw_fft, t);
+            } else {
+#pragma unroll
+                for (int r = 0; r < E; ++r) v[r] = decode_elem(ws, st, lev[s], t + T * r, sh, N);
+                __syncthreads();
+                fft_forward<E, T>(zs, v, a.tw_fft, t);
+            }
 
             const PhasedParam &P = pp[jf];
             const float2 frg = gsn == 0 ? P.fr1 : P.fr2;
@@ -196,10 +212,19 @@ __global__ __launch_bounds__(2 * T) void k_phased(SectorArgs a, int accumulate) 
 template <int E, int T>
 static hipError_t launch_phased_t(const SectorArgs &a, int precision, int accumulate, int n_wg,
                                   hipStream_t s) {
-    if (precision == 0)
+    if (precision == 0) {
         hipLaunchKernelGGL((k_phased<E, T, 0>), dim3(n_wg), dim3(2 * T), 0, s, a, accumulate);
-    else
-        hipLaunchKernelGGL((k_phased<E, T, 1>), dim3(n_wg), dim3(2 * T), 0, s, a, accumulate);
+        return hipGetLastError();
+    }
+    // a.mf_tab: the chosen plan's first FFT pass runs on FP16 MFMA (whole
+    // waves of 32-point DFT rows: E >= 8, T a multiple of 64)
+    if constexpr (E >= 8 && T % 64 == 0) {
+        if (a.mf_tab != nullptr) {
+            hipLaunchKernelGGL((k_phased<E, T, 1, true>), dim3(n_wg), dim3(2 * T), 0, s, a, accumulate);
+            return hipGetLastError();
+        }
+    }
+    hipLaunchKernelGGL((k_phased<E, T, 1>), dim3(n_wg), dim3(2 * T), 0, s, a, accumulate);
     return hipGetLastError();
 }
 
