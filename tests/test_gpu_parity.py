@@ -889,6 +889,26 @@ def test_large_fused_form(n, bits, sb, grid, prods, lsb, delay, nf, chunk):
         assert zero_pattern_equal(fused[k], hbm[k]), k
 
 
+@pytest.mark.parametrize("n,bits", [(32768, (3, 5)), (65536, (5, 3))])
+def test_large_fused_odd_widths_and_shuffle(n, bits):
+    """The fused four-step's run-time code width (3 and 5 bits: codes that
+    straddle words, the generic decode instance) with a generic bit shuffle
+    on station 1, against the FP64 oracle and the four-step through HBM."""
+    fs = 256e6
+    rng = np.random.default_rng(n + 7)
+    c = make_case(n=n, fs=fs, nf=20, bits=bits, shuffles=(rng.permutation(32), None),
+                  lsb=(True, False), rho=0.4, delay_samples=-6.4, seed=n + 3,
+                  delay_user_samples=-6.4, extra_delay_rate_sps=1e-9, lo1_hz=6e9, lo2_hz=6e9)
+    o64 = oracle_run(c, R.MODE_F64)
+    p = gpu_plan(c, F.PRECISION_CHOSEN)
+    assert p.info.fft_form == 2 and p.info.variant == 1
+    fused = gpu_run(c, plan=p)
+    _compare(fused, o64, CHOSEN_TOL)
+    q = gpu_plan(c, F.PRECISION_CHOSEN)
+    F.plan_set_tuning(q, variant=0, split_m1=n // 2 // 128, chunk_frames=20)
+    _compare(fused, gpu_run(c, plan=q), CHOSEN_TOL)
+
+
 @pytest.mark.parametrize("n,prec", [(4096, F.PRECISION_PLAIN_FP32), (1024, F.PRECISION_CHOSEN),
                                     (65536, F.PRECISION_CHOSEN)])
 def test_autotune_picks_a_candidate(n, prec):
