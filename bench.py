@@ -421,14 +421,15 @@ def _uniform_levels(bits):
     return np.arange(1 << bits, dtype=np.float64) - ((1 << bits) - 1) / 2.0
 
 
-def batch_bench(stream, scale=0.125, reps=3, seed=48):
+def batch_bench(stream, scale=0.125, reps=3, seed=48, branches=4):
     """Batched uneven scans in one graph: 48 seeded scans, FFT sizes from
     {256, 1024, 4096, 16384}, per-station bits from {1, 2, 4, 8}, sampling
     rates from {128, 256, 512, 1024} Msps, lengths uniform in 0.5..20 s
     scaled by `scale` (device-generated input), each with its own delay
     model and clock terms; ACF+XCF of every scan, and 12 scans also phased
     into an 8-bit series.  Per form, every ACF/XCF plan is autotuned on its
-    own scan, all sector calls are captured into one graph, and the graph is
+    own scan, all sector calls are captured into one graph (as parallel
+    branches on `branches` forked streams), and the graph is
     replayed `reps` times after one warm-up replay; the phased plans run the
     form's own arithmetic (chosen: rotation phases in FP64, no f32 chains)."""
     rng = np.random.default_rng(seed)
@@ -485,19 +486,30 @@ def batch_bench(stream, scale=0.125, reps=3, seed=48):
                            praw=F.DeviceBuffer(sc["nf"] * pp.info.bpf_out))
             jobs.append(job)
 
-        def enqueue_all():
-            for j in jobs:
-                sc = j["sc"]
+        def enqueue_all(streams=(stream,)):
+            for k, j in enumerate(jobs):
+                sc, st = j["sc"], streams[k % len(streams)]
                 j["plan"].sector_run_device(sc["d1"].ptr, sc["d2"].ptr, 0, sc["nf"], sc["cfg"],
-                                            j["acc"].ptr, stream=stream.ptr)
+                                            j["acc"].ptr, stream=st.ptr)
                 if "pplan" in j:
                     j["pplan"].sector_run_device(sc["d1"].ptr, sc["d2"].ptr, 0, sc["nf"], sc["cfg"],
-                                                 j["pacc"].ptr, j["praw"].ptr, stream=stream.ptr)
+                                                 j["pacc"].ptr, j["praw"].ptr, stream=st.ptr)
 
         enqueue_all()                       # eager: delay tables, scratch sizes
         stream.sync()
+        # the graph forks onto `branches` streams: the scans run as parallel
+        # branches, so one scan's launch tail overlaps the next scan's work
         g = F.Graph(stream)
-        enqueue_all()
+        side = [F.Stream() for _ in range(branches)]
+        fork = F.Event()
+        fork.record(stream)
+        for st in side:
+            st.wait(fork)
+        enqueue_all(side)
+        for st in side:
+            ev = F.Event()
+            ev.record(st)
+            stream.wait(ev)
         g.end()
         g.launch()
         stream.sync()
@@ -506,7 +518,8 @@ def batch_bench(stream, scale=0.125, reps=3, seed=48):
             g.launch()
         stream.sync()
         el = (time.perf_counter() - t0) / reps
-        out[form] = {"value": round(total / el / 1e6, 1), "ms_per_batch": round(el * 1e3, 3)}
+        out[form] = {"value": round(total / el / 1e6, 1), "ms_per_batch": round(el * 1e3, 3),
+                     "graph_branches": branches}
         print(f"[bench] batch {form}: {total / el / 1e6:,.0f} Msamples/s/station "
               f"({el * 1e3:.1f} ms per 48-scan graph)", file=sys.stderr, flush=True)
         g.close()

@@ -290,6 +290,11 @@ extern "C" fxc_status fxc_event_record(void *ev, void *stream) {
     HIP_TRY(hipEventRecord((hipEvent_t)ev, (hipStream_t)stream));
     return FXC_OK;
 }
+extern "C" fxc_status fxc_stream_wait_event(void *stream, void *ev) {
+    if (!ev) return fail(FXC_ERR_INVALID_ARG, "ev is null");
+    HIP_TRY(hipStreamWaitEvent((hipStream_t)stream, (hipEvent_t)ev, 0));
+    return FXC_OK;
+}
 extern "C" fxc_status fxc_event_elapsed_ms(void *a, void *b, float *ms) {
     if (!ms) return fail(FXC_ERR_INVALID_ARG, "ms is null");
     HIP_TRY(hipEventSynchronize((hipEvent_t)b));
@@ -881,11 +886,13 @@ extern "C" fxc_status fxc_plan_create(const fxc_plan_desc *desc, fxc_plan **out)
     return FXC_OK;
 }
 
-// Plans whose release waits for graphs, and the plans each capturing stream
-// has enqueued work of (collected by enqueue_sector, handed to the graph by
-// fxc_graph_end).
+// Plans whose release waits for graphs, and the plans each active capture
+// has enqueued work of, keyed by the capture's id so that streams forked
+// into a capture count too (collected by enqueue_sector, handed to the
+// graph by fxc_graph_end).
 static std::mutex g_graph_mu;
-static std::vector<std::pair<hipStream_t, fxc_plan *>> g_captured;
+static std::vector<std::pair<unsigned long long, fxc_plan *>> g_captured;
+static std::vector<std::pair<unsigned long long, hipStream_t>> g_capture_origin;   // id -> begin stream
 
 static void plan_free(fxc_plan *p) {
     if (p->stream) (void)hipStreamSynchronize(p->stream);
@@ -951,10 +958,19 @@ struct fxc_graph {
     std::vector<fxc_plan *> plans;   // plans whose buffers the graph captured
 };
 
-static std::vector<fxc_plan *> take_captured(hipStream_t s) {
+// Id of the capture `s` takes part in (0: none).
+static unsigned long long capture_id(hipStream_t s) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    unsigned long long id = 0;
+    if (s && hipStreamGetCaptureInfo(s, &cs, &id) == hipSuccess && cs == hipStreamCaptureStatusActive)
+        return id;
+    return 0;
+}
+
+static std::vector<fxc_plan *> take_captured(unsigned long long id) {
     std::vector<fxc_plan *> out;
     for (auto it = g_captured.begin(); it != g_captured.end();) {
-        if (it->first == s) {
+        if (it->first == id) {
             if (std::find(out.begin(), out.end(), it->second) == out.end()) out.push_back(it->second);
             it = g_captured.erase(it);
         } else {
@@ -965,19 +981,30 @@ static std::vector<fxc_plan *> take_captured(hipStream_t s) {
 }
 
 static void note_captured(fxc_plan *p, hipStream_t s) {
+    const unsigned long long id = capture_id(s);
     std::lock_guard<std::mutex> lk(g_graph_mu);
     for (auto &e : g_captured)
-        if (e.first == s && e.second == p) return;
-    g_captured.emplace_back(s, p);
+        if (e.first == id && e.second == p) return;
+    g_captured.emplace_back(id, p);
 }
 
 extern "C" fxc_status fxc_graph_begin(void *stream) {
     if (!stream) return fail(FXC_ERR_INVALID_ARG, "graph capture needs a non-null stream");
     {
         std::lock_guard<std::mutex> lk(g_graph_mu);
-        (void)take_captured((hipStream_t)stream);   // left over from an abandoned capture
+        // left over from an abandoned capture begun on this stream
+        for (auto it = g_capture_origin.begin(); it != g_capture_origin.end();) {
+            if (it->second == (hipStream_t)stream) {
+                (void)take_captured(it->first);
+                it = g_capture_origin.erase(it);
+            } else {
+                ++it;
+            }
+        }
     }
     HIP_TRY(hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal));
+    std::lock_guard<std::mutex> lk(g_graph_mu);
+    g_capture_origin.emplace_back(capture_id((hipStream_t)stream), (hipStream_t)stream);
     return FXC_OK;
 }
 
@@ -986,8 +1013,11 @@ extern "C" fxc_status fxc_graph_end(void *stream, fxc_graph **out) {
     *out = nullptr;
     std::vector<fxc_plan *> plans;
     {
+        const unsigned long long id = capture_id((hipStream_t)stream);
         std::lock_guard<std::mutex> lk(g_graph_mu);
-        plans = take_captured((hipStream_t)stream);
+        plans = take_captured(id);
+        for (auto it = g_capture_origin.begin(); it != g_capture_origin.end();)
+            it = it->first == id ? g_capture_origin.erase(it) : it + 1;
     }
     hipGraph_t g = nullptr;
     HIP_TRY(hipStreamEndCapture((hipStream_t)stream, &g));

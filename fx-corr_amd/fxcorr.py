@@ -27,7 +27,8 @@ EXPORTED = [
     "fxc_dev_free", "fxc_memcpy_h2d", "fxc_memcpy_d2h", "fxc_memset_d", "fxc_memcpy_d2d",
     "fxc_stream_create",
     "fxc_stream_destroy", "fxc_stream_sync", "fxc_event_create", "fxc_event_destroy",
-    "fxc_event_record", "fxc_event_elapsed_ms", "fxc_parse_shuffle", "fxc_parse_levels",
+    "fxc_event_record", "fxc_event_elapsed_ms", "fxc_stream_wait_event", "fxc_parse_shuffle",
+    "fxc_parse_levels",
     "fxc_quantization_mean_power", "fxc_compute_band_alignment", "fxc_sector_partition",
     "fxc_split_delay", "fxc_plan_create", "fxc_plan_destroy", "fxc_plan_get_info",
     "fxc_sector_run", "fxc_sector_run_device", "fxc_sector_run_device_noalloc",
@@ -140,6 +141,7 @@ def lib():
             "fxc_stream_sync": ([vp], i32), "fxc_event_create": ([_P(vp)], i32),
             "fxc_event_destroy": ([vp], i32), "fxc_event_record": ([vp, vp], i32),
             "fxc_event_elapsed_ms": ([vp, vp, _P(C.c_float)], i32),
+            "fxc_stream_wait_event": ([vp, vp], i32),
             "fxc_parse_shuffle": ([C.c_char_p, _P(i32)], i32),
             "fxc_parse_levels": ([i32, C.c_char_p, _P(d), i32, _P(i32)], i32),
             "fxc_quantization_mean_power": ([_P(d), i32, _P(d)], i32),
@@ -447,6 +449,10 @@ class Stream:
 
     def sync(self):
         check(lib().fxc_stream_sync(C.c_void_p(self.ptr)))
+
+    def wait(self, event: "Event"):
+        """Later work on this stream waits for the event (fxc_stream_wait_event)."""
+        check(lib().fxc_stream_wait_event(C.c_void_p(self.ptr), C.c_void_p(event.ptr)))
 
     def destroy(self):
         if self.ptr:

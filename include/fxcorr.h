@@ -161,6 +161,10 @@ fxc_status fxc_event_create(void **ev);
 fxc_status fxc_event_destroy(void *ev);
 fxc_status fxc_event_record(void *ev, void *stream);
 fxc_status fxc_event_elapsed_ms(void *ev_start, void *ev_end, float *ms);
+/* Work enqueued on `stream` after this call waits for the work before the
+ * event's last record (no reference counterpart).  Inside a graph capture
+ * it forks the capture onto another stream or joins it back. */
+fxc_status fxc_stream_wait_event(void *stream, void *ev);
 
 /* ----------------------------------------------------------------------- */
 /* Host-side helpers of the hot-path boundary                               */
@@ -456,7 +460,12 @@ fxc_status fxc_plan_set_tuning(fxc_plan *plan, const fxc_tune_report *cfg);
  *     launch (size the scratch with the largest call before capturing);
  *   - fxc_plan_destroy returns at once and the plan's memory is released
  *     when the last such graph is destroyed.
- * A call that would grow plan scratch inside a capture fails the same way. */
+ * A call that would grow plan scratch inside a capture fails the same way.
+ * The capture may fork onto other streams and join back through events
+ * (fxc_event_record on the capturing stream, fxc_stream_wait_event on the
+ * other): sector calls on those streams join the graph as parallel
+ * branches, and their plans are held like the capturing stream's.  End the
+ * capture on the stream that began it, after every branch has joined. */
 typedef struct fxc_graph fxc_graph;
 fxc_status fxc_graph_begin(void *stream);
 fxc_status fxc_graph_end(void *stream, fxc_graph **out);

@@ -12,8 +12,9 @@ alternately in the chosen form (FP64 rotation phases) and the plain,
 reference-faithful one.
 
 Every ACF/XCF plan is tuned by fxc_plan_autotune on its own scan, all the
-sector calls are captured on one stream into one graph, and every replay
-must equal the eager results bit for bit.  Each eager result is held to the
+sector calls are captured into one graph (forked onto three streams, so the
+scans are parallel branches), and every replay must equal the eager
+results bit for bit.  Each eager result is held to the
 oracle: ACF/XCF to the FP64 oracle within 1e-4, the phased power to the
 FP64 oracle within 1e-4 (chosen) or the reference-faithful FP32 oracle
 within 1e-5 (plain), and the 8-bit series to the same oracle's levels
@@ -107,14 +108,15 @@ def test_48_uneven_scans_one_graph():
                        praw=F.DeviceBuffer(c["nf"] * pp.info.bpf_out))
         jobs.append(job)
 
-    def enqueue_all():
-        for j in jobs:
+    def enqueue_all(streams=(s,)):
+        for k, j in enumerate(jobs):
             c = j["sc"]["case"]
+            st = streams[k % len(streams)]
             j["plan"].sector_run_device(j["d1"].ptr, j["d2"].ptr, 0, c["nf"], j["cfg"],
-                                        j["acc"].ptr, stream=s.ptr)
+                                        j["acc"].ptr, stream=st.ptr)
             if "pplan" in j:
                 j["pplan"].sector_run_device(j["d1"].ptr, j["d2"].ptr, 0, c["nf"], j["cfg"],
-                                             j["pacc"].ptr, j["praw"].ptr, stream=s.ptr)
+                                             j["pacc"].ptr, j["praw"].ptr, stream=st.ptr)
 
     def collect():
         out = []
@@ -129,9 +131,24 @@ def test_48_uneven_scans_one_graph():
     enqueue_all()                                   # eager (also uploads the delay tables)
     s.sync()
     eager = collect()
+    # the capture forks onto three streams (the scans as parallel branches)
+    # and joins back before it ends
+    side = [F.Stream() for _ in range(3)]
     g = F.Graph(s)
-    enqueue_all()
+    fork = F.Event()
+    fork.record(s)
+    for st in side:
+        st.wait(fork)
+    enqueue_all(side)
+    joins = [F.Event() for _ in side]
+    for ev, st in zip(joins, side):
+        ev.record(st)
+        s.wait(ev)
     g.end()
+    for j in jobs:                                  # every plan is held, whichever branch ran it
+        for key in ("plan", "pplan"):
+            if key in j:
+                assert F.plan_live_graphs(j[key]) == 1, key
     for rep in range(2):
         for j in jobs:
             for key in ("acc", "pacc", "praw"):
