@@ -1701,10 +1701,7 @@ __device__ __forceinline__ float2 decode_pair_local(const uint32_t *words, const
     if constexpr (BITS == 1) {
         x0 = (v & 1u) ? st.lev4[1] : st.lev4[0];
         x1 = (v & 2u) ? st.lev4[1] : st.lev4[0];
-    } else if constexpr (BITS == 2) {
-        x0 = level_of(v & 3u, st, lev_lds);
-        x1 = level_of((v >> 2) & 3u, st, lev_lds);
-    } else if constexpr (BITS == 4 || BITS == 8) {
+    } else if constexpr (BITS == 2 || BITS == 4 || BITS == 8) {
         constexpr uint32_t mask = (1u << BITS) - 1u;
         x0 = lev_lds[v & mask];
         x1 = lev_lds[(v >> BITS) & mask];
@@ -1788,7 +1785,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
     const uint4 *b1g = reinterpret_cast<const uint4 *>(a.mf_tab);
     for (int i = tid; i < 16 * 64; i += 256) b1[i] = b1g[i];
     if (tid < R * 32) twv[tid >> 5][tid & 31] = tw_m[(tid >> 5) * (tid & 31) * M2];
-    if (st.bits > 2 && st.bits <= 8)
+    if (st.bits >= 2 && st.bits <= 8)
         for (int i = tid; i < 256; i += 256) lev[i] = (i <= (int)st.code_mask) ? st.levels[i] : 0.0f;
     const int sh = clamp_shift(shifts[f * 2 + station], a.N);
     const int base0 = ((128 * part - sh) * bits) >> 5;   // arithmetic shift: floor
