@@ -570,7 +570,8 @@ static bool levels_fp16_exact(const fxc_plan *p) {
 
 static bool mfma_eligible(const fxc_plan *p) {
     if ((p->N != 1024 && p->N != 2048) || p->desc.precision != FXC_PRECISION_CHOSEN) return false;
-    if (p->desc.products & (FXC_PRODUCT_PHASED | FXC_PRODUCT_RAW_OUT)) return false;
+    // phased plans: the producer/consumer kernel with a phased consumer, N = 1024
+    if ((p->desc.products & (FXC_PRODUCT_PHASED | FXC_PRODUCT_RAW_OUT)) && p->N != 1024) return false;
     if (p->desc.a1s != p->desc.a2s) return false;
     for (int i = 0; i < 2; ++i) {
         const int b = p->desc.st[i].bits;
@@ -1222,8 +1223,48 @@ static int64_t chain_span(int64_t nf, int64_t chunk, size_t row_bytes, int len) 
     return std::min(fit, need) * chunk;
 }
 
+// Phased sum on the N = 1024 producer/consumer kernel (chosen form): one
+// workgroup per CU over a contiguous range of frames; for the .raw output the
+// sector runs in chunks of kPhasedMfCbBytes of combined spectra, each
+// followed by the inverse FFT and quantiser (no chunks for |cb|^2 alone).
+constexpr size_t kPhasedMfCbBytes = (size_t)512 << 20;
+
+static fxc_status enqueue_phased_mf3(fxc_plan *p, const SectorArgs &base, int64_t nf,
+                                     uint8_t *d_raw_out, double *d_acc, hipStream_t s) {
+    const bool raw = (p->desc.products & FXC_PRODUCT_RAW_OUT) && d_raw_out != nullptr;
+    const size_t half = (size_t)p->half;
+    const int64_t bpf_o = plan_bpf_out(p);
+    const int64_t chunk = raw ? std::max<int64_t>(1, std::min<int64_t>(nf, (int64_t)(kPhasedMfCbBytes / (half * 8))))
+                              : nf;
+    if (raw) GROW(p, p->d_cb, (size_t)chunk * half * 8, s, "phased spectra");
+    const int batch = 8;   // whole frames per producer wave
+    int64_t fpw = (chunk + p->max_wg - 1) / p->max_wg;
+    fpw = ((fpw + batch - 1) / batch) * batch;
+    const int n_wg = (int)((chunk + fpw - 1) / fpw);
+    for (int64_t c0 = 0; c0 < nf; c0 += chunk) {
+        const int64_t cn = std::min(chunk, nf - c0);
+        SectorArgs a = base;
+        a.raw1 = base.raw1 + c0 * p->st[0].bpf;
+        a.raw2 = base.raw2 + c0 * p->st[1].bpf;
+        a.first_frame = base.first_frame + c0;
+        a.nf = cn;
+        a.frames_per_wg = fpw;
+        a.part = (double *)p->part.p;
+        a.cb_out = raw ? (float2 *)p->d_cb.p : nullptr;
+        a.chain = nullptr;
+        HIP_TRY(launch_phased_mf3(a, c0 > 0 ? 1 : 0, (int)((cn + fpw - 1) / fpw), s));
+        if (raw)
+            HIP_TRY(launch_irfft_quantise(a, (const float2 *)p->d_cb.p, cn, d_raw_out + c0 * bpf_o,
+                                          p->desc.output_lsb != 0, s));
+    }
+    HIP_TRY(launch_reduce((const double *)p->part.p, n_wg, p->half, d_acc,
+                          (double *)p->red_scratch.p, 64, s));
+    return FXC_OK;
+}
+
 static fxc_status enqueue_phased(fxc_plan *p, const SectorArgs &base, int64_t nf,
                                  uint8_t *d_raw_out, double *d_acc, hipStream_t s) {
+    if (p->fft_form == 1) return enqueue_phased_mf3(p, base, nf, d_raw_out, d_acc, s);
     const bool raw = (p->desc.products & FXC_PRODUCT_RAW_OUT) && d_raw_out != nullptr;
     const size_t half = (size_t)p->half;
     const int64_t bpf_o = plan_bpf_out(p);

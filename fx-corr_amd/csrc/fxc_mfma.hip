@@ -1013,14 +1013,10 @@ __device__ void mf3_consumer(const SectorArgs &a, MfShared3 &sh, int c, int l, i
     if constexpr (ACC32) mf_load_totals(gacc, first, acc, l);
 }
 
-template <bool ACC32>
-__global__ __launch_bounds__(mf::THREADS3) void k_sector_mf3(SectorArgs a) {
-    __shared__ MfShared3 sh;
-    constexpr int M = mf::M;
+// The producer/consumer kernels' LDS set-up: operand, twiddle and decode
+// tables from the plan's MFT_* table, zeroed staging rows, hand-over flags.
+__device__ __forceinline__ void mf3_setup(const SectorArgs &a, MfShared3 &sh, int tid) {
     constexpr int NT = mf::THREADS3;
-    const int tid = threadIdx.x;
-    const int l = tid & 63;
-    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
     const uint8_t *tab = reinterpret_cast<const uint8_t *>(a.mf_tab);
     {
         const uint4 *g = reinterpret_cast<const uint4 *>(tab + MFT_B1);
@@ -1046,6 +1042,17 @@ __global__ __launch_bounds__(mf::THREADS3) void k_sector_mf3(SectorArgs a) {
         }
         if (tid == 0) sh.broken = 0;
     }
+}
+
+template <bool ACC32>
+__global__ __launch_bounds__(mf::THREADS3) void k_sector_mf3(SectorArgs a) {
+    __shared__ MfShared3 sh;
+    constexpr int M = mf::M;
+    constexpr int NT = mf::THREADS3;
+    const int tid = threadIdx.x;
+    const int l = tid & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    mf3_setup(a, sh, tid);
     __syncthreads();
 
     // frames of producer p: a contiguous eighth of the workgroup's range
@@ -1122,6 +1129,272 @@ __global__ __launch_bounds__(mf::THREADS3) void k_sector_mf3(SectorArgs a) {
         out[P_12R * H + k] = red[2 * H + k] + bad;
         out[P_12I * H + k] = red[3 * H + k] + bad;
     }
+}
+
+// ------------------------------------------- phased sum at N = 1024
+// The producer/consumer kernel with a phased-array consumer (chosen form;
+// main.rs:2264-2302, utils.rs:69-124).  The producers are k_sector_mf3's.
+// Per frame and output bin the consumer takes each station's bin through its
+// fractional-delay rotation exp(2 pi i b th) (th = -df frac turns per bin,
+// from the FP64 delay) and fringe phasor, forms cb = w1 c1 + w2 c2, adds
+// |cb|^2 to FP64 sums and, for the .raw output, writes cb (DC and Nyquist
+// imaginary parts zeroed) to cb_out[frame][M + 1].  Both stations share the
+// band start (a1s == a2s), so a lane's bins are its pairs (p, M - p) on both
+// stations, and R(M - p) = R(M) conj R(p).
+struct MfPhFrame {
+    double thg, tho;         // rotation per bin, turns: grid station, other station
+    float2 frg, fro;         // fringe phasors
+    float2 Sg, So, Cg, Co;   // exp(2 pi i 64 th), exp(2 pi i M th)
+};
+
+__device__ __forceinline__ float2 mf_phasor(double turns) {
+    const float r = (float)wrap_turns(turns);
+    return make_float2(__builtin_amdgcn_cosf(r), __builtin_amdgcn_sinf(r));
+}
+
+// The chosen phased parameters of frame idx (make_phased_param's values).
+__device__ MfPhFrame mfp_frame(int64_t idx, const SectorArgs &a) {
+    double tau1, tau2, tf1, tf2;
+    frame_taus(idx, a.dly, tau1, tau2, tf1, tf2);
+    int64_t n1, n2;
+    double fr1, fr2;
+    split_delay(tau1, a.dly.fs, n1, fr1);
+    split_delay(tau2, a.dly.fs, n2, fr2);
+    const float w1 = a.dly.lo1 == 0.0 ? 0.0f : (float)wrap_turns(-a.dly.lo1 * tf1);
+    const float w2 = a.dly.lo2 == 0.0 ? 0.0f : (float)wrap_turns(-a.dly.lo2 * tf2);
+    const float2 f1 = make_float2(__builtin_amdgcn_cosf(w1), __builtin_amdgcn_sinf(w1));
+    const float2 f2 = make_float2(__builtin_amdgcn_cosf(w2), __builtin_amdgcn_sinf(w2));
+    const double th1 = -a.df * fr1, th2 = -a.df * fr2;
+    const bool g0 = a.grid == 0;
+    MfPhFrame o;
+    o.thg = g0 ? th1 : th2;
+    o.tho = g0 ? th2 : th1;
+    o.frg = g0 ? f1 : f2;
+    o.fro = g0 ? f2 : f1;
+    o.Sg = mf_phasor(64.0 * o.thg);
+    o.So = mf_phasor(64.0 * o.tho);
+    o.Cg = mf_phasor((double)mf::M * o.thg);
+    o.Co = mf_phasor((double)mf::M * o.tho);
+    return o;
+}
+
+__device__ __forceinline__ float2 readlane_f2(float2 v, int lane) {
+    return make_float2(readlane_f(v.x, lane), readlane_f(v.y, lane));
+}
+
+// One output bin (k_phased's arithmetic): the grid station's spectrum value
+// xg and the other station's xo on the same bin, their rotations rg / ro;
+// nyq zeroes the rotated imaginary parts (bin M, utils.rs:118-123); the
+// other station counts only inside the overlap.
+__device__ __forceinline__ float2 mfp_combine(float2 xg, float2 rg, float2 xo, float2 ro, bool nyq,
+                                              bool in_ov, float2 frg, float2 fro, int gsn, float w1,
+                                              float w2) {
+    float2 sg = c_mul(xg, rg);
+    float2 so = c_mul(xo, ro);
+    if (nyq) {
+        sg.y = 0.0f;
+        so.y = 0.0f;
+    }
+    const float2 cg = c_mul(sg, frg);
+    const float2 co0 = c_mul(so, fro);
+    const float2 co = in_ov ? co0 : make_float2(0.0f, 0.0f);
+    const float2 c1 = gsn == 0 ? cg : co, c2 = gsn == 0 ? co : cg;
+    float2 cb;
+    {
+#pragma clang fp contract(off)
+        const float2 a1 = make_float2(c1.x * w1, c1.y * w1), a2 = make_float2(c2.x * w2, c2.y * w2);
+        cb = make_float2(a1.x + a2.x, a1.y + a2.y);
+    }
+    return cb;
+}
+
+// The phased products of one frame (slot z1 / z2) for lane l's eight bins.
+__device__ __forceinline__ void mfp_mac(const float2 *z1, const float2 *z2, const float *twu,
+                                        const MfPhFrame &pf, int jf, int l, int gsn, uint32_t m_ov,
+                                        float w1, float w2, float2 *cbo, double (&acc)[8]) {
+    constexpr int M = mf::M;
+    const double thg = readlane_d(pf.thg, jf), tho = readlane_d(pf.tho, jf);
+    const float2 frg = readlane_f2(pf.frg, jf), fro = readlane_f2(pf.fro, jf);
+    const float2 Sg = readlane_f2(pf.Sg, jf), So = readlane_f2(pf.So, jf);
+    const float2 Cg = readlane_f2(pf.Cg, jf), Co = readlane_f2(pf.Co, jf);
+    float2 Rg = mf_phasor((double)l * thg), Ro = mf_phasor((double)l * tho);
+    const float2 *zg = gsn == 0 ? z1 : z2;
+    const float2 *zo = gsn == 0 ? z2 : z1;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int p = l + 64 * j;
+        const int pm = (M - p) & (M - 1);
+        const float2 w = make_float2(twu[(2 * j) * 64 + l], twu[(2 * j + 1) * 64 + l]);
+        float2 xgp, xgm, xop, xom;
+        untangle_pair(zg[p], zg[pm], w, xgp, xgm);
+        untangle_pair(zo[p], zo[pm], w, xop, xom);
+        const int sp = 2 * j, sm = 2 * j + 1;
+        const bool nyq = j == 0 && l == 0;   // lane 0's slot 1 is bin M
+        const float2 cbp = mfp_combine(xgp, Rg, xop, Ro, false, (m_ov >> sp) & 1u, frg, fro, gsn, w1, w2);
+        const float2 cbm = mfp_combine(xgm, c_mul(Cg, c_conj(Rg)), xom, c_mul(Co, c_conj(Ro)), nyq,
+                                       (m_ov >> sm) & 1u, frg, fro, gsn, w1, w2);
+        acc[sp] += (double)norm_sqr(cbp);
+        acc[sm] += (double)norm_sqr(cbm);
+        if (cbo != nullptr) {   // main.rs:2283-2284: DC and Nyquist imaginary parts zeroed
+            cbo[p] = make_float2(cbp.x, p == 0 ? 0.0f : cbp.y);
+            cbo[M - p] = make_float2(cbm.x, nyq ? 0.0f : cbm.y);
+        }
+        Rg = c_mul(Rg, Sg);
+        Ro = c_mul(Ro, So);
+    }
+}
+
+// Bin M/2 (X = 2 conj Z'[M/2] on each station) of the frame whose
+// parameters this lane holds, formed once per batch as in mf_mid.
+__device__ __forceinline__ void mfp_mid(const MfPhFrame &pf, float2 u1, float2 u2, int gsn, bool in_ov,
+                                        float w1, float w2, float2 *cb_mid, double &macc) {
+    constexpr int K = mf::M / 2;
+    const float2 x1 = make_float2(2.0f * u1.x, -2.0f * u1.y);
+    const float2 x2 = make_float2(2.0f * u2.x, -2.0f * u2.y);
+    const float2 xg = gsn == 0 ? x1 : x2, xo = gsn == 0 ? x2 : x1;
+    const float2 cb = mfp_combine(xg, mf_phasor((double)K * pf.thg), xo, mf_phasor((double)K * pf.tho),
+                                  false, in_ov, pf.frg, pf.fro, gsn, w1, w2);
+    macc += (double)norm_sqr(cb);
+    if (cb_mid != nullptr) *cb_mid = cb;
+}
+
+// Consumer c of the phased kernel: the frames of producers c and c + 4,
+// alternating, as mf3_consumer.
+__device__ void mfp_consumer(const SectorArgs &a, MfShared3 &sh, int c, int l, int64_t fa0,
+                             int64_t fa1, int64_t fb0, int64_t fb1, double (&acc)[8], double &macc) {
+    constexpr int H = mf::M + 1;
+    const int gsn = a.grid;
+    const int gs = a.a1s, ge = a.a1s + a.ov;   // a1s == a2s on this path
+    uint32_t m_ov = 0;
+#pragma unroll
+    for (int s = 0; s < 9; ++s) {
+        const int k = mf_bin(s, l);
+        if (k >= gs && k < ge) m_ov |= 1u << s;
+    }
+    const bool mid_ov = (m_ov >> 8) & 1u;
+    const float w1 = (float)a.w1, w2 = (float)a.w2;
+    float2 *cb = ((a.flags & FLAG_RAW_OUT) != 0) ? a.cb_out : nullptr;
+    const int64_t na = fa1 - fa0, nb = fb1 - fb0;
+    const int64_t n = na > nb ? na : nb;
+    const int q = c + mf::CONS;
+    bool ok = true;
+    for (int64_t g = 0; g < n; g += mf::FB) {
+        const int cnt = (int)((n - g) < mf::FB ? (n - g) : mf::FB);
+        MfPhFrame pa = {}, pb = {};
+        if (l < cnt && g + l < na) pa = mfp_frame(a.first_frame + fa0 + g + l, a);
+        if (l < cnt && g + l < nb) pb = mfp_frame(a.first_frame + fb0 + g + l, a);
+        float2 ua1 = make_float2(0.f, 0.f), ua2 = ua1, ub1 = ua1, ub2 = ua1;   // Z'[M/2] of frame g + l
+        for (int jf = 0; jf < cnt; ++jf) {
+            const int32_t i = (int32_t)(g + jf);
+            if (i < na) {
+                mf_wait_ge(&sh.prod[c], i + 1, ok, &sh.broken);
+                mfp_mac(sh.z[c][0], sh.z[c][1], sh.twu, pa, jf, l, gsn, m_ov, w1, w2,
+                        cb ? cb + (fa0 + i) * H : nullptr, acc);
+                const float2 u1 = sh.z[c][0][mf::M / 2], u2 = sh.z[c][1][mf::M / 2];
+                if (l == jf) {
+                    ua1 = u1;
+                    ua2 = u2;
+                }
+                mf_publish(&sh.cons[c], i + 1);
+            }
+            if (i < nb) {
+                mf_wait_ge(&sh.prod[q], i + 1, ok, &sh.broken);
+                mfp_mac(sh.z[q][0], sh.z[q][1], sh.twu, pb, jf, l, gsn, m_ov, w1, w2,
+                        cb ? cb + (fb0 + i) * H : nullptr, acc);
+                const float2 u1 = sh.z[q][0][mf::M / 2], u2 = sh.z[q][1][mf::M / 2];
+                if (l == jf) {
+                    ub1 = u1;
+                    ub2 = u2;
+                }
+                mf_publish(&sh.cons[q], i + 1);
+            }
+        }
+        if (l < cnt && g + l < na)
+            mfp_mid(pa, ua1, ua2, gsn, mid_ov, w1, w2, cb ? cb + (fa0 + g + l) * H + mf::M / 2 : nullptr, macc);
+        if (l < cnt && g + l < nb)
+            mfp_mid(pb, ub1, ub2, gsn, mid_ov, w1, w2, cb ? cb + (fb0 + g + l) * H + mf::M / 2 : nullptr, macc);
+    }
+}
+
+// accumulate = 0: the workgroup's partial slot is set; 1: its |cb|^2 sums
+// are added (a sector in chunks that reuse the slots in order).
+__global__ __launch_bounds__(mf::THREADS3) void k_phased_mf3(SectorArgs a, int accumulate) {
+    __shared__ MfShared3 sh;
+    constexpr int M = mf::M;
+    constexpr int NT = mf::THREADS3;
+    const int tid = threadIdx.x;
+    const int l = tid & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    mf3_setup(a, sh, tid);
+    __syncthreads();
+    const int64_t wg0 = (int64_t)blockIdx.x * a.frames_per_wg;
+    int64_t wg1 = wg0 + a.frames_per_wg;
+    if (wg1 > a.nf) wg1 = a.nf;
+    const int64_t per = (a.frames_per_wg + mf::PROD - 1) / mf::PROD;
+    auto range = [&](int p, int64_t &r0, int64_t &r1) {
+        r0 = wg0 + per * p;
+        if (r0 > wg1) r0 = wg1;
+        r1 = (r0 + per < wg1) ? r0 + per : wg1;
+    };
+    double acc[8], macc = 0.0;
+#pragma unroll
+    for (int s = 0; s < 8; ++s) acc[s] = 0.0;
+    if (wv < mf::PROD) {
+        int64_t f0, f1;
+        range(wv, f0, f1);
+        mf3_producer(a, sh, wv, l, f0, f1);
+    } else {
+        const int c = wv - mf::PROD;
+        int64_t fa0, fa1, fb0, fb1;
+        range(c, fa0, fa1);
+        range(c + mf::CONS, fb0, fb1);
+        mfp_consumer(a, sh, c, l, fa0, fa1, fb0, fb1, acc, macc);
+    }
+
+    // ---- workgroup partial: consumers add their bins in a fixed order
+    __syncthreads();
+    double *red = reinterpret_cast<double *>(&sh.z[0][0][0]);   // [M + 1]
+    double *mids = reinterpret_cast<double *>(&sh.z[4][0][0]);  // [CONS][64] lanes' bin-M/2 sums
+    constexpr int H = M + 1;
+    if (wv >= mf::PROD) mids[(wv - mf::PROD) * 64 + l] = macc;
+    __syncthreads();
+    for (int w = mf::PROD; w < mf::PROD + mf::CONS; ++w) {
+        if (wv == w) {
+            const bool first = w == mf::PROD;
+#pragma unroll
+            for (int s = 0; s < 8; ++s) {
+                const int k = mf_bin(s, l);
+                red[k] = (first ? 0.0 : red[k]) + acc[s];
+            }
+            if (l == 0) {
+                const double *mc = mids + (w - mf::PROD) * 64;
+                double sum = 0.0;
+                for (int i = 0; i < 64; ++i) sum += mc[i];   // lanes in order
+                red[M / 2] = (first ? 0.0 : red[M / 2]) + sum;
+            }
+        }
+        __syncthreads();
+    }
+    double *out = a.part + (size_t)blockIdx.x * kNumProducts * H;
+    const double bad = sh.broken ? __builtin_nan("") : 0.0;   // NaN marks a failed hand-over
+    if (sh.broken && tid == 0 && a.err)
+        __hip_atomic_fetch_or(a.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int k = tid; k < H; k += NT) {
+        out[P_PH * H + k] = (accumulate ? out[P_PH * H + k] : 0.0) + red[k] + bad;
+        if (!accumulate) {   // the other products stay zero over the later chunks
+            out[P_11 * H + k] = 0.0;
+            out[P_22 * H + k] = 0.0;
+            out[P_12R * H + k] = 0.0;
+            out[P_12I * H + k] = 0.0;
+        }
+    }
+}
+
+hipError_t launch_phased_mf3(const SectorArgs &a, int accumulate, int n_wg, hipStream_t s) {
+    if (a.N != mf::N || a.mf_tab == nullptr || a.a1s != a.a2s) return hipErrorInvalidValue;
+    if (n_wg <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_phased_mf3, dim3(n_wg), dim3(mf::THREADS3), 0, s, a, accumulate);
+    return hipGetLastError();
 }
 
 // ------------------------------------------- N = 2048: k_sector_mf4

@@ -613,6 +613,8 @@ def _phased_desc(c, w, bit_out, output_lsb):
 
 PHASED_CASES = [
     (1024, (2, 2), 0, 0, False),
+    (1024, (8, 1), 1, 0, True),        # chosen: the producer/consumer MFMA kernel, other grid
+    (1024, (4, 4), 0, 8, False),       # band shift: the chosen form on the Stockham kernel
     (512, (4, 2), 1, 8, True),
     (2048, (2, 1), 0, -16, False),
     (4096, (8, 2), 0, 64, True),
@@ -680,6 +682,75 @@ def test_phased_chosen_vs_fp64_oracle(n, bits, grid, band_shift, lsb_out):
     |cb|^2 within the chosen bar, .raw codes equal except at rare quantiser
     decision boundaries."""
     _phased_case(n, bits, grid, band_shift, lsb_out, F.PRECISION_CHOSEN)
+
+
+PHASED_MF_CHUNK = (512 << 20) // (513 * 8)   # frames per .raw chunk of the N = 1024 MFMA kernel
+
+
+@pytest.mark.parametrize("nf,bits,grid,band,lsb_out", [
+    (3000, (4, 2), 1, (0, 16, 400, 16), True),          # partial overlap on a common band start
+    (140_000, (2, 1), 0, (0, 0, 512, 0), False),        # two .raw chunks, 64-frame batches
+])
+def test_phased_chosen_mfma_kernel(nf, bits, grid, band, lsb_out):
+    """Chosen phased sum at N = 1024 on the producer/consumer MFMA kernel
+    (both stations on a common band start).  Device-generated stations;
+    |cb|^2 within the chosen bar of the oracle's FP64 phased branch over
+    every frame, and the .raw codes of sampled frames (every frame around
+    the chunk boundary) equal except at rare quantiser decision
+    boundaries."""
+    n, fs = 1024, 256e6
+    sh1 = np.random.default_rng(nf).permutation(32).astype(np.int32)
+    s = F.Stream()
+    d1 = F.DeviceBuffer(nf * n * bits[0] // 8)
+    d2 = F.DeviceBuffer(nf * n * bits[1] // 8)
+    F.synth_station_device(d1.ptr, nf * n, bits[0], sh1, True, 77, 0, 0.4, 3, s.ptr)
+    F.synth_station_device(d2.ptr, nf * n, bits[1], None, False, 77, 1, 0.4, 3, s.ptr)
+    raw1 = d1.download(np.zeros(nf * n * bits[0] // 8, dtype=np.uint8), s.ptr)
+    raw2 = d2.download(np.zeros(nf * n * bits[1] // 8, dtype=np.uint8), s.ptr)
+    s.sync()
+    lv = [np.arange(1 << b, dtype=np.float64) - ((1 << b) - 1) / 2.0 for b in bits]
+    ident = np.arange(32, dtype=np.int32)
+    table = np.zeros(int(np.ceil(nf * n / fs)) + 2)
+    delay = dict(frame_dt=n / fs, fs=fs, delay_user_samples=-2.2, extra_delay_rate_sps=2e-9,
+                 lo1_hz=3e9, lo2_hz=3e9)
+    w = (0.9, 1.2)
+    bit_out = max(bits)
+    p = F.Plan(n, bits=bits, levels=lv, shuffles=(sh1, None), lsb=(True, False), band=band,
+               df_hz=fs / n, products=F.PRODUCT_PHASED | F.PRODUCT_RAW_OUT, out_grid=grid, w=w,
+               bit_out=bit_out, output_lsb=lsb_out, precision=F.PRECISION_CHOSEN)
+    assert p.info.fft_form == 1
+    acc = F.DeviceBuffer(F.N_PRODUCTS * p.half * 8)
+    rawo = F.DeviceBuffer(nf * p.info.bpf_out)
+    p.sector_run_device(d1.ptr, d2.ptr, 0, nf, F.make_delay_cfg(table, **delay), acc.ptr, rawo.ptr,
+                        stream=s.ptr)
+    got_acc = acc.download(np.zeros(F.N_PRODUCTS * p.half), s.ptr)
+    got_raw = rawo.download(np.zeros(nf * p.info.bpf_out, dtype=np.uint8), s.ptr)
+    s.sync()
+    desc = dict(fft_len=n, bits1=bits[0], bits2=bits[1], levels1=lv[0], levels2=lv[1],
+                shuffle1=sh1, shuffle2=ident, lsb1=1, lsb2=0, a1s=band[1], a1e=band[2],
+                a2s=band[3], out_grid=grid, df_hz=fs / n, need_acf=0, need_xcf=0, need_phased=1,
+                write_raw=1, w1=w[0], w2=w[1], bit_out=bit_out, output_lsb=int(lsb_out),
+                cpu_threads=8)
+    ref = R.sector_run(desc, raw1, raw2, 0, nf, R.make_delay_cfg(table, **delay), mode=R.MODE_F64,
+                       threads=8)
+    h = p.half
+    e = rel_err(got_acc[:h], ref["acc_ph"])
+    print(f"[phased-mf] nf={nf} bits={bits}: acc_ph {e:.2e}", flush=True)
+    assert e < CHOSEN_TOL
+    assert np.all(got_acc[h:] == 0)
+    bpf_o = n * bit_out // 8
+    frames = sorted(set(range(0, nf, max(1, nf // 200))) |
+                    set(range(max(0, PHASED_MF_CHUNK - 40), min(nf, PHASED_MF_CHUNK + 40))))
+    dp = out_decode_plan(bit_out, sh1, lv[0])
+    diff = 0
+    for f in frames:
+        seg = slice(f * bpf_o, (f + 1) * bpf_o)
+        _, vg = dp.decode(got_raw[seg], n, False)
+        _, vr = dp.decode(ref["raw_out"][seg], n, False)
+        diff += int(np.count_nonzero(vg != vr))
+    assert diff <= 1e-4 * n * len(frames) + 2, diff
+    for b in (d1, d2, acc, rawo):
+        b.free()
 
 
 @pytest.mark.parametrize("nf,chunk", [(5, 0), (23, 7)])
