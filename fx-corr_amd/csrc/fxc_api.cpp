@@ -490,12 +490,36 @@ static void fill_args(fxc_plan *p, SectorArgs &a) {
     a.w2 = p->desc.w2;
     a.bit_out = p->desc.bit_out;
     a.output_lsb = p->desc.output_lsb;
-    for (int k = 0; k < 32; ++k) a.out_shifts[k] = p->desc.st[0].shuffle_internal[k];
+    a.out_shuffle_ident = 1;
+    for (int k = 0; k < 32; ++k) {
+        a.out_shifts[k] = p->desc.st[0].shuffle_internal[k];
+        if (a.out_shifts[k] != k) a.out_shuffle_ident = 0;
+    }
     a.out_levels = (const float *)p->out_lev.p;
     a.out_nlevels = p->desc.st[0].nlevels;
     a.out_sorted = 1;
     for (int k = 1; k < (int)p->levels[0].size(); ++k)
         if (!((float)p->levels[0][k - 1] <= (float)p->levels[0][k])) a.out_sorted = 0;
+    // quantiser search: non-decreasing levels within a quarter step of an
+    // even spacing start from a guess (mode 2); the result is the bisection's
+    a.out_qmode = a.out_sorted ? 1 : 0;
+    a.out_l0 = 0.0f;
+    a.out_inv_step = 0.0f;
+    {
+        const int nl = (int)std::min<size_t>(p->levels[0].size(), 256);
+        if (a.out_sorted && nl >= 2) {
+            const double l0 = (float)p->levels[0][0], ln = (float)p->levels[0][nl - 1];
+            const double d = (ln - l0) / (nl - 1);
+            bool even = d > 0.0 && std::isfinite(d);
+            for (int k = 0; even && k < nl; ++k)
+                even = std::fabs((double)(float)p->levels[0][k] - (l0 + k * d)) <= 0.25 * d;
+            if (even) {
+                a.out_qmode = 2;
+                a.out_l0 = (float)l0;
+                a.out_inv_step = (float)(1.0 / d);
+            }
+        }
+    }
     a.bpf_out = (int32_t)(((int64_t)p->N * p->desc.bit_out + 7) / 8);
     uint32_t fl = 0;
     if (p->desc.products & FXC_PRODUCT_ACF) fl |= FLAG_ACF;
@@ -1234,8 +1258,10 @@ static fxc_status enqueue_phased_mf3(fxc_plan *p, const SectorArgs &base, int64_
     const bool raw = (p->desc.products & FXC_PRODUCT_RAW_OUT) && d_raw_out != nullptr;
     const size_t half = (size_t)p->half;
     const int64_t bpf_o = plan_bpf_out(p);
-    const int64_t chunk = raw ? std::max<int64_t>(1, std::min<int64_t>(nf, (int64_t)(kPhasedMfCbBytes / (half * 8))))
-                              : nf;
+    // chunks of equal size (a short last chunk would run on a few workgroups)
+    const int64_t cmax = raw ? std::max<int64_t>(1, (int64_t)(kPhasedMfCbBytes / (half * 8))) : nf;
+    const int64_t nchunks = (nf + cmax - 1) / cmax;
+    const int64_t chunk = (nf + nchunks - 1) / nchunks;
     if (raw) GROW(p, p->d_cb, (size_t)chunk * half * 8, s, "phased spectra");
     const int batch = 8;   // whole frames per producer wave
     int64_t fpw = (chunk + p->max_wg - 1) / p->max_wg;

@@ -556,20 +556,14 @@ __device__ __forceinline__ uint32_t quant_code(float x, const float *lv, int nl)
     return best;
 }
 
-// The same code for non-decreasing levels, by bisection.  |fl(x - l_k)| is
-// non-increasing in k up to j, the last level <= x, and non-decreasing after
-// it (rounding is monotone), so the scan's first minimum is j + 1 when that
-// error is strictly smaller, else the first of the run of errors equal to
-// level j's.  As in the scan, an error that does not compare below FLT_MAX
-// (x NaN or infinite) leaves code 0.
-__device__ __forceinline__ uint32_t quant_code_sorted(float x, const float *lv, int nl) {
-    if (!(x == x)) return 0u;
-    int lo = -1, hi = nl;   // lv[lo] <= x < lv[hi] (lo = -1, hi = nl: none)
-    while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (lv[mid] <= x) lo = mid;
-        else hi = mid;
-    }
+// The same code for non-decreasing levels, from lo, the last level <= x
+// (-1: none).  |fl(x - l_k)| is non-increasing in k up to lo and
+// non-decreasing after it (rounding is monotone), so the scan's first
+// minimum is lo + 1 when that error is strictly smaller, else the first of
+// the run of errors equal to level lo's.  As in the scan, an error that does
+// not compare below FLT_MAX (x infinite) leaves code 0.
+__device__ __forceinline__ uint32_t quant_code_from_lo(float x, const float *lv, int nl, int lo) {
+    const int hi = lo + 1;
     int best;
     float eb;
     if (lo < 0) {
@@ -589,6 +583,51 @@ __device__ __forceinline__ uint32_t quant_code_sorted(float x, const float *lv, 
     return eb < 3.40282347e+38f ? (uint32_t)best : 0u;
 }
 
+// Non-decreasing levels: lo by bisection.  x NaN: code 0, as the scan.
+__device__ __forceinline__ uint32_t quant_code_sorted(float x, const float *lv, int nl) {
+    if (!(x == x)) return 0u;
+    int lo = -1, hi = nl;   // lv[lo] <= x < lv[hi] (lo = -1, hi = nl: none)
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (lv[mid] <= x) lo = mid;
+        else hi = mid;
+    }
+    return quant_code_from_lo(x, lv, nl, lo);
+}
+
+// Output quantiser levels and the search that finds lo:
+// mode 0 the scan (any level order), 1 bisection (non-decreasing levels),
+// 2 (non-decreasing levels within a quarter step of an even spacing) a
+// guess floor((x - l0) inv) moved to the last level <= x by comparisons,
+// the same lo as the bisection's in about one step.
+struct QLev {
+    const float *lv;
+    int nl;
+    int mode;
+    float l0, inv;
+};
+
+__device__ __forceinline__ uint32_t quant_code_q(float x, const QLev &q) {
+    if (q.mode == 0) return quant_code(x, q.lv, q.nl);
+    if (q.mode == 1) return quant_code_sorted(x, q.lv, q.nl);
+    if (!(x == x)) return 0u;
+    const float gf = fminf(fmaxf(floorf((x - q.l0) * q.inv), -1.0f), (float)(q.nl - 1));
+    int g = (int)gf;
+    while (g + 1 < q.nl && q.lv[g + 1] <= x) ++g;
+    while (g >= 0 && q.lv[g] > x) --g;
+    return quant_code_from_lo(x, q.lv, q.nl, g);
+}
+
+__device__ __forceinline__ QLev out_qlev(const SectorArgs &a, const float *lv_lds) {
+    QLev q;
+    q.lv = lv_lds;
+    q.nl = a.out_nlevels < 256 ? a.out_nlevels : 256;
+    q.mode = a.out_qmode;
+    q.l0 = a.out_l0;
+    q.inv = a.out_inv_step;
+    return q;
+}
+
 __device__ __forceinline__ uint32_t shuffle_out_word(uint32_t w, const int32_t *sh) {
     uint32_t o = 0;
 #pragma unroll
@@ -598,20 +637,53 @@ __device__ __forceinline__ uint32_t shuffle_out_word(uint32_t w, const int32_t *
 
 // One thread per output word of one frame; x(s) gives sample s.
 template <typename X>
-__device__ __forceinline__ uint32_t quant_word(X x, int N, int w, int bits, const float *lv, int nl,
-                                               uint32_t mask, bool sorted) {
+__device__ __forceinline__ uint32_t quant_word(X x, int N, int w, int bits, const QLev &ql,
+                                               uint32_t mask) {
     uint64_t acc = 0;
     const int64_t b0 = (int64_t)w * 32, b1 = b0 + 32;
     int64_t s = b0 / bits;
     for (; s * bits < b1 && s < N; ++s) {
         const float xs = x((int)s);
-        const uint32_t q = sorted ? quant_code_sorted(xs, lv, nl) : quant_code(xs, lv, nl);
+        const uint32_t q = quant_code_q(xs, ql);
         uint64_t code = (uint64_t)(q & mask);
         int64_t pos = s * bits - b0;  // may be negative for the straddling first sample
         if (pos >= 0) acc |= code << pos;
         else acc |= code >> (-pos);
     }
     return (uint32_t)(acc & 0xFFFFFFFFull);
+}
+
+// The same word when the code width divides 32 (1, 2, 4, 8, 16, 32 bits):
+// samples w spw .. w spw + spw - 1, no code straddles a word, 32-bit
+// arithmetic only.
+template <typename X>
+__device__ __forceinline__ uint32_t quant_word_aligned(X x, int w, int bits, const QLev &ql,
+                                                       uint32_t mask) {
+    const int spw = 32 / bits;
+    uint32_t acc = 0;
+    for (int k = 0; k < spw; ++k) {
+        const float xs = x(w * spw + k);
+        const uint32_t q = quant_code_q(xs, ql);
+        acc |= (q & mask) << (k * bits);
+    }
+    return acc;
+}
+
+// The output shuffle as four byte tables in LDS: bit 8 b + j of a word goes
+// to bit out_shifts[8 b + j] (shuffle_out_word's map).
+__device__ __forceinline__ void build_out_shuffle(uint32_t (*tab)[256], const int32_t *sh, int tid,
+                                                  int nt) {
+    for (int i = tid; i < 4 * 256; i += nt) {
+        const int b = i >> 8, v = i & 255;
+        uint32_t o = 0;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) o |= ((uint32_t)(v >> j) & 1u) << sh[8 * b + j];
+        tab[b][v] = o;
+    }
+}
+
+__device__ __forceinline__ uint32_t shuffle_out_lut(uint32_t w, const uint32_t (*tab)[256]) {
+    return tab[0][w & 255u] | tab[1][(w >> 8) & 255u] | tab[2][(w >> 16) & 255u] | tab[3][w >> 24];
 }
 
 __global__ void k_quantise_frames(SectorArgs a, const float *in, int64_t nframes, uint8_t *out) {
@@ -624,57 +696,89 @@ __global__ void k_quantise_frames(SectorArgs a, const float *in, int64_t nframes
     const int64_t f = blockIdx.x;
     const float *x = in + f * a.N;
     for (int w = threadIdx.x; w < nwo; w += blockDim.x) {
-        uint32_t word = quant_word([x](int i) { return x[i]; }, a.N, w, a.bit_out, lv,
-                                   nl < 256 ? nl : 256, mask, a.out_sorted != 0);
-        uint32_t sh = shuffle_out_word(word, a.out_shifts);
-        reinterpret_cast<uint32_t *>(out + f * a.bpf_out)[w] = sh;
+        uint32_t word = quant_word([x](int i) { return x[i]; }, a.N, w, a.bit_out, out_qlev(a, lv), mask);
+        reinterpret_cast<uint32_t *>(out + f * a.bpf_out)[w] =
+            a.out_shuffle_ident ? word : shuffle_out_word(word, a.out_shifts);
     }
 }
 
 // The inverse FFT and the quantiser fused for the phased .raw output: the
 // time series never leaves LDS.  Its samples are the floats k_irfft_frames
 // writes (z.x * scale, odd_sign * (z.y * scale)), so the words are the same.
+// Each workgroup takes G frames at a time (T threads each; G = 2 from 128
+// threads per frame, else 256 threads in all) and walks frame groups
+// blockIdx.x, blockIdx.x + gridDim.x, ...: its levels, output shuffle tables
+// (dynamic LDS, only for a shuffled output) and, for M <= 2048, FFT
+// twiddles are staged once per workgroup.
+template <int T>
+constexpr int irq_frames() { return T >= 128 ? 2 : 256 / T; }
+
 template <int E, int T>
-__global__ __launch_bounds__(2 * T) void k_irfft_quantise(SectorArgs a, const float2 *in,
+__global__ __launch_bounds__(irq_frames<T>() * T) void k_irfft_quantise(SectorArgs a, const float2 *in,
                                                           int64_t nframes, uint8_t *out,
                                                           int negate_odd) {
     constexpr int M = E * T;
     constexpr int MP = M + M / 16 + 2;
-    __shared__ float2 zbuf[2][MP];
-    __shared__ float lv[256];
-    const int s = half_index<T>(threadIdx.x);
+    constexpr bool TWL = M <= 2048;   // FFT twiddles in LDS (at most 16 KB)
+    constexpr int G = irq_frames<T>();   // frames per workgroup pass
+    constexpr int NT = G * T;
+    __shared__ float2 zbuf[G][MP];
+    __shared__ float2 twl[TWL ? M : 1];
+    extern __shared__ uint32_t dyn[];  // levels (256 floats), then the output shuffle tables
+    float *lv = reinterpret_cast<float *>(dyn);
+    uint32_t(*oshuf)[256] = reinterpret_cast<uint32_t(*)[256]>(dyn + 256);
+    const int s = T >= 64 ? __builtin_amdgcn_readfirstlane((int)threadIdx.x / T) : (int)threadIdx.x / T;
     const int t = threadIdx.x - s * T;
-    const int64_t f = 2 * (int64_t)blockIdx.x + s;
-    const bool live = f < nframes;
     const int nl = a.out_nlevels < 256 ? a.out_nlevels : 256;
-    for (int i = threadIdx.x; i < nl; i += 2 * T) lv[i] = a.out_levels[i];
-    float2 v[E];
-    if (live) irfft_prepare<E, T>(in + f * (M + 1), M, a.tw_post, v, t);
-    else
-        for (int r = 0; r < E; ++r) v[r] = make_float2(0.f, 0.f);
-    fft_forward<E, T>(zbuf[s], v, a.tw_fft, t);   // ends with a barrier: zbuf and lv complete
+    for (int i = threadIdx.x; i < nl; i += NT) lv[i] = a.out_levels[i];
+    const bool ident = a.out_shuffle_ident != 0;
+    if (!ident) build_out_shuffle(oshuf, a.out_shifts, threadIdx.x, NT);
+    if constexpr (TWL)
+        for (int i = threadIdx.x; i < M; i += NT) twl[i] = a.tw_fft[i];
+    const float2 *tw = TWL ? twl : a.tw_fft;
     const float scale = 1.0f / (float)(2 * M);
     const float odd_sign = negate_odd ? 1.0f : -1.0f;   // main.rs:2290-2292
     const float2 *zs = zbuf[s];
     const int nwo = a.bpf_out / 4;
     const uint32_t mask = (a.bit_out >= 32) ? 0xFFFFFFFFu : ((1u << a.bit_out) - 1u);
-    if (live)
-        for (int w = t; w < nwo; w += T) {
-            const uint32_t word = quant_word(
-                [zs, scale, odd_sign](int i) {
-                    const float2 z = zs[pad(i >> 1)];
-                    return (i & 1) ? odd_sign * (z.y * scale) : z.x * scale;
-                },
-                2 * M, w, a.bit_out, lv, nl, mask, a.out_sorted != 0);
-            reinterpret_cast<uint32_t *>(out + f * a.bpf_out)[w] = shuffle_out_word(word, a.out_shifts);
-        }
+    const bool aligned = (32 % a.bit_out) == 0;
+    const QLev ql = out_qlev(a, lv);
+    auto sample = [zs, scale, odd_sign](int i) {
+        const float2 z = zs[pad(i >> 1)];
+        return (i & 1) ? odd_sign * (z.y * scale) : z.x * scale;
+    };
+    const int64_t ngroups = (nframes + G - 1) / G;
+    for (int64_t fp = blockIdx.x; fp < ngroups; fp += gridDim.x) {
+        const int64_t f = G * fp + s;
+        const bool live = f < nframes;
+        const int tl = loop_local(t);   // keeps per-thread addresses out of loop-invariant registers
+        float2 v[E];
+        if (live) irfft_prepare<E, T>(in + f * (M + 1), M, a.tw_post, v, tl);
+        else
+            for (int r = 0; r < E; ++r) v[r] = make_float2(0.f, 0.f);
+        fft_forward<E, T>(zbuf[s], v, tw, tl);   // ends with a barrier: zbuf and the tables complete
+        if (live)
+            for (int w = tl; w < nwo; w += T) {
+                const uint32_t word = aligned ? quant_word_aligned(sample, w, a.bit_out, ql, mask)
+                                              : quant_word(sample, 2 * M, w, a.bit_out, ql, mask);
+                reinterpret_cast<uint32_t *>(out + f * a.bpf_out)[w] =
+                    ident ? word : shuffle_out_lut(word, oshuf);
+            }
+        __syncthreads();   // zbuf is rewritten by the next group's FFT
+    }
 }
 
 template <int E, int T>
 static hipError_t launch_irfft_quantise_t(const SectorArgs &a, const float2 *in, int64_t nframes,
                                           uint8_t *out, bool negate_odd, hipStream_t s) {
-    hipLaunchKernelGGL((k_irfft_quantise<E, T>), dim3((unsigned)((nframes + 1) / 2)), dim3(2 * T), 0,
-                       s, a, in, nframes, out, negate_odd ? 1 : 0);
+    // at most 2^22 threads (several resident rounds on 256 CUs); each
+    // workgroup walks groups of G frames
+    constexpr int G = irq_frames<T>();
+    const int64_t ngroups = (nframes + G - 1) / G;
+    const int64_t cap = ((int64_t)1 << 22) / (G * T);
+    const size_t dyn = (size_t)(256 + (a.out_shuffle_ident ? 0 : 4 * 256)) * 4;
+    hipLaunchKernelGGL((k_irfft_quantise<E, T>), dim3((unsigned)std::min<int64_t>(ngroups, cap)),
+                       dim3(G * T), dyn, s, a, in, nframes, out, negate_odd ? 1 : 0);
     return hipGetLastError();
 }
 
@@ -722,8 +826,9 @@ __global__ __launch_bounds__(256) void k_quantise_cplx(SectorArgs a, const float
             const float2 v = zf[i >> 1];
             return (i & 1) ? odd_sign * (v.y * scale) : v.x * scale;
         },
-        a.N, w, a.bit_out, lv, nl, mask, a.out_sorted != 0);
-    reinterpret_cast<uint32_t *>(out + f * a.bpf_out)[w] = shuffle_out_word(word, a.out_shifts);
+        a.N, w, a.bit_out, out_qlev(a, lv), mask);
+    reinterpret_cast<uint32_t *>(out + f * a.bpf_out)[w] =
+        a.out_shuffle_ident ? word : shuffle_out_word(word, a.out_shifts);
 }
 
 hipError_t launch_quantise_cplx(const SectorArgs &a, const float2 *z, int64_t nframes,

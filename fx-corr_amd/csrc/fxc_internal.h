@@ -70,9 +70,12 @@ struct SectorArgs {
     int32_t bit_out;
     int32_t output_lsb;
     int32_t out_shifts[32];  // station-1 shuffle used on output words
+    int32_t out_shuffle_ident;  // 1: out_shifts is the identity
     const float *out_levels; // station-1 levels (f32 as the reference compares)
     int32_t out_nlevels;
     int32_t out_sorted;      // 1: out_levels non-decreasing (nearest level by bisection)
+    int32_t out_qmode;       // quantiser search: 0 scan, 1 bisection, 2 spacing guess
+    float out_l0, out_inv_step;  // mode 2: first level, 1 / mean level spacing
     int32_t bpf_out;
     uint8_t *raw_out;        // phased .raw frames (nf * bpf_out) or null
     double *part;            // [n_wg][kNumProducts][half]

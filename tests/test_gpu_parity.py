@@ -615,6 +615,7 @@ PHASED_CASES = [
     (1024, (2, 2), 0, 0, False),
     (1024, (8, 1), 1, 0, True),        # chosen: the producer/consumer MFMA kernel, other grid
     (1024, (4, 4), 0, 8, False),       # band shift: the chosen form on the Stockham kernel
+    (1024, (3, 2), 1, 0, True),        # 3-bit output: codes straddle the output words
     (512, (4, 2), 1, 8, True),
     (2048, (2, 1), 0, -16, False),
     (4096, (8, 2), 0, 64, True),
@@ -684,7 +685,7 @@ def test_phased_chosen_vs_fp64_oracle(n, bits, grid, band_shift, lsb_out):
     _phased_case(n, bits, grid, band_shift, lsb_out, F.PRECISION_CHOSEN)
 
 
-PHASED_MF_CHUNK = (512 << 20) // (513 * 8)   # frames per .raw chunk of the N = 1024 MFMA kernel
+PHASED_MF_CHUNK = (512 << 20) // (513 * 8)   # most frames per .raw chunk of the N = 1024 MFMA kernel
 
 
 @pytest.mark.parametrize("nf,bits,grid,band,lsb_out", [
@@ -739,8 +740,10 @@ def test_phased_chosen_mfma_kernel(nf, bits, grid, band, lsb_out):
     assert e < CHOSEN_TOL
     assert np.all(got_acc[h:] == 0)
     bpf_o = n * bit_out // 8
+    nchunks = -(-nf // PHASED_MF_CHUNK)
+    edge = -(-nf // nchunks)                           # equal chunks: the first boundary
     frames = sorted(set(range(0, nf, max(1, nf // 200))) |
-                    set(range(max(0, PHASED_MF_CHUNK - 40), min(nf, PHASED_MF_CHUNK + 40))))
+                    set(range(max(0, edge - 40), min(nf, edge + 40))))
     dp = out_decode_plan(bit_out, sh1, lv[0])
     diff = 0
     for f in frames:
