@@ -148,7 +148,7 @@ struct fxc_plan {
     int32_t decode_ok[2] = {1, 1};
     // device tables
     DevBuf tw_fft, tw_post, lev_dev[2], out_lev, mf_tab;
-    int fft_form = 0;  // 0: FP32 Stockham, 1: FP16/FP32-MFMA four-step (N = 1024, 2048),
+    int fft_form = 0;  // 0: FP32 Stockham, 1: FP16/FP32-MFMA four-step (N = 256..2048),
                        // 2: four-step through HBM scratch (N = 32768, 65536)
     bool mf_b1 = false;   // fft_form 0 / 2: mf_tab holds the DFT-32 operand table (scale 1)
                           // and the FFT's first pass runs on FP16 MFMA
@@ -593,7 +593,9 @@ static bool levels_fp16_exact(const fxc_plan *p) {
 }
 
 static bool mfma_eligible(const fxc_plan *p) {
-    if ((p->N != 1024 && p->N != 2048) || p->desc.precision != FXC_PRECISION_CHOSEN) return false;
+    if ((p->N != 256 && p->N != 512 && p->N != 1024 && p->N != 2048) ||
+        p->desc.precision != FXC_PRECISION_CHOSEN)
+        return false;
     // phased plans: the producer/consumer kernel with a phased consumer, N = 1024
     if ((p->desc.products & (FXC_PRODUCT_PHASED | FXC_PRODUCT_RAW_OUT)) && p->N != 1024) return false;
     if (p->desc.a1s != p->desc.a2s) return false;
@@ -699,18 +701,23 @@ static std::vector<uint8_t> mfma_tables(const fxc_plan *p) {
                 const double v = rt == 0 ? (pin == 0 ? cs : sn) : (pin == 0 ? -sn : cs);
                 putf(MFT_A2 + ((size_t)(rt * 8 + b) * 64 + l) * 4, (float)v);
             }
+    // N = 256 / 512: the packed kernel's 16 tile rows are rho = R2 q + m2 for
+    // frame q of the slot (R2 = N / 64 rows per frame, M = 32 R2), so the
+    // twiddle of row rho is W_M^{(rho % R2) k1}; the untangle twiddles are
+    // W_N^p of the frame's pairs
+    const int M = p->N / 2, R2 = M / 32;
     for (int ct = 0; ct < 2 && !wide; ++ct)
         for (int i = 0; i < 4; ++i)
             for (int l = 0; l < 64; ++l) {
-                const int m2 = 4 * (l >> 4) + i, k1 = (l & 15) + 16 * ct;
-                const double th = -2.0 * M_PI * (double)((m2 * k1) % 512) / 512.0;
+                const int rho = 4 * (l >> 4) + i, m2 = rho % R2, k1 = (l & 15) + 16 * ct;
+                const double th = -2.0 * M_PI * (double)((m2 * k1) % M) / (double)M;
                 const size_t idx = (size_t)(ct * 4 + i) * 2;
                 putf(MFT_TW2 + (idx * 64 + l) * 4, (float)std::cos(th));
                 putf(MFT_TW2 + ((idx + 1) * 64 + l) * 4, (float)std::sin(th));
             }
     for (int j = 0; j < 4 && !wide; ++j)
         for (int l = 0; l < 64; ++l) {
-            const double th = -2.0 * M_PI * (double)(l + 64 * j) / 1024.0;
+            const double th = -2.0 * M_PI * (double)(l + 64 * j) / (double)p->N;
             putf(MFT_TWU + ((size_t)(2 * j) * 64 + l) * 4, (float)std::cos(th));
             putf(MFT_TWU + ((size_t)(2 * j + 1) * 64 + l) * 4, (float)std::sin(th));
         }

@@ -139,17 +139,14 @@ __device__ __forceinline__ void mf_load(const uint8_t *frame, int nwords, int l,
         if (l + 64 * u < nwords) w[u] = __builtin_nontemporal_load(src + l + 64 * u);
 }
 
-// Decode this lane's words into FP16 samples at their frame positions
-// (utils.rs:215-376: LSB-first codes, levels, odd samples negated for LSB
-// stations -- the sign is folded into the per-station byte table).
-__device__ __forceinline__ void mf_stage(const uint32_t w[4], int bits, int kind, int nwords,
-                                         const uint32_t *slut, const uint16_t *dlut,
-                                         uint16_t *frame, int l) {
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-        const int wi = l + 64 * u;
-        if (wi >= nwords) continue;
-        const uint32_t v = mf_shuffle(w[u], kind, slut);
+// One packed word w (word wi of its frame) decoded into FP16 samples at their
+// frame positions (utils.rs:215-376: LSB-first codes, levels, odd samples
+// negated for LSB stations -- the sign is folded into the per-station byte
+// table).
+__device__ __forceinline__ void mf_stage_word(uint32_t w, int bits, int kind, const uint32_t *slut,
+                                              const uint16_t *dlut, uint16_t *frame, int wi) {
+    {
+        const uint32_t v = mf_shuffle(w, kind, slut);
         const uint32_t b0 = v & 255u, b1 = (v >> 8) & 255u, b2 = (v >> 16) & 255u, b3 = v >> 24;
         if (bits == 2) {
             const uint2 e0 = *reinterpret_cast<const uint2 *>(dlut + 4 * b0);
@@ -172,6 +169,17 @@ __device__ __forceinline__ void mf_stage(const uint32_t w[4], int bits, int kind
             const uint32_t h0 = dlut[b0], h1 = dlut[256 + b1], h2 = dlut[b2], h3 = dlut[256 + b3];
             *reinterpret_cast<uint2 *>(frame + 4 * wi) = make_uint2(h0 | (h1 << 16), h2 | (h3 << 16));
         }
+    }
+}
+
+// This lane's words (l + 64 u) of one station-frame.
+__device__ __forceinline__ void mf_stage(const uint32_t w[4], int bits, int kind, int nwords,
+                                         const uint32_t *slut, const uint16_t *dlut,
+                                         uint16_t *frame, int l) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int wi = l + 64 * u;
+        if (wi < nwords) mf_stage_word(w[u], bits, kind, slut, dlut, frame, wi);
     }
 }
 
@@ -634,6 +642,9 @@ __global__ __launch_bounds__(512) void k_sector_mf2(SectorArgs a) {
 namespace mf {
 constexpr int PROD = 8, CONS = 4;
 constexpr int THREADS3 = 64 * (PROD + CONS);
+// staging row of the producer/consumer kernels: one N = 1024 station-frame
+// with its padding (SROW halves), or four N = 256 frames with theirs
+constexpr int SROW3 = 1280;
 }  // namespace mf
 constexpr uint32_t kMfSpinLimit = 1u << 22;
 
@@ -644,7 +655,7 @@ struct MfShared3 {
     float twu[8 * 64];
     uint32_t slut[2][4 * 256];
     uint16_t dlut[2][MFT_DLUT_HALVES];
-    uint16_t stage[mf::PROD][2][mf::SROW];     // producers' staging rows
+    uint16_t stage[mf::PROD][2][mf::SROW3];    // producers' staging rows
     alignas(16) float2 z[mf::PROD][2][mf::M];  // Z/2 slot per producer (16-byte stores)
     int32_t prod[mf::PROD], cons[mf::PROD];
     int32_t broken;
@@ -749,12 +760,12 @@ __device__ void mf3_producer(const SectorArgs &a, MfShared3 &sh, int p, int l, i
 // FULL: the overlap is bins 0..M-1 and every ACF bin is wanted, so the only
 // bin outside the overlap is M (lane 0's slot 1): per-lane factors k11 / k22
 // / kM (0 there, else 1) replace the slot masks.
-template <bool FULL>
+// M: points per frame, NP: bin pairs per lane (p = l + 64 j, j < NP).
+template <bool FULL, int M = mf::M, int NP = 4>
 __device__ __forceinline__ void mf_mac(const float2 *z1, const float2 *z2, const float *twu,
                                        const MfFrame &pf, int jf, int l, bool do_acf, bool do_xcf,
                                        uint32_t m11, uint32_t m22, uint32_t m_ov, float k11,
-                                       float k22, float kM, double (&acc)[8][4]) {
-    constexpr int M = mf::M;
+                                       float k22, float kM, double (&acc)[2 * NP][4]) {
     const double phw = readlane_d(pf.phw, jf);
     const double dth = readlane_d(pf.dth, jf);
     const float2 F1 = make_float2(readlane_f(pf.F1.x, jf), readlane_f(pf.F1.y, jf));
@@ -762,7 +773,7 @@ __device__ __forceinline__ void mf_mac(const float2 *z1, const float2 *z2, const
     const float r0 = (float)wrap_turns(fma((double)l, dth, phw));
     float2 P = make_float2(__builtin_amdgcn_cosf(r0), __builtin_amdgcn_sinf(r0));
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
+    for (int j = 0; j < NP; ++j) {
         const int p = l + 64 * j;
         const int pm = (M - p) & (M - 1);
         const float2 w = make_float2(twu[(2 * j) * 64 + l], twu[(2 * j + 1) * 64 + l]);
@@ -1027,7 +1038,7 @@ __device__ __forceinline__ void mf3_setup(const SectorArgs &a, MfShared3 &sh, in
         for (int i = tid; i < 2 * MFT_DLUT_HALVES / 2; i += NT)
             reinterpret_cast<uint32_t *>(&sh.dlut[0][0])[i] = gd[i];
         uint32_t *st32 = reinterpret_cast<uint32_t *>(&sh.stage[0][0][0]);
-        for (int i = tid; i < mf::PROD * 2 * mf::SROW / 2; i += NT) st32[i] = 0u;
+        for (int i = tid; i < mf::PROD * 2 * mf::SROW3 / 2; i += NT) st32[i] = 0u;
         const float *ga = reinterpret_cast<const float *>(tab + MFT_A2);
         const float *gt = reinterpret_cast<const float *>(tab + MFT_TW2);
         const float *gu = reinterpret_cast<const float *>(tab + MFT_TWU);
@@ -1121,6 +1132,398 @@ __global__ __launch_bounds__(mf::THREADS3) void k_sector_mf3(SectorArgs a) {
     double *out = a.part + (size_t)blockIdx.x * kNumProducts * H;
     const double bad = sh.broken ? __builtin_nan("") : 0.0;   // NaN marks a failed hand-over
     if (sh.broken && tid == 0 && a.err)   // and the plan's error word reports it to the host
+        __hip_atomic_fetch_or(a.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int k = tid; k < H; k += NT) {
+        out[P_PH * H + k] = bad;
+        out[P_11 * H + k] = red[0 * H + k] + bad;
+        out[P_22 * H + k] = red[1 * H + k] + bad;
+        out[P_12R * H + k] = red[2 * H + k] + bad;
+        out[P_12I * H + k] = red[3 * H + k] + bad;
+    }
+}
+
+// ------------------------------- N = 256 / 512: packed station-frames
+// The producer/consumer kernel for FFT lengths N' = 1024 / FPS (FPS = 4:
+// N' = 256; FPS = 2: N' = 512).  A producer slot holds FPS consecutive
+// frames of each station, whose M' = 32 R2 points (R2 = 16 / FPS, m = R2 m1
+// + m2) fill the 16 rows of one DFT-32 tile: row rho = R2 q + m2 for frame
+// q.  Stage 1 is k_sector_mf3's (the same operands and MFMAs); the plan's
+// twiddle table holds W_M'^{m2 k1}; stage 2 is a DFT-R2 over m2 per frame in
+// each column lane, which writes Z'/2 of frame q at slot offset q M'.  The
+// consumers take the frames of a slot in turn: lane l the bin pairs
+// (p, M' - p), p = l + 64 j (j < M' / 128), with k_sector_mf3's products,
+// and bin M'/2 once per 64-frame batch.
+namespace mfs {
+template <int FPS>
+struct Shape {
+    static constexpr int NF = mf::N / FPS;     // samples per frame
+    static constexpr int MF = mf::M / FPS;     // complex points per frame
+    static constexpr int R2 = 16 / FPS;        // rows m2 per frame
+    static constexpr int PAD = 32;             // zero samples each side of a staged frame
+    static constexpr int SUB = NF + 2 * PAD;   // halves per staged frame
+    static constexpr int NP = MF / 128;        // bin pairs per lane per frame
+    static_assert(FPS * SUB <= mf::SROW3, "staging row");
+};
+}  // namespace mfs
+
+// Bins of lane l: slot 2j -> p = l + 64 j, slot 2j + 1 -> M' - p; slot 2 NP
+// -> M'/2.
+template <int FPS>
+__device__ __forceinline__ int mfs_bin(int slot, int l) {
+    using S = mfs::Shape<FPS>;
+    if (slot == 2 * S::NP) return S::MF / 2;
+    const int p = l + 64 * (slot >> 1);
+    return (slot & 1) ? S::MF - p : p;
+}
+
+// A operands of a packed slot: lane l supplies row rho = l % 16 (frame q =
+// rho / R2, m2 = rho % R2) with that frame's shift sh: sample 2(R2 m1 + m2)
+// + e of frame q, m1 = 16 kap + g + 4 j, staged at q SUB + PAD + 2(R2 m1 +
+// m2) - sh.  Shifts within the padding read each FP16 pair by a funnel shift
+// of two dwords; a lane with a larger shift checks every sample against its
+// frame.
+template <int FPS>
+__device__ __forceinline__ void mfs_build_a(const uint16_t *row, int sh, int l, half8 &a0, half8 &a1) {
+    using S = mfs::Shape<FPS>;
+    constexpr int R2 = S::R2, OK = 16 * R2, OJ = 4 * R2;   // dword offsets of K-step kap and slot pair j
+    const int rho = l & 15, q = rho / R2, m2 = rho % R2;
+    uint32_t d[8];
+    if (sh >= -S::PAD && sh <= S::PAD) {
+        const int base = q * S::SUB + S::PAD + 2 * m2 - sh + 2 * R2 * (l >> 4);   // halves, >= 0
+        const uint32_t *p0 = reinterpret_cast<const uint32_t *>(row) + (base >> 1);
+        const uint32_t bs = (uint32_t)(base & 1) * 2u;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int o = OK * (k >> 2) + OJ * (k & 3);
+            d[k] = __builtin_amdgcn_alignbyte(p0[o + 1], p0[o], bs);
+        }
+    } else {
+        const uint16_t *fr = row + q * S::SUB + S::PAD;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int m1 = 16 * (k >> 2) + (l >> 4) + 4 * (k & 3);
+            const int s0 = 2 * (R2 * m1 + m2) - sh;
+            const uint32_t lo = (unsigned)s0 < (unsigned)S::NF ? (uint32_t)fr[s0] : 0u;
+            const uint32_t hi = (unsigned)(s0 + 1) < (unsigned)S::NF ? (uint32_t)fr[s0 + 1] : 0u;
+            d[k] = lo | (hi << 16);
+        }
+    }
+    a0 = __builtin_bit_cast(half8, make_uint4(d[0], d[1], d[2], d[3]));
+    a1 = __builtin_bit_cast(half8, make_uint4(d[4], d[5], d[6], d[7]));
+}
+
+// DFT-8 of x[0..7] (radix 2 x 4: even and odd DFT-4s, W8 twiddles).
+__device__ __forceinline__ void mf_dft8(const float2 (&x)[8], float2 (&o)[8]) {
+    float2 e[4], d[4];
+    mf_dft4(x[0], x[2], x[4], x[6], e[0], e[1], e[2], e[3]);
+    mf_dft4(x[1], x[3], x[5], x[7], d[0], d[1], d[2], d[3]);
+    constexpr float c = 0.70710678118654752f;
+    const float2 t1 = make_float2(c * (d[1].x + d[1].y), c * (d[1].y - d[1].x));    // W8^1 = (1 - i)/sqrt2
+    const float2 t2 = make_float2(d[2].y, -d[2].x);                                 // W8^2 = -i
+    const float2 t3 = make_float2(c * (d[3].y - d[3].x), -c * (d[3].x + d[3].y));   // W8^3 = -(1 + i)/sqrt2
+    const float2 t[4] = {d[0], t1, t2, t3};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        o[k] = make_float2(e[k].x + t[k].x, e[k].y + t[k].y);
+        o[k + 4] = make_float2(e[k].x - t[k].x, e[k].y - t[k].y);
+    }
+}
+
+// Stage 2 of a packed slot: lane l takes column k1 = l & 31 of station
+// l >> 5 (its 16 rows from the transposed copy) and runs a DFT-R2 over m2
+// for each frame q, writing Z'/2[q M' + k1 + 32 k2].
+template <int FPS>
+__device__ __forceinline__ void mfs_columns(float2 *slot, int l) {
+    using S = mfs::Shape<FPS>;
+    const int s = l >> 5, k1 = l & 31;
+    float2 x[16];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const float4 v = *reinterpret_cast<const float4 *>(slot + mf_tidx(s, k1, j));
+        x[2 * j] = make_float2(v.x, v.y);
+        x[2 * j + 1] = make_float2(v.z, v.w);
+    }
+    wave_lds_sync();   // every lane's column is read before Z overwrites the slot
+    float2 *zo = slot + s * mf::M + k1;
+#pragma unroll
+    for (int q = 0; q < FPS; ++q) {
+        if constexpr (S::R2 == 4) {
+            float2 o0, o1, o2, o3;
+            mf_dft4(x[4 * q], x[4 * q + 1], x[4 * q + 2], x[4 * q + 3], o0, o1, o2, o3);
+            zo[q * S::MF] = o0;
+            zo[q * S::MF + 32] = o1;
+            zo[q * S::MF + 64] = o2;
+            zo[q * S::MF + 96] = o3;
+        } else {
+            const float2 xi[8] = {x[8 * q], x[8 * q + 1], x[8 * q + 2], x[8 * q + 3],
+                                  x[8 * q + 4], x[8 * q + 5], x[8 * q + 6], x[8 * q + 7]};
+            float2 o[8];
+            mf_dft8(xi, o);
+#pragma unroll
+            for (int k2 = 0; k2 < 8; ++k2) zo[q * S::MF + 32 * k2] = o[k2];
+        }
+    }
+}
+
+// Producer p of the packed kernel: frames [f0, f1) in slots of FPS frames
+// (the last slot may hold fewer); the next slot's words are decoded into
+// the staging rows while this one is transformed, as in mf3_producer.
+template <int FPS>
+__device__ void mfs_producer(const SectorArgs &a, MfShared3 &sh, int p, int l, int64_t f0, int64_t f1) {
+    using S = mfs::Shape<FPS>;
+    const int bits1 = a.st[0].bits, bits2 = a.st[1].bits;
+    const int kind1 = a.st[0].shuffle_kind, kind2 = a.st[1].shuffle_kind;
+    const int nw1 = a.st[0].nwords, nw2 = a.st[1].nwords;   // words per frame (powers of two)
+    const int lg1 = __builtin_ctz((unsigned)nw1), lg2 = __builtin_ctz((unsigned)nw2);
+    const int64_t bpf1 = a.st[0].bpf, bpf2 = a.st[1].bpf;
+    uint16_t *row1 = sh.stage[p][0];
+    uint16_t *row2 = sh.stage[p][1];
+    const int64_t n = f1 - f0;
+    const int nslots = (int)((n + FPS - 1) / FPS);
+    auto nvalid = [&](int i) { return (int)((n - (int64_t)FPS * i) < FPS ? (n - (int64_t)FPS * i) : FPS); };
+    auto load = [&](int i, uint32_t (&w1)[4], uint32_t (&w2)[4]) {
+        const int64_t fs = f0 + (int64_t)FPS * i;
+        const int nv = nvalid(i);
+        mf_load(a.raw1 + fs * bpf1, nv * nw1, l, w1);
+        mf_load(a.raw2 + fs * bpf2, nv * nw2, l, w2);
+    };
+    auto stage = [&](int i, const uint32_t (&w1)[4], const uint32_t (&w2)[4]) {
+        const int nv = nvalid(i);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int wi = l + 64 * u;
+            if (wi < nv * nw1) {
+                const int q = wi >> lg1;
+                mf_stage_word(w1[u], bits1, kind1, sh.slut[0], sh.dlut[0], row1 + q * S::SUB + S::PAD,
+                              wi - (q << lg1));
+            }
+            if (wi < nv * nw2) {
+                const int q = wi >> lg2;
+                mf_stage_word(w2[u], bits2, kind2, sh.slut[1], sh.dlut[1], row2 + q * S::SUB + S::PAD,
+                              wi - (q << lg2));
+            }
+        }
+    };
+    uint32_t cur1[4] = {0, 0, 0, 0}, cur2[4] = {0, 0, 0, 0};
+    bool ok = true;
+    if (nslots > 0) {
+        load(0, cur1, cur2);
+        stage(0, cur1, cur2);
+        if (nslots > 1) load(1, cur1, cur2);
+        wave_lds_sync();
+    }
+    const int qf = (l & 15) / S::R2;   // this lane's frame within a slot
+    for (int64_t fb = f0; fb < f1; fb += mf::FB) {
+        const int nb = (int)((f1 - fb) < mf::FB ? (f1 - fb) : mf::FB);
+        MfFrame pf = {};
+        if (l < nb) pf = mf_frame(a.first_frame + fb + l, a);
+        const int ns = (nb + FPS - 1) / FPS;
+        for (int js = 0; js < ns; ++js) {
+            const int i = (int)((fb - f0) / FPS) + js;
+            uint32_t nxt1[4] = {0, 0, 0, 0}, nxt2[4] = {0, 0, 0, 0};
+            if (i + 2 < nslots) load(i + 2, nxt1, nxt2);
+            int s1 = 0, s2 = 0;
+#pragma unroll
+            for (int q = 0; q < FPS; ++q) {
+                const int jf = (js * FPS + q) < nb ? js * FPS + q : nb - 1;
+                const int t1 = __builtin_amdgcn_readlane(pf.s1, jf);
+                const int t2 = __builtin_amdgcn_readlane(pf.s2, jf);
+                if (qf == q) {
+                    s1 = t1;
+                    s2 = t2;
+                }
+            }
+            half8 a10, a11, a20, a21;
+            mfs_build_a<FPS>(row1, s1, l, a10, a11);
+            mfs_build_a<FPS>(row2, s2, l, a20, a21);
+#pragma unroll 1
+            for (int h = 0; h < 2; ++h) {
+                f4 y1[2], y2[2];   // tiles h (re) and 2 + h (im)
+#pragma unroll
+                for (int q = 0; q < 2; ++q) {
+                    const int t = h + 2 * q;
+                    const half8 bh0 = ld_b1(sh.b1, t * 2 + 0, l), bl0 = ld_b1(sh.b1, t * 2 + 1, l);
+                    const half8 bh1 = ld_b1(sh.b1, (4 + t) * 2 + 0, l), bl1 = ld_b1(sh.b1, (4 + t) * 2 + 1, l);
+                    y1[q] = mf_stage1(a10, a11, bh0, bl0, bh1, bl1);
+                    y2[q] = mf_stage1(a20, a21, bh0, bl0, bh1, bl1);
+                }
+                float2 tw[4];
+                mf_load_tw(sh.tw2, h, l, tw);
+                // the slot was last read by the consumer for slot i - 1
+                if (h == 0) mf_wait_ge(&sh.cons[p], i, ok, &sh.broken);
+                mf_twiddle_store_half(y1[0], y1[1], tw, &sh.z[p][0][0], 0, h, l);
+                mf_twiddle_store_half(y2[0], y2[1], tw, &sh.z[p][0][0], 1, h, l);
+            }
+            wave_lds_sync();
+            mfs_columns<FPS>(&sh.z[p][0][0], l);
+            mf_publish(&sh.prod[p], i + 1);
+            if (i + 1 < nslots) {   // decode slot i + 1 (the staging rows are this wave's own)
+                wave_lds_sync();
+                stage(i + 1, cur1, cur2);
+            }
+            wave_lds_sync();
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                cur1[u] = nxt1[u];
+                cur2[u] = nxt2[u];
+            }
+        }
+    }
+}
+
+// Consumer c: the slots of producers c and c + 4, alternating, each slot's
+// frames in turn.
+template <int FPS, bool FULL>
+__device__ void mfs_consumer(const SectorArgs &a, MfShared3 &sh, int c, int l, int64_t fa0,
+                             int64_t fa1, int64_t fb0, int64_t fb1,
+                             double (&acc)[2 * mfs::Shape<FPS>::NP][4], double (&macc)[4]) {
+    using S = mfs::Shape<FPS>;
+    constexpr int NP = S::NP, MF = S::MF;
+    const bool do_acf = (a.flags & FLAG_ACF) != 0;
+    const bool do_xcf = (a.flags & FLAG_XCF) != 0;
+    const bool ov_only = (a.flags & FLAG_ACF_OVERLAP) != 0;
+    const int gs = a.a1s, ge = a.a1s + a.ov;   // a1s == a2s on this path
+    uint32_t m_grid = 0, m_ov = 0;
+#pragma unroll
+    for (int s = 0; s <= 2 * NP; ++s) {
+        const int k = mfs_bin<FPS>(s, l);
+        const bool in_ov = k >= gs && k < ge;
+        if (!ov_only || in_ov) m_grid |= 1u << s;
+        if (in_ov) m_ov |= 1u << s;
+    }
+    const uint32_t m11 = a.grid == 0 ? m_grid : m_ov;
+    const uint32_t m22 = a.grid == 0 ? m_ov : m_grid;
+    const float kM = (l == 0) ? 0.0f : 1.0f;   // FULL: bin M' (lane 0, slot 1) is outside the overlap
+    const float k11 = a.grid == 0 ? 1.0f : kM, k22 = a.grid == 0 ? kM : 1.0f;
+    const bool in11 = (m11 >> (2 * NP)) & 1u, in22 = (m22 >> (2 * NP)) & 1u;
+    const bool mid_ov = (m_ov >> (2 * NP)) & 1u;
+    const int64_t na = fa1 - fa0, nb = fb1 - fb0;
+    const int64_t n = na > nb ? na : nb;
+    const int q2 = c + mf::CONS;
+    bool ok = true;
+    for (int64_t g = 0; g < n; g += mf::FB) {
+        const int cnt = (int)((n - g) < mf::FB ? (n - g) : mf::FB);
+        MfFrame pa = {}, pb = {};
+        if (l < cnt && g + l < na) pa = mf_frame(a.first_frame + fa0 + g + l, a);
+        if (l < cnt && g + l < nb) pb = mf_frame(a.first_frame + fb0 + g + l, a);
+        float2 ua1 = make_float2(0.f, 0.f), ua2 = ua1, ub1 = ua1, ub2 = ua1;   // Z'[M'/2] of frame g + l
+        const int ns = (cnt + FPS - 1) / FPS;
+        for (int js = 0; js < ns; ++js) {
+            const int i = (int)(g / FPS) + js;
+            const int64_t fi = (int64_t)FPS * i;   // the slot's first frame (offset in each range)
+            if (fi < na) {
+                mf_wait_ge(&sh.prod[c], i + 1, ok, &sh.broken);
+#pragma unroll
+                for (int q = 0; q < FPS; ++q) {
+                    if (fi + q < na) {
+                        const int jf = js * FPS + q;
+                        const float2 *z1 = &sh.z[c][0][q * MF], *z2 = &sh.z[c][1][q * MF];
+                        mf_mac<FULL, MF, NP>(z1, z2, sh.twu, pa, jf, l, do_acf, do_xcf, m11, m22, m_ov,
+                                             k11, k22, kM, acc);
+                        const float2 u1 = z1[MF / 2], u2 = z2[MF / 2];
+                        if (l == jf) {
+                            ua1 = u1;
+                            ua2 = u2;
+                        }
+                    }
+                }
+                mf_publish(&sh.cons[c], i + 1);   // this slot's reads are done
+            }
+            if (fi < nb) {
+                mf_wait_ge(&sh.prod[q2], i + 1, ok, &sh.broken);
+#pragma unroll
+                for (int q = 0; q < FPS; ++q) {
+                    if (fi + q < nb) {
+                        const int jf = js * FPS + q;
+                        const float2 *z1 = &sh.z[q2][0][q * MF], *z2 = &sh.z[q2][1][q * MF];
+                        mf_mac<FULL, MF, NP>(z1, z2, sh.twu, pb, jf, l, do_acf, do_xcf, m11, m22, m_ov,
+                                             k11, k22, kM, acc);
+                        const float2 u1 = z1[MF / 2], u2 = z2[MF / 2];
+                        if (l == jf) {
+                            ub1 = u1;
+                            ub2 = u2;
+                        }
+                    }
+                }
+                mf_publish(&sh.cons[q2], i + 1);
+            }
+        }
+        if (l < cnt && g + l < na) mf_mid(pa, ua1, ua2, do_acf, do_xcf, in11, in22, mid_ov, macc, MF / 2);
+        if (l < cnt && g + l < nb) mf_mid(pb, ub1, ub2, do_acf, do_xcf, in11, in22, mid_ov, macc, MF / 2);
+    }
+}
+
+template <int FPS>
+__global__ __launch_bounds__(mf::THREADS3) void k_sector_mfs(SectorArgs a) {
+    __shared__ MfShared3 sh;
+    using S = mfs::Shape<FPS>;
+    constexpr int M = S::MF, NS = 2 * S::NP;
+    constexpr int NT = mf::THREADS3;
+    const int tid = threadIdx.x;
+    const int l = tid & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    mf3_setup(a, sh, tid);
+    __syncthreads();
+    const int64_t wg0 = (int64_t)blockIdx.x * a.frames_per_wg;
+    int64_t wg1 = wg0 + a.frames_per_wg;
+    if (wg1 > a.nf) wg1 = a.nf;
+    const int64_t per = (a.frames_per_wg + mf::PROD - 1) / mf::PROD;
+    auto range = [&](int p, int64_t &r0, int64_t &r1) {
+        r0 = wg0 + per * p;
+        if (r0 > wg1) r0 = wg1;
+        r1 = (r0 + per < wg1) ? r0 + per : wg1;
+    };
+    double acc[NS][4], macc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[s][c] = 0.0;
+    if (wv < mf::PROD) {
+        int64_t f0, f1;
+        range(wv, f0, f1);
+        mfs_producer<FPS>(a, sh, wv, l, f0, f1);
+    } else {
+        const int c = wv - mf::PROD;
+        int64_t fa0, fa1, fb0, fb1;
+        range(c, fa0, fa1);
+        range(c + mf::CONS, fb0, fb1);
+        const bool full = (a.flags & FLAG_ACF_OVERLAP) == 0 && a.a1s == 0 && a.ov == M;
+        if (full) mfs_consumer<FPS, true>(a, sh, c, l, fa0, fa1, fb0, fb1, acc, macc);
+        else mfs_consumer<FPS, false>(a, sh, c, l, fa0, fa1, fb0, fb1, acc, macc);
+    }
+
+    // ---- workgroup partial: consumers add their bins in a fixed order
+    __syncthreads();
+    double *red = reinterpret_cast<double *>(&sh.z[0][0][0]);   // [4][M' + 1]
+    double *mids = reinterpret_cast<double *>(&sh.z[4][0][0]);  // [CONS][4][64] lanes' bin-M'/2 sums
+    constexpr int H = M + 1;
+    if (wv >= mf::PROD)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) mids[((wv - mf::PROD) * 4 + c) * 64 + l] = macc[c];
+    __syncthreads();
+    for (int w = mf::PROD; w < mf::PROD + mf::CONS; ++w) {
+        if (wv == w) {
+            const bool first = w == mf::PROD;
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                const int k = mfs_bin<FPS>(s, l);
+#pragma unroll
+                for (int c = 0; c < 4; ++c) red[c * H + k] = (first ? 0.0 : red[c * H + k]) + acc[s][c];
+            }
+            if (l == 0)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const double *mc = mids + ((w - mf::PROD) * 4 + c) * 64;
+                    double sum = 0.0;
+                    for (int i = 0; i < 64; ++i) sum += mc[i];   // lanes in order
+                    red[c * H + M / 2] = (first ? 0.0 : red[c * H + M / 2]) + sum;
+                }
+        }
+        __syncthreads();
+    }
+    double *out = a.part + (size_t)blockIdx.x * kNumProducts * H;
+    const double bad = sh.broken ? __builtin_nan("") : 0.0;   // NaN marks a failed hand-over
+    if (sh.broken && tid == 0 && a.err)
         __hip_atomic_fetch_or(a.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     for (int k = tid; k < H; k += NT) {
         out[P_PH * H + k] = bad;
@@ -1794,8 +2197,14 @@ __global__ __launch_bounds__(mf4::THREADS) void k_sector_mf4(SectorArgs a) {
 }
 
 hipError_t launch_sector_mfma(const SectorArgs &a, int precision, int n_wg, hipStream_t s) {
-    if ((a.N != mf::N && a.N != mf4::N) || a.mf_tab == nullptr || precision != 1 || a.a1s != a.a2s)
+    if ((a.N != mf::N && a.N != mf4::N && a.N != mf::N / 4 && a.N != mf::N / 2) || a.mf_tab == nullptr ||
+        precision != 1 || a.a1s != a.a2s)
         return hipErrorInvalidValue;
+    if (a.N == mf::N / 4 || a.N == mf::N / 2) {   // packed frames, one organization
+        if (a.N == mf::N / 4) hipLaunchKernelGGL(k_sector_mfs<4>, dim3(n_wg), dim3(mf::THREADS3), 0, s, a);
+        else hipLaunchKernelGGL(k_sector_mfs<2>, dim3(n_wg), dim3(mf::THREADS3), 0, s, a);
+        return hipGetLastError();
+    }
     if (a.N == mf4::N) {   // one organization at N = 2048
         hipLaunchKernelGGL(k_sector_mf4, dim3(n_wg), dim3(mf4::THREADS), 0, s, a);
         return hipGetLastError();

@@ -125,7 +125,7 @@ typedef struct {
     int32_t bpf1, bpf2, bpf_out;  /* bytes per station-frame (main.rs:1214-1215)    */
     int32_t decode_ok1, decode_ok2; /* 0: every frame fails decode (utils.rs:372)   */
     int32_t fft_form;             /* 0 = FP32 Stockham, 1 = FP16/FP32-MFMA four-step
-                                     (N = 1024, 2048), 2 = four-step through HBM
+                                     (N = 256..2048), 2 = four-step through HBM
                                      (N >= 32768)                                   */
     int32_t threads_per_frame;
     int32_t device_id;

@@ -386,7 +386,11 @@ def test_mfma_eligibility():
     assert F.Plan(1024, levels=lv2, df_hz=1.0, precision=F.PRECISION_CHOSEN).info.fft_mfma == 1
     mid = F.Plan(4096, levels=lv2, df_hz=1.0, precision=F.PRECISION_CHOSEN).info
     assert mid.fft_form == 0 and mid.fft_mfma == 1                      # Stockham, MFMA first pass
-    assert F.Plan(512, levels=lv2, df_hz=1.0, precision=F.PRECISION_CHOSEN).info.fft_mfma == 0
+    for n in (256, 512):                                                  # packed frames per slot
+        small = F.Plan(n, levels=lv2, df_hz=1.0, precision=F.PRECISION_CHOSEN).info
+        assert small.fft_form == 1 and small.fft_mfma == 1
+    assert F.Plan(512, levels=lv2, df_hz=1.0).info.fft_mfma == 0
+    assert F.Plan(128, levels=lv2, df_hz=1.0, precision=F.PRECISION_CHOSEN).info.fft_mfma == 0
     assert F.Plan(4096, levels=lv2, df_hz=1.0).info.fft_mfma == 0
     big = F.Plan(65536, levels=lv2, df_hz=1.0, precision=F.PRECISION_CHOSEN).info
     assert big.fft_form == 2 and big.fft_mfma == 1
@@ -566,6 +570,105 @@ def test_mfma2048_ragged_frame_counts(nf):
     o64 = oracle_run(c, R.MODE_F64)
     _compare(got, o64, CHOSEN_TOL)
     _check_plain(c, gpu_run(c, plan=gpu_plan(c, F.PRECISION_PLAIN_FP32)))
+
+
+# N = 256 / 512: four (two) station-frames packed into one producer slot,
+# their 32-point rows filling one DFT-32 tile; producer/consumer
+# organization only.
+@pytest.mark.parametrize("n", [256, 512])
+@pytest.mark.parametrize("bits", [(4, 1), (1, 2), (8, 2), (2, 2)])
+def test_mfma_packed_sector_mixed_bits(n, bits):
+    fs = 512e6
+    rng = np.random.default_rng(200 + n + sum(bits))
+    c = make_case(n=n, fs=fs, nf=3000, bits=bits, shuffles=(rng.permutation(32), None),
+                  lsb=(True, False), rho=0.5, delay_samples=-3.4, seed=29,
+                  delay_user_samples=-3.4, extra_delay_rate_sps=2e-9, lo1_hz=8.4e9,
+                  lo2_hz=8.4e9)
+    p = gpu_plan(c, F.PRECISION_CHOSEN)
+    assert p.info.fft_form == 1 and p.info.variant == 1
+    _compare(gpu_run(c, plan=p), oracle_run(c, R.MODE_F64), CHOSEN_TOL)
+
+
+@pytest.mark.parametrize("n", [256, 512])
+@pytest.mark.parametrize("grid", [0, 1])
+@pytest.mark.parametrize("overlap_only", [False, True])
+def test_mfma_packed_partial_band_grids(n, grid, overlap_only):
+    fs = 256e6
+    prods = F.PRODUCT_ACF | F.PRODUCT_XCF
+    if overlap_only:
+        prods |= F.PRODUCT_ACF_OVERLAP_ONLY
+    band = (0, 20, n // 2 - 30, 20)
+    c = make_case(n=n, fs=fs, nf=2000, band=band, grid=grid, products=prods, rho=0.5,
+                  delay_samples=2.6, seed=37, delay_user_samples=2.6, lo1_hz=8e9, lo2_hz=8e9)
+    p = gpu_plan(c, F.PRECISION_CHOSEN)
+    assert p.info.fft_form == 1
+    got = gpu_run(c, plan=p)
+    ref = oracle_run(c, R.MODE_F64)
+    _compare(got, ref, CHOSEN_TOL)
+    for k in ("acc11", "acc22", "acc12"):
+        assert zero_pattern_equal(got[k], ref[k]), k
+
+
+@pytest.mark.parametrize("n", [256, 512])
+def test_mfma_packed_large_and_odd_shifts(n):
+    """Integer shifts beyond the staged frames' padding (the checked
+    operand path), beyond +-N (all-zero frames) and of both parities."""
+    fs = 128e6
+    base, rate = (-2.8e-6, 1.3e-3) if n == 256 else (-5e-6, 1.17e-3)
+    c = make_case(n=n, fs=fs, nf=1500, no_table=True, net_d1_base=base, total_rate1_base=rate,
+                  lo1_hz=2e9, seed=9, rho=0.3)
+    d = R.make_delay_cfg(None, **c["delay"])
+    ints = [R.frame_delay(i, d).int1 for i in range(c["nf"])]
+    assert min(ints) < -n and max(ints) > 40 and any(i % 2 for i in ints)
+    assert any(-32 <= i <= 32 for i in ints)
+    p = gpu_plan(c, F.PRECISION_CHOSEN)
+    assert p.info.fft_form == 1
+    _compare(gpu_run(c, plan=p), oracle_run(c, R.MODE_F64), CHOSEN_TOL)
+
+
+@pytest.mark.parametrize("n", [256, 512])
+@pytest.mark.parametrize("nf", [1, 3, 5, 37, 2051])
+def test_mfma_packed_ragged_frame_counts(n, nf):
+    """Frame counts that leave slots partly filled and producers, streams or
+    workgroups without frames; against the plain form and the FP64 oracle."""
+    fs = 1024e6
+    rng = np.random.default_rng(nf + n)
+    c = make_case(n=n, fs=fs, nf=nf, shuffles=(None, rng.permutation(32)), lsb=(False, True),
+                  rho=0.4, delay_samples=1.3, seed=60 + nf, delay_user_samples=1.3,
+                  extra_delay_rate_sps=1e-9, lo1_hz=6.7e9, lo2_hz=6.7e9)
+    p = gpu_plan(c, F.PRECISION_CHOSEN)
+    assert p.info.fft_form == 1
+    _compare(gpu_run(c, plan=p), oracle_run(c, R.MODE_F64), CHOSEN_TOL)
+    _check_plain(c, gpu_run(c, plan=gpu_plan(c, F.PRECISION_PLAIN_FP32)))
+
+
+@pytest.mark.parametrize("n", [256, 512])
+def test_mfma_packed_many_batches(n):
+    """Device-generated stations long enough for several 64-frame batches per
+    producer (and slots that straddle them), against the FP64 oracle."""
+    fs, bits = 1024e6, (2, 4)
+    nf = 150_000 if n == 256 else 80_000
+    sh1 = np.random.default_rng(n).permutation(32).astype(np.int32)
+    s = F.Stream()
+    d1 = F.DeviceBuffer(nf * n * bits[0] // 8)
+    d2 = F.DeviceBuffer(nf * n * bits[1] // 8)
+    F.synth_station_device(d1.ptr, nf * n, bits[0], sh1, False, 91, 0, 0.4, 3, s.ptr)
+    F.synth_station_device(d2.ptr, nf * n, bits[1], None, True, 91, 1, 0.4, 3, s.ptr)
+    raw1 = d1.download(np.zeros(nf * n * bits[0] // 8, dtype=np.uint8), s.ptr)
+    raw2 = d2.download(np.zeros(nf * n * bits[1] // 8, dtype=np.uint8), s.ptr)
+    s.sync()
+    lv = [np.arange(1 << b, dtype=np.float64) - ((1 << b) - 1) / 2.0 for b in bits]
+    table = np.zeros(int(np.ceil(nf * n / fs)) + 2)
+    delay = dict(frame_dt=n / fs, fs=fs, delay_user_samples=2.7, extra_delay_rate_sps=3e-9,
+                 lo1_hz=5e9, lo2_hz=5e9)
+    c = dict(n=n, fs=fs, nf=nf, bits=bits, levels=lv, shuffles=(sh1, IDENT), lsb=(False, True),
+             band=(0, 0, n // 2, 0), grid=0, products=F.PRODUCT_ACF | F.PRODUCT_XCF, raw1=raw1,
+             raw2=raw2, delay=delay, table=table, first_frame=0)
+    p = gpu_plan(c, F.PRECISION_CHOSEN)
+    assert p.info.fft_form == 1
+    _compare(gpu_run(c, plan=p), oracle_run(c, R.MODE_F64), CHOSEN_TOL)
+    d1.free()
+    d2.free()
 
 
 def test_mfma_needs_common_band_start():
