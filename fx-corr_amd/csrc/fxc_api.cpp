@@ -596,8 +596,8 @@ static bool mfma_eligible(const fxc_plan *p) {
     if ((p->N != 256 && p->N != 512 && p->N != 1024 && p->N != 2048) ||
         p->desc.precision != FXC_PRECISION_CHOSEN)
         return false;
-    // phased plans: the producer/consumer kernel with a phased consumer, N = 1024
-    if ((p->desc.products & (FXC_PRODUCT_PHASED | FXC_PRODUCT_RAW_OUT)) && p->N != 1024) return false;
+    // phased plans: the producer/consumer kernel with a phased consumer, N = 256..1024
+    if ((p->desc.products & (FXC_PRODUCT_PHASED | FXC_PRODUCT_RAW_OUT)) && p->N == 2048) return false;
     if (p->desc.a1s != p->desc.a2s) return false;
     for (int i = 0; i < 2; ++i) {
         const int b = p->desc.st[i].bits;

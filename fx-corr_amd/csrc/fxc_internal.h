@@ -156,7 +156,7 @@ hipError_t launch_quantise_frames(const SectorArgs &a, const float *in,
 
 hipError_t launch_sector_mfma(const SectorArgs &a, int precision, int n_wg, hipStream_t stream);
 int mfma_wg_threads();
-// Chosen phased sum at N = 1024 on the producer/consumer MFMA kernel (a1s == a2s)
+// Chosen phased sum at N = 256 / 512 / 1024 on the producer/consumer MFMA kernel (a1s == a2s)
 hipError_t launch_phased_mf3(const SectorArgs &a, int accumulate, int n_wg, hipStream_t stream);
 
 // fxc_phased.hip

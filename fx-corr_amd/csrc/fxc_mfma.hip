@@ -1534,9 +1534,10 @@ __global__ __launch_bounds__(mf::THREADS3) void k_sector_mfs(SectorArgs a) {
     }
 }
 
-// ------------------------------------------- phased sum at N = 1024
+// ------------------------------------ phased sum at N = 256 / 512 / 1024
 // The producer/consumer kernel with a phased-array consumer (chosen form;
-// main.rs:2264-2302, utils.rs:69-124).  The producers are k_sector_mf3's.
+// main.rs:2264-2302, utils.rs:69-124).  The producers are k_sector_mf3's
+// (N = 1024, FPS = 1 frame per slot) or the packed kernel's (N = 256 / 512).
 // Per frame and output bin the consumer takes each station's bin through its
 // fractional-delay rotation exp(2 pi i b th) (th = -df frac turns per bin,
 // from the FP64 delay) and fringe phasor, forms cb = w1 c1 + w2 c2, adds
@@ -1547,7 +1548,7 @@ __global__ __launch_bounds__(mf::THREADS3) void k_sector_mfs(SectorArgs a) {
 struct MfPhFrame {
     double thg, tho;         // rotation per bin, turns: grid station, other station
     float2 frg, fro;         // fringe phasors
-    float2 Sg, So, Cg, Co;   // exp(2 pi i 64 th), exp(2 pi i M th)
+    float2 Sg, So, Cg, Co;   // exp(2 pi i 64 th), exp(2 pi i M th), M = N / 2 of the plan
 };
 
 __device__ __forceinline__ float2 mf_phasor(double turns) {
@@ -1576,8 +1577,8 @@ __device__ MfPhFrame mfp_frame(int64_t idx, const SectorArgs &a) {
     o.fro = g0 ? f2 : f1;
     o.Sg = mf_phasor(64.0 * o.thg);
     o.So = mf_phasor(64.0 * o.tho);
-    o.Cg = mf_phasor((double)mf::M * o.thg);
-    o.Co = mf_phasor((double)mf::M * o.tho);
+    o.Cg = mf_phasor((double)a.M * o.thg);
+    o.Co = mf_phasor((double)a.M * o.tho);
     return o;
 }
 
@@ -1611,11 +1612,12 @@ __device__ __forceinline__ float2 mfp_combine(float2 xg, float2 rg, float2 xo, f
     return cb;
 }
 
-// The phased products of one frame (slot z1 / z2) for lane l's eight bins.
+// The phased products of one frame of M points (z1 / z2) for lane l's bin
+// pairs p = l + 64 j, j < NP.
+template <int M, int NP>
 __device__ __forceinline__ void mfp_mac(const float2 *z1, const float2 *z2, const float *twu,
                                         const MfPhFrame &pf, int jf, int l, int gsn, uint32_t m_ov,
-                                        float w1, float w2, float2 *cbo, double (&acc)[8]) {
-    constexpr int M = mf::M;
+                                        float w1, float w2, float2 *cbo, double (&acc)[2 * NP]) {
     const double thg = readlane_d(pf.thg, jf), tho = readlane_d(pf.tho, jf);
     const float2 frg = readlane_f2(pf.frg, jf), fro = readlane_f2(pf.fro, jf);
     const float2 Sg = readlane_f2(pf.Sg, jf), So = readlane_f2(pf.So, jf);
@@ -1624,7 +1626,7 @@ __device__ __forceinline__ void mfp_mac(const float2 *z1, const float2 *z2, cons
     const float2 *zg = gsn == 0 ? z1 : z2;
     const float2 *zo = gsn == 0 ? z2 : z1;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
+    for (int j = 0; j < NP; ++j) {
         const int p = l + 64 * j;
         const int pm = (M - p) & (M - 1);
         const float2 w = make_float2(twu[(2 * j) * 64 + l], twu[(2 * j + 1) * 64 + l]);
@@ -1647,11 +1649,10 @@ __device__ __forceinline__ void mfp_mac(const float2 *z1, const float2 *z2, cons
     }
 }
 
-// Bin M/2 (X = 2 conj Z'[M/2] on each station) of the frame whose
+// Bin K = M/2 (X = 2 conj Z'[M/2] on each station) of the frame whose
 // parameters this lane holds, formed once per batch as in mf_mid.
 __device__ __forceinline__ void mfp_mid(const MfPhFrame &pf, float2 u1, float2 u2, int gsn, bool in_ov,
-                                        float w1, float w2, float2 *cb_mid, double &macc) {
-    constexpr int K = mf::M / 2;
+                                        float w1, float w2, float2 *cb_mid, double &macc, int K) {
     const float2 x1 = make_float2(2.0f * u1.x, -2.0f * u1.y);
     const float2 x2 = make_float2(2.0f * u2.x, -2.0f * u2.y);
     const float2 xg = gsn == 0 ? x1 : x2, xo = gsn == 0 ? x2 : x1;
@@ -1661,25 +1662,29 @@ __device__ __forceinline__ void mfp_mid(const MfPhFrame &pf, float2 u1, float2 u
     if (cb_mid != nullptr) *cb_mid = cb;
 }
 
-// Consumer c of the phased kernel: the frames of producers c and c + 4,
-// alternating, as mf3_consumer.
+// Consumer c of the phased kernel: the slots of producers c and c + 4,
+// alternating, each slot's FPS frames in turn (as mf3_consumer and
+// mfs_consumer).
+template <int FPS>
 __device__ void mfp_consumer(const SectorArgs &a, MfShared3 &sh, int c, int l, int64_t fa0,
-                             int64_t fa1, int64_t fb0, int64_t fb1, double (&acc)[8], double &macc) {
-    constexpr int H = mf::M + 1;
+                             int64_t fa1, int64_t fb0, int64_t fb1,
+                             double (&acc)[2 * mfs::Shape<FPS>::NP], double &macc) {
+    using S = mfs::Shape<FPS>;
+    constexpr int M = S::MF, NP = S::NP, H = M + 1;
     const int gsn = a.grid;
     const int gs = a.a1s, ge = a.a1s + a.ov;   // a1s == a2s on this path
     uint32_t m_ov = 0;
 #pragma unroll
-    for (int s = 0; s < 9; ++s) {
-        const int k = mf_bin(s, l);
+    for (int s = 0; s <= 2 * NP; ++s) {
+        const int k = mfs_bin<FPS>(s, l);
         if (k >= gs && k < ge) m_ov |= 1u << s;
     }
-    const bool mid_ov = (m_ov >> 8) & 1u;
+    const bool mid_ov = (m_ov >> (2 * NP)) & 1u;
     const float w1 = (float)a.w1, w2 = (float)a.w2;
     float2 *cb = ((a.flags & FLAG_RAW_OUT) != 0) ? a.cb_out : nullptr;
     const int64_t na = fa1 - fa0, nb = fb1 - fb0;
     const int64_t n = na > nb ? na : nb;
-    const int q = c + mf::CONS;
+    const int q2 = c + mf::CONS;
     bool ok = true;
     for (int64_t g = 0; g < n; g += mf::FB) {
         const int cnt = (int)((n - g) < mf::FB ? (n - g) : mf::FB);
@@ -1687,43 +1692,63 @@ __device__ void mfp_consumer(const SectorArgs &a, MfShared3 &sh, int c, int l, i
         if (l < cnt && g + l < na) pa = mfp_frame(a.first_frame + fa0 + g + l, a);
         if (l < cnt && g + l < nb) pb = mfp_frame(a.first_frame + fb0 + g + l, a);
         float2 ua1 = make_float2(0.f, 0.f), ua2 = ua1, ub1 = ua1, ub2 = ua1;   // Z'[M/2] of frame g + l
-        for (int jf = 0; jf < cnt; ++jf) {
-            const int32_t i = (int32_t)(g + jf);
-            if (i < na) {
+        const int ns = (cnt + FPS - 1) / FPS;
+        for (int js = 0; js < ns; ++js) {
+            const int i = (int)(g / FPS) + js;
+            const int64_t fi = (int64_t)FPS * i;   // the slot's first frame (offset in each range)
+            if (fi < na) {
                 mf_wait_ge(&sh.prod[c], i + 1, ok, &sh.broken);
-                mfp_mac(sh.z[c][0], sh.z[c][1], sh.twu, pa, jf, l, gsn, m_ov, w1, w2,
-                        cb ? cb + (fa0 + i) * H : nullptr, acc);
-                const float2 u1 = sh.z[c][0][mf::M / 2], u2 = sh.z[c][1][mf::M / 2];
-                if (l == jf) {
-                    ua1 = u1;
-                    ua2 = u2;
+#pragma unroll
+                for (int q = 0; q < FPS; ++q) {
+                    if (fi + q < na) {
+                        const int jf = js * FPS + q;
+                        const float2 *z1 = &sh.z[c][0][q * M], *z2 = &sh.z[c][1][q * M];
+                        mfp_mac<M, NP>(z1, z2, sh.twu, pa, jf, l, gsn, m_ov, w1, w2,
+                                       cb ? cb + (fa0 + fi + q) * H : nullptr, acc);
+                        const float2 u1 = z1[M / 2], u2 = z2[M / 2];
+                        if (l == jf) {
+                            ua1 = u1;
+                            ua2 = u2;
+                        }
+                    }
                 }
                 mf_publish(&sh.cons[c], i + 1);
             }
-            if (i < nb) {
-                mf_wait_ge(&sh.prod[q], i + 1, ok, &sh.broken);
-                mfp_mac(sh.z[q][0], sh.z[q][1], sh.twu, pb, jf, l, gsn, m_ov, w1, w2,
-                        cb ? cb + (fb0 + i) * H : nullptr, acc);
-                const float2 u1 = sh.z[q][0][mf::M / 2], u2 = sh.z[q][1][mf::M / 2];
-                if (l == jf) {
-                    ub1 = u1;
-                    ub2 = u2;
+            if (fi < nb) {
+                mf_wait_ge(&sh.prod[q2], i + 1, ok, &sh.broken);
+#pragma unroll
+                for (int q = 0; q < FPS; ++q) {
+                    if (fi + q < nb) {
+                        const int jf = js * FPS + q;
+                        const float2 *z1 = &sh.z[q2][0][q * M], *z2 = &sh.z[q2][1][q * M];
+                        mfp_mac<M, NP>(z1, z2, sh.twu, pb, jf, l, gsn, m_ov, w1, w2,
+                                       cb ? cb + (fb0 + fi + q) * H : nullptr, acc);
+                        const float2 u1 = z1[M / 2], u2 = z2[M / 2];
+                        if (l == jf) {
+                            ub1 = u1;
+                            ub2 = u2;
+                        }
+                    }
                 }
-                mf_publish(&sh.cons[q], i + 1);
+                mf_publish(&sh.cons[q2], i + 1);
             }
         }
         if (l < cnt && g + l < na)
-            mfp_mid(pa, ua1, ua2, gsn, mid_ov, w1, w2, cb ? cb + (fa0 + g + l) * H + mf::M / 2 : nullptr, macc);
+            mfp_mid(pa, ua1, ua2, gsn, mid_ov, w1, w2, cb ? cb + (fa0 + g + l) * H + M / 2 : nullptr, macc,
+                    M / 2);
         if (l < cnt && g + l < nb)
-            mfp_mid(pb, ub1, ub2, gsn, mid_ov, w1, w2, cb ? cb + (fb0 + g + l) * H + mf::M / 2 : nullptr, macc);
+            mfp_mid(pb, ub1, ub2, gsn, mid_ov, w1, w2, cb ? cb + (fb0 + g + l) * H + M / 2 : nullptr, macc,
+                    M / 2);
     }
 }
 
 // accumulate = 0: the workgroup's partial slot is set; 1: its |cb|^2 sums
 // are added (a sector in chunks that reuse the slots in order).
+template <int FPS>
 __global__ __launch_bounds__(mf::THREADS3) void k_phased_mf3(SectorArgs a, int accumulate) {
     __shared__ MfShared3 sh;
-    constexpr int M = mf::M;
+    using S = mfs::Shape<FPS>;
+    constexpr int M = S::MF, NS = 2 * S::NP;
     constexpr int NT = mf::THREADS3;
     const int tid = threadIdx.x;
     const int l = tid & 63;
@@ -1739,19 +1764,20 @@ __global__ __launch_bounds__(mf::THREADS3) void k_phased_mf3(SectorArgs a, int a
         if (r0 > wg1) r0 = wg1;
         r1 = (r0 + per < wg1) ? r0 + per : wg1;
     };
-    double acc[8], macc = 0.0;
+    double acc[NS], macc = 0.0;
 #pragma unroll
-    for (int s = 0; s < 8; ++s) acc[s] = 0.0;
+    for (int s = 0; s < NS; ++s) acc[s] = 0.0;
     if (wv < mf::PROD) {
         int64_t f0, f1;
         range(wv, f0, f1);
-        mf3_producer(a, sh, wv, l, f0, f1);
+        if constexpr (FPS == 1) mf3_producer(a, sh, wv, l, f0, f1);
+        else mfs_producer<FPS>(a, sh, wv, l, f0, f1);
     } else {
         const int c = wv - mf::PROD;
         int64_t fa0, fa1, fb0, fb1;
         range(c, fa0, fa1);
         range(c + mf::CONS, fb0, fb1);
-        mfp_consumer(a, sh, c, l, fa0, fa1, fb0, fb1, acc, macc);
+        mfp_consumer<FPS>(a, sh, c, l, fa0, fa1, fb0, fb1, acc, macc);
     }
 
     // ---- workgroup partial: consumers add their bins in a fixed order
@@ -1765,8 +1791,8 @@ __global__ __launch_bounds__(mf::THREADS3) void k_phased_mf3(SectorArgs a, int a
         if (wv == w) {
             const bool first = w == mf::PROD;
 #pragma unroll
-            for (int s = 0; s < 8; ++s) {
-                const int k = mf_bin(s, l);
+            for (int s = 0; s < NS; ++s) {
+                const int k = mfs_bin<FPS>(s, l);
                 red[k] = (first ? 0.0 : red[k]) + acc[s];
             }
             if (l == 0) {
@@ -1794,9 +1820,15 @@ __global__ __launch_bounds__(mf::THREADS3) void k_phased_mf3(SectorArgs a, int a
 }
 
 hipError_t launch_phased_mf3(const SectorArgs &a, int accumulate, int n_wg, hipStream_t s) {
-    if (a.N != mf::N || a.mf_tab == nullptr || a.a1s != a.a2s) return hipErrorInvalidValue;
+    if ((a.N != mf::N && a.N != mf::N / 2 && a.N != mf::N / 4) || a.mf_tab == nullptr || a.a1s != a.a2s)
+        return hipErrorInvalidValue;
     if (n_wg <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_phased_mf3, dim3(n_wg), dim3(mf::THREADS3), 0, s, a, accumulate);
+    if (a.N == mf::N)
+        hipLaunchKernelGGL(k_phased_mf3<1>, dim3(n_wg), dim3(mf::THREADS3), 0, s, a, accumulate);
+    else if (a.N == mf::N / 2)
+        hipLaunchKernelGGL(k_phased_mf3<2>, dim3(n_wg), dim3(mf::THREADS3), 0, s, a, accumulate);
+    else
+        hipLaunchKernelGGL(k_phased_mf3<4>, dim3(n_wg), dim3(mf::THREADS3), 0, s, a, accumulate);
     return hipGetLastError();
 }
 

@@ -715,6 +715,8 @@ def _phased_desc(c, w, bit_out, output_lsb):
 
 
 PHASED_CASES = [
+    (256, (2, 8), 1, 0, True),         # chosen: the packed-frame MFMA kernel
+    (512, (4, 2), 0, 0, False),
     (1024, (2, 2), 0, 0, False),
     (1024, (8, 1), 1, 0, True),        # chosen: the producer/consumer MFMA kernel, other grid
     (1024, (4, 4), 0, 8, False),       # band shift: the chosen form on the Stockham kernel
@@ -788,21 +790,21 @@ def test_phased_chosen_vs_fp64_oracle(n, bits, grid, band_shift, lsb_out):
     _phased_case(n, bits, grid, band_shift, lsb_out, F.PRECISION_CHOSEN)
 
 
-PHASED_MF_CHUNK = (512 << 20) // (513 * 8)   # most frames per .raw chunk of the N = 1024 MFMA kernel
-
-
-@pytest.mark.parametrize("nf,bits,grid,band,lsb_out", [
-    (3000, (4, 2), 1, (0, 16, 400, 16), True),          # partial overlap on a common band start
-    (140_000, (2, 1), 0, (0, 0, 512, 0), False),        # two .raw chunks, 64-frame batches
+@pytest.mark.parametrize("n,nf,bits,grid,band,lsb_out", [
+    (1024, 3000, (4, 2), 1, (0, 16, 400, 16), True),        # partial overlap on a common band start
+    (1024, 140_000, (2, 1), 0, (0, 0, 512, 0), False),      # two .raw chunks, 64-frame batches
+    (256, 5001, (8, 1), 0, (0, 9, 100, 9), True),           # packed frames, partial slots
+    (256, 530_000, (1, 2), 1, (0, 0, 128, 0), False),       # packed frames, two .raw chunks
+    (512, 270_000, (2, 4), 0, (0, 0, 256, 0), True),
 ])
-def test_phased_chosen_mfma_kernel(nf, bits, grid, band, lsb_out):
-    """Chosen phased sum at N = 1024 on the producer/consumer MFMA kernel
-    (both stations on a common band start).  Device-generated stations;
-    |cb|^2 within the chosen bar of the oracle's FP64 phased branch over
-    every frame, and the .raw codes of sampled frames (every frame around
-    the chunk boundary) equal except at rare quantiser decision
+def test_phased_chosen_mfma_kernel(n, nf, bits, grid, band, lsb_out):
+    """Chosen phased sum at N = 256 / 512 / 1024 on the producer/consumer
+    MFMA kernels (both stations on a common band start).  Device-generated
+    stations; |cb|^2 within the chosen bar of the oracle's FP64 phased branch
+    over every frame, and the .raw codes of sampled frames (every frame
+    around the first chunk boundary) equal except at rare quantiser decision
     boundaries."""
-    n, fs = 1024, 256e6
+    fs = 256e6
     sh1 = np.random.default_rng(nf).permutation(32).astype(np.int32)
     s = F.Stream()
     d1 = F.DeviceBuffer(nf * n * bits[0] // 8)
@@ -843,7 +845,8 @@ def test_phased_chosen_mfma_kernel(nf, bits, grid, band, lsb_out):
     assert e < CHOSEN_TOL
     assert np.all(got_acc[h:] == 0)
     bpf_o = n * bit_out // 8
-    nchunks = -(-nf // PHASED_MF_CHUNK)
+    cmax = (512 << 20) // ((n // 2 + 1) * 8)           # most frames per .raw chunk
+    nchunks = -(-nf // cmax)
     edge = -(-nf // nchunks)                           # equal chunks: the first boundary
     frames = sorted(set(range(0, nf, max(1, nf // 200))) |
                     set(range(max(0, edge - 40), min(nf, edge + 40))))
