@@ -315,7 +315,7 @@ FORM_NAMES = {0: "FP32 Stockham", 1: "FP16-MFMA four-step", 2: "FP32 four-step t
 
 def form_name(info):
     """The FFT a plan runs (fxc_plan_info.fft_form / fft_mfma)."""
-    if info.fft_form == 2 and info.fft_mfma and info.variant == 1:
+    if info.fft_mfma and info.variant == 1 and (info.fft_form == 2 or info.fft_len == 16384):
         return "fused four-step: decode + FP16-MFMA DFT-128 columns, row DFTs + products"
     if info.fft_form == 2 and info.fft_mfma:
         return "four-step through HBM, FP16-MFMA DFT-32 first pass"

@@ -161,6 +161,7 @@ struct fxc_plan {
                              // MFMA: kernel organization (1: producer/consumer waves)
     int grid_div = 1;        // workgroups = max_wg / grid_div
     DevBuf tw_l1, tw_l2, tw_lm, d_x[2], d_y, d_z[2], d_params, d_lshifts;
+    DevBuf tw_l0;   // W_64 (the fused four-step's rows at N = 16384)
     // scratch
     DevBuf part, red_scratch, table_dev, d_raw[2], d_acc, d_raw_out, d_tmp_in, d_tmp_out,
         d_shifts, d_cb, d_tbuf;
@@ -185,7 +186,7 @@ struct fxc_plan {
     // Every device buffer of the plan (release, byte count).
     template <class Fn> void each_buf(Fn fn) {
         DevBuf *bufs[] = {&tw_fft, &tw_post, &lev_dev[0], &lev_dev[1], &out_lev, &mf_tab,
-                          &tw_l1, &tw_l2, &tw_lm, &d_x[0], &d_x[1], &d_y, &d_z[0], &d_z[1],
+                          &tw_l1, &tw_l2, &tw_lm, &tw_l0, &d_x[0], &d_x[1], &d_y, &d_z[0], &d_z[1],
                           &d_params, &d_lshifts, &part, &red_scratch, &table_dev, &d_raw[0],
                           &d_raw[1], &d_acc, &d_raw_out, &d_tmp_in, &d_tmp_out, &d_shifts,
                           &d_cb, &d_tbuf, &d_chain, &d_fringe, &d_err, &d_cacc};
@@ -903,6 +904,29 @@ extern "C" fxc_status fxc_plan_create(const fxc_plan_desc *desc, fxc_plan **out)
         p->variant = 1;
         p->M1 = 128;
         p->M2 = (int)M / 128;
+        if (p->fft_form == 0) {
+            // N = 16384: Y of both stations for a chunk of frames, the frame
+            // parameters and shifts, the W_M and W_64 tables
+            const int64_t per_frame = 2 * (int64_t)M * 8 + 16 + (int64_t)frame_param_bytes();
+            p->chunk = std::max<int64_t>(16, ((int64_t)512 << 20) / per_frame);
+            p->chunk_use = p->chunk;
+            const size_t C = (size_t)p->chunk;
+            auto table = [](DevBuf &b, int n) {
+                std::vector<float> t(2 * (size_t)n);
+                for (int k = 0; k < n; ++k) {
+                    const double ang = -2.0 * M_PI * (double)k / (double)n;
+                    t[2 * k] = (float)std::cos(ang);
+                    t[2 * k + 1] = (float)std::sin(ang);
+                }
+                return b.ensure(t.size() * 4) == hipSuccess &&
+                       hipMemcpy(b.p, t.data(), t.size() * 4, hipMemcpyHostToDevice) == hipSuccess;
+            };
+            if (!table(p->tw_l0, 64) || !table(p->tw_lm, (int)M) ||
+                p->d_z[0].ensure(C * M * 8) != hipSuccess || p->d_z[1].ensure(C * M * 8) != hipSuccess ||
+                p->d_params.ensure(C * frame_param_bytes()) != hipSuccess ||
+                p->d_lshifts.ensure(C * 16) != hipSuccess)
+                return cleanup(fail(FXC_ERR_NOMEM, "fused four-step scratch allocation failed"));
+        }
     }
     p->max_wg = p->fft_form == 1 ? cus
                 : p->fft_form == 2 ? kLargeGroups
@@ -1430,11 +1454,14 @@ static fxc_status enqueue_phased_large(fxc_plan *p, const SectorArgs &base, int6
 // bins of a product lie on the same rows (band shift a multiple of M1) and
 // both code widths are at most 16 bits (two codes per funnel shift).
 // The fused four-step runs at M1 = 128 (variant 1 of the large form).
+// The fused four-step (M1 = 128): N = 32768 / 65536 (the large form's
+// default), and N = 16384 (M2 = 64) as variant 1 of the Stockham form.
 static bool large_fused_possible(const fxc_plan *p) {
     const int64_t sb = p->desc.a2s - p->desc.a1s;
-    return p->fft_form == 2 && p->desc.precision == FXC_PRECISION_CHOSEN && p->mf_b1 &&
-           !(p->desc.products & (FXC_PRODUCT_PHASED | FXC_PRODUCT_RAW_OUT)) &&
-           (p->M / 128 == 128 || p->M / 128 == 256) && (sb % 128) == 0 &&
+    const bool shape = (p->fft_form == 2 && (p->M / 128 == 128 || p->M / 128 == 256)) ||
+                       (p->fft_form == 0 && p->N == 16384);
+    return shape && p->desc.precision == FXC_PRECISION_CHOSEN && p->mf_b1 &&
+           !(p->desc.products & (FXC_PRODUCT_PHASED | FXC_PRODUCT_RAW_OUT)) && (sb % 128) == 0 &&
            p->st[0].bits <= 16 && p->st[1].bits <= 16;
 }
 
@@ -1444,16 +1471,17 @@ static bool large_fused_ok(const fxc_plan *p) {
 
 // Frame groups of the fused second pass: enough workgroups (M1 / 2 row
 // pairs x groups) for one resident round at the kernel's occupancy (three
-// workgroups per CU at M2 = 256, five at M2 = 128), at most kLargeGroups.
+// workgroups per CU at M2 = 256, five at M2 = 64 / 128), at most the plan's
+// partial slots.
 static int large_fused_groups(const fxc_plan *p) {
     const int per_cu = p->M2 == 256 ? 3 : 5;
-    return std::max(1, std::min(kLargeGroups, num_cus() * per_cu / (p->M1 / 2)));
+    return std::max(1, std::min(p->max_wg, num_cus() * per_cu / (p->M1 / 2)));
 }
 
 static fxc_status enqueue_large(fxc_plan *p, const SectorArgs &base, int64_t nf, double *d_acc,
                                 hipStream_t s) {
     const float2 *twl1 = (const float2 *)(p->M1 == 128 ? p->tw_l1.p : p->tw_l2.p);
-    const float2 *twl2 = (const float2 *)(p->M2 == 128 ? p->tw_l1.p : p->tw_l2.p);
+    const float2 *twl2 = (const float2 *)(p->M2 == 64 ? p->tw_l0.p : p->M2 == 128 ? p->tw_l1.p : p->tw_l2.p);
     if (large_fused_ok(p)) {
         const int ng = large_fused_groups(p);
         for (int64_t c0 = 0; c0 < nf; c0 += p->chunk_use) {
@@ -1549,7 +1577,7 @@ static fxc_status enqueue_sector(fxc_plan *p, const uint8_t *d_raw1, const uint8
     a.nf = nf;
     if (phased && p->fft_form == 2) return enqueue_phased_large(p, a, nf, d_raw_out, d_acc, s);
     if (phased) return enqueue_phased(p, a, nf, d_raw_out, d_acc, s);
-    if (p->fft_form == 2) return enqueue_large(p, a, nf, d_acc, s);
+    if (p->fft_form == 2 || large_fused_ok(p)) return enqueue_large(p, a, nf, d_acc, s);
     const int wg_cap = std::max(1, p->max_wg / p->grid_div);
     // Plain XCF over more than kChainMinOverlap bins: slabs of frames whose
     // f32 phase chains fit the chain scratch; the slabs add into the same
@@ -1638,6 +1666,9 @@ extern "C" fxc_status fxc_plan_autotune(fxc_plan *p, const uint8_t *d_raw1, cons
             for (int64_t c : {p->chunk, p->chunk / 2, p->chunk / 4})
                 if (c >= 16) cands.push_back({0, 1, m1, c});
     } else if (p->fft_form == 0) {
+        if (large_fused_possible(p))   // N = 16384: the fused four-step (the default) first
+            for (int64_t c : {p->chunk, p->chunk / 2, p->chunk / 4})
+                if (c >= 16) cands.push_back({1, 1, 128, c});
         const int nv = sector_has_variant(p->N) ? 2 : 1;
         for (int v = 0; v < nv; ++v)
             for (int d : {1, 2, 4}) cands.push_back({v, d, 0, 0});
@@ -1665,7 +1696,7 @@ extern "C" fxc_status fxc_plan_autotune(fxc_plan *p, const uint8_t *d_raw1, cons
         const Cand &c = cands[i];
         p->variant = c.variant;
         p->grid_div = c.grid_div;
-        if (p->fft_form == 2) {
+        if (c.m1) {
             p->M1 = c.m1;
             p->M2 = p->M / c.m1;
             p->chunk_use = c.chunk;
@@ -1705,17 +1736,18 @@ extern "C" fxc_status fxc_plan_autotune(fxc_plan *p, const uint8_t *d_raw1, cons
     const Cand &c = cands[best_i];
     p->variant = c.variant;
     p->grid_div = c.grid_div;
-    if (p->fft_form == 2) {
+    if (c.m1) {
         p->M1 = c.m1;
         p->M2 = p->M / c.m1;
         p->chunk_use = c.chunk;
     }
     if (rep) {
+        const bool four_step = p->fft_form == 2 || large_fused_ok(p);
         rep->candidates = (int32_t)cands.size();
         rep->variant = p->variant;
         rep->grid_divisor = p->grid_div;
-        rep->split_m1 = p->fft_form == 2 ? p->M1 : 0;
-        rep->chunk_frames = p->fft_form == 2 ? p->chunk_use : 0;
+        rep->split_m1 = four_step ? p->M1 : 0;
+        rep->chunk_frames = four_step ? p->chunk_use : 0;
         rep->best_ms = best;
         rep->default_ms = dflt;
     }
@@ -1741,6 +1773,10 @@ extern "C" fxc_status fxc_plan_set_tuning(fxc_plan *p, const fxc_tune_report *c)
             return fail(FXC_ERR_INVALID_ARG, "chunk_frames outside 1..%lld", (long long)p->chunk);
         p->M1 = c->split_m1;
         p->M2 = p->M / c->split_m1;
+        p->chunk_use = c->chunk_frames;
+    } else if (c->variant == 1 && large_fused_possible(p)) {   // N = 16384: the fused four-step
+        if (c->chunk_frames < 1 || c->chunk_frames > p->chunk)
+            return fail(FXC_ERR_INVALID_ARG, "chunk_frames outside 1..%lld", (long long)p->chunk);
         p->chunk_use = c->chunk_frames;
     }
     p->variant = c->variant;

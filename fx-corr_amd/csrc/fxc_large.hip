@@ -626,14 +626,190 @@ __global__ __launch_bounds__(256) void k_large_pass2_r16(SectorArgs a, const flo
     }
 }
 
+// One bin of a row pair for the second pass at any M2: bin g = row + M1 k2
+// of row ra (rb_sel false) or rb, or bin M (nyq: row 0, k2 = M2); the
+// untangle of both stations and the ACF/XCF products (pair_bins' arithmetic).
+template <int M2, int LP>
+__device__ __forceinline__ void row_bin(const float2 (*zb)[LP], const float2 (*twp)[M2], bool rb_sel,
+                                        int k2, bool nyq, int M, int M1, int ra, int rb, int gs, int ov,
+                                        int sbo, int jo, int grid, bool do_acf, bool do_xcf,
+                                        bool ov_only, double dth, double phw, double (&acc)[4]) {
+    const int row = nyq ? 0 : (rb_sel ? rb : ra);
+    if (nyq) k2 = M2;
+    const int g = nyq ? M : row + M1 * k2;
+    const int bg = row == ra ? 0 : 1;
+    const int bm = (row == 0 ? row : M1 - row) == ra ? 0 : 1;
+    const int km = row == 0 ? (M2 - k2) & (M2 - 1) : M2 - 1 - k2;
+    const float2 Zg = zb[bg][pad(k2 & (M2 - 1))], Zm = zb[bm][pad(km)];
+    const float2 Xg = untangle_ab(Zg, Zm, g, M, twp[bg][k2 & (M2 - 1)]);
+    const bool in_ov = g >= gs && g < gs + ov;
+    float2 Xo = make_float2(0.0f, 0.0f);
+    if (in_ov) {
+        const int h = g + sbo, kh = k2 + jo;
+        const int khm = row == 0 ? (M2 - kh) & (M2 - 1) : M2 - 1 - kh;
+        const float2 Oh = zb[2 + bg][pad(kh & (M2 - 1))];
+        const float2 Om = zb[2 + bm][pad(khm & (M2 - 1))];
+        Xo = untangle_ab(Oh, Om, h, M, twp[bg][kh & (M2 - 1)]);
+    }
+    if (do_acf) {
+        const float pgrid = (!ov_only || in_ov) ? norm_sqr(Xg) : 0.0f;
+        const float poth = in_ov ? norm_sqr(Xo) : 0.0f;
+        acc[0] += (double)(grid == 0 ? pgrid : poth);
+        acc[1] += (double)(grid == 0 ? poth : pgrid);
+    }
+    if (in_ov && do_xcf) {
+        const float r = (float)wrap_turns(fma((double)(g - gs), dth, phw));
+        const float2 phf = make_float2(__builtin_amdgcn_cosf(r), __builtin_amdgcn_sinf(r));
+        const float2 X1 = (grid == 0) ? Xg : Xo;
+        const float2 X2 = (grid == 0) ? Xo : Xg;
+        const float2 v2 = c_mul(c_mul(X1, c_conj(X2)), phf);
+        acc[2] += (double)v2.x;
+        acc[3] += (double)v2.y;
+    }
+}
+
+// The second pass at M2 = 64 (N = 16384, M1 = 128) with two radix-8 passes
+// per row: lane 8 q + j of wave w takes row slot q (frame 2 w + q / 4 of the
+// iteration, row q % 4 of [grid ra, grid rb, other ra, other rb]) and
+// butterfly j -- the first pass from registers (inputs j + 8 r), its outputs
+// multiplied by W_64^{j r} and transposed through LDS, the second pass
+// writing X[j + 8 r] -- so the four waves transform eight frames per
+// iteration.  After a barrier thread t forms bin t % 128 of the pair (row ra
+// for t % 128 < 64, else rb; k2 = t % 64) for frames t / 128, + 2, + 4, + 6
+// of the eight, and a second barrier frees the rows.  Threads t and t + 128
+// hold the same bin; their sums are added in that order at the end (bin M,
+// pair 0: threads 0 and 128).
+__global__ __launch_bounds__(256) void k_large_pass2_r8(SectorArgs a, const float2 *y1,
+                                                        const float2 *y2, const FrameParam *params,
+                                                        int M1, const float2 *tw_m2, int64_t fpg,
+                                                        int accumulate) {
+    constexpr int M2 = 64, LP = M2 + M2 / 16 + 2;
+    __shared__ float2 zr[8][4][LP];   // [frame of the iteration][grid ra, grid rb, other ra, other rb]
+    __shared__ float2 twp[2][M2];     // tw_post[row + M1 k2], rows ra / rb
+    __shared__ double hsum[128][4];   // threads 128..255's sums
+    __shared__ double msum[4];        // thread 128's bin-M sums (pair 0)
+    const int M = a.M;
+    const int half = M + 1;
+    const int tid = threadIdx.x;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int lane = tid & 63;
+    const int q = lane >> 3, j = lane & 7;
+    const int fw = q >> 2, rk = q & 3;
+    const int p = blockIdx.x;
+    const int ra = p, rb = p == 0 ? M1 / 2 : M1 - p;
+    for (int i = tid; i < 2 * M2; i += 256)
+        twp[i / M2][i % M2] = a.tw_post[(i < M2 ? ra : rb) + M1 * (i % M2)];
+    float2 tw[8];
+#pragma unroll
+    for (int r = 1; r < 8; ++r) tw[r] = tw_m2[(j * r) & (M2 - 1)];
+    const int64_t g0 = (int64_t)blockIdx.y * fpg;
+    int64_t g1 = g0 + fpg;
+    if (g1 > a.nf) g1 = a.nf;
+    const bool do_acf = (a.flags & FLAG_ACF) != 0;
+    const bool do_xcf = (a.flags & FLAG_XCF) != 0;
+    const bool ov_only = (a.flags & FLAG_ACF_OVERLAP) != 0;
+    const int gs = (a.grid == 0) ? a.a1s : a.a2s;
+    const int sbo = (a.grid == 0) ? (a.a2s - a.a1s) : (a.a1s - a.a2s);
+    const int jo = sbo / M1;
+    const float2 *src_row = ((rk & 2) ? (a.grid == 0 ? y2 : y1) : (a.grid == 0 ? y1 : y2)) +
+                            (int64_t)((rk & 1) ? rb : ra) * M2 + j;
+    const int bsel = tid & 127, fsel = tid >> 7;
+    const bool rb_sel = bsel >= M2;
+    const int k2 = bsel & (M2 - 1);
+    const bool nyq = p == 0 && bsel == 0;   // threads 0 and 128 also take bin M
+    double acc[4] = {0.0, 0.0, 0.0, 0.0}, nacc[4] = {0.0, 0.0, 0.0, 0.0};
+    const int slot = 2 * w + fw;   // this lane's frame of the iteration
+    float2 cur[8];
+    {
+        const int64_t f = g0 + slot;
+#pragma unroll
+        for (int r = 0; r < 8; ++r)
+            cur[r] = f < g1 ? src_row[f * (int64_t)M + 8 * r] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();   // twiddles staged
+    float2 *z = zr[slot][rk];
+    for (int64_t it = g0; it < g1; it += 8) {
+        const int64_t f = it + slot, fn = f + 8;
+        float2 nxt[8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r)
+            nxt[r] = fn < g1 ? src_row[fn * (int64_t)M + 8 * r] : make_float2(0.f, 0.f);
+        if (f < g1) {
+            dftR<8>(cur);                                // pass 1: inputs j + 8 r
+#pragma unroll
+            for (int r = 1; r < 8; ++r) cur[r] = c_mul(cur[r], tw[r]);
+#pragma unroll
+            for (int r = 0; r < 8; ++r) z[pad(8 * j + r)] = cur[r];
+            wave_sync_lds();
+#pragma unroll
+            for (int r = 0; r < 8; ++r) cur[r] = z[pad(j + 8 * r)];
+            dftR<8>(cur);                                // pass 2
+            wave_sync_lds();
+#pragma unroll
+            for (int r = 0; r < 8; ++r) z[pad(j + 8 * r)] = cur[r];   // X[j + 8 r]
+        }
+        __syncthreads();   // the iteration's rows are ready
+        const int nfr = (int)((g1 - it) < 8 ? (g1 - it) : 8);
+        for (int s2 = fsel; s2 < nfr; s2 += 2) {
+            const double dth = params[it + s2].dth, phw = params[it + s2].phw;
+            row_bin<M2, LP>(zr[s2], twp, rb_sel, k2, false, M, M1, ra, rb, gs, a.ov, sbo, jo, a.grid,
+                            do_acf, do_xcf, ov_only, dth, phw, acc);
+            if (nyq)
+                row_bin<M2, LP>(zr[s2], twp, false, 0, true, M, M1, ra, rb, gs, a.ov, sbo, jo, a.grid,
+                                do_acf, do_xcf, ov_only, dth, phw, nacc);
+        }
+        __syncthreads();   // every thread is done with the rows
+#pragma unroll
+        for (int r = 0; r < 8; ++r) cur[r] = nxt[r];
+    }
+    if (tid >= 128) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) hsum[bsel][c] = acc[c];
+        if (nyq)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) msum[c] = nacc[c];
+    }
+    __syncthreads();
+    if (tid >= 128) return;
+    // partial slot in pair-major order (as k_large_pass2_mac)
+    double *out = a.part + (size_t)blockIdx.y * kNumProducts * half;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        int g;
+        double v[4];
+        if (i == 0) {
+            g = p * 2 * M2 + bsel;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) v[c] = acc[c] + hsum[bsel][c];
+        } else {
+            if (!nyq) continue;
+            g = M;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) v[c] = nacc[c] + msum[c];
+        }
+        const double prev[4] = {accumulate ? out[P_11 * half + g] : 0.0,
+                                accumulate ? out[P_22 * half + g] : 0.0,
+                                accumulate ? out[P_12R * half + g] : 0.0,
+                                accumulate ? out[P_12I * half + g] : 0.0};
+        out[P_PH * half + g] = 0.0;
+        out[P_11 * half + g] = prev[0] + v[0];
+        out[P_22 * half + g] = prev[1] + v[1];
+        out[P_12R * half + g] = prev[2] + v[2];
+        out[P_12I * half + g] = prev[3] + v[3];
+    }
+}
+
 hipError_t launch_large_pass2_mac(const SectorArgs &a, const float2 *y1, const float2 *y2,
                                   const FrameParam *params, int M1, int M2, const float2 *tw_m2,
                                   int n_groups, int accumulate, hipStream_t s) {
     const int sbo = (a.grid == 0) ? (a.a2s - a.a1s) : (a.a1s - a.a2s);
-    if (M1 * M2 != a.M || sbo % M1 != 0 || (M2 != 128 && M2 != 256)) return hipErrorInvalidValue;
+    if (M1 * M2 != a.M || sbo % M1 != 0 || (M2 != 64 && M2 != 128 && M2 != 256)) return hipErrorInvalidValue;
     const int64_t fpg = (a.nf + n_groups - 1) / n_groups;
     const dim3 grid((unsigned)(M1 / 2), (unsigned)n_groups);
-    if (M2 == 128)
+    if (M2 == 64)
+        hipLaunchKernelGGL(k_large_pass2_r8, grid, dim3(256), 0, s, a, y1, y2, params, M1, tw_m2, fpg,
+                           accumulate);
+    else if (M2 == 128)
         hipLaunchKernelGGL(k_large_pass2_mac<128>, grid, dim3(256), 0, s, a, y1, y2, params, M1, tw_m2,
                            fpg, accumulate);
     else

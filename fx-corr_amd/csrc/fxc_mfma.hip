@@ -2598,7 +2598,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
 hipError_t launch_large_pass1_fused(const SectorArgs &a, const int64_t *shifts, float2 *out0,
                                     float2 *out1, int M1, int M2, const float2 *tw_m,
                                     hipStream_t s) {
-    if (a.mf_tab == nullptr || M1 != 128 || (M2 != 128 && M2 != 256) || M1 * M2 != a.M)
+    if (a.mf_tab == nullptr || M1 != 128 || (M2 != 64 && M2 != 128 && M2 != 256) || M1 * M2 != a.M)
         return hipErrorInvalidValue;
     if (a.nf <= 0) return hipSuccess;
     // 16-column blocks per frame: M2 / 16; one per wave, four waves per part

@@ -1069,6 +1069,46 @@ def test_large_fused_form(n, bits, sb, grid, prods, lsb, delay, nf, chunk):
         assert zero_pattern_equal(fused[k], hbm[k]), k
 
 
+@pytest.mark.parametrize("bits,sb,grid,prods,lsb,delay,nf,chunk,shuffle", [
+    ((2, 2), 0, 0, "xa", (False, False), 5.3, 300, 0, False),
+    ((1, 8), 128, 0, "xa", (True, False), -3.6, 5, 0, False),      # fewer frames than groups
+    ((4, 2), -256, 1, "xa", (False, True), 17.2, 300, 37, True),   # several chunks, shuffle
+    ((4, 1), 2048, 0, "ov", (True, True), 2.5, 200, 0, False),     # ACF on the overlap only
+    ((2, 4), -128, 1, "x", (False, False), -40.7, 100, 23, False),  # XCF only, odd shifts
+    ((3, 5), 0, 0, "xa", (True, False), 1.7, 120, 0, True),         # codes straddling words
+])
+def test_fused_four_step_fft16384(bits, sb, grid, prods, lsb, delay, nf, chunk, shuffle):
+    """The fused four-step at N = 16384 (M1 = 128 x M2 = 64, the chosen
+    form's default there; rows by two radix-8 passes) against the FP64
+    oracle and against the Stockham kernel on the same plan (variant 0)."""
+    n, fs = 16384, 512e6
+    band = (0, 0, n // 2, 0)
+    if sb:
+        band = (sb, 0, n // 2 - abs(sb), abs(sb)) if sb > 0 else (sb, abs(sb), n // 2, 0)
+    products = {"xa": F.PRODUCT_ACF | F.PRODUCT_XCF, "x": F.PRODUCT_XCF,
+                "ov": F.PRODUCT_ACF | F.PRODUCT_ACF_OVERLAP_ONLY | F.PRODUCT_XCF}[prods]
+    rng = np.random.default_rng(nf + abs(sb))
+    c = make_case(n=n, fs=fs, nf=nf, bits=bits, lsb=lsb, band=band, grid=grid,
+                  shuffles=(rng.permutation(32) if shuffle else None, None),
+                  products=products, rho=0.4, delay_samples=delay, seed=n + nf + abs(sb),
+                  delay_user_samples=delay, extra_delay_rate_sps=2e-9, lo1_hz=8.4e9,
+                  lo2_hz=8.4e9 - sb * fs / n)
+    o64 = oracle_run(c, R.MODE_F64)
+    keys = ("acc11", "acc22", "acc12") if products & F.PRODUCT_ACF else ("acc12",)
+    p = gpu_plan(c, F.PRECISION_CHOSEN)
+    assert p.info.fft_form == 0 and p.info.variant == 1
+    if chunk:
+        F.plan_set_tuning(p, variant=1, split_m1=128, chunk_frames=chunk)
+    fused = gpu_run(c, plan=p)
+    _compare(fused, o64, CHOSEN_TOL, keys)
+    q = gpu_plan(c, F.PRECISION_CHOSEN)
+    F.plan_set_tuning(q, variant=0)
+    stockham = gpu_run(c, plan=q)
+    _compare(fused, stockham, CHOSEN_TOL, keys)
+    for k in keys:
+        assert zero_pattern_equal(fused[k], stockham[k]), k
+
+
 @pytest.mark.parametrize("n,bits", [(32768, (3, 5)), (65536, (5, 3))])
 def test_large_fused_odd_widths_and_shuffle(n, bits):
     """The fused four-step's run-time code width (3 and 5 bits: codes that
