@@ -430,13 +430,16 @@ fxc_status fxc_plan_device_bytes(const fxc_plan *plan, uint64_t *bytes);
  * the spectrum's scale) for the FP32-batch organization. */
 typedef struct {
     int32_t candidates;
-    int32_t variant;           /* Stockham: 0 default passes, 1 radix-16 passes;
+    int32_t variant;           /* Stockham: 0 default passes, 1 radix-16 passes
+                                  (N = 2048..8192) or, chosen form at N = 16384, the
+                                  fused four-step (split_m1 = 128, the default there);
                                   MFMA: 1 producer/consumer (default), 2 producer/consumer
                                   with FP32 sums over batches of 128 frames added to FP64
-                                  totals, 0 whole-frame waves (2 and 0: N = 1024 only) */
+                                  totals, 0 whole-frame waves (2 and 0: N = 1024 only);
+                                  N >= 32768: 1 fused four-step, 0 through HBM */
     int32_t grid_divisor;      /* workgroups = default / grid_divisor */
-    int32_t split_m1;          /* N >= 32768: column DFT length M1 (else 0) */
-    int64_t chunk_frames;      /* N >= 32768: frames per chunk (else 0) */
+    int32_t split_m1;          /* four-step forms: column DFT length M1 (else 0) */
+    int64_t chunk_frames;      /* four-step forms: frames per chunk (else 0) */
     double best_ms, default_ms;
 } fxc_tune_report;
 fxc_status fxc_plan_autotune(fxc_plan *plan, const uint8_t *d_raw1, const uint8_t *d_raw2,
