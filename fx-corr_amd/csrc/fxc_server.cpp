@@ -2,7 +2,8 @@
 // admits queued jobs by device footprint (plan tables + scratch, input
 // window, FP64 spectra) against an HBM budget and a slot count, spreads the
 // admitted jobs over a few HIP streams, and enqueues each job's chunks in
-// order on its stream (copy-in, fused sector kernels, spectra += chunk).
+// order on its stream (fused sector kernels, spectra += chunk), with host
+// input copied in on a companion copy stream into two alternating windows.
 // Jobs on different streams overlap on the device; a job's delay-model
 // frame index and spectra stay resident between its chunks.  Results are
 // copied back through pinned memory behind an event, so completion needs no
@@ -59,10 +60,19 @@ struct Job {
     std::string err;
     fxc_plan *plan = nullptr;
     int stream = -1;
-    void *win[2] = {nullptr, nullptr};   // device input window (host input)
-    size_t win_cap[2] = {0, 0};
+    // host input: two device windows per station, used by alternate chunks,
+    // so a chunk's copy (on the copy stream) overlaps the previous chunk's
+    // kernels (on the job's stream); copy_ev[b] / comp_ev[b] order a window's
+    // copy before its kernels and its kernels before the next copy into it
+    void *win[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};   // [window][station]
+    size_t win_cap[2][2] = {{0, 0}, {0, 0}};
+    hipEvent_t copy_ev[2] = {nullptr, nullptr}, comp_ev[2] = {nullptr, nullptr};
+    bool win_used[2] = {false, false};
+    int64_t nchunk = 0;
     double *d_acc = nullptr, *d_chunk = nullptr;
+    size_t acc_cap = 0, chunk_cap = 0;   // their capacities (pooled buffers may be larger)
     double *h_result = nullptr;          // pinned
+    size_t res_cap = 0;
     size_t half = 0;
     int64_t frames = 0;
     bool final_enqueued = false;
@@ -105,6 +115,7 @@ std::string plan_key(const fxc_job_desc &d, const std::vector<double> *lev) {
 struct fxc_server {
     fxc_server_cfg cfg{};
     std::vector<hipStream_t> streams;
+    std::vector<hipStream_t> copy_streams;   // host-input copies, one per stream
     std::vector<int> stream_jobs;
     std::mutex mu;
     std::condition_variable cv;       // wakes the worker
@@ -124,9 +135,82 @@ struct fxc_server {
     int64_t failed = 0;
     int max_conc = 0;
     uint64_t peak_bytes = 0;
+    // Buffers and events of finished jobs, kept for later ones: releasing
+    // device or pinned memory waits for the whole device, and creating
+    // events costs a runtime call per job.  Device buffers count against
+    // the HBM budget until they are released to make room.
+    std::multimap<size_t, void *> dev_pool;
+    std::multimap<size_t, double *> host_pool;
+    std::vector<hipEvent_t> ev_pool;
+    uint64_t pool_bytes = 0;
 
-    uint64_t in_use() const { return active_bytes + idle_bytes; }
+    uint64_t in_use() const { return active_bytes + idle_bytes + pool_bytes; }
     void note_peak() { peak_bytes = std::max(peak_bytes, in_use()); }
+
+    // A device buffer of at least n bytes (a pooled one of at most 2n, else
+    // a new one); *cap receives its size.
+    void *dev_get(size_t n, size_t *cap) {
+        auto it = dev_pool.lower_bound(n);
+        if (it != dev_pool.end() && it->first <= 2 * n) {
+            void *b = it->second;
+            *cap = it->first;
+            pool_bytes -= it->first;
+            dev_pool.erase(it);
+            return b;
+        }
+        void *b = nullptr;
+        if (hipMalloc(&b, n) != hipSuccess) {
+            trim_pool(~0ull);   // the pool may hold the memory
+            if (hipMalloc(&b, n) != hipSuccess) return nullptr;
+        }
+        *cap = n;
+        return b;
+    }
+    void dev_put(void *b, size_t cap) {
+        if (!b) return;
+        dev_pool.emplace(cap, b);
+        pool_bytes += cap;
+    }
+    // Releases pooled device buffers (largest first) until `need` more
+    // bytes fit the budget (all of them for need = ~0).
+    void trim_pool(uint64_t need) {
+        while (!dev_pool.empty() && (need == ~0ull || in_use() + need > cfg.hbm_budget_bytes)) {
+            auto it = std::prev(dev_pool.end());
+            (void)hipFree(it->second);
+            pool_bytes -= it->first;
+            dev_pool.erase(it);
+        }
+    }
+    double *host_get(size_t n, size_t *cap) {
+        auto it = host_pool.lower_bound(n);
+        if (it != host_pool.end() && it->first <= 2 * n) {
+            double *b = it->second;
+            *cap = it->first;
+            host_pool.erase(it);
+            return b;
+        }
+        double *b = nullptr;
+        if (hipHostMalloc(reinterpret_cast<void **>(&b), n, 0) != hipSuccess) return nullptr;
+        *cap = n;
+        return b;
+    }
+    hipEvent_t ev_get() {
+        if (!ev_pool.empty()) {
+            hipEvent_t e = ev_pool.back();
+            ev_pool.pop_back();
+            return e;
+        }
+        hipEvent_t e = nullptr;
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return nullptr;
+        return e;
+    }
+    void release_pools() {
+        trim_pool(~0ull);
+        for (auto &kv : host_pool) (void)hipHostFree(kv.second);
+        host_pool.clear();
+        for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
+        ev_pool.clear();
+    }
 
     static uint64_t plan_bytes(fxc_plan *p) {
         uint64_t b = 0;
@@ -135,6 +219,7 @@ struct fxc_server {
     }
 
     void evict_idle(uint64_t need) {
+        trim_pool(need);
         while (!idle_plans.empty() && in_use() + need > cfg.hbm_budget_bytes) {
             auto it = idle_plans.begin();
             idle_bytes -= plan_bytes(it->second);
@@ -144,6 +229,7 @@ struct fxc_server {
     }
     // Evicts idle plans of other descriptions until `need` fits.
     void evict_idle_except(const std::string &key, uint64_t need) {
+        trim_pool(need);
         for (auto it = idle_plans.begin(); it != idle_plans.end();) {
             const uint64_t keep = idle_plans.count(key) ? key_info[key].first : 0;
             if (in_use() - keep + need <= cfg.hbm_budget_bytes) break;
@@ -165,26 +251,38 @@ struct fxc_server {
         done_cv.notify_all();
     }
 
+    // Returns a job's buffers and event to the pools (its device work has
+    // finished: complete() saw its event, or the failure path synchronised
+    // its stream).
     void free_job_device(Job &j) {
-        for (int i = 0; i < 2; ++i)
-            if (j.win[i]) (void)hipFree(j.win[i]);
-        if (j.d_acc) (void)hipFree(j.d_acc);
-        if (j.d_chunk) (void)hipFree(j.d_chunk);
-        if (j.h_result) (void)hipHostFree(j.h_result);
-        if (j.done_ev) (void)hipEventDestroy(j.done_ev);
-        j.win[0] = j.win[1] = nullptr;
+        for (int b = 0; b < 2; ++b) {
+            for (int i = 0; i < 2; ++i) {
+                dev_put(j.win[b][i], j.win_cap[b][i]);
+                j.win[b][i] = nullptr;
+                j.win_cap[b][i] = 0;
+            }
+            if (j.copy_ev[b]) ev_pool.push_back(j.copy_ev[b]);
+            if (j.comp_ev[b]) ev_pool.push_back(j.comp_ev[b]);
+            j.copy_ev[b] = j.comp_ev[b] = nullptr;
+            j.win_used[b] = false;
+        }
+        dev_put(j.d_acc, j.acc_cap);
+        dev_put(j.d_chunk, j.chunk_cap);
+        if (j.h_result) host_pool.emplace(j.res_cap, j.h_result);
+        if (j.done_ev) ev_pool.push_back(j.done_ev);
         j.d_acc = j.d_chunk = j.h_result = nullptr;
+        j.acc_cap = j.chunk_cap = j.res_cap = 0;
         j.done_ev = nullptr;
     }
 
     // Device bytes of a job besides its plan: spectra (job + chunk) and, for
-    // host input, the input window of its largest queued chunk.
+    // host input, the two input windows of its largest queued chunk.
     static uint64_t job_extra(const Job &j, const fxc_plan_info &info) {
         uint64_t win = 0;
         if (!j.desc.input_on_device)
             for (const ChunkIn &c : j.chunks)
                 win = std::max<uint64_t>(win, (uint64_t)c.nf * (uint64_t)(info.bpf1 + info.bpf2));
-        return 2ull * kNumProducts * (uint64_t)info.half * sizeof(double) + win;
+        return 2ull * kNumProducts * (uint64_t)info.half * sizeof(double) + 2 * win;
     }
 
     // Returns true when the job left the queue (admitted or failed).  A plan
@@ -242,10 +340,18 @@ struct fxc_server {
             }
             return false;
         }
-        if (hipMalloc(&j.d_acc, acc_bytes) != hipSuccess ||
-            hipMalloc(&j.d_chunk, acc_bytes) != hipSuccess ||
-            hipHostMalloc(&j.h_result, acc_bytes + sizeof(double), 0) != hipSuccess ||
-            hipEventCreateWithFlags(&j.done_ev, hipEventDisableTiming) != hipSuccess) {
+        j.d_acc = static_cast<double *>(dev_get(acc_bytes, &j.acc_cap));
+        j.d_chunk = static_cast<double *>(dev_get(acc_bytes, &j.chunk_cap));
+        j.h_result = host_get(acc_bytes + sizeof(double), &j.res_cap);
+        j.done_ev = ev_get();
+        bool evs = j.done_ev != nullptr;
+        if (!j.desc.input_on_device)
+            for (int b = 0; b < 2; ++b) {
+                j.copy_ev[b] = ev_get();
+                j.comp_ev[b] = ev_get();
+                evs = evs && j.copy_ev[b] && j.comp_ev[b];
+            }
+        if (!j.d_acc || !j.d_chunk || !j.h_result || !evs) {
             free_job_device(j);
             fxc_plan_destroy(plan);
             fail_job(j, "device allocation failed at admission");
@@ -264,7 +370,7 @@ struct fxc_server {
             return true;
         }
         j.plan = plan;
-        j.footprint = pb + 2 * acc_bytes;   // the input window is counted as it grows
+        j.footprint = pb + j.acc_cap + j.chunk_cap;   // the input window is counted as it grows
         active_bytes += j.footprint;
         j.state = ACTIVE;
         j.t_admit = clk::now();
@@ -285,32 +391,40 @@ struct fxc_server {
             did = true;
             const uint8_t *d1 = c.r1, *d2 = c.r2;
             if (!j.desc.input_on_device) {
+                const int w = (int)(j.nchunk & 1);
+                hipStream_t cs = copy_streams[j.stream];
                 const size_t b[2] = {(size_t)c.nf * info.bpf1, (size_t)c.nf * info.bpf2};
                 for (int i = 0; i < 2; ++i) {
-                    if (b[i] > j.win_cap[i]) {   // grow: earlier chunks may still read it
-                        (void)hipStreamSynchronize(s);
-                        if (j.win[i]) (void)hipFree(j.win[i]);
-                        active_bytes -= j.win_cap[i];
-                        j.footprint -= j.win_cap[i];
-                        j.win[i] = nullptr;
-                        j.win_cap[i] = 0;
-                        if (hipMalloc(&j.win[i], b[i]) != hipSuccess) {
+                    if (b[i] > j.win_cap[w][i]) {   // grow: the chunk two back may still read it
+                        if (j.win_used[w]) (void)hipEventSynchronize(j.comp_ev[w]);
+                        dev_put(j.win[w][i], j.win_cap[w][i]);
+                        active_bytes -= j.win_cap[w][i];
+                        j.footprint -= j.win_cap[w][i];
+                        j.win[w][i] = nullptr;
+                        j.win_cap[w][i] = 0;
+                        size_t cap = 0;
+                        j.win[w][i] = dev_get(b[i], &cap);
+                        if (!j.win[w][i]) {
                             fail_active(j, "input window allocation failed");
                             return true;
                         }
-                        j.win_cap[i] = b[i];
-                        active_bytes += b[i];
-                        j.footprint += b[i];
+                        j.win_cap[w][i] = cap;
+                        active_bytes += cap;
+                        j.footprint += cap;
                         note_peak();
                     }
                 }
-                if (hipMemcpyAsync(j.win[0], c.r1, b[0], hipMemcpyHostToDevice, s) != hipSuccess ||
-                    hipMemcpyAsync(j.win[1], c.r2, b[1], hipMemcpyHostToDevice, s) != hipSuccess) {
+                // the window's previous chunk has finished with it before the copy
+                if ((j.win_used[w] && hipStreamWaitEvent(cs, j.comp_ev[w], 0) != hipSuccess) ||
+                    hipMemcpyAsync(j.win[w][0], c.r1, b[0], hipMemcpyHostToDevice, cs) != hipSuccess ||
+                    hipMemcpyAsync(j.win[w][1], c.r2, b[1], hipMemcpyHostToDevice, cs) != hipSuccess ||
+                    hipEventRecord(j.copy_ev[w], cs) != hipSuccess ||
+                    hipStreamWaitEvent(s, j.copy_ev[w], 0) != hipSuccess) {
                     fail_active(j, "input copy failed");
                     return true;
                 }
-                d1 = (const uint8_t *)j.win[0];
-                d2 = (const uint8_t *)j.win[1];
+                d1 = (const uint8_t *)j.win[w][0];
+                d2 = (const uint8_t *)j.win[w][1];
             }
             fxc_delay_cfg dc = j.desc.delay;
             dc.geom_table = j.table.empty() ? nullptr : j.table.data();
@@ -324,6 +438,15 @@ struct fxc_server {
                 fail_active(j, "accumulate failed");
                 return true;
             }
+            if (!j.desc.input_on_device) {
+                const int w = (int)(j.nchunk & 1);
+                if (hipEventRecord(j.comp_ev[w], s) != hipSuccess) {
+                    fail_active(j, "event record failed");
+                    return true;
+                }
+                j.win_used[w] = true;
+            }
+            j.nchunk++;
             j.frames += c.nf;
         }
         if (j.closed && !j.final_enqueued) {
@@ -363,6 +486,7 @@ struct fxc_server {
     }
 
     void fail_active(Job &j, const std::string &msg) {
+        (void)hipStreamSynchronize(copy_streams[j.stream]);
         (void)hipStreamSynchronize(streams[j.stream]);
         retire(j, false);
         fail_job(j, msg);
@@ -427,13 +551,17 @@ extern "C" fxc_status fxc_server_create(const fxc_server_cfg *cfg, fxc_server **
     auto *s = new fxc_server();
     s->cfg = *cfg;
     for (int i = 0; i < cfg->n_streams; ++i) {
-        hipStream_t st = nullptr;
-        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+        hipStream_t st = nullptr, cs = nullptr;
+        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
+            hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) {
+            if (st) (void)hipStreamDestroy(st);
             for (hipStream_t x : s->streams) (void)hipStreamDestroy(x);
+            for (hipStream_t x : s->copy_streams) (void)hipStreamDestroy(x);
             delete s;
             return sfail(FXC_ERR_HIP, "stream creation failed");
         }
         s->streams.push_back(st);
+        s->copy_streams.push_back(cs);
         s->stream_jobs.push_back(0);
     }
     s->worker = std::thread([s] { s->run(); });
@@ -452,7 +580,9 @@ extern "C" fxc_status fxc_server_destroy(fxc_server *s) {
     s->cv.notify_all();
     s->worker.join();
     for (auto &kv : s->idle_plans) fxc_plan_destroy(kv.second);
+    s->release_pools();
     for (hipStream_t st : s->streams) (void)hipStreamDestroy(st);
+    for (hipStream_t st : s->copy_streams) (void)hipStreamDestroy(st);
     delete s;
     return FXC_OK;
 }
