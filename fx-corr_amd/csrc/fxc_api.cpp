@@ -758,6 +758,7 @@ static std::vector<uint8_t> mfma_tables(const fxc_plan *p) {
 }
 
 static bool large_fused_possible(const fxc_plan *p);
+static bool phased_fused_possible(const fxc_plan *p);
 
 extern "C" fxc_status fxc_plan_create(const fxc_plan_desc *desc, fxc_plan **out) {
     if (!desc || !out) return fail(FXC_ERR_INVALID_ARG, "null argument");
@@ -900,14 +901,15 @@ extern "C" fxc_status fxc_plan_create(const fxc_plan_desc *desc, fxc_plan **out)
     // threads each); MFMA form one 8-wave workgroup per CU; large-N form one
     // partial per frame group.  Each workgroup runs a contiguous range of
     // frames.
-    if (large_fused_possible(p)) {   // the fused four-step is the large form's default
+    if (large_fused_possible(p) || phased_fused_possible(p)) {   // the fused four-step is the default
         p->variant = 1;
         p->M1 = 128;
         p->M2 = (int)M / 128;
         if (p->fft_form == 0) {
             // N = 16384: Y of both stations for a chunk of frames, the frame
-            // parameters and shifts, the W_M and W_64 tables
-            const int64_t per_frame = 2 * (int64_t)M * 8 + 16 + (int64_t)frame_param_bytes();
+            // (or phased) parameters and shifts, the W_M and W_64 tables
+            const size_t pb = std::max(frame_param_bytes(), phased_param_bytes());
+            const int64_t per_frame = 2 * (int64_t)M * 8 + 16 + (int64_t)pb;
             p->chunk = std::max<int64_t>(16, ((int64_t)512 << 20) / per_frame);
             p->chunk_use = p->chunk;
             const size_t C = (size_t)p->chunk;
@@ -923,8 +925,7 @@ extern "C" fxc_status fxc_plan_create(const fxc_plan_desc *desc, fxc_plan **out)
             };
             if (!table(p->tw_l0, 64) || !table(p->tw_lm, (int)M) ||
                 p->d_z[0].ensure(C * M * 8) != hipSuccess || p->d_z[1].ensure(C * M * 8) != hipSuccess ||
-                p->d_params.ensure(C * frame_param_bytes()) != hipSuccess ||
-                p->d_lshifts.ensure(C * 16) != hipSuccess)
+                p->d_params.ensure(C * pb) != hipSuccess || p->d_lshifts.ensure(C * 16) != hipSuccess)
                 return cleanup(fail(FXC_ERR_NOMEM, "fused four-step scratch allocation failed"));
         }
     }
@@ -1469,6 +1470,21 @@ static bool large_fused_ok(const fxc_plan *p) {
     return p->variant == 1 && p->M1 == 128 && large_fused_possible(p);
 }
 
+// The phased sum at N = 16384 on the fused four-step (chosen form): the
+// same first pass, the phased bin in the second pass; as for the products,
+// both stations' bins of a pair on the same rows (band shift a multiple of
+// M1) and code widths of at most 16 bits.
+static bool phased_fused_possible(const fxc_plan *p) {
+    const int64_t sb = p->desc.a2s - p->desc.a1s;
+    return p->fft_form == 0 && p->N == 16384 && p->desc.precision == FXC_PRECISION_CHOSEN &&
+           p->mf_b1 && (p->desc.products & (FXC_PRODUCT_PHASED | FXC_PRODUCT_RAW_OUT)) &&
+           (sb % 128) == 0 && p->st[0].bits <= 16 && p->st[1].bits <= 16;
+}
+
+static bool phased_fused_ok(const fxc_plan *p) {
+    return p->variant == 1 && p->M1 == 128 && phased_fused_possible(p);
+}
+
 // Frame groups of the fused second pass: enough workgroups (M1 / 2 row
 // pairs x groups) for one resident round at the kernel's occupancy (three
 // workgroups per CU at M2 = 256, five at M2 = 64 / 128), at most the plan's
@@ -1476,6 +1492,43 @@ static bool large_fused_ok(const fxc_plan *p) {
 static int large_fused_groups(const fxc_plan *p) {
     const int per_cu = p->M2 == 256 ? 3 : 5;
     return std::max(1, std::min(p->max_wg, num_cus() * per_cu / (p->M1 / 2)));
+}
+
+// Phased sum at N = 16384 on the fused four-step: per chunk the phased
+// parameters and shifts, the fused first pass of both stations, the second
+// pass with the phased bin into the frame-group partial slots (pair-major)
+// and, for the .raw output, cb of every frame in row order (bin k1 + M1 k2
+// at M2 k1 + k2: the pair's rows are contiguous), then the inverse FFT and
+// quantiser reading that order.
+static fxc_status enqueue_phased_fused(fxc_plan *p, const SectorArgs &base, int64_t nf,
+                                       uint8_t *d_raw_out, double *d_acc, hipStream_t s) {
+    const bool raw = (p->desc.products & FXC_PRODUCT_RAW_OUT) && d_raw_out != nullptr;
+    const int64_t bpf_o = plan_bpf_out(p);
+    const int ng = large_fused_groups(p);
+    if (raw) GROW(p, p->d_cb, (size_t)p->chunk_use * p->half * 8, s, "phased spectra");
+    for (int64_t c0 = 0; c0 < nf; c0 += p->chunk_use) {
+        const int64_t cn = std::min(p->chunk_use, nf - c0);
+        SectorArgs a = base;
+        a.raw1 = base.raw1 + c0 * p->st[0].bpf;
+        a.raw2 = base.raw2 + c0 * p->st[1].bpf;
+        a.first_frame = base.first_frame + c0;
+        a.nf = cn;
+        a.part = (double *)p->part.p;
+        a.cb_out = raw ? (float2 *)p->d_cb.p : nullptr;
+        int64_t *shifts = (int64_t *)p->d_lshifts.p;
+        HIP_TRY(launch_phased_params(a, 1, p->d_params.p, shifts, s));
+        HIP_TRY(launch_large_pass1_fused(a, shifts, (float2 *)p->d_z[0].p, (float2 *)p->d_z[1].p,
+                                         p->M1, p->M2, (const float2 *)p->tw_lm.p, s));
+        HIP_TRY(launch_large_pass2_phased(a, (const float2 *)p->d_z[0].p, (const float2 *)p->d_z[1].p,
+                                          p->d_params.p, p->M1, p->M2, (const float2 *)p->tw_l0.p, ng,
+                                          c0 > 0 ? 1 : 0, s));
+        if (raw)   // cb in the second pass's row order
+            HIP_TRY(launch_irfft_quantise(a, (const float2 *)p->d_cb.p, cn, d_raw_out + c0 * bpf_o,
+                                          p->desc.output_lsb != 0, s, true));
+    }
+    HIP_TRY(launch_reduce_pairs((const double *)p->part.p, ng, p->half, p->M1, p->M2, d_acc,
+                                (double *)p->red_scratch.p, 64, s));
+    return FXC_OK;
 }
 
 static fxc_status enqueue_large(fxc_plan *p, const SectorArgs &base, int64_t nf, double *d_acc,
@@ -1576,6 +1629,7 @@ static fxc_status enqueue_sector(fxc_plan *p, const uint8_t *d_raw1, const uint8
     a.first_frame = first_frame;
     a.nf = nf;
     if (phased && p->fft_form == 2) return enqueue_phased_large(p, a, nf, d_raw_out, d_acc, s);
+    if (phased && phased_fused_ok(p)) return enqueue_phased_fused(p, a, nf, d_raw_out, d_acc, s);
     if (phased) return enqueue_phased(p, a, nf, d_raw_out, d_acc, s);
     if (p->fft_form == 2 || large_fused_ok(p)) return enqueue_large(p, a, nf, d_acc, s);
     const int wg_cap = std::max(1, p->max_wg / p->grid_div);
@@ -1757,7 +1811,7 @@ extern "C" fxc_status fxc_plan_autotune(fxc_plan *p, const uint8_t *d_raw1, cons
 extern "C" fxc_status fxc_plan_set_tuning(fxc_plan *p, const fxc_tune_report *c) {
     if (!p || !c) return fail(FXC_ERR_INVALID_ARG, "null argument");
     const bool v1_ok = (p->fft_form == 0 && sector_has_variant(p->N)) || p->fft_form == 1 ||
-                       (large_fused_possible(p) && c->split_m1 == 128);
+                       ((large_fused_possible(p) || phased_fused_possible(p)) && c->split_m1 == 128);
     const bool v0_ok = !(p->fft_form == 1 && p->N != 1024);
     const bool v2_ok = p->fft_form == 1 && p->N == 1024;
     if ((c->variant == 1 && !v1_ok) || (c->variant == 0 && !v0_ok) || (c->variant == 2 && !v2_ok) ||
@@ -1774,7 +1828,8 @@ extern "C" fxc_status fxc_plan_set_tuning(fxc_plan *p, const fxc_tune_report *c)
         p->M1 = c->split_m1;
         p->M2 = p->M / c->split_m1;
         p->chunk_use = c->chunk_frames;
-    } else if (c->variant == 1 && large_fused_possible(p)) {   // N = 16384: the fused four-step
+    } else if (c->variant == 1 && (large_fused_possible(p) || phased_fused_possible(p))) {
+        // N = 16384: the fused four-step
         if (c->chunk_frames < 1 || c->chunk_frames > p->chunk)
             return fail(FXC_ERR_INVALID_ARG, "chunk_frames outside 1..%lld", (long long)p->chunk);
         p->chunk_use = c->chunk_frames;

@@ -146,6 +146,14 @@ __device__ void compute_frame_delay(int64_t idx, const DelayDev &c, FrameDelayDe
 // x reduced to [-1/2, 1/2] turns
 __device__ __forceinline__ double wrap_turns(double x) { return x - rint(x); }
 
+// exp(2 pi i x) for x in turns (reduced mod 1 in FP64, f32 sincos)
+__device__ __forceinline__ float2 phasor_turns(double x) {
+    const float r = (float)wrap_turns(x);
+    return make_float2(__builtin_amdgcn_cosf(r), __builtin_amdgcn_sinf(r));
+}
+
+__device__ __forceinline__ float2 c_scale(float2 a, float w) { return make_float2(a.x * w, a.y * w); }
+
 struct FrameParam {
     int32_t s1, s2;      // integer shifts clamped to [-N, N]
     float2 frmix;        // fr_lo1 * conj(fr_lo2) in f32 (plain form)
@@ -153,6 +161,13 @@ struct FrameParam {
     cd pw[kMaxPow];      // XCF phase step^(2^b) (plain form)
     double phw, dth;     // chosen form: XCF phase at overlap index 0 (fringe
                          // difference included) and its step per index, turns
+};
+
+// Per-frame parameters of the phased-array sum (k_phased_params).
+struct PhasedParam {
+    int32_t s1, s2;        // integer shifts clamped to [-N, N]
+    float2 fr1, fr2;       // fringe phasors (plain: main.rs:235-240, f32; chosen: FP64 phase)
+    double th1, th2;       // chosen: rotation per bin, turns (-df frac)
 };
 
 __device__ __forceinline__ int32_t clamp_shift(int64_t s, int n) {
@@ -485,6 +500,24 @@ __device__ __forceinline__ float2 untangle_np(const float2 *zs, int k, int M, co
 // (M + 1 bins): Z'[k] = E' + i O' (the split method run backwards), returned
 // conjugated for IFFT = conj(FFT(conj(.))).  DC/Nyquist imaginary parts are
 // ignored (the reference zeroes them first).
+template <typename At>
+__device__ __forceinline__ float2 irfft_pre_elem_at(At X, int k, int M, const float2 *tw_post) {
+    float2 z;
+    if (k == 0) {
+        const float2 x0 = X(0), xm = X(M);
+        z = make_float2(x0.x + xm.x, x0.x - xm.x);
+    } else {
+        const float2 A = X(k), Bc = X(M - k);
+        const float er = A.x + Bc.x, ei = A.y - Bc.y;
+        const float dr = A.x - Bc.x, di = A.y + Bc.y;
+        const float2 w = tw_post[k];
+        const float orr = dr * w.x - di * w.y;
+        const float oi = dr * w.y + di * w.x;
+        z = make_float2(er - oi, ei + orr);
+    }
+    return c_conj(z);
+}
+
 __device__ __forceinline__ float2 irfft_pre_elem(const float2 *X, int k, int M,
                                                  const float2 *tw_post) {
     float2 z;

@@ -145,8 +145,9 @@ hipError_t launch_irfft_frames(const SectorArgs &a, const float2 *in,
 hipError_t launch_delay_eval(const DelayDev &d, int64_t first, int64_t n,
                              FrameDelayDev *out, hipStream_t stream);
 // Inverse FFT and quantiser in one kernel (phased .raw output, N <= 16384).
+// rows: the spectra in the fused four-step's row order (N = 16384 only).
 hipError_t launch_irfft_quantise(const SectorArgs &a, const float2 *in, int64_t nframes,
-                                 uint8_t *out, bool negate_odd, hipStream_t s);
+                                 uint8_t *out, bool negate_odd, hipStream_t s, bool rows = false);
 // Quantiser over the HBM four-step inverse's complex output (N >= 32768).
 hipError_t launch_quantise_cplx(const SectorArgs &a, const float2 *z, int64_t nframes,
                                 bool negate_odd, uint8_t *out, hipStream_t s);
@@ -221,6 +222,11 @@ hipError_t launch_reduce_pairs(const double *part, int n_wg, int half, int M1, i
 hipError_t launch_large_pass2_mac(const SectorArgs &a, const float2 *y1, const float2 *y2,
                                   const FrameParam *params, int M1, int M2, const float2 *tw_m2,
                                   int n_groups, int accumulate, hipStream_t s);
+// the phased sum on the fused second pass (N = 16384, M2 = 64); pp: the
+// chunk's phased parameters (launch_phased_params)
+hipError_t launch_large_pass2_phased(const SectorArgs &a, const float2 *y1, const float2 *y2,
+                                     const void *pp, int M1, int M2, const float2 *tw_m2,
+                                     int n_groups, int accumulate, hipStream_t s);
 hipError_t launch_synth(uint8_t *d_out, int64_t nsamples, int bits, const int32_t *shuffle,
                         int lsb, uint64_t seed, int station, double rho, int64_t delay,
                         hipStream_t stream);

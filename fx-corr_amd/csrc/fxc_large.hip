@@ -799,6 +799,184 @@ __global__ __launch_bounds__(256) void k_large_pass2_r8(SectorArgs a, const floa
     }
 }
 
+// One bin of a row pair for the phased sum (k_phased_large's per-bin
+// arithmetic, main.rs:2264-2302): both stations' untangled bins, each
+// rotated by exp(2 pi i b th) from the FP64 phase and by its fringe phasor,
+// cb = w1 c1 + w2 c2, |cb|^2 into the FP64 sum and, with cbf, cb (DC and
+// Nyquist imaginary parts zeroed) into the frame's spectrum in row order
+// (bin row + M1 k2 at M2 row + k2, bin M at M).
+template <int M2, int LP>
+__device__ __forceinline__ void row_bin_ph(const float2 (*zb)[LP], const float2 (*twp)[M2], bool rb_sel,
+                                           int k2, bool nyq, int M, int M1, int ra, int rb, int gs,
+                                           int ov, int sbo, int jo, int grid, const PhasedParam &P,
+                                           float w1, float w2, float2 *cbf, double &acc) {
+    const int row = nyq ? 0 : (rb_sel ? rb : ra);
+    if (nyq) k2 = M2;
+    const int g = nyq ? M : row + M1 * k2;
+    const int bg = row == ra ? 0 : 1;
+    const int bm = (row == 0 ? row : M1 - row) == ra ? 0 : 1;
+    const int km = row == 0 ? (M2 - k2) & (M2 - 1) : M2 - 1 - k2;
+    const float2 Zg = zb[bg][pad(k2 & (M2 - 1))], Zm = zb[bm][pad(km)];
+    const float2 Xg = untangle_ab(Zg, Zm, g, M, twp[bg][k2 & (M2 - 1)]);
+    const bool in_ov = g >= gs && g < gs + ov;
+    const double thg = grid == 0 ? P.th1 : P.th2, tho = grid == 0 ? P.th2 : P.th1;
+    const float2 frg = grid == 0 ? P.fr1 : P.fr2, fro = grid == 0 ? P.fr2 : P.fr1;
+    float2 sg = c_mul(Xg, phasor_turns((double)g * thg));
+    if (g == M) sg.y = 0.0f;                              // Nyquist (utils.rs:118-123)
+    const float2 cg = c_mul(sg, frg);
+    float2 co = make_float2(0.0f, 0.0f);
+    if (in_ov) {
+        const int h = g + sbo, kh = k2 + jo;
+        const int khm = row == 0 ? (M2 - kh) & (M2 - 1) : M2 - 1 - kh;
+        const float2 Oh = zb[2 + bg][pad(kh & (M2 - 1))];
+        const float2 Om = zb[2 + bm][pad(khm & (M2 - 1))];
+        float2 so = c_mul(untangle_ab(Oh, Om, h, M, twp[bg][kh & (M2 - 1)]), phasor_turns((double)h * tho));
+        if (h == M) so.y = 0.0f;
+        co = c_mul(so, fro);
+    }
+    const float2 c1 = grid == 0 ? cg : co, c2 = grid == 0 ? co : cg;
+    float2 cb;
+    {
+#pragma clang fp contract(off)
+        const float2 a1 = c_scale(c1, w1), a2 = c_scale(c2, w2);
+        cb = make_float2(a1.x + a2.x, a1.y + a2.y);
+    }
+    acc += (double)norm_sqr(cb);
+    if (cbf) {
+        if (g == 0 || g == M) cb.y = 0.0f;                // main.rs:2283-2284
+        cbf[nyq ? M : row * M2 + k2] = cb;
+    }
+}
+
+// The phased sum on the second pass at M2 = 64 (N = 16384): k_large_pass2_r8
+// with the phased bin in place of the ACF/XCF products.  For the .raw
+// output each bin's cb goes to a.cb_out (frame-major, M + 1 entries, row
+// order: a wave's 64 bins of one row are one contiguous 512-byte store) for
+// the inverse FFT and quantiser.
+__global__ __launch_bounds__(256, 5) void k_large_pass2_r8_ph(SectorArgs a, const float2 *y1,
+                                                           const float2 *y2, const PhasedParam *pp,
+                                                           int M1, const float2 *tw_m2, int64_t fpg,
+                                                           int accumulate) {
+    constexpr int M2 = 64, LP = M2 + M2 / 16 + 2;
+    __shared__ float2 zr[8][4][LP];   // [frame of the iteration][grid ra, grid rb, other ra, other rb]
+    __shared__ float2 twp[2][M2];     // tw_post[row + M1 k2], rows ra / rb
+    __shared__ double hsum[128];      // threads 128..255's sums
+    __shared__ double msum;           // thread 128's bin-M sum (pair 0)
+    const int M = a.M;
+    const int half = M + 1;
+    const int tid = threadIdx.x;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int lane = tid & 63;
+    const int q = lane >> 3, j = lane & 7;
+    const int fw = q >> 2, rk = q & 3;
+    const int p = blockIdx.x;
+    const int ra = p, rb = p == 0 ? M1 / 2 : M1 - p;
+    for (int i = tid; i < 2 * M2; i += 256)
+        twp[i / M2][i % M2] = a.tw_post[(i < M2 ? ra : rb) + M1 * (i % M2)];
+    float2 tw[8];
+#pragma unroll
+    for (int r = 1; r < 8; ++r) tw[r] = tw_m2[(j * r) & (M2 - 1)];
+    const int64_t g0 = (int64_t)blockIdx.y * fpg;
+    int64_t g1 = g0 + fpg;
+    if (g1 > a.nf) g1 = a.nf;
+    const float w1 = (float)a.w1, w2 = (float)a.w2;
+    float2 *cbo = ((a.flags & FLAG_RAW_OUT) != 0) ? a.cb_out : nullptr;
+    const int gs = (a.grid == 0) ? a.a1s : a.a2s;
+    const int sbo = (a.grid == 0) ? (a.a2s - a.a1s) : (a.a1s - a.a2s);
+    const int jo = sbo / M1;
+    const float2 *src_row = ((rk & 2) ? (a.grid == 0 ? y2 : y1) : (a.grid == 0 ? y1 : y2)) +
+                            (int64_t)((rk & 1) ? rb : ra) * M2 + j;
+    const int bsel = tid & 127, fsel = tid >> 7;
+    const bool rb_sel = bsel >= M2;
+    const int k2 = bsel & (M2 - 1);
+    const bool nyq = p == 0 && bsel == 0;   // threads 0 and 128 also take bin M
+    double acc = 0.0, nacc = 0.0;
+    const int slot = 2 * w + fw;   // this lane's frame of the iteration
+    float2 cur[8];
+    {
+        const int64_t f = g0 + slot;
+#pragma unroll
+        for (int r = 0; r < 8; ++r)
+            cur[r] = f < g1 ? src_row[f * (int64_t)M + 8 * r] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();   // twiddles staged
+    float2 *z = zr[slot][rk];
+    for (int64_t it = g0; it < g1; it += 8) {
+        const int64_t f = it + slot, fn = f + 8;
+        float2 nxt[8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r)
+            nxt[r] = fn < g1 ? src_row[fn * (int64_t)M + 8 * r] : make_float2(0.f, 0.f);
+        if (f < g1) {
+            dftR<8>(cur);                                // pass 1: inputs j + 8 r
+#pragma unroll
+            for (int r = 1; r < 8; ++r) cur[r] = c_mul(cur[r], tw[r]);
+#pragma unroll
+            for (int r = 0; r < 8; ++r) z[pad(8 * j + r)] = cur[r];
+            wave_sync_lds();
+#pragma unroll
+            for (int r = 0; r < 8; ++r) cur[r] = z[pad(j + 8 * r)];
+            dftR<8>(cur);                                // pass 2
+            wave_sync_lds();
+#pragma unroll
+            for (int r = 0; r < 8; ++r) z[pad(j + 8 * r)] = cur[r];   // X[j + 8 r]
+        }
+        __syncthreads();   // the iteration's rows are ready
+        const int nfr = (int)((g1 - it) < 8 ? (g1 - it) : 8);
+        for (int s2 = fsel; s2 < nfr; s2 += 2) {
+            const PhasedParam P = pp[it + s2];
+            float2 *cbf = cbo ? cbo + (it + s2) * (int64_t)half : nullptr;
+            row_bin_ph<M2, LP>(zr[s2], twp, rb_sel, k2, false, M, M1, ra, rb, gs, a.ov, sbo, jo,
+                               a.grid, P, w1, w2, cbf, acc);
+            if (nyq)
+                row_bin_ph<M2, LP>(zr[s2], twp, false, 0, true, M, M1, ra, rb, gs, a.ov, sbo, jo,
+                                   a.grid, P, w1, w2, cbf, nacc);
+        }
+        __syncthreads();   // every thread is done with the rows
+#pragma unroll
+        for (int r = 0; r < 8; ++r) cur[r] = nxt[r];
+    }
+    if (tid >= 128) {
+        hsum[bsel] = acc;
+        if (nyq) msum = nacc;
+    }
+    __syncthreads();
+    if (tid >= 128) return;
+    // partial slot in pair-major order (as k_large_pass2_mac); the other
+    // products stay zero
+    double *out = a.part + (size_t)blockIdx.y * kNumProducts * half;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        int g;
+        double v;
+        if (i == 0) {
+            g = p * 2 * M2 + bsel;
+            v = acc + hsum[bsel];
+        } else {
+            if (!nyq) continue;
+            g = M;
+            v = nacc + msum;
+        }
+        out[P_PH * half + g] = accumulate ? out[P_PH * half + g] + v : v;
+        out[P_11 * half + g] = 0.0;
+        out[P_22 * half + g] = 0.0;
+        out[P_12R * half + g] = 0.0;
+        out[P_12I * half + g] = 0.0;
+    }
+}
+
+hipError_t launch_large_pass2_phased(const SectorArgs &a, const float2 *y1, const float2 *y2,
+                                     const void *pp, int M1, int M2, const float2 *tw_m2,
+                                     int n_groups, int accumulate, hipStream_t s) {
+    const int sbo = (a.grid == 0) ? (a.a2s - a.a1s) : (a.a1s - a.a2s);
+    if (M1 * M2 != a.M || sbo % M1 != 0 || M2 != 64) return hipErrorInvalidValue;
+    const int64_t fpg = (a.nf + n_groups - 1) / n_groups;
+    const dim3 grid((unsigned)(M1 / 2), (unsigned)n_groups);
+    hipLaunchKernelGGL(k_large_pass2_r8_ph, grid, dim3(256), 0, s, a, y1, y2,
+                       (const PhasedParam *)pp, M1, tw_m2, fpg, accumulate);
+    return hipGetLastError();
+}
+
 hipError_t launch_large_pass2_mac(const SectorArgs &a, const float2 *y1, const float2 *y2,
                                   const FrameParam *params, int M1, int M2, const float2 *tw_m2,
                                   int n_groups, int accumulate, hipStream_t s) {

@@ -15,12 +15,6 @@
 
 namespace fxc {
 
-struct PhasedParam {
-    int32_t s1, s2;        // integer shifts clamped to [-N, N]
-    float2 fr1, fr2;       // fringe phasors (plain: main.rs:235-240, f32; chosen: FP64 phase)
-    double th1, th2;       // chosen: rotation per bin, turns (-df frac)
-};
-
 template <int PREC>
 __device__ void make_phased_param(int64_t idx, const SectorArgs &a, PhasedParam &p) {
     FrameDelayDev d;
@@ -42,14 +36,6 @@ __device__ void make_phased_param(int64_t idx, const SectorArgs &a, PhasedParam 
         p.th2 = -a.df * d.frac2;
     }
 }
-
-// exp(2 pi i x) for x in turns (reduced mod 1 in FP64, f32 sincos)
-__device__ __forceinline__ float2 phasor_turns(double x) {
-    const float r = (float)wrap_turns(x);
-    return make_float2(__builtin_amdgcn_cosf(r), __builtin_amdgcn_sinf(r));
-}
-
-__device__ __forceinline__ float2 c_scale(float2 a, float w) { return make_float2(a.x * w, a.y * w); }
 
 // accumulate = 0: the workgroup's partial slot is set; 1: added to (the
 // sector is processed in chunks that reuse the same slots in order).

@@ -1109,6 +1109,76 @@ def test_fused_four_step_fft16384(bits, sb, grid, prods, lsb, delay, nf, chunk, 
         assert zero_pattern_equal(fused[k], stockham[k]), k
 
 
+def _raw_code_diff(got, ref, n, bit_out, shuffle, levels, nf):
+    dp = out_decode_plan(bit_out, shuffle, levels)
+    bpf_o = n * bit_out // 8
+    diff = 0
+    for f in range(nf):
+        seg = slice(f * bpf_o, (f + 1) * bpf_o)
+        _, vg = dp.decode(got[seg], n, False)
+        _, vr = dp.decode(ref[seg], n, False)
+        diff += int(np.count_nonzero(vg != vr))
+    return diff
+
+
+@pytest.mark.parametrize("bits,sb,grid,lsb_out,raw,nf,chunk,shuffle", [
+    ((2, 8), 0, 0, False, True, 300, 0, True),
+    ((8, 1), 128, 1, True, True, 5, 0, False),        # fewer frames than groups
+    ((4, 2), -256, 0, False, True, 300, 37, True),    # several chunks
+    ((1, 8), 2048, 1, True, True, 200, 0, False),
+    ((5, 3), 0, 0, True, True, 120, 0, True),         # 5-bit output: codes straddle words
+    ((2, 2), -128, 1, False, False, 100, 23, False),  # power only, several chunks
+])
+def test_phased_fused_four_step_fft16384(bits, sb, grid, lsb_out, raw, nf, chunk, shuffle):
+    """The phased sum on the fused four-step at N = 16384 (the chosen form's
+    default there when the band shift is a multiple of 128): |cb|^2 within
+    the chosen bar of the oracle's FP64 phased branch and of the Stockham
+    phased kernel (variant 0), .raw codes equal except at rare quantiser
+    decision boundaries."""
+    n, fs = 16384, 256e6
+    half = n // 2
+    a1s = max(0, -sb)
+    band = (sb, a1s, min(half, half - sb), a1s + sb)
+    rng = np.random.default_rng(nf + abs(sb) + bits[0])
+    prods = F.PRODUCT_PHASED | (F.PRODUCT_RAW_OUT if raw else 0)
+    c = make_case(n=n, fs=fs, nf=nf, bits=bits, shuffles=(rng.permutation(32) if shuffle else None, None),
+                  lsb=(False, True), band=band, grid=grid, rho=0.4, delay_samples=-7.3,
+                  seed=n + nf + abs(sb), delay_user_samples=-7.3, extra_delay_rate_sps=2e-9,
+                  lo1_hz=6.1e9, lo2_hz=6.1e9 - sb * fs / n, products=prods)
+    w = (0.8, 1.25)
+    bit_out = max(bits)
+
+    def plan(variant):
+        q = F.Plan(n, bits=bits, levels=c["levels"], shuffles=c["shuffles"], lsb=c["lsb"], band=band,
+                   df_hz=fs / n, products=prods, out_grid=grid, w=w, bit_out=bit_out,
+                   output_lsb=lsb_out, precision=F.PRECISION_CHOSEN)
+        assert q.info.fft_form == 0 and q.info.variant == 1
+        if variant == 1 and chunk:
+            F.plan_set_tuning(q, variant=1, split_m1=128, chunk_frames=chunk)
+        elif variant == 0:
+            F.plan_set_tuning(q, variant=0)
+        return q
+
+    cfg = F.make_delay_cfg(c["table"], **c["delay"])
+    got = plan(1).sector_run(c["raw1"], c["raw2"], 0, nf, cfg)
+    stk = plan(0).sector_run(c["raw1"], c["raw2"], 0, nf, cfg)
+    ref = R.sector_run(_phased_desc(c, w, bit_out, lsb_out), c["raw1"], c["raw2"], 0, nf,
+                       R.make_delay_cfg(c["table"], **c["delay"]), mode=R.MODE_F64, threads=8)
+    assert got["n_failed"] == 0 and ref["n_failed"] == 0
+    e_ref, e_stk = rel_err(got["acc_ph"], ref["acc_ph"]), rel_err(got["acc_ph"], stk["acc_ph"])
+    print(f"[phased fused] bits {bits} sb {sb}: acc_ph vs oracle {e_ref:.2e}, vs Stockham {e_stk:.2e}",
+          flush=True)
+    assert e_ref < CHOSEN_TOL and e_stk < CHOSEN_TOL
+    for k in ("acc11", "acc22", "acc12"):
+        assert np.all(got[k] == 0)
+    if raw:
+        assert got["raw_out"].size == nf * n * bit_out // 8
+        for other in (ref, stk):
+            d = _raw_code_diff(got["raw_out"], other["raw_out"], n, bit_out, c["shuffles"][0],
+                               c["levels"][0], nf)
+            assert d <= 1e-4 * n * nf + 2, d
+
+
 @pytest.mark.parametrize("n,bits", [(32768, (3, 5)), (65536, (5, 3))])
 def test_large_fused_odd_widths_and_shuffle(n, bits):
     """The fused four-step's run-time code width (3 and 5 bits: codes that
