@@ -760,6 +760,16 @@ static std::vector<uint8_t> mfma_tables(const fxc_plan *p) {
 static bool large_fused_possible(const fxc_plan *p);
 static bool phased_fused_possible(const fxc_plan *p);
 
+// The fused four-step's default chunk: both stations' Y (2 M complex per
+// frame) in 128 MiB, so that the second pass reads what the first wrote
+// from the 256 MB Infinity Cache rather than from HBM (1024 / 512 / 256
+// frames at N = 16384 / 32768 / 65536; measured 5-14 % faster than chunks
+// twice as large, and faster than half as large).
+static int64_t fused_chunk_default(const fxc_plan *p) {
+    const int64_t c = std::max<int64_t>(16, ((int64_t)128 << 20) / (16 * (int64_t)p->M));
+    return std::min<int64_t>(p->chunk, c);
+}
+
 extern "C" fxc_status fxc_plan_create(const fxc_plan_desc *desc, fxc_plan **out) {
     if (!desc || !out) return fail(FXC_ERR_INVALID_ARG, "null argument");
     *out = nullptr;
@@ -905,13 +915,13 @@ extern "C" fxc_status fxc_plan_create(const fxc_plan_desc *desc, fxc_plan **out)
         p->variant = 1;
         p->M1 = 128;
         p->M2 = (int)M / 128;
+        if (p->fft_form == 2) p->chunk_use = fused_chunk_default(p);
         if (p->fft_form == 0) {
             // N = 16384: Y of both stations for a chunk of frames, the frame
             // (or phased) parameters and shifts, the W_M and W_64 tables
             const size_t pb = std::max(frame_param_bytes(), phased_param_bytes());
             const int64_t per_frame = 2 * (int64_t)M * 8 + 16 + (int64_t)pb;
             p->chunk = std::max<int64_t>(16, ((int64_t)512 << 20) / per_frame);
-            p->chunk_use = p->chunk;
             const size_t C = (size_t)p->chunk;
             auto table = [](DevBuf &b, int n) {
                 std::vector<float> t(2 * (size_t)n);
@@ -923,6 +933,7 @@ extern "C" fxc_status fxc_plan_create(const fxc_plan_desc *desc, fxc_plan **out)
                 return b.ensure(t.size() * 4) == hipSuccess &&
                        hipMemcpy(b.p, t.data(), t.size() * 4, hipMemcpyHostToDevice) == hipSuccess;
             };
+            p->chunk_use = fused_chunk_default(p);
             if (!table(p->tw_l0, 64) || !table(p->tw_lm, (int)M) ||
                 p->d_z[0].ensure(C * M * 8) != hipSuccess || p->d_z[1].ensure(C * M * 8) != hipSuccess ||
                 p->d_params.ensure(C * pb) != hipSuccess || p->d_lshifts.ensure(C * 16) != hipSuccess)
@@ -1706,23 +1717,25 @@ extern "C" fxc_status fxc_plan_autotune(fxc_plan *p, const uint8_t *d_raw1, cons
         int64_t chunk;
     };
     std::vector<Cand> cands;
+    // the fused four-step at its default chunk (timed first), twice and half it
+    auto add_fused = [p](std::vector<Cand> &cs) {
+        const int64_t d = fused_chunk_default(p);
+        for (int64_t c : {d, 2 * d, d / 2})
+            if (c >= 16 && c <= p->chunk) cs.push_back({1, 1, 128, c});
+    };
     if (p->fft_form == 2) {
         // variant 1: the fused four-step (M1 = 128), the default where it
         // applies and so timed first; variant 0: the four-step through HBM
         // at either split
         const int M = p->M;
-        if (large_fused_possible(p))
-            for (int64_t c : {p->chunk, p->chunk / 2, p->chunk / 4})
-                if (c >= 16) cands.push_back({1, 1, 128, c});
+        if (large_fused_possible(p)) add_fused(cands);
         std::vector<int> m1s = {M / 128};
         if (M / 128 != 128) m1s.push_back(128);
         for (int m1 : m1s)
             for (int64_t c : {p->chunk, p->chunk / 2, p->chunk / 4})
                 if (c >= 16) cands.push_back({0, 1, m1, c});
     } else if (p->fft_form == 0) {
-        if (large_fused_possible(p))   // N = 16384: the fused four-step (the default) first
-            for (int64_t c : {p->chunk, p->chunk / 2, p->chunk / 4})
-                if (c >= 16) cands.push_back({1, 1, 128, c});
+        if (large_fused_possible(p)) add_fused(cands);   // N = 16384: the default first
         const int nv = sector_has_variant(p->N) ? 2 : 1;
         for (int v = 0; v < nv; ++v)
             for (int d : {1, 2, 4}) cands.push_back({v, d, 0, 0});

@@ -422,7 +422,9 @@ fxc_status fxc_plan_device_bytes(const fxc_plan *plan, uint64_t *bytes);
  * later calls.  Candidates: the FFT pass sequence (radix-8 or radix-16
  * passes, N = 2048..8192) and workgroups per launch (frames per workgroup)
  * for the Stockham form; the four-step split M1 x M2 and the chunk length
- * for N >= 32768; the kernel organization for the MFMA form at N = 1024
+ * for N >= 32768 (the fused four-step, also at N = 16384: its default chunk
+ * keeps both stations' intermediate rows within 128 MiB, twice and half
+ * that are tried too); the kernel organization for the MFMA form at N = 1024
  * (every wave carrying whole frames, producer/consumer waves, or
  * producer/consumer waves with FP32 batch sums; N = 2048 has the
  * producer/consumer organization only).  Results are the same sums up to

@@ -1128,6 +1128,7 @@ def _raw_code_diff(got, ref, n, bit_out, shuffle, levels, nf):
     ((1, 8), 2048, 1, True, True, 200, 0, False),
     ((5, 3), 0, 0, True, True, 120, 0, True),         # 5-bit output: codes straddle words
     ((2, 2), -128, 1, False, False, 100, 23, False),  # power only, several chunks
+    ((2, 1), 0, 1, False, True, 1100, 0, False),      # past the default chunk (1024 frames)
 ])
 def test_phased_fused_four_step_fft16384(bits, sb, grid, lsb_out, raw, nf, chunk, shuffle):
     """The phased sum on the fused four-step at N = 16384 (the chosen form's
