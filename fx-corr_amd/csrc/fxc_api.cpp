@@ -1508,9 +1508,10 @@ static int large_fused_groups(const fxc_plan *p) {
 // Phased sum at N = 16384 on the fused four-step: per chunk the phased
 // parameters and shifts, the fused first pass of both stations, the second
 // pass with the phased bin into the frame-group partial slots (pair-major)
-// and, for the .raw output, cb of every frame in row order (bin k1 + M1 k2
-// at M2 k1 + k2: the pair's rows are contiguous), then the inverse FFT and
-// quantiser reading that order.
+// and, for the .raw output, the inverse real FFT as a four-step too: the
+// second pass forms the half-length spectrum of each pair's rows and their
+// DFT-64 (rows of 128 x 64 per frame), the column kernel the DFT-128 over
+// the rows and the quantiser.
 static fxc_status enqueue_phased_fused(fxc_plan *p, const SectorArgs &base, int64_t nf,
                                        uint8_t *d_raw_out, double *d_acc, hipStream_t s) {
     const bool raw = (p->desc.products & FXC_PRODUCT_RAW_OUT) && d_raw_out != nullptr;
@@ -1531,11 +1532,11 @@ static fxc_status enqueue_phased_fused(fxc_plan *p, const SectorArgs &base, int6
         HIP_TRY(launch_large_pass1_fused(a, shifts, (float2 *)p->d_z[0].p, (float2 *)p->d_z[1].p,
                                          p->M1, p->M2, (const float2 *)p->tw_lm.p, s));
         HIP_TRY(launch_large_pass2_phased(a, (const float2 *)p->d_z[0].p, (const float2 *)p->d_z[1].p,
-                                          p->d_params.p, p->M1, p->M2, (const float2 *)p->tw_l0.p, ng,
-                                          c0 > 0 ? 1 : 0, s));
-        if (raw)   // cb in the second pass's row order
-            HIP_TRY(launch_irfft_quantise(a, (const float2 *)p->d_cb.p, cn, d_raw_out + c0 * bpf_o,
-                                          p->desc.output_lsb != 0, s, true));
+                                          p->d_params.p, p->M1, p->M2, (const float2 *)p->tw_l0.p,
+                                          (const float2 *)p->tw_lm.p, raw, ng, c0 > 0 ? 1 : 0, s));
+        if (raw)   // the inverse FFT's row half done: columns and quantiser
+            HIP_TRY(launch_irfft_cols_quantise(a, (const float2 *)p->d_cb.p, cn, d_raw_out + c0 * bpf_o,
+                                               p->desc.output_lsb != 0, (const float2 *)p->tw_lm.p, s));
     }
     HIP_TRY(launch_reduce_pairs((const double *)p->part.p, ng, p->half, p->M1, p->M2, d_acc,
                                 (double *)p->red_scratch.p, 64, s));

@@ -780,6 +780,90 @@ __global__ __launch_bounds__(irq_frames<T>() * T) void k_irfft_quantise(SectorAr
     }
 }
 
+// The second half of the inverse real FFT at N = 16384 (M = 128 x 64) and
+// the quantiser.  Input per frame: the 128 x 64 block the phased second pass
+// wrote, B[k1][n2] = W_M^{n2 k1} DFT-64_{k2}(y[k1 + 128 k2]) with y the
+// conjugated half-length spectrum.  Column n2's DFT-128 over k1 gives the
+// transform of y at n2 + 64 n1, split as k1 = j + 8 r, n1 = a + 16 b:
+// thread (n2, j) takes the DFT-16 over r, times W_128^{j a}; after an LDS
+// transpose thread (n2, a pair) takes the DFT-8 over j for two a, writing
+// the natural-order series to LDS; then the samples and words as
+// k_irfft_quantise (x[2 n] = Re, x[2 n + 1] = odd_sign Im, times 1 / N).
+// One frame per workgroup pass (512 threads), frames blockIdx.x, +
+// gridDim.x, ...
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k_irfft_cols_quantise(SectorArgs a, const float2 *in,
+                                                             int64_t nframes, uint8_t *out,
+                                                             int negate_odd, const float2 *tw_m) {
+    constexpr int M = 8192, M2 = 64, ZP = M + M / 16 + 2;
+    __shared__ float2 zb[ZP];
+    __shared__ float2 w128[128];
+    extern __shared__ uint32_t dyn[];  // levels (256 floats), then the output shuffle tables
+    float *lv = reinterpret_cast<float *>(dyn);
+    uint32_t(*oshuf)[256] = reinterpret_cast<uint32_t(*)[256]>(dyn + 256);
+    const int t0 = threadIdx.x, t = t0;
+    const int jw = __builtin_amdgcn_readfirstlane(t >> 6);   // j (first phase), a pair (second)
+    const int nl = a.out_nlevels < 256 ? a.out_nlevels : 256;
+    for (int i = t; i < nl; i += 512) lv[i] = a.out_levels[i];
+    const bool ident = a.out_shuffle_ident != 0;
+    if (!ident) build_out_shuffle(oshuf, a.out_shifts, t, 512);
+    if (t < 128) w128[t] = tw_m[64 * t];
+    __syncthreads();
+    const float scale = 1.0f / (float)(2 * M);
+    const float odd_sign = negate_odd ? 1.0f : -1.0f;   // main.rs:2290-2292
+    const int nwo = a.bpf_out / 4;
+    const uint32_t mask = (a.bit_out >= 32) ? 0xFFFFFFFFu : ((1u << a.bit_out) - 1u);
+    const bool aligned = (32 % a.bit_out) == 0;
+    const QLev ql = out_qlev(a, lv);
+    auto sample = [scale, odd_sign](int i) {
+        const float2 z = zb[pad(i >> 1)];
+        return (i & 1) ? odd_sign * (z.y * scale) : z.x * scale;
+    };
+    for (int64_t f = blockIdx.x; f < nframes; f += gridDim.x) {
+        const int n2 = loop_local(t0) & 63;   // keeps per-thread addresses out of loop-invariant registers
+        const float2 *B = in + f * M;
+        float2 v[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) v[r] = B[(jw + 8 * r) * M2 + n2];
+        dftR<16>(v);                                   // over r: index a
+#pragma unroll
+        for (int q = 1; q < 16; ++q) v[q] = c_mul(v[q], w128[jw * q]);
+#pragma unroll
+        for (int q = 0; q < 16; ++q) zb[pad((q * 8 + jw) * 64 + n2)] = v[q];
+        __syncthreads();
+        float2 u[2][8];
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int q = 0; q < 8; ++q) u[h][q] = zb[pad(((2 * jw + h) * 8 + q) * 64 + n2)];
+        __syncthreads();
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            dftR<8>(u[h]);                             // over j: index b
+#pragma unroll
+            for (int b = 0; b < 8; ++b) zb[pad(n2 + 64 * (2 * jw + h + 16 * b))] = u[h][b];
+        }
+        __syncthreads();
+        for (int w = t; w < nwo; w += 512) {
+            const uint32_t word = aligned ? quant_word_aligned(sample, w, a.bit_out, ql, mask)
+                                          : quant_word(sample, 2 * M, w, a.bit_out, ql, mask);
+            reinterpret_cast<uint32_t *>(out + f * a.bpf_out)[w] = ident ? word : shuffle_out_lut(word, oshuf);
+        }
+        __syncthreads();   // zb is rewritten by the next frame
+    }
+}
+
+hipError_t launch_irfft_cols_quantise(const SectorArgs &a, const float2 *in, int64_t nframes,
+                                      uint8_t *out, bool negate_odd, const float2 *tw_m,
+                                      hipStream_t s) {
+    if (nframes <= 0) return hipSuccess;
+    if (a.N != 16384 || a.M != 8192) return hipErrorInvalidValue;
+    const size_t dyn = (size_t)(256 + (a.out_shuffle_ident ? 0 : 4 * 256)) * 4;
+    const int64_t grid = std::min<int64_t>(nframes, 2048);
+    hipLaunchKernelGGL(k_irfft_cols_quantise, dim3((unsigned)grid), dim3(512), dyn, s, a, in, nframes,
+                       out, negate_odd ? 1 : 0, tw_m);
+    return hipGetLastError();
+}
+
 template <int E, int T, bool ROWS = false>
 static hipError_t launch_irfft_quantise_t(const SectorArgs &a, const float2 *in, int64_t nframes,
                                           uint8_t *out, bool negate_odd, hipStream_t s) {

@@ -223,10 +223,16 @@ hipError_t launch_large_pass2_mac(const SectorArgs &a, const float2 *y1, const f
                                   const FrameParam *params, int M1, int M2, const float2 *tw_m2,
                                   int n_groups, int accumulate, hipStream_t s);
 // the phased sum on the fused second pass (N = 16384, M2 = 64); pp: the
-// chunk's phased parameters (launch_phased_params)
+// chunk's phased parameters (launch_phased_params); raw: also the first
+// half of the inverse FFT into a.cb_out (M per frame), finished by
+// launch_irfft_cols_quantise (tw_m: W_M table)
 hipError_t launch_large_pass2_phased(const SectorArgs &a, const float2 *y1, const float2 *y2,
                                      const void *pp, int M1, int M2, const float2 *tw_m2,
-                                     int n_groups, int accumulate, hipStream_t s);
+                                     const float2 *tw_m, bool raw, int n_groups, int accumulate,
+                                     hipStream_t s);
+hipError_t launch_irfft_cols_quantise(const SectorArgs &a, const float2 *in, int64_t nframes,
+                                      uint8_t *out, bool negate_odd, const float2 *tw_m,
+                                      hipStream_t s);
 hipError_t launch_synth(uint8_t *d_out, int64_t nsamples, int bits, const int32_t *shuffle,
                         int lsb, uint64_t seed, int station, double rho, int64_t delay,
                         hipStream_t stream);

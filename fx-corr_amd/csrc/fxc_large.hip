@@ -802,14 +802,15 @@ __global__ __launch_bounds__(256) void k_large_pass2_r8(SectorArgs a, const floa
 // One bin of a row pair for the phased sum (k_phased_large's per-bin
 // arithmetic, main.rs:2264-2302): both stations' untangled bins, each
 // rotated by exp(2 pi i b th) from the FP64 phase and by its fringe phasor,
-// cb = w1 c1 + w2 c2, |cb|^2 into the FP64 sum and, with cbf, cb (DC and
-// Nyquist imaginary parts zeroed) into the frame's spectrum in row order
-// (bin row + M1 k2 at M2 row + k2, bin M at M).
+// cb = w1 c1 + w2 c2, |cb|^2 into the FP64 sum and, with cbl, cb (DC and
+// Nyquist imaginary parts zeroed) into the frame's pair rows in LDS
+// (cbl[row ra / rb][k2], bin M in *cbn).
 template <int M2, int LP>
 __device__ __forceinline__ void row_bin_ph(const float2 (*zb)[LP], const float2 (*twp)[M2], bool rb_sel,
                                            int k2, bool nyq, int M, int M1, int ra, int rb, int gs,
                                            int ov, int sbo, int jo, int grid, const PhasedParam &P,
-                                           float w1, float w2, float2 *cbf, double &acc) {
+                                           float w1, float w2, float2 (*cbl)[M2], float2 *cbn,
+                                           double &acc) {
     const int row = nyq ? 0 : (rb_sel ? rb : ra);
     if (nyq) k2 = M2;
     const int g = nyq ? M : row + M1 * k2;
@@ -842,26 +843,38 @@ __device__ __forceinline__ void row_bin_ph(const float2 (*zb)[LP], const float2 
         cb = make_float2(a1.x + a2.x, a1.y + a2.y);
     }
     acc += (double)norm_sqr(cb);
-    if (cbf) {
+    if (cbl) {
         if (g == 0 || g == M) cb.y = 0.0f;                // main.rs:2283-2284
-        cbf[nyq ? M : row * M2 + k2] = cb;
+        if (nyq) *cbn = cb;
+        else cbl[bg][k2] = cb;
     }
 }
 
 // The phased sum on the second pass at M2 = 64 (N = 16384): k_large_pass2_r8
-// with the phased bin in place of the ACF/XCF products.  For the .raw
-// output each bin's cb goes to a.cb_out (frame-major, M + 1 entries, row
-// order: a wave's 64 bins of one row are one contiguous 512-byte store) for
-// the inverse FFT and quantiser.
-__global__ __launch_bounds__(256, 5) void k_large_pass2_r8_ph(SectorArgs a, const float2 *y1,
+// with the phased bin in place of the ACF/XCF products.  RAW (the .raw
+// output): the first half of the inverse real FFT as well.  The pair's cb
+// rows of the iteration's frames are kept in LDS; after the barrier that
+// ends the bins, lanes 0..31 of wave w take (frame, row) tasks 4 w ..
+// 4 w + 3 (frame task / 2, row ra or rb), 8 lanes each: y[k] = conj(Z'[k])
+// of the half-length complex inverse (irfft_pre_elem's arithmetic: bins k
+// and M - k lie in the pair), the row's DFT-64 over k2 (two radix-8 passes,
+// the wave's own row buffers as scratch), times W_M^{n2 k1}, stored as row
+// k1 of the frame's 128 x 64 block in a.cb_out (M entries per frame).
+// k_irfft_cols_quantise finishes the transform over k1 and quantises.
+template <bool RAW>
+__global__ __launch_bounds__(256, RAW ? 4 : 5) void k_large_pass2_r8_ph(SectorArgs a, const float2 *y1,
                                                            const float2 *y2, const PhasedParam *pp,
-                                                           int M1, const float2 *tw_m2, int64_t fpg,
+                                                           int M1, const float2 *tw_m2,
+                                                           const float2 *tw_m, int64_t fpg,
                                                            int accumulate) {
     constexpr int M2 = 64, LP = M2 + M2 / 16 + 2;
     __shared__ float2 zr[8][4][LP];   // [frame of the iteration][grid ra, grid rb, other ra, other rb]
     __shared__ float2 twp[2][M2];     // tw_post[row + M1 k2], rows ra / rb
     __shared__ double hsum[128];      // threads 128..255's sums
     __shared__ double msum;           // thread 128's bin-M sum (pair 0)
+    __shared__ float2 cbl[RAW ? 8 : 1][2][M2];   // cb of the pair's rows, per frame
+    __shared__ float2 cbn[8];                    // cb of bin M (pair 0)
+    __shared__ float2 twr[RAW ? 2 : 1][M2];      // W_M^{n2 row}, rows ra / rb
     const int M = a.M;
     const int half = M + 1;
     const int tid = threadIdx.x;
@@ -871,8 +884,10 @@ __global__ __launch_bounds__(256, 5) void k_large_pass2_r8_ph(SectorArgs a, cons
     const int fw = q >> 2, rk = q & 3;
     const int p = blockIdx.x;
     const int ra = p, rb = p == 0 ? M1 / 2 : M1 - p;
-    for (int i = tid; i < 2 * M2; i += 256)
+    for (int i = tid; i < 2 * M2; i += 256) {
         twp[i / M2][i % M2] = a.tw_post[(i < M2 ? ra : rb) + M1 * (i % M2)];
+        if constexpr (RAW) twr[i / M2][i % M2] = tw_m[((i % M2) * (i < M2 ? ra : rb)) & (M - 1)];
+    }
     float2 tw[8];
 #pragma unroll
     for (int r = 1; r < 8; ++r) tw[r] = tw_m2[(j * r) & (M2 - 1)];
@@ -880,7 +895,6 @@ __global__ __launch_bounds__(256, 5) void k_large_pass2_r8_ph(SectorArgs a, cons
     int64_t g1 = g0 + fpg;
     if (g1 > a.nf) g1 = a.nf;
     const float w1 = (float)a.w1, w2 = (float)a.w2;
-    float2 *cbo = ((a.flags & FLAG_RAW_OUT) != 0) ? a.cb_out : nullptr;
     const int gs = (a.grid == 0) ? a.a1s : a.a2s;
     const int sbo = (a.grid == 0) ? (a.a2s - a.a1s) : (a.a1s - a.a2s);
     const int jo = sbo / M1;
@@ -925,14 +939,58 @@ __global__ __launch_bounds__(256, 5) void k_large_pass2_r8_ph(SectorArgs a, cons
         const int nfr = (int)((g1 - it) < 8 ? (g1 - it) : 8);
         for (int s2 = fsel; s2 < nfr; s2 += 2) {
             const PhasedParam P = pp[it + s2];
-            float2 *cbf = cbo ? cbo + (it + s2) * (int64_t)half : nullptr;
+            float2(*cbf)[M2] = RAW ? cbl[RAW ? s2 : 0] : nullptr;
             row_bin_ph<M2, LP>(zr[s2], twp, rb_sel, k2, false, M, M1, ra, rb, gs, a.ov, sbo, jo,
-                               a.grid, P, w1, w2, cbf, acc);
+                               a.grid, P, w1, w2, cbf, cbn + s2, acc);
             if (nyq)
                 row_bin_ph<M2, LP>(zr[s2], twp, false, 0, true, M, M1, ra, rb, gs, a.ov, sbo, jo,
-                                   a.grid, P, w1, w2, cbf, nacc);
+                                   a.grid, P, w1, w2, cbf, cbn + s2, nacc);
         }
-        __syncthreads();   // every thread is done with the rows
+        __syncthreads();   // every thread is done with the rows; cb complete
+        if constexpr (RAW) {
+            // the wave's row buffers zr[2 w][*] are scratch here: the next
+            // frames' rows are written to them by this wave only
+            const int task = 4 * w + (lane >> 3);
+            const int s3 = task >> 1, b = task & 1;
+            if (lane < 32 && s3 < nfr) {
+                const int row = b ? rb : ra;
+                const int bm = (row == 0 ? row : M1 - row) == ra ? 0 : 1;
+                float2 v[8];
+#pragma unroll
+                for (int r = 0; r < 8; ++r) {
+                    const int kk = j + 8 * r;                  // k2 of bin k = row + M1 kk
+                    float2 zq;
+                    if (row == 0 && kk == 0) {
+                        const float2 x0 = cbl[s3][0][0], xm = cbn[s3];
+                        zq = make_float2(x0.x + xm.x, x0.x - xm.x);
+                    } else {
+                        const int km = row == 0 ? (M2 - kk) & (M2 - 1) : M2 - 1 - kk;
+                        const float2 A = cbl[s3][b][kk], Bc = cbl[s3][bm][km];
+                        const float er = A.x + Bc.x, ei = A.y - Bc.y;
+                        const float dr = A.x - Bc.x, di = A.y + Bc.y;
+                        const float2 wp = twp[b][kk];
+                        const float orr = dr * wp.x - di * wp.y;
+                        const float oi = dr * wp.y + di * wp.x;
+                        zq = make_float2(er - oi, ei + orr);
+                    }
+                    v[r] = c_conj(zq);
+                }
+                float2 *zs = zr[2 * w][lane >> 3];
+                dftR<8>(v);                                  // pass 1: inputs j + 8 r
+#pragma unroll
+                for (int r = 1; r < 8; ++r) v[r] = c_mul(v[r], tw[r]);
+#pragma unroll
+                for (int r = 0; r < 8; ++r) zs[pad(8 * j + r)] = v[r];
+                wave_sync_lds();
+#pragma unroll
+                for (int r = 0; r < 8; ++r) v[r] = zs[pad(j + 8 * r)];
+                dftR<8>(v);                                  // pass 2: outputs n2 = j + 8 r
+                float2 *bo = a.cb_out + (it + s3) * (int64_t)M + row * M2;
+#pragma unroll
+                for (int r = 0; r < 8; ++r) bo[j + 8 * r] = c_mul(v[r], twr[b][j + 8 * r]);
+                wave_sync_lds();
+            }
+        }
 #pragma unroll
         for (int r = 0; r < 8; ++r) cur[r] = nxt[r];
     }
@@ -967,13 +1025,19 @@ __global__ __launch_bounds__(256, 5) void k_large_pass2_r8_ph(SectorArgs a, cons
 
 hipError_t launch_large_pass2_phased(const SectorArgs &a, const float2 *y1, const float2 *y2,
                                      const void *pp, int M1, int M2, const float2 *tw_m2,
-                                     int n_groups, int accumulate, hipStream_t s) {
+                                     const float2 *tw_m, bool raw, int n_groups, int accumulate,
+                                     hipStream_t s) {
     const int sbo = (a.grid == 0) ? (a.a2s - a.a1s) : (a.a1s - a.a2s);
-    if (M1 * M2 != a.M || sbo % M1 != 0 || M2 != 64) return hipErrorInvalidValue;
+    if (M1 * M2 != a.M || sbo % M1 != 0 || M2 != 64 || M1 != 128 || (raw && !a.cb_out))
+        return hipErrorInvalidValue;
     const int64_t fpg = (a.nf + n_groups - 1) / n_groups;
     const dim3 grid((unsigned)(M1 / 2), (unsigned)n_groups);
-    hipLaunchKernelGGL(k_large_pass2_r8_ph, grid, dim3(256), 0, s, a, y1, y2,
-                       (const PhasedParam *)pp, M1, tw_m2, fpg, accumulate);
+    if (raw)
+        hipLaunchKernelGGL(k_large_pass2_r8_ph<true>, grid, dim3(256), 0, s, a, y1, y2,
+                           (const PhasedParam *)pp, M1, tw_m2, tw_m, fpg, accumulate);
+    else
+        hipLaunchKernelGGL(k_large_pass2_r8_ph<false>, grid, dim3(256), 0, s, a, y1, y2,
+                           (const PhasedParam *)pp, M1, tw_m2, tw_m, fpg, accumulate);
     return hipGetLastError();
 }
 
