@@ -607,12 +607,33 @@ struct QLev {
     float l0, inv;
 };
 
+// Mode 2 fast path: the four levels k0 .. k0 + 3 around the guess (k0 =
+// g - 1, clamped into the levels).  When they bracket x (lv[k0] <= x or
+// k0 = 0; lv[k0 + 3] > x or k0 + 3 the last level) they hold lo and lo + 1,
+// and the errors are non-increasing up to lo and non-decreasing after it, so
+// the scan's first minimum is the window's first minimum, unless that is
+// level k0 and level k0 - 1 ties with it (then the general path walks the
+// tie run).  Four independent LDS reads, no data-dependent loop.
 __device__ __forceinline__ uint32_t quant_code_q(float x, const QLev &q) {
     if (q.mode == 0) return quant_code(x, q.lv, q.nl);
     if (q.mode == 1) return quant_code_sorted(x, q.lv, q.nl);
     if (!(x == x)) return 0u;
     const float gf = fminf(fmaxf(floorf((x - q.l0) * q.inv), -1.0f), (float)(q.nl - 1));
     int g = (int)gf;
+    if (q.nl >= 4) {
+        const int k0 = min(max(g - 1, 0), q.nl - 4);
+        const float v0 = q.lv[k0], v1 = q.lv[k0 + 1], v2 = q.lv[k0 + 2], v3 = q.lv[k0 + 3];
+        if ((k0 == 0 || v0 <= x) && (k0 + 3 == q.nl - 1 || v3 > x)) {
+            float eb = fabsf(x - v0);
+            int best = k0;
+            const float e1 = fabsf(x - v1), e2 = fabsf(x - v2), e3 = fabsf(x - v3);
+            if (e1 < eb) { eb = e1; best = k0 + 1; }
+            if (e2 < eb) { eb = e2; best = k0 + 2; }
+            if (e3 < eb) { eb = e3; best = k0 + 3; }
+            if (!(best == k0 && k0 > 0 && fabsf(x - q.lv[k0 - 1]) == eb))
+                return eb < 3.40282347e+38f ? (uint32_t)best : 0u;
+        }
+    }
     while (g + 1 < q.nl && q.lv[g + 1] <= x) ++g;
     while (g >= 0 && q.lv[g] > x) --g;
     return quant_code_from_lo(x, q.lv, q.nl, g);

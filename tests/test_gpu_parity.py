@@ -162,15 +162,20 @@ def test_quantise_bit_exact(bits):
     p.close()
 
 
-@pytest.mark.parametrize("kind", ["uniform", "gaussian", "duplicates", "unsorted"])
+@pytest.mark.parametrize("kind", ["uniform", "near_uniform", "gaussian", "duplicates", "unsorted"])
 def test_quantise_nearest_level_edges(kind):
-    """Ascending levels take the bisection path, others the linear scan; both
-    give the reference's first minimum of |x - level| (f32) bit for bit at
-    midpoints (ties go to the lower index), at the levels, one ulp either
-    side of the midpoints, beyond the outer levels and for NaN / +-inf."""
+    """Evenly spaced levels (exactly, or within a quarter step) take the
+    spacing guess and its four-level window, other ascending levels the
+    bisection, others the linear scan; all give the reference's first
+    minimum of |x - level| (f32) bit for bit at midpoints (ties go to the
+    lower index), at the levels, one ulp either side of the midpoints,
+    beyond the outer levels and for NaN / +-inf."""
     rng = np.random.default_rng(len(kind))
     bits, n = 8, 1024
+    jitter = rng.uniform(-0.12, 0.12, 256)
+    jitter[[0, -1]] = 0.0                               # within a quarter step of the end-to-end line
     lv = {"uniform": uniform_levels(8) * 0.5,
+          "near_uniform": (np.arange(256) - 127.5 + jitter) * 0.37,
           "gaussian": np.sort(rng.standard_normal(256)) * 3.0,
           "duplicates": np.repeat(np.sort(rng.standard_normal(64)), 4),
           "unsorted": rng.standard_normal(256) * 2.0}[kind]
@@ -187,6 +192,33 @@ def test_quantise_nearest_level_edges(kind):
                products=F.PRODUCT_ACF, bit_out=bits)
     got = p.quantise_frames(x)
     for f in range(6):
+        assert np.array_equal(got[f], R.quantise_frame(x[f], bits, lv, sh)), f
+    p.close()
+
+
+@pytest.mark.parametrize("bits", [2, 3, 4])
+def test_quantise_near_uniform_narrow_codes(bits):
+    """Narrow output codes with levels within a quarter step of even spacing
+    (the spacing guess with a window that covers most or all of the levels):
+    bit for bit the reference's first minimum at midpoints, levels, one ulp
+    either side of the midpoints, beyond the outer levels and NaN / +-inf."""
+    rng = np.random.default_rng(bits + 90)
+    nl, n = 1 << bits, 1024
+    jitter = rng.uniform(-0.12, 0.12, nl)
+    jitter[[0, -1]] = 0.0
+    lv = ((np.arange(nl) - (nl - 1) / 2 + jitter) * 0.8).astype(np.float32)
+    srt = lv.astype(np.float64)
+    mids = ((srt[:-1] + srt[1:]) / 2).astype(np.float32)
+    pool = np.concatenate([mids, lv, np.nextafter(mids, np.float32(np.inf)),
+                           np.nextafter(mids, np.float32(-np.inf)),
+                           np.float32([np.nan, np.inf, -np.inf, 1e30, -1e30, 0.0, -0.0]),
+                           (rng.standard_normal(2000) * nl * 0.5).astype(np.float32)])
+    x = rng.choice(pool, size=(4, n)).astype(np.float32)
+    sh = rng.permutation(32).astype(np.int32)
+    p = F.Plan(n, bits=(bits, bits), levels=[lv, lv], shuffles=(sh, None), df_hz=1.0,
+               products=F.PRODUCT_ACF, bit_out=bits)
+    got = p.quantise_frames(x)
+    for f in range(4):
         assert np.array_equal(got[f], R.quantise_frame(x[f], bits, lv, sh)), f
     p.close()
 
