@@ -434,7 +434,9 @@ typedef struct {
     int32_t candidates;
     int32_t variant;           /* Stockham: 0 default passes, 1 radix-16 passes
                                   (N = 2048..8192) or, chosen form at N = 16384, the
-                                  fused four-step (split_m1 = 128, the default there);
+                                  fused four-step (split_m1 = 128, the default there;
+                                  also for phased plans through fxc_plan_set_tuning,
+                                  0 then selects the Stockham phased kernel);
                                   MFMA: 1 producer/consumer (default), 2 producer/consumer
                                   with FP32 sums over batches of 128 frames added to FP64
                                   totals, 0 whole-frame waves (2 and 0: N = 1024 only);
