@@ -497,6 +497,11 @@ fxc_status fxc_plan_check_device_error(fxc_plan *plan, void *stream);
 /* (frames in order, e.g. 64 MB e-VLBI blocks), its delay-model frame index */
 /* and FP64 partial spectra stay resident on the device between chunks, and */
 /* jobs are admitted while others run as slots and HBM budget free up.      */
+/* Host input is copied on a copy stream per job stream into two windows    */
+/* per station, alternating between chunks.  Device buffers, page-locked    */
+/* result buffers and events of finished jobs are pooled for later jobs     */
+/* (pooled bytes count against hbm_budget_bytes and are released, largest   */
+/* first, when an admission needs the room, and at fxc_server_destroy).     */
 /* ----------------------------------------------------------------------- */
 typedef struct fxc_server fxc_server;
 typedef struct {
