@@ -690,6 +690,34 @@ __device__ __forceinline__ uint32_t quant_word_aligned(X x, int w, int bits, con
     return acc;
 }
 
+// The aligned word at a compile-time width: the sample loop unrolled, so
+// the two samples of one complex value share its LDS read.
+template <int BITS, typename X>
+__device__ __forceinline__ uint32_t quant_word_bits(X x, int w, const QLev &ql) {
+    constexpr int spw = 32 / BITS;
+    constexpr uint32_t mask = BITS >= 32 ? 0xFFFFFFFFu : ((1u << BITS) - 1u);
+    uint32_t acc = 0;
+#pragma unroll
+    for (int k = 0; k < spw; ++k) acc |= (quant_code_q(x(w * spw + k), ql) & mask) << (k * BITS);
+    return acc;
+}
+
+// Word w of a frame of N samples at any width: the common widths unrolled.
+template <typename X>
+__device__ __forceinline__ uint32_t quant_word_any(X x, int N, int w, int bits, const QLev &ql,
+                                                   uint32_t mask) {
+    switch (bits) {
+    case 1: return quant_word_bits<1>(x, w, ql);
+    case 2: return quant_word_bits<2>(x, w, ql);
+    case 4: return quant_word_bits<4>(x, w, ql);
+    case 8: return quant_word_bits<8>(x, w, ql);
+    case 16: return quant_word_bits<16>(x, w, ql);
+    default:
+        return (32 % bits) == 0 ? quant_word_aligned(x, w, bits, ql, mask)
+                                : quant_word(x, N, w, bits, ql, mask);
+    }
+}
+
 // The output shuffle as four byte tables in LDS: bit 8 b + j of a word goes
 // to bit out_shifts[8 b + j] (shuffle_out_word's map).
 __device__ __forceinline__ void build_out_shuffle(uint32_t (*tab)[256], const int32_t *sh, int tid,
@@ -764,7 +792,6 @@ __global__ __launch_bounds__(irq_frames<T>() * T) void k_irfft_quantise(SectorAr
     const float2 *zs = zbuf[s];
     const int nwo = a.bpf_out / 4;
     const uint32_t mask = (a.bit_out >= 32) ? 0xFFFFFFFFu : ((1u << a.bit_out) - 1u);
-    const bool aligned = (32 % a.bit_out) == 0;
     const QLev ql = out_qlev(a, lv);
     auto sample = [zs, scale, odd_sign](int i) {
         const float2 z = zs[pad(i >> 1)];
@@ -792,8 +819,7 @@ __global__ __launch_bounds__(irq_frames<T>() * T) void k_irfft_quantise(SectorAr
         fft_forward<E, T>(zbuf[s], v, tw, tl);   // ends with a barrier: zbuf and the tables complete
         if (live)
             for (int w = tl; w < nwo; w += T) {
-                const uint32_t word = aligned ? quant_word_aligned(sample, w, a.bit_out, ql, mask)
-                                              : quant_word(sample, 2 * M, w, a.bit_out, ql, mask);
+                const uint32_t word = quant_word_any(sample, 2 * M, w, a.bit_out, ql, mask);
                 reinterpret_cast<uint32_t *>(out + f * a.bpf_out)[w] =
                     ident ? word : shuffle_out_lut(word, oshuf);
             }
@@ -833,7 +859,6 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
     const float odd_sign = negate_odd ? 1.0f : -1.0f;   // main.rs:2290-2292
     const int nwo = a.bpf_out / 4;
     const uint32_t mask = (a.bit_out >= 32) ? 0xFFFFFFFFu : ((1u << a.bit_out) - 1u);
-    const bool aligned = (32 % a.bit_out) == 0;
     const QLev ql = out_qlev(a, lv);
     auto sample = [scale, odd_sign](int i) {
         const float2 z = zb[pad(i >> 1)];
@@ -865,8 +890,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
         }
         __syncthreads();
         for (int w = t; w < nwo; w += 512) {
-            const uint32_t word = aligned ? quant_word_aligned(sample, w, a.bit_out, ql, mask)
-                                          : quant_word(sample, 2 * M, w, a.bit_out, ql, mask);
+            const uint32_t word = quant_word_any(sample, 2 * M, w, a.bit_out, ql, mask);
             reinterpret_cast<uint32_t *>(out + f * a.bpf_out)[w] = ident ? word : shuffle_out_lut(word, oshuf);
         }
         __syncthreads();   // zb is rewritten by the next frame
