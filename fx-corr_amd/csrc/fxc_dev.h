@@ -500,24 +500,6 @@ __device__ __forceinline__ float2 untangle_np(const float2 *zs, int k, int M, co
 // (M + 1 bins): Z'[k] = E' + i O' (the split method run backwards), returned
 // conjugated for IFFT = conj(FFT(conj(.))).  DC/Nyquist imaginary parts are
 // ignored (the reference zeroes them first).
-template <typename At>
-__device__ __forceinline__ float2 irfft_pre_elem_at(At X, int k, int M, const float2 *tw_post) {
-    float2 z;
-    if (k == 0) {
-        const float2 x0 = X(0), xm = X(M);
-        z = make_float2(x0.x + xm.x, x0.x - xm.x);
-    } else {
-        const float2 A = X(k), Bc = X(M - k);
-        const float er = A.x + Bc.x, ei = A.y - Bc.y;
-        const float dr = A.x - Bc.x, di = A.y + Bc.y;
-        const float2 w = tw_post[k];
-        const float orr = dr * w.x - di * w.y;
-        const float oi = dr * w.y + di * w.x;
-        z = make_float2(er - oi, ei + orr);
-    }
-    return c_conj(z);
-}
-
 __device__ __forceinline__ float2 irfft_pre_elem(const float2 *X, int k, int M,
                                                  const float2 *tw_post) {
     float2 z;

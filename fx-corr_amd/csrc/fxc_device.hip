@@ -762,9 +762,7 @@ __global__ void k_quantise_frames(SectorArgs a, const float *in, int64_t nframes
 template <int T>
 constexpr int irq_frames() { return T >= 128 ? 2 : 256 / T; }
 
-// ROWS: the spectra are in the fused four-step's row order (N = 16384:
-// bin k1 + 128 k2 at 64 k1 + k2 for k < M, bin M at M).
-template <int E, int T, bool ROWS = false>
+template <int E, int T>
 __global__ __launch_bounds__(irq_frames<T>() * T) void k_irfft_quantise(SectorArgs a, const float2 *in,
                                                           int64_t nframes, uint8_t *out,
                                                           int negate_odd) {
@@ -803,19 +801,9 @@ __global__ __launch_bounds__(irq_frames<T>() * T) void k_irfft_quantise(SectorAr
         const bool live = f < nframes;
         const int tl = loop_local(t);   // keeps per-thread addresses out of loop-invariant registers
         float2 v[E];
-        if (live) {
-            if constexpr (ROWS) {
-                constexpr int M1 = 128, M2 = M / 128;
-                const float2 *X = in + f * (M + 1);
-                auto at = [X](int k) { return X[k == M ? M : (k % M1) * M2 + k / M1]; };
-#pragma unroll
-                for (int r = 0; r < E; ++r) v[r] = irfft_pre_elem_at(at, tl + T * r, M, a.tw_post);
-            } else {
-                irfft_prepare<E, T>(in + f * (M + 1), M, a.tw_post, v, tl);
-            }
-        } else {
+        if (live) irfft_prepare<E, T>(in + f * (M + 1), M, a.tw_post, v, tl);
+        else
             for (int r = 0; r < E; ++r) v[r] = make_float2(0.f, 0.f);
-        }
         fft_forward<E, T>(zbuf[s], v, tw, tl);   // ends with a barrier: zbuf and the tables complete
         if (live)
             for (int w = tl; w < nwo; w += T) {
@@ -909,7 +897,7 @@ hipError_t launch_irfft_cols_quantise(const SectorArgs &a, const float2 *in, int
     return hipGetLastError();
 }
 
-template <int E, int T, bool ROWS = false>
+template <int E, int T>
 static hipError_t launch_irfft_quantise_t(const SectorArgs &a, const float2 *in, int64_t nframes,
                                           uint8_t *out, bool negate_odd, hipStream_t s) {
     // at most 2^22 threads (several resident rounds on 256 CUs); each
@@ -918,18 +906,14 @@ static hipError_t launch_irfft_quantise_t(const SectorArgs &a, const float2 *in,
     const int64_t ngroups = (nframes + G - 1) / G;
     const int64_t cap = ((int64_t)1 << 22) / (G * T);
     const size_t dyn = (size_t)(256 + (a.out_shuffle_ident ? 0 : 4 * 256)) * 4;
-    hipLaunchKernelGGL((k_irfft_quantise<E, T, ROWS>), dim3((unsigned)std::min<int64_t>(ngroups, cap)),
+    hipLaunchKernelGGL((k_irfft_quantise<E, T>), dim3((unsigned)std::min<int64_t>(ngroups, cap)),
                        dim3(G * T), dyn, s, a, in, nframes, out, negate_odd ? 1 : 0);
     return hipGetLastError();
 }
 
 hipError_t launch_irfft_quantise(const SectorArgs &a, const float2 *in, int64_t nframes,
-                                 uint8_t *out, bool negate_odd, hipStream_t s, bool rows) {
+                                 uint8_t *out, bool negate_odd, hipStream_t s) {
     if (nframes <= 0) return hipSuccess;
-    if (rows) {
-        if (a.N != 16384) return hipErrorInvalidValue;
-        return launch_irfft_quantise_t<16, 512, true>(a, in, nframes, out, negate_odd, s);
-    }
     switch (a.N) {
     case 64: return launch_irfft_quantise_t<2, 16>(a, in, nframes, out, negate_odd, s);
     case 128: return launch_irfft_quantise_t<2, 32>(a, in, nframes, out, negate_odd, s);

@@ -145,9 +145,8 @@ hipError_t launch_irfft_frames(const SectorArgs &a, const float2 *in,
 hipError_t launch_delay_eval(const DelayDev &d, int64_t first, int64_t n,
                              FrameDelayDev *out, hipStream_t stream);
 // Inverse FFT and quantiser in one kernel (phased .raw output, N <= 16384).
-// rows: the spectra in the fused four-step's row order (N = 16384 only).
 hipError_t launch_irfft_quantise(const SectorArgs &a, const float2 *in, int64_t nframes,
-                                 uint8_t *out, bool negate_odd, hipStream_t s, bool rows = false);
+                                 uint8_t *out, bool negate_odd, hipStream_t s);
 // Quantiser over the HBM four-step inverse's complex output (N >= 32768).
 hipError_t launch_quantise_cplx(const SectorArgs &a, const float2 *z, int64_t nframes,
                                 bool negate_odd, uint8_t *out, hipStream_t s);
