@@ -1481,6 +1481,13 @@ static bool large_fused_ok(const fxc_plan *p) {
     return p->variant == 1 && p->M1 == 128 && large_fused_possible(p);
 }
 
+// Frames whose parameters one launch computes on the fused four-step: as
+// many whole chunks as the parameter buffer (sized for the plan's largest
+// chunk) holds, so that smaller chunks do not each pay a launch for them.
+static int64_t fused_param_span(const fxc_plan *p) {
+    return std::max<int64_t>(1, p->chunk / p->chunk_use) * p->chunk_use;
+}
+
 // The phased sum at N = 16384 on the fused four-step (chosen form): the
 // same first pass, the phased bin in the second pass; as for the products,
 // both stations' bins of a pair on the same rows (band shift a multiple of
@@ -1518,6 +1525,8 @@ static fxc_status enqueue_phased_fused(fxc_plan *p, const SectorArgs &base, int6
     const int64_t bpf_o = plan_bpf_out(p);
     const int ng = large_fused_groups(p);
     if (raw) GROW(p, p->d_cb, (size_t)p->chunk_use * p->half * 8, s, "phased spectra");
+    const int64_t span = fused_param_span(p);
+    int64_t span0 = -1;   // first frame (of this sector) of the computed parameters
     for (int64_t c0 = 0; c0 < nf; c0 += p->chunk_use) {
         const int64_t cn = std::min(p->chunk_use, nf - c0);
         SectorArgs a = base;
@@ -1527,12 +1536,18 @@ static fxc_status enqueue_phased_fused(fxc_plan *p, const SectorArgs &base, int6
         a.nf = cn;
         a.part = (double *)p->part.p;
         a.cb_out = raw ? (float2 *)p->d_cb.p : nullptr;
-        int64_t *shifts = (int64_t *)p->d_lshifts.p;
-        HIP_TRY(launch_phased_params(a, 1, p->d_params.p, shifts, s));
+        if (span0 < 0 || c0 >= span0 + span) {
+            SectorArgs pa = a;
+            pa.nf = std::min(span, nf - c0);
+            HIP_TRY(launch_phased_params(pa, 1, p->d_params.p, (int64_t *)p->d_lshifts.p, s));
+            span0 = c0;
+        }
+        const uint8_t *pp = (const uint8_t *)p->d_params.p + (size_t)(c0 - span0) * phased_param_bytes();
+        const int64_t *shifts = (const int64_t *)p->d_lshifts.p + 2 * (c0 - span0);
         HIP_TRY(launch_large_pass1_fused(a, shifts, (float2 *)p->d_z[0].p, (float2 *)p->d_z[1].p,
                                          p->M1, p->M2, (const float2 *)p->tw_lm.p, s));
         HIP_TRY(launch_large_pass2_phased(a, (const float2 *)p->d_z[0].p, (const float2 *)p->d_z[1].p,
-                                          p->d_params.p, p->M1, p->M2, (const float2 *)p->tw_l0.p,
+                                          pp, p->M1, p->M2, (const float2 *)p->tw_l0.p,
                                           (const float2 *)p->tw_lm.p, raw, ng, c0 > 0 ? 1 : 0, s));
         if (raw)   // the inverse FFT's row half done: columns and quantiser
             HIP_TRY(launch_irfft_cols_quantise(a, (const float2 *)p->d_cb.p, cn, d_raw_out + c0 * bpf_o,
@@ -1549,6 +1564,8 @@ static fxc_status enqueue_large(fxc_plan *p, const SectorArgs &base, int64_t nf,
     const float2 *twl2 = (const float2 *)(p->M2 == 64 ? p->tw_l0.p : p->M2 == 128 ? p->tw_l1.p : p->tw_l2.p);
     if (large_fused_ok(p)) {
         const int ng = large_fused_groups(p);
+        const int64_t span = fused_param_span(p);
+        int64_t span0 = -1;   // first frame (of this sector) of the computed parameters
         for (int64_t c0 = 0; c0 < nf; c0 += p->chunk_use) {
             const int64_t cn = std::min(p->chunk_use, nf - c0);
             SectorArgs a = base;
@@ -1557,9 +1574,16 @@ static fxc_status enqueue_large(fxc_plan *p, const SectorArgs &base, int64_t nf,
             a.first_frame = base.first_frame + c0;
             a.nf = cn;
             a.part = (double *)p->part.p;
-            FrameParam *params = (FrameParam *)p->d_params.p;
-            int64_t *shifts = (int64_t *)p->d_lshifts.p;
-            HIP_TRY(launch_frame_params(a, p->desc.precision, params, shifts, s));
+            if (span0 < 0 || c0 >= span0 + span) {
+                SectorArgs pa = a;
+                pa.nf = std::min(span, nf - c0);
+                HIP_TRY(launch_frame_params(pa, p->desc.precision, (FrameParam *)p->d_params.p,
+                                            (int64_t *)p->d_lshifts.p, s));
+                span0 = c0;
+            }
+            const FrameParam *params = (const FrameParam *)((const uint8_t *)p->d_params.p +
+                                                            (size_t)(c0 - span0) * frame_param_bytes());
+            const int64_t *shifts = (const int64_t *)p->d_lshifts.p + 2 * (c0 - span0);
             HIP_TRY(launch_large_pass1_fused(a, shifts, (float2 *)p->d_z[0].p, (float2 *)p->d_z[1].p,
                                              p->M1, p->M2, (const float2 *)p->tw_lm.p, s));
             HIP_TRY(launch_large_pass2_mac(a, (const float2 *)p->d_z[0].p, (const float2 *)p->d_z[1].p,
