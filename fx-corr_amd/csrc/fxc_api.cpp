@@ -156,6 +156,7 @@ struct fxc_plan {
     int M1 = 0, M2 = 0;
     int64_t chunk = 0;       // allocated chunk (frames)
     int64_t chunk_use = 0;   // chunk in use (<= chunk; autotuned)
+    int64_t param_cap = 0;   // frames the per-frame parameter buffers hold (>= chunk)
     // autotuned choices of the Stockham form
     int variant = 0;         // Stockham: FFT pass sequence (1: radix-16 passes);
                              // MFMA: kernel organization (1: producer/consumer waves)
@@ -760,6 +761,12 @@ static std::vector<uint8_t> mfma_tables(const fxc_plan *p) {
 static bool large_fused_possible(const fxc_plan *p);
 static bool phased_fused_possible(const fxc_plan *p);
 
+// The four-step plans' per-frame parameter buffers hold at least 8192
+// frames (a few MB), so the fused four-step computes a sector's parameters
+// in one launch or a few rather than one per chunk.
+constexpr size_t kParamFrames = 8192;
+static size_t param_frames(size_t chunk) { return std::max(chunk, kParamFrames); }
+
 // The fused four-step's default chunk: both stations' Y (2 M complex per
 // frame) in 128 MiB, so that the second pass reads what the first wrote
 // from the 256 MB Infinity Cache rather than from HBM (1024 / 512 / 256
@@ -895,9 +902,11 @@ extern "C" fxc_status fxc_plan_create(const fxc_plan_desc *desc, fxc_plan **out)
             p->d_x[0].ensure(C * N * 4) != hipSuccess || p->d_x[1].ensure(C * N * 4) != hipSuccess ||
             p->d_y.ensure(C * M * 8) != hipSuccess || p->d_z[0].ensure(C * M * 8) != hipSuccess ||
             p->d_z[1].ensure(C * M * 8) != hipSuccess ||
-            p->d_params.ensure(C * std::max(frame_param_bytes(), phased_param_bytes())) != hipSuccess ||
-            p->d_lshifts.ensure(C * 16) != hipSuccess)
+            p->d_params.ensure(param_frames(C) * std::max(frame_param_bytes(), phased_param_bytes())) !=
+                hipSuccess ||
+            p->d_lshifts.ensure(param_frames(C) * 16) != hipSuccess)
             return cleanup(fail(FXC_ERR_NOMEM, "large-FFT scratch allocation failed"));
+        p->param_cap = (int64_t)param_frames(C);
     }
     if (first_pass_mfma_eligible(p)) {
         std::vector<uint8_t> tab(MFT_A2 - MFT_B1, 0);
@@ -936,8 +945,10 @@ extern "C" fxc_status fxc_plan_create(const fxc_plan_desc *desc, fxc_plan **out)
             p->chunk_use = fused_chunk_default(p);
             if (!table(p->tw_l0, 64) || !table(p->tw_lm, (int)M) ||
                 p->d_z[0].ensure(C * M * 8) != hipSuccess || p->d_z[1].ensure(C * M * 8) != hipSuccess ||
-                p->d_params.ensure(C * pb) != hipSuccess || p->d_lshifts.ensure(C * 16) != hipSuccess)
+                p->d_params.ensure(param_frames(C) * pb) != hipSuccess ||
+                p->d_lshifts.ensure(param_frames(C) * 16) != hipSuccess)
                 return cleanup(fail(FXC_ERR_NOMEM, "fused four-step scratch allocation failed"));
+            p->param_cap = (int64_t)param_frames(C);
         }
     }
     p->max_wg = p->fft_form == 1 ? cus
@@ -1485,7 +1496,7 @@ static bool large_fused_ok(const fxc_plan *p) {
 // many whole chunks as the parameter buffer (sized for the plan's largest
 // chunk) holds, so that smaller chunks do not each pay a launch for them.
 static int64_t fused_param_span(const fxc_plan *p) {
-    return std::max<int64_t>(1, p->chunk / p->chunk_use) * p->chunk_use;
+    return std::max<int64_t>(1, p->param_cap / p->chunk_use) * p->chunk_use;
 }
 
 // The phased sum at N = 16384 on the fused four-step (chosen form): the

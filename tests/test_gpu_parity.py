@@ -1212,6 +1212,40 @@ def test_phased_fused_four_step_fft16384(bits, sb, grid, lsb_out, raw, nf, chunk
             assert d <= 1e-4 * n * nf + 2, d
 
 
+def test_fused_four_step_parameter_spans_fft16384():
+    """A 9,000-frame sector at N = 16384: the fused four-step computes the
+    frame parameters for spans of whole chunks (8,192 frames here), so the
+    sector takes two spans.  ACF/XCF and the phased sum (power and 2-bit
+    .raw) against the Stockham kernels on the same plans."""
+    n, fs, nf = 16384, 1024e6, 9000
+    rng = np.random.default_rng(16384 + nf)
+    raw1 = rng.integers(0, 256, nf * n // 4, dtype=np.uint8)
+    raw2 = rng.integers(0, 256, nf * n // 4, dtype=np.uint8)
+    cfg = F.make_delay_cfg(np.zeros(2 + int(nf * n / fs)), frame_dt=n / fs, fs=fs,
+                           delay_user_samples=6.7, extra_delay_rate_sps=3e-9, lo1_hz=7e9, lo2_hz=7e9)
+    lv = [LEVELS_2BIT, LEVELS_2BIT]
+    res = {}
+    for v in (1, 0):
+        p = F.Plan(n, levels=lv, df_hz=fs / n, precision=F.PRECISION_CHOSEN)
+        q = F.Plan(n, levels=lv, df_hz=fs / n, products=F.PRODUCT_PHASED | F.PRODUCT_RAW_OUT,
+                   w=(0.9, 1.1), bit_out=2, precision=F.PRECISION_CHOSEN)
+        assert p.info.variant == 1 and q.info.variant == 1
+        if v == 0:
+            F.plan_set_tuning(p, variant=0)
+            F.plan_set_tuning(q, variant=0)
+        res[v] = (p.sector_run(raw1, raw2, 0, nf, cfg), q.sector_run(raw1, raw2, 0, nf, cfg))
+        p.close()
+        q.close()
+    _compare(res[1][0], res[0][0], CHOSEN_TOL)
+    assert rel_err(res[1][1]["acc_ph"], res[0][1]["acc_ph"]) < CHOSEN_TOL
+    # 2-bit codes with the identity output map: compare the fields directly
+    sh = np.array([0, 2, 4, 6], dtype=np.uint8)
+    a, b = res[1][1]["raw_out"], res[0][1]["raw_out"]
+    assert a.size == b.size == nf * n // 4
+    d = int(np.count_nonzero(((a[:, None] >> sh) & 3) != ((b[:, None] >> sh) & 3)))
+    assert d <= 1e-4 * n * nf + 2, d
+
+
 @pytest.mark.parametrize("n,bits", [(32768, (3, 5)), (65536, (5, 3))])
 def test_large_fused_odd_widths_and_shuffle(n, bits):
     """The fused four-step's run-time code width (3 and 5 bits: codes that
