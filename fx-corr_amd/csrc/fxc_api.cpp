@@ -1534,7 +1534,10 @@ static fxc_status enqueue_phased_fused(fxc_plan *p, const SectorArgs &base, int6
                                        uint8_t *d_raw_out, double *d_acc, hipStream_t s) {
     const bool raw = (p->desc.products & FXC_PRODUCT_RAW_OUT) && d_raw_out != nullptr;
     const int64_t bpf_o = plan_bpf_out(p);
-    const int ng = large_fused_groups(p);
+    // the .raw second pass holds four waves per SIMD (the power-only one
+    // five): one resident round of workgroups at that occupancy
+    const int ng = raw ? std::max(1, std::min(p->max_wg, num_cus() * 4 / (p->M1 / 2)))
+                       : large_fused_groups(p);
     if (raw) GROW(p, p->d_cb, (size_t)p->chunk_use * p->half * 8, s, "phased spectra");
     const int64_t span = fused_param_span(p);
     int64_t span0 = -1;   // first frame (of this sector) of the computed parameters
